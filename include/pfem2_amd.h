@@ -1,0 +1,164 @@
+/*
+ * pfem2_amd.h — C ABI of the MI355X particle stage of PFEM2.
+ *
+ * One handle holds one simulation: the mesh (uploaded once), the nodal
+ * velocity fields V and V_old, and the particles, all resident in HBM.
+ * Every entry point returns 0 on success or a negative PF2_E* code; the
+ * message of the last failure on the calling thread is pf2_last_error().
+ * All work of a handle is issued on one HIP stream owned by the handle;
+ * calls on one handle must not overlap (the reference is single-threaded
+ * per process as well).  Calls on different handles are independent.
+ *
+ * Each entry point names the member of the reference interface it replaces
+ * (paths relative to the andreyypopov/pfem2 repository).  The deal.II
+ * objects behind those members (Triangulation, DoFHandler, solution
+ * vectors) become plain arrays here:
+ *   vertices       n_vertices x dim doubles, row-major
+ *   cell_vertices  n_cells x 2^dim ints, deal.II vertex order
+ *                  (lexicographic: bit d of the local index = unit coordinate d)
+ *   cell_dofs      n_cells x 2^dim ints: cell->vertex_dof_index(v, 0)
+ *   nodal fields   dim x n_dofs doubles, component-major
+ *                  (solutionVx, solutionVy[, solutionVz] back to back)
+ *   particles      cell-major, ascending id inside a cell;
+ *                  per-particle vectors n x dim doubles, row-major
+ */
+#ifndef PFEM2_AMD_H
+#define PFEM2_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PF2_OK 0
+#define PF2_E_ARG (-1)      /* invalid argument */
+#define PF2_E_HIP (-2)      /* HIP runtime failure */
+#define PF2_E_STATE (-3)    /* call not valid in the current state */
+#define PF2_E_CAPACITY (-4) /* particle capacity exceeded */
+
+#define PF2_FP64 0          /* FP64 everywhere (the reference's arithmetic) */
+#define PF2_FP32_MIXED 1    /* cell + FP32 reference coordinates, FP32 velocities */
+
+typedef struct pf2_handle_s* pf2_handle;
+
+/* Triangulation + DoFHandler of the reference's pfem2Solver
+ * (cylinder2D/pfem2particle.h:170-175; TurekBenchmark/pfem2particle.h:196-205). */
+typedef struct pf2_mesh {
+    int dim;                    /* 2 or 3 */
+    int n_cells;
+    int n_vertices;
+    int n_dofs;                 /* Q1 velocity DoFs per component */
+    const double* vertices;
+    const int* cell_vertices;
+    const int* cell_dofs;       /* NULL: same as cell_vertices */
+} pf2_mesh;
+
+/* Compile-time knobs of the reference made run-time
+ * (cylinder2D/pfem2particle.h:4-5: PARTICLES_MOVEMENT_STEPS,
+ * MAX_PARTICLES_PER_CELL_PART), the seeding lattice
+ * (seed_particles' `quantities`, cylinder2D/pfem2particle.h:132) and the
+ * time step (pfem2Solver::time_step, cylinder2D/pfem2particle.h:161). */
+typedef struct pf2_params {
+    double dt;
+    int substeps;
+    int max_per_part;
+    int quantities[3];
+    int precision;              /* PF2_FP64 or PF2_FP32_MIXED */
+    int keep_velocity_ext;      /* store the last advection velocity per particle
+                                   (needed by pf2_sort_particles after a mesh move) */
+} pf2_params;
+
+const char* pf2_last_error(void);
+
+/* pfem2Solver constructor + build_grid/setup_system hand-over
+ * (cylinder2D/pfem2particle.cpp:467-480).  Also builds the vertex-to-cell
+ * star and the cell-centre directions, i.e. pfem2ParticleHandler::initialize_maps
+ * (TurekBenchmark/pfem2particle.cpp:215-219).  `capacity` = maximum number of
+ * particles; 0 chooses the bound that reseeding guarantees. */
+int pf2_create(const pf2_mesh* mesh, const pf2_params* params, long long capacity, pf2_handle* out);
+int pf2_destroy(pf2_handle h);
+int pf2_set_params(pf2_handle h, const pf2_params* params);
+
+/* Nodal fields.  V replaces solutionVx/Vy[/Vz]; V_old replaces
+ * old_solutionVx/... (cylinder2D/pfem2particle.h:163-164).  Either pointer
+ * may be NULL to keep the resident copy.  After a projection both resident
+ * copies hold the projected field (the reference sets old_solution =
+ * solution before the FEM step, cylinder2D/cylinder2D.cc:389-391), so a
+ * driver uploads only the new V each step. */
+int pf2_upload_fields(pf2_handle h, const double* V, const double* V_old);
+/* Same with device pointers (copied on the handle's stream). */
+int pf2_upload_fields_device(pf2_handle h, const double* V, const double* V_old);
+int pf2_download_field(pf2_handle h, double* V);
+
+/* pfem2Solver::seed_particles (cylinder2D/pfem2particle.cpp:563-580). */
+int pf2_seed_particles(pf2_handle h);
+/* pfem2ParticleHandler::insert_particle (cylinder2D/pfem2particle.cpp:178-184)
+ * for n particles given by cell and reference coordinates; the location is
+ * mapped from the reference point, the velocity is interpolated from V when
+ * v is NULL, and ids continue the particle counter in input order. */
+int pf2_insert_particles(pf2_handle h, int n, const int* cell, const double* xi, const double* v);
+
+/* pfem2Solver::correct_particles_velocities (cylinder2D/pfem2particle.cpp:582-603). */
+int pf2_correct_particles_velocities(pf2_handle h);
+/* pfem2Solver::move_particles (cylinder2D/pfem2particle.cpp:605-647):
+ * substeps of advection + relocation, then reseed/delete in every cell. */
+int pf2_move_particles(pf2_handle h);
+/* pfem2Solver::distribute_particle_velocities_to_grid
+ * (TurekBenchmark/pfem2particle.cpp:858-918) including the wall fix-up of
+ * cylinderDyM/pfem2particle.cpp:960-992.  V_out (host, component-major) may
+ * be NULL. */
+int pf2_distribute_particle_velocities_to_grid(pf2_handle h, double* V_out);
+/* correct -> move -> distribute in one call (the driver sequence of
+ * cylinder2D/cylinder2D.cc:865-867); V_out may be NULL. */
+int pf2_step(pf2_handle h, double* V_out);
+
+/* Wall / body nodes of the projection fix-up (wallsAndBodyDoFs,
+ * cylinderDyM/pfem2particle.h:253): vertex index, DoF index, boundary values
+ * (n x dim) and a mask of enforced components per node.  n = 0 clears. */
+int pf2_set_walls(pf2_handle h, int n, const int* vertex, const int* dof, const double* bc, const int* mask);
+
+/* pfem2ParticleHandler::sort_particles_into_subdomains_and_cells
+ * (cylinder2D/pfem2particle.cpp:214-445), as called after a mesh move
+ * (cylinderDyM/cylinderDyM.cc:1159). */
+int pf2_sort_particles_into_subdomains_and_cells(pf2_handle h);
+/* pfem2Solver::check_empty_cells (cylinderDyM/pfem2particle.cpp:759-763). */
+int pf2_check_empty_cells(pf2_handle h);
+/* pfem2Solver::transform_grid (cylinderDyM/pfem2particle.cpp:1252-1273):
+ * new absolute vertex coordinates (n_vertices x dim).  The cached centre
+ * directions are kept unless recompute_maps != 0, as in the reference where
+ * initialize_maps runs once. */
+int pf2_update_vertices(pf2_handle h, const double* vertices, int recompute_maps);
+/* pfem2ParticleHandler::initialize_maps (TurekBenchmark/pfem2particle.cpp:215-219). */
+int pf2_initialize_maps(pf2_handle h);
+
+/* pfem2ParticleHandler::n_global_particles (cylinder2D/pfem2particle.cpp:186-189). */
+int pf2_n_global_particles(pf2_handle h, long long* n);
+/* iteration over particle_handler.begin()/end() reading location, reference
+ * location, velocity and id (cylinder2D/cylinder2D.cc:816-835).  Any output
+ * pointer may be NULL; arrays must hold pf2_n_global_particles entries. */
+int pf2_download_particles(pf2_handle h, int* cell, uint32_t* id, double* x, double* xi, double* v);
+/* pfem2ParticleHandler::n_particles_in_cell for every cell (n_cells ints). */
+int pf2_cell_counts(pf2_handle h, int* counts);
+/* the particleCount member of pfem2Solver (last id handed out) */
+int pf2_particle_counter(pf2_handle h, uint32_t* count);
+
+/* Counters since creation: [0] particles lost in relocation, [1] deleted by
+ * the per-part limit, [2] spawned by reseeding, [3] relocation searches,
+ * [4] zero-weight projection nodes. */
+int pf2_stats(pf2_handle h, unsigned long long* out5);
+int pf2_synchronize(pf2_handle h);
+
+/* Per-kernel timing with HIP events on the handle's stream (off by
+ * default).  pf2_kernel_times writes up to `cap` entries of total
+ * milliseconds and launch counts in the order of pf2_kernel_name(i), and
+ * returns the number of kernels. */
+int pf2_set_profiling(pf2_handle h, int on);
+int pf2_kernel_times(pf2_handle h, double* total_ms, long long* launches, int cap);
+const char* pf2_kernel_name(int i);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PFEM2_AMD_H */

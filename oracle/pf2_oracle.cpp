@@ -1,0 +1,912 @@
+// pf2_oracle.cpp — CPU restatement of the PFEM2 particle stage (FP64, single thread).
+//
+// TEST INFRASTRUCTURE ONLY.  This file is the checker for the HIP product path
+// in pfem2_amd/: only tests/, __graft_entry__.smoke() and bench.py's
+// cpu_baseline leg may load it.  Nothing in the product links or calls it.
+//
+// What it restates (paths relative to the andreyypopov/pfem2 repository):
+//   * per-step call order           cylinder2D/cylinder2D.cc:865-867
+//   * seeding                       cylinder2D/pfem2particle.cpp:487-508, 563-580
+//                                   tubePoiseuille3D/pfem2particle.cpp:489-511 (3-D lattice order)
+//   * velocity correction           TurekBenchmark/pfem2particle.cpp:792-811
+//   * advection substeps            TurekBenchmark/pfem2particle.cpp:813-856 (velocity_ext = u, :839)
+//   * relocation                    TurekBenchmark/pfem2particle.cpp:276-507
+//     closest vertex                cylinder2D/pfem2particle.cpp:120-144
+//     candidate ordering            cylinder2D/pfem2particle.cpp:206-212, 300-335
+//   * reseed / delete               cylinder2D/pfem2particle.cpp:510-561
+//                                   tubePoiseuille3D/pfem2particle.cpp:513-567
+//   * projection (per-cell local    TurekBenchmark/pfem2particle.cpp:858-918
+//     sums, then node sums)
+//   * wall fix-up                   cylinderDyM/pfem2particle.cpp:960-992
+//                                   (nacaAirfoil/pfem2particle.cpp:920-949 is the
+//                                    special case bc = 0, both components enforced)
+//   * moving-mesh re-sort           cylinderDyM/cylinderDyM.cc:1157-1160,
+//                                   cylinderDyM/pfem2particle.cpp:1252-1273
+//
+// deal.II primitives (deal.II >= 9.0 is not vendored; the API used implies
+// 9.2).  Restated from their published definitions:
+//   * FE_Q<dim>(1) shape values: tensor products of L0(t) = 1 - t, L1(t) = t,
+//     multiplied left to right over the coordinate directions.
+//   * MappingQ1 forward map: sum_k x_k * N_k(xi), k ascending, starting from 0.
+//   * MappingQ1 inverse map, 2-D: the closed-form quadratic solve of
+//     internal::MappingQ1::transform_real_to_unit_cell (accepted when the result
+//     lies in [-1e-15, 1+1e-15]^2), otherwise the Newton iteration with line
+//     search of MappingQGeneric (affine least-squares initial guess, residual
+//     early-out 1e-10*diameter, weighted tolerance 1e-11, 20 iterations,
+//     step halving down to 0.05, failure when det J <= 0).  3-D: Newton only.
+//   * GeometryInfo::is_inside_unit_cell: 0 <= xi_d <= 1 (no tolerance).
+//   * GridTools::vertex_to_cell_map (std::set: ascending cell index) and
+//     vertex_to_cell_centers_directions (normalised vertex-average centre minus
+//     vertex).
+//   * Tensor division by a scalar multiplies by the reciprocal.
+// Agreement with deal.II itself below the last ulp is not pinned (deal.II is
+// absent).  All floating-point expressions are written without fused
+// multiply-add; the build uses -ffp-contract=off.
+//
+// Canonical container order: within a cell particles are kept in ascending id
+// order.  The reference keeps them in std::unordered_multimap order; the only
+// result that depends on that order is which particles survive in an
+// over-full cell part (cylinder2D/pfem2particle.cpp:526-527).  Here the MAX
+// particles with the smallest ids survive.  The projection sums follow the
+// same canonical order.
+//
+// An extra, result-preserving test is applied before an inverse map: a point
+// that lies outside a convex quadrilateral cell by more than 1e-8 * diameter
+// (or outside a cell's bounding box by that margin in 3-D) is reported as
+// "not inside" without running the mapping.  Neither the closed form nor the
+// Newton iteration can return a point of the unit cell for such a point
+// (its residual is bounded below by the distance); tests check this.
+
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <cstdio>
+#include <vector>
+#include <algorithm>
+#include <chrono>
+#include <stdexcept>
+
+namespace {
+
+constexpr int MAXV = 8;
+
+struct Particle {
+    double x[3];
+    double xi[3];
+    double v[3];
+    double vext[3];
+    uint32_t id;
+};
+
+struct Wall {
+    int vertex;
+    int dof;
+    double bc[3];
+    int mask;   // bit d set: component d is enforced
+};
+
+struct Oracle {
+    int dim = 2;
+    int nvpc = 4;
+    int n_cells = 0, n_vertices = 0, n_dofs = 0;
+    std::vector<double> X;          // n_vertices * dim
+    std::vector<int> cell_vertices; // n_cells * nvpc
+    std::vector<int> cell_dofs;     // n_cells * nvpc
+    // vertex star: cells around each vertex in ascending order, and the
+    // normalised direction (cell centre - vertex) of each entry
+    std::vector<int> star_off, star_cell;
+    std::vector<double> star_dir;   // per entry, dim doubles
+    // fields
+    std::vector<double> V, Vold;    // dim * n_dofs, component-major
+    // parameters
+    double dt = 0.01;
+    int substeps = 1;
+    int max_per_part = 3;
+    int quant[3] = {2, 2, 2};
+    bool filter = true;
+    // particles
+    std::vector<std::vector<Particle>> cells;
+    uint32_t particle_count = 0;
+    // statistics
+    uint64_t n_lost = 0, n_deleted = 0, n_spawned = 0, n_moved = 0, n_newton = 0;
+    std::vector<Wall> walls;
+};
+
+// ---- deal.II FE_Q(1) / MappingQ1 restatement ------------------------------
+
+inline void shape_values(int dim, const double* xi, double* N) {
+    const int nv = 1 << dim;
+    for (int k = 0; k < nv; ++k) {
+        double val = 1.0;
+        for (int d = 0; d < dim; ++d) {
+            const double t = xi[d];
+            val *= ((k >> d) & 1) ? t : (1.0 - t);
+        }
+        N[k] = val;
+    }
+}
+
+// dN_k/dxi_j: product over directions d of (d == j ? +-1 : L(xi_d)), left to right
+inline void shape_grads(int dim, const double* xi, double* G /* nv*dim */) {
+    const int nv = 1 << dim;
+    for (int k = 0; k < nv; ++k)
+        for (int j = 0; j < dim; ++j) {
+            double g = 1.0;
+            for (int d = 0; d < dim; ++d) {
+                const int bit = (k >> d) & 1;
+                if (d == j) g *= bit ? 1.0 : -1.0;
+                else g *= bit ? xi[d] : (1.0 - xi[d]);
+            }
+            G[k * dim + j] = g;
+        }
+}
+
+inline void gather_cell_vertices(const Oracle& o, int c, double* P /* nvpc*dim */) {
+    for (int k = 0; k < o.nvpc; ++k) {
+        const int gv = o.cell_vertices[(size_t)c * o.nvpc + k];
+        for (int d = 0; d < o.dim; ++d) P[k * o.dim + d] = o.X[(size_t)gv * o.dim + d];
+    }
+}
+
+inline void forward_map(int dim, const double* P, const double* xi, double* x) {
+    double N[MAXV];
+    shape_values(dim, xi, N);
+    const int nv = 1 << dim;
+    for (int d = 0; d < dim; ++d) x[d] = 0.0;
+    for (int k = 0; k < nv; ++k)
+        for (int d = 0; d < dim; ++d) x[d] += P[k * dim + d] * N[k];
+}
+
+inline double norm(int dim, const double* a) {
+    double s = 0.0;
+    for (int d = 0; d < dim; ++d) s += a[d] * a[d];
+    return std::sqrt(s);
+}
+
+inline double dot(int dim, const double* a, const double* b) {
+    double s = a[0] * b[0];
+    for (int d = 1; d < dim; ++d) s += a[d] * b[d];
+    return s;
+}
+
+// cell diameter as deal.II defines it for quads / hexes: longest diagonal
+inline double diameter(int dim, const double* P) {
+    double best = 0.0, t[3];
+    const int pairs2[2][2] = {{3, 0}, {2, 1}};
+    const int pairs3[4][2] = {{7, 0}, {6, 1}, {5, 2}, {4, 3}};
+    const int np = dim == 2 ? 2 : 4;
+    for (int i = 0; i < np; ++i) {
+        const int a = dim == 2 ? pairs2[i][0] : pairs3[i][0];
+        const int b = dim == 2 ? pairs2[i][1] : pairs3[i][1];
+        for (int d = 0; d < dim; ++d) t[d] = P[a * dim + d] - P[b * dim + d];
+        const double n = norm(dim, t);
+        if (n > best) best = n;
+    }
+    return best;
+}
+
+inline double det_t(int dim, const double (*t)[3]) {
+    if (dim == 2) return t[0][0] * t[1][1] - t[1][0] * t[0][1];
+    return t[0][0] * t[1][1] * t[2][2] - t[0][0] * t[1][2] * t[2][1]
+         - t[1][0] * t[0][1] * t[2][2] + t[1][0] * t[0][2] * t[2][1]
+         + t[2][0] * t[0][1] * t[1][2] - t[2][0] * t[0][2] * t[1][1];
+}
+
+inline void invert_t(int dim, const double (*t)[3], double (*r)[3]) {
+    const double inv = 1.0 / det_t(dim, t);
+    if (dim == 2) {
+        r[0][0] = t[1][1] * inv;
+        r[0][1] = -t[0][1] * inv;
+        r[1][0] = -t[1][0] * inv;
+        r[1][1] = t[0][0] * inv;
+        return;
+    }
+    r[0][0] = (t[1][1] * t[2][2] - t[1][2] * t[2][1]) * inv;
+    r[0][1] = (t[0][2] * t[2][1] - t[0][1] * t[2][2]) * inv;
+    r[0][2] = (t[0][1] * t[1][2] - t[0][2] * t[1][1]) * inv;
+    r[1][0] = (t[1][2] * t[2][0] - t[1][0] * t[2][2]) * inv;
+    r[1][1] = (t[0][0] * t[2][2] - t[0][2] * t[2][0]) * inv;
+    r[1][2] = (t[0][2] * t[1][0] - t[0][0] * t[1][2]) * inv;
+    r[2][0] = (t[1][0] * t[2][1] - t[1][1] * t[2][0]) * inv;
+    r[2][1] = (t[0][1] * t[2][0] - t[0][0] * t[2][1]) * inv;
+    r[2][2] = (t[0][0] * t[1][1] - t[0][1] * t[1][0]) * inv;
+}
+
+inline void mat_vec(int dim, const double (*m)[3], const double* v, double* r) {
+    for (int i = 0; i < dim; ++i) {
+        double s = m[i][0] * v[0];
+        for (int j = 1; j < dim; ++j) s += m[i][j] * v[j];
+        r[i] = s;
+    }
+}
+
+// Closed-form inverse of the bilinear map (deal.II internal::MappingQ1,
+// 2-D).  Returns false where that routine throws ExcTransformationFailed.
+bool inverse_closed_form_2d(const double* P, const double* p, double* out) {
+    const double x = p[0], y = p[1];
+    const double x0 = P[0], x1 = P[2], x2 = P[4], x3 = P[6];
+    const double y0 = P[1], y1 = P[3], y2 = P[5], y3 = P[7];
+    const double a = (x1 - x3) * (y0 - y2) - (x0 - x2) * (y1 - y3);
+    const double b = -(x0 - x1 - x2 + x3) * y + (x - 2 * x1 + x3) * y0 - (x - 2 * x0 + x2) * y1
+                     - (x - x1) * y2 + (x - x0) * y3;
+    const double c = (x0 - x1) * y - (x - x1) * y0 + (x - x0) * y1;
+    const double disc = b * b - 4 * a * c;
+    if (!(disc >= 0.0)) return false;
+    const double sq = std::sqrt(disc);
+    double eta1, eta2;
+    if (b != 0.0 && std::fabs(b) == sq) {
+        eta1 = -c / b;
+        eta2 = -c / b;
+    } else if (std::fabs(a) < 1e-8 * std::fabs(b)) {
+        eta1 = 2 * c / (-b - sq);
+        eta2 = 2 * c / (-b + sq);
+    } else {
+        eta1 = (-b - sq) / (2 * a);
+        eta2 = (-b + sq) / (2 * a);
+    }
+    const double eta = (std::fabs(eta1 - 0.5) < std::fabs(eta2 - 0.5)) ? eta1 : eta2;
+    const double sub0 = -eta * x2 + x0 * (eta - 1);
+    const double den0 = eta * x3 - x1 * (eta - 1) + sub0;
+    const double max_x = std::max(std::max(std::fabs(x0), std::fabs(x1)), std::max(std::fabs(x2), std::fabs(x3)));
+    if (std::fabs(den0) > 1e-10 * max_x) {
+        out[0] = (x + sub0) / den0;
+        out[1] = eta;
+        return true;
+    }
+    const double max_y = std::max(std::max(std::fabs(y0), std::fabs(y1)), std::max(std::fabs(y2), std::fabs(y3)));
+    const double sub1 = -eta * y2 + y0 * (eta - 1);
+    const double den1 = eta * y3 - y1 * (eta - 1) + sub1;
+    if (std::fabs(den1) > 1e-10 * max_y) {
+        out[0] = (sub1 + y) / den1;
+        out[1] = eta;
+        return true;
+    }
+    return false;
+}
+
+// Newton iteration with line search (deal.II MappingQGeneric restatement).
+bool inverse_newton(int dim, const double* P, const double* p, double* out, uint64_t* counter) {
+    if (counter) ++*counter;
+    const int nv = 1 << dim;
+    // affine least-squares initial guess: A = sum_k x_k (x) KA[k], b = sum_k Kb[k] x_k
+    static const double KA2[4][2] = {{-0.5, -0.5}, {0.5, -0.5}, {-0.5, 0.5}, {0.5, 0.5}};
+    static const double Kb2[4] = {0.75, 0.25, 0.25, -0.25};
+    static const double KA3[8][3] = {{-0.25, -0.25, -0.25}, {0.25, -0.25, -0.25}, {-0.25, 0.25, -0.25},
+                                     {0.25, 0.25, -0.25},   {-0.25, -0.25, 0.25}, {0.25, -0.25, 0.25},
+                                     {-0.25, 0.25, 0.25},   {0.25, 0.25, 0.25}};
+    static const double Kb3[8] = {0.5, 0.25, 0.25, 0.0, 0.25, 0.0, 0.0, -0.25};
+    double A[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    double bvec[3] = {0, 0, 0};
+    for (int k = 0; k < nv; ++k)
+        for (int i = 0; i < dim; ++i) {
+            const double kb = dim == 2 ? Kb2[k] : Kb3[k];
+            bvec[i] += kb * P[k * dim + i];
+            for (int j = 0; j < dim; ++j) A[i][j] += P[k * dim + i] * (dim == 2 ? KA2[k][j] : KA3[k][j]);
+        }
+    double Ainv[3][3];
+    invert_t(dim, A, Ainv);
+    double pb[3], xi[3];
+    for (int d = 0; d < dim; ++d) pb[d] = p[d] - bvec[d];
+    mat_vec(dim, Ainv, pb, xi);
+
+    double pr[3], f[3];
+    forward_map(dim, P, xi, pr);
+    for (int d = 0; d < dim; ++d) f[d] = pr[d] - p[d];
+    if (norm(dim, f) < 1e-10 * diameter(dim, P)) {
+        for (int d = 0; d < dim; ++d) out[d] = xi[d];
+        return true;
+    }
+    int it = 0;
+    double wn;
+    do {
+        double G[MAXV * 3];
+        shape_grads(dim, xi, G);
+        double df[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+        for (int k = 0; k < nv; ++k)
+            for (int i = 0; i < dim; ++i)
+                for (int j = 0; j < dim; ++j) df[i][j] += P[k * dim + i] * G[k * dim + j];
+        if (!(det_t(dim, df) > 0)) return false;
+        double dfi[3][3];
+        invert_t(dim, df, dfi);
+        double delta[3];
+        mat_vec(dim, dfi, f, delta);
+        double step = 1.0;
+        for (;;) {
+            double xt[3], prt[3], ft[3];
+            for (int d = 0; d < dim; ++d) xt[d] = xi[d] - step * delta[d];
+            forward_map(dim, P, xt, prt);
+            for (int d = 0; d < dim; ++d) ft[d] = prt[d] - p[d];
+            if (norm(dim, ft) < norm(dim, f)) {
+                for (int d = 0; d < dim; ++d) { xi[d] = xt[d]; f[d] = ft[d]; }
+                break;
+            } else if (step > 0.05) {
+                step /= 2;
+            } else {
+                return false;
+            }
+        }
+        ++it;
+        if (it > 20) return false;
+        double wf[3];
+        mat_vec(dim, dfi, f, wf);
+        wn = norm(dim, wf);
+    } while (wn > 1e-11);
+    for (int d = 0; d < dim; ++d) out[d] = xi[d];
+    return true;
+}
+
+// tolerance of the clearly-outside test: 1e-8 * diameter, -1 for a
+// non-convex quadrilateral (no test)
+double outside_tol(int dim, const double* P) {
+    const double tol = 1e-8 * diameter(dim, P);
+    if (dim == 2) {
+        static const int cyc[4] = {0, 1, 3, 2};
+        for (int i = 0; i < 4; ++i) {
+            const int a = cyc[i], b = cyc[(i + 1) & 3], c = cyc[(i + 2) & 3];
+            const double e0x = P[2 * b] - P[2 * a], e0y = P[2 * b + 1] - P[2 * a + 1];
+            const double e1x = P[2 * c] - P[2 * b], e1y = P[2 * c + 1] - P[2 * b + 1];
+            if (!(e0x * e1y - e0y * e1x > 0.0)) return -1.0;
+        }
+    }
+    return tol;
+}
+
+// true: the point lies outside the cell by more than 1e-8 * diameter
+bool clearly_outside(int dim, const double* P, const double* p) {
+    const double tol = outside_tol(dim, P);
+    if (!(tol >= 0.0)) return false;
+    if (dim == 2) {
+        static const int cyc[4] = {0, 1, 3, 2};
+        for (int i = 0; i < 4; ++i) {
+            const int a = cyc[i], b = cyc[(i + 1) & 3];
+            const double ex = P[2 * b] - P[2 * a], ey = P[2 * b + 1] - P[2 * a + 1];
+            const double wx = p[0] - P[2 * a], wy = p[1] - P[2 * a + 1];
+            const double cr = ex * wy - ey * wx;
+            const double el2 = ex * ex + ey * ey;
+            if (cr < 0.0 && cr * cr > tol * tol * el2) return true;
+        }
+        return false;
+    }
+    for (int d = 0; d < 3; ++d) {
+        double lo = P[d], hi = P[d];
+        for (int k = 1; k < 8; ++k) {
+            lo = std::min(lo, P[k * 3 + d]);
+            hi = std::max(hi, P[k * 3 + d]);
+        }
+        if (p[d] < lo - tol || p[d] > hi + tol) return true;
+    }
+    return false;
+}
+
+// deal.II MappingQ1::transform_real_to_unit_cell restated; false = exception.
+// filter_first: apply the clearly-outside test before the closed form (used
+// for star candidates, most of which do not contain the point); otherwise
+// the closed form runs first and the test only guards the Newton fallback
+// (used for the own-cell test, where the point is usually inside).  Both
+// orders give the same decision because the test is result-preserving.
+bool inverse_map(const Oracle& o, const double* P, const double* p, double* out, uint64_t* counter,
+                 bool filter_first) {
+    if (o.filter && (filter_first || o.dim == 3) && clearly_outside(o.dim, P, p)) return false;
+    if (o.dim == 2) {
+        double r[2];
+        if (inverse_closed_form_2d(P, p, r)) {
+            const double eps = 1e-15;
+            if (-eps <= r[1] && r[1] <= 1 + eps && -eps <= r[0] && r[0] <= 1 + eps) {
+                out[0] = r[0];
+                out[1] = r[1];
+                return true;
+            }
+        }
+        if (o.filter && !filter_first && clearly_outside(o.dim, P, p)) return false;
+    }
+    return inverse_newton(o.dim, P, p, out, counter);
+}
+
+inline bool inside_unit(int dim, const double* xi) {
+    for (int d = 0; d < dim; ++d)
+        if (!(xi[d] >= 0.0 && xi[d] <= 1.0)) return false;
+    return true;
+}
+
+// ---- connectivity ----------------------------------------------------------
+
+void build_star(Oracle& o) {
+    const int nv = o.n_vertices, nc = o.n_cells, k = o.nvpc, dim = o.dim;
+    o.star_off.assign(nv + 1, 0);
+    for (int c = 0; c < nc; ++c)
+        for (int j = 0; j < k; ++j) o.star_off[o.cell_vertices[(size_t)c * k + j] + 1]++;
+    for (int v = 0; v < nv; ++v) o.star_off[v + 1] += o.star_off[v];
+    o.star_cell.assign(o.star_off[nv], 0);
+    std::vector<int> fill(o.star_off.begin(), o.star_off.end() - 1);
+    for (int c = 0; c < nc; ++c)   // ascending c => each star list ascending
+        for (int j = 0; j < k; ++j) {
+            const int v = o.cell_vertices[(size_t)c * k + j];
+            // a cell appears once per vertex it contains
+            o.star_cell[fill[v]++] = c;
+        }
+    o.star_dir.assign((size_t)o.star_off[nv] * dim, 0.0);
+    (void)dim;
+}
+
+// vertex_to_cell_centers_directions restated: normalise(centre(cell) - X[v])
+void compute_star_dirs(Oracle& o) {
+    const int dim = o.dim;
+    for (int v = 0; v < o.n_vertices; ++v)
+        for (int e = o.star_off[v]; e < o.star_off[v + 1]; ++e) {
+            const int c = o.star_cell[e];
+            double ctr[3] = {0, 0, 0};
+            for (int k = 0; k < o.nvpc; ++k) {
+                const int gv = o.cell_vertices[(size_t)c * o.nvpc + k];
+                for (int d = 0; d < dim; ++d) ctr[d] += o.X[(size_t)gv * dim + d];
+            }
+            const double inv_n = 1.0 / o.nvpc;
+            double t[3];
+            for (int d = 0; d < dim; ++d) t[d] = ctr[d] * inv_n - o.X[(size_t)v * dim + d];
+            const double r = 1.0 / norm(dim, t);
+            for (int d = 0; d < dim; ++d) o.star_dir[(size_t)e * dim + d] = t[d] * r;
+        }
+}
+
+// ---- particle stage -------------------------------------------------------
+
+void interp(const Oracle& o, const std::vector<double>& F, int c, const double* xi, double* out) {
+    double N[MAXV];
+    shape_values(o.dim, xi, N);
+    for (int d = 0; d < o.dim; ++d) out[d] = 0.0;
+    for (int k = 0; k < o.nvpc; ++k) {
+        const int dof = o.cell_dofs[(size_t)c * o.nvpc + k];
+        for (int d = 0; d < o.dim; ++d) out[d] += N[k] * F[(size_t)d * o.n_dofs + dof];
+    }
+}
+
+Particle make_particle(Oracle& o, int c, const double* xi) {
+    Particle p;
+    std::memset(&p, 0, sizeof(p));
+    double P[MAXV * 3];
+    gather_cell_vertices(o, c, P);
+    for (int d = 0; d < o.dim; ++d) p.xi[d] = xi[d];
+    forward_map(o.dim, P, p.xi, p.x);
+    interp(o, o.V, c, p.xi, p.v);   // velocity starts at 0 and accumulates N_k * V_k
+    p.id = ++o.particle_count;
+    return p;
+}
+
+void lattice_point(const Oracle& o, const int* ijk, double* xi) {
+    for (int d = 0; d < o.dim; ++d) {
+        const double h = 1.0 / o.quant[d];
+        xi[d] = (ijk[d] + 1.0 / 2) * h;
+    }
+}
+
+// (cell, reference point) of a particle after its position changed;
+// returns -1 when the reference rule cannot place it (particle is deleted)
+int locate(Oracle& o, int c, Particle& p) {
+    const int dim = o.dim;
+    double P[MAXV * 3], r[3];
+    gather_cell_vertices(o, c, P);
+    if (inverse_map(o, P, p.x, r, &o.n_newton, false) && inside_unit(dim, r)) {
+        for (int d = 0; d < dim; ++d) p.xi[d] = r[d];
+        return c;
+    }
+    ++o.n_moved;
+    // closest vertex in the direction of motion (pfem2Particle::find_closest_vertex_of_cell)
+    double xold[3];
+    forward_map(dim, P, p.xi, xold);
+    double vn[3], t[3];
+    const double rv = 1.0 / norm(dim, p.vext);
+    for (int d = 0; d < dim; ++d) vn[d] = p.vext[d] * rv;
+    for (int d = 0; d < dim; ++d) t[d] = P[d] - xold[d];
+    double rn = 1.0 / norm(dim, t);
+    for (int d = 0; d < dim; ++d) t[d] *= rn;
+    double best = dot(dim, vn, t);
+    int cv = 0;
+    for (int k = 1; k < o.nvpc; ++k) {
+        for (int d = 0; d < dim; ++d) t[d] = P[k * dim + d] - xold[d];
+        rn = 1.0 / norm(dim, t);
+        for (int d = 0; d < dim; ++d) t[d] *= rn;
+        const double a = dot(dim, vn, t);
+        if (a > best) {
+            cv = k;
+            best = a;
+        }
+    }
+    const int gv = o.cell_vertices[(size_t)c * o.nvpc + cv];
+    double dir[3];
+    for (int d = 0; d < dim; ++d) dir[d] = p.x[d] - o.X[(size_t)gv * dim + d];
+    rn = 1.0 / norm(dim, dir);
+    for (int d = 0; d < dim; ++d) dir[d] *= rn;
+    const int s0 = o.star_off[gv], n = o.star_off[gv + 1] - s0;
+    int perm[64];
+    double key[64];
+    if (n > 64) throw std::runtime_error("vertex star larger than 64 cells");
+    for (int i = 0; i < n; ++i) {
+        perm[i] = i;
+        key[i] = dot(dim, &o.star_dir[(size_t)(s0 + i) * dim], dir);
+    }
+    // std::sort with comp(a, b) = key[a] > key[b]; for n <= 16 libstdc++ runs
+    // this insertion sort, which keeps equal keys in index order
+    for (int i = 1; i < n; ++i) {
+        const int val = perm[i];
+        if (key[val] > key[perm[0]]) {
+            for (int j = i; j > 0; --j) perm[j] = perm[j - 1];
+            perm[0] = val;
+        } else {
+            int j = i;
+            while (key[val] > key[perm[j - 1]]) {
+                perm[j] = perm[j - 1];
+                --j;
+            }
+            perm[j] = val;
+        }
+    }
+    for (int i = 0; i < n; ++i) {
+        const int cc = o.star_cell[s0 + perm[i]];
+        double Q[MAXV * 3];
+        gather_cell_vertices(o, cc, Q);
+        if (inverse_map(o, Q, p.x, r, &o.n_newton, true) && inside_unit(dim, r)) {
+            for (int d = 0; d < dim; ++d) p.xi[d] = r[d];
+            return cc;
+        }
+    }
+    return -1;
+}
+
+void sort_particles(Oracle& o) {
+    std::vector<std::vector<Particle>> next(o.n_cells);
+    for (int c = 0; c < o.n_cells; ++c)
+        for (Particle& p : o.cells[c]) {
+            const int nc = locate(o, c, p);
+            if (nc < 0) {
+                ++o.n_lost;
+                continue;
+            }
+            next[nc].push_back(p);
+        }
+    for (auto& lst : next)
+        std::sort(lst.begin(), lst.end(), [](const Particle& a, const Particle& b) { return a.id < b.id; });
+    o.cells.swap(next);
+}
+
+inline int part_key(const Oracle& o, const double* xi) {
+    int key = 0, stride = 1;
+    for (int d = 0; d < o.dim; ++d) {
+        const double h = 1.0 / o.quant[d];
+        const unsigned num = (unsigned)(xi[d] / h);   // truncation, no clamp
+        key += (int)num * stride;
+        stride *= o.quant[d] + 1;
+    }
+    return key;
+}
+
+void check_cell_for_empty_parts(Oracle& o, int c) {
+    std::vector<Particle>& lst = o.cells[c];
+    int nparts = 1;
+    for (int d = 0; d < o.dim; ++d) nparts *= o.quant[d] + 1;
+    std::vector<int> cnt(nparts, 0);
+    std::vector<char> drop(lst.size(), 0);
+    for (size_t i = 0; i < lst.size(); ++i) {
+        const int key = part_key(o, lst[i].xi);
+        if (++cnt[key] > o.max_per_part) drop[i] = 1;
+    }
+    std::vector<Particle> out;
+    out.reserve(lst.size() + 8);
+    for (size_t i = 0; i < lst.size(); ++i)
+        if (!drop[i]) out.push_back(lst[i]);
+        else ++o.n_deleted;
+    const int qz = o.dim == 3 ? o.quant[2] : 1;
+    for (int i = 0; i < o.quant[0]; ++i)
+        for (int j = 0; j < o.quant[1]; ++j)
+            for (int k = 0; k < qz; ++k) {
+                const int key = i + j * (o.quant[0] + 1) + k * (o.quant[0] + 1) * (o.quant[1] + 1);
+                if (cnt[key] == 0) {
+                    int ijk[3] = {i, j, k};
+                    double xi[3];
+                    lattice_point(o, ijk, xi);
+                    out.push_back(make_particle(o, c, xi));
+                    ++o.n_spawned;
+                }
+            }
+    lst.swap(out);
+}
+
+}  // namespace
+
+// ---- C interface (ctypes) --------------------------------------------------
+
+extern "C" {
+
+void* orc_create(int dim, int n_cells, int n_vertices, const double* vertices, const int* cell_vertices,
+                 int n_dofs, const int* cell_dofs) {
+    if (dim != 2 && dim != 3) return nullptr;
+    Oracle* o = new Oracle();
+    o->dim = dim;
+    o->nvpc = 1 << dim;
+    o->n_cells = n_cells;
+    o->n_vertices = n_vertices;
+    o->n_dofs = n_dofs;
+    o->X.assign(vertices, vertices + (size_t)n_vertices * dim);
+    o->cell_vertices.assign(cell_vertices, cell_vertices + (size_t)n_cells * o->nvpc);
+    if (cell_dofs) o->cell_dofs.assign(cell_dofs, cell_dofs + (size_t)n_cells * o->nvpc);
+    else o->cell_dofs = o->cell_vertices;
+    o->V.assign((size_t)dim * n_dofs, 0.0);
+    o->Vold.assign((size_t)dim * n_dofs, 0.0);
+    o->cells.assign(n_cells, {});
+    build_star(*o);
+    compute_star_dirs(*o);
+    return o;
+}
+
+void orc_destroy(void* h) { delete static_cast<Oracle*>(h); }
+
+void orc_set_params(void* h, double dt, int substeps, int max_per_part, const int* quantities) {
+    Oracle& o = *static_cast<Oracle*>(h);
+    o.dt = dt;
+    o.substeps = substeps;
+    o.max_per_part = max_per_part;
+    for (int d = 0; d < o.dim; ++d) o.quant[d] = quantities[d];
+}
+
+void orc_set_filter(void* h, int on) { static_cast<Oracle*>(h)->filter = on != 0; }
+
+void orc_set_fields(void* h, const double* V, const double* Vold) {
+    Oracle& o = *static_cast<Oracle*>(h);
+    const size_t n = (size_t)o.dim * o.n_dofs;
+    if (V) o.V.assign(V, V + n);
+    if (Vold) o.Vold.assign(Vold, Vold + n);
+}
+
+void orc_get_field(void* h, double* V) {
+    Oracle& o = *static_cast<Oracle*>(h);
+    std::memcpy(V, o.V.data(), sizeof(double) * o.V.size());
+}
+
+// pfem2Solver::seed_particles (cylinder2D/pfem2particle.cpp:563-580)
+void orc_seed(void* h) {
+    Oracle& o = *static_cast<Oracle*>(h);
+    const int qz = o.dim == 3 ? o.quant[2] : 1;
+    for (int c = 0; c < o.n_cells; ++c)
+        for (int i = 0; i < o.quant[0]; ++i)
+            for (int j = 0; j < o.quant[1]; ++j)
+                for (int k = 0; k < qz; ++k) {
+                    int ijk[3] = {i, j, k};
+                    double xi[3];
+                    lattice_point(o, ijk, xi);
+                    o.cells[c].push_back(make_particle(o, c, xi));
+                }
+}
+
+// insert particles given by (cell, reference point); velocity interpolated
+// from V when v is null; ids continue the particle counter
+void orc_insert(void* h, int n, const int* cell, const double* xi, const double* v) {
+    Oracle& o = *static_cast<Oracle*>(h);
+    for (int i = 0; i < n; ++i) {
+        Particle p = make_particle(o, cell[i], xi + (size_t)i * o.dim);
+        if (v)
+            for (int d = 0; d < o.dim; ++d) p.v[d] = v[(size_t)i * o.dim + d];
+        o.cells[cell[i]].push_back(p);
+    }
+    for (auto& lst : o.cells)
+        std::sort(lst.begin(), lst.end(), [](const Particle& a, const Particle& b) { return a.id < b.id; });
+}
+
+// correct_particles_velocities (TurekBenchmark/pfem2particle.cpp:792-811)
+void orc_correct(void* h) {
+    Oracle& o = *static_cast<Oracle*>(h);
+    double N[MAXV];
+    for (int c = 0; c < o.n_cells; ++c)
+        for (Particle& p : o.cells[c]) {
+            shape_values(o.dim, p.xi, N);
+            for (int k = 0; k < o.nvpc; ++k) {
+                const int dof = o.cell_dofs[(size_t)c * o.nvpc + k];
+                for (int d = 0; d < o.dim; ++d)
+                    p.v[d] = p.v[d] + N[k] * (o.V[(size_t)d * o.n_dofs + dof] - o.Vold[(size_t)d * o.n_dofs + dof]);
+            }
+        }
+}
+
+// particle_handler.sort_particles_into_subdomains_and_cells
+void orc_sort(void* h) { sort_particles(*static_cast<Oracle*>(h)); }
+
+// check_empty_cells (cylinderDyM/pfem2particle.cpp:759-763)
+void orc_check_empty_cells(void* h) {
+    Oracle& o = *static_cast<Oracle*>(h);
+    for (int c = 0; c < o.n_cells; ++c) check_cell_for_empty_parts(o, c);
+}
+
+// move_particles (TurekBenchmark/pfem2particle.cpp:813-856)
+void orc_move(void* h) {
+    Oracle& o = *static_cast<Oracle*>(h);
+    const double hstep = o.dt / o.substeps;
+    for (int s = 0; s < o.substeps; ++s) {
+        for (int c = 0; c < o.n_cells; ++c)
+            for (Particle& p : o.cells[c]) {
+                double u[3];
+                interp(o, o.V, c, p.xi, u);
+                for (int d = 0; d < o.dim; ++d) p.vext[d] = u[d];
+                for (int d = 0; d < o.dim; ++d) u[d] *= hstep;
+                for (int d = 0; d < o.dim; ++d) p.x[d] = p.x[d] + u[d];
+            }
+        sort_particles(o);
+    }
+    orc_check_empty_cells(h);
+}
+
+// distribute_particle_velocities_to_grid (TurekBenchmark/pfem2particle.cpp:858-918)
+// followed by the wall fix-up (cylinderDyM/pfem2particle.cpp:960-992)
+void orc_project(void* h, double* Vout) {
+    Oracle& o = *static_cast<Oracle*>(h);
+    const int dim = o.dim;
+    std::vector<double> nodeV((size_t)dim * o.n_dofs, 0.0), nodeW(o.n_dofs, 0.0);
+    double N[MAXV];
+    for (int c = 0; c < o.n_cells; ++c) {
+        double lv[MAXV][3], lw[MAXV];
+        for (int k = 0; k < o.nvpc; ++k) {
+            lw[k] = 0.0;
+            for (int d = 0; d < dim; ++d) lv[k][d] = 0.0;
+        }
+        for (int k = 0; k < o.nvpc; ++k)
+            for (const Particle& p : o.cells[c]) {
+                shape_values(dim, p.xi, N);
+                for (int d = 0; d < dim; ++d) lv[k][d] += N[k] * p.v[d];
+                lw[k] += N[k];
+            }
+        for (int k = 0; k < o.nvpc; ++k) {
+            const int dof = o.cell_dofs[(size_t)c * o.nvpc + k];
+            for (int d = 0; d < dim; ++d) nodeV[(size_t)d * o.n_dofs + dof] += lv[k][d];
+            nodeW[dof] += lw[k];
+        }
+    }
+    for (int i = 0; i < o.n_dofs; ++i)
+        for (int d = 0; d < dim; ++d) nodeV[(size_t)d * o.n_dofs + i] /= nodeW[i];
+    o.V = nodeV;
+    // wall fix-up, wall nodes in ascending vertex order
+    for (const Wall& w : o.walls) {
+        double val[3];
+        bool fix = false;
+        for (int d = 0; d < dim; ++d) {
+            val[d] = o.V[(size_t)d * o.n_dofs + w.dof];
+            if (((w.mask >> d) & 1) && std::fabs(val[d] - w.bc[d]) > 1e-5) fix = true;
+        }
+        if (fix)
+            for (int e = o.star_off[w.vertex]; e < o.star_off[w.vertex + 1]; ++e) {
+                const int c = o.star_cell[e];
+                int vno = -1;
+                for (int k = 0; k < o.nvpc; ++k)
+                    if (o.cell_vertices[(size_t)c * o.nvpc + k] == w.vertex) {
+                        vno = k;
+                        break;
+                    }
+                if (vno < 0) continue;
+                for (Particle& p : o.cells[c]) {
+                    shape_values(dim, p.xi, N);
+                    for (int d = 0; d < dim; ++d)
+                        if ((w.mask >> d) & 1) p.v[d] = p.v[d] + N[vno] * (w.bc[d] - val[d]);
+                }
+            }
+        for (int d = 0; d < dim; ++d)
+            if ((w.mask >> d) & 1) o.V[(size_t)d * o.n_dofs + w.dof] = w.bc[d];
+    }
+    // the driver's FEM step starts with old_solution = solution
+    // (cylinder2D/cylinder2D.cc:389-391): V_old is the projected field
+    o.Vold = o.V;
+    if (Vout) std::memcpy(Vout, o.V.data(), sizeof(double) * o.V.size());
+}
+
+// wall nodes: vertex index, dof index, boundary values (n x dim), component mask
+void orc_set_walls(void* h, int n, const int* vertex, const int* dof, const double* bc, const int* mask) {
+    Oracle& o = *static_cast<Oracle*>(h);
+    o.walls.clear();
+    for (int i = 0; i < n; ++i) {
+        Wall w;
+        w.vertex = vertex[i];
+        w.dof = dof[i];
+        for (int d = 0; d < 3; ++d) w.bc[d] = d < o.dim ? bc[(size_t)i * o.dim + d] : 0.0;
+        w.mask = mask[i];
+        o.walls.push_back(w);
+    }
+    std::sort(o.walls.begin(), o.walls.end(), [](const Wall& a, const Wall& b) { return a.vertex < b.vertex; });
+}
+
+// one full particle step: correct -> move -> project
+void orc_step(void* h, double* Vout) {
+    orc_correct(h);
+    orc_move(h);
+    orc_project(h, Vout);
+}
+
+// moving mesh: replace vertex coordinates (transform_grid); star directions
+// are recomputed only when recompute_maps != 0 (initialize_maps)
+void orc_update_vertices(void* h, const double* vertices, int recompute_maps) {
+    Oracle& o = *static_cast<Oracle*>(h);
+    o.X.assign(vertices, vertices + (size_t)o.n_vertices * o.dim);
+    if (recompute_maps) compute_star_dirs(o);
+}
+
+void orc_initialize_maps(void* h) { compute_star_dirs(*static_cast<Oracle*>(h)); }
+
+long long orc_count(void* h) {
+    Oracle& o = *static_cast<Oracle*>(h);
+    long long n = 0;
+    for (auto& l : o.cells) n += (long long)l.size();
+    return n;
+}
+
+uint32_t orc_particle_counter(void* h) { return static_cast<Oracle*>(h)->particle_count; }
+
+// cell-major, id-ascending within a cell; any output pointer may be null
+void orc_download(void* h, int* cell, uint32_t* id, double* x, double* xi, double* v) {
+    Oracle& o = *static_cast<Oracle*>(h);
+    size_t i = 0;
+    const int dim = o.dim;
+    for (int c = 0; c < o.n_cells; ++c)
+        for (const Particle& p : o.cells[c]) {
+            if (cell) cell[i] = c;
+            if (id) id[i] = p.id;
+            for (int d = 0; d < dim; ++d) {
+                if (x) x[i * dim + d] = p.x[d];
+                if (xi) xi[i * dim + d] = p.xi[d];
+                if (v) v[i * dim + d] = p.v[d];
+            }
+            ++i;
+        }
+}
+
+void orc_cell_counts(void* h, int* counts) {
+    Oracle& o = *static_cast<Oracle*>(h);
+    for (int c = 0; c < o.n_cells; ++c) counts[c] = (int)o.cells[c].size();
+}
+
+void orc_stats(void* h, unsigned long long* out /* 5 */) {
+    Oracle& o = *static_cast<Oracle*>(h);
+    out[0] = o.n_lost;
+    out[1] = o.n_deleted;
+    out[2] = o.n_spawned;
+    out[3] = o.n_moved;
+    out[4] = o.n_newton;
+}
+
+// ---- geometry probes for tests --------------------------------------------
+
+// returns 1 and writes xi when the point maps into the cell's unit cell
+// neighbourhood (deal.II semantics), 0 when the mapping fails
+int orc_inverse_map(int dim, const double* P, const double* p, int use_filter, double* xi) {
+    Oracle o;
+    o.dim = dim;
+    o.filter = use_filter != 0;
+    return inverse_map(o, P, p, xi, nullptr, use_filter == 2) ? 1 : 0;
+}
+
+int orc_inverse_newton(int dim, const double* P, const double* p, double* xi) {
+    return inverse_newton(dim, P, p, xi, nullptr) ? 1 : 0;
+}
+
+int orc_inverse_closed_form(const double* P, const double* p, double* xi) {
+    return inverse_closed_form_2d(P, p, xi) ? 1 : 0;
+}
+
+int orc_clearly_outside(int dim, const double* P, const double* p) { return clearly_outside(dim, P, p) ? 1 : 0; }
+
+void orc_forward_map(int dim, const double* P, const double* xi, double* x) { forward_map(dim, P, xi, x); }
+
+void orc_shape_values(int dim, const double* xi, double* N) { shape_values(dim, xi, N); }
+
+// star of a vertex: writes up to cap entries, returns the star size
+int orc_star(void* h, int vertex, int* cells, double* dirs, int cap) {
+    Oracle& o = *static_cast<Oracle*>(h);
+    const int s0 = o.star_off[vertex], n = o.star_off[vertex + 1] - s0;
+    for (int i = 0; i < n && i < cap; ++i) {
+        cells[i] = o.star_cell[s0 + i];
+        for (int d = 0; d < o.dim; ++d) dirs[i * o.dim + d] = o.star_dir[(size_t)(s0 + i) * o.dim + d];
+    }
+    return n;
+}
+
+// wall-clock of `steps` particle steps, in seconds (CPU baseline timing)
+double orc_time_steps(void* h, int steps) {
+    auto t0 = std::chrono::steady_clock::now();
+    for (int s = 0; s < steps; ++s) orc_step(h, nullptr);
+    auto t1 = std::chrono::steady_clock::now();
+    return std::chrono::duration<double>(t1 - t0).count();
+}
+
+}  // extern "C"

@@ -1,0 +1,309 @@
+"""pfem2_amd — the PFEM2 particle stage on MI355X (gfx950), driven through a C ABI.
+
+The native library `libpfem2_amd.so` (hand-written HIP, built in-tree by
+`python -m pfem2_amd.build`) holds one simulation per handle.  This module is
+the Python mirror of the reference's particle interface
+(pfem2Solver / pfem2ParticleHandler in pfem2particle.h): the method names
+are the reference's, the mesh and the nodal vectors are numpy arrays.
+
+There is no fallback: if the library is missing or no GPU is visible the
+calls raise.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from . import mesh  # noqa: F401
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libpfem2_amd.so")
+
+FP64 = 0
+FP32_MIXED = 1
+
+_lib = None
+
+
+class Pf2Error(RuntimeError):
+    pass
+
+
+class _Mesh(ctypes.Structure):
+    _fields_ = [("dim", ctypes.c_int), ("n_cells", ctypes.c_int), ("n_vertices", ctypes.c_int),
+                ("n_dofs", ctypes.c_int), ("vertices", ctypes.c_void_p), ("cell_vertices", ctypes.c_void_p),
+                ("cell_dofs", ctypes.c_void_p)]
+
+
+class _Params(ctypes.Structure):
+    _fields_ = [("dt", ctypes.c_double), ("substeps", ctypes.c_int), ("max_per_part", ctypes.c_int),
+                ("quantities", ctypes.c_int * 3), ("precision", ctypes.c_int),
+                ("keep_velocity_ext", ctypes.c_int)]
+
+
+# every entry point of include/pfem2_amd.h
+EXPORTS = [
+    "pf2_last_error", "pf2_create", "pf2_destroy", "pf2_set_params", "pf2_upload_fields",
+    "pf2_upload_fields_device", "pf2_download_field", "pf2_seed_particles", "pf2_insert_particles",
+    "pf2_correct_particles_velocities", "pf2_move_particles", "pf2_distribute_particle_velocities_to_grid",
+    "pf2_step", "pf2_set_walls", "pf2_sort_particles_into_subdomains_and_cells", "pf2_check_empty_cells",
+    "pf2_update_vertices", "pf2_initialize_maps", "pf2_n_global_particles", "pf2_download_particles",
+    "pf2_cell_counts", "pf2_particle_counter", "pf2_stats", "pf2_synchronize", "pf2_set_profiling",
+    "pf2_kernel_times", "pf2_kernel_name",
+]
+
+
+def load_library(path=LIB_PATH):
+    """Load the HIP library (raises if it is not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise Pf2Error(f"{path} is missing: build it with `python -m pfem2_amd.build`")
+    lib = ctypes.CDLL(path)
+    vp, ip, dp = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
+    lib.pf2_last_error.restype = ctypes.c_char_p
+    lib.pf2_kernel_name.restype = ctypes.c_char_p
+    lib.pf2_kernel_name.argtypes = [ip]
+    lib.pf2_create.argtypes = [ctypes.POINTER(_Mesh), ctypes.POINTER(_Params), ctypes.c_longlong,
+                               ctypes.POINTER(vp)]
+    lib.pf2_destroy.argtypes = [vp]
+    lib.pf2_set_params.argtypes = [vp, ctypes.POINTER(_Params)]
+    lib.pf2_upload_fields.argtypes = [vp, vp, vp]
+    lib.pf2_upload_fields_device.argtypes = [vp, vp, vp]
+    lib.pf2_download_field.argtypes = [vp, vp]
+    lib.pf2_seed_particles.argtypes = [vp]
+    lib.pf2_insert_particles.argtypes = [vp, ip, vp, vp, vp]
+    for f in ("pf2_correct_particles_velocities", "pf2_move_particles",
+              "pf2_sort_particles_into_subdomains_and_cells", "pf2_check_empty_cells", "pf2_initialize_maps",
+              "pf2_synchronize"):
+        getattr(lib, f).argtypes = [vp]
+    lib.pf2_distribute_particle_velocities_to_grid.argtypes = [vp, vp]
+    lib.pf2_step.argtypes = [vp, vp]
+    lib.pf2_set_walls.argtypes = [vp, ip, vp, vp, vp, vp]
+    lib.pf2_update_vertices.argtypes = [vp, vp, ip]
+    lib.pf2_n_global_particles.argtypes = [vp, ctypes.POINTER(ctypes.c_longlong)]
+    lib.pf2_download_particles.argtypes = [vp, vp, vp, vp, vp, vp]
+    lib.pf2_cell_counts.argtypes = [vp, vp]
+    lib.pf2_particle_counter.argtypes = [vp, ctypes.POINTER(ctypes.c_uint32)]
+    lib.pf2_stats.argtypes = [vp, vp]
+    lib.pf2_set_profiling.argtypes = [vp, ip]
+    lib.pf2_kernel_times.argtypes = [vp, vp, vp, ip]
+    _ = dp
+    _lib = lib
+    return lib
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _f64(a, shape=None):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    if shape is not None and a.shape != shape:
+        a = a.reshape(shape)
+    return a
+
+
+class ParticleStage:
+    """One simulation's particle stage (one pf2 handle).
+
+    Methods carry the reference's names: seed_particles,
+    correct_particles_velocities, move_particles,
+    distribute_particle_velocities_to_grid (pfem2Solver), and
+    sort_particles_into_subdomains_and_cells, initialize_maps,
+    n_global_particles (pfem2ParticleHandler).
+    """
+
+    def __init__(self, m, dt, substeps, max_per_part, quantities, precision=FP64, keep_velocity_ext=False,
+                 capacity=0):
+        self._lib = load_library()
+        self.mesh = m
+        self.dim = m.dim
+        self._vertices = _f64(m.vertices)
+        self._cells = np.ascontiguousarray(m.cells, dtype=np.int32)
+        q = list(quantities) + [1] * (3 - len(quantities))
+        self._params = _Params(dt, substeps, max_per_part, (ctypes.c_int * 3)(*q[:3]), precision,
+                               1 if keep_velocity_ext else 0)
+        ms = _Mesh(m.dim, m.n_cells, m.n_vertices, m.n_dofs, _ptr(self._vertices), _ptr(self._cells), None)
+        h = ctypes.c_void_p()
+        self._check(self._lib.pf2_create(ctypes.byref(ms), ctypes.byref(self._params), capacity, ctypes.byref(h)))
+        self._h = h
+
+    # -- plumbing --
+    def _check(self, rc):
+        if rc != 0:
+            raise Pf2Error(f"pf2 error {rc}: {self._lib.pf2_last_error().decode()}")
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.pf2_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # -- parameters / fields --
+    def set_params(self, dt=None, substeps=None, max_per_part=None, quantities=None):
+        p = self._params
+        if dt is not None:
+            p.dt = dt
+        if substeps is not None:
+            p.substeps = substeps
+        if max_per_part is not None:
+            p.max_per_part = max_per_part
+        if quantities is not None:
+            q = list(quantities) + [1] * (3 - len(quantities))
+            p.quantities = (ctypes.c_int * 3)(*q[:3])
+        self._check(self._lib.pf2_set_params(self._h, ctypes.byref(p)))
+
+    def upload_fields(self, V=None, V_old=None):
+        """Nodal velocity V (solutionVx/Vy[/Vz]) and V_old, shape (dim, n_dofs)."""
+        shape = (self.dim, self.mesh.n_dofs)
+        a = None if V is None else _f64(V, shape)
+        b = None if V_old is None else _f64(V_old, shape)
+        self._check(self._lib.pf2_upload_fields(self._h, _ptr(a), _ptr(b)))
+
+    def download_field(self):
+        out = np.empty((self.dim, self.mesh.n_dofs))
+        self._check(self._lib.pf2_download_field(self._h, _ptr(out)))
+        return out
+
+    # -- pfem2Solver particle methods --
+    def seed_particles(self):
+        self._check(self._lib.pf2_seed_particles(self._h))
+
+    def insert_particles(self, cell, xi, v=None):
+        cell = np.ascontiguousarray(cell, dtype=np.int32)
+        xi = _f64(xi, (len(cell), self.dim))
+        vv = None if v is None else _f64(v, (len(cell), self.dim))
+        self._check(self._lib.pf2_insert_particles(self._h, len(cell), _ptr(cell), _ptr(xi), _ptr(vv)))
+
+    def correct_particles_velocities(self):
+        self._check(self._lib.pf2_correct_particles_velocities(self._h))
+
+    def move_particles(self):
+        self._check(self._lib.pf2_move_particles(self._h))
+
+    def distribute_particle_velocities_to_grid(self, download=True):
+        out = np.empty((self.dim, self.mesh.n_dofs)) if download else None
+        self._check(self._lib.pf2_distribute_particle_velocities_to_grid(self._h, _ptr(out)))
+        return out
+
+    def step(self, download=False):
+        out = np.empty((self.dim, self.mesh.n_dofs)) if download else None
+        self._check(self._lib.pf2_step(self._h, _ptr(out)))
+        return out
+
+    def check_empty_cells(self):
+        self._check(self._lib.pf2_check_empty_cells(self._h))
+
+    def set_walls(self, vertex, dof, bc, mask):
+        vertex = np.ascontiguousarray(vertex, dtype=np.int32)
+        dof = np.ascontiguousarray(dof, dtype=np.int32)
+        bc = _f64(bc, (len(vertex), self.dim))
+        mask = np.ascontiguousarray(mask, dtype=np.int32)
+        self._check(self._lib.pf2_set_walls(self._h, len(vertex), _ptr(vertex), _ptr(dof), _ptr(bc), _ptr(mask)))
+
+    def update_vertices(self, vertices, recompute_maps=False):
+        self._vertices = _f64(vertices, (self.mesh.n_vertices, self.dim))
+        self._check(self._lib.pf2_update_vertices(self._h, _ptr(self._vertices), 1 if recompute_maps else 0))
+
+    # -- pfem2ParticleHandler methods --
+    def sort_particles_into_subdomains_and_cells(self):
+        self._check(self._lib.pf2_sort_particles_into_subdomains_and_cells(self._h))
+
+    def initialize_maps(self):
+        self._check(self._lib.pf2_initialize_maps(self._h))
+
+    def n_global_particles(self):
+        n = ctypes.c_longlong()
+        self._check(self._lib.pf2_n_global_particles(self._h, ctypes.byref(n)))
+        return n.value
+
+    def particles(self):
+        """cell, id, x, xi, v of all particles (cell-major, ascending id inside a cell)."""
+        n = self.n_global_particles()
+        cell = np.empty(n, dtype=np.int32)
+        pid = np.empty(n, dtype=np.uint32)
+        x = np.empty((n, self.dim))
+        xi = np.empty((n, self.dim))
+        v = np.empty((n, self.dim))
+        self._check(self._lib.pf2_download_particles(self._h, _ptr(cell), _ptr(pid), _ptr(x), _ptr(xi), _ptr(v)))
+        return {"cell": cell, "id": pid, "x": x, "xi": xi, "v": v}
+
+    def cell_counts(self):
+        out = np.empty(self.mesh.n_cells, dtype=np.int32)
+        self._check(self._lib.pf2_cell_counts(self._h, _ptr(out)))
+        return out
+
+    def particle_counter(self):
+        c = ctypes.c_uint32()
+        self._check(self._lib.pf2_particle_counter(self._h, ctypes.byref(c)))
+        return c.value
+
+    def stats(self):
+        out = np.zeros(5, dtype=np.uint64)
+        self._check(self._lib.pf2_stats(self._h, _ptr(out)))
+        return dict(zip(("lost", "deleted", "spawned", "searches", "zero_weight"), out.tolist()))
+
+    def synchronize(self):
+        self._check(self._lib.pf2_synchronize(self._h))
+
+    # -- per-kernel HIP-event timing --
+    def set_profiling(self, on=True):
+        self._check(self._lib.pf2_set_profiling(self._h, 1 if on else 0))
+
+    def kernel_times(self):
+        ms = np.zeros(32)
+        n = np.zeros(32, dtype=np.int64)
+        k = self._lib.pf2_kernel_times(self._h, _ptr(ms), _ptr(n), 32)
+        if k < 0:
+            self._check(k)
+        return {self._lib.pf2_kernel_name(i).decode(): (float(ms[i]), int(n[i])) for i in range(k) if n[i] > 0}
+
+
+class Pfem2Solver:
+    """Mirror of the reference's pfem2Solver particle interface
+    (cylinder2D/pfem2particle.h:115-192): the driver sets `solution` (nodal V,
+    shape (dim, n_dofs)) and calls the four particle methods in the
+    reference's order; `old_solution` is kept on the device and equals the
+    last projected field."""
+
+    def __init__(self, m, time_step, substeps=3, max_per_part=3, precision=FP64, keep_velocity_ext=False):
+        self.mesh = m
+        self.time_step = time_step
+        self._substeps = substeps
+        self._max = max_per_part
+        self._precision = precision
+        self._keep = keep_velocity_ext
+        self.stage = None
+        self.particle_handler = None
+
+    def seed_particles(self, quantities, V):
+        self.stage = ParticleStage(self.mesh, self.time_step, self._substeps, self._max, quantities,
+                                   self._precision, self._keep)
+        self.stage.upload_fields(V, V)
+        self.stage.seed_particles()
+        self.particle_handler = self.stage
+
+    def set_solution(self, V):
+        self.stage.upload_fields(V, None)
+
+    def correct_particles_velocities(self):
+        self.stage.correct_particles_velocities()
+
+    def move_particles(self):
+        self.stage.move_particles()
+
+    def distribute_particle_velocities_to_grid(self):
+        return self.stage.distribute_particle_velocities_to_grid(download=True)

@@ -1,0 +1,918 @@
+// pf2_kernels.hpp — device kernels of the particle stage (FP64 path).
+//
+// Data layout in HBM (one simulation):
+//   mesh     X[n_vertices*DIM] (row-major), cell_vertices / cell_dofs [n_cells*NV],
+//            vertex star CSR (ascending cells) + normalised centre directions,
+//            DoF -> (cell, local vertex) CSR in ascending cell order.
+//   fields   V, Vold [n_dofs*DIM] node-major (one DoF's components adjacent).
+//   particles structure of arrays, component-major (x[d*cap + i]), sorted by
+//            cell, ascending id inside a cell; cell_off[n_cells+1] is the CSR.
+// Per time step (pf2_step):
+//   k_move         one thread per particle: correction, then all substeps of
+//                  advection + relocation in registers; writes in place,
+//                  records the destination cell and the part key, appends
+//                  particles that changed cell to the movers list.
+//   k_group        movers grouped by destination cell (arrival lists).
+//   k_cell_count   one wavefront per cell: part counts over stayers +
+//                  arrivals, survivors (MAX smallest ids per part), empty
+//                  parts to reseed, final count.
+//   k_cell_write   one wavefront per cell: survivors and new particles into
+//                  the other buffer at their final positions.
+//   k_cell_partials one wavefront per cell: per-cell projection sums in
+//                  ascending id order.
+//   k_node         one thread per DoF: sum of the cell partials in ascending
+//                  cell order, then division by the weight.
+// No floating-point atomics anywhere; integer atomics only build lists whose
+// order is erased again by the id ordering.
+#pragma once
+
+#include "pf2_geom.hpp"
+#include <stdint.h>
+
+namespace pf2 {
+
+constexpr int WAVE = 64;
+
+template <int DIM>
+struct DevMesh {
+    int n_cells, n_vertices, n_dofs;
+    const double* X;
+    const int* cv;
+    const int* cd;
+    const int* star_off;
+    const int* star_cell;
+    const double* star_dir;
+    const int* dof_off;
+    const int* dof_cell;   // packed: cell * NV + local vertex
+    const double* ftol;    // per cell: tolerance of the clearly-outside test (-1: none)
+};
+
+struct DevParticles {
+    int* cell;
+    uint32_t* id;
+    double* x;
+    double* xi;
+    double* v;
+    double* vext;   // may be null
+    long long cap;
+};
+
+struct DevParams {
+    double h;           // dt / substeps
+    int substeps;
+    int max_per_part;
+    int q[3];
+    int nparts_in;      // q0*q1[*q2]
+    int keep_vext;
+};
+
+struct DevWork {
+    int* dest;          // per particle: new cell or -1
+    int* pkey;          // per particle: part key of the final position
+    int* spos;          // per particle: position among the cell's survivors, -1 = dropped
+    int* movers;        // particle indices that changed cell
+    int* arr_cnt;       // per cell: arrivals
+    int* arr_off;       // per cell: exclusive scan of arr_cnt
+    int* arr_fill;      // per cell: fill counter
+    int* arr_list;      // arrivals grouped by destination cell
+    int* final_cnt;     // per cell (n_cells + 1 for the scan)
+    int* spawn_cnt;
+    int* new_off;       // n_cells + 1
+    int* spawn_off;     // n_cells + 1
+    unsigned long long* hitmask;   // per cell: in-range parts that hold a particle
+    int* nsurv;         // per cell: survivors
+    double* partial;    // per cell: NV * (DIM + 1) projection sums
+    int* counters;      // [0] n_particles, [1] n_movers, [2] overflow flag
+    uint32_t* particle_counter;
+    unsigned long long* stats;   // lost, deleted, spawned, searches, zero-weight
+};
+
+template <int DIM>
+__device__ __forceinline__ void load_cell(const DevMesh<DIM>& m, const double* __restrict__ F, int c,
+                                          double (*P)[DIM], double (*Fc)[DIM]) {
+    constexpr int NV = Q1<DIM>::NV;
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        const int gv = m.cv[(size_t)c * NV + k];
+        const int dof = m.cd[(size_t)c * NV + k];
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) {
+            P[k][d] = m.X[(size_t)gv * DIM + d];
+            Fc[k][d] = F[(size_t)dof * DIM + d];
+        }
+    }
+}
+
+template <int DIM>
+__device__ __forceinline__ void load_vertices(const DevMesh<DIM>& m, int c, double (*P)[DIM]) {
+    constexpr int NV = Q1<DIM>::NV;
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        const int gv = m.cv[(size_t)c * NV + k];
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) P[k][d] = m.X[(size_t)gv * DIM + d];
+    }
+}
+
+// u = sum_k N_k * F_k accumulated from zero (velocity at a reference point)
+template <int DIM>
+__device__ __forceinline__ void interp(const double (*Fc)[DIM], const double* xi, double* u) {
+    double N[Q1<DIM>::NV];
+    shape<DIM>(xi, N);
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) u[d] = 0.0;
+#pragma unroll
+    for (int k = 0; k < Q1<DIM>::NV; ++k)
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) u[d] += N[k] * Fc[k][d];
+}
+
+template <int DIM>
+__device__ __forceinline__ int part_key(const DevParams& pp, const double* xi) {
+    int key = 0, stride = 1;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+        const double h = 1.0 / pp.q[d];
+        const unsigned num = (unsigned)(xi[d] / h);
+        key += (int)num * stride;
+        stride *= pp.q[d] + 1;
+    }
+    return key;
+}
+
+// ordinal of an in-range part in seeding order (first direction outermost), -1 for phantom parts
+template <int DIM>
+__device__ __forceinline__ int part_ordinal(const DevParams& pp, int key) {
+    int num[3];
+    int k = key;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+        num[d] = k % (pp.q[d] + 1);
+        k /= pp.q[d] + 1;
+    }
+    int o = 0;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+        if (num[d] >= pp.q[d]) return -1;
+        o = o * pp.q[d] + num[d];
+    }
+    return o;
+}
+
+template <int DIM>
+__device__ __forceinline__ void lattice_point(const DevParams& pp, int ordinal, double* xi) {
+    int num[3];
+    int o = ordinal;
+#pragma unroll
+    for (int d = DIM - 1; d >= 0; --d) {
+        num[d] = o % pp.q[d];
+        o /= pp.q[d];
+    }
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+        const double h = 1.0 / pp.q[d];
+        xi[d] = (num[d] + 1.0 / 2) * h;
+    }
+}
+
+// Relocation of a particle whose position left cell c
+// (pfem2ParticleHandler::sort_particles_into_subdomains_and_cells, the part
+// after the own-cell test, cylinder2D/pfem2particle.cpp:278-386).  The
+// reference sorts the star of the selected vertex by decreasing alignment
+// (stable for equal keys) and takes the first cell that contains the point;
+// that is the containing cell with the largest key, earliest star position on
+// ties, which is what the loop below selects.  Returns -1 if no star cell
+// contains the point.
+template <int DIM>
+__device__ int relocate(const DevMesh<DIM>& m, int c, const double (*P)[DIM], const double* xi_old,
+                        const double* vext, const double* x, double* xi_out) {
+    constexpr int NV = Q1<DIM>::NV;
+    double xold[DIM], vn[DIM], t[DIM];
+    forward<DIM>(P, xi_old, xold);
+    const double rv = 1.0 / vnorm<DIM>(vext);
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) vn[d] = vext[d] * rv;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) t[d] = P[0][d] - xold[d];
+    double rn = 1.0 / vnorm<DIM>(t);
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) t[d] *= rn;
+    double best = vdot<DIM>(vn, t);
+    int cvk = 0;
+#pragma unroll
+    for (int k = 1; k < NV; ++k) {
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) t[d] = P[k][d] - xold[d];
+        rn = 1.0 / vnorm<DIM>(t);
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) t[d] *= rn;
+        const double a = vdot<DIM>(vn, t);
+        if (a > best) {
+            cvk = k;
+            best = a;
+        }
+    }
+    const int gv = m.cv[(size_t)c * NV + cvk];
+    double dir[DIM];
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) dir[d] = x[d] - m.X[(size_t)gv * DIM + d];
+    rn = 1.0 / vnorm<DIM>(dir);
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) dir[d] *= rn;
+    const int s0 = m.star_off[gv], s1 = m.star_off[gv + 1];
+    int bestc = -1;
+    double bestkey = 0.0;
+    for (int e = s0; e < s1; ++e) {
+        double sd[DIM];
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) sd[d] = m.star_dir[(size_t)e * DIM + d];
+        const double key = vdot<DIM>(sd, dir);
+        if (bestc >= 0 && !(key > bestkey)) continue;
+        const int cc = m.star_cell[e];
+        double Q[NV][DIM], r[DIM];
+        load_vertices<DIM>(m, cc, Q);
+        if (inverse_map<DIM>(Q, x, r, m.ftol[cc], true) && inside_unit<DIM>(r)) {
+            bestc = cc;
+            bestkey = key;
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) xi_out[d] = r[d];
+        }
+    }
+    return bestc;
+}
+
+// ---------------------------------------------------------------------------
+// k_move: [correction] + substeps of advection and relocation, per particle.
+//   correct_particles_velocities  TurekBenchmark/pfem2particle.cpp:792-811
+//   advection                     TurekBenchmark/pfem2particle.cpp:822-846
+//   own-cell test + relocation    TurekBenchmark/pfem2particle.cpp:286-450
+// Particles are independent within move_particles, so the substeps of one
+// particle run back to back in registers.  A particle whose own-cell test
+// fails waits (its position already advanced) until at least SEARCH_BATCH
+// lanes of its wavefront wait, or no lane can advance; then the waiting lanes
+// run the vertex-star search together, so the expensive divergent branch is
+// taken a few times per wavefront instead of once per substep.
+constexpr int SEARCH_BATCH = 8;
+
+template <int DIM>
+__global__ void __launch_bounds__(256) k_move(DevMesh<DIM> m, const double* __restrict__ V,
+                                              const double* __restrict__ Vold, DevParticles A, DevParams pp,
+                                              DevWork w, int do_correct, int do_move) {
+    constexpr int NV = Q1<DIM>::NV;
+    const int n = w.counters[0];
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int stride = gridDim.x * blockDim.x;
+    int lost = 0, searches = 0;
+    for (int base = blockIdx.x * blockDim.x + (threadIdx.x - lane); base < n; base += stride) {
+        const int i = base + lane;
+        const bool valid = i < n;
+        const int c0 = valid ? A.cell[i] : 0;
+        int c = c0;
+        double x[DIM], xi[DIM];
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) xi[d] = valid ? A.xi[(size_t)d * A.cap + i] : 0.5;
+        if (do_correct && valid) {
+            double v[DIM], N[NV];
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) v[d] = A.v[(size_t)d * A.cap + i];
+            shape<DIM>(xi, N);
+#pragma unroll
+            for (int k = 0; k < NV; ++k) {
+                const int dof = m.cd[(size_t)c * NV + k];
+#pragma unroll
+                for (int d = 0; d < DIM; ++d)
+                    v[d] = v[d] + N[k] * (V[(size_t)dof * DIM + d] - Vold[(size_t)dof * DIM + d]);
+            }
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) A.v[(size_t)d * A.cap + i] = v[d];
+        }
+        if (!do_move) continue;
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) x[d] = valid ? A.x[(size_t)d * A.cap + i] : 0.0;
+        double P[NV][DIM], Vc[NV][DIM], vext[DIM], tol = -1.0;
+        load_cell<DIM>(m, V, c, P, Vc);
+        tol = m.ftol[c];
+        int s = valid ? 0 : pp.substeps;
+        bool need = false;
+        for (;;) {
+            if (s < pp.substeps && !need) {
+                double u[DIM], r[DIM];
+                interp<DIM>(Vc, xi, u);
+#pragma unroll
+                for (int d = 0; d < DIM; ++d) vext[d] = u[d];
+#pragma unroll
+                for (int d = 0; d < DIM; ++d) u[d] *= pp.h;
+#pragma unroll
+                for (int d = 0; d < DIM; ++d) x[d] = x[d] + u[d];
+                if (pp.keep_vext && s == pp.substeps - 1)
+#pragma unroll
+                    for (int d = 0; d < DIM; ++d) A.vext[(size_t)d * A.cap + i] = vext[d];
+                if (inverse_map<DIM>(P, x, r, tol, false) && inside_unit<DIM>(r)) {
+#pragma unroll
+                    for (int d = 0; d < DIM; ++d) xi[d] = r[d];
+                    ++s;
+                } else {
+                    need = true;
+                }
+            }
+            const unsigned long long nm = __ballot(need);
+            const unsigned long long rm = __ballot(s < pp.substeps && !need);
+            if (nm == 0) {
+                if (rm == 0) break;
+                continue;
+            }
+            if (rm != 0 && __popcll(nm) < SEARCH_BATCH) continue;
+            if (need) {
+                ++searches;
+                double r[DIM];
+                const int cn = relocate<DIM>(m, c, P, xi, vext, x, r);
+                if (cn < 0) {
+                    c = -1;
+                    ++lost;
+                    s = pp.substeps;
+                } else {
+                    c = cn;
+#pragma unroll
+                    for (int d = 0; d < DIM; ++d) xi[d] = r[d];
+                    load_cell<DIM>(m, V, c, P, Vc);
+                    tol = m.ftol[c];
+                    ++s;
+                }
+                need = false;
+            }
+        }
+        if (!valid) continue;
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) {
+            A.x[(size_t)d * A.cap + i] = x[d];
+            A.xi[(size_t)d * A.cap + i] = xi[d];
+        }
+        w.dest[i] = c;
+        if (c >= 0) {
+            w.pkey[i] = part_key<DIM>(pp, xi);
+            if (c != c0) {
+                const int slot = atomicAdd(&w.counters[1], 1);
+                w.movers[slot] = i;
+                atomicAdd(&w.arr_cnt[c], 1);
+            }
+        }
+    }
+    if (lost) atomicAdd(&w.stats[0], (unsigned long long)lost);
+    if (searches) atomicAdd(&w.stats[3], (unsigned long long)searches);
+}
+
+// movers grouped by destination cell
+__global__ void k_group(DevWork w) {
+    const int n = w.counters[1];
+    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+        const int i = w.movers[j];
+        const int c = w.dest[i];
+        const int slot = atomicAdd(&w.arr_fill[c], 1);
+        w.arr_list[w.arr_off[c] + slot] = i;
+    }
+}
+
+// candidate j of cell c: stayers of the old segment first, then arrivals
+__device__ __forceinline__ int candidate(const DevWork& w, const int* old_off, int c, int n_st, int j) {
+    return j < n_st ? old_off[c] + j : w.arr_list[w.arr_off[c] + (j - n_st)];
+}
+
+// ---------------------------------------------------------------------------
+// k_cell_count: per-cell part census (check_cell_for_empty_parts,
+// cylinder2D/pfem2particle.cpp:510-561).  Part index per direction is
+// (unsigned)(xi_d / (1/q_d)) without clamping (xi = 1 gives the extra index
+// q_d).  In every part the MAX particles with the smallest ids survive;
+// every in-range part that holds no particle is reseeded.  With reseed = 0
+// (relocation only) every particle that found a cell survives.
+// Each wavefront handles a sequence of cells; a cell's candidates are staged
+// in LDS when there are at most CCAP of them, else read from global memory.
+constexpr int CCAP = 256;
+constexpr int CELL_WAVES = 4;
+
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ int load_agent(const int* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <int DIM>
+__global__ void __launch_bounds__(256) k_cell_count(int n_cells, const int* __restrict__ old_off,
+                                                    const uint32_t* __restrict__ pid, DevParams pp, DevWork w,
+                                                    int reseed) {
+    __shared__ int s_idx[CELL_WAVES][CCAP];
+    __shared__ int s_key[CELL_WAVES][CCAP];
+    __shared__ uint32_t s_id[CELL_WAVES][CCAP];
+    __shared__ unsigned char s_surv[CELL_WAVES][CCAP];
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int wv = threadIdx.x / WAVE;
+    const unsigned long long all = pp.nparts_in >= 64 ? ~0ull : ((1ull << pp.nparts_in) - 1);
+    for (int c = blockIdx.x * CELL_WAVES + wv; c < n_cells; c += gridDim.x * CELL_WAVES) {
+        const int n_st = old_off[c + 1] - old_off[c];
+        const int k = n_st + w.arr_cnt[c];
+        unsigned long long hit = 0;
+        int nvalid = 0, nsurv = 0;
+        if (k <= CCAP) {
+            for (int j = lane; j < k; j += WAVE) {
+                const int idx = candidate(w, old_off, c, n_st, j);
+                const bool valid = w.dest[idx] == c;
+                s_idx[wv][j] = valid ? idx : -1;
+                s_key[wv][j] = valid ? w.pkey[idx] : -1;
+                s_id[wv][j] = pid[idx];
+            }
+            wave_sync();
+            for (int j = lane; j < k; j += WAVE) {
+                int surv = 0;
+                if (s_idx[wv][j] >= 0) {
+                    ++nvalid;
+                    const int key = s_key[wv][j];
+                    const uint32_t id = s_id[wv][j];
+                    const int o = part_ordinal<DIM>(pp, key);
+                    if (o >= 0) hit |= 1ull << o;
+                    int rank = 0;
+                    if (reseed)
+                        for (int t = 0; t < k; ++t) rank += (s_key[wv][t] == key && s_id[wv][t] < id) ? 1 : 0;
+                    surv = rank < pp.max_per_part || !reseed;
+                    nsurv += surv;
+                }
+                s_surv[wv][j] = (unsigned char)surv;
+            }
+            wave_sync();
+            for (int j = lane; j < k; j += WAVE) {
+                const int idx = s_idx[wv][j];
+                if (idx < 0) continue;
+                if (!s_surv[wv][j]) {
+                    w.spos[idx] = -1;
+                    continue;
+                }
+                const uint32_t id = s_id[wv][j];
+                int pos = 0;
+                for (int t = 0; t < k; ++t) pos += (s_surv[wv][t] && s_id[wv][t] < id) ? 1 : 0;
+                w.spos[idx] = pos;
+            }
+            wave_sync();
+        } else {
+            // large cell: same census straight from global memory
+            for (int j = lane; j < k; j += WAVE) {
+                const int idx = candidate(w, old_off, c, n_st, j);
+                if (w.dest[idx] != c) continue;
+                ++nvalid;
+                const int key = w.pkey[idx];
+                const uint32_t id = pid[idx];
+                const int o = part_ordinal<DIM>(pp, key);
+                if (o >= 0) hit |= 1ull << o;
+                int rank = 0;
+                if (reseed)
+                    for (int t = 0; t < k; ++t) {
+                        const int it = candidate(w, old_off, c, n_st, t);
+                        rank += (w.dest[it] == c && w.pkey[it] == key && pid[it] < id) ? 1 : 0;
+                    }
+                const int surv = rank < pp.max_per_part || !reseed;
+                nsurv += surv;
+                w.spos[idx] = surv ? 0 : -1;
+            }
+            __threadfence();
+            wave_sync();
+            for (int j = lane; j < k; j += WAVE) {
+                const int idx = candidate(w, old_off, c, n_st, j);
+                if (w.dest[idx] != c || load_agent(&w.spos[idx]) < 0) continue;
+                const uint32_t id = pid[idx];
+                int pos = 0;
+                for (int t = 0; t < k; ++t) {
+                    const int it = candidate(w, old_off, c, n_st, t);
+                    pos += (w.dest[it] == c && load_agent(&w.spos[it]) >= 0 && pid[it] < id) ? 1 : 0;
+                }
+                w.spos[idx] = pos;
+            }
+        }
+#pragma unroll
+        for (int off = 1; off < WAVE; off <<= 1) {
+            const unsigned lo = __shfl_xor((unsigned)(hit & 0xffffffffu), off);
+            const unsigned hi = __shfl_xor((unsigned)(hit >> 32), off);
+            hit |= ((unsigned long long)hi << 32) | lo;
+            nvalid += __shfl_xor(nvalid, off);
+            nsurv += __shfl_xor(nsurv, off);
+        }
+        if (lane == 0) {
+            const int spawns = reseed ? __popcll(all & ~hit) : 0;
+            w.hitmask[c] = reseed ? hit : all;
+            w.nsurv[c] = nsurv;
+            w.final_cnt[c] = nsurv + spawns;
+            w.spawn_cnt[c] = spawns;
+            if (nvalid > nsurv) atomicAdd(&w.stats[1], (unsigned long long)(nvalid - nsurv));
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_cell_write: survivors and reseeded particles into buffer B.
+// New particles: reference point at the part centre, location mapped,
+// velocity interpolated from V (cylinder2D/pfem2particle.cpp:533-549), ids
+// continuing the counter in ascending cell, then part, order.
+template <int DIM>
+__device__ __forceinline__ void cell_write_one(const DevMesh<DIM>& m, const double* __restrict__ V,
+                                               const int* __restrict__ old_off, const DevParticles& A,
+                                               const DevParticles& B, const DevParams& pp, const DevWork& w, int c,
+                                               int lane) {
+    constexpr int NV = Q1<DIM>::NV;
+    const int n_st = old_off[c + 1] - old_off[c];
+    const int k = n_st + w.arr_cnt[c];
+    const long long out0 = w.new_off[c];
+    const int nsurv = w.nsurv[c];
+    const int nfinal = w.final_cnt[c];
+    if (out0 + nfinal > B.cap) {
+        if (lane == 0) atomicOr(&w.counters[2], 1);
+        return;
+    }
+    for (int j = lane; j < k; j += WAVE) {
+        const int idx = candidate(w, old_off, c, n_st, j);
+        if (w.dest[idx] != c) continue;
+        const int pos = w.spos[idx];
+        if (pos < 0) continue;
+        const long long o = out0 + pos;
+        B.cell[o] = c;
+        B.id[o] = A.id[idx];
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) {
+            B.x[(size_t)d * B.cap + o] = A.x[(size_t)d * A.cap + idx];
+            B.xi[(size_t)d * B.cap + o] = A.xi[(size_t)d * A.cap + idx];
+            B.v[(size_t)d * B.cap + o] = A.v[(size_t)d * A.cap + idx];
+        }
+        if (pp.keep_vext)
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) B.vext[(size_t)d * B.cap + o] = A.vext[(size_t)d * A.cap + idx];
+    }
+    const int spawns = nfinal - nsurv;
+    if (spawns > 0) {
+        const unsigned long long all = pp.nparts_in >= 64 ? ~0ull : ((1ull << pp.nparts_in) - 1);
+        const unsigned long long empty = all & ~w.hitmask[c];
+        const uint32_t id0 = *w.particle_counter + (uint32_t)w.spawn_off[c];
+        for (int o = lane; o < pp.nparts_in; o += WAVE) {
+            if (!((empty >> o) & 1)) continue;
+            const int r = __popcll(empty & ((1ull << o) - 1));
+            double xi[DIM], P[NV][DIM], Vc[NV][DIM], x[DIM], v[DIM];
+            lattice_point<DIM>(pp, o, xi);
+            load_cell<DIM>(m, V, c, P, Vc);
+            forward<DIM>(P, xi, x);
+            interp<DIM>(Vc, xi, v);
+            const long long q = out0 + nsurv + r;
+            B.cell[q] = c;
+            B.id[q] = id0 + (uint32_t)r + 1u;
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) {
+                B.x[(size_t)d * B.cap + q] = x[d];
+                B.xi[(size_t)d * B.cap + q] = xi[d];
+                B.v[(size_t)d * B.cap + q] = v[d];
+            }
+            if (pp.keep_vext)
+#pragma unroll
+                for (int d = 0; d < DIM; ++d) B.vext[(size_t)d * B.cap + q] = 0.0;
+        }
+    }
+    if (lane == 0) {
+        w.arr_cnt[c] = 0;
+        w.arr_fill[c] = 0;
+    }
+}
+
+template <int DIM>
+__global__ void __launch_bounds__(256) k_cell_write(DevMesh<DIM> m, const double* __restrict__ V,
+                                                    const int* __restrict__ old_off, DevParticles A,
+                                                    DevParticles B, DevParams pp, DevWork w) {
+    const int lane = threadIdx.x & (WAVE - 1);
+    for (int c = blockIdx.x * CELL_WAVES + threadIdx.x / WAVE; c < m.n_cells; c += gridDim.x * CELL_WAVES)
+        cell_write_one<DIM>(m, V, old_off, A, B, pp, w, c, lane);
+}
+
+// totals after the scans: particle count, id counter, statistics
+__global__ void k_finalize(int n_cells, DevWork w) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const int total = w.new_off[n_cells];
+    const int spawned = w.spawn_off[n_cells];
+    w.stats[2] += (unsigned long long)spawned;
+    *w.particle_counter += (uint32_t)spawned;
+    w.counters[0] = total;
+    w.counters[1] = 0;
+}
+
+// ---------------------------------------------------------------------------
+// k_cell_partials: per-cell local sums of the projection
+// (TurekBenchmark/pfem2particle.cpp:880-893): for each cell vertex k,
+// sum_p N_k(xi_p) v_p and sum_p N_k(xi_p), particles in ascending id order.
+// Lanes stage a chunk of particles in LDS; lane a = k*(DIM+1) + comp then
+// accumulates its sum sequentially.
+template <int DIM>
+__global__ void __launch_bounds__(256) k_cell_partials(int n_cells, const int* __restrict__ off, DevParticles A,
+                                                       double* __restrict__ partial) {
+    constexpr int NV = Q1<DIM>::NV;
+    constexpr int NACC = NV * (DIM + 1);
+    __shared__ double s_xi[4][WAVE][DIM];
+    __shared__ double s_v[4][WAVE][DIM];
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int wv = threadIdx.x / WAVE;
+    const int kk = lane / (DIM + 1);
+    const int comp = lane - kk * (DIM + 1);
+    for (int c = blockIdx.x * 4 + wv; c < n_cells; c += gridDim.x * 4) {
+    const int b = off[c], e = off[c + 1];
+    double acc = 0.0;
+    for (int j0 = b; j0 < e; j0 += WAVE) {
+        const int j = j0 + lane;
+        if (j < e) {
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) {
+                s_xi[wv][lane][d] = A.xi[(size_t)d * A.cap + j];
+                s_v[wv][lane][d] = A.v[(size_t)d * A.cap + j];
+            }
+        }
+        wave_sync();
+        const int cnt = (e - j0) < WAVE ? (e - j0) : WAVE;
+        if (lane < NACC) {
+            for (int t = 0; t < cnt; ++t) {
+                double xi[DIM], N[NV];
+#pragma unroll
+                for (int d = 0; d < DIM; ++d) xi[d] = s_xi[wv][t][d];
+                shape<DIM>(xi, N);
+                double nk = N[0];
+#pragma unroll
+                for (int q = 1; q < NV; ++q)
+                    if (q == kk) nk = N[q];
+                if (comp < DIM) {
+                    double vc = s_v[wv][t][0];
+#pragma unroll
+                    for (int d = 1; d < DIM; ++d)
+                        if (d == comp) vc = s_v[wv][t][d];
+                    acc += nk * vc;
+                } else {
+                    acc += nk;
+                }
+            }
+        }
+        wave_sync();
+    }
+    if (lane < NACC) partial[(size_t)c * NACC + lane] = acc;
+    }
+}
+
+// k_node: node sums over the cells that contain the DoF (ascending cell
+// order, TurekBenchmark/pfem2particle.cpp:895-909) divided by the weight.
+// Writes V and V_old (both hold the projected field afterwards).
+template <int DIM>
+__global__ void k_node(DevMesh<DIM> m, const double* __restrict__ partial, double* __restrict__ V,
+                       double* __restrict__ Vold, unsigned long long* stats) {
+    constexpr int NV = Q1<DIM>::NV;
+    constexpr int NACC = NV * (DIM + 1);
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < m.n_dofs; i += gridDim.x * blockDim.x) {
+        double acc[DIM], wsum = 0.0;
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) acc[d] = 0.0;
+        for (int e = m.dof_off[i]; e < m.dof_off[i + 1]; ++e) {
+            const int ck = m.dof_cell[e];
+            const int c = ck / NV, k = ck - c * NV;
+            const double* p = partial + (size_t)c * NACC + k * (DIM + 1);
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) acc[d] += p[d];
+            wsum += p[DIM];
+        }
+        if (wsum == 0.0) atomicAdd(&stats[4], 1ull);
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) {
+            const double val = acc[d] / wsum;
+            V[(size_t)i * DIM + d] = val;
+            Vold[(size_t)i * DIM + d] = val;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// wall fix-up (cylinderDyM/pfem2particle.cpp:960-992)
+struct DevWalls {
+    int n;
+    const int* dof;         // per wall node
+    const double* bc;       // n * 3
+    const int* mask;        // per wall node
+    int* fix;               // per wall node: deviation above 1e-5 in an enforced component
+    double* val;            // n * 3 projected values before the fix
+    const int* vwall;       // per vertex: wall index or -1
+    int n_cells;            // cells adjacent to a wall node
+    const int* cells;
+};
+
+template <int DIM>
+__global__ void k_wall_flags(DevWalls wl, const double* __restrict__ V) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < wl.n; i += gridDim.x * blockDim.x) {
+        const int dof = wl.dof[i];
+        int fix = 0;
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) {
+            const double val = V[(size_t)dof * DIM + d];
+            wl.val[i * 3 + d] = val;
+            if (((wl.mask[i] >> d) & 1) && fabs(val - wl.bc[i * 3 + d]) > 1e-5) fix = 1;
+        }
+        wl.fix[i] = fix;
+    }
+}
+
+// one wavefront per wall-adjacent cell; per particle the corrections of the
+// cell's wall vertices are applied in ascending vertex order, as the
+// reference loops over wall nodes in ascending order
+template <int DIM>
+__global__ void k_wall_particles(DevMesh<DIM> m, DevWalls wl, const int* __restrict__ off, DevParticles A) {
+    constexpr int NV = Q1<DIM>::NV;
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int t = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
+    if (t >= wl.n_cells) return;
+    const int c = wl.cells[t];
+    int order[NV], widx[NV], nw = 0;
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        const int gv = m.cv[(size_t)c * NV + k];
+        const int wi = wl.vwall[gv];
+        if (wi >= 0 && wl.fix[wi]) {
+            // insertion by ascending global vertex index
+            int p = nw;
+            while (p > 0 && m.cv[(size_t)c * NV + order[p - 1]] > gv) {
+                order[p] = order[p - 1];
+                widx[p] = widx[p - 1];
+                --p;
+            }
+            order[p] = k;
+            widx[p] = wi;
+            ++nw;
+        }
+    }
+    if (nw == 0) return;
+    for (int j = off[c] + lane; j < off[c + 1]; j += WAVE) {
+        double xi[DIM], v[DIM], N[NV];
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) {
+            xi[d] = A.xi[(size_t)d * A.cap + j];
+            v[d] = A.v[(size_t)d * A.cap + j];
+        }
+        shape<DIM>(xi, N);
+        for (int q = 0; q < nw; ++q) {
+            const int wi = widx[q];
+            double nk = N[0];
+#pragma unroll
+            for (int r = 1; r < NV; ++r)
+                if (r == order[q]) nk = N[r];
+#pragma unroll
+            for (int d = 0; d < DIM; ++d)
+                if ((wl.mask[wi] >> d) & 1) v[d] = v[d] + nk * (wl.bc[wi * 3 + d] - wl.val[wi * 3 + d]);
+        }
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) A.v[(size_t)d * A.cap + j] = v[d];
+    }
+}
+
+template <int DIM>
+__global__ void k_wall_set(DevWalls wl, double* __restrict__ V, double* __restrict__ Vold) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < wl.n; i += gridDim.x * blockDim.x) {
+        const int dof = wl.dof[i];
+#pragma unroll
+        for (int d = 0; d < DIM; ++d)
+            if ((wl.mask[i] >> d) & 1) {
+                V[(size_t)dof * DIM + d] = wl.bc[i * 3 + d];
+                Vold[(size_t)dof * DIM + d] = wl.bc[i * 3 + d];
+            }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// seeding: q0*q1[*q2] particles per cell in lattice order, ids in cell order
+// (cylinder2D/pfem2particle.cpp:487-508, 563-580)
+template <int DIM>
+__global__ void k_seed(DevMesh<DIM> m, const double* __restrict__ V, DevParticles A, DevParams pp,
+                       uint32_t id_base, int* off) {
+    constexpr int NV = Q1<DIM>::NV;
+    const long long total = (long long)m.n_cells * pp.nparts_in;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < total;
+         i += (long long)gridDim.x * blockDim.x) {
+        const int c = (int)(i / pp.nparts_in);
+        const int o = (int)(i - (long long)c * pp.nparts_in);
+        double xi[DIM], P[NV][DIM], Vc[NV][DIM], x[DIM], v[DIM];
+        lattice_point<DIM>(pp, o, xi);
+        load_cell<DIM>(m, V, c, P, Vc);
+        forward<DIM>(P, xi, x);
+        interp<DIM>(Vc, xi, v);
+        A.cell[i] = c;
+        A.id[i] = id_base + (uint32_t)i + 1u;
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) {
+            A.x[(size_t)d * A.cap + i] = x[d];
+            A.xi[(size_t)d * A.cap + i] = xi[d];
+            A.v[(size_t)d * A.cap + i] = v[d];
+            if (pp.keep_vext) A.vext[(size_t)d * A.cap + i] = 0.0;
+        }
+        if (o == 0) off[c] = (int)i;
+        if (i == total - 1) off[m.n_cells] = (int)total;
+    }
+}
+
+// location from (cell, reference point) and, optionally, velocity from V, for
+// particles placed by pf2_insert_particles
+template <int DIM>
+__global__ void k_init_particles(DevMesh<DIM> m, const double* __restrict__ V, DevParticles A,
+                                 const unsigned char* __restrict__ need_v, long long n, int keep_vext) {
+    constexpr int NV = Q1<DIM>::NV;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        const int c = A.cell[i];
+        double xi[DIM], P[NV][DIM], Vc[NV][DIM], x[DIM], v[DIM];
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) xi[d] = A.xi[(size_t)d * A.cap + i];
+        load_cell<DIM>(m, V, c, P, Vc);
+        forward<DIM>(P, xi, x);
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) A.x[(size_t)d * A.cap + i] = x[d];
+        if (need_v[i]) {
+            interp<DIM>(Vc, xi, v);
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) A.v[(size_t)d * A.cap + i] = v[d];
+        }
+        if (keep_vext)
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) A.vext[(size_t)d * A.cap + i] = 0.0;
+    }
+}
+
+// re-sort after a mesh move: own-cell test and relocation with the stored
+// last advection velocity, no advection
+template <int DIM>
+__global__ void __launch_bounds__(256) k_resort(DevMesh<DIM> m, DevParticles A, DevParams pp, DevWork w) {
+    constexpr int NV = Q1<DIM>::NV;
+    const int n = w.counters[0];
+    int lost = 0, searches = 0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int c0 = A.cell[i];
+        int c = c0;
+        double x[DIM], xi[DIM], vext[DIM], r[DIM], P[NV][DIM];
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) {
+            xi[d] = A.xi[(size_t)d * A.cap + i];
+            x[d] = A.x[(size_t)d * A.cap + i];
+            vext[d] = A.vext ? A.vext[(size_t)d * A.cap + i] : 0.0;
+        }
+        load_vertices<DIM>(m, c, P);
+        if (inverse_map<DIM>(P, x, r, m.ftol[c], false) && inside_unit<DIM>(r)) {
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) xi[d] = r[d];
+        } else {
+            ++searches;
+            c = relocate<DIM>(m, c, P, xi, vext, x, r);
+            if (c < 0) ++lost;
+            else
+#pragma unroll
+                for (int d = 0; d < DIM; ++d) xi[d] = r[d];
+        }
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) A.xi[(size_t)d * A.cap + i] = xi[d];
+        w.dest[i] = c;
+        if (c >= 0) {
+            w.pkey[i] = part_key<DIM>(pp, xi);
+            if (c != c0) {
+                const int slot = atomicAdd(&w.counters[1], 1);
+                w.movers[slot] = i;
+                atomicAdd(&w.arr_cnt[c], 1);
+            }
+        }
+    }
+    if (lost) atomicAdd(&w.stats[0], (unsigned long long)lost);
+    if (searches) atomicAdd(&w.stats[3], (unsigned long long)searches);
+}
+
+// dest / key of every particle for a reseed pass without movement
+template <int DIM>
+__global__ void k_keys(DevParticles A, DevParams pp, DevWork w) {
+    const int n = w.counters[0];
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        double xi[DIM];
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) xi[d] = A.xi[(size_t)d * A.cap + i];
+        w.dest[i] = A.cell[i];
+        w.pkey[i] = part_key<DIM>(pp, xi);
+    }
+}
+
+__global__ void k_fill_int(int* p, int val, long long n) {
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        p[i] = val;
+}
+
+// nodal field layout conversion: component-major <-> node-major
+template <int DIM>
+__global__ void k_to_node_major(const double* __restrict__ src, double* __restrict__ dst, int n) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) dst[(size_t)i * DIM + d] = src[(size_t)d * n + i];
+}
+
+template <int DIM>
+__global__ void k_to_comp_major(const double* __restrict__ src, double* __restrict__ dst, int n) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) dst[(size_t)d * n + i] = src[(size_t)i * DIM + d];
+}
+
+}  // namespace pf2

@@ -1,0 +1,1046 @@
+// pf2_lib.hip — handle management and the C ABI declared in include/pfem2_amd.h.
+//
+// Host side of the particle stage: it uploads the mesh once, builds the
+// vertex star (pfem2ParticleHandler::initialize_maps,
+// TurekBenchmark/pfem2particle.cpp:215-219) and the DoF -> cell table of the
+// projection, and orders the kernel launches of one step on the handle's
+// stream.  No host synchronisation happens inside a step; counts stay on
+// the device and every particle kernel loops over the device-side count.
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/pfem2_amd.h"
+#include "pf2_kernels.hpp"
+#include "pf2_mixed.hpp"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIPC(expr)                                                                            \
+    do {                                                                                      \
+        hipError_t e_ = (expr);                                                               \
+        if (e_ != hipSuccess)                                                                 \
+            return fail(PF2_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));        \
+    } while (0)
+
+enum KernelId {
+    K_MOVE = 0, K_GROUP, K_CELL_COUNT, K_CELL_WRITE, K_SCAN, K_FINALIZE, K_PARTIALS, K_NODE, K_WALL,
+    K_SEED, K_MISC, K_MOVE32, K_CELL_WRITE32, K_PARTIALS32, K_NKERNELS
+};
+const char* kKernelNames[K_NKERNELS] = {"k_move", "k_group", "k_cell_count", "k_cell_write", "scan",
+                                        "k_finalize", "k_cell_partials", "k_node", "wall_fixup",
+                                        "k_seed", "misc", "k_move32", "k_cell_write32", "k_cell_partials32"};
+
+template <typename T>
+int dalloc(T** p, size_t n) {
+    *p = nullptr;
+    if (n == 0) n = 1;
+    hipError_t e = hipMalloc((void**)p, n * sizeof(T));
+    if (e != hipSuccess) return fail(PF2_E_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+    return 0;
+}
+
+struct PBufs {
+    int* cell = nullptr;
+    uint32_t* id = nullptr;
+    double* x = nullptr;
+    double* xi = nullptr;
+    double* v = nullptr;
+    double* vext = nullptr;
+    // FP32-mixed storage
+    float* xi32 = nullptr;
+    float* v32 = nullptr;
+};
+
+}  // namespace
+
+struct pf2_handle_s {
+    int dim = 2, nv = 4;
+    int n_cells = 0, n_vertices = 0, n_dofs = 0;
+    pf2_params params{};
+    long long cap = 0;
+    long long n_est = 0;   // host-side estimate of the particle count (grid sizes only)
+    hipStream_t stream = nullptr;
+    // host copies
+    std::vector<double> X;
+    std::vector<int> cv, cd, star_off, star_cell, dof_off, dof_cell;
+    std::vector<double> star_dir;
+    // device mesh
+    double* dX = nullptr;
+    int *dcv = nullptr, *dcd = nullptr, *dstar_off = nullptr, *dstar_cell = nullptr;
+    double* dstar_dir = nullptr;
+    int *ddof_off = nullptr, *ddof_cell = nullptr;
+    double* dftol = nullptr;
+    // fields (node-major)
+    double *dV = nullptr, *dVold = nullptr, *dtmp = nullptr;
+    // mixed-precision fields and geometry
+    float *dV32 = nullptr, *dVold32 = nullptr;
+    float* dgeo32 = nullptr;
+    // particles
+    PBufs buf[2];
+    int cur = 0;
+    bool fp64_valid = true;   // FP64 copy of the state is current (always true in the FP64 path)
+    int* off[2] = {nullptr, nullptr};
+    int cur_off = 0;
+    // work
+    int *dest = nullptr, *pkey = nullptr, *spos = nullptr, *movers = nullptr;
+    int *arr_cnt = nullptr, *arr_off = nullptr, *arr_fill = nullptr, *arr_list = nullptr;
+    int *final_cnt = nullptr, *spawn_cnt = nullptr, *spawn_off = nullptr;
+    unsigned long long* hitmask = nullptr;
+    int* nsurv = nullptr;
+    double* partial = nullptr;
+    int* counters = nullptr;
+    uint32_t* pcounter = nullptr;
+    unsigned long long* stats = nullptr;
+    void* scan_tmp = nullptr;
+    size_t scan_tmp_bytes = 0;
+    // walls
+    int n_walls = 0, n_wall_cells = 0;
+    int *w_dof = nullptr, *w_mask = nullptr, *w_fix = nullptr, *w_vwall = nullptr, *w_cells = nullptr;
+    double *w_bc = nullptr, *w_val = nullptr;
+    // profiling
+    bool profiling = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_free;
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_pending;
+    double k_ms[K_NKERNELS] = {};
+    long long k_n[K_NKERNELS] = {};
+};
+
+namespace {
+
+using namespace pf2;
+
+int mixed(const pf2_handle_s* h) { return h->params.precision == PF2_FP32_MIXED; }
+
+int flush_events(pf2_handle_s* h) {
+    if (h->ev_pending.empty()) return 0;
+    HIPC(hipStreamSynchronize(h->stream));
+    for (auto& p : h->ev_pending) {
+        float ms = 0.f;
+        HIPC(hipEventElapsedTime(&ms, p.second.first, p.second.second));
+        h->k_ms[p.first] += ms;
+        h->k_n[p.first] += 1;
+        h->ev_free.push_back(p.second);
+    }
+    h->ev_pending.clear();
+    return 0;
+}
+
+// brackets one launch with events when profiling is on
+struct Timed {
+    pf2_handle_s* h;
+    int kid;
+    std::pair<hipEvent_t, hipEvent_t> ev{};
+    bool on = false;
+    Timed(pf2_handle_s* h_, int kid_) : h(h_), kid(kid_) {
+        if (!h->profiling) return;
+        if (h->ev_free.empty()) {
+            if (h->ev_pending.size() >= 4096) flush_events(h);
+        }
+        if (h->ev_free.empty()) {
+            hipEvent_t a, b;
+            if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return;
+            h->ev_free.push_back({a, b});
+        }
+        ev = h->ev_free.back();
+        h->ev_free.pop_back();
+        hipEventRecord(ev.first, h->stream);
+        on = true;
+    }
+    ~Timed() {
+        if (!on) return;
+        hipEventRecord(ev.second, h->stream);
+        h->ev_pending.push_back({kid, ev});
+    }
+};
+
+// cell kernels: four wavefronts per block, each looping over cells
+int cell_grid(int n_cells) {
+    int g = (n_cells + 3) / 4;
+    return g < 2048 ? (g < 1 ? 1 : g) : 2048;
+}
+
+int grid_for(long long n, int block, int cap_blocks = 65535 * 4) {
+    long long g = (n + block - 1) / block;
+    if (g < 1) g = 1;
+    if (g > cap_blocks) g = cap_blocks;
+    return (int)g;
+}
+
+template <int DIM>
+DevMesh<DIM> dev_mesh(pf2_handle_s* h) {
+    DevMesh<DIM> m;
+    m.n_cells = h->n_cells;
+    m.n_vertices = h->n_vertices;
+    m.n_dofs = h->n_dofs;
+    m.X = h->dX;
+    m.cv = h->dcv;
+    m.cd = h->dcd;
+    m.star_off = h->dstar_off;
+    m.star_cell = h->dstar_cell;
+    m.star_dir = h->dstar_dir;
+    m.dof_off = h->ddof_off;
+    m.dof_cell = h->ddof_cell;
+    m.ftol = h->dftol;
+    return m;
+}
+
+DevParticles dev_particles(pf2_handle_s* h, int which) {
+    DevParticles p;
+    const PBufs& b = h->buf[which];
+    p.cell = b.cell;
+    p.id = b.id;
+    p.x = b.x;
+    p.xi = b.xi;
+    p.v = b.v;
+    p.vext = b.vext;
+    p.cap = h->cap;
+    return p;
+}
+
+DevParams dev_params(pf2_handle_s* h) {
+    DevParams p;
+    p.h = h->params.dt / h->params.substeps;
+    p.substeps = h->params.substeps;
+    p.max_per_part = h->params.max_per_part;
+    p.nparts_in = 1;
+    for (int d = 0; d < 3; ++d) p.q[d] = d < h->dim ? h->params.quantities[d] : 1;
+    for (int d = 0; d < h->dim; ++d) p.nparts_in *= p.q[d];
+    p.keep_vext = h->params.keep_velocity_ext;
+    return p;
+}
+
+DevWork dev_work(pf2_handle_s* h) {
+    DevWork w;
+    w.dest = h->dest;
+    w.pkey = h->pkey;
+    w.spos = h->spos;
+    w.movers = h->movers;
+    w.arr_cnt = h->arr_cnt;
+    w.arr_off = h->arr_off;
+    w.arr_fill = h->arr_fill;
+    w.arr_list = h->arr_list;
+    w.final_cnt = h->final_cnt;
+    w.spawn_cnt = h->spawn_cnt;
+    w.new_off = h->off[1 - h->cur_off];
+    w.spawn_off = h->spawn_off;
+    w.hitmask = h->hitmask;
+    w.nsurv = h->nsurv;
+    w.partial = h->partial;
+    w.counters = h->counters;
+    w.particle_counter = h->pcounter;
+    w.stats = h->stats;
+    return w;
+}
+
+// vertex_to_cell_centers_directions restated: normalise(vertex average - vertex)
+void compute_star_dirs(pf2_handle_s* h) {
+    const int dim = h->dim;
+    h->star_dir.assign((size_t)h->star_off[h->n_vertices] * dim, 0.0);
+    for (int v = 0; v < h->n_vertices; ++v)
+        for (int e = h->star_off[v]; e < h->star_off[v + 1]; ++e) {
+            const int c = h->star_cell[e];
+            double ctr[3] = {0, 0, 0};
+            for (int k = 0; k < h->nv; ++k) {
+                const int gv = h->cv[(size_t)c * h->nv + k];
+                for (int d = 0; d < dim; ++d) ctr[d] += h->X[(size_t)gv * dim + d];
+            }
+            const double inv_n = 1.0 / h->nv;
+            double t[3], s = 0.0;
+            for (int d = 0; d < dim; ++d) t[d] = ctr[d] * inv_n - h->X[(size_t)v * dim + d];
+            for (int d = 0; d < dim; ++d) s += t[d] * t[d];
+            const double r = 1.0 / std::sqrt(s);
+            for (int d = 0; d < dim; ++d) h->star_dir[(size_t)e * dim + d] = t[d] * r;
+        }
+}
+
+int upload_star_dirs(pf2_handle_s* h) {
+    HIPC(hipMemcpyAsync(h->dstar_dir, h->star_dir.data(), h->star_dir.size() * sizeof(double),
+                        hipMemcpyHostToDevice, h->stream));
+    return 0;
+}
+
+int check_params(int dim, const pf2_params* p) {
+    if (!p) return fail(PF2_E_ARG, "params is NULL");
+    if (!(p->dt > 0.0)) return fail(PF2_E_ARG, "dt must be positive");
+    if (p->substeps < 1) return fail(PF2_E_ARG, "substeps must be >= 1");
+    if (p->max_per_part < 1) return fail(PF2_E_ARG, "max_per_part must be >= 1");
+    long long np = 1;
+    for (int d = 0; d < dim; ++d) {
+        if (p->quantities[d] < 1) return fail(PF2_E_ARG, "quantities must be >= 1");
+        np *= p->quantities[d];
+    }
+    if (np > 64) return fail(PF2_E_ARG, "at most 64 lattice parts per cell are supported");
+    if (p->precision != PF2_FP64 && p->precision != PF2_FP32_MIXED) return fail(PF2_E_ARG, "unknown precision");
+    return 0;
+}
+
+long long default_capacity(int dim, int n_cells, const pf2_params* p) {
+    long long per = p->max_per_part;
+    for (int d = 0; d < dim; ++d) per *= (p->quantities[d] + 1);
+    long long lattice = 1;
+    for (int d = 0; d < dim; ++d) lattice *= p->quantities[d];
+    per = std::max(per, lattice);
+    return per * (long long)n_cells;
+}
+
+int alloc_particles(pf2_handle_s* h) {
+    for (int b = 0; b < 2; ++b) {
+        PBufs& pb = h->buf[b];
+        if (dalloc(&pb.cell, h->cap) || dalloc(&pb.id, h->cap) || dalloc(&pb.x, h->cap * h->dim) ||
+            dalloc(&pb.xi, h->cap * h->dim) || dalloc(&pb.v, h->cap * h->dim))
+            return PF2_E_HIP;
+        if (h->params.keep_velocity_ext && dalloc(&pb.vext, h->cap * h->dim)) return PF2_E_HIP;
+        if (mixed(h) && (dalloc(&pb.xi32, h->cap * h->dim) || dalloc(&pb.v32, h->cap * h->dim))) return PF2_E_HIP;
+    }
+    if (dalloc(&h->dest, h->cap) || dalloc(&h->pkey, h->cap) || dalloc(&h->spos, h->cap) ||
+        dalloc(&h->movers, h->cap) || dalloc(&h->arr_list, h->cap))
+        return PF2_E_HIP;
+    return 0;
+}
+
+void free_particles(pf2_handle_s* h) {
+    for (int b = 0; b < 2; ++b) {
+        PBufs& pb = h->buf[b];
+        hipFree(pb.cell); hipFree(pb.id); hipFree(pb.x); hipFree(pb.xi); hipFree(pb.v); hipFree(pb.vext);
+        hipFree(pb.xi32); hipFree(pb.v32);
+        pb = PBufs();
+    }
+    hipFree(h->dest); hipFree(h->pkey); hipFree(h->spos); hipFree(h->movers); hipFree(h->arr_list);
+    h->dest = h->pkey = h->spos = h->movers = h->arr_list = nullptr;
+}
+
+int scan(pf2_handle_s* h, const int* in, int* out, int n) {
+    Timed t(h, K_SCAN);
+    size_t bytes = h->scan_tmp_bytes;
+    HIPC(hipcub::DeviceScan::ExclusiveSum(h->scan_tmp, bytes, in, out, n, h->stream));
+    return 0;
+}
+
+int get_count(pf2_handle_s* h, long long* n) {
+    int cnt[3];
+    HIPC(hipMemcpyAsync(cnt, h->counters, sizeof(cnt), hipMemcpyDeviceToHost, h->stream));
+    HIPC(hipStreamSynchronize(h->stream));
+    if (cnt[2] & 1) return fail(PF2_E_CAPACITY, "particle capacity exceeded");
+    if (cnt[2] & 2) return fail(PF2_E_CAPACITY, "more than 1024 particles in one cell");
+    *n = cnt[0];
+    h->n_est = cnt[0];
+    return 0;
+}
+
+// Regroup after k_move / k_resort / k_keys filled dest and pkey:
+// arrivals, census, survivors + reseed into the other buffer, swap.
+template <int DIM>
+int regroup(pf2_handle_s* h, int reseed) {
+    const int nc = h->n_cells;
+    DevParams pp = dev_params(h);
+    DevWork w = dev_work(h);
+    const int* old_off = h->off[h->cur_off];
+    if (int r = scan(h, h->arr_cnt, h->arr_off, nc)) return r;
+    {
+        Timed t(h, K_GROUP);
+        hipLaunchKernelGGL(k_group, dim3(grid_for(h->n_est / 8 + 1, 256, 4096)), dim3(256), 0, h->stream, w);
+    }
+    {
+        Timed t(h, K_CELL_COUNT);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cell_count<DIM>), dim3(cell_grid(nc)), dim3(256), 0, h->stream, nc,
+                           old_off, h->buf[h->cur].id, pp, w, reseed);
+    }
+    if (int r = scan(h, h->final_cnt, w.new_off, nc + 1)) return r;
+    if (int r = scan(h, h->spawn_cnt, h->spawn_off, nc + 1)) return r;
+    if (mixed(h)) {
+        Timed t(h, K_CELL_WRITE32);
+        launch_cell_write32<DIM>(h->stream, cell_grid(nc), dev_mesh<DIM>(h), h->dV, old_off, h->buf[h->cur].id,
+                                 h->buf[h->cur].xi32, h->buf[h->cur].v32, h->buf[1 - h->cur].cell,
+                                 h->buf[1 - h->cur].id, h->buf[1 - h->cur].xi32, h->buf[1 - h->cur].v32,
+                                 h->buf[h->cur].vext, h->buf[1 - h->cur].vext, h->cap, pp, w);
+    } else {
+        Timed t(h, K_CELL_WRITE);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cell_write<DIM>), dim3(cell_grid(nc)), dim3(256), 0, h->stream,
+                           dev_mesh<DIM>(h), h->dV, old_off, dev_particles(h, h->cur), dev_particles(h, 1 - h->cur),
+                           pp, w);
+    }
+    {
+        Timed t(h, K_FINALIZE);
+        hipLaunchKernelGGL(k_finalize, dim3(1), dim3(64), 0, h->stream, nc, w);
+    }
+    HIPC(hipGetLastError());
+    h->cur = 1 - h->cur;
+    h->cur_off = 1 - h->cur_off;
+    return 0;
+}
+
+template <int DIM>
+int run_move(pf2_handle_s* h, int do_correct, int do_move) {
+    DevParams pp = dev_params(h);
+    DevWork w = dev_work(h);
+    const int grid = grid_for(h->n_est, 256, 256 * 16);
+    if (mixed(h)) {
+        h->fp64_valid = false;
+        Timed t(h, K_MOVE32);
+        launch_move32<DIM>(h->stream, grid, dev_mesh<DIM>(h), h->dV32, h->dVold32, h->dgeo32, h->buf[h->cur].cell,
+                           h->buf[h->cur].xi32, h->buf[h->cur].v32, h->buf[h->cur].vext, h->cap, pp, w, do_correct,
+                           do_move);
+    } else {
+        Timed t(h, K_MOVE);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_move<DIM>), dim3(grid), dim3(256), 0, h->stream, dev_mesh<DIM>(h), h->dV,
+                           h->dVold, dev_particles(h, h->cur), pp, w, do_correct, do_move);
+    }
+    HIPC(hipGetLastError());
+    if (!do_move) return 0;
+    return regroup<DIM>(h, 1);
+}
+
+template <int DIM>
+int run_project(pf2_handle_s* h) {
+    const int nc = h->n_cells;
+    DevWork w = dev_work(h);
+    if (mixed(h)) {
+        Timed t(h, K_PARTIALS32);
+        launch_partials32<DIM>(h->stream, cell_grid(nc), nc, h->off[h->cur_off], h->buf[h->cur].xi32, h->buf[h->cur].v32, h->cap,
+                               h->partial);
+    } else {
+        Timed t(h, K_PARTIALS);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cell_partials<DIM>), dim3(cell_grid(nc)), dim3(256), 0, h->stream, nc,
+                           h->off[h->cur_off], dev_particles(h, h->cur), h->partial);
+    }
+    {
+        Timed t(h, K_NODE);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_node<DIM>), dim3(grid_for(h->n_dofs, 256)), dim3(256), 0, h->stream,
+                           dev_mesh<DIM>(h), h->partial, h->dV, h->dVold, h->stats);
+    }
+    if (h->n_walls > 0) {
+        Timed t(h, K_WALL);
+        DevWalls wl;
+        wl.n = h->n_walls;
+        wl.dof = h->w_dof;
+        wl.bc = h->w_bc;
+        wl.mask = h->w_mask;
+        wl.fix = h->w_fix;
+        wl.val = h->w_val;
+        wl.vwall = h->w_vwall;
+        wl.n_cells = h->n_wall_cells;
+        wl.cells = h->w_cells;
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_wall_flags<DIM>), dim3(grid_for(h->n_walls, 256)), dim3(256), 0,
+                           h->stream, wl, h->dV);
+        if (mixed(h)) {
+            h->fp64_valid = false;
+            launch_wall_particles32<DIM>(h->stream, dev_mesh<DIM>(h), wl, h->off[h->cur_off], h->buf[h->cur].xi32,
+                                         h->buf[h->cur].v32, h->cap);
+        } else {
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_wall_particles<DIM>), dim3((h->n_wall_cells + 3) / 4), dim3(256), 0,
+                               h->stream, dev_mesh<DIM>(h), wl, h->off[h->cur_off], dev_particles(h, h->cur));
+        }
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_wall_set<DIM>), dim3(grid_for(h->n_walls, 256)), dim3(256), 0, h->stream,
+                           wl, h->dV, h->dVold);
+    }
+    if (mixed(h)) {
+        Timed t(h, K_MISC);
+        launch_fields_to32<DIM>(h->stream, h->dV, h->dVold, h->dV32, h->dVold32, h->n_dofs);
+    }
+    HIPC(hipGetLastError());
+    return 0;
+}
+
+int download_field(pf2_handle_s* h, double* V) {
+    const int n = h->n_dofs;
+    if (h->dim == 2)
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_to_comp_major<2>), dim3(grid_for(n, 256)), dim3(256), 0, h->stream, h->dV,
+                           h->dtmp, n);
+    else
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_to_comp_major<3>), dim3(grid_for(n, 256)), dim3(256), 0, h->stream, h->dV,
+                           h->dtmp, n);
+    HIPC(hipGetLastError());
+    HIPC(hipMemcpyAsync(V, h->dtmp, sizeof(double) * (size_t)n * h->dim, hipMemcpyDeviceToHost, h->stream));
+    HIPC(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+int upload_field(pf2_handle_s* h, const double* src, double* dst, hipMemcpyKind kind) {
+    const int n = h->n_dofs;
+    HIPC(hipMemcpyAsync(h->dtmp, src, sizeof(double) * (size_t)n * h->dim, kind, h->stream));
+    if (h->dim == 2)
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_to_node_major<2>), dim3(grid_for(n, 256)), dim3(256), 0, h->stream,
+                           h->dtmp, dst, n);
+    else
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_to_node_major<3>), dim3(grid_for(n, 256)), dim3(256), 0, h->stream,
+                           h->dtmp, dst, n);
+    HIPC(hipGetLastError());
+    // the host source buffer may be reused by the caller after return
+    if (kind == hipMemcpyHostToDevice) HIPC(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+int sync_fields32(pf2_handle_s* h) {
+    if (!mixed(h)) return 0;
+    if (h->dim == 2) launch_fields_to32<2>(h->stream, h->dV, h->dVold, h->dV32, h->dVold32, h->n_dofs);
+    else launch_fields_to32<3>(h->stream, h->dV, h->dVold, h->dV32, h->dVold32, h->n_dofs);
+    HIPC(hipGetLastError());
+    return 0;
+}
+
+// FP64 state <-> FP32-mixed state (downloads and inserts go through FP64)
+int to_mixed(pf2_handle_s* h) {
+    if (!mixed(h)) return 0;
+    h->fp64_valid = true;
+    launch_state_to32(h->stream, h->dim, h->buf[h->cur].xi, h->buf[h->cur].v, h->buf[h->cur].xi32,
+                      h->buf[h->cur].v32, h->cap, h->counters, h->n_est);
+    HIPC(hipGetLastError());
+    return 0;
+}
+
+int from_mixed(pf2_handle_s* h) {
+    if (!mixed(h) || h->fp64_valid) return 0;
+    if (h->dim == 2)
+        launch_state_from32<2>(h->stream, dev_mesh<2>(h), h->buf[h->cur].cell, h->buf[h->cur].xi32,
+                               h->buf[h->cur].v32, h->buf[h->cur].x, h->buf[h->cur].xi, h->buf[h->cur].v, h->cap,
+                               h->counters, h->n_est);
+    else
+        launch_state_from32<3>(h->stream, dev_mesh<3>(h), h->buf[h->cur].cell, h->buf[h->cur].xi32,
+                               h->buf[h->cur].v32, h->buf[h->cur].x, h->buf[h->cur].xi, h->buf[h->cur].v, h->cap,
+                               h->counters, h->n_est);
+    HIPC(hipGetLastError());
+    h->fp64_valid = true;
+    return 0;
+}
+
+}  // namespace
+
+// ============================================================================
+extern "C" {
+
+const char* pf2_last_error(void) { return g_err.c_str(); }
+
+int pf2_create(const pf2_mesh* mesh, const pf2_params* params, long long capacity, pf2_handle* out) {
+    if (!mesh || !out) return fail(PF2_E_ARG, "mesh/out is NULL");
+    *out = nullptr;
+    if (mesh->dim != 2 && mesh->dim != 3) return fail(PF2_E_ARG, "dim must be 2 or 3");
+    if (mesh->n_cells < 1 || mesh->n_vertices < 1 || mesh->n_dofs < 1 || !mesh->vertices || !mesh->cell_vertices)
+        return fail(PF2_E_ARG, "empty mesh");
+    if (int r = check_params(mesh->dim, params)) return r;
+    pf2_handle_s* h = new pf2_handle_s();
+    h->dim = mesh->dim;
+    h->nv = 1 << mesh->dim;
+    h->n_cells = mesh->n_cells;
+    h->n_vertices = mesh->n_vertices;
+    h->n_dofs = mesh->n_dofs;
+    h->params = *params;
+    h->cap = capacity > 0 ? capacity : default_capacity(h->dim, h->n_cells, params);
+    if (h->cap > 0x7fffffffLL) {
+        delete h;
+        return fail(PF2_E_ARG, "capacity above 2^31-1 particles");
+    }
+    const size_t nck = (size_t)h->n_cells * h->nv;
+    h->X.assign(mesh->vertices, mesh->vertices + (size_t)h->n_vertices * h->dim);
+    h->cv.assign(mesh->cell_vertices, mesh->cell_vertices + nck);
+    h->cd.assign(mesh->cell_dofs ? mesh->cell_dofs : mesh->cell_vertices,
+                 (mesh->cell_dofs ? mesh->cell_dofs : mesh->cell_vertices) + nck);
+    for (size_t i = 0; i < nck; ++i) {
+        if (h->cv[i] < 0 || h->cv[i] >= h->n_vertices || h->cd[i] < 0 || h->cd[i] >= h->n_dofs) {
+            delete h;
+            return fail(PF2_E_ARG, "cell vertex/dof index out of range");
+        }
+    }
+    // vertex star (GridTools::vertex_to_cell_map: std::set => ascending cells)
+    h->star_off.assign(h->n_vertices + 1, 0);
+    for (size_t i = 0; i < nck; ++i) h->star_off[h->cv[i] + 1]++;
+    for (int v = 0; v < h->n_vertices; ++v) h->star_off[v + 1] += h->star_off[v];
+    h->star_cell.assign(h->star_off[h->n_vertices], 0);
+    {
+        std::vector<int> fill(h->star_off.begin(), h->star_off.end() - 1);
+        for (int c = 0; c < h->n_cells; ++c)
+            for (int k = 0; k < h->nv; ++k) h->star_cell[fill[h->cv[(size_t)c * h->nv + k]]++] = c;
+    }
+    compute_star_dirs(h);
+    // DoF -> (cell, local vertex), ascending cells
+    h->dof_off.assign(h->n_dofs + 1, 0);
+    for (size_t i = 0; i < nck; ++i) h->dof_off[h->cd[i] + 1]++;
+    for (int v = 0; v < h->n_dofs; ++v) h->dof_off[v + 1] += h->dof_off[v];
+    h->dof_cell.assign(h->dof_off[h->n_dofs], 0);
+    {
+        std::vector<int> fill(h->dof_off.begin(), h->dof_off.end() - 1);
+        for (int c = 0; c < h->n_cells; ++c)
+            for (int k = 0; k < h->nv; ++k) h->dof_cell[fill[h->cd[(size_t)c * h->nv + k]]++] = c * h->nv + k;
+    }
+    auto bail = [&](int r) {
+        pf2_destroy(h);
+        return r;
+    };
+    if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess)
+        return bail(fail(PF2_E_HIP, "hipStreamCreate failed"));
+    const int nc = h->n_cells, nvt = h->n_vertices, nd = h->n_dofs, dim = h->dim;
+    if (dalloc(&h->dX, (size_t)nvt * dim) || dalloc(&h->dcv, nck) || dalloc(&h->dcd, nck) ||
+        dalloc(&h->dstar_off, nvt + 1) || dalloc(&h->dstar_cell, h->star_cell.size()) ||
+        dalloc(&h->dstar_dir, h->star_dir.size()) || dalloc(&h->ddof_off, nd + 1) ||
+        dalloc(&h->ddof_cell, h->dof_cell.size()) || dalloc(&h->dftol, nc) || dalloc(&h->dV, (size_t)nd * dim) ||
+        dalloc(&h->dVold, (size_t)nd * dim) || dalloc(&h->dtmp, (size_t)nd * dim) || dalloc(&h->off[0], nc + 1) ||
+        dalloc(&h->off[1], nc + 1) || dalloc(&h->arr_cnt, nc + 1) || dalloc(&h->arr_off, nc + 1) ||
+        dalloc(&h->arr_fill, nc + 1) || dalloc(&h->final_cnt, nc + 1) || dalloc(&h->spawn_cnt, nc + 1) ||
+        dalloc(&h->spawn_off, nc + 1) || dalloc(&h->hitmask, nc) || dalloc(&h->nsurv, nc) ||
+        dalloc(&h->partial, (size_t)nc * h->nv * (dim + 1)) || dalloc(&h->counters, 4) ||
+        dalloc(&h->pcounter, 1) || dalloc(&h->stats, 5))
+        return bail(PF2_E_HIP);
+    if (mixed(h) || true) {
+        if (dalloc(&h->dV32, (size_t)nd * dim) || dalloc(&h->dVold32, (size_t)nd * dim) ||
+            dalloc(&h->dgeo32, (size_t)nc * geo32_floats(dim)))
+            return bail(PF2_E_HIP);
+    }
+    if (alloc_particles(h)) return bail(PF2_E_HIP);
+    auto up = [&](void* d, const void* s, size_t bytes) {
+        return hipMemcpyAsync(d, s, bytes, hipMemcpyHostToDevice, h->stream);
+    };
+    if (up(h->dX, h->X.data(), h->X.size() * 8) || up(h->dcv, h->cv.data(), nck * 4) ||
+        up(h->dcd, h->cd.data(), nck * 4) || up(h->dstar_off, h->star_off.data(), h->star_off.size() * 4) ||
+        up(h->dstar_cell, h->star_cell.data(), h->star_cell.size() * 4) ||
+        up(h->dstar_dir, h->star_dir.data(), h->star_dir.size() * 8) ||
+        up(h->ddof_off, h->dof_off.data(), h->dof_off.size() * 4) ||
+        up(h->ddof_cell, h->dof_cell.data(), h->dof_cell.size() * 4))
+        return bail(fail(PF2_E_HIP, "mesh upload failed"));
+    if (hipMemsetAsync(h->dV, 0, sizeof(double) * nd * dim, h->stream) ||
+        hipMemsetAsync(h->dVold, 0, sizeof(double) * nd * dim, h->stream) ||
+        hipMemsetAsync(h->off[0], 0, sizeof(int) * (nc + 1), h->stream) ||
+        hipMemsetAsync(h->off[1], 0, sizeof(int) * (nc + 1), h->stream) ||
+        hipMemsetAsync(h->arr_cnt, 0, sizeof(int) * (nc + 1), h->stream) ||
+        hipMemsetAsync(h->arr_fill, 0, sizeof(int) * (nc + 1), h->stream) ||
+        hipMemsetAsync(h->final_cnt, 0, sizeof(int) * (nc + 1), h->stream) ||
+        hipMemsetAsync(h->spawn_cnt, 0, sizeof(int) * (nc + 1), h->stream) ||
+        hipMemsetAsync(h->counters, 0, sizeof(int) * 4, h->stream) ||
+        hipMemsetAsync(h->pcounter, 0, sizeof(uint32_t), h->stream) ||
+        hipMemsetAsync(h->stats, 0, sizeof(unsigned long long) * 5, h->stream))
+        return bail(fail(PF2_E_HIP, "initialisation failed"));
+    if (hipcub::DeviceScan::ExclusiveSum(nullptr, h->scan_tmp_bytes, h->final_cnt, h->off[0], nc + 1, h->stream) !=
+        hipSuccess)
+        return bail(fail(PF2_E_HIP, "scan sizing failed"));
+    if (dalloc((char**)&h->scan_tmp, h->scan_tmp_bytes)) return bail(PF2_E_HIP);
+    if (h->dim == 2) launch_geo<2>(h->stream, dev_mesh<2>(h), h->dgeo32, h->dftol);
+    else launch_geo<3>(h->stream, dev_mesh<3>(h), h->dgeo32, h->dftol);
+    if (hipStreamSynchronize(h->stream) != hipSuccess) return bail(fail(PF2_E_HIP, "initialisation sync failed"));
+    *out = h;
+    return 0;
+}
+
+int pf2_destroy(pf2_handle h) {
+    if (!h) return 0;
+    if (h->stream) hipStreamSynchronize(h->stream);
+    free_particles(h);
+    void* ptrs[] = {h->dX, h->dcv, h->dcd, h->dstar_off, h->dstar_cell, h->dstar_dir, h->ddof_off, h->ddof_cell,
+                    h->dftol, h->dV, h->dVold, h->dtmp, h->dV32, h->dVold32, h->dgeo32, h->off[0], h->off[1], h->arr_cnt,
+                    h->arr_off, h->arr_fill, h->final_cnt, h->spawn_cnt, h->spawn_off, h->hitmask, h->nsurv,
+                    h->partial, h->counters, h->pcounter, h->stats, h->scan_tmp, h->w_dof, h->w_mask, h->w_fix,
+                    h->w_vwall, h->w_cells, h->w_bc, h->w_val};
+    for (void* p : ptrs)
+        if (p) hipFree(p);
+    for (auto& e : h->ev_free) {
+        hipEventDestroy(e.first);
+        hipEventDestroy(e.second);
+    }
+    for (auto& e : h->ev_pending) {
+        hipEventDestroy(e.second.first);
+        hipEventDestroy(e.second.second);
+    }
+    if (h->stream) hipStreamDestroy(h->stream);
+    delete h;
+    return 0;
+}
+
+int pf2_set_params(pf2_handle h, const pf2_params* params) {
+    if (!h) return fail(PF2_E_ARG, "handle is NULL");
+    if (int r = check_params(h->dim, params)) return r;
+    if (params->precision != h->params.precision)
+        return fail(PF2_E_STATE, "precision is fixed at pf2_create");
+    if (params->keep_velocity_ext != h->params.keep_velocity_ext)
+        return fail(PF2_E_STATE, "keep_velocity_ext is fixed at pf2_create");
+    if (default_capacity(h->dim, h->n_cells, params) > h->cap)
+        return fail(PF2_E_CAPACITY, "new lattice / MAX exceed the particle capacity");
+    h->params = *params;
+    return 0;
+}
+
+int pf2_upload_fields(pf2_handle h, const double* V, const double* V_old) {
+    if (!h) return fail(PF2_E_ARG, "handle is NULL");
+    if (V)
+        if (int r = upload_field(h, V, h->dV, hipMemcpyHostToDevice)) return r;
+    if (V_old)
+        if (int r = upload_field(h, V_old, h->dVold, hipMemcpyHostToDevice)) return r;
+    return sync_fields32(h);
+}
+
+int pf2_upload_fields_device(pf2_handle h, const double* V, const double* V_old) {
+    if (!h) return fail(PF2_E_ARG, "handle is NULL");
+    if (V)
+        if (int r = upload_field(h, V, h->dV, hipMemcpyDeviceToDevice)) return r;
+    if (V_old)
+        if (int r = upload_field(h, V_old, h->dVold, hipMemcpyDeviceToDevice)) return r;
+    return sync_fields32(h);
+}
+
+int pf2_download_field(pf2_handle h, double* V) {
+    if (!h || !V) return fail(PF2_E_ARG, "NULL argument");
+    return download_field(h, V);
+}
+
+int pf2_seed_particles(pf2_handle h) {
+    if (!h) return fail(PF2_E_ARG, "handle is NULL");
+    long long n = 0;
+    if (int r = get_count(h, &n)) return r;
+    if (n != 0) return fail(PF2_E_STATE, "seed_particles on a handle that already holds particles");
+    DevParams pp = dev_params(h);
+    const long long total = (long long)h->n_cells * pp.nparts_in;
+    if (total > h->cap) return fail(PF2_E_CAPACITY, "seeding exceeds the particle capacity");
+    uint32_t base = 0;
+    HIPC(hipMemcpyAsync(&base, h->pcounter, 4, hipMemcpyDeviceToHost, h->stream));
+    HIPC(hipStreamSynchronize(h->stream));
+    {
+        Timed t(h, K_SEED);
+        if (h->dim == 2)
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_seed<2>), dim3(grid_for(total, 256)), dim3(256), 0, h->stream,
+                               dev_mesh<2>(h), h->dV, dev_particles(h, h->cur), pp, base, h->off[h->cur_off]);
+        else
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_seed<3>), dim3(grid_for(total, 256)), dim3(256), 0, h->stream,
+                               dev_mesh<3>(h), h->dV, dev_particles(h, h->cur), pp, base, h->off[h->cur_off]);
+    }
+    HIPC(hipGetLastError());
+    const int cnt = (int)total;
+    const uint32_t nb = base + (uint32_t)total;
+    HIPC(hipMemcpyAsync(h->counters, &cnt, 4, hipMemcpyHostToDevice, h->stream));
+    HIPC(hipMemcpyAsync(h->pcounter, &nb, 4, hipMemcpyHostToDevice, h->stream));
+    h->n_est = total;
+    if (int r = to_mixed(h)) return r;
+    HIPC(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+int pf2_download_particles(pf2_handle h, int* cell, uint32_t* id, double* x, double* xi, double* v) {
+    if (!h) return fail(PF2_E_ARG, "handle is NULL");
+    long long n = 0;
+    if (int r = get_count(h, &n)) return r;
+    if (int r = from_mixed(h)) return r;
+    const PBufs& b = h->buf[h->cur];
+    const int dim = h->dim;
+    if (cell) HIPC(hipMemcpyAsync(cell, b.cell, n * 4, hipMemcpyDeviceToHost, h->stream));
+    if (id) HIPC(hipMemcpyAsync(id, b.id, n * 4, hipMemcpyDeviceToHost, h->stream));
+    std::vector<double> tmp((size_t)n * dim);
+    double* srcs[3] = {b.x, b.xi, b.v};
+    double* dsts[3] = {x, xi, v};
+    for (int a = 0; a < 3; ++a) {
+        if (!dsts[a]) continue;
+        for (int d = 0; d < dim; ++d)
+            HIPC(hipMemcpyAsync(tmp.data() + (size_t)d * n, srcs[a] + (size_t)d * h->cap, n * 8,
+                                hipMemcpyDeviceToHost, h->stream));
+        HIPC(hipStreamSynchronize(h->stream));
+        for (long long i = 0; i < n; ++i)
+            for (int d = 0; d < dim; ++d) dsts[a][i * dim + d] = tmp[(size_t)d * n + i];
+    }
+    HIPC(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+int pf2_insert_particles(pf2_handle h, int n, const int* cell, const double* xi, const double* v) {
+    if (!h) return fail(PF2_E_ARG, "handle is NULL");
+    if (n < 0 || (n > 0 && (!cell || !xi))) return fail(PF2_E_ARG, "invalid particle arrays");
+    const int dim = h->dim;
+    for (int i = 0; i < n; ++i) {
+        if (cell[i] < 0 || cell[i] >= h->n_cells) return fail(PF2_E_ARG, "cell index out of range");
+        for (int d = 0; d < dim; ++d)
+            if (!(xi[(size_t)i * dim + d] >= 0.0 && xi[(size_t)i * dim + d] <= 1.0))
+                return fail(PF2_E_ARG, "reference coordinates must lie in [0,1]");
+    }
+    long long n0 = 0;
+    if (int r = get_count(h, &n0)) return r;
+    if (n0 + n > h->cap) return fail(PF2_E_CAPACITY, "insert exceeds the particle capacity");
+    std::vector<int> c0(n0);
+    std::vector<uint32_t> id0(n0);
+    std::vector<double> x0((size_t)n0 * dim), xi0((size_t)n0 * dim), v0((size_t)n0 * dim);
+    if (n0 > 0)
+        if (int r = pf2_download_particles(h, c0.data(), id0.data(), x0.data(), xi0.data(), v0.data())) return r;
+    uint32_t base = 0;
+    HIPC(hipMemcpyAsync(&base, h->pcounter, 4, hipMemcpyDeviceToHost, h->stream));
+    HIPC(hipStreamSynchronize(h->stream));
+    const long long N = n0 + n;
+    struct Rec { int cell; uint32_t id; long long src; };
+    std::vector<Rec> recs(N);
+    for (long long i = 0; i < n0; ++i) recs[i] = {c0[i], id0[i], i};
+    for (int i = 0; i < n; ++i) recs[n0 + i] = {cell[i], base + (uint32_t)i + 1u, n0 + i};
+    std::stable_sort(recs.begin(), recs.end(), [](const Rec& a, const Rec& b) {
+        return a.cell != b.cell ? a.cell < b.cell : a.id < b.id;
+    });
+    std::vector<int> hc(N), offh(h->n_cells + 1, 0);
+    std::vector<uint32_t> hid(N);
+    std::vector<unsigned char> flags(N);
+    std::vector<double> hx((size_t)dim * N), hxi((size_t)dim * N), hv((size_t)dim * N);
+    for (long long i = 0; i < N; ++i) {
+        const Rec& r = recs[i];
+        hc[i] = r.cell;
+        hid[i] = r.id;
+        offh[r.cell + 1]++;
+        const bool is_new = r.src >= n0;
+        flags[i] = is_new ? (v ? 1 : 3) : 0;   // bit0: map location, bit1: interpolate velocity
+        for (int d = 0; d < dim; ++d) {
+            if (!is_new) {
+                hx[(size_t)d * N + i] = x0[r.src * dim + d];
+                hxi[(size_t)d * N + i] = xi0[r.src * dim + d];
+                hv[(size_t)d * N + i] = v0[r.src * dim + d];
+            } else {
+                const long long s = r.src - n0;
+                hx[(size_t)d * N + i] = 0.0;
+                hxi[(size_t)d * N + i] = xi[s * dim + d];
+                hv[(size_t)d * N + i] = v ? v[s * dim + d] : 0.0;
+            }
+        }
+    }
+    for (int c = 0; c < h->n_cells; ++c) offh[c + 1] += offh[c];
+    const PBufs& b = h->buf[h->cur];
+    unsigned char* dflags = nullptr;
+    if (dalloc(&dflags, N)) return PF2_E_HIP;
+    HIPC(hipMemcpyAsync(b.cell, hc.data(), N * 4, hipMemcpyHostToDevice, h->stream));
+    HIPC(hipMemcpyAsync(b.id, hid.data(), N * 4, hipMemcpyHostToDevice, h->stream));
+    for (int d = 0; d < dim; ++d) {
+        HIPC(hipMemcpyAsync(b.x + (size_t)d * h->cap, hx.data() + (size_t)d * N, N * 8, hipMemcpyHostToDevice, h->stream));
+        HIPC(hipMemcpyAsync(b.xi + (size_t)d * h->cap, hxi.data() + (size_t)d * N, N * 8, hipMemcpyHostToDevice, h->stream));
+        HIPC(hipMemcpyAsync(b.v + (size_t)d * h->cap, hv.data() + (size_t)d * N, N * 8, hipMemcpyHostToDevice, h->stream));
+    }
+    HIPC(hipMemcpyAsync(dflags, flags.data(), N, hipMemcpyHostToDevice, h->stream));
+    HIPC(hipMemcpyAsync(h->off[h->cur_off], offh.data(), (h->n_cells + 1) * 4, hipMemcpyHostToDevice, h->stream));
+    {
+        Timed t(h, K_SEED);
+        if (dim == 2)
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_init_particles<2>), dim3(grid_for(N, 256)), dim3(256), 0, h->stream,
+                               dev_mesh<2>(h), h->dV, dev_particles(h, h->cur), dflags, N,
+                               h->params.keep_velocity_ext);
+        else
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_init_particles<3>), dim3(grid_for(N, 256)), dim3(256), 0, h->stream,
+                               dev_mesh<3>(h), h->dV, dev_particles(h, h->cur), dflags, N,
+                               h->params.keep_velocity_ext);
+    }
+    HIPC(hipGetLastError());
+    const int cnt = (int)N;
+    const uint32_t nb = base + (uint32_t)n;
+    HIPC(hipMemcpyAsync(h->counters, &cnt, 4, hipMemcpyHostToDevice, h->stream));
+    HIPC(hipMemcpyAsync(h->pcounter, &nb, 4, hipMemcpyHostToDevice, h->stream));
+    h->n_est = N;
+    if (int r = to_mixed(h)) return r;
+    HIPC(hipStreamSynchronize(h->stream));
+    hipFree(dflags);
+    return 0;
+}
+
+int pf2_correct_particles_velocities(pf2_handle h) {
+    if (!h) return fail(PF2_E_ARG, "handle is NULL");
+    return h->dim == 2 ? run_move<2>(h, 1, 0) : run_move<3>(h, 1, 0);
+}
+
+int pf2_move_particles(pf2_handle h) {
+    if (!h) return fail(PF2_E_ARG, "handle is NULL");
+    return h->dim == 2 ? run_move<2>(h, 0, 1) : run_move<3>(h, 0, 1);
+}
+
+int pf2_distribute_particle_velocities_to_grid(pf2_handle h, double* V_out) {
+    if (!h) return fail(PF2_E_ARG, "handle is NULL");
+    if (int r = (h->dim == 2 ? run_project<2>(h) : run_project<3>(h))) return r;
+    if (V_out) return download_field(h, V_out);
+    return 0;
+}
+
+int pf2_step(pf2_handle h, double* V_out) {
+    if (!h) return fail(PF2_E_ARG, "handle is NULL");
+    if (int r = (h->dim == 2 ? run_move<2>(h, 1, 1) : run_move<3>(h, 1, 1))) return r;
+    return pf2_distribute_particle_velocities_to_grid(h, V_out);
+}
+
+int pf2_set_walls(pf2_handle h, int n, const int* vertex, const int* dof, const double* bc, const int* mask) {
+    if (!h) return fail(PF2_E_ARG, "handle is NULL");
+    hipFree(h->w_dof); hipFree(h->w_mask); hipFree(h->w_fix); hipFree(h->w_vwall); hipFree(h->w_cells);
+    hipFree(h->w_bc); hipFree(h->w_val);
+    h->w_dof = h->w_mask = h->w_fix = h->w_vwall = h->w_cells = nullptr;
+    h->w_bc = h->w_val = nullptr;
+    h->n_walls = h->n_wall_cells = 0;
+    if (n <= 0) return 0;
+    if (!vertex || !dof || !bc || !mask) return fail(PF2_E_ARG, "wall arrays are NULL");
+    struct W { int v, dof, mask; double bc[3]; };
+    std::vector<W> ws(n);
+    for (int i = 0; i < n; ++i) {
+        if (vertex[i] < 0 || vertex[i] >= h->n_vertices || dof[i] < 0 || dof[i] >= h->n_dofs)
+            return fail(PF2_E_ARG, "wall vertex/dof out of range");
+        ws[i].v = vertex[i];
+        ws[i].dof = dof[i];
+        ws[i].mask = mask[i];
+        for (int d = 0; d < 3; ++d) ws[i].bc[d] = d < h->dim ? bc[(size_t)i * h->dim + d] : 0.0;
+    }
+    std::stable_sort(ws.begin(), ws.end(), [](const W& a, const W& b) { return a.v < b.v; });
+    for (int i = 1; i < n; ++i)
+        if (ws[i].v == ws[i - 1].v) return fail(PF2_E_ARG, "duplicate wall vertex");
+    std::vector<int> wd(n), wm(n), vwall(h->n_vertices, -1);
+    std::vector<double> wbc((size_t)n * 3);
+    std::vector<char> mark(h->n_cells, 0);
+    for (int i = 0; i < n; ++i) {
+        wd[i] = ws[i].dof;
+        wm[i] = ws[i].mask;
+        for (int d = 0; d < 3; ++d) wbc[(size_t)i * 3 + d] = ws[i].bc[d];
+        vwall[ws[i].v] = i;
+        for (int e = h->star_off[ws[i].v]; e < h->star_off[ws[i].v + 1]; ++e) mark[h->star_cell[e]] = 1;
+    }
+    std::vector<int> cells;
+    for (int c = 0; c < h->n_cells; ++c)
+        if (mark[c]) cells.push_back(c);
+    if (dalloc(&h->w_dof, n) || dalloc(&h->w_mask, n) || dalloc(&h->w_fix, n) || dalloc(&h->w_vwall, h->n_vertices) ||
+        dalloc(&h->w_cells, cells.size()) || dalloc(&h->w_bc, (size_t)n * 3) || dalloc(&h->w_val, (size_t)n * 3))
+        return PF2_E_HIP;
+    HIPC(hipMemcpyAsync(h->w_dof, wd.data(), n * 4, hipMemcpyHostToDevice, h->stream));
+    HIPC(hipMemcpyAsync(h->w_mask, wm.data(), n * 4, hipMemcpyHostToDevice, h->stream));
+    HIPC(hipMemcpyAsync(h->w_vwall, vwall.data(), h->n_vertices * 4, hipMemcpyHostToDevice, h->stream));
+    HIPC(hipMemcpyAsync(h->w_cells, cells.data(), cells.size() * 4, hipMemcpyHostToDevice, h->stream));
+    HIPC(hipMemcpyAsync(h->w_bc, wbc.data(), wbc.size() * 8, hipMemcpyHostToDevice, h->stream));
+    HIPC(hipStreamSynchronize(h->stream));
+    h->n_walls = n;
+    h->n_wall_cells = (int)cells.size();
+    return 0;
+}
+
+int pf2_sort_particles_into_subdomains_and_cells(pf2_handle h) {
+    if (!h) return fail(PF2_E_ARG, "handle is NULL");
+    if (!h->params.keep_velocity_ext)
+        return fail(PF2_E_STATE, "relocation without advection needs keep_velocity_ext");
+    if (int r = from_mixed(h)) return r;
+    DevParams pp = dev_params(h);
+    DevWork w = dev_work(h);
+    {
+        Timed t(h, K_MOVE);
+        if (h->dim == 2)
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_resort<2>), dim3(grid_for(h->n_est, 256, 4096)), dim3(256), 0,
+                               h->stream, dev_mesh<2>(h), dev_particles(h, h->cur), pp, w);
+        else
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_resort<3>), dim3(grid_for(h->n_est, 256, 4096)), dim3(256), 0,
+                               h->stream, dev_mesh<3>(h), dev_particles(h, h->cur), pp, w);
+    }
+    HIPC(hipGetLastError());
+    const int saved = h->params.precision;
+    h->params.precision = PF2_FP64;   // regroup the FP64 copy, then convert back
+    int r = h->dim == 2 ? regroup<2>(h, 0) : regroup<3>(h, 0);
+    h->params.precision = saved;
+    if (r) return r;
+    long long n;
+    if (int q = get_count(h, &n)) return q;
+    return to_mixed(h);
+}
+
+int pf2_check_empty_cells(pf2_handle h) {
+    if (!h) return fail(PF2_E_ARG, "handle is NULL");
+    if (int r = from_mixed(h)) return r;
+    DevParams pp = dev_params(h);
+    DevWork w = dev_work(h);
+    {
+        Timed t(h, K_MISC);
+        if (h->dim == 2)
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_keys<2>), dim3(grid_for(h->n_est, 256, 4096)), dim3(256), 0,
+                               h->stream, dev_particles(h, h->cur), pp, w);
+        else
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_keys<3>), dim3(grid_for(h->n_est, 256, 4096)), dim3(256), 0,
+                               h->stream, dev_particles(h, h->cur), pp, w);
+    }
+    HIPC(hipGetLastError());
+    const int saved = h->params.precision;
+    h->params.precision = PF2_FP64;
+    int r = h->dim == 2 ? regroup<2>(h, 1) : regroup<3>(h, 1);
+    h->params.precision = saved;
+    if (r) return r;
+    long long n;
+    if (int q = get_count(h, &n)) return q;
+    return to_mixed(h);
+}
+
+int pf2_update_vertices(pf2_handle h, const double* vertices, int recompute_maps) {
+    if (!h || !vertices) return fail(PF2_E_ARG, "NULL argument");
+    // positions of FP32-mixed particles are F_K(xi) on the mesh before the move
+    if (int r = from_mixed(h)) return r;
+    h->X.assign(vertices, vertices + (size_t)h->n_vertices * h->dim);
+    HIPC(hipMemcpyAsync(h->dX, h->X.data(), h->X.size() * 8, hipMemcpyHostToDevice, h->stream));
+    if (recompute_maps) {
+        compute_star_dirs(h);
+        if (int r = upload_star_dirs(h)) return r;
+    }
+    if (h->dim == 2) launch_geo<2>(h->stream, dev_mesh<2>(h), h->dgeo32, h->dftol);
+    else launch_geo<3>(h->stream, dev_mesh<3>(h), h->dgeo32, h->dftol);
+    HIPC(hipGetLastError());
+    HIPC(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+int pf2_initialize_maps(pf2_handle h) {
+    if (!h) return fail(PF2_E_ARG, "handle is NULL");
+    compute_star_dirs(h);
+    if (int r = upload_star_dirs(h)) return r;
+    HIPC(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+int pf2_n_global_particles(pf2_handle h, long long* n) {
+    if (!h || !n) return fail(PF2_E_ARG, "NULL argument");
+    return get_count(h, n);
+}
+
+int pf2_cell_counts(pf2_handle h, int* counts) {
+    if (!h || !counts) return fail(PF2_E_ARG, "NULL argument");
+    std::vector<int> off(h->n_cells + 1);
+    HIPC(hipMemcpyAsync(off.data(), h->off[h->cur_off], off.size() * 4, hipMemcpyDeviceToHost, h->stream));
+    HIPC(hipStreamSynchronize(h->stream));
+    for (int c = 0; c < h->n_cells; ++c) counts[c] = off[c + 1] - off[c];
+    return 0;
+}
+
+int pf2_particle_counter(pf2_handle h, uint32_t* count) {
+    if (!h || !count) return fail(PF2_E_ARG, "NULL argument");
+    HIPC(hipMemcpyAsync(count, h->pcounter, 4, hipMemcpyDeviceToHost, h->stream));
+    HIPC(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+int pf2_stats(pf2_handle h, unsigned long long* out5) {
+    if (!h || !out5) return fail(PF2_E_ARG, "NULL argument");
+    HIPC(hipMemcpyAsync(out5, h->stats, 5 * 8, hipMemcpyDeviceToHost, h->stream));
+    HIPC(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+int pf2_synchronize(pf2_handle h) {
+    if (!h) return fail(PF2_E_ARG, "handle is NULL");
+    long long n;
+    return get_count(h, &n);
+}
+
+int pf2_set_profiling(pf2_handle h, int on) {
+    if (!h) return fail(PF2_E_ARG, "handle is NULL");
+    if (int r = flush_events(h)) return r;
+    h->profiling = on != 0;
+    for (int i = 0; i < K_NKERNELS; ++i) {
+        h->k_ms[i] = 0;
+        h->k_n[i] = 0;
+    }
+    return 0;
+}
+
+int pf2_kernel_times(pf2_handle h, double* total_ms, long long* launches, int cap) {
+    if (!h) return fail(PF2_E_ARG, "handle is NULL");
+    if (int r = flush_events(h)) return r;
+    for (int i = 0; i < K_NKERNELS && i < cap; ++i) {
+        if (total_ms) total_ms[i] = h->k_ms[i];
+        if (launches) launches[i] = h->k_n[i];
+    }
+    return K_NKERNELS;
+}
+
+const char* pf2_kernel_name(int i) { return (i >= 0 && i < K_NKERNELS) ? kKernelNames[i] : ""; }
+
+}  // extern "C"

@@ -1,0 +1,573 @@
+// pf2_mixed.hpp — FP32-mixed variant of the particle stage.
+//
+// A particle is stored as (cell, id, xi in FP32, v in FP32): 8 + 4*DIM*2
+// bytes instead of 8 + 8*DIM*3 in the FP64 path (there is no absolute
+// position; it is F_K(xi) on demand).  Within a substep the displacement
+// h*u is applied in reference coordinates of the current cell by a short
+// FP32 Newton solve of F_K(xi + delta) - F_K(xi) = h*u, written with the
+// exact multilinear difference so no absolute coordinates enter the FP32
+// arithmetic.  Only a particle whose new reference point leaves the unit
+// cell goes through the FP64 path: F_K(xi) + h*u in FP64 and the relocation
+// rule of the reference (vertex star, FP64 inverse map).  Projection sums use
+// Kahan-compensated FP32 per cell and FP64 node sums.
+#pragma once
+
+#include "pf2_kernels.hpp"
+
+namespace pf2 {
+
+inline int geo32_floats(int dim) { return dim == 2 ? 6 : 21; }
+
+// multilinear coefficients of F_K relative to vertex 0, in FP32
+//   2-D: F = x0 + a xi0 + b xi1 + c xi0 xi1
+//   3-D: F = x0 + a xi0 + b xi1 + c xi2 + d xi0 xi1 + e xi0 xi2 + f xi1 xi2 + g xi0 xi1 xi2
+template <int DIM>
+__global__ void k_geo32(DevMesh<DIM> m, float* __restrict__ geo) {
+    constexpr int NV = Q1<DIM>::NV;
+    constexpr int NG = DIM == 2 ? 6 : 21;
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < m.n_cells; c += gridDim.x * blockDim.x) {
+        double P[NV][DIM];
+        load_vertices<DIM>(m, c, P);
+        float* g = geo + (size_t)c * NG;
+        if constexpr (DIM == 2) {
+            for (int d = 0; d < 2; ++d) {
+                g[0 + d] = (float)(P[1][d] - P[0][d]);
+                g[2 + d] = (float)(P[2][d] - P[0][d]);
+                g[4 + d] = (float)((P[0][d] - P[1][d]) - (P[2][d] - P[3][d]));
+            }
+        } else {
+            for (int d = 0; d < 3; ++d) {
+                const double x0 = P[0][d], x1 = P[1][d], x2 = P[2][d], x3 = P[3][d];
+                const double x4 = P[4][d], x5 = P[5][d], x6 = P[6][d], x7 = P[7][d];
+                g[0 + d] = (float)(x1 - x0);                                    // a
+                g[3 + d] = (float)(x2 - x0);                                    // b
+                g[6 + d] = (float)(x4 - x0);                                    // c
+                g[9 + d] = (float)((x0 - x1) - (x2 - x3));                      // d (xi0 xi1)
+                g[12 + d] = (float)((x0 - x1) - (x4 - x5));                     // e (xi0 xi2)
+                g[15 + d] = (float)((x0 - x2) - (x4 - x6));                     // f (xi1 xi2)
+                g[18 + d] = (float)(((x1 - x0) - (x3 - x2)) - ((x5 - x4) - (x7 - x6)));   // g
+            }
+        }
+    }
+}
+
+// per-cell FP64 tolerance of the clearly-outside test
+template <int DIM>
+__global__ void k_ftol(DevMesh<DIM> m, double* __restrict__ ftol) {
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < m.n_cells; c += gridDim.x * blockDim.x) {
+        double P[Q1<DIM>::NV][DIM];
+        load_vertices<DIM>(m, c, P);
+        ftol[c] = outside_tol<DIM>(P);
+    }
+}
+
+template <int DIM>
+void launch_geo(hipStream_t s, DevMesh<DIM> m, float* geo, double* ftol) {
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_geo32<DIM>), dim3((m.n_cells + 255) / 256), dim3(256), 0, s, m, geo);
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_ftol<DIM>), dim3((m.n_cells + 255) / 256), dim3(256), 0, s, m, ftol);
+}
+
+template <int DIM>
+__device__ __forceinline__ void shape32(const float* xi, float* N) {
+#pragma unroll
+    for (int k = 0; k < Q1<DIM>::NV; ++k) {
+        float val = 1.0f;
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) val *= ((k >> d) & 1) ? xi[d] : (1.0f - xi[d]);
+        N[k] = val;
+    }
+}
+
+// G(delta) = F(xi + delta) - F(xi) and J(xi + delta), from the coefficients
+template <int DIM>
+__device__ __forceinline__ void diff_and_jac(const float* g, const float* xi, const float* dl, float* G,
+                                             float (*J)[DIM]) {
+    if constexpr (DIM == 2) {
+        const float e0 = xi[0] + dl[0], e1 = xi[1] + dl[1];
+        // e0 e1 - xi0 xi1 = dl0 xi1 + xi0 dl1 + dl0 dl1
+        const float p01 = __fmaf_rn(dl[0], xi[1], __fmaf_rn(xi[0], dl[1], dl[0] * dl[1]));
+#pragma unroll
+        for (int d = 0; d < 2; ++d) {
+            const float a = g[0 + d], b = g[2 + d], c = g[4 + d];
+            G[d] = __fmaf_rn(a, dl[0], __fmaf_rn(b, dl[1], c * p01));
+            J[d][0] = __fmaf_rn(c, e1, a);
+            J[d][1] = __fmaf_rn(c, e0, b);
+        }
+    } else {
+        const float e0 = xi[0] + dl[0], e1 = xi[1] + dl[1], e2 = xi[2] + dl[2];
+        const float p01 = __fmaf_rn(dl[0], xi[1], __fmaf_rn(xi[0], dl[1], dl[0] * dl[1]));
+        const float p02 = __fmaf_rn(dl[0], xi[2], __fmaf_rn(xi[0], dl[2], dl[0] * dl[2]));
+        const float p12 = __fmaf_rn(dl[1], xi[2], __fmaf_rn(xi[1], dl[2], dl[1] * dl[2]));
+        // e0 e1 e2 - xi0 xi1 xi2 = p01 * e2 + xi0 xi1 dl2
+        const float p012 = __fmaf_rn(p01, e2, xi[0] * xi[1] * dl[2]);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const float a = g[0 + d], b = g[3 + d], c = g[6 + d];
+            const float dd = g[9 + d], ee = g[12 + d], ff = g[15 + d], gg = g[18 + d];
+            G[d] = a * dl[0] + b * dl[1] + c * dl[2] + dd * p01 + ee * p02 + ff * p12 + gg * p012;
+            J[d][0] = a + dd * e1 + ee * e2 + gg * e1 * e2;
+            J[d][1] = b + dd * e0 + ff * e2 + gg * e0 * e2;
+            J[d][2] = c + ee * e0 + ff * e1 + gg * e0 * e1;
+        }
+    }
+}
+
+template <int DIM>
+__device__ __forceinline__ bool solve32(const float (*J)[DIM], const float* r, float* x) {
+    if constexpr (DIM == 2) {
+        const float det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+        if (!(det > 0.0f)) return false;
+        const float inv = 1.0f / det;
+        x[0] = (J[1][1] * r[0] - J[0][1] * r[1]) * inv;
+        x[1] = (J[0][0] * r[1] - J[1][0] * r[0]) * inv;
+        return true;
+    } else {
+        const float c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+        const float c01 = J[0][2] * J[2][1] - J[0][1] * J[2][2];
+        const float c02 = J[0][1] * J[1][2] - J[0][2] * J[1][1];
+        const float det = J[0][0] * c00 + J[1][0] * c01 + J[2][0] * c02;
+        if (!(det > 0.0f)) return false;
+        const float inv = 1.0f / det;
+        const float c10 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+        const float c11 = J[0][0] * J[2][2] - J[0][2] * J[2][0];
+        const float c12 = J[0][2] * J[1][0] - J[0][0] * J[1][2];
+        const float c20 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+        const float c21 = J[0][1] * J[2][0] - J[0][0] * J[2][1];
+        const float c22 = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+        x[0] = (c00 * r[0] + c01 * r[1] + c02 * r[2]) * inv;
+        x[1] = (c10 * r[0] + c11 * r[1] + c12 * r[2]) * inv;
+        x[2] = (c20 * r[0] + c21 * r[1] + c22 * r[2]) * inv;
+        return true;
+    }
+}
+
+// delta with F(xi + delta) - F(xi) = dx; three Newton steps from delta = 0
+template <int DIM>
+__device__ __forceinline__ bool local_step(const float* g, const float* xi, const float* dx, float* dl) {
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) dl[d] = 0.0f;
+#pragma unroll
+    for (int it = 0; it < 3; ++it) {
+        float G[DIM], J[DIM][DIM], r[DIM], s[DIM];
+        diff_and_jac<DIM>(g, xi, dl, G, J);
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) r[d] = dx[d] - G[d];
+        if (!solve32<DIM>(J, r, s)) return false;
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) dl[d] += s[d];
+    }
+    return true;
+}
+
+template <int DIM>
+__device__ __forceinline__ void load_cell32(const DevMesh<DIM>& m, const float* __restrict__ geo,
+                                            const float* __restrict__ F, int c, float* g, float (*Fc)[DIM]) {
+    constexpr int NV = Q1<DIM>::NV;
+    constexpr int NG = DIM == 2 ? 6 : 21;
+#pragma unroll
+    for (int q = 0; q < NG; ++q) g[q] = geo[(size_t)c * NG + q];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        const int dof = m.cd[(size_t)c * NV + k];
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) Fc[k][d] = F[(size_t)dof * DIM + d];
+    }
+}
+
+template <int DIM>
+__device__ __forceinline__ int part_key32(const DevParams& pp, const float* xi) {
+    double xd[DIM];
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) xd[d] = (double)xi[d];
+    return part_key<DIM>(pp, xd);
+}
+
+template <int DIM>
+__global__ void __launch_bounds__(256) k_move32(DevMesh<DIM> m, const float* __restrict__ V32,
+                                                const float* __restrict__ dV32, const float* __restrict__ geo,
+                                                int* __restrict__ pcell, float* __restrict__ pxi,
+                                                float* __restrict__ pv, double* __restrict__ pvext, long long cap,
+                                                DevParams pp, DevWork w, int do_correct, int do_move) {
+    constexpr int NV = Q1<DIM>::NV;
+    constexpr int NG = DIM == 2 ? 6 : 21;
+    const int n = w.counters[0];
+    const float h32 = (float)pp.h;
+    int lost = 0, searches = 0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int c0 = pcell[i];
+        int c = c0;
+        float xi[DIM];
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) xi[d] = pxi[(size_t)d * cap + i];
+        if (do_correct) {
+            float v[DIM], N[NV];
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) v[d] = pv[(size_t)d * cap + i];
+            shape32<DIM>(xi, N);
+#pragma unroll
+            for (int k = 0; k < NV; ++k) {
+                const int dof = m.cd[(size_t)c * NV + k];
+#pragma unroll
+                for (int d = 0; d < DIM; ++d) v[d] = __fmaf_rn(N[k], dV32[(size_t)dof * DIM + d], v[d]);
+            }
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) pv[(size_t)d * cap + i] = v[d];
+        }
+        if (!do_move) continue;
+        float g[NG], Vc[NV][DIM];
+        load_cell32<DIM>(m, geo, V32, c, g, Vc);
+        for (int s = 0; s < pp.substeps; ++s) {
+            float N[NV], u[DIM], dx[DIM], dl[DIM];
+            shape32<DIM>(xi, N);
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) u[d] = 0.0f;
+#pragma unroll
+            for (int k = 0; k < NV; ++k)
+#pragma unroll
+                for (int d = 0; d < DIM; ++d) u[d] = __fmaf_rn(N[k], Vc[k][d], u[d]);
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) dx[d] = u[d] * h32;
+            if (pp.keep_vext && s == pp.substeps - 1 && pvext)
+#pragma unroll
+                for (int d = 0; d < DIM; ++d) pvext[(size_t)d * cap + i] = (double)u[d];
+            if (local_step<DIM>(g, xi, dx, dl)) {
+                float e[DIM];
+                bool in = true;
+#pragma unroll
+                for (int d = 0; d < DIM; ++d) {
+                    e[d] = xi[d] + dl[d];
+                    in = in && (e[d] >= 0.0f) && (e[d] <= 1.0f);
+                }
+                if (in) {
+#pragma unroll
+                    for (int d = 0; d < DIM; ++d) xi[d] = e[d];
+                    continue;
+                }
+            }
+            // FP64 relocation with the reference rule
+            ++searches;
+            double P[NV][DIM], xid[DIM], x[DIM], vext[DIM], r[DIM];
+            load_vertices<DIM>(m, c, P);
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) {
+                xid[d] = (double)xi[d];
+                vext[d] = (double)u[d];
+            }
+            forward<DIM>(P, xid, x);
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) x[d] = x[d] + (double)dx[d];
+            const int cn = relocate<DIM>(m, c, P, xid, vext, x, r);
+            if (cn < 0) {
+                c = -1;
+                ++lost;
+                break;
+            }
+            // reference points are kept inside [0,1] after rounding to FP32
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) {
+                float t = (float)r[d];
+                xi[d] = t < 0.0f ? 0.0f : (t > 1.0f ? 1.0f : t);
+            }
+            if (cn != c) {
+                c = cn;
+                load_cell32<DIM>(m, geo, V32, c, g, Vc);
+            }
+        }
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) pxi[(size_t)d * cap + i] = xi[d];
+        w.dest[i] = c;
+        if (c >= 0) {
+            w.pkey[i] = part_key32<DIM>(pp, xi);
+            if (c != c0) {
+                const int slot = atomicAdd(&w.counters[1], 1);
+                w.movers[slot] = i;
+                atomicAdd(&w.arr_cnt[c], 1);
+            }
+        }
+    }
+    if (lost) atomicAdd(&w.stats[0], (unsigned long long)lost);
+    if (searches) atomicAdd(&w.stats[3], (unsigned long long)searches);
+}
+
+template <int DIM>
+void launch_move32(hipStream_t s, int grid, DevMesh<DIM> m, const float* V32, const float* dV32, const float* geo,
+                   int* pcell, float* pxi, float* pv, double* pvext, long long cap, DevParams pp, DevWork w,
+                   int do_correct, int do_move) {
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_move32<DIM>), dim3(grid), dim3(256), 0, s, m, V32, dV32, geo, pcell, pxi, pv,
+                       pvext, cap, pp, w, do_correct, do_move);
+}
+
+template <int DIM>
+__device__ __forceinline__ void cell_write32_one(const DevMesh<DIM>& m, const double* __restrict__ V,
+                                                 const int* __restrict__ old_off, const uint32_t* __restrict__ aid,
+                                                 const float* __restrict__ axi, const float* __restrict__ av,
+                                                 int* __restrict__ bcell, uint32_t* __restrict__ bid,
+                                                 float* __restrict__ bxi, float* __restrict__ bv,
+                                                 const double* __restrict__ avext, double* __restrict__ bvext,
+                                                 long long cap, const DevParams& pp, const DevWork& w, int c,
+                                                 int lane) {
+    constexpr int NV = Q1<DIM>::NV;
+    const int n_st = old_off[c + 1] - old_off[c];
+    const int k = n_st + w.arr_cnt[c];
+    const long long out0 = w.new_off[c];
+    const int nsurv = w.nsurv[c];
+    const int nfinal = w.final_cnt[c];
+    if (out0 + nfinal > cap) {
+        if (lane == 0) atomicOr(&w.counters[2], 1);
+        return;
+    }
+    for (int j = lane; j < k; j += WAVE) {
+        const int idx = candidate(w, old_off, c, n_st, j);
+        if (w.dest[idx] != c) continue;
+        const int pos = w.spos[idx];
+        if (pos < 0) continue;
+        const long long o = out0 + pos;
+        bcell[o] = c;
+        bid[o] = aid[idx];
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) {
+            bxi[(size_t)d * cap + o] = axi[(size_t)d * cap + idx];
+            bv[(size_t)d * cap + o] = av[(size_t)d * cap + idx];
+        }
+        if (pp.keep_vext)
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) bvext[(size_t)d * cap + o] = avext[(size_t)d * cap + idx];
+    }
+    const int spawns = nfinal - nsurv;
+    if (spawns > 0) {
+        const unsigned long long all = pp.nparts_in >= 64 ? ~0ull : ((1ull << pp.nparts_in) - 1);
+        const unsigned long long empty = all & ~w.hitmask[c];
+        const uint32_t id0 = *w.particle_counter + (uint32_t)w.spawn_off[c];
+        for (int o = lane; o < pp.nparts_in; o += WAVE) {
+            if (!((empty >> o) & 1)) continue;
+            const int r = __popcll(empty & ((1ull << o) - 1));
+            double xi[DIM], P[NV][DIM], Vc[NV][DIM], v[DIM];
+            lattice_point<DIM>(pp, o, xi);
+            load_cell<DIM>(m, V, c, P, Vc);
+            interp<DIM>(Vc, xi, v);
+            const long long q = out0 + nsurv + r;
+            bcell[q] = c;
+            bid[q] = id0 + (uint32_t)r + 1u;
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) {
+                bxi[(size_t)d * cap + q] = (float)xi[d];
+                bv[(size_t)d * cap + q] = (float)v[d];
+                if (pp.keep_vext) bvext[(size_t)d * cap + q] = 0.0;
+            }
+        }
+    }
+    if (lane == 0) {
+        w.arr_cnt[c] = 0;
+        w.arr_fill[c] = 0;
+    }
+}
+
+template <int DIM>
+__global__ void __launch_bounds__(256) k_cell_write32(DevMesh<DIM> m, const double* __restrict__ V,
+                                                     const int* __restrict__ old_off, const uint32_t* __restrict__ aid,
+                                                     const float* __restrict__ axi, const float* __restrict__ av,
+                                                     int* __restrict__ bcell, uint32_t* __restrict__ bid,
+                                                     float* __restrict__ bxi, float* __restrict__ bv,
+                                                     const double* __restrict__ avext, double* __restrict__ bvext,
+                                                     long long cap, DevParams pp, DevWork w) {
+    const int lane = threadIdx.x & (WAVE - 1);
+    for (int c = blockIdx.x * CELL_WAVES + threadIdx.x / WAVE; c < m.n_cells; c += gridDim.x * CELL_WAVES)
+        cell_write32_one<DIM>(m, V, old_off, aid, axi, av, bcell, bid, bxi, bv, avext, bvext, cap, pp, w, c, lane);
+}
+
+template <int DIM>
+void launch_cell_write32(hipStream_t s, int grid, DevMesh<DIM> m, const double* V, const int* old_off, const uint32_t* aid,
+                         const float* axi, const float* av, int* bcell, uint32_t* bid, float* bxi, float* bv,
+                         const double* avext, double* bvext, long long cap, DevParams pp, DevWork w) {
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cell_write32<DIM>), dim3(grid), dim3(256), 0, s, m, V, old_off, aid, axi,
+                       av, bcell, bid, bxi, bv, avext, bvext, cap, pp, w);
+}
+
+// per-cell projection sums in FP32 with Kahan compensation, ascending id order
+template <int DIM>
+__global__ void __launch_bounds__(256) k_cell_partials32(int n_cells, const int* __restrict__ off,
+                                                         const float* __restrict__ pxi, const float* __restrict__ pv,
+                                                         long long cap, double* __restrict__ partial) {
+    constexpr int NV = Q1<DIM>::NV;
+    constexpr int NACC = NV * (DIM + 1);
+    __shared__ float s_xi[4][WAVE][DIM];
+    __shared__ float s_v[4][WAVE][DIM];
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int wv = threadIdx.x / WAVE;
+    const int kk = lane / (DIM + 1);
+    const int comp = lane - kk * (DIM + 1);
+    for (int c = blockIdx.x * 4 + wv; c < n_cells; c += gridDim.x * 4) {
+    const int b = off[c], e = off[c + 1];
+    float sum = 0.0f, cmp = 0.0f;
+    for (int j0 = b; j0 < e; j0 += WAVE) {
+        const int j = j0 + lane;
+        if (j < e) {
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) {
+                s_xi[wv][lane][d] = pxi[(size_t)d * cap + j];
+                s_v[wv][lane][d] = pv[(size_t)d * cap + j];
+            }
+        }
+        wave_sync();
+        const int cnt = (e - j0) < WAVE ? (e - j0) : WAVE;
+        if (lane < NACC) {
+            for (int t = 0; t < cnt; ++t) {
+                float xi[DIM], N[NV];
+#pragma unroll
+                for (int d = 0; d < DIM; ++d) xi[d] = s_xi[wv][t][d];
+                shape32<DIM>(xi, N);
+                float nk = N[0];
+#pragma unroll
+                for (int q = 1; q < NV; ++q)
+                    if (q == kk) nk = N[q];
+                float term = nk;
+                if (comp < DIM) {
+                    float vc = s_v[wv][t][0];
+#pragma unroll
+                    for (int d = 1; d < DIM; ++d)
+                        if (d == comp) vc = s_v[wv][t][d];
+                    term = nk * vc;
+                }
+                const float y = term - cmp;
+                const float tt = sum + y;
+                cmp = (tt - sum) - y;
+                sum = tt;
+            }
+        }
+        wave_sync();
+    }
+    if (lane < NACC) partial[(size_t)c * NACC + lane] = (double)sum - (double)cmp;
+    }
+}
+
+template <int DIM>
+void launch_partials32(hipStream_t s, int grid, int n_cells, const int* off, const float* pxi, const float* pv, long long cap,
+                       double* partial) {
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cell_partials32<DIM>), dim3(grid), dim3(256), 0, s, n_cells, off,
+                       pxi, pv, cap, partial);
+}
+
+template <int DIM>
+__global__ void k_wall_particles32(DevMesh<DIM> m, DevWalls wl, const int* __restrict__ off,
+                                   const float* __restrict__ pxi, float* __restrict__ pv, long long cap) {
+    constexpr int NV = Q1<DIM>::NV;
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int t = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
+    if (t >= wl.n_cells) return;
+    const int c = wl.cells[t];
+    int order[NV], widx[NV], nw = 0;
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        const int gv = m.cv[(size_t)c * NV + k];
+        const int wi = wl.vwall[gv];
+        if (wi >= 0 && wl.fix[wi]) {
+            int p = nw;
+            while (p > 0 && m.cv[(size_t)c * NV + order[p - 1]] > gv) {
+                order[p] = order[p - 1];
+                widx[p] = widx[p - 1];
+                --p;
+            }
+            order[p] = k;
+            widx[p] = wi;
+            ++nw;
+        }
+    }
+    if (nw == 0) return;
+    for (int j = off[c] + lane; j < off[c + 1]; j += WAVE) {
+        float xi[DIM], v[DIM], N[NV];
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) {
+            xi[d] = pxi[(size_t)d * cap + j];
+            v[d] = pv[(size_t)d * cap + j];
+        }
+        shape32<DIM>(xi, N);
+        for (int q = 0; q < nw; ++q) {
+            const int wi = widx[q];
+            float nk = N[0];
+#pragma unroll
+            for (int r = 1; r < NV; ++r)
+                if (r == order[q]) nk = N[r];
+#pragma unroll
+            for (int d = 0; d < DIM; ++d)
+                if ((wl.mask[wi] >> d) & 1) v[d] = __fmaf_rn(nk, (float)(wl.bc[wi * 3 + d] - wl.val[wi * 3 + d]), v[d]);
+        }
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) pv[(size_t)d * cap + j] = v[d];
+    }
+}
+
+template <int DIM>
+void launch_wall_particles32(hipStream_t s, DevMesh<DIM> m, DevWalls wl, const int* off, const float* pxi, float* pv,
+                             long long cap) {
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_wall_particles32<DIM>), dim3((wl.n_cells + 3) / 4), dim3(256), 0, s, m, wl,
+                       off, pxi, pv, cap);
+}
+
+// V32 = V, dV32 = V - V_old (node-major, FP32)
+template <int DIM>
+__global__ void k_fields_to32(const double* __restrict__ V, const double* __restrict__ Vold, float* __restrict__ V32,
+                              float* __restrict__ dV32, int n) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n * DIM; i += gridDim.x * blockDim.x) {
+        V32[i] = (float)V[i];
+        dV32[i] = (float)(V[i] - Vold[i]);
+    }
+}
+
+template <int DIM>
+void launch_fields_to32(hipStream_t s, const double* V, const double* Vold, float* V32, float* dV32, int n) {
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_fields_to32<DIM>), dim3((n * DIM + 255) / 256), dim3(256), 0, s, V, Vold, V32,
+                       dV32, n);
+}
+
+__global__ void k_state_to32(int dim, const double* __restrict__ xi, const double* __restrict__ v,
+                             float* __restrict__ xi32, float* __restrict__ v32, long long cap,
+                             const int* __restrict__ counters) {
+    const long long n = counters[0];
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        for (int d = 0; d < dim; ++d) {
+            xi32[(size_t)d * cap + i] = (float)xi[(size_t)d * cap + i];
+            v32[(size_t)d * cap + i] = (float)v[(size_t)d * cap + i];
+        }
+}
+
+inline void launch_state_to32(hipStream_t s, int dim, const double* xi, const double* v, float* xi32, float* v32,
+                              long long cap, const int* counters, long long n_est) {
+    long long g = (n_est + 255) / 256;
+    if (g < 1) g = 1;
+    if (g > 4096) g = 4096;
+    hipLaunchKernelGGL(k_state_to32, dim3((int)g), dim3(256), 0, s, dim, xi, v, xi32, v32, cap, counters);
+}
+
+// FP64 view of the FP32 state: xi, v widened, location mapped from (cell, xi)
+template <int DIM>
+__global__ void k_state_from32(DevMesh<DIM> m, const int* __restrict__ pcell, const float* __restrict__ xi32,
+                               const float* __restrict__ v32, double* __restrict__ x, double* __restrict__ xi,
+                               double* __restrict__ v, long long cap, const int* __restrict__ counters) {
+    constexpr int NV = Q1<DIM>::NV;
+    const long long n = counters[0];
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        double xd[DIM], P[NV][DIM], xx[DIM];
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) {
+            xd[d] = (double)xi32[(size_t)d * cap + i];
+            xi[(size_t)d * cap + i] = xd[d];
+            v[(size_t)d * cap + i] = (double)v32[(size_t)d * cap + i];
+        }
+        load_vertices<DIM>(m, pcell[i], P);
+        forward<DIM>(P, xd, xx);
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) x[(size_t)d * cap + i] = xx[d];
+    }
+}
+
+template <int DIM>
+void launch_state_from32(hipStream_t s, DevMesh<DIM> m, const int* pcell, const float* xi32, const float* v32,
+                         double* x, double* xi, double* v, long long cap, const int* counters, long long n_est) {
+    long long g = (n_est + 255) / 256;
+    if (g < 1) g = 1;
+    if (g > 4096) g = 4096;
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_state_from32<DIM>), dim3((int)g), dim3(256), 0, s, m, pcell, xi32, v32, x, xi,
+                       v, cap, counters);
+}
+
+}  // namespace pf2

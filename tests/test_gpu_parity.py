@@ -1,0 +1,178 @@
+"""GPU (HIP, FP64 path) versus the CPU oracle on identical seeded inputs.
+
+Bar: cell ownership, ids and per-cell counts bit-exact; x, xi, v and the
+projected nodal field within 1e-12 relative (the kernels are written to give
+the oracle's bits; the report records whether they are bitwise equal).
+"""
+import numpy as np
+import pytest
+
+from oracle_binding import Oracle
+from parity_util import assert_particles_match, compare_particles, field_maxrel
+from pfem2_amd import FP64, ParticleStage
+from pfem2_amd import mesh as M
+
+pytestmark = pytest.mark.gpu
+
+REL = 1e-12
+
+
+def _pair(m, V, dt, S, MAX, q, jitter=0.05, seed=0, keep_vext=False, seed_lattice=False):
+    g = ParticleStage(m, dt, S, MAX, q, FP64, keep_velocity_ext=keep_vext)
+    o = Oracle(m, dt, S, MAX, q)
+    for s in (g, o):
+        s.upload_fields(V, V)
+    if seed_lattice:
+        g.seed_particles()
+        o.seed_particles()
+    else:
+        cell, xi = M.jittered_lattice(m, q, jitter, seed)
+        g.insert_particles(cell, xi)
+        o.insert_particles(cell, xi)
+    return g, o
+
+
+def _run_steps(g, o, V_of_step, steps, where):
+    reps = []
+    for s in range(steps):
+        Vg = g.step(download=True)
+        Vo = o.step()
+        pg, po = g.particles(), o.particles()
+        rep = assert_particles_match(pg, po, REL, f"{where} step {s}")
+        assert np.array_equal(g.cell_counts(), o.cell_counts()), f"{where} step {s}: cell counts"
+        fr = field_maxrel(Vg, Vo)
+        assert fr <= REL, f"{where} step {s}: projected field differs by {fr}"
+        rep["field_maxrel"] = fr
+        rep["field_bitwise"] = bool(np.array_equal(np.nan_to_num(Vg), np.nan_to_num(Vo)))
+        reps.append(rep)
+        Vn = V_of_step(s)
+        g.upload_fields(Vn, None)
+        o.upload_fields(Vn, None)
+    print(where, reps[-1], flush=True)
+    return reps
+
+
+def test_seed_lattice_bitwise():
+    m = M.channel_2d(40, 8, 2.0, 1.0)
+    V = M.poiseuille_2d(m, 1.0, 1.0, 0.01, seed=3)
+    g, o = _pair(m, V, 0.01, 10, 3, [4, 4], seed_lattice=True)
+    rep = compare_particles(g.particles(), o.particles())
+    print("seed", rep, flush=True)
+    assert rep["same_count"] and rep["cell_equal"] and rep["id_equal"]
+    assert rep["x_bitwise"] and rep["xi_bitwise"] and rep["v_bitwise"]
+    assert g.particle_counter() == o.particle_counter() == m.n_cells * 16
+
+
+def test_channel_steps_2d():
+    """Structured channel (rectangles): closed-form inverse, 10 substeps."""
+    m = M.channel_2d(100, 10, 5.0, 1.0)
+    V = M.poiseuille_2d(m, 1.0, 1.0, 0.01, seed=4)
+    g, o = _pair(m, V, 0.01, 10, 3, [4, 4], seed=5)
+    V2 = M.poiseuille_2d(m, 1.0, 1.0, 0.02, seed=6)
+    _run_steps(g, o, lambda s: V2 if s % 2 == 0 else V, 6, "channel")
+    assert g.particle_counter() == o.particle_counter()
+    sg, so = g.stats(), o.stats()
+    assert sg["lost"] == so["lost"] and sg["spawned"] == so["spawned"] and sg["deleted"] == so["deleted"]
+
+
+def test_cylinder_steps_2d():
+    """Unstructured-style quads around a cylinder: general bilinear cells."""
+    m = M.cylinder_channel(0.15, seed=7)
+    V = M.cylinder_flow(m, 1.0, eps=0.02, seed=8)
+    g, o = _pair(m, V, 0.004, 10, 3, [4, 4], seed=9)
+    _run_steps(g, o, lambda s: V, 5, "cylinder")
+
+
+def test_high_cfl_relocation_and_losses():
+    """Large time step: particles cross several cells per step, many leave
+    through the outlet and some are not found by the vertex-star rule."""
+    m = M.quad_2d(12, 6, (0.0, 0.0), (3.0, 1.5), jitter=0.2, seed=10)
+    V = M.uniform_field(m, [1.0, 0.3]) + 0.2 * M.fourier_perturbation(m.vertices, [3.0, 1.5], seed=11)
+    g, o = _pair(m, V, 0.25, 5, 2, [3, 3], seed=12)
+    _run_steps(g, o, lambda s: V, 6, "high-cfl")
+    assert o.stats()["lost"] > 0
+
+
+def test_box_steps_3d():
+    m = M.box_3d(6, 5, 4, (0, 0, 0), (1.2, 1.0, 0.8), jitter=0.15, seed=13)
+    V = M.uniform_field(m, [0.5, 0.2, -0.1]) + 0.3 * M.fourier_perturbation(m.vertices, [1.2, 1.0, 0.8], seed=14)
+    g, o = _pair(m, V, 0.05, 4, 2, [2, 2, 2], seed=15)
+    _run_steps(g, o, lambda s: V, 4, "box3d")
+
+
+def test_pipe_steps_3d():
+    m = M.pipe_3d(10, 10, 12, 0.5, 1.0)
+    V = M.pipe_flow(m, 1.0, 0.5, eps=0.05, seed=16)
+    g, o = _pair(m, V, 0.02, 5, 3, [2, 2, 2], seed=17)
+    _run_steps(g, o, lambda s: V, 3, "pipe3d")
+
+
+def test_separate_calls_equal_fused_step():
+    m = M.cylinder_channel(0.08, seed=18)
+    V = M.cylinder_flow(m, 1.0, eps=0.02, seed=19)
+    g1, o = _pair(m, V, 0.004, 5, 3, [3, 3], seed=20)
+    g2, _ = _pair(m, V, 0.004, 5, 3, [3, 3], seed=20)
+    for _ in range(3):
+        a = g1.step(download=True)
+        g2.correct_particles_velocities()
+        g2.move_particles()
+        b = g2.distribute_particle_velocities_to_grid()
+        assert np.array_equal(a, b)
+        c = o.step()
+        assert field_maxrel(a, c) <= REL
+    assert_particles_match(g1.particles(), g2.particles(), 0.0, "fused vs separate")
+
+
+def test_run_twice_bitwise():
+    m = M.cylinder_channel(0.1, seed=21)
+    V = M.cylinder_flow(m, 1.0, eps=0.02, seed=22)
+    outs = []
+    for _ in range(2):
+        g = ParticleStage(m, 0.004, 10, 3, [4, 4], FP64)
+        g.upload_fields(V, V)
+        cell, xi = M.jittered_lattice(m, [4, 4], 0.05, 23)
+        g.insert_particles(cell, xi)
+        for _s in range(4):
+            Vp = g.step(download=True)
+        outs.append((g.particles(), Vp))
+        g.close()
+    assert_particles_match(outs[0][0], outs[1][0], 0.0, "run twice")
+    assert np.array_equal(outs[0][1], outs[1][1])
+
+
+def test_walls_fixup():
+    m = M.channel_2d(40, 8, 4.0, 1.0)
+    V = M.poiseuille_2d(m, 1.0, 1.0, 0.05, seed=24)
+    g, o = _pair(m, V, 0.01, 5, 3, [3, 3], seed=25)
+    wall = np.concatenate([m.boundary_vertices["bottom"], m.boundary_vertices["top"]])
+    wall = np.unique(wall)
+    bc = np.zeros((len(wall), 2))
+    mask = np.full(len(wall), 2, dtype=np.int32)   # normal component only
+    body = np.arange(0, len(wall), 7)
+    mask[body] = 3                                   # both components (body-style nodes)
+    bc[body, 1] = 0.01
+    for s in (g, o):
+        s.set_walls(wall, wall, bc, mask)
+    _run_steps(g, o, lambda s: V, 4, "walls")
+
+
+def test_moving_mesh_resort():
+    """cylinderDyM order: step, then move vertices, re-sort without advection,
+    reseed empty parts (cylinderDyM/cylinderDyM.cc:1147-1160)."""
+    m = M.cylinder_channel(0.08, seed=26)
+    V = M.cylinder_flow(m, 1.0, eps=0.02, seed=27)
+    g, o = _pair(m, V, 0.004, 5, 3, [3, 3], seed=28, keep_vext=True)
+    X0 = m.vertices.copy()
+    for s in range(3):
+        g.step()
+        o.step()
+        d = np.hypot(X0[:, 0] - 0.2, X0[:, 1] - 0.2)
+        w = np.clip(1.0 - (d - 0.05) / 0.15, 0.0, 1.0)        # displacement decays away from the cylinder
+        X = X0.copy()
+        X[:, 1] += 0.004 * np.sin(0.7 * (s + 1)) * w
+        for st in (g, o):
+            st.update_vertices(X, recompute_maps=(s == 1))
+            st.sort_particles_into_subdomains_and_cells()
+            st.check_empty_cells()
+        assert_particles_match(g.particles(), o.particles(), REL, f"moving mesh {s}")
+        assert np.array_equal(g.cell_counts(), o.cell_counts())
