@@ -8,11 +8,12 @@
 //   particles structure of arrays, component-major (x[d*cap + i]), sorted by
 //            cell, ascending id inside a cell; cell_off[n_cells+1] is the CSR.
 // Per time step (pf2_step):
-//   k_move         one thread per particle: correction, then all substeps of
-//                  advection + relocation in registers; writes in place,
-//                  records the destination cell and the part key, appends
-//                  particles that changed cell to the movers list.
-//   k_group        movers grouped by destination cell (arrival lists).
+//   k_move         one thread per particle: correction, then substeps of
+//                  advection in registers until done or the first failed
+//                  own-cell test; such particles are parked in a queue.
+//   k_search_resume  MOVE_ROUNDS + 1 rounds over the queue: vertex-star
+//                  search in full wavefronts, then the remaining substeps.
+//   k_group        particles that changed cell, grouped by destination cell.
 //   k_cell_count   one wavefront per cell: part counts over stayers +
 //                  arrivals, survivors (MAX smallest ids per part), empty
 //                  parts to reseed, final count.
@@ -22,8 +23,9 @@
 //                  ascending id order.
 //   k_node         one thread per DoF: sum of the cell partials in ascending
 //                  cell order, then division by the weight.
-// No floating-point atomics anywhere; integer atomics only build lists whose
-// order is erased again by the id ordering.
+// No floating-point atomics anywhere.  Integer atomics only build lists whose
+// order is erased again by the id ordering; counters that many wavefronts
+// update are sharded over separate cache lines.
 #pragma once
 
 #include "pf2_geom.hpp"
@@ -32,6 +34,34 @@
 namespace pf2 {
 
 constexpr int WAVE = 64;
+
+// sum over the wavefront (all lanes must call)
+__device__ __forceinline__ int wave_sum(int v) {
+#pragma unroll
+    for (int off = WAVE / 2; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
+
+// Counters that many wavefronts update are spread over STAT_SHARDS cache
+// lines: atomics on one address serialise in the memory system (about 11 ns
+// each on MI355X), so a single counter updated once per wavefront would
+// cost hundreds of microseconds per launch.
+constexpr int STAT_SHARDS = 256;
+constexpr int NSTAT = 8;   // u64 per shard row (64 bytes)
+
+__device__ __forceinline__ int wave_shard() {
+    return (int)((blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE) & (STAT_SHARDS - 1));
+}
+
+__device__ __forceinline__ void stat_add(unsigned long long* stats, int k, unsigned long long v) {
+    atomicAdd(&stats[wave_shard() * NSTAT + k], v);
+}
+
+// flush per-lane statistics with one sharded atomic per wavefront (all lanes must call)
+__device__ __forceinline__ void wave_stat(unsigned long long* stats, int k, int v) {
+    const int t = wave_sum(v);
+    if ((threadIdx.x & (WAVE - 1)) == 0 && t) stat_add(stats, k, (unsigned long long)t);
+}
 
 template <int DIM>
 struct DevMesh {
@@ -70,7 +100,6 @@ struct DevWork {
     int* dest;          // per particle: new cell or -1
     int* pkey;          // per particle: part key of the final position
     int* spos;          // per particle: position among the cell's survivors, -1 = dropped
-    int* movers;        // particle indices that changed cell
     int* arr_cnt;       // per cell: arrivals
     int* arr_off;       // per cell: exclusive scan of arr_cnt
     int* arr_fill;      // per cell: fill counter
@@ -82,9 +111,9 @@ struct DevWork {
     unsigned long long* hitmask;   // per cell: in-range parts that hold a particle
     int* nsurv;         // per cell: survivors
     double* partial;    // per cell: NV * (DIM + 1) projection sums
-    int* counters;      // [0] n_particles, [1] n_movers, [2] overflow flag
+    int* counters;      // [0] n_particles, [2] overflow flags
     uint32_t* particle_counter;
-    unsigned long long* stats;   // lost, deleted, spawned, searches, zero-weight
+    unsigned long long* stats;   // STAT_SHARDS rows: lost, deleted, spawned, searches, zero-weight
 };
 
 template <int DIM>
@@ -242,33 +271,181 @@ __device__ int relocate(const DevMesh<DIM>& m, int c, const double (*P)[DIM], co
 }
 
 // ---------------------------------------------------------------------------
-// k_move: [correction] + substeps of advection and relocation, per particle.
-//   correct_particles_velocities  TurekBenchmark/pfem2particle.cpp:792-811
-//   advection                     TurekBenchmark/pfem2particle.cpp:822-846
-//   own-cell test + relocation    TurekBenchmark/pfem2particle.cpp:286-450
+// Advection substeps (move_particles, TurekBenchmark/pfem2particle.cpp:813-856).
 // Particles are independent within move_particles, so the substeps of one
-// particle run back to back in registers.  A particle whose own-cell test
-// fails waits (its position already advanced) until at least SEARCH_BATCH
-// lanes of its wavefront wait, or no lane can advance; then the waiting lanes
-// run the vertex-star search together, so the expensive divergent branch is
-// taken a few times per wavefront instead of once per substep.
-constexpr int SEARCH_BATCH = 8;
+// particle run back to back in registers: u = sum N_k V_k at the current
+// reference point, velocity_ext = u, x += (dt/S) u, then the own-cell test
+// (pfem2ParticleHandler::sort_particles_into_subdomains_and_cells,
+// TurekBenchmark/pfem2particle.cpp:286-302).  A particle whose own-cell test
+// fails is parked (position advanced, old reference point, velocity_ext and
+// substep index stored) and queued; the vertex-star searches of all queued
+// particles then run in full wavefronts (k_search_resume), which continue
+// the remaining substeps and queue again if needed.  After MOVE_ROUNDS
+// rounds the last kernel finishes every remaining particle inline.
+constexpr int MOVE_ROUNDS = 4;
 
+// Queues of parked particles, one per round, ping-ponging between two
+// buffers.  A queue is QSHARDS shards of shard_cap entries plus an overflow
+// region of cap entries; each shard's counter sits on its own 128-byte line
+// and a wavefront appends to shard (wavefront id mod QSHARDS) with one atomic.
+constexpr int QSHARDS = 256;
+constexpr int QSTRIDE = 32;   // ints between shard counters
+
+struct DevQueues {
+    int* data[2];         // ping-pong buffers: QSHARDS * shard_cap + overflow
+    int* count;           // (MOVE_ROUNDS + 2) rounds x (QSHARDS + 1) counters, QSTRIDE apart
+    int shard_cap;
+    double* park_vext;    // DIM x cap: velocity_ext of a parked particle
+    int* park_s;          // substep index of a parked particle
+};
+
+__device__ __forceinline__ int* queue_count(const DevQueues& qu, int round, int shard) {
+    return qu.count + ((size_t)round * (QSHARDS + 1) + shard) * QSTRIDE;
+}
+
+// append `value` for the lanes with pred set (all lanes of the wavefront call)
+__device__ __forceinline__ void queue_push(const DevQueues& qu, int round, bool pred, int value) {
+    const unsigned long long mask = __ballot(pred);
+    if (mask == 0) return;
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int leader = __ffsll((long long)mask) - 1;
+    const int sh = wave_shard() & (QSHARDS - 1);
+    int base = 0;
+    if (lane == leader) base = atomicAdd(queue_count(qu, round, sh), __popcll(mask));
+    base = __shfl(base, leader);
+    const int slot = base + __popcll(mask & ((1ull << lane) - 1));
+    const bool spill = pred && slot >= qu.shard_cap;
+    int* data = qu.data[round & 1];
+    if (pred && !spill) data[(size_t)sh * qu.shard_cap + slot] = value;
+    const unsigned long long smask = __ballot(spill);
+    if (smask == 0) return;
+    const int l2 = __ffsll((long long)smask) - 1;
+    int ob = 0;
+    if (lane == l2) ob = atomicAdd(queue_count(qu, round, QSHARDS), __popcll(smask));
+    ob = __shfl(ob, l2);
+    if (spill) data[(size_t)QSHARDS * qu.shard_cap + ob + __popcll(smask & ((1ull << lane) - 1))] = value;
+}
+
+// block-wide exclusive prefix of the valid entries per shard (+ overflow);
+// s_pref must hold QSHARDS + 2 ints; returns the total
+__device__ __forceinline__ int queue_prefix(const DevQueues& qu, int round, int* s_pref) {
+    __shared__ int s_wave[8];
+    const int t = threadIdx.x;   // blockDim.x == QSHARDS
+    const int lane = t & (WAVE - 1), wv = t / WAVE;
+    int v = *queue_count(qu, round, t);
+    v = v < qu.shard_cap ? v : qu.shard_cap;
+    int incl = v;
+#pragma unroll
+    for (int off = 1; off < WAVE; off <<= 1) {
+        const int o = __shfl_up(incl, off);
+        if (lane >= off) incl += o;
+    }
+    if (lane == WAVE - 1) s_wave[wv] = incl;
+    __syncthreads();
+    int wbase = 0;
+    for (int q = 0; q < wv; ++q) wbase += s_wave[q];
+    s_pref[t] = wbase + incl - v;
+    if (t == QSHARDS - 1) {
+        const int tot = wbase + incl;
+        s_pref[QSHARDS] = tot;
+        s_pref[QSHARDS + 1] = tot + *queue_count(qu, round, QSHARDS);
+    }
+    __syncthreads();
+    return s_pref[QSHARDS + 1];
+}
+
+// particle index of queue entry j
+__device__ __forceinline__ int queue_get(const DevQueues& qu, int round, const int* s_pref, int j) {
+    const int* data = qu.data[round & 1];
+    if (j >= s_pref[QSHARDS]) return data[(size_t)QSHARDS * qu.shard_cap + (j - s_pref[QSHARDS])];
+    int lo = 0, hi = QSHARDS - 1;   // last shard with s_pref <= j
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (s_pref[mid] <= j) lo = mid;
+        else hi = mid - 1;
+    }
+    return data[(size_t)lo * qu.shard_cap + (j - s_pref[lo])];
+}
+
+// runs substeps until done (returns 0) or the own-cell test fails (returns 1)
+template <int DIM>
+__device__ __forceinline__ int advance64(const DevMesh<DIM>& m, const double* __restrict__ V, const DevParams& pp,
+                                         const DevParticles& A, int i, int c, double* x, double* xi, double* vext,
+                                         int& s, const double (*P)[DIM], const double (*Vc)[DIM], double tol) {
+    for (; s < pp.substeps; ++s) {
+        double u[DIM], r[DIM];
+        interp<DIM>(Vc, xi, u);
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) vext[d] = u[d];
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) u[d] *= pp.h;
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) x[d] = x[d] + u[d];
+        if (pp.keep_vext && s == pp.substeps - 1)
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) A.vext[(size_t)d * A.cap + i] = vext[d];
+        if (inverse_map<DIM>(P, x, r, tol, false) && inside_unit<DIM>(r)) {
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) xi[d] = r[d];
+        } else {
+            return 1;
+        }
+    }
+    return 0;
+}
+
+// final bookkeeping of a particle whose substeps are complete
+template <int DIM>
+__device__ __forceinline__ void finish_particle(const DevParams& pp, const DevParticles& A, const DevWork& w,
+                                                bool valid, int i, int c, const double* x, const double* xi,
+                                                bool write_state) {
+    if (!valid) return;
+    if (write_state) {
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) {
+            A.x[(size_t)d * A.cap + i] = x[d];
+            A.xi[(size_t)d * A.cap + i] = xi[d];
+        }
+    }
+    w.dest[i] = c;
+    if (c >= 0) {
+        w.pkey[i] = part_key<DIM>(pp, xi);
+        if (c != A.cell[i]) atomicAdd(&w.arr_cnt[c], 1);
+    }
+}
+
+// park a particle for the search of the next round (all lanes call)
+template <int DIM>
+__device__ __forceinline__ void park_particle(const DevParticles& A, const DevWork& w, const DevQueues& qu,
+                                              int round_out, bool park, int i, int c, const double* x,
+                                              const double* xi, const double* vext, int s) {
+    queue_push(qu, round_out, park, i);
+    if (!park) return;
+    w.dest[i] = c;
+    qu.park_s[i] = s;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+        A.x[(size_t)d * A.cap + i] = x[d];
+        A.xi[(size_t)d * A.cap + i] = xi[d];
+        qu.park_vext[(size_t)d * A.cap + i] = vext[d];
+    }
+}
+
+// k_move: [correction] + substeps until done or the first failed own-cell test.
+//   correct_particles_velocities  TurekBenchmark/pfem2particle.cpp:792-811
 template <int DIM>
 __global__ void __launch_bounds__(256) k_move(DevMesh<DIM> m, const double* __restrict__ V,
                                               const double* __restrict__ Vold, DevParticles A, DevParams pp,
-                                              DevWork w, int do_correct, int do_move) {
+                                              DevWork w, DevQueues qu, int do_correct, int do_move) {
     constexpr int NV = Q1<DIM>::NV;
     const int n = w.counters[0];
     const int lane = threadIdx.x & (WAVE - 1);
     const int stride = gridDim.x * blockDim.x;
-    int lost = 0, searches = 0;
     for (int base = blockIdx.x * blockDim.x + (threadIdx.x - lane); base < n; base += stride) {
         const int i = base + lane;
         const bool valid = i < n;
-        const int c0 = valid ? A.cell[i] : 0;
-        int c = c0;
-        double x[DIM], xi[DIM];
+        const int c = valid ? A.cell[i] : 0;
+        double x[DIM], xi[DIM], vext[DIM];
 #pragma unroll
         for (int d = 0; d < DIM; ++d) xi[d] = valid ? A.xi[(size_t)d * A.cap + i] : 0.5;
         if (do_correct && valid) {
@@ -287,86 +464,82 @@ __global__ void __launch_bounds__(256) k_move(DevMesh<DIM> m, const double* __re
             for (int d = 0; d < DIM; ++d) A.v[(size_t)d * A.cap + i] = v[d];
         }
         if (!do_move) continue;
+        int st = 0, s = 0;
+        if (valid) {
 #pragma unroll
-        for (int d = 0; d < DIM; ++d) x[d] = valid ? A.x[(size_t)d * A.cap + i] : 0.0;
-        double P[NV][DIM], Vc[NV][DIM], vext[DIM], tol = -1.0;
-        load_cell<DIM>(m, V, c, P, Vc);
-        tol = m.ftol[c];
-        int s = valid ? 0 : pp.substeps;
-        bool need = false;
-        for (;;) {
-            if (s < pp.substeps && !need) {
-                double u[DIM], r[DIM];
-                interp<DIM>(Vc, xi, u);
+            for (int d = 0; d < DIM; ++d) x[d] = A.x[(size_t)d * A.cap + i];
+            double P[NV][DIM], Vc[NV][DIM];
+            load_cell<DIM>(m, V, c, P, Vc);
+            st = advance64<DIM>(m, V, pp, A, i, c, x, xi, vext, s, P, Vc, m.ftol[c]);
+        }
+        park_particle<DIM>(A, w, qu, 0, valid && st == 1, i, c, x, xi, vext, s);
+        finish_particle<DIM>(pp, A, w, valid && st == 0, i, c, x, xi, true);
+    }
+}
+
+// k_search_resume: vertex-star search of the particles queued in round
+// `round`, then the remaining substeps; a particle that needs another search
+// is queued for round + 1, or, in the last round, searched inline.
+template <int DIM>
+__global__ void __launch_bounds__(256) k_search_resume(DevMesh<DIM> m, const double* __restrict__ V,
+                                                       DevParticles A, DevParams pp, DevWork w, DevQueues qu,
+                                                       int round) {
+    constexpr int NV = Q1<DIM>::NV;
+    __shared__ int s_pref[QSHARDS + 2];
+    const int n = queue_prefix(qu, round, s_pref);
+    const bool last = round == MOVE_ROUNDS;
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int stride = gridDim.x * blockDim.x;
+    int lost = 0, searches = 0;
+    for (int base = blockIdx.x * blockDim.x + (threadIdx.x - lane); base < n; base += stride) {
+        const int j = base + lane;
+        const bool valid = j < n;
+        const int i = valid ? queue_get(qu, round, s_pref, j) : -1;
+        int c = valid ? w.dest[i] : -1;
+        int s = valid ? qu.park_s[i] : 0;
+        int st = 0;
+        double x[DIM], xi[DIM], vext[DIM];
+        if (valid) {
 #pragma unroll
-                for (int d = 0; d < DIM; ++d) vext[d] = u[d];
-#pragma unroll
-                for (int d = 0; d < DIM; ++d) u[d] *= pp.h;
-#pragma unroll
-                for (int d = 0; d < DIM; ++d) x[d] = x[d] + u[d];
-                if (pp.keep_vext && s == pp.substeps - 1)
-#pragma unroll
-                    for (int d = 0; d < DIM; ++d) A.vext[(size_t)d * A.cap + i] = vext[d];
-                if (inverse_map<DIM>(P, x, r, tol, false) && inside_unit<DIM>(r)) {
-#pragma unroll
-                    for (int d = 0; d < DIM; ++d) xi[d] = r[d];
-                    ++s;
-                } else {
-                    need = true;
-                }
+            for (int d = 0; d < DIM; ++d) {
+                x[d] = A.x[(size_t)d * A.cap + i];
+                xi[d] = A.xi[(size_t)d * A.cap + i];
+                vext[d] = qu.park_vext[(size_t)d * A.cap + i];
             }
-            const unsigned long long nm = __ballot(need);
-            const unsigned long long rm = __ballot(s < pp.substeps && !need);
-            if (nm == 0) {
-                if (rm == 0) break;
-                continue;
-            }
-            if (rm != 0 && __popcll(nm) < SEARCH_BATCH) continue;
-            if (need) {
+            for (;;) {
+                double P[NV][DIM], r[DIM];
+                load_vertices<DIM>(m, c, P);
                 ++searches;
-                double r[DIM];
                 const int cn = relocate<DIM>(m, c, P, xi, vext, x, r);
                 if (cn < 0) {
                     c = -1;
                     ++lost;
-                    s = pp.substeps;
-                } else {
-                    c = cn;
-#pragma unroll
-                    for (int d = 0; d < DIM; ++d) xi[d] = r[d];
-                    load_cell<DIM>(m, V, c, P, Vc);
-                    tol = m.ftol[c];
-                    ++s;
+                    st = 0;
+                    break;
                 }
-                need = false;
-            }
-        }
-        if (!valid) continue;
+                c = cn;
 #pragma unroll
-        for (int d = 0; d < DIM; ++d) {
-            A.x[(size_t)d * A.cap + i] = x[d];
-            A.xi[(size_t)d * A.cap + i] = xi[d];
-        }
-        w.dest[i] = c;
-        if (c >= 0) {
-            w.pkey[i] = part_key<DIM>(pp, xi);
-            if (c != c0) {
-                const int slot = atomicAdd(&w.counters[1], 1);
-                w.movers[slot] = i;
-                atomicAdd(&w.arr_cnt[c], 1);
+                for (int d = 0; d < DIM; ++d) xi[d] = r[d];
+                ++s;
+                double Vc[NV][DIM];
+                load_cell<DIM>(m, V, c, P, Vc);
+                st = advance64<DIM>(m, V, pp, A, i, c, x, xi, vext, s, P, Vc, m.ftol[c]);
+                if (st == 0 || !last) break;
             }
         }
+        if (!last) park_particle<DIM>(A, w, qu, round + 1, valid && st == 1, i, c, x, xi, vext, s);
+        finish_particle<DIM>(pp, A, w, valid && st == 0, i, c, x, xi, true);
     }
-    if (lost) atomicAdd(&w.stats[0], (unsigned long long)lost);
-    if (searches) atomicAdd(&w.stats[3], (unsigned long long)searches);
+    wave_stat(w.stats, 0, lost);
+    wave_stat(w.stats, 3, searches);
 }
 
-// movers grouped by destination cell
-__global__ void k_group(DevWork w) {
-    const int n = w.counters[1];
-    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
-        const int i = w.movers[j];
+// particles that changed cell, grouped by destination cell
+__global__ void k_group(const int* __restrict__ pcell, DevWork w) {
+    const int n = w.counters[0];
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const int c = w.dest[i];
+        if (c < 0 || c == pcell[i]) continue;
         const int slot = atomicAdd(&w.arr_fill[c], 1);
         w.arr_list[w.arr_off[c] + slot] = i;
     }
@@ -410,6 +583,7 @@ __global__ void __launch_bounds__(256) k_cell_count(int n_cells, const int* __re
     const int lane = threadIdx.x & (WAVE - 1);
     const int wv = threadIdx.x / WAVE;
     const unsigned long long all = pp.nparts_in >= 64 ? ~0ull : ((1ull << pp.nparts_in) - 1);
+    int deleted = 0;
     for (int c = blockIdx.x * CELL_WAVES + wv; c < n_cells; c += gridDim.x * CELL_WAVES) {
         const int n_st = old_off[c + 1] - old_off[c];
         const int k = n_st + w.arr_cnt[c];
@@ -502,9 +676,10 @@ __global__ void __launch_bounds__(256) k_cell_count(int n_cells, const int* __re
             w.nsurv[c] = nsurv;
             w.final_cnt[c] = nsurv + spawns;
             w.spawn_cnt[c] = spawns;
-            if (nvalid > nsurv) atomicAdd(&w.stats[1], (unsigned long long)(nvalid - nsurv));
+            deleted += nvalid - nsurv;
         }
     }
+    if (lane == 0 && deleted) stat_add(w.stats, 1, (unsigned long long)deleted);
 }
 
 // ---------------------------------------------------------------------------
@@ -587,15 +762,20 @@ __global__ void __launch_bounds__(256) k_cell_write(DevMesh<DIM> m, const double
         cell_write_one<DIM>(m, V, old_off, A, B, pp, w, c, lane);
 }
 
-// totals after the scans: particle count, id counter, statistics
-__global__ void k_finalize(int n_cells, DevWork w) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// totals after the scans: particle count, id counter, statistics; resets the
+// queue counters for the next step (launched with QSHARDS threads)
+__global__ void k_finalize(int n_cells, DevWork w, int* queue_count) {
+    const int t = threadIdx.x;
+    for (int r = 0; r <= MOVE_ROUNDS + 1; ++r) {
+        queue_count[((size_t)r * (QSHARDS + 1) + t) * QSTRIDE] = 0;
+        if (t == 0) queue_count[((size_t)r * (QSHARDS + 1) + QSHARDS) * QSTRIDE] = 0;
+    }
+    if (t != 0) return;
     const int total = w.new_off[n_cells];
     const int spawned = w.spawn_off[n_cells];
     w.stats[2] += (unsigned long long)spawned;
     *w.particle_counter += (uint32_t)spawned;
     w.counters[0] = total;
-    w.counters[1] = 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -676,7 +856,7 @@ __global__ void k_node(DevMesh<DIM> m, const double* __restrict__ partial, doubl
             for (int d = 0; d < DIM; ++d) acc[d] += p[d];
             wsum += p[DIM];
         }
-        if (wsum == 0.0) atomicAdd(&stats[4], 1ull);
+        if (wsum == 0.0) stat_add(stats, 4, 1ull);
 #pragma unroll
         for (int d = 0; d < DIM; ++d) {
             const double val = acc[d] / wsum;
@@ -843,11 +1023,15 @@ template <int DIM>
 __global__ void __launch_bounds__(256) k_resort(DevMesh<DIM> m, DevParticles A, DevParams pp, DevWork w) {
     constexpr int NV = Q1<DIM>::NV;
     const int n = w.counters[0];
+    const int lane = threadIdx.x & (WAVE - 1);
     int lost = 0, searches = 0;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const int c0 = A.cell[i];
+    for (int base = blockIdx.x * blockDim.x + (threadIdx.x - lane); base < n; base += gridDim.x * blockDim.x) {
+        const int i = base + lane;
+        const bool valid = i < n;
+        const int c0 = valid ? A.cell[i] : -1;
         int c = c0;
         double x[DIM], xi[DIM], vext[DIM], r[DIM], P[NV][DIM];
+        if (valid) {
 #pragma unroll
         for (int d = 0; d < DIM; ++d) {
             xi[d] = A.xi[(size_t)d * A.cap + i];
@@ -869,17 +1053,12 @@ __global__ void __launch_bounds__(256) k_resort(DevMesh<DIM> m, DevParticles A, 
 #pragma unroll
         for (int d = 0; d < DIM; ++d) A.xi[(size_t)d * A.cap + i] = xi[d];
         w.dest[i] = c;
-        if (c >= 0) {
-            w.pkey[i] = part_key<DIM>(pp, xi);
-            if (c != c0) {
-                const int slot = atomicAdd(&w.counters[1], 1);
-                w.movers[slot] = i;
-                atomicAdd(&w.arr_cnt[c], 1);
-            }
+        if (c >= 0) w.pkey[i] = part_key<DIM>(pp, xi);
         }
+        if (valid && c >= 0 && c != c0) atomicAdd(&w.arr_cnt[c], 1);
     }
-    if (lost) atomicAdd(&w.stats[0], (unsigned long long)lost);
-    if (searches) atomicAdd(&w.stats[3], (unsigned long long)searches);
+    wave_stat(w.stats, 0, lost);
+    wave_stat(w.stats, 3, searches);
 }
 
 // dest / key of every particle for a reseed pass without movement

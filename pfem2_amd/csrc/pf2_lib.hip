@@ -38,11 +38,12 @@ int fail(int code, const std::string& msg) {
 
 enum KernelId {
     K_MOVE = 0, K_GROUP, K_CELL_COUNT, K_CELL_WRITE, K_SCAN, K_FINALIZE, K_PARTIALS, K_NODE, K_WALL,
-    K_SEED, K_MISC, K_MOVE32, K_CELL_WRITE32, K_PARTIALS32, K_NKERNELS
+    K_SEED, K_MISC, K_MOVE32, K_CELL_WRITE32, K_PARTIALS32, K_SEARCH, K_SEARCH32, K_NKERNELS
 };
 const char* kKernelNames[K_NKERNELS] = {"k_move", "k_group", "k_cell_count", "k_cell_write", "scan",
                                         "k_finalize", "k_cell_partials", "k_node", "wall_fixup",
-                                        "k_seed", "misc", "k_move32", "k_cell_write32", "k_cell_partials32"};
+                                        "k_seed", "misc", "k_move32", "k_cell_write32", "k_cell_partials32",
+                                        "k_search_resume", "k_search_resume32"};
 
 template <typename T>
 int dalloc(T** p, size_t n) {
@@ -96,13 +97,18 @@ struct pf2_handle_s {
     int* off[2] = {nullptr, nullptr};
     int cur_off = 0;
     // work
-    int *dest = nullptr, *pkey = nullptr, *spos = nullptr, *movers = nullptr;
+    int *dest = nullptr, *pkey = nullptr, *spos = nullptr;
     int *arr_cnt = nullptr, *arr_off = nullptr, *arr_fill = nullptr, *arr_list = nullptr;
     int *final_cnt = nullptr, *spawn_cnt = nullptr, *spawn_off = nullptr;
     unsigned long long* hitmask = nullptr;
     int* nsurv = nullptr;
     double* partial = nullptr;
     int* counters = nullptr;
+    int* qcount = nullptr;
+    long long shard_cap = 0;
+    int* queue[2] = {nullptr, nullptr};
+    double* park_vext = nullptr;
+    int* park_s = nullptr;
     uint32_t* pcounter = nullptr;
     unsigned long long* stats = nullptr;
     void* scan_tmp = nullptr;
@@ -211,6 +217,17 @@ DevParticles dev_particles(pf2_handle_s* h, int which) {
     return p;
 }
 
+DevQueues dev_queues(pf2_handle_s* h) {
+    DevQueues q;
+    q.data[0] = h->queue[0];
+    q.data[1] = h->queue[1];
+    q.count = h->qcount;
+    q.shard_cap = (int)h->shard_cap;
+    q.park_vext = h->park_vext;
+    q.park_s = h->park_s;
+    return q;
+}
+
 DevParams dev_params(pf2_handle_s* h) {
     DevParams p;
     p.h = h->params.dt / h->params.substeps;
@@ -228,7 +245,6 @@ DevWork dev_work(pf2_handle_s* h) {
     w.dest = h->dest;
     w.pkey = h->pkey;
     w.spos = h->spos;
-    w.movers = h->movers;
     w.arr_cnt = h->arr_cnt;
     w.arr_off = h->arr_off;
     w.arr_fill = h->arr_fill;
@@ -306,8 +322,11 @@ int alloc_particles(pf2_handle_s* h) {
         if (h->params.keep_velocity_ext && dalloc(&pb.vext, h->cap * h->dim)) return PF2_E_HIP;
         if (mixed(h) && (dalloc(&pb.xi32, h->cap * h->dim) || dalloc(&pb.v32, h->cap * h->dim))) return PF2_E_HIP;
     }
+    h->shard_cap = 2 * ((h->cap + QSHARDS - 1) / QSHARDS) + 64;
+    const long long qlen = QSHARDS * h->shard_cap + h->cap;
     if (dalloc(&h->dest, h->cap) || dalloc(&h->pkey, h->cap) || dalloc(&h->spos, h->cap) ||
-        dalloc(&h->movers, h->cap) || dalloc(&h->arr_list, h->cap))
+        dalloc(&h->arr_list, h->cap) || dalloc(&h->queue[0], qlen) || dalloc(&h->queue[1], qlen) ||
+        dalloc(&h->park_s, h->cap) || dalloc(&h->park_vext, h->cap * h->dim))
         return PF2_E_HIP;
     return 0;
 }
@@ -319,8 +338,10 @@ void free_particles(pf2_handle_s* h) {
         hipFree(pb.xi32); hipFree(pb.v32);
         pb = PBufs();
     }
-    hipFree(h->dest); hipFree(h->pkey); hipFree(h->spos); hipFree(h->movers); hipFree(h->arr_list);
-    h->dest = h->pkey = h->spos = h->movers = h->arr_list = nullptr;
+    hipFree(h->dest); hipFree(h->pkey); hipFree(h->spos); hipFree(h->arr_list);
+    hipFree(h->queue[0]); hipFree(h->queue[1]); hipFree(h->park_s); hipFree(h->park_vext);
+    h->dest = h->pkey = h->spos = h->arr_list = h->queue[0] = h->queue[1] = h->park_s = nullptr;
+    h->park_vext = nullptr;
 }
 
 int scan(pf2_handle_s* h, const int* in, int* out, int n) {
@@ -352,7 +373,8 @@ int regroup(pf2_handle_s* h, int reseed) {
     if (int r = scan(h, h->arr_cnt, h->arr_off, nc)) return r;
     {
         Timed t(h, K_GROUP);
-        hipLaunchKernelGGL(k_group, dim3(grid_for(h->n_est / 8 + 1, 256, 4096)), dim3(256), 0, h->stream, w);
+        hipLaunchKernelGGL(k_group, dim3(grid_for(h->n_est, 256, 4096)), dim3(256), 0, h->stream,
+                           h->buf[h->cur].cell, w);
     }
     {
         Timed t(h, K_CELL_COUNT);
@@ -375,7 +397,7 @@ int regroup(pf2_handle_s* h, int reseed) {
     }
     {
         Timed t(h, K_FINALIZE);
-        hipLaunchKernelGGL(k_finalize, dim3(1), dim3(64), 0, h->stream, nc, w);
+        hipLaunchKernelGGL(k_finalize, dim3(1), dim3(QSHARDS), 0, h->stream, nc, w, h->qcount);
     }
     HIPC(hipGetLastError());
     h->cur = 1 - h->cur;
@@ -387,17 +409,35 @@ template <int DIM>
 int run_move(pf2_handle_s* h, int do_correct, int do_move) {
     DevParams pp = dev_params(h);
     DevWork w = dev_work(h);
+    DevQueues qu = dev_queues(h);
     const int grid = grid_for(h->n_est, 256, 256 * 16);
+    const int qgrid = grid_for(h->n_est / 2 + 1, 256, 256 * 8);
     if (mixed(h)) {
         h->fp64_valid = false;
-        Timed t(h, K_MOVE32);
-        launch_move32<DIM>(h->stream, grid, dev_mesh<DIM>(h), h->dV32, h->dVold32, h->dgeo32, h->buf[h->cur].cell,
-                           h->buf[h->cur].xi32, h->buf[h->cur].v32, h->buf[h->cur].vext, h->cap, pp, w, do_correct,
-                           do_move);
+        PBufs& b = h->buf[h->cur];
+        {
+            Timed t(h, K_MOVE32);
+            launch_move32<DIM>(h->stream, grid, dev_mesh<DIM>(h), h->dV32, h->dVold32, h->dgeo32, b.cell, b.xi32,
+                               b.v32, b.vext, h->cap, pp, w, qu, do_correct, do_move);
+        }
+        if (do_move)
+            for (int r = 0; r <= MOVE_ROUNDS; ++r) {
+                Timed t(h, K_SEARCH32);
+                launch_search_resume32<DIM>(h->stream, qgrid, dev_mesh<DIM>(h), h->dV32, h->dgeo32, b.cell, b.xi32,
+                                            b.vext, h->cap, pp, w, qu, r);
+            }
     } else {
-        Timed t(h, K_MOVE);
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_move<DIM>), dim3(grid), dim3(256), 0, h->stream, dev_mesh<DIM>(h), h->dV,
-                           h->dVold, dev_particles(h, h->cur), pp, w, do_correct, do_move);
+        {
+            Timed t(h, K_MOVE);
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_move<DIM>), dim3(grid), dim3(256), 0, h->stream, dev_mesh<DIM>(h),
+                               h->dV, h->dVold, dev_particles(h, h->cur), pp, w, qu, do_correct, do_move);
+        }
+        if (do_move)
+            for (int r = 0; r <= MOVE_ROUNDS; ++r) {
+                Timed t(h, K_SEARCH);
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_search_resume<DIM>), dim3(qgrid), dim3(256), 0, h->stream,
+                                   dev_mesh<DIM>(h), h->dV, dev_particles(h, h->cur), pp, w, qu, r);
+            }
     }
     HIPC(hipGetLastError());
     if (!do_move) return 0;
@@ -591,7 +631,8 @@ int pf2_create(const pf2_mesh* mesh, const pf2_params* params, long long capacit
         dalloc(&h->arr_fill, nc + 1) || dalloc(&h->final_cnt, nc + 1) || dalloc(&h->spawn_cnt, nc + 1) ||
         dalloc(&h->spawn_off, nc + 1) || dalloc(&h->hitmask, nc) || dalloc(&h->nsurv, nc) ||
         dalloc(&h->partial, (size_t)nc * h->nv * (dim + 1)) || dalloc(&h->counters, 4) ||
-        dalloc(&h->pcounter, 1) || dalloc(&h->stats, 5))
+        dalloc(&h->qcount, (size_t)(MOVE_ROUNDS + 2) * (QSHARDS + 1) * QSTRIDE) ||
+        dalloc(&h->pcounter, 1) || dalloc(&h->stats, STAT_SHARDS * NSTAT))
         return bail(PF2_E_HIP);
     if (mixed(h) || true) {
         if (dalloc(&h->dV32, (size_t)nd * dim) || dalloc(&h->dVold32, (size_t)nd * dim) ||
@@ -618,8 +659,9 @@ int pf2_create(const pf2_mesh* mesh, const pf2_params* params, long long capacit
         hipMemsetAsync(h->final_cnt, 0, sizeof(int) * (nc + 1), h->stream) ||
         hipMemsetAsync(h->spawn_cnt, 0, sizeof(int) * (nc + 1), h->stream) ||
         hipMemsetAsync(h->counters, 0, sizeof(int) * 4, h->stream) ||
+        hipMemsetAsync(h->qcount, 0, sizeof(int) * (MOVE_ROUNDS + 2) * (QSHARDS + 1) * QSTRIDE, h->stream) ||
         hipMemsetAsync(h->pcounter, 0, sizeof(uint32_t), h->stream) ||
-        hipMemsetAsync(h->stats, 0, sizeof(unsigned long long) * 5, h->stream))
+        hipMemsetAsync(h->stats, 0, sizeof(unsigned long long) * STAT_SHARDS * NSTAT, h->stream))
         return bail(fail(PF2_E_HIP, "initialisation failed"));
     if (hipcub::DeviceScan::ExclusiveSum(nullptr, h->scan_tmp_bytes, h->final_cnt, h->off[0], nc + 1, h->stream) !=
         hipSuccess)
@@ -639,7 +681,7 @@ int pf2_destroy(pf2_handle h) {
     void* ptrs[] = {h->dX, h->dcv, h->dcd, h->dstar_off, h->dstar_cell, h->dstar_dir, h->ddof_off, h->ddof_cell,
                     h->dftol, h->dV, h->dVold, h->dtmp, h->dV32, h->dVold32, h->dgeo32, h->off[0], h->off[1], h->arr_cnt,
                     h->arr_off, h->arr_fill, h->final_cnt, h->spawn_cnt, h->spawn_off, h->hitmask, h->nsurv,
-                    h->partial, h->counters, h->pcounter, h->stats, h->scan_tmp, h->w_dof, h->w_mask, h->w_fix,
+                    h->partial, h->counters, h->qcount, h->pcounter, h->stats, h->scan_tmp, h->w_dof, h->w_mask, h->w_fix,
                     h->w_vwall, h->w_cells, h->w_bc, h->w_val};
     for (void* p : ptrs)
         if (p) hipFree(p);
@@ -1009,8 +1051,14 @@ int pf2_particle_counter(pf2_handle h, uint32_t* count) {
 
 int pf2_stats(pf2_handle h, unsigned long long* out5) {
     if (!h || !out5) return fail(PF2_E_ARG, "NULL argument");
-    HIPC(hipMemcpyAsync(out5, h->stats, 5 * 8, hipMemcpyDeviceToHost, h->stream));
+    std::vector<unsigned long long> rows((size_t)STAT_SHARDS * NSTAT);
+    HIPC(hipMemcpyAsync(rows.data(), h->stats, rows.size() * 8, hipMemcpyDeviceToHost, h->stream));
     HIPC(hipStreamSynchronize(h->stream));
+    for (int k = 0; k < 5; ++k) {
+        unsigned long long t = 0;
+        for (int r = 0; r < STAT_SHARDS; ++r) t += rows[(size_t)r * NSTAT + k];
+        out5[k] = t;
+    }
     return 0;
 }
 

@@ -182,24 +182,91 @@ __device__ __forceinline__ int part_key32(const DevParams& pp, const float* xi) 
     return part_key<DIM>(pp, xd);
 }
 
+// FP32 substeps until done (0) or the reference point leaves the unit cell (1);
+// u (the last velocity) is returned for the relocation
+template <int DIM>
+__device__ __forceinline__ int advance32(const DevParams& pp, float h32, int i, double* __restrict__ pvext,
+                                         long long cap, float* xi, float* u, int& s, const float* g,
+                                         const float (*Vc)[DIM]) {
+    constexpr int NV = Q1<DIM>::NV;
+    for (; s < pp.substeps; ++s) {
+        float N[NV], dx[DIM], dl[DIM];
+        shape32<DIM>(xi, N);
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) u[d] = 0.0f;
+#pragma unroll
+        for (int k = 0; k < NV; ++k)
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) u[d] = __fmaf_rn(N[k], Vc[k][d], u[d]);
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) dx[d] = u[d] * h32;
+        if (pp.keep_vext && s == pp.substeps - 1 && pvext)
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) pvext[(size_t)d * cap + i] = (double)u[d];
+        bool in = false;
+        float e[DIM];
+        if (local_step<DIM>(g, xi, dx, dl)) {
+            in = true;
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) {
+                e[d] = xi[d] + dl[d];
+                in = in && (e[d] >= 0.0f) && (e[d] <= 1.0f);
+            }
+        }
+        if (!in) return 1;
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) xi[d] = e[d];
+    }
+    return 0;
+}
+
+template <int DIM>
+__device__ __forceinline__ void finish32(const DevParams& pp, const int* __restrict__ pcell, float* __restrict__ pxi,
+                                         long long cap, const DevWork& w, bool valid, int i, int c, const float* xi) {
+    if (!valid) return;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) pxi[(size_t)d * cap + i] = xi[d];
+    w.dest[i] = c;
+    if (c >= 0) {
+        w.pkey[i] = part_key32<DIM>(pp, xi);
+        if (c != pcell[i]) atomicAdd(&w.arr_cnt[c], 1);
+    }
+}
+
+template <int DIM>
+__device__ __forceinline__ void park32(float* __restrict__ pxi, long long cap, const DevWork& w, const DevQueues& qu,
+                                       int round_out, bool park, int i, int c, const float* xi, const float* u,
+                                       int s) {
+    queue_push(qu, round_out, park, i);
+    if (!park) return;
+    w.dest[i] = c;
+    qu.park_s[i] = s;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+        pxi[(size_t)d * cap + i] = xi[d];
+        qu.park_vext[(size_t)d * cap + i] = (double)u[d];
+    }
+}
+
 template <int DIM>
 __global__ void __launch_bounds__(256) k_move32(DevMesh<DIM> m, const float* __restrict__ V32,
                                                 const float* __restrict__ dV32, const float* __restrict__ geo,
-                                                int* __restrict__ pcell, float* __restrict__ pxi,
+                                                const int* __restrict__ pcell, float* __restrict__ pxi,
                                                 float* __restrict__ pv, double* __restrict__ pvext, long long cap,
-                                                DevParams pp, DevWork w, int do_correct, int do_move) {
+                                                DevParams pp, DevWork w, DevQueues qu, int do_correct, int do_move) {
     constexpr int NV = Q1<DIM>::NV;
     constexpr int NG = DIM == 2 ? 6 : 21;
     const int n = w.counters[0];
     const float h32 = (float)pp.h;
-    int lost = 0, searches = 0;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const int c0 = pcell[i];
-        int c = c0;
+    const int lane = threadIdx.x & (WAVE - 1);
+    for (int base = blockIdx.x * blockDim.x + (threadIdx.x - lane); base < n; base += gridDim.x * blockDim.x) {
+        const int i = base + lane;
+        const bool valid = i < n;
+        const int c = valid ? pcell[i] : 0;
         float xi[DIM];
 #pragma unroll
-        for (int d = 0; d < DIM; ++d) xi[d] = pxi[(size_t)d * cap + i];
-        if (do_correct) {
+        for (int d = 0; d < DIM; ++d) xi[d] = valid ? pxi[(size_t)d * cap + i] : 0.5f;
+        if (do_correct && valid) {
             float v[DIM], N[NV];
 #pragma unroll
             for (int d = 0; d < DIM; ++d) v[d] = pv[(size_t)d * cap + i];
@@ -214,87 +281,102 @@ __global__ void __launch_bounds__(256) k_move32(DevMesh<DIM> m, const float* __r
             for (int d = 0; d < DIM; ++d) pv[(size_t)d * cap + i] = v[d];
         }
         if (!do_move) continue;
-        float g[NG], Vc[NV][DIM];
-        load_cell32<DIM>(m, geo, V32, c, g, Vc);
-        for (int s = 0; s < pp.substeps; ++s) {
-            float N[NV], u[DIM], dx[DIM], dl[DIM];
-            shape32<DIM>(xi, N);
+        int st = 0, s = 0;
+        float u[DIM];
+        if (valid) {
+            float g[NG], Vc[NV][DIM];
+            load_cell32<DIM>(m, geo, V32, c, g, Vc);
+            st = advance32<DIM>(pp, h32, i, pvext, cap, xi, u, s, g, Vc);
+        }
+        park32<DIM>(pxi, cap, w, qu, 0, valid && st == 1, i, c, xi, u, s);
+        finish32<DIM>(pp, pcell, pxi, cap, w, valid && st == 0, i, c, xi);
+    }
+}
+
+// FP64 relocation of queued particles (the reference rule), then the
+// remaining FP32 substeps
+template <int DIM>
+__global__ void __launch_bounds__(256) k_search_resume32(DevMesh<DIM> m, const float* __restrict__ V32,
+                                                         const float* __restrict__ geo, const int* __restrict__ pcell,
+                                                         float* __restrict__ pxi, double* __restrict__ pvext,
+                                                         long long cap, DevParams pp, DevWork w, DevQueues qu,
+                                                         int round) {
+    constexpr int NV = Q1<DIM>::NV;
+    constexpr int NG = DIM == 2 ? 6 : 21;
+    __shared__ int s_pref[QSHARDS + 2];
+    const int n = queue_prefix(qu, round, s_pref);
+    const bool last = round == MOVE_ROUNDS;
+    const float h32 = (float)pp.h;
+    const int lane = threadIdx.x & (WAVE - 1);
+    int lost = 0, searches = 0;
+    for (int base = blockIdx.x * blockDim.x + (threadIdx.x - lane); base < n; base += gridDim.x * blockDim.x) {
+        const int j = base + lane;
+        const bool valid = j < n;
+        const int i = valid ? queue_get(qu, round, s_pref, j) : -1;
+        int c = valid ? w.dest[i] : -1;
+        int s = valid ? qu.park_s[i] : 0;
+        int st = 0;
+        float xi[DIM], u[DIM];
+        if (valid) {
 #pragma unroll
-            for (int d = 0; d < DIM; ++d) u[d] = 0.0f;
-#pragma unroll
-            for (int k = 0; k < NV; ++k)
-#pragma unroll
-                for (int d = 0; d < DIM; ++d) u[d] = __fmaf_rn(N[k], Vc[k][d], u[d]);
-#pragma unroll
-            for (int d = 0; d < DIM; ++d) dx[d] = u[d] * h32;
-            if (pp.keep_vext && s == pp.substeps - 1 && pvext)
-#pragma unroll
-                for (int d = 0; d < DIM; ++d) pvext[(size_t)d * cap + i] = (double)u[d];
-            if (local_step<DIM>(g, xi, dx, dl)) {
-                float e[DIM];
-                bool in = true;
+            for (int d = 0; d < DIM; ++d) {
+                xi[d] = pxi[(size_t)d * cap + i];
+                u[d] = (float)qu.park_vext[(size_t)d * cap + i];
+            }
+            for (;;) {
+                ++searches;
+                double P[NV][DIM], xid[DIM], x[DIM], vext[DIM], r[DIM];
+                load_vertices<DIM>(m, c, P);
 #pragma unroll
                 for (int d = 0; d < DIM; ++d) {
-                    e[d] = xi[d] + dl[d];
-                    in = in && (e[d] >= 0.0f) && (e[d] <= 1.0f);
+                    xid[d] = (double)xi[d];
+                    vext[d] = (double)u[d];
                 }
-                if (in) {
+                forward<DIM>(P, xid, x);
 #pragma unroll
-                    for (int d = 0; d < DIM; ++d) xi[d] = e[d];
-                    continue;
+                for (int d = 0; d < DIM; ++d) x[d] = x[d] + (double)(u[d] * h32);
+                const int cn = relocate<DIM>(m, c, P, xid, vext, x, r);
+                if (cn < 0) {
+                    c = -1;
+                    ++lost;
+                    st = 0;
+                    break;
                 }
-            }
-            // FP64 relocation with the reference rule
-            ++searches;
-            double P[NV][DIM], xid[DIM], x[DIM], vext[DIM], r[DIM];
-            load_vertices<DIM>(m, c, P);
+                // reference points are kept inside [0,1] after rounding to FP32
 #pragma unroll
-            for (int d = 0; d < DIM; ++d) {
-                xid[d] = (double)xi[d];
-                vext[d] = (double)u[d];
-            }
-            forward<DIM>(P, xid, x);
-#pragma unroll
-            for (int d = 0; d < DIM; ++d) x[d] = x[d] + (double)dx[d];
-            const int cn = relocate<DIM>(m, c, P, xid, vext, x, r);
-            if (cn < 0) {
-                c = -1;
-                ++lost;
-                break;
-            }
-            // reference points are kept inside [0,1] after rounding to FP32
-#pragma unroll
-            for (int d = 0; d < DIM; ++d) {
-                float t = (float)r[d];
-                xi[d] = t < 0.0f ? 0.0f : (t > 1.0f ? 1.0f : t);
-            }
-            if (cn != c) {
+                for (int d = 0; d < DIM; ++d) {
+                    const float t = (float)r[d];
+                    xi[d] = t < 0.0f ? 0.0f : (t > 1.0f ? 1.0f : t);
+                }
                 c = cn;
+                ++s;
+                float g[NG], Vc[NV][DIM];
                 load_cell32<DIM>(m, geo, V32, c, g, Vc);
+                st = advance32<DIM>(pp, h32, i, pvext, cap, xi, u, s, g, Vc);
+                if (st == 0 || !last) break;
             }
         }
-#pragma unroll
-        for (int d = 0; d < DIM; ++d) pxi[(size_t)d * cap + i] = xi[d];
-        w.dest[i] = c;
-        if (c >= 0) {
-            w.pkey[i] = part_key32<DIM>(pp, xi);
-            if (c != c0) {
-                const int slot = atomicAdd(&w.counters[1], 1);
-                w.movers[slot] = i;
-                atomicAdd(&w.arr_cnt[c], 1);
-            }
-        }
+        if (!last) park32<DIM>(pxi, cap, w, qu, round + 1, valid && st == 1, i, c, xi, u, s);
+        finish32<DIM>(pp, pcell, pxi, cap, w, valid && st == 0, i, c, xi);
     }
-    if (lost) atomicAdd(&w.stats[0], (unsigned long long)lost);
-    if (searches) atomicAdd(&w.stats[3], (unsigned long long)searches);
+    wave_stat(w.stats, 0, lost);
+    wave_stat(w.stats, 3, searches);
 }
 
 template <int DIM>
 void launch_move32(hipStream_t s, int grid, DevMesh<DIM> m, const float* V32, const float* dV32, const float* geo,
                    int* pcell, float* pxi, float* pv, double* pvext, long long cap, DevParams pp, DevWork w,
-                   int do_correct, int do_move) {
+                   DevQueues qu, int do_correct, int do_move) {
     hipLaunchKernelGGL(HIP_KERNEL_NAME(k_move32<DIM>), dim3(grid), dim3(256), 0, s, m, V32, dV32, geo, pcell, pxi, pv,
-                       pvext, cap, pp, w, do_correct, do_move);
+                       pvext, cap, pp, w, qu, do_correct, do_move);
+}
+
+template <int DIM>
+void launch_search_resume32(hipStream_t s, int grid, DevMesh<DIM> m, const float* V32, const float* geo,
+                            const int* pcell, float* pxi, double* pvext, long long cap, DevParams pp, DevWork w,
+                            DevQueues qu, int round) {
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_search_resume32<DIM>), dim3(grid), dim3(256), 0, s, m, V32, geo, pcell, pxi,
+                       pvext, cap, pp, w, qu, round);
 }
 
 template <int DIM>
