@@ -57,6 +57,25 @@ __device__ __forceinline__ void stat_add(unsigned long long* stats, int k, unsig
     atomicAdd(&stats[wave_shard() * NSTAT + k], v);
 }
 
+// atomicAdd(&cnt[key], 1) for the lanes with pred set, one atomic per distinct
+// key in the wavefront; returns the previous value for this lane (all lanes call)
+__device__ __forceinline__ int wave_inc_by_key(int* cnt, bool pred, int key) {
+    int result = 0;
+    unsigned long long todo = __ballot(pred);
+    const int lane = threadIdx.x & (WAVE - 1);
+    while (todo) {
+        const int leader = __ffsll((long long)todo) - 1;
+        const int k = __shfl(key, leader);
+        const unsigned long long same = __ballot(pred && key == k) & todo;
+        int base = 0;
+        if (lane == leader) base = atomicAdd(&cnt[k], __popcll(same));
+        base = __shfl(base, leader);
+        if ((same >> lane) & 1) result = base + __popcll(same & ((1ull << lane) - 1));
+        todo &= ~same;
+    }
+    return result;
+}
+
 // flush per-lane statistics with one sharded atomic per wavefront (all lanes must call)
 __device__ __forceinline__ void wave_stat(unsigned long long* stats, int k, int v) {
     const int t = wave_sum(v);
@@ -282,7 +301,7 @@ __device__ int relocate(const DevMesh<DIM>& m, int c, const double (*P)[DIM], co
 // particles then run in full wavefronts (k_search_resume), which continue
 // the remaining substeps and queue again if needed.  After MOVE_ROUNDS
 // rounds the last kernel finishes every remaining particle inline.
-constexpr int MOVE_ROUNDS = 4;
+constexpr int MOVE_ROUNDS = 2;
 
 // Queues of parked particles, one per round, ping-ponging between two
 // buffers.  A queue is QSHARDS shards of shard_cap entries plus an overflow
@@ -294,6 +313,7 @@ constexpr int QSTRIDE = 32;   // ints between shard counters
 struct DevQueues {
     int* data[2];         // ping-pong buffers: QSHARDS * shard_cap + overflow
     int* count;           // (MOVE_ROUNDS + 2) rounds x (QSHARDS + 1) counters, QSTRIDE apart
+    int* prefix;          // per round: QSHARDS + 2 ints (exclusive prefix, overflow start, total)
     int shard_cap;
     double* park_vext;    // DIM x cap: velocity_ext of a parked particle
     int* park_s;          // substep index of a parked particle
@@ -326,11 +346,11 @@ __device__ __forceinline__ void queue_push(const DevQueues& qu, int round, bool 
     if (spill) data[(size_t)QSHARDS * qu.shard_cap + ob + __popcll(smask & ((1ull << lane) - 1))] = value;
 }
 
-// block-wide exclusive prefix of the valid entries per shard (+ overflow);
-// s_pref must hold QSHARDS + 2 ints; returns the total
-__device__ __forceinline__ int queue_prefix(const DevQueues& qu, int round, int* s_pref) {
-    __shared__ int s_wave[8];
-    const int t = threadIdx.x;   // blockDim.x == QSHARDS
+// exclusive prefix of the valid entries per shard, then the overflow start
+// and the total; one block of QSHARDS threads per round
+__global__ void k_queue_prefix(DevQueues qu, int round) {
+    __shared__ int s_wave[QSHARDS / WAVE];
+    const int t = threadIdx.x;
     const int lane = t & (WAVE - 1), wv = t / WAVE;
     int v = *queue_count(qu, round, t);
     v = v < qu.shard_cap ? v : qu.shard_cap;
@@ -344,12 +364,19 @@ __device__ __forceinline__ int queue_prefix(const DevQueues& qu, int round, int*
     __syncthreads();
     int wbase = 0;
     for (int q = 0; q < wv; ++q) wbase += s_wave[q];
-    s_pref[t] = wbase + incl - v;
+    int* pref = qu.prefix + (size_t)round * (QSHARDS + 2);
+    pref[t] = wbase + incl - v;
     if (t == QSHARDS - 1) {
         const int tot = wbase + incl;
-        s_pref[QSHARDS] = tot;
-        s_pref[QSHARDS + 1] = tot + *queue_count(qu, round, QSHARDS);
+        pref[QSHARDS] = tot;
+        pref[QSHARDS + 1] = tot + *queue_count(qu, round, QSHARDS);
     }
+}
+
+// the prefix of a round into LDS (blockDim.x >= QSHARDS + 2 is not required)
+__device__ __forceinline__ int queue_prefix(const DevQueues& qu, int round, int* s_pref) {
+    const int* pref = qu.prefix + (size_t)round * (QSHARDS + 2);
+    for (int t = threadIdx.x; t < QSHARDS + 2; t += blockDim.x) s_pref[t] = pref[t];
     __syncthreads();
     return s_pref[QSHARDS + 1];
 }
@@ -394,11 +421,13 @@ __device__ __forceinline__ int advance64(const DevMesh<DIM>& m, const double* __
     return 0;
 }
 
-// final bookkeeping of a particle whose substeps are complete
+// final bookkeeping of a particle whose substeps are complete (all lanes call)
 template <int DIM>
 __device__ __forceinline__ void finish_particle(const DevParams& pp, const DevParticles& A, const DevWork& w,
                                                 bool valid, int i, int c, const double* x, const double* xi,
                                                 bool write_state) {
+    const bool arrive = valid && c >= 0 && c != A.cell[i < 0 ? 0 : i];
+    wave_inc_by_key(w.arr_cnt, arrive, c);
     if (!valid) return;
     if (write_state) {
 #pragma unroll
@@ -408,10 +437,7 @@ __device__ __forceinline__ void finish_particle(const DevParams& pp, const DevPa
         }
     }
     w.dest[i] = c;
-    if (c >= 0) {
-        w.pkey[i] = part_key<DIM>(pp, xi);
-        if (c != A.cell[i]) atomicAdd(&w.arr_cnt[c], 1);
-    }
+    if (c >= 0) w.pkey[i] = part_key<DIM>(pp, xi);
 }
 
 // park a particle for the search of the next round (all lanes call)
@@ -537,11 +563,13 @@ __global__ void __launch_bounds__(256) k_search_resume(DevMesh<DIM> m, const dou
 // particles that changed cell, grouped by destination cell
 __global__ void k_group(const int* __restrict__ pcell, DevWork w) {
     const int n = w.counters[0];
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const int c = w.dest[i];
-        if (c < 0 || c == pcell[i]) continue;
-        const int slot = atomicAdd(&w.arr_fill[c], 1);
-        w.arr_list[w.arr_off[c] + slot] = i;
+    const int lane = threadIdx.x & (WAVE - 1);
+    for (int base = blockIdx.x * blockDim.x + (threadIdx.x - lane); base < n; base += gridDim.x * blockDim.x) {
+        const int i = base + lane;
+        const int c = i < n ? w.dest[i] : -1;
+        const bool arrive = c >= 0 && c != pcell[i];
+        const int slot = wave_inc_by_key(w.arr_fill, arrive, c);
+        if (arrive) w.arr_list[w.arr_off[c] + slot] = i;
     }
 }
 
@@ -551,16 +579,12 @@ __device__ __forceinline__ int candidate(const DevWork& w, const int* old_off, i
 }
 
 // ---------------------------------------------------------------------------
-// k_cell_count: per-cell part census (check_cell_for_empty_parts,
-// cylinder2D/pfem2particle.cpp:510-561).  Part index per direction is
-// (unsigned)(xi_d / (1/q_d)) without clamping (xi = 1 gives the extra index
-// q_d).  In every part the MAX particles with the smallest ids survive;
-// every in-range part that holds no particle is reseeded.  With reseed = 0
-// (relocation only) every particle that found a cell survives.
-// Each wavefront handles a sequence of cells; a cell's candidates are staged
-// in LDS when there are at most CCAP of them, else read from global memory.
-constexpr int CCAP = 256;
-constexpr int CELL_WAVES = 4;
+// Cell kernels.  A cell is handled by a group of GS = 16 lanes, so a
+// wavefront works on four cells at once (the per-cell work is a short chain
+// of dependent loads; several cells per wavefront hide that latency).
+constexpr int GS = 16;
+constexpr int GROUPS = 256 / GS;     // groups per 256-thread block
+constexpr int GCAP = 64;             // candidates staged in LDS per group
 
 __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -572,65 +596,81 @@ __device__ __forceinline__ int load_agent(const int* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// sum / OR over a 16-lane group
+__device__ __forceinline__ int group_sum(int v) {
+#pragma unroll
+    for (int off = 1; off < GS; off <<= 1) v += __shfl_xor(v, off);
+    return v;
+}
+
+__device__ __forceinline__ unsigned long long group_or(unsigned long long v) {
+#pragma unroll
+    for (int off = 1; off < GS; off <<= 1) {
+        const unsigned lo = __shfl_xor((unsigned)(v & 0xffffffffu), off);
+        const unsigned hi = __shfl_xor((unsigned)(v >> 32), off);
+        v |= ((unsigned long long)hi << 32) | lo;
+    }
+    return v;
+}
+
+// k_cell_count: per-cell part census (check_cell_for_empty_parts,
+// cylinder2D/pfem2particle.cpp:510-561).  Part index per direction is
+// (unsigned)(xi_d / (1/q_d)) without clamping (xi = 1 gives the extra index
+// q_d).  In every part the MAX particles with the smallest ids survive;
+// every in-range part that holds no particle is reseeded.  With reseed = 0
+// (relocation only) every particle that found a cell survives.
+// A cell's candidates (stayers of its old segment, then arrivals) are staged
+// in LDS when there are at most GCAP of them, else read from global memory.
 template <int DIM>
 __global__ void __launch_bounds__(256) k_cell_count(int n_cells, const int* __restrict__ old_off,
                                                     const uint32_t* __restrict__ pid, DevParams pp, DevWork w,
                                                     int reseed) {
-    __shared__ int s_idx[CELL_WAVES][CCAP];
-    __shared__ int s_key[CELL_WAVES][CCAP];
-    __shared__ uint32_t s_id[CELL_WAVES][CCAP];
-    __shared__ unsigned char s_surv[CELL_WAVES][CCAP];
-    const int lane = threadIdx.x & (WAVE - 1);
-    const int wv = threadIdx.x / WAVE;
+    // rows padded by one word so that the groups of a wavefront reading the
+    // same column hit different LDS banks
+    __shared__ int s_idx[GROUPS][GCAP + 1];
+    __shared__ int s_key[GROUPS][GCAP + 1];
+    __shared__ uint32_t s_id[GROUPS][GCAP + 1];
+    __shared__ unsigned char s_surv[GROUPS][GCAP + 4];
+    const int g = threadIdx.x / GS, gl = threadIdx.x % GS;
     const unsigned long long all = pp.nparts_in >= 64 ? ~0ull : ((1ull << pp.nparts_in) - 1);
     int deleted = 0;
-    for (int c = blockIdx.x * CELL_WAVES + wv; c < n_cells; c += gridDim.x * CELL_WAVES) {
-        const int n_st = old_off[c + 1] - old_off[c];
-        const int k = n_st + w.arr_cnt[c];
+    for (int cb = blockIdx.x * GROUPS; cb < n_cells; cb += gridDim.x * GROUPS) {
+        const int c = cb + g;
+        const bool cv = c < n_cells;
+        const int n_st = cv ? old_off[c + 1] - old_off[c] : 0;
+        const int k = cv ? n_st + w.arr_cnt[c] : 0;
+        const bool small = k <= GCAP;
         unsigned long long hit = 0;
         int nvalid = 0, nsurv = 0;
-        if (k <= CCAP) {
-            for (int j = lane; j < k; j += WAVE) {
+        if (small)
+            for (int j = gl; j < k; j += GS) {
                 const int idx = candidate(w, old_off, c, n_st, j);
                 const bool valid = w.dest[idx] == c;
-                s_idx[wv][j] = valid ? idx : -1;
-                s_key[wv][j] = valid ? w.pkey[idx] : -1;
-                s_id[wv][j] = pid[idx];
+                s_idx[g][j] = valid ? idx : -1;
+                s_key[g][j] = valid ? w.pkey[idx] : -1;
+                s_id[g][j] = pid[idx];
             }
-            wave_sync();
-            for (int j = lane; j < k; j += WAVE) {
+        wave_sync();
+        if (small)
+            for (int j = gl; j < k; j += GS) {
                 int surv = 0;
-                if (s_idx[wv][j] >= 0) {
+                if (s_idx[g][j] >= 0) {
                     ++nvalid;
-                    const int key = s_key[wv][j];
-                    const uint32_t id = s_id[wv][j];
+                    const int key = s_key[g][j];
+                    const uint32_t id = s_id[g][j];
                     const int o = part_ordinal<DIM>(pp, key);
                     if (o >= 0) hit |= 1ull << o;
                     int rank = 0;
                     if (reseed)
-                        for (int t = 0; t < k; ++t) rank += (s_key[wv][t] == key && s_id[wv][t] < id) ? 1 : 0;
+                        for (int t = 0; t < k; ++t) rank += (s_key[g][t] == key && s_id[g][t] < id) ? 1 : 0;
                     surv = rank < pp.max_per_part || !reseed;
                     nsurv += surv;
                 }
-                s_surv[wv][j] = (unsigned char)surv;
+                s_surv[g][j] = (unsigned char)surv;
             }
-            wave_sync();
-            for (int j = lane; j < k; j += WAVE) {
-                const int idx = s_idx[wv][j];
-                if (idx < 0) continue;
-                if (!s_surv[wv][j]) {
-                    w.spos[idx] = -1;
-                    continue;
-                }
-                const uint32_t id = s_id[wv][j];
-                int pos = 0;
-                for (int t = 0; t < k; ++t) pos += (s_surv[wv][t] && s_id[wv][t] < id) ? 1 : 0;
-                w.spos[idx] = pos;
-            }
-            wave_sync();
-        } else {
-            // large cell: same census straight from global memory
-            for (int j = lane; j < k; j += WAVE) {
+        else
+            // large cell: the same census straight from global memory
+            for (int j = gl; j < k; j += GS) {
                 const int idx = candidate(w, old_off, c, n_st, j);
                 if (w.dest[idx] != c) continue;
                 ++nvalid;
@@ -648,9 +688,24 @@ __global__ void __launch_bounds__(256) k_cell_count(int n_cells, const int* __re
                 nsurv += surv;
                 w.spos[idx] = surv ? 0 : -1;
             }
-            __threadfence();
-            wave_sync();
-            for (int j = lane; j < k; j += WAVE) {
+        if (__ballot(!small) != 0) __threadfence();
+        wave_sync();
+        // positions of the survivors in ascending id order
+        if (small)
+            for (int j = gl; j < k; j += GS) {
+                const int idx = s_idx[g][j];
+                if (idx < 0) continue;
+                if (!s_surv[g][j]) {
+                    w.spos[idx] = -1;
+                    continue;
+                }
+                const uint32_t id = s_id[g][j];
+                int pos = 0;
+                for (int t = 0; t < k; ++t) pos += (s_surv[g][t] && s_id[g][t] < id) ? 1 : 0;
+                w.spos[idx] = pos;
+            }
+        else
+            for (int j = gl; j < k; j += GS) {
                 const int idx = candidate(w, old_off, c, n_st, j);
                 if (w.dest[idx] != c || load_agent(&w.spos[idx]) < 0) continue;
                 const uint32_t id = pid[idx];
@@ -661,16 +716,11 @@ __global__ void __launch_bounds__(256) k_cell_count(int n_cells, const int* __re
                 }
                 w.spos[idx] = pos;
             }
-        }
-#pragma unroll
-        for (int off = 1; off < WAVE; off <<= 1) {
-            const unsigned lo = __shfl_xor((unsigned)(hit & 0xffffffffu), off);
-            const unsigned hi = __shfl_xor((unsigned)(hit >> 32), off);
-            hit |= ((unsigned long long)hi << 32) | lo;
-            nvalid += __shfl_xor(nvalid, off);
-            nsurv += __shfl_xor(nsurv, off);
-        }
-        if (lane == 0) {
+        wave_sync();
+        hit = group_or(hit);
+        nvalid = group_sum(nvalid);
+        nsurv = group_sum(nsurv);
+        if (cv && gl == 0) {
             const int spawns = reseed ? __popcll(all & ~hit) : 0;
             w.hitmask[c] = reseed ? hit : all;
             w.nsurv[c] = nsurv;
@@ -679,7 +729,8 @@ __global__ void __launch_bounds__(256) k_cell_count(int n_cells, const int* __re
             deleted += nvalid - nsurv;
         }
     }
-    if (lane == 0 && deleted) stat_add(w.stats, 1, (unsigned long long)deleted);
+    deleted = wave_sum(deleted);
+    if ((threadIdx.x & (WAVE - 1)) == 0 && deleted) stat_add(w.stats, 1, (unsigned long long)deleted);
 }
 
 // ---------------------------------------------------------------------------
@@ -691,7 +742,7 @@ template <int DIM>
 __device__ __forceinline__ void cell_write_one(const DevMesh<DIM>& m, const double* __restrict__ V,
                                                const int* __restrict__ old_off, const DevParticles& A,
                                                const DevParticles& B, const DevParams& pp, const DevWork& w, int c,
-                                               int lane) {
+                                               int gl) {
     constexpr int NV = Q1<DIM>::NV;
     const int n_st = old_off[c + 1] - old_off[c];
     const int k = n_st + w.arr_cnt[c];
@@ -699,10 +750,10 @@ __device__ __forceinline__ void cell_write_one(const DevMesh<DIM>& m, const doub
     const int nsurv = w.nsurv[c];
     const int nfinal = w.final_cnt[c];
     if (out0 + nfinal > B.cap) {
-        if (lane == 0) atomicOr(&w.counters[2], 1);
+        if (gl == 0) atomicOr(&w.counters[2], 1);
         return;
     }
-    for (int j = lane; j < k; j += WAVE) {
+    for (int j = gl; j < k; j += GS) {
         const int idx = candidate(w, old_off, c, n_st, j);
         if (w.dest[idx] != c) continue;
         const int pos = w.spos[idx];
@@ -725,7 +776,7 @@ __device__ __forceinline__ void cell_write_one(const DevMesh<DIM>& m, const doub
         const unsigned long long all = pp.nparts_in >= 64 ? ~0ull : ((1ull << pp.nparts_in) - 1);
         const unsigned long long empty = all & ~w.hitmask[c];
         const uint32_t id0 = *w.particle_counter + (uint32_t)w.spawn_off[c];
-        for (int o = lane; o < pp.nparts_in; o += WAVE) {
+        for (int o = gl; o < pp.nparts_in; o += GS) {
             if (!((empty >> o) & 1)) continue;
             const int r = __popcll(empty & ((1ull << o) - 1));
             double xi[DIM], P[NV][DIM], Vc[NV][DIM], x[DIM], v[DIM];
@@ -747,7 +798,7 @@ __device__ __forceinline__ void cell_write_one(const DevMesh<DIM>& m, const doub
                 for (int d = 0; d < DIM; ++d) B.vext[(size_t)d * B.cap + q] = 0.0;
         }
     }
-    if (lane == 0) {
+    if (gl == 0) {
         w.arr_cnt[c] = 0;
         w.arr_fill[c] = 0;
     }
@@ -757,9 +808,9 @@ template <int DIM>
 __global__ void __launch_bounds__(256) k_cell_write(DevMesh<DIM> m, const double* __restrict__ V,
                                                     const int* __restrict__ old_off, DevParticles A,
                                                     DevParticles B, DevParams pp, DevWork w) {
-    const int lane = threadIdx.x & (WAVE - 1);
-    for (int c = blockIdx.x * CELL_WAVES + threadIdx.x / WAVE; c < m.n_cells; c += gridDim.x * CELL_WAVES)
-        cell_write_one<DIM>(m, V, old_off, A, B, pp, w, c, lane);
+    const int g = threadIdx.x / GS, gl = threadIdx.x % GS;
+    for (int c = blockIdx.x * GROUPS + g; c < m.n_cells; c += gridDim.x * GROUPS)
+        cell_write_one<DIM>(m, V, old_off, A, B, pp, w, c, gl);
 }
 
 // totals after the scans: particle count, id counter, statistics; resets the
@@ -782,57 +833,70 @@ __global__ void k_finalize(int n_cells, DevWork w, int* queue_count) {
 // k_cell_partials: per-cell local sums of the projection
 // (TurekBenchmark/pfem2particle.cpp:880-893): for each cell vertex k,
 // sum_p N_k(xi_p) v_p and sum_p N_k(xi_p), particles in ascending id order.
-// Lanes stage a chunk of particles in LDS; lane a = k*(DIM+1) + comp then
-// accumulates its sum sequentially.
+// A group of PGS lanes per cell stages PGS particles at a time in LDS; lane
+// a = k*(DIM+1) + comp of the group then adds its terms in order.
+template <int DIM>
+struct PartialsShape {
+    static constexpr int NACC = Q1<DIM>::NV * (DIM + 1);
+    static constexpr int PGS = NACC <= 16 ? 16 : 32;
+    static constexpr int NG = 256 / PGS;
+};
+
 template <int DIM>
 __global__ void __launch_bounds__(256) k_cell_partials(int n_cells, const int* __restrict__ off, DevParticles A,
                                                        double* __restrict__ partial) {
     constexpr int NV = Q1<DIM>::NV;
-    constexpr int NACC = NV * (DIM + 1);
-    __shared__ double s_xi[4][WAVE][DIM];
-    __shared__ double s_v[4][WAVE][DIM];
-    const int lane = threadIdx.x & (WAVE - 1);
-    const int wv = threadIdx.x / WAVE;
-    const int kk = lane / (DIM + 1);
-    const int comp = lane - kk * (DIM + 1);
-    for (int c = blockIdx.x * 4 + wv; c < n_cells; c += gridDim.x * 4) {
-    const int b = off[c], e = off[c + 1];
-    double acc = 0.0;
-    for (int j0 = b; j0 < e; j0 += WAVE) {
-        const int j = j0 + lane;
-        if (j < e) {
+    constexpr int NACC = PartialsShape<DIM>::NACC;
+    constexpr int PGS = PartialsShape<DIM>::PGS;
+    constexpr int NG = PartialsShape<DIM>::NG;
+    __shared__ double s_xi[NG][PGS][DIM];
+    __shared__ double s_v[NG][PGS][DIM];
+    const int g = threadIdx.x / PGS, gl = threadIdx.x % PGS;
+    const int kk = gl / (DIM + 1);
+    const int comp = gl - kk * (DIM + 1);
+    for (int cb = blockIdx.x * NG; cb < n_cells; cb += gridDim.x * NG) {
+        const int c = cb + g;
+        const bool cv = c < n_cells;
+        const int b = cv ? off[c] : 0, e = cv ? off[c + 1] : 0;
+        int trips = (e - b + PGS - 1) / PGS;
 #pragma unroll
-            for (int d = 0; d < DIM; ++d) {
-                s_xi[wv][lane][d] = A.xi[(size_t)d * A.cap + j];
-                s_v[wv][lane][d] = A.v[(size_t)d * A.cap + j];
-            }
-        }
-        wave_sync();
-        const int cnt = (e - j0) < WAVE ? (e - j0) : WAVE;
-        if (lane < NACC) {
-            for (int t = 0; t < cnt; ++t) {
-                double xi[DIM], N[NV];
+        for (int o = 1; o < WAVE; o <<= 1) trips = max(trips, __shfl_xor(trips, o));
+        double acc = 0.0;
+        for (int t = 0; t < trips; ++t) {
+            const int j0 = b + t * PGS;
+            const int j = j0 + gl;
+            if (j < e) {
 #pragma unroll
-                for (int d = 0; d < DIM; ++d) xi[d] = s_xi[wv][t][d];
-                shape<DIM>(xi, N);
-                double nk = N[0];
-#pragma unroll
-                for (int q = 1; q < NV; ++q)
-                    if (q == kk) nk = N[q];
-                if (comp < DIM) {
-                    double vc = s_v[wv][t][0];
-#pragma unroll
-                    for (int d = 1; d < DIM; ++d)
-                        if (d == comp) vc = s_v[wv][t][d];
-                    acc += nk * vc;
-                } else {
-                    acc += nk;
+                for (int d = 0; d < DIM; ++d) {
+                    s_xi[g][gl][d] = A.xi[(size_t)d * A.cap + j];
+                    s_v[g][gl][d] = A.v[(size_t)d * A.cap + j];
                 }
             }
+            wave_sync();
+            const int cnt = (e - j0) < PGS ? (e - j0) : PGS;
+            if (gl < NACC)
+                for (int q = 0; q < cnt; ++q) {
+                    double xi[DIM], N[NV];
+#pragma unroll
+                    for (int d = 0; d < DIM; ++d) xi[d] = s_xi[g][q][d];
+                    shape<DIM>(xi, N);
+                    double nk = N[0];
+#pragma unroll
+                    for (int r = 1; r < NV; ++r)
+                        if (r == kk) nk = N[r];
+                    if (comp < DIM) {
+                        double vc = s_v[g][q][0];
+#pragma unroll
+                        for (int d = 1; d < DIM; ++d)
+                            if (d == comp) vc = s_v[g][q][d];
+                        acc += nk * vc;
+                    } else {
+                        acc += nk;
+                    }
+                }
+            wave_sync();
         }
-        wave_sync();
-    }
-    if (lane < NACC) partial[(size_t)c * NACC + lane] = acc;
+        if (cv && gl < NACC) partial[(size_t)c * NACC + gl] = acc;
     }
 }
 
@@ -1055,7 +1119,7 @@ __global__ void __launch_bounds__(256) k_resort(DevMesh<DIM> m, DevParticles A, 
         w.dest[i] = c;
         if (c >= 0) w.pkey[i] = part_key<DIM>(pp, xi);
         }
-        if (valid && c >= 0 && c != c0) atomicAdd(&w.arr_cnt[c], 1);
+        wave_inc_by_key(w.arr_cnt, valid && c >= 0 && c != c0, c);
     }
     wave_stat(w.stats, 0, lost);
     wave_stat(w.stats, 3, searches);

@@ -105,6 +105,7 @@ struct pf2_handle_s {
     double* partial = nullptr;
     int* counters = nullptr;
     int* qcount = nullptr;
+    int* qprefix = nullptr;
     long long shard_cap = 0;
     int* queue[2] = {nullptr, nullptr};
     double* park_vext = nullptr;
@@ -173,9 +174,9 @@ struct Timed {
     }
 };
 
-// cell kernels: four wavefronts per block, each looping over cells
+// cell kernels: 16 (or 8) cells per 256-thread block, blocks looping over cells
 int cell_grid(int n_cells) {
-    int g = (n_cells + 3) / 4;
+    int g = (n_cells + 15) / 16;
     return g < 2048 ? (g < 1 ? 1 : g) : 2048;
 }
 
@@ -222,6 +223,7 @@ DevQueues dev_queues(pf2_handle_s* h) {
     q.data[0] = h->queue[0];
     q.data[1] = h->queue[1];
     q.count = h->qcount;
+    q.prefix = h->qprefix;
     q.shard_cap = (int)h->shard_cap;
     q.park_vext = h->park_vext;
     q.park_s = h->park_s;
@@ -423,8 +425,9 @@ int run_move(pf2_handle_s* h, int do_correct, int do_move) {
         if (do_move)
             for (int r = 0; r <= MOVE_ROUNDS; ++r) {
                 Timed t(h, K_SEARCH32);
-                launch_search_resume32<DIM>(h->stream, qgrid, dev_mesh<DIM>(h), h->dV32, h->dgeo32, b.cell, b.xi32,
-                                            b.vext, h->cap, pp, w, qu, r);
+                hipLaunchKernelGGL(k_queue_prefix, dim3(1), dim3(QSHARDS), 0, h->stream, qu, r);
+                launch_search_resume32<DIM>(h->stream, r == 0 ? qgrid : qgrid / 4 + 1, dev_mesh<DIM>(h), h->dV32,
+                                            h->dgeo32, b.cell, b.xi32, b.vext, h->cap, pp, w, qu, r);
             }
     } else {
         {
@@ -435,8 +438,10 @@ int run_move(pf2_handle_s* h, int do_correct, int do_move) {
         if (do_move)
             for (int r = 0; r <= MOVE_ROUNDS; ++r) {
                 Timed t(h, K_SEARCH);
-                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_search_resume<DIM>), dim3(qgrid), dim3(256), 0, h->stream,
-                                   dev_mesh<DIM>(h), h->dV, dev_particles(h, h->cur), pp, w, qu, r);
+                hipLaunchKernelGGL(k_queue_prefix, dim3(1), dim3(QSHARDS), 0, h->stream, qu, r);
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_search_resume<DIM>), dim3(r == 0 ? qgrid : qgrid / 4 + 1),
+                                   dim3(256), 0, h->stream, dev_mesh<DIM>(h), h->dV, dev_particles(h, h->cur), pp, w,
+                                   qu, r);
             }
     }
     HIPC(hipGetLastError());
@@ -632,6 +637,7 @@ int pf2_create(const pf2_mesh* mesh, const pf2_params* params, long long capacit
         dalloc(&h->spawn_off, nc + 1) || dalloc(&h->hitmask, nc) || dalloc(&h->nsurv, nc) ||
         dalloc(&h->partial, (size_t)nc * h->nv * (dim + 1)) || dalloc(&h->counters, 4) ||
         dalloc(&h->qcount, (size_t)(MOVE_ROUNDS + 2) * (QSHARDS + 1) * QSTRIDE) ||
+        dalloc(&h->qprefix, (size_t)(MOVE_ROUNDS + 2) * (QSHARDS + 2)) ||
         dalloc(&h->pcounter, 1) || dalloc(&h->stats, STAT_SHARDS * NSTAT))
         return bail(PF2_E_HIP);
     if (mixed(h) || true) {
@@ -681,7 +687,7 @@ int pf2_destroy(pf2_handle h) {
     void* ptrs[] = {h->dX, h->dcv, h->dcd, h->dstar_off, h->dstar_cell, h->dstar_dir, h->ddof_off, h->ddof_cell,
                     h->dftol, h->dV, h->dVold, h->dtmp, h->dV32, h->dVold32, h->dgeo32, h->off[0], h->off[1], h->arr_cnt,
                     h->arr_off, h->arr_fill, h->final_cnt, h->spawn_cnt, h->spawn_off, h->hitmask, h->nsurv,
-                    h->partial, h->counters, h->qcount, h->pcounter, h->stats, h->scan_tmp, h->w_dof, h->w_mask, h->w_fix,
+                    h->partial, h->counters, h->qcount, h->qprefix, h->pcounter, h->stats, h->scan_tmp, h->w_dof, h->w_mask, h->w_fix,
                     h->w_vwall, h->w_cells, h->w_bc, h->w_val};
     for (void* p : ptrs)
         if (p) hipFree(p);

@@ -223,14 +223,13 @@ __device__ __forceinline__ int advance32(const DevParams& pp, float h32, int i, 
 template <int DIM>
 __device__ __forceinline__ void finish32(const DevParams& pp, const int* __restrict__ pcell, float* __restrict__ pxi,
                                          long long cap, const DevWork& w, bool valid, int i, int c, const float* xi) {
+    const bool arrive = valid && c >= 0 && c != pcell[i < 0 ? 0 : i];
+    wave_inc_by_key(w.arr_cnt, arrive, c);
     if (!valid) return;
 #pragma unroll
     for (int d = 0; d < DIM; ++d) pxi[(size_t)d * cap + i] = xi[d];
     w.dest[i] = c;
-    if (c >= 0) {
-        w.pkey[i] = part_key32<DIM>(pp, xi);
-        if (c != pcell[i]) atomicAdd(&w.arr_cnt[c], 1);
-    }
+    if (c >= 0) w.pkey[i] = part_key32<DIM>(pp, xi);
 }
 
 template <int DIM>
@@ -387,7 +386,7 @@ __device__ __forceinline__ void cell_write32_one(const DevMesh<DIM>& m, const do
                                                  float* __restrict__ bxi, float* __restrict__ bv,
                                                  const double* __restrict__ avext, double* __restrict__ bvext,
                                                  long long cap, const DevParams& pp, const DevWork& w, int c,
-                                                 int lane) {
+                                                 int gl) {
     constexpr int NV = Q1<DIM>::NV;
     const int n_st = old_off[c + 1] - old_off[c];
     const int k = n_st + w.arr_cnt[c];
@@ -395,10 +394,10 @@ __device__ __forceinline__ void cell_write32_one(const DevMesh<DIM>& m, const do
     const int nsurv = w.nsurv[c];
     const int nfinal = w.final_cnt[c];
     if (out0 + nfinal > cap) {
-        if (lane == 0) atomicOr(&w.counters[2], 1);
+        if (gl == 0) atomicOr(&w.counters[2], 1);
         return;
     }
-    for (int j = lane; j < k; j += WAVE) {
+    for (int j = gl; j < k; j += GS) {
         const int idx = candidate(w, old_off, c, n_st, j);
         if (w.dest[idx] != c) continue;
         const int pos = w.spos[idx];
@@ -420,7 +419,7 @@ __device__ __forceinline__ void cell_write32_one(const DevMesh<DIM>& m, const do
         const unsigned long long all = pp.nparts_in >= 64 ? ~0ull : ((1ull << pp.nparts_in) - 1);
         const unsigned long long empty = all & ~w.hitmask[c];
         const uint32_t id0 = *w.particle_counter + (uint32_t)w.spawn_off[c];
-        for (int o = lane; o < pp.nparts_in; o += WAVE) {
+        for (int o = gl; o < pp.nparts_in; o += GS) {
             if (!((empty >> o) & 1)) continue;
             const int r = __popcll(empty & ((1ull << o) - 1));
             double xi[DIM], P[NV][DIM], Vc[NV][DIM], v[DIM];
@@ -438,7 +437,7 @@ __device__ __forceinline__ void cell_write32_one(const DevMesh<DIM>& m, const do
             }
         }
     }
-    if (lane == 0) {
+    if (gl == 0) {
         w.arr_cnt[c] = 0;
         w.arr_fill[c] = 0;
     }
@@ -452,9 +451,9 @@ __global__ void __launch_bounds__(256) k_cell_write32(DevMesh<DIM> m, const doub
                                                      float* __restrict__ bxi, float* __restrict__ bv,
                                                      const double* __restrict__ avext, double* __restrict__ bvext,
                                                      long long cap, DevParams pp, DevWork w) {
-    const int lane = threadIdx.x & (WAVE - 1);
-    for (int c = blockIdx.x * CELL_WAVES + threadIdx.x / WAVE; c < m.n_cells; c += gridDim.x * CELL_WAVES)
-        cell_write32_one<DIM>(m, V, old_off, aid, axi, av, bcell, bid, bxi, bv, avext, bvext, cap, pp, w, c, lane);
+    const int g = threadIdx.x / GS, gl = threadIdx.x % GS;
+    for (int c = blockIdx.x * GROUPS + g; c < m.n_cells; c += gridDim.x * GROUPS)
+        cell_write32_one<DIM>(m, V, old_off, aid, axi, av, bcell, bid, bxi, bv, avext, bvext, cap, pp, w, c, gl);
 }
 
 template <int DIM>
@@ -466,59 +465,66 @@ void launch_cell_write32(hipStream_t s, int grid, DevMesh<DIM> m, const double* 
 }
 
 // per-cell projection sums in FP32 with Kahan compensation, ascending id order
+// (groups of lanes per cell as in k_cell_partials)
 template <int DIM>
 __global__ void __launch_bounds__(256) k_cell_partials32(int n_cells, const int* __restrict__ off,
                                                          const float* __restrict__ pxi, const float* __restrict__ pv,
                                                          long long cap, double* __restrict__ partial) {
     constexpr int NV = Q1<DIM>::NV;
-    constexpr int NACC = NV * (DIM + 1);
-    __shared__ float s_xi[4][WAVE][DIM];
-    __shared__ float s_v[4][WAVE][DIM];
-    const int lane = threadIdx.x & (WAVE - 1);
-    const int wv = threadIdx.x / WAVE;
-    const int kk = lane / (DIM + 1);
-    const int comp = lane - kk * (DIM + 1);
-    for (int c = blockIdx.x * 4 + wv; c < n_cells; c += gridDim.x * 4) {
-    const int b = off[c], e = off[c + 1];
-    float sum = 0.0f, cmp = 0.0f;
-    for (int j0 = b; j0 < e; j0 += WAVE) {
-        const int j = j0 + lane;
-        if (j < e) {
+    constexpr int NACC = PartialsShape<DIM>::NACC;
+    constexpr int PGS = PartialsShape<DIM>::PGS;
+    constexpr int NG = PartialsShape<DIM>::NG;
+    __shared__ float s_xi[NG][PGS][DIM];
+    __shared__ float s_v[NG][PGS][DIM];
+    const int g = threadIdx.x / PGS, gl = threadIdx.x % PGS;
+    const int kk = gl / (DIM + 1);
+    const int comp = gl - kk * (DIM + 1);
+    for (int cb = blockIdx.x * NG; cb < n_cells; cb += gridDim.x * NG) {
+        const int c = cb + g;
+        const bool cv = c < n_cells;
+        const int b = cv ? off[c] : 0, e = cv ? off[c + 1] : 0;
+        int trips = (e - b + PGS - 1) / PGS;
 #pragma unroll
-            for (int d = 0; d < DIM; ++d) {
-                s_xi[wv][lane][d] = pxi[(size_t)d * cap + j];
-                s_v[wv][lane][d] = pv[(size_t)d * cap + j];
-            }
-        }
-        wave_sync();
-        const int cnt = (e - j0) < WAVE ? (e - j0) : WAVE;
-        if (lane < NACC) {
-            for (int t = 0; t < cnt; ++t) {
-                float xi[DIM], N[NV];
+        for (int o = 1; o < WAVE; o <<= 1) trips = max(trips, __shfl_xor(trips, o));
+        float sum = 0.0f, cmp = 0.0f;
+        for (int t = 0; t < trips; ++t) {
+            const int j0 = b + t * PGS;
+            const int j = j0 + gl;
+            if (j < e) {
 #pragma unroll
-                for (int d = 0; d < DIM; ++d) xi[d] = s_xi[wv][t][d];
-                shape32<DIM>(xi, N);
-                float nk = N[0];
-#pragma unroll
-                for (int q = 1; q < NV; ++q)
-                    if (q == kk) nk = N[q];
-                float term = nk;
-                if (comp < DIM) {
-                    float vc = s_v[wv][t][0];
-#pragma unroll
-                    for (int d = 1; d < DIM; ++d)
-                        if (d == comp) vc = s_v[wv][t][d];
-                    term = nk * vc;
+                for (int d = 0; d < DIM; ++d) {
+                    s_xi[g][gl][d] = pxi[(size_t)d * cap + j];
+                    s_v[g][gl][d] = pv[(size_t)d * cap + j];
                 }
-                const float y = term - cmp;
-                const float tt = sum + y;
-                cmp = (tt - sum) - y;
-                sum = tt;
             }
+            wave_sync();
+            const int cnt = (e - j0) < PGS ? (e - j0) : PGS;
+            if (gl < NACC)
+                for (int q = 0; q < cnt; ++q) {
+                    float xi[DIM], N[NV];
+#pragma unroll
+                    for (int d = 0; d < DIM; ++d) xi[d] = s_xi[g][q][d];
+                    shape32<DIM>(xi, N);
+                    float nk = N[0];
+#pragma unroll
+                    for (int r = 1; r < NV; ++r)
+                        if (r == kk) nk = N[r];
+                    float term = nk;
+                    if (comp < DIM) {
+                        float vc = s_v[g][q][0];
+#pragma unroll
+                        for (int d = 1; d < DIM; ++d)
+                            if (d == comp) vc = s_v[g][q][d];
+                        term = nk * vc;
+                    }
+                    const float y = term - cmp;
+                    const float tt = sum + y;
+                    cmp = (tt - sum) - y;
+                    sum = tt;
+                }
+            wave_sync();
         }
-        wave_sync();
-    }
-    if (lane < NACC) partial[(size_t)c * NACC + lane] = (double)sum - (double)cmp;
+        if (cv && gl < NACC) partial[(size_t)c * NACC + gl] = (double)sum - (double)cmp;
     }
 }
 
