@@ -49,7 +49,7 @@ EXPORTS = [
     "pf2_step", "pf2_set_walls", "pf2_sort_particles_into_subdomains_and_cells", "pf2_check_empty_cells",
     "pf2_update_vertices", "pf2_initialize_maps", "pf2_n_global_particles", "pf2_download_particles",
     "pf2_cell_counts", "pf2_particle_counter", "pf2_stats", "pf2_synchronize", "pf2_set_profiling",
-    "pf2_kernel_times", "pf2_kernel_name",
+    "pf2_kernel_times", "pf2_kernel_name", "pf2_step_info",
 ]
 
 
@@ -89,6 +89,7 @@ def load_library(path=LIB_PATH):
     lib.pf2_stats.argtypes = [vp, vp]
     lib.pf2_set_profiling.argtypes = [vp, ip]
     lib.pf2_kernel_times.argtypes = [vp, vp, vp, ip]
+    lib.pf2_step_info.argtypes = [vp, vp, ip]
     _ = dp
     _lib = lib
     return lib
@@ -258,6 +259,16 @@ class ParticleStage:
 
     def synchronize(self):
         self._check(self._lib.pf2_synchronize(self._h))
+
+    def step_info(self):
+        """Work counts of the last move: particles before/after, particles queued
+        for the vertex-star search per relocation round, particles that changed cell."""
+        out = np.zeros(8, dtype=np.int64)
+        k = self._lib.pf2_step_info(self._h, _ptr(out), 8)
+        if k < 0:
+            self._check(k)
+        return {"n_before": int(out[0]), "n_after": int(out[1]),
+                "queued": [int(v) for v in out[2:5]], "arrivals": int(out[5])}
 
     # -- per-kernel HIP-event timing --
     def set_profiling(self, on=True):

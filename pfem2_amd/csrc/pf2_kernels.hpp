@@ -130,7 +130,7 @@ struct DevWork {
     unsigned long long* hitmask;   // per cell: in-range parts that hold a particle
     int* nsurv;         // per cell: survivors
     double* partial;    // per cell: NV * (DIM + 1) projection sums
-    int* counters;      // [0] n_particles, [2] overflow flags
+    int* counters;      // [0] n_particles, [2] overflow flags, [3] count before the last regroup
     uint32_t* particle_counter;
     unsigned long long* stats;   // STAT_SHARDS rows: lost, deleted, spawned, searches, zero-weight
 };
@@ -826,6 +826,7 @@ __global__ void k_finalize(int n_cells, DevWork w, int* queue_count) {
     const int spawned = w.spawn_off[n_cells];
     w.stats[2] += (unsigned long long)spawned;
     *w.particle_counter += (uint32_t)spawned;
+    w.counters[3] = w.counters[0];   // count before the step (diagnostics)
     w.counters[0] = total;
 }
 

@@ -38,12 +38,12 @@ int fail(int code, const std::string& msg) {
 
 enum KernelId {
     K_MOVE = 0, K_GROUP, K_CELL_COUNT, K_CELL_WRITE, K_SCAN, K_FINALIZE, K_PARTIALS, K_NODE, K_WALL,
-    K_SEED, K_MISC, K_MOVE32, K_CELL_WRITE32, K_PARTIALS32, K_SEARCH, K_SEARCH32, K_NKERNELS
+    K_SEED, K_MISC, K_MOVE32, K_CELL_WRITE32, K_PARTIALS32, K_SEARCH, K_SEARCH32, K_QPREFIX, K_NKERNELS
 };
 const char* kKernelNames[K_NKERNELS] = {"k_move", "k_group", "k_cell_count", "k_cell_write", "scan",
                                         "k_finalize", "k_cell_partials", "k_node", "wall_fixup",
                                         "k_seed", "misc", "k_move32", "k_cell_write32", "k_cell_partials32",
-                                        "k_search_resume", "k_search_resume32"};
+                                        "k_search_resume", "k_search_resume32", "k_queue_prefix"};
 
 template <typename T>
 int dalloc(T** p, size_t n) {
@@ -372,7 +372,7 @@ int regroup(pf2_handle_s* h, int reseed) {
     DevParams pp = dev_params(h);
     DevWork w = dev_work(h);
     const int* old_off = h->off[h->cur_off];
-    if (int r = scan(h, h->arr_cnt, h->arr_off, nc)) return r;
+    if (int r = scan(h, h->arr_cnt, h->arr_off, nc + 1)) return r;
     {
         Timed t(h, K_GROUP);
         hipLaunchKernelGGL(k_group, dim3(grid_for(h->n_est, 256, 4096)), dim3(256), 0, h->stream,
@@ -424,8 +424,11 @@ int run_move(pf2_handle_s* h, int do_correct, int do_move) {
         }
         if (do_move)
             for (int r = 0; r <= MOVE_ROUNDS; ++r) {
+                {
+                    Timed tq(h, K_QPREFIX);
+                    hipLaunchKernelGGL(k_queue_prefix, dim3(1), dim3(QSHARDS), 0, h->stream, qu, r);
+                }
                 Timed t(h, K_SEARCH32);
-                hipLaunchKernelGGL(k_queue_prefix, dim3(1), dim3(QSHARDS), 0, h->stream, qu, r);
                 launch_search_resume32<DIM>(h->stream, r == 0 ? qgrid : qgrid / 4 + 1, dev_mesh<DIM>(h), h->dV32,
                                             h->dgeo32, b.cell, b.xi32, b.vext, h->cap, pp, w, qu, r);
             }
@@ -437,8 +440,11 @@ int run_move(pf2_handle_s* h, int do_correct, int do_move) {
         }
         if (do_move)
             for (int r = 0; r <= MOVE_ROUNDS; ++r) {
+                {
+                    Timed tq(h, K_QPREFIX);
+                    hipLaunchKernelGGL(k_queue_prefix, dim3(1), dim3(QSHARDS), 0, h->stream, qu, r);
+                }
                 Timed t(h, K_SEARCH);
-                hipLaunchKernelGGL(k_queue_prefix, dim3(1), dim3(QSHARDS), 0, h->stream, qu, r);
                 hipLaunchKernelGGL(HIP_KERNEL_NAME(k_search_resume<DIM>), dim3(r == 0 ? qgrid : qgrid / 4 + 1),
                                    dim3(256), 0, h->stream, dev_mesh<DIM>(h), h->dV, dev_particles(h, h->cur), pp, w,
                                    qu, r);
@@ -1066,6 +1072,26 @@ int pf2_stats(pf2_handle h, unsigned long long* out5) {
         out5[k] = t;
     }
     return 0;
+}
+
+int pf2_step_info(pf2_handle h, long long* out, int cap) {
+    if (!h || !out) return fail(PF2_E_ARG, "NULL argument");
+    std::vector<int> pref((size_t)(MOVE_ROUNDS + 2) * (QSHARDS + 2));
+    int cnt[4];
+    std::vector<int> offs(2);
+    HIPC(hipMemcpyAsync(pref.data(), h->qprefix, pref.size() * 4, hipMemcpyDeviceToHost, h->stream));
+    HIPC(hipMemcpyAsync(cnt, h->counters, sizeof(cnt), hipMemcpyDeviceToHost, h->stream));
+    int arrivals = 0;
+    HIPC(hipMemcpyAsync(&arrivals, h->arr_off + h->n_cells, 4, hipMemcpyDeviceToHost, h->stream));
+    HIPC(hipStreamSynchronize(h->stream));
+    long long vals[6];
+    vals[0] = cnt[3];
+    vals[1] = cnt[0];
+    for (int r = 0; r <= MOVE_ROUNDS; ++r) vals[2 + r] = pref[(size_t)r * (QSHARDS + 2) + QSHARDS + 1];
+    vals[5] = arrivals;
+    int k = 0;
+    for (; k < 6 && k < cap; ++k) out[k] = vals[k];
+    return k;
 }
 
 int pf2_synchronize(pf2_handle h) {
