@@ -94,6 +94,10 @@ struct DevMesh {
     const int* dof_off;
     const int* dof_cell;   // packed: cell * NV + local vertex
     const double* ftol;    // per cell: tolerance of the clearly-outside test (-1: none)
+    const double* cellX;   // per cell: its vertex coordinates packed [NV][DIM] (copies of X)
+    const int* cell_star;  // per cell and local vertex: offset of that vertex's star
+    const double* cellV;   // per cell: V at its DoFs [NV][DIM] (refreshed when V changes)
+    const double* cellDV;  // per cell: V - V_old at its DoFs [NV][DIM]
 };
 
 struct DevParticles {
@@ -135,24 +139,35 @@ struct DevWork {
     unsigned long long* stats;   // STAT_SHARDS rows: lost, deleted, spawned, searches, zero-weight
 };
 
+// vertex coordinates and nodal velocities of cell c, from the packed per-cell copies
 template <int DIM>
-__device__ __forceinline__ void load_cell(const DevMesh<DIM>& m, const double* __restrict__ F, int c,
-                                          double (*P)[DIM], double (*Fc)[DIM]) {
+__device__ __forceinline__ void load_cell(const DevMesh<DIM>& m, int c, double (*P)[DIM], double (*Fc)[DIM]) {
     constexpr int NV = Q1<DIM>::NV;
+    const double* cx = m.cellX + (size_t)c * NV * DIM;
+    const double* cf = m.cellV + (size_t)c * NV * DIM;
 #pragma unroll
-    for (int k = 0; k < NV; ++k) {
-        const int gv = m.cv[(size_t)c * NV + k];
-        const int dof = m.cd[(size_t)c * NV + k];
+    for (int k = 0; k < NV; ++k)
 #pragma unroll
         for (int d = 0; d < DIM; ++d) {
-            P[k][d] = m.X[(size_t)gv * DIM + d];
-            Fc[k][d] = F[(size_t)dof * DIM + d];
+            P[k][d] = cx[k * DIM + d];
+            Fc[k][d] = cf[k * DIM + d];
         }
-    }
 }
 
+// vertex coordinates of cell c (packed per-cell copy: one dependent load level)
 template <int DIM>
 __device__ __forceinline__ void load_vertices(const DevMesh<DIM>& m, int c, double (*P)[DIM]) {
+    constexpr int NV = Q1<DIM>::NV;
+    const double* cx = m.cellX + (size_t)c * NV * DIM;
+#pragma unroll
+    for (int k = 0; k < NV; ++k)
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) P[k][d] = cx[k * DIM + d];
+}
+
+// vertex coordinates gathered from X through the cell's vertex indices
+template <int DIM>
+__device__ __forceinline__ void gather_vertices(const DevMesh<DIM>& m, int c, double (*P)[DIM]) {
     constexpr int NV = Q1<DIM>::NV;
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
@@ -231,6 +246,8 @@ __device__ __forceinline__ void lattice_point(const DevParams& pp, int ordinal, 
 // that is the containing cell with the largest key, earliest star position on
 // ties, which is what the loop below selects.  Returns -1 if no star cell
 // contains the point.
+constexpr int MAXSTAR = 16;
+
 template <int DIM>
 __device__ int relocate(const DevMesh<DIM>& m, int c, const double (*P)[DIM], const double* xi_old,
                         const double* vext, const double* x, double* xi_out) {
@@ -260,14 +277,56 @@ __device__ int relocate(const DevMesh<DIM>& m, int c, const double (*P)[DIM], co
             best = a;
         }
     }
-    const int gv = m.cv[(size_t)c * NV + cvk];
     double dir[DIM];
 #pragma unroll
-    for (int d = 0; d < DIM; ++d) dir[d] = x[d] - m.X[(size_t)gv * DIM + d];
+    for (int d = 0; d < DIM; ++d) dir[d] = x[d] - P[cvk][d];
     rn = 1.0 / vnorm<DIM>(dir);
 #pragma unroll
     for (int d = 0; d < DIM; ++d) dir[d] *= rn;
-    const int s0 = m.star_off[gv], s1 = m.star_off[gv + 1];
+    const int s0 = m.cell_star[(size_t)c * NV + cvk];
+    const int gv = m.cv[(size_t)c * NV + cvk];
+    const int s1 = m.star_off[gv + 1];
+    const int n = s1 - s0;
+    if (n <= MAXSTAR) {
+        // candidates in decreasing key order (ties: star order), first
+        // containing cell wins -- the reference's sorted loop, stopping early
+        double key[MAXSTAR];
+        int scell[MAXSTAR];
+#pragma unroll
+        for (int e = 0; e < MAXSTAR; ++e) {
+            if (e < n) {
+                double sd[DIM];
+#pragma unroll
+                for (int d = 0; d < DIM; ++d) sd[d] = m.star_dir[(size_t)(s0 + e) * DIM + d];
+                key[e] = vdot<DIM>(sd, dir);
+                scell[e] = m.star_cell[s0 + e];
+            } else {
+                key[e] = 0.0;
+                scell[e] = -1;
+            }
+        }
+        unsigned tried = 0;
+        for (int it = 0; it < n; ++it) {
+            int best = -1, cc = -1;
+            double bk = 0.0;
+#pragma unroll
+            for (int e = 0; e < MAXSTAR; ++e)
+                if (e < n && !((tried >> e) & 1u) && (best < 0 || key[e] > bk)) {
+                    best = e;
+                    bk = key[e];
+                    cc = scell[e];
+                }
+            tried |= 1u << best;
+            double Q[NV][DIM], r[DIM];
+            load_vertices<DIM>(m, cc, Q);
+            if (inverse_map<DIM>(Q, x, r, m.ftol[cc], true) && inside_unit<DIM>(r)) {
+#pragma unroll
+                for (int d = 0; d < DIM; ++d) xi_out[d] = r[d];
+                return cc;
+            }
+        }
+        return -1;
+    }
     int bestc = -1;
     double bestkey = 0.0;
     for (int e = s0; e < s1; ++e) {
@@ -426,9 +485,9 @@ template <int DIM>
 __device__ __forceinline__ void finish_particle(const DevParams& pp, const DevParticles& A, const DevWork& w,
                                                 bool valid, int i, int c, const double* x, const double* xi,
                                                 bool write_state) {
-    const bool arrive = valid && c >= 0 && c != A.cell[i < 0 ? 0 : i];
-    wave_inc_by_key(w.arr_cnt, arrive, c);
     if (!valid) return;
+    // arrival count: one non-returning atomic per lane (addresses are spread over cells)
+    if (c >= 0 && c != A.cell[i]) atomicAdd(&w.arr_cnt[c], 1);
     if (write_state) {
 #pragma unroll
         for (int d = 0; d < DIM; ++d) {
@@ -479,13 +538,11 @@ __global__ void __launch_bounds__(256) k_move(DevMesh<DIM> m, const double* __re
 #pragma unroll
             for (int d = 0; d < DIM; ++d) v[d] = A.v[(size_t)d * A.cap + i];
             shape<DIM>(xi, N);
+            const double* dv = m.cellDV + (size_t)c * NV * DIM;
 #pragma unroll
-            for (int k = 0; k < NV; ++k) {
-                const int dof = m.cd[(size_t)c * NV + k];
+            for (int k = 0; k < NV; ++k)
 #pragma unroll
-                for (int d = 0; d < DIM; ++d)
-                    v[d] = v[d] + N[k] * (V[(size_t)dof * DIM + d] - Vold[(size_t)dof * DIM + d]);
-            }
+                for (int d = 0; d < DIM; ++d) v[d] = v[d] + N[k] * dv[k * DIM + d];
 #pragma unroll
             for (int d = 0; d < DIM; ++d) A.v[(size_t)d * A.cap + i] = v[d];
         }
@@ -495,7 +552,7 @@ __global__ void __launch_bounds__(256) k_move(DevMesh<DIM> m, const double* __re
 #pragma unroll
             for (int d = 0; d < DIM; ++d) x[d] = A.x[(size_t)d * A.cap + i];
             double P[NV][DIM], Vc[NV][DIM];
-            load_cell<DIM>(m, V, c, P, Vc);
+            load_cell<DIM>(m, c, P, Vc);
             st = advance64<DIM>(m, V, pp, A, i, c, x, xi, vext, s, P, Vc, m.ftol[c]);
         }
         park_particle<DIM>(A, w, qu, 0, valid && st == 1, i, c, x, xi, vext, s);
@@ -548,7 +605,7 @@ __global__ void __launch_bounds__(256) k_search_resume(DevMesh<DIM> m, const dou
                 for (int d = 0; d < DIM; ++d) xi[d] = r[d];
                 ++s;
                 double Vc[NV][DIM];
-                load_cell<DIM>(m, V, c, P, Vc);
+                load_cell<DIM>(m, c, P, Vc);
                 st = advance64<DIM>(m, V, pp, A, i, c, x, xi, vext, s, P, Vc, m.ftol[c]);
                 if (st == 0 || !last) break;
             }
@@ -781,7 +838,7 @@ __device__ __forceinline__ void cell_write_one(const DevMesh<DIM>& m, const doub
             const int r = __popcll(empty & ((1ull << o) - 1));
             double xi[DIM], P[NV][DIM], Vc[NV][DIM], x[DIM], v[DIM];
             lattice_point<DIM>(pp, o, xi);
-            load_cell<DIM>(m, V, c, P, Vc);
+            load_cell<DIM>(m, c, P, Vc);
             forward<DIM>(P, xi, x);
             interp<DIM>(Vc, xi, v);
             const long long q = out0 + nsurv + r;
@@ -1039,7 +1096,7 @@ __global__ void k_seed(DevMesh<DIM> m, const double* __restrict__ V, DevParticle
         const int o = (int)(i - (long long)c * pp.nparts_in);
         double xi[DIM], P[NV][DIM], Vc[NV][DIM], x[DIM], v[DIM];
         lattice_point<DIM>(pp, o, xi);
-        load_cell<DIM>(m, V, c, P, Vc);
+        load_cell<DIM>(m, c, P, Vc);
         forward<DIM>(P, xi, x);
         interp<DIM>(Vc, xi, v);
         A.cell[i] = c;
@@ -1067,7 +1124,7 @@ __global__ void k_init_particles(DevMesh<DIM> m, const double* __restrict__ V, D
         double xi[DIM], P[NV][DIM], Vc[NV][DIM], x[DIM], v[DIM];
 #pragma unroll
         for (int d = 0; d < DIM; ++d) xi[d] = A.xi[(size_t)d * A.cap + i];
-        load_cell<DIM>(m, V, c, P, Vc);
+        load_cell<DIM>(m, c, P, Vc);
         forward<DIM>(P, xi, x);
 #pragma unroll
         for (int d = 0; d < DIM; ++d) A.x[(size_t)d * A.cap + i] = x[d];
@@ -1120,7 +1177,7 @@ __global__ void __launch_bounds__(256) k_resort(DevMesh<DIM> m, DevParticles A, 
         w.dest[i] = c;
         if (c >= 0) w.pkey[i] = part_key<DIM>(pp, xi);
         }
-        wave_inc_by_key(w.arr_cnt, valid && c >= 0 && c != c0, c);
+        if (valid && c >= 0 && c != c0) atomicAdd(&w.arr_cnt[c], 1);
     }
     wave_stat(w.stats, 0, lost);
     wave_stat(w.stats, 3, searches);
@@ -1136,6 +1193,25 @@ __global__ void k_keys(DevParticles A, DevParams pp, DevWork w) {
         for (int d = 0; d < DIM; ++d) xi[d] = A.xi[(size_t)d * A.cap + i];
         w.dest[i] = A.cell[i];
         w.pkey[i] = part_key<DIM>(pp, xi);
+    }
+}
+
+// per-cell copies of V and V - V_old at the cell's DoFs (one load level in
+// the particle kernels); V - V_old is the same per-node difference the
+// correction forms (TurekBenchmark/pfem2particle.cpp:806-807)
+template <int DIM>
+__global__ void k_pack_fields(DevMesh<DIM> m, const double* __restrict__ V, const double* __restrict__ Vold,
+                              double* __restrict__ cellV, double* __restrict__ cellDV) {
+    constexpr int NV = Q1<DIM>::NV;
+    const long long n = (long long)m.n_cells * NV;
+    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x) {
+        const int dof = m.cd[t];
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) {
+            const double a = V[(size_t)dof * DIM + d], b = Vold[(size_t)dof * DIM + d];
+            cellV[(size_t)t * DIM + d] = a;
+            cellDV[(size_t)t * DIM + d] = a - b;
+        }
     }
 }
 

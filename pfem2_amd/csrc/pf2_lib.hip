@@ -77,7 +77,7 @@ struct pf2_handle_s {
     hipStream_t stream = nullptr;
     // host copies
     std::vector<double> X;
-    std::vector<int> cv, cd, star_off, star_cell, dof_off, dof_cell;
+    std::vector<int> cv, cd, star_off, star_cell, dof_off, dof_cell, cell_star;
     std::vector<double> star_dir;
     // device mesh
     double* dX = nullptr;
@@ -85,6 +85,9 @@ struct pf2_handle_s {
     double* dstar_dir = nullptr;
     int *ddof_off = nullptr, *ddof_cell = nullptr;
     double* dftol = nullptr;
+    double* dcellX = nullptr;
+    int* dcell_star = nullptr;
+    double *dcellV = nullptr, *dcellDV = nullptr;
     // fields (node-major)
     double *dV = nullptr, *dVold = nullptr, *dtmp = nullptr;
     // mixed-precision fields and geometry
@@ -180,6 +183,8 @@ int cell_grid(int n_cells) {
     return g < 2048 ? (g < 1 ? 1 : g) : 2048;
 }
 
+int pack_fields(pf2_handle_s* h);
+
 int grid_for(long long n, int block, int cap_blocks = 65535 * 4) {
     long long g = (n + block - 1) / block;
     if (g < 1) g = 1;
@@ -202,6 +207,10 @@ DevMesh<DIM> dev_mesh(pf2_handle_s* h) {
     m.dof_off = h->ddof_off;
     m.dof_cell = h->ddof_cell;
     m.ftol = h->dftol;
+    m.cellX = h->dcellX;
+    m.cell_star = h->dcell_star;
+    m.cellV = h->dcellV;
+    m.cellDV = h->dcellDV;
     return m;
 }
 
@@ -498,9 +507,9 @@ int run_project(pf2_handle_s* h) {
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_wall_set<DIM>), dim3(grid_for(h->n_walls, 256)), dim3(256), 0, h->stream,
                            wl, h->dV, h->dVold);
     }
-    if (mixed(h)) {
+    {
         Timed t(h, K_MISC);
-        launch_fields_to32<DIM>(h->stream, h->dV, h->dVold, h->dV32, h->dVold32, h->n_dofs);
+        if (int r = pack_fields(h)) return r;
     }
     HIPC(hipGetLastError());
     return 0;
@@ -535,10 +544,24 @@ int upload_field(pf2_handle_s* h, const double* src, double* dst, hipMemcpyKind 
     return 0;
 }
 
-int sync_fields32(pf2_handle_s* h) {
-    if (!mixed(h)) return 0;
-    if (h->dim == 2) launch_fields_to32<2>(h->stream, h->dV, h->dVold, h->dV32, h->dVold32, h->n_dofs);
-    else launch_fields_to32<3>(h->stream, h->dV, h->dVold, h->dV32, h->dVold32, h->n_dofs);
+// per-cell copies of V and V - V_old (FP64, and FP32 for the mixed path);
+// called whenever V or V_old change
+int pack_fields(pf2_handle_s* h) {
+    const long long n = (long long)h->n_cells * h->nv;
+    const int g = grid_for(n, 256, 4096);
+    if (h->dim == 2) {
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pack_fields<2>), dim3(g), dim3(256), 0, h->stream, dev_mesh<2>(h), h->dV,
+                           h->dVold, h->dcellV, h->dcellDV);
+        if (mixed(h))
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pack_fields32<2>), dim3(g), dim3(256), 0, h->stream, dev_mesh<2>(h),
+                               h->dV, h->dVold, h->dV32, h->dVold32);
+    } else {
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pack_fields<3>), dim3(g), dim3(256), 0, h->stream, dev_mesh<3>(h), h->dV,
+                           h->dVold, h->dcellV, h->dcellDV);
+        if (mixed(h))
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pack_fields32<3>), dim3(g), dim3(256), 0, h->stream, dev_mesh<3>(h),
+                               h->dV, h->dVold, h->dV32, h->dVold32);
+    }
     HIPC(hipGetLastError());
     return 0;
 }
@@ -616,6 +639,8 @@ int pf2_create(const pf2_mesh* mesh, const pf2_params* params, long long capacit
             for (int k = 0; k < h->nv; ++k) h->star_cell[fill[h->cv[(size_t)c * h->nv + k]]++] = c;
     }
     compute_star_dirs(h);
+    h->cell_star.resize(nck);
+    for (size_t i = 0; i < nck; ++i) h->cell_star[i] = h->star_off[h->cv[i]];
     // DoF -> (cell, local vertex), ascending cells
     h->dof_off.assign(h->n_dofs + 1, 0);
     for (size_t i = 0; i < nck; ++i) h->dof_off[h->cd[i] + 1]++;
@@ -636,7 +661,8 @@ int pf2_create(const pf2_mesh* mesh, const pf2_params* params, long long capacit
     if (dalloc(&h->dX, (size_t)nvt * dim) || dalloc(&h->dcv, nck) || dalloc(&h->dcd, nck) ||
         dalloc(&h->dstar_off, nvt + 1) || dalloc(&h->dstar_cell, h->star_cell.size()) ||
         dalloc(&h->dstar_dir, h->star_dir.size()) || dalloc(&h->ddof_off, nd + 1) ||
-        dalloc(&h->ddof_cell, h->dof_cell.size()) || dalloc(&h->dftol, nc) || dalloc(&h->dV, (size_t)nd * dim) ||
+        dalloc(&h->ddof_cell, h->dof_cell.size()) || dalloc(&h->dftol, nc) || dalloc(&h->dcellX, nck * dim) || dalloc(&h->dcell_star, nck) ||
+        dalloc(&h->dcellV, nck * dim) || dalloc(&h->dcellDV, nck * dim) || dalloc(&h->dV, (size_t)nd * dim) ||
         dalloc(&h->dVold, (size_t)nd * dim) || dalloc(&h->dtmp, (size_t)nd * dim) || dalloc(&h->off[0], nc + 1) ||
         dalloc(&h->off[1], nc + 1) || dalloc(&h->arr_cnt, nc + 1) || dalloc(&h->arr_off, nc + 1) ||
         dalloc(&h->arr_fill, nc + 1) || dalloc(&h->final_cnt, nc + 1) || dalloc(&h->spawn_cnt, nc + 1) ||
@@ -647,7 +673,7 @@ int pf2_create(const pf2_mesh* mesh, const pf2_params* params, long long capacit
         dalloc(&h->pcounter, 1) || dalloc(&h->stats, STAT_SHARDS * NSTAT))
         return bail(PF2_E_HIP);
     if (mixed(h) || true) {
-        if (dalloc(&h->dV32, (size_t)nd * dim) || dalloc(&h->dVold32, (size_t)nd * dim) ||
+        if (dalloc(&h->dV32, nck * dim) || dalloc(&h->dVold32, nck * dim) ||
             dalloc(&h->dgeo32, (size_t)nc * geo32_floats(dim)))
             return bail(PF2_E_HIP);
     }
@@ -660,7 +686,8 @@ int pf2_create(const pf2_mesh* mesh, const pf2_params* params, long long capacit
         up(h->dstar_cell, h->star_cell.data(), h->star_cell.size() * 4) ||
         up(h->dstar_dir, h->star_dir.data(), h->star_dir.size() * 8) ||
         up(h->ddof_off, h->dof_off.data(), h->dof_off.size() * 4) ||
-        up(h->ddof_cell, h->dof_cell.data(), h->dof_cell.size() * 4))
+        up(h->ddof_cell, h->dof_cell.data(), h->dof_cell.size() * 4) ||
+        up(h->dcell_star, h->cell_star.data(), h->cell_star.size() * 4))
         return bail(fail(PF2_E_HIP, "mesh upload failed"));
     if (hipMemsetAsync(h->dV, 0, sizeof(double) * nd * dim, h->stream) ||
         hipMemsetAsync(h->dVold, 0, sizeof(double) * nd * dim, h->stream) ||
@@ -679,8 +706,9 @@ int pf2_create(const pf2_mesh* mesh, const pf2_params* params, long long capacit
         hipSuccess)
         return bail(fail(PF2_E_HIP, "scan sizing failed"));
     if (dalloc((char**)&h->scan_tmp, h->scan_tmp_bytes)) return bail(PF2_E_HIP);
-    if (h->dim == 2) launch_geo<2>(h->stream, dev_mesh<2>(h), h->dgeo32, h->dftol);
-    else launch_geo<3>(h->stream, dev_mesh<3>(h), h->dgeo32, h->dftol);
+    if (h->dim == 2) launch_geo<2>(h->stream, dev_mesh<2>(h), h->dgeo32, h->dftol, h->dcellX);
+    else launch_geo<3>(h->stream, dev_mesh<3>(h), h->dgeo32, h->dftol, h->dcellX);
+    if (pack_fields(h)) return bail(PF2_E_HIP);
     if (hipStreamSynchronize(h->stream) != hipSuccess) return bail(fail(PF2_E_HIP, "initialisation sync failed"));
     *out = h;
     return 0;
@@ -691,7 +719,7 @@ int pf2_destroy(pf2_handle h) {
     if (h->stream) hipStreamSynchronize(h->stream);
     free_particles(h);
     void* ptrs[] = {h->dX, h->dcv, h->dcd, h->dstar_off, h->dstar_cell, h->dstar_dir, h->ddof_off, h->ddof_cell,
-                    h->dftol, h->dV, h->dVold, h->dtmp, h->dV32, h->dVold32, h->dgeo32, h->off[0], h->off[1], h->arr_cnt,
+                    h->dftol, h->dcellX, h->dcell_star, h->dcellV, h->dcellDV, h->dV, h->dVold, h->dtmp, h->dV32, h->dVold32, h->dgeo32, h->off[0], h->off[1], h->arr_cnt,
                     h->arr_off, h->arr_fill, h->final_cnt, h->spawn_cnt, h->spawn_off, h->hitmask, h->nsurv,
                     h->partial, h->counters, h->qcount, h->qprefix, h->pcounter, h->stats, h->scan_tmp, h->w_dof, h->w_mask, h->w_fix,
                     h->w_vwall, h->w_cells, h->w_bc, h->w_val};
@@ -729,7 +757,7 @@ int pf2_upload_fields(pf2_handle h, const double* V, const double* V_old) {
         if (int r = upload_field(h, V, h->dV, hipMemcpyHostToDevice)) return r;
     if (V_old)
         if (int r = upload_field(h, V_old, h->dVold, hipMemcpyHostToDevice)) return r;
-    return sync_fields32(h);
+    return pack_fields(h);
 }
 
 int pf2_upload_fields_device(pf2_handle h, const double* V, const double* V_old) {
@@ -738,7 +766,7 @@ int pf2_upload_fields_device(pf2_handle h, const double* V, const double* V_old)
         if (int r = upload_field(h, V, h->dV, hipMemcpyDeviceToDevice)) return r;
     if (V_old)
         if (int r = upload_field(h, V_old, h->dVold, hipMemcpyDeviceToDevice)) return r;
-    return sync_fields32(h);
+    return pack_fields(h);
 }
 
 int pf2_download_field(pf2_handle h, double* V) {
@@ -1025,8 +1053,8 @@ int pf2_update_vertices(pf2_handle h, const double* vertices, int recompute_maps
         compute_star_dirs(h);
         if (int r = upload_star_dirs(h)) return r;
     }
-    if (h->dim == 2) launch_geo<2>(h->stream, dev_mesh<2>(h), h->dgeo32, h->dftol);
-    else launch_geo<3>(h->stream, dev_mesh<3>(h), h->dgeo32, h->dftol);
+    if (h->dim == 2) launch_geo<2>(h->stream, dev_mesh<2>(h), h->dgeo32, h->dftol, h->dcellX);
+    else launch_geo<3>(h->stream, dev_mesh<3>(h), h->dgeo32, h->dftol, h->dcellX);
     HIPC(hipGetLastError());
     HIPC(hipStreamSynchronize(h->stream));
     return 0;

@@ -27,7 +27,7 @@ __global__ void k_geo32(DevMesh<DIM> m, float* __restrict__ geo) {
     constexpr int NG = DIM == 2 ? 6 : 21;
     for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < m.n_cells; c += gridDim.x * blockDim.x) {
         double P[NV][DIM];
-        load_vertices<DIM>(m, c, P);
+        gather_vertices<DIM>(m, c, P);
         float* g = geo + (size_t)c * NG;
         if constexpr (DIM == 2) {
             for (int d = 0; d < 2; ++d) {
@@ -53,18 +53,22 @@ __global__ void k_geo32(DevMesh<DIM> m, float* __restrict__ geo) {
 
 // per-cell FP64 tolerance of the clearly-outside test
 template <int DIM>
-__global__ void k_ftol(DevMesh<DIM> m, double* __restrict__ ftol) {
+__global__ void k_ftol(DevMesh<DIM> m, double* __restrict__ ftol, double* __restrict__ cellX) {
     for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < m.n_cells; c += gridDim.x * blockDim.x) {
         double P[Q1<DIM>::NV][DIM];
-        load_vertices<DIM>(m, c, P);
+        gather_vertices<DIM>(m, c, P);
         ftol[c] = outside_tol<DIM>(P);
+#pragma unroll
+        for (int k = 0; k < Q1<DIM>::NV; ++k)
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) cellX[((size_t)c * Q1<DIM>::NV + k) * DIM + d] = P[k][d];
     }
 }
 
 template <int DIM>
-void launch_geo(hipStream_t s, DevMesh<DIM> m, float* geo, double* ftol) {
+void launch_geo(hipStream_t s, DevMesh<DIM> m, float* geo, double* ftol, double* cellX) {
     hipLaunchKernelGGL(HIP_KERNEL_NAME(k_geo32<DIM>), dim3((m.n_cells + 255) / 256), dim3(256), 0, s, m, geo);
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_ftol<DIM>), dim3((m.n_cells + 255) / 256), dim3(256), 0, s, m, ftol);
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_ftol<DIM>), dim3((m.n_cells + 255) / 256), dim3(256), 0, s, m, ftol, cellX);
 }
 
 template <int DIM>
@@ -78,48 +82,56 @@ __device__ __forceinline__ void shape32(const float* xi, float* N) {
     }
 }
 
-// G(delta) = F(xi + delta) - F(xi) and J(xi + delta), from the coefficients
+// J(xi) (Jacobian of F_K at the current reference point) and the nonlinear
+// part Q of the local difference:
+//   F(xi + delta) - F(xi) = J(xi) delta + Q(delta)
+//   2-D: Q = c d0 d1
+//   3-D: Q = d d0 d1 + e d0 d2 + f d1 d2 + g (xi0 d1 d2 + xi1 d0 d2 + xi2 d0 d1 + d0 d1 d2)
 template <int DIM>
-__device__ __forceinline__ void diff_and_jac(const float* g, const float* xi, const float* dl, float* G,
-                                             float (*J)[DIM]) {
+__device__ __forceinline__ void jac32(const float* g, const float* xi, float (*J)[DIM]) {
     if constexpr (DIM == 2) {
-        const float e0 = xi[0] + dl[0], e1 = xi[1] + dl[1];
-        // e0 e1 - xi0 xi1 = dl0 xi1 + xi0 dl1 + dl0 dl1
-        const float p01 = __fmaf_rn(dl[0], xi[1], __fmaf_rn(xi[0], dl[1], dl[0] * dl[1]));
 #pragma unroll
         for (int d = 0; d < 2; ++d) {
-            const float a = g[0 + d], b = g[2 + d], c = g[4 + d];
-            G[d] = __fmaf_rn(a, dl[0], __fmaf_rn(b, dl[1], c * p01));
-            J[d][0] = __fmaf_rn(c, e1, a);
-            J[d][1] = __fmaf_rn(c, e0, b);
+            J[d][0] = __fmaf_rn(g[4 + d], xi[1], g[0 + d]);
+            J[d][1] = __fmaf_rn(g[4 + d], xi[0], g[2 + d]);
         }
     } else {
-        const float e0 = xi[0] + dl[0], e1 = xi[1] + dl[1], e2 = xi[2] + dl[2];
-        const float p01 = __fmaf_rn(dl[0], xi[1], __fmaf_rn(xi[0], dl[1], dl[0] * dl[1]));
-        const float p02 = __fmaf_rn(dl[0], xi[2], __fmaf_rn(xi[0], dl[2], dl[0] * dl[2]));
-        const float p12 = __fmaf_rn(dl[1], xi[2], __fmaf_rn(xi[1], dl[2], dl[1] * dl[2]));
-        // e0 e1 e2 - xi0 xi1 xi2 = p01 * e2 + xi0 xi1 dl2
-        const float p012 = __fmaf_rn(p01, e2, xi[0] * xi[1] * dl[2]);
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
             const float a = g[0 + d], b = g[3 + d], c = g[6 + d];
             const float dd = g[9 + d], ee = g[12 + d], ff = g[15 + d], gg = g[18 + d];
-            G[d] = a * dl[0] + b * dl[1] + c * dl[2] + dd * p01 + ee * p02 + ff * p12 + gg * p012;
-            J[d][0] = a + dd * e1 + ee * e2 + gg * e1 * e2;
-            J[d][1] = b + dd * e0 + ff * e2 + gg * e0 * e2;
-            J[d][2] = c + ee * e0 + ff * e1 + gg * e0 * e1;
+            J[d][0] = a + dd * xi[1] + ee * xi[2] + gg * xi[1] * xi[2];
+            J[d][1] = b + dd * xi[0] + ff * xi[2] + gg * xi[0] * xi[2];
+            J[d][2] = c + ee * xi[0] + ff * xi[1] + gg * xi[0] * xi[1];
         }
     }
 }
 
 template <int DIM>
-__device__ __forceinline__ bool solve32(const float (*J)[DIM], const float* r, float* x) {
+__device__ __forceinline__ void nonlin32(const float* g, const float* xi, const float* dl, float* Q) {
+    if constexpr (DIM == 2) {
+        const float p01 = dl[0] * dl[1];
+#pragma unroll
+        for (int d = 0; d < 2; ++d) Q[d] = g[4 + d] * p01;
+    } else {
+        const float p01 = dl[0] * dl[1], p02 = dl[0] * dl[2], p12 = dl[1] * dl[2];
+        const float cub = xi[0] * p12 + xi[1] * p02 + xi[2] * p01 + p01 * dl[2];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) Q[d] = g[9 + d] * p01 + g[12 + d] * p02 + g[15 + d] * p12 + g[18 + d] * cub;
+    }
+}
+
+// inverse of J by the adjugate and the hardware reciprocal; false if det <= 0
+template <int DIM>
+__device__ __forceinline__ bool inv32(const float (*J)[DIM], float (*Ji)[DIM]) {
     if constexpr (DIM == 2) {
         const float det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
         if (!(det > 0.0f)) return false;
-        const float inv = 1.0f / det;
-        x[0] = (J[1][1] * r[0] - J[0][1] * r[1]) * inv;
-        x[1] = (J[0][0] * r[1] - J[1][0] * r[0]) * inv;
+        const float r = __builtin_amdgcn_rcpf(det);
+        Ji[0][0] = J[1][1] * r;
+        Ji[0][1] = -J[0][1] * r;
+        Ji[1][0] = -J[1][0] * r;
+        Ji[1][1] = J[0][0] * r;
         return true;
     } else {
         const float c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
@@ -127,51 +139,73 @@ __device__ __forceinline__ bool solve32(const float (*J)[DIM], const float* r, f
         const float c02 = J[0][1] * J[1][2] - J[0][2] * J[1][1];
         const float det = J[0][0] * c00 + J[1][0] * c01 + J[2][0] * c02;
         if (!(det > 0.0f)) return false;
-        const float inv = 1.0f / det;
-        const float c10 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
-        const float c11 = J[0][0] * J[2][2] - J[0][2] * J[2][0];
-        const float c12 = J[0][2] * J[1][0] - J[0][0] * J[1][2];
-        const float c20 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
-        const float c21 = J[0][1] * J[2][0] - J[0][0] * J[2][1];
-        const float c22 = J[0][0] * J[1][1] - J[0][1] * J[1][0];
-        x[0] = (c00 * r[0] + c01 * r[1] + c02 * r[2]) * inv;
-        x[1] = (c10 * r[0] + c11 * r[1] + c12 * r[2]) * inv;
-        x[2] = (c20 * r[0] + c21 * r[1] + c22 * r[2]) * inv;
+        const float r = __builtin_amdgcn_rcpf(det);
+        Ji[0][0] = c00 * r;
+        Ji[0][1] = c01 * r;
+        Ji[0][2] = c02 * r;
+        Ji[1][0] = (J[1][2] * J[2][0] - J[1][0] * J[2][2]) * r;
+        Ji[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * r;
+        Ji[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * r;
+        Ji[2][0] = (J[1][0] * J[2][1] - J[1][1] * J[2][0]) * r;
+        Ji[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * r;
+        Ji[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * r;
         return true;
     }
 }
 
-// delta with F(xi + delta) - F(xi) = dx; three Newton steps from delta = 0
 template <int DIM>
-__device__ __forceinline__ bool local_step(const float* g, const float* xi, const float* dx, float* dl) {
+__device__ __forceinline__ void mv32(const float (*M)[DIM], const float* v, float* r) {
 #pragma unroll
-    for (int d = 0; d < DIM; ++d) dl[d] = 0.0f;
+    for (int i = 0; i < DIM; ++i) {
+        float acc = M[i][0] * v[0];
 #pragma unroll
-    for (int it = 0; it < 3; ++it) {
-        float G[DIM], J[DIM][DIM], r[DIM], s[DIM];
-        diff_and_jac<DIM>(g, xi, dl, G, J);
-#pragma unroll
-        for (int d = 0; d < DIM; ++d) r[d] = dx[d] - G[d];
-        if (!solve32<DIM>(J, r, s)) return false;
-#pragma unroll
-        for (int d = 0; d < DIM; ++d) dl[d] += s[d];
+        for (int j = 1; j < DIM; ++j) acc = __fmaf_rn(M[i][j], v[j], acc);
+        r[i] = acc;
     }
-    return true;
 }
 
+// delta with F(xi + delta) - F(xi) = dx: delta = J^-1 (dx - Q(delta)) by
+// fixed-point iteration from delta = J^-1 dx (contraction ~ |Q'| / |J| ~
+// the step in cell units).  Returns false when J is not positive or the
+// final residual exceeds 1e-5 |dx| (the caller then takes the FP64 path).
 template <int DIM>
-__device__ __forceinline__ void load_cell32(const DevMesh<DIM>& m, const float* __restrict__ geo,
-                                            const float* __restrict__ F, int c, float* g, float (*Fc)[DIM]) {
+__device__ __forceinline__ bool local_step(const float* g, const float* xi, const float* dx, float* dl) {
+    float J[DIM][DIM], Ji[DIM][DIM];
+    jac32<DIM>(g, xi, J);
+    if (!inv32<DIM>(J, Ji)) return false;
+    mv32<DIM>(Ji, dx, dl);
+#pragma unroll
+    for (int it = 0; it < 2; ++it) {
+        float Q[DIM], rhs[DIM];
+        nonlin32<DIM>(g, xi, dl, Q);
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) rhs[d] = dx[d] - Q[d];
+        mv32<DIM>(Ji, rhs, dl);
+    }
+    float Q[DIM], Jd[DIM], r2 = 0.0f, dx2 = 0.0f;
+    nonlin32<DIM>(g, xi, dl, Q);
+    mv32<DIM>(J, dl, Jd);
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+        const float r = dx[d] - Jd[d] - Q[d];
+        r2 = __fmaf_rn(r, r, r2);
+        dx2 = __fmaf_rn(dx[d], dx[d], dx2);
+    }
+    return r2 <= 1e-10f * dx2 + 1e-36f;
+}
+
+// FP32 coefficients and nodal velocities of cell c (packed per-cell copies)
+template <int DIM>
+__device__ __forceinline__ void load_cell32(const float* __restrict__ geo, const float* __restrict__ cellV32, int c,
+                                            float* g, float (*Fc)[DIM]) {
     constexpr int NV = Q1<DIM>::NV;
     constexpr int NG = DIM == 2 ? 6 : 21;
 #pragma unroll
     for (int q = 0; q < NG; ++q) g[q] = geo[(size_t)c * NG + q];
 #pragma unroll
-    for (int k = 0; k < NV; ++k) {
-        const int dof = m.cd[(size_t)c * NV + k];
+    for (int k = 0; k < NV; ++k)
 #pragma unroll
-        for (int d = 0; d < DIM; ++d) Fc[k][d] = F[(size_t)dof * DIM + d];
-    }
+        for (int d = 0; d < DIM; ++d) Fc[k][d] = cellV32[((size_t)c * NV + k) * DIM + d];
 }
 
 template <int DIM>
@@ -223,9 +257,8 @@ __device__ __forceinline__ int advance32(const DevParams& pp, float h32, int i, 
 template <int DIM>
 __device__ __forceinline__ void finish32(const DevParams& pp, const int* __restrict__ pcell, float* __restrict__ pxi,
                                          long long cap, const DevWork& w, bool valid, int i, int c, const float* xi) {
-    const bool arrive = valid && c >= 0 && c != pcell[i < 0 ? 0 : i];
-    wave_inc_by_key(w.arr_cnt, arrive, c);
     if (!valid) return;
+    if (c >= 0 && c != pcell[i]) atomicAdd(&w.arr_cnt[c], 1);
 #pragma unroll
     for (int d = 0; d < DIM; ++d) pxi[(size_t)d * cap + i] = xi[d];
     w.dest[i] = c;
@@ -271,11 +304,9 @@ __global__ void __launch_bounds__(256) k_move32(DevMesh<DIM> m, const float* __r
             for (int d = 0; d < DIM; ++d) v[d] = pv[(size_t)d * cap + i];
             shape32<DIM>(xi, N);
 #pragma unroll
-            for (int k = 0; k < NV; ++k) {
-                const int dof = m.cd[(size_t)c * NV + k];
+            for (int k = 0; k < NV; ++k)
 #pragma unroll
-                for (int d = 0; d < DIM; ++d) v[d] = __fmaf_rn(N[k], dV32[(size_t)dof * DIM + d], v[d]);
-            }
+                for (int d = 0; d < DIM; ++d) v[d] = __fmaf_rn(N[k], dV32[((size_t)c * NV + k) * DIM + d], v[d]);
 #pragma unroll
             for (int d = 0; d < DIM; ++d) pv[(size_t)d * cap + i] = v[d];
         }
@@ -284,7 +315,7 @@ __global__ void __launch_bounds__(256) k_move32(DevMesh<DIM> m, const float* __r
         float u[DIM];
         if (valid) {
             float g[NG], Vc[NV][DIM];
-            load_cell32<DIM>(m, geo, V32, c, g, Vc);
+            load_cell32<DIM>(geo, V32, c, g, Vc);
             st = advance32<DIM>(pp, h32, i, pvext, cap, xi, u, s, g, Vc);
         }
         park32<DIM>(pxi, cap, w, qu, 0, valid && st == 1, i, c, xi, u, s);
@@ -350,7 +381,7 @@ __global__ void __launch_bounds__(256) k_search_resume32(DevMesh<DIM> m, const f
                 c = cn;
                 ++s;
                 float g[NG], Vc[NV][DIM];
-                load_cell32<DIM>(m, geo, V32, c, g, Vc);
+                load_cell32<DIM>(geo, V32, c, g, Vc);
                 st = advance32<DIM>(pp, h32, i, pvext, cap, xi, u, s, g, Vc);
                 if (st == 0 || !last) break;
             }
@@ -424,7 +455,7 @@ __device__ __forceinline__ void cell_write32_one(const DevMesh<DIM>& m, const do
             const int r = __popcll(empty & ((1ull << o) - 1));
             double xi[DIM], P[NV][DIM], Vc[NV][DIM], v[DIM];
             lattice_point<DIM>(pp, o, xi);
-            load_cell<DIM>(m, V, c, P, Vc);
+            load_cell<DIM>(m, c, P, Vc);
             interp<DIM>(Vc, xi, v);
             const long long q = out0 + nsurv + r;
             bcell[q] = c;
@@ -591,20 +622,21 @@ void launch_wall_particles32(hipStream_t s, DevMesh<DIM> m, DevWalls wl, const i
                        off, pxi, pv, cap);
 }
 
-// V32 = V, dV32 = V - V_old (node-major, FP32)
+// per-cell FP32 copies of V and V - V_old at the cell's DoFs
 template <int DIM>
-__global__ void k_fields_to32(const double* __restrict__ V, const double* __restrict__ Vold, float* __restrict__ V32,
-                              float* __restrict__ dV32, int n) {
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n * DIM; i += gridDim.x * blockDim.x) {
-        V32[i] = (float)V[i];
-        dV32[i] = (float)(V[i] - Vold[i]);
+__global__ void k_pack_fields32(DevMesh<DIM> m, const double* __restrict__ V, const double* __restrict__ Vold,
+                                float* __restrict__ cellV32, float* __restrict__ cellDV32) {
+    constexpr int NV = Q1<DIM>::NV;
+    const long long n = (long long)m.n_cells * NV;
+    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x) {
+        const int dof = m.cd[t];
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) {
+            const double a = V[(size_t)dof * DIM + d], b = Vold[(size_t)dof * DIM + d];
+            cellV32[(size_t)t * DIM + d] = (float)a;
+            cellDV32[(size_t)t * DIM + d] = (float)(a - b);
+        }
     }
-}
-
-template <int DIM>
-void launch_fields_to32(hipStream_t s, const double* V, const double* Vold, float* V32, float* dV32, int n) {
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_fields_to32<DIM>), dim3((n * DIM + 255) / 256), dim3(256), 0, s, V, Vold, V32,
-                       dV32, n);
 }
 
 __global__ void k_state_to32(int dim, const double* __restrict__ xi, const double* __restrict__ v,
