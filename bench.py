@@ -1,17 +1,27 @@
 """Benchmark of the PFEM2 particle stage on one MI355X.
 
-Workload (the metric's configuration that fits one GPU): 2-D channel with a
-circular obstacle (Turek-Schaefer geometry 2.2 x 0.41, cylinder r = 0.05),
-graded quad mesh of ~100k cells with seeded vertex jitter, 16 particles per
-cell (4x4 lattice, +-0.05 seeded jitter, ~1.6M particles at the start),
-velocity = potential flow around the cylinder + 1% seeded Fourier
+Default workload (the metric's configuration that fits one GPU): 2-D channel
+with a circular obstacle (Turek-Schaefer geometry 2.2 x 0.41, cylinder
+r = 0.05), graded quad mesh of ~100k cells with seeded vertex jitter, 16
+particles per cell (4x4 lattice, +-0.05 seeded jitter, ~1.6M particles at the
+start), velocity = potential flow around the cylinder + 1% seeded Fourier
 perturbation, dt = 0.001, 10 advection substeps, MAX = 2 particles per cell
 part.
 
 One step = correct -> move (10 substeps of advection + relocation, then
-reseed/delete) -> project, for every particle, plus the FEM stand-in (the new
-nodal V, a device-resident analytic field, copied into the stage).  All
-inputs are resident in HBM before the timed region.
+reseed/delete) -> project, for every particle, after the FEM stand-in (the
+new nodal V, a device-resident field, copied into the stage).  All inputs are
+resident in HBM before the timed region; steps are replayed from HIP graphs.
+
+The default run also serves the request stream of the latency metric
+(256 independent simulations, Poisson arrivals, 2k-200k cells, 9/16/25
+particles per cell, 50-500 steps; admission by HBM footprint) and reports its
+p50/p99 completion latency under "serving".
+
+Other workloads (--workload): channel (200x20 quads, FP64, against the CPU
+oracle over the whole run), pipe3d (823k hexahedra in a pipe, 2x2x2 lattice),
+moving (~400k quads around an oscillating cylinder, 25 particles per cell,
+mesh motion + re-sort every step), serve (the request stream alone).
 
 Prints one JSON line.  `value` is the FP32-mixed path's whole-job
 throughput; "paths" holds the FP64 path (bitwise equal to the CPU oracle)
@@ -19,6 +29,7 @@ and the per-field error of the FP32-mixed path against it.
 """
 import argparse
 import json
+import math
 import os
 import re
 import sys
@@ -29,9 +40,70 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-SEEDS = {"mesh": 2024, "field": 7, "particles": 11}
+SEEDS = {"mesh": 2024, "field": 7, "particles": 11, "serve": 99, "pipe": 31, "moving": 57}
 HBM_PEAK_GBS = 8000.0
-Q = [4, 4]
+DTYPE32 = "fp32-mixed (f32 reference coords/velocities, f64 relocation and node sums)"
+
+
+class Workload:
+    """Mesh, field, lattice and stepping parameters of one benchmark case."""
+
+    def __init__(self, name, m, V, q, dt, substeps, max_per_part, jitter=0.05, keep_vext=False, motion=None,
+                 desc=""):
+        self.name, self.m, self.V, self.q = name, m, V, q
+        self.dt, self.substeps, self.max_per_part = dt, substeps, max_per_part
+        self.jitter, self.keep_vext, self.motion, self.desc = jitter, keep_vext, motion, desc
+
+
+def cylinder_workload(args):
+    from pfem2_amd import mesh as M
+    m = M.cylinder_channel(args.scale, seed=SEEDS["mesh"])
+    V = M.cylinder_flow(m, 1.0, eps=0.01, seed=SEEDS["field"])
+    return Workload("cylinder", m, V, [4, 4], args.dt, args.substeps, args.max_per_part,
+                    desc="2D channel with circular obstacle (2.2x0.41, r=0.05), graded quad mesh, 16 particles/cell, "
+                         "potential flow + 1% Fourier perturbation, step = correct+move(10 substeps)+project")
+
+
+def channel_workload():
+    from pfem2_amd import mesh as M
+    m = M.channel_2d(200, 20, 10.0, 1.0)
+    V = M.poiseuille_2d(m, 1.0, 1.0, 0.01, seed=SEEDS["field"])
+    return Workload("channel", m, V, [4, 4], 0.01, 10, 3,
+                    desc="200x20 quads on [0,10]x[0,1], Poiseuille Umax=1 + 1% Fourier perturbation, 4x4 lattice "
+                         "+-0.05 jitter, dt=0.01, 10 substeps, MAX=3, FP64")
+
+
+def pipe_workload():
+    from pfem2_amd import mesh as M
+    m = M.pipe_3d(64, 64, 256)
+    V = M.pipe_flow(m, 1.0, 0.5, 0.01, seed=SEEDS["pipe"])
+    return Workload("pipe3d", m, V, [2, 2, 2], 0.01, 10, 2,
+                    desc="3D pipe r=0.5 L=4, 64x64x256 hexahedra clipped to the circle, Poiseuille in z + 1% "
+                         "Fourier perturbation, 2x2x2 lattice +-0.05 jitter, dt=0.01, 10 substeps, MAX=2")
+
+
+def moving_workload():
+    """~400k quads around a cylinder oscillating across the channel
+    (amplitude 0.1 D, 2 Hz); the displacement blends from rigid at the
+    cylinder to zero at 3 radii."""
+    from pfem2_amd import mesh as M
+    m = M.cylinder_channel(2.0, seed=SEEDS["moving"])
+    V = M.cylinder_flow(m, 1.0, eps=0.01, seed=SEEDS["field"])
+    X0 = m.vertices.copy()
+    c = np.array([0.2, 0.2])
+    R, Rout, A, omega = 0.05, 0.15, 0.01, 2.0 * math.pi * 2.0
+    r = np.linalg.norm(X0 - c, axis=1)
+    t = np.clip((r - R) / (Rout - R), 0.0, 1.0)
+    w = 0.5 * (1.0 + np.cos(math.pi * t))
+
+    def motion(step, dt):
+        X = X0.copy()
+        X[:, 1] += A * math.sin(omega * step * dt) * w
+        return X
+
+    return Workload("moving", m, V, [5, 5], 0.001, 10, 2, keep_vext=True, motion=motion,
+                    desc="~400k graded quads around a cylinder oscillating vertically (amplitude 0.1 D, 2 Hz), "
+                         "25 particles/cell, per step: correct+move+project, vertex update, re-sort, reseed")
 
 
 def kernel_bytes(name, info, m, dim=2):
@@ -42,8 +114,6 @@ def kernel_bytes(name, info, m, dim=2):
     n0, n1 = info["n_before"], info["n_after"]
     q = info["queued"]
     arr = max(info["arrivals"], 0)
-    f64 = not name.endswith("32") and name not in ("k_group", "k_cell_count")
-    w_xi = 8 if (f64 or name == "k_cell_count") else 4
     if name in ("k_move", "k_move32"):
         wx, wv = (8, 8) if name == "k_move" else (4, 4)
         reads = n0 * (4 + dim * wx * (3 if name == "k_move" else 1) + dim * wv)
@@ -72,32 +142,46 @@ def kernel_bytes(name, info, m, dim=2):
     return None
 
 
-def make_stage(m, V, precision, args):
+def make_stage(W, precision):
     from pfem2_amd import ParticleStage
     from pfem2_amd import mesh as M
-    st = ParticleStage(m, args.dt, args.substeps, args.max_per_part, Q, precision)
-    st.upload_fields(V, V)
-    cell, xi = M.jittered_lattice(m, Q, 0.05, SEEDS["particles"])
+    st = ParticleStage(W.m, W.dt, W.substeps, W.max_per_part, W.q, precision, keep_velocity_ext=W.keep_vext)
+    st.upload_fields(W.V, W.V)
+    cell, xi = M.jittered_lattice(W.m, W.q, W.jitter, SEEDS["particles"])
     st.insert_particles(cell, xi)
     return st
 
 
-def run_path(args, m, V, precision, torch):
-    st = make_stage(m, V, precision, args)
+def run_path(args, W, precision, torch):
+    """Timed steps (graph replay) of one precision path, then synchronised
+    per-step latencies, an eager (no graph) timing and per-kernel HIP-event
+    times with the work counts of the same steps."""
+    st = make_stage(W, precision)
     n0 = st.n_global_particles()
-    dV = torch.from_numpy(np.ascontiguousarray(V)).to("cuda")   # FEM stand-in output, resident in HBM
+    dV = torch.from_numpy(np.ascontiguousarray(W.V)).to("cuda")   # FEM stand-in output, resident in HBM
     torch.cuda.synchronize()
     ptr = dV.data_ptr()
     lib = st._lib
+    k = [0]
+    Xs = None
+    if W.motion is not None:
+        # vertex positions of every step, prepared outside the timed region
+        total = args.warmup + 2 * args.steps + max(args.steps, 20) + 12
+        Xs = [np.ascontiguousarray(W.motion(i + 1, W.dt)) for i in range(total)]
 
     def step():
-        rc = lib.pf2_upload_fields_device(st._h, ptr, None)
+        rc = lib.pf2_step_with_field(st._h, ptr)
         if rc:
             st._check(rc)
-        rc = lib.pf2_step(st._h, None)
-        if rc:
-            st._check(rc)
+        if Xs is not None:
+            # moving mesh, in the order of the reference's moving-mesh driver:
+            # vertices, re-sort of the particles, reseed
+            st.update_vertices(Xs[k[0] % len(Xs)])
+            st.sort_particles_into_subdomains_and_cells()
+            st.check_empty_cells()
+        k[0] += 1
 
+    st.set_graphs(True)
     for _ in range(args.warmup):
         step()
     st.synchronize()
@@ -109,57 +193,66 @@ def run_path(args, m, V, precision, torch):
     n1 = st.n_global_particles()
     ms = (t1 - t0) * 1e3 / args.steps
     updates = 0.5 * (n0 + n1)
-    # per-step latency of synchronised steps
     lat = []
     for _ in range(max(args.steps, 20)):
         a = time.perf_counter()
         step()
         st.synchronize()
         lat.append((time.perf_counter() - a) * 1e3)
-    # per-kernel HIP-event times on the stage's stream
+    st.set_graphs(False)
+    te0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    st.synchronize()
+    ms_eager = (time.perf_counter() - te0) * 1e3 / args.steps
     st.set_profiling(True)
     infos = []
-    for _ in range(args.steps):
+    for _ in range(min(args.steps, 10)):
         step()
         infos.append(st.step_info())
     kt = st.kernel_times()
+    nprof = len(infos)
     st.set_profiling(False)
-    info = {k: (np.mean([i[k] for i in infos], axis=0).tolist() if k == "queued" else
-                float(np.mean([i[k] for i in infos]))) for k in infos[0]}
+    info = {key: (np.mean([i[key] for i in infos], axis=0).tolist() if key == "queued" else
+                  float(np.mean([i[key] for i in infos]))) for key in infos[0]}
     res = {
         "updates_per_s": updates / (ms * 1e-3),
         "ms_per_step": ms,
+        "ms_per_step_eager": ms_eager,
+        "particles_start": int(n0),
         "particles": int(n1),
         "p50_ms": float(np.percentile(lat, 50)),
         "p99_ms": float(np.percentile(lat, 99)),
-        "kernels_ms_avg": {k: v[0] / max(v[1], 1) for k, v in kt.items()},
-        "kernels_ms_per_step": {k: v[0] / args.steps for k, v in kt.items()},
-        "kernel_launches_per_step": {k: v[1] / args.steps for k, v in kt.items()},
+        "kernels_ms_avg": {key: v[0] / max(v[1], 1) for key, v in kt.items()},
+        "kernels_ms_per_step": {key: v[0] / nprof for key, v in kt.items()},
+        "kernel_launches_per_step": {key: v[1] / nprof for key, v in kt.items()},
         "step_info": info,
         "stats": st.stats(),
+        "memory_bytes": st.memory_bytes(),
     }
     st.close()
     return res
 
 
-def accuracy(args, m, V, steps):
+def accuracy(W, steps):
     """Per-field error of the FP32-mixed path against the FP64 path (which is
     bitwise equal to the CPU oracle) after `steps` steps."""
     from pfem2_amd import FP32_MIXED, FP64
-    a = make_stage(m, V, FP64, args)
-    b = make_stage(m, V, FP32_MIXED, args)
+    a = make_stage(W, FP64)
+    b = make_stage(W, FP32_MIXED)
     n0 = a.n_global_particles()
     for _ in range(steps):
         Va = a.step(download=True)
         Vb = b.step(download=True)
-        a.upload_fields(V, None)
-        b.upload_fields(V, None)
+        a.upload_fields(W.V, None)
+        b.upload_fields(W.V, None)
     pa, pb = a.particles(), b.particles()
     ia = {int(k): n for n, k in enumerate(pa["id"]) if k <= n0}
     pairs = [(ia[int(k)], n) for n, k in enumerate(pb["id"]) if k <= n0 and int(k) in ia]
     ai = np.array([p[0] for p in pairs])
     bi = np.array([p[1] for p in pairs])
     same = pa["cell"][ai] == pb["cell"][bi]
+    m = W.m
     L = float(np.max(m.vertices.max(axis=0) - m.vertices.min(axis=0)))
     out = {
         "steps": steps,
@@ -175,33 +268,38 @@ def accuracy(args, m, V, steps):
     return out
 
 
-def cpu_baseline(m, V, args, seconds=10.0):
-    """Oracle (CPU restatement, 1 thread) on a bounded sample of the workload:
-    the same mesh and particles, stepped until about `seconds` of CPU time."""
+def _oracle(W):
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from oracle_binding import Oracle
     from pfem2_amd import mesh as M
-    o = Oracle(m, args.dt, args.substeps, args.max_per_part, Q)
-    o.upload_fields(V, V)
-    cell, xi = M.jittered_lattice(m, Q, 0.05, SEEDS["particles"])
+    o = Oracle(W.m, W.dt, W.substeps, W.max_per_part, W.q)
+    o.upload_fields(W.V, W.V)
+    cell, xi = M.jittered_lattice(W.m, W.q, W.jitter, SEEDS["particles"])
     o.insert_particles(cell, xi)
+    return o
+
+
+def cpu_baseline(W, seconds=10.0):
+    """Oracle (CPU restatement, 1 thread) on a bounded sample of the workload:
+    the same mesh and particles, stepped until about `seconds` of CPU time."""
+    o = _oracle(W)
     done, t, upd = 0, 0.0, 0.0
     while t < seconds and done < 50:
         n = o.n_global_particles()
         dt = o.time_steps(1)
-        o.upload_fields(V, None)
+        o.upload_fields(W.V, None)
         t += dt
         upd += n
         done += 1
     return {"value": upd / t, "unit": "particle-updates/s", "cores": 1, "kind": "port",
-            "sample": f"{done} full steps of the same workload ({m.n_cells} cells, ~{int(upd / done)} particles), "
+            "sample": f"{done} full steps of the same workload ({W.m.n_cells} cells, ~{int(upd / done)} particles), "
                       f"{t:.1f} s single-threaded"}
 
 
 def measured_traffic(kernel):
-    """HBM bytes per launch of `kernel` for this workload from the committed
-    rocprofv3 --pmc summary (FETCH_SIZE and WRITE_SIZE scaled by a plain copy
-    kernel of known size), or None."""
+    """HBM bytes per launch of `kernel` for the default workload from the
+    committed rocprofv3 --pmc summary (FETCH_SIZE and WRITE_SIZE scaled by a
+    plain copy kernel of known size), or None."""
     path = os.path.join(ROOT, "profiles", "hbm_traffic_cylinder100k.txt")
     if not os.path.exists(path):
         return None
@@ -212,26 +310,75 @@ def measured_traffic(kernel):
     return None
 
 
-def roofline(res, m):
+def roofline(res, m, traffic=True):
     per_step = res["kernels_ms_per_step"]
-    fam = {}
-    for k, v in per_step.items():
-        fam[k] = v
-    dom = max(fam, key=lambda k: fam[k])
-    b = kernel_bytes(dom, res["step_info"], m)
+    dom = max(per_step, key=lambda key: per_step[key])
+    b = kernel_bytes(dom, res["step_info"], m, m.dim)
     dur = res["kernels_ms_avg"][dom] * 1e-3
     achieved = b / dur / 1e9 if b else None
     others = {}
-    for k in per_step:
-        kb = kernel_bytes(k, res["step_info"], m)
+    for key in per_step:
+        kb = kernel_bytes(key, res["step_info"], m, m.dim)
         if kb:
-            others[k] = {"ms_avg": res["kernels_ms_avg"][k], "bytes_per_launch": kb,
-                         "GBs": kb / (res["kernels_ms_avg"][k] * 1e-3) / 1e9,
-                         "share_of_step": per_step[k] / sum(per_step.values())}
+            others[key] = {"ms_avg": res["kernels_ms_avg"][key], "bytes_per_launch": kb,
+                           "GBs": kb / (res["kernels_ms_avg"][key] * 1e-3) / 1e9,
+                           "share_of_step": per_step[key] / sum(per_step.values())}
     return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": measured_traffic(dom),
+            "frac": achieved / HBM_PEAK_GBS if achieved else None,
+            "traffic": measured_traffic(dom) if traffic else None,
             "kernel": dom, "bytes_per_launch": b, "avg_launch_ms": res["kernels_ms_avg"][dom],
             "all_kernels": others}
+
+
+def serve(args):
+    """The request stream: independent simulations with Poisson arrivals,
+    served by pf2_server (FP32-mixed, graph replay, admission by footprint)."""
+    from pfem2_amd import FP32_MIXED, Server
+    from pfem2_amd import mesh as M
+    rng = np.random.default_rng(SEEDS["serve"])
+    reqs = []
+    for i in range(args.serve_requests):
+        nc = float(np.exp(rng.uniform(np.log(2e3), np.log(2e5))))
+        ny = max(4, int(round(math.sqrt(nc / 5.0))))
+        nx = max(4, int(round(nc / ny)))
+        q = int(rng.choice([3, 4, 5]))
+        umax = float(rng.uniform(0.2, 2.0))
+        steps = int(rng.integers(50, 501))
+        m = M.channel_2d(nx, ny, 5.0, 1.0)
+        V = M.poiseuille_2d(m, umax, 1.0, 0.01, seed=1000 + i)
+        dt = 0.5 * min(5.0 / nx, 1.0 / ny) / umax
+        reqs.append((m, V, steps, dt, q))
+    arrivals = np.cumsum(rng.exponential(1.0 / args.serve_rate, len(reqs)))
+    print(f"# serving {len(reqs)} requests over ~{arrivals[-1]:.1f} s", file=sys.stderr, flush=True)
+    srv = Server(hbm_budget=int(args.serve_budget_gib * (1 << 30)), use_graphs=True)
+    ids = []
+    t0 = time.perf_counter()
+    i = 0
+    while True:
+        now = time.perf_counter() - t0
+        while i < len(reqs) and arrivals[i] <= now:
+            m, V, steps, dt, q = reqs[i]
+            ids.append(srv.submit(m, V, steps, dt, 10, 2, [q, q], FP32_MIXED))
+            i += 1
+        npend, nact = srv.poll()
+        if i == len(reqs) and npend == 0 and nact == 0:
+            break
+        if nact == 0 and npend == 0:
+            time.sleep(max(0.0, min(arrivals[i] - now, 0.001)))
+    wall = time.perf_counter() - t0
+    st = srv.stats()
+    lat = [srv.result(r)[2] for r in ids]
+    srv.close()
+    cells = [r[0].n_cells for r in reqs]
+    return {"requests": len(reqs), "finished": int(st["finished"]), "failed": int(st["failed"]),
+            "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"], "mean_ms": st["mean_ms"],
+            "max_ms": float(max(lat)), "updates_per_s": st["updates_per_s"],
+            "particle_updates": st["particle_updates"], "wall_s": wall,
+            "peak_hbm_gib": st["peak_bytes"] / 2 ** 30, "max_active": int(st["max_active"]),
+            "arrival_rate_per_s": args.serve_rate, "budget_gib": args.serve_budget_gib,
+            "cells_mean": float(np.mean(cells)), "precision": "fp32-mixed",
+            "stream": f"{len(reqs)} requests, Poisson arrivals ({args.serve_rate}/s), cells log-uniform 2k-200k "
+                      "(channel quads), lattice 3/4/5, Umax U[0.2,2], 50-500 steps, 10 substeps, MAX=2"}
 
 
 def copy_calibration(torch, mib=512):
@@ -243,6 +390,97 @@ def copy_calibration(torch, mib=512):
         b.copy_(a)
     torch.cuda.synchronize()
     print(json.dumps({"copy_bytes_each_way": n * 4}))
+
+
+def base_line(args, W, r, dtype):
+    return {
+        "metric": "particle-updates/s (advect+project+correct) and p50/p99 sim latency",
+        "value": r["updates_per_s"],
+        "unit": "particle-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": r["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": dtype,
+        "data": "synthetic",
+        "config": {"workload": W.desc, "cells": W.m.n_cells, "particles": r["particles"], "dt": W.dt,
+                   "substeps": W.substeps, "max_per_part": W.max_per_part, "seeds": SEEDS},
+        "p50_ms": r["p50_ms"],
+        "p99_ms": r["p99_ms"],
+    }
+
+
+def workload_main(args, W, torch, with_cpu=True, with_accuracy=True):
+    from pfem2_amd import FP32_MIXED, FP64
+    print(f"# {W.name}: mesh {W.m.n_cells} cells {W.m.n_vertices} vertices", file=sys.stderr, flush=True)
+    r64 = run_path(args, W, FP64, torch)
+    print(f"# fp64 {r64['updates_per_s']:.3e} updates/s, {r64['ms_per_step']:.3f} ms/step "
+          f"(eager {r64['ms_per_step_eager']:.3f})", file=sys.stderr, flush=True)
+    r32 = run_path(args, W, FP32_MIXED, torch)
+    print(f"# fp32-mixed {r32['updates_per_s']:.3e} updates/s, {r32['ms_per_step']:.3f} ms/step "
+          f"(eager {r32['ms_per_step_eager']:.3f})", file=sys.stderr, flush=True)
+    acc = accuracy(W, args.accuracy_steps) if with_accuracy and args.accuracy_steps > 0 else None
+    if acc:
+        print(f"# fp32-mixed vs fp64: {acc}", file=sys.stderr, flush=True)
+    cpu = cpu_baseline(W, args.cpu_seconds) if with_cpu and not args.no_cpu else None
+    out = base_line(args, W, r32, DTYPE32)
+    out["roofline"] = roofline(r32, W.m, traffic=W.name == "cylinder")
+    out["cpu_baseline"] = cpu
+    out["paths"] = {
+        "fp64": {"updates_per_s": r64["updates_per_s"], "ms_per_step": r64["ms_per_step"],
+                 "ms_per_step_eager": r64["ms_per_step_eager"], "p50_ms": r64["p50_ms"], "p99_ms": r64["p99_ms"],
+                 "kernels_ms_per_step": r64["kernels_ms_per_step"], "roofline": roofline(r64, W.m, False),
+                 "stats": r64["stats"], "step_info": r64["step_info"], "memory_gib": r64["memory_bytes"] / 2 ** 30},
+        "fp32_mixed": {"updates_per_s": r32["updates_per_s"], "ms_per_step": r32["ms_per_step"],
+                       "ms_per_step_eager": r32["ms_per_step_eager"],
+                       "kernels_ms_per_step": r32["kernels_ms_per_step"], "stats": r32["stats"],
+                       "step_info": r32["step_info"], "error_vs_fp64": acc,
+                       "memory_gib": r32["memory_bytes"] / 2 ** 30},
+        "speedup_fp32_over_fp64": r32["updates_per_s"] / r64["updates_per_s"],
+    }
+    return out
+
+
+def channel_main(args, torch):
+    """200x20 channel, FP64: the GPU run and the CPU oracle over the same
+    whole run, with the final fields compared bit for bit."""
+    from pfem2_amd import FP64
+    W = channel_workload()
+    steps = args.channel_steps
+    st = make_stage(W, FP64)
+    n0 = st.n_global_particles()
+    dV = torch.from_numpy(np.ascontiguousarray(W.V)).to("cuda")
+    torch.cuda.synchronize()
+    st.set_graphs(True)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        st.step_with_field(dV.data_ptr())
+    st.synchronize()
+    gpu_s = time.perf_counter() - t0
+    Vg = st.download_field()
+    n1 = st.n_global_particles()
+    st.close()
+    o = _oracle(W)
+    cpu_s, upd = 0.0, 0
+    for _ in range(steps):
+        upd += o.n_global_particles()
+        cpu_s += o.time_steps(1)
+        o.upload_fields(W.V, None)
+    Vo = o.field()
+    gpu_rate = 0.5 * (n0 + n1) * steps / gpu_s
+    out = base_line(args, W, {"updates_per_s": gpu_rate, "ms_per_step": gpu_s * 1e3 / steps, "particles": n1,
+                              "p50_ms": None, "p99_ms": None}, "f64")
+    out["steps"] = steps
+    out["warmup"] = 0
+    out["roofline"] = None
+    out["cpu_baseline"] = {"value": upd / cpu_s, "unit": "particle-updates/s", "cores": 1, "kind": "port",
+                           "sample": f"the whole run: {steps} steps of {W.m.n_cells} cells, {cpu_s:.1f} s"}
+    out["parity"] = {"final_field_bitwise_equal": bool(np.array_equal(Vg, Vo, equal_nan=True)),
+                     "particles_gpu": int(n1), "particles_oracle": int(o.n_global_particles())}
+    return out
 
 
 def main():
@@ -257,7 +495,12 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--accuracy-steps", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--only", choices=["fp64", "fp32"], default=None)
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--workload", choices=["cylinder", "channel", "pipe3d", "moving", "serve"], default="cylinder")
+    ap.add_argument("--serve-requests", type=int, default=256)
+    ap.add_argument("--serve-rate", type=float, default=10.0, help="Poisson arrival rate, requests/s")
+    ap.add_argument("--serve-budget-gib", type=float, default=200.0)
+    ap.add_argument("--channel-steps", type=int, default=100)
     ap.add_argument("--copy-calibration", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -268,60 +511,31 @@ def main():
     if args.copy_calibration:
         copy_calibration(torch)
         return
-    from pfem2_amd import FP32_MIXED, FP64
-    from pfem2_amd import mesh as M
-    m = M.cylinder_channel(args.scale, seed=SEEDS["mesh"])
-    V = M.cylinder_flow(m, 1.0, eps=0.01, seed=SEEDS["field"])
-    print(f"# mesh {m.n_cells} cells {m.n_vertices} vertices", file=sys.stderr, flush=True)
-    r64 = r32 = None
-    if args.only in (None, "fp64"):
-        r64 = run_path(args, m, V, FP64, torch)
-        print(f"# fp64 {r64['updates_per_s']:.3e} updates/s, {r64['ms_per_step']:.3f} ms/step", file=sys.stderr,
-              flush=True)
-    if args.only in (None, "fp32"):
-        r32 = run_path(args, m, V, FP32_MIXED, torch)
-        print(f"# fp32-mixed {r32['updates_per_s']:.3e} updates/s, {r32['ms_per_step']:.3f} ms/step",
-              file=sys.stderr, flush=True)
-    if args.only:
-        r = r64 or r32
-        print(json.dumps({"path": args.only, **r, "roofline": roofline(r, m)}))
+    if args.workload == "serve":
+        s = serve(args)
+        out = {"metric": "p50/p99 sim latency and particle-updates/s of a request stream", "value": s["p50_ms"],
+               "unit": "ms (p50 latency)", "n_gpus": 1, "higher_is_better": False, "scaling": "weak",
+               "vs_baseline": None, "dtype": DTYPE32, "data": "synthetic", "config": {"workload": s["stream"]},
+               "serving": s}
+        print(json.dumps(out))
         return
-    acc = accuracy(args, m, V, args.accuracy_steps) if args.accuracy_steps > 0 else None
-    print(f"# fp32-mixed vs fp64: {acc}", file=sys.stderr, flush=True)
-    cpu = None if args.no_cpu else cpu_baseline(m, V, args, args.cpu_seconds)
-    out = {
-        "metric": "particle-updates/s (advect+project+correct) and p50/p99 sim latency",
-        "value": r32["updates_per_s"],
-        "unit": "particle-updates/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": r32["ms_per_step"],
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "fp32-mixed (f32 reference coords/velocities, f64 relocation and node sums)",
-        "data": "synthetic",
-        "config": {"workload": "2D channel with circular obstacle (2.2x0.41, r=0.05), graded quad mesh, "
-                               "16 particles/cell, potential flow + 1% Fourier perturbation, "
-                               "step = correct+move(10 substeps)+project",
-                   "cells": m.n_cells, "particles": r32["particles"], "dt": args.dt, "substeps": args.substeps,
-                   "max_per_part": args.max_per_part, "seeds": SEEDS},
-        "p50_ms": r32["p50_ms"],
-        "p99_ms": r32["p99_ms"],
-        "roofline": roofline(r32, m),
-        "cpu_baseline": cpu,
-        "paths": {
-            "fp64": {"updates_per_s": r64["updates_per_s"], "ms_per_step": r64["ms_per_step"],
-                     "p50_ms": r64["p50_ms"], "p99_ms": r64["p99_ms"],
-                     "kernels_ms_per_step": r64["kernels_ms_per_step"], "roofline": roofline(r64, m),
-                     "stats": r64["stats"], "step_info": r64["step_info"]},
-            "fp32_mixed": {"updates_per_s": r32["updates_per_s"], "ms_per_step": r32["ms_per_step"],
-                           "kernels_ms_per_step": r32["kernels_ms_per_step"], "stats": r32["stats"],
-                           "step_info": r32["step_info"], "error_vs_fp64": acc},
-            "speedup_fp32_over_fp64": r32["updates_per_s"] / r64["updates_per_s"],
-        },
-    }
+    if args.workload == "channel":
+        print(json.dumps(channel_main(args, torch)))
+        return
+    if args.workload == "pipe3d":
+        print(json.dumps(workload_main(args, pipe_workload(), torch, with_cpu=True, with_accuracy=False)))
+        return
+    if args.workload == "moving":
+        print(json.dumps(workload_main(args, moving_workload(), torch, with_cpu=False, with_accuracy=False)))
+        return
+    out = workload_main(args, cylinder_workload(args), torch)
+    if not args.no_serve:
+        s = serve(args)
+        print(f"# serving: p50 {s['p50_ms']:.1f} ms p99 {s['p99_ms']:.1f} ms, {s['updates_per_s']:.3e} updates/s, "
+              f"max active {s['max_active']}", file=sys.stderr, flush=True)
+        out["serving"] = s
+        out["sim_latency_p50_ms"] = s["p50_ms"]
+        out["sim_latency_p99_ms"] = s["p99_ms"]
     print(json.dumps(out))
 
 

@@ -77,6 +77,18 @@ const char* pf2_last_error(void);
  * (TurekBenchmark/pfem2particle.cpp:215-219).  `capacity` = maximum number of
  * particles; 0 chooses the bound that reseeding guarantees. */
 int pf2_create(const pf2_mesh* mesh, const pf2_params* params, long long capacity, pf2_handle* out);
+/* pf2_create inside a caller-owned device memory block of block_bytes
+ * (at least pf2_estimate_bytes) and on a caller-owned HIP stream (a
+ * hipStream_t; NULL creates one).  No counterpart in the reference: it lets a
+ * server place many simulations in one allocation without hipMalloc/hipFree
+ * while others run.  pf2_destroy then frees neither the block nor the stream. */
+int pf2_create_in(const pf2_mesh* mesh, const pf2_params* params, long long capacity, void* block,
+                  unsigned long long block_bytes, void* stream, pf2_handle* out);
+/* Device bytes pf2_create takes for this mesh / params / capacity (upper
+ * bound), and the bytes a handle holds. */
+int pf2_estimate_bytes(const pf2_mesh* mesh, const pf2_params* params, long long capacity,
+                       unsigned long long* bytes);
+int pf2_memory_bytes(pf2_handle h, unsigned long long* bytes);
 int pf2_destroy(pf2_handle h);
 int pf2_set_params(pf2_handle h, const pf2_params* params);
 
@@ -112,6 +124,16 @@ int pf2_distribute_particle_velocities_to_grid(pf2_handle h, double* V_out);
 /* correct -> move -> distribute in one call (the driver sequence of
  * cylinder2D/cylinder2D.cc:865-867); V_out may be NULL. */
 int pf2_step(pf2_handle h, double* V_out);
+/* One step of a driver whose FEM update leaves V in device memory
+ * (component-major): V is refreshed from V_device, then correct -> move ->
+ * distribute.  No host synchronisation. */
+int pf2_step_with_field(pf2_handle h, const double* V_device);
+/* Capture pf2_step / pf2_step_with_field (when no V_out is requested and
+ * profiling is off) as HIP graphs, one per buffer parity, replayed on later
+ * steps; results are identical to the eager launches.  pf2_set_params,
+ * pf2_set_walls, pf2_seed_particles and pf2_insert_particles drop the
+ * graphs (they change launch arguments or sizes).  Off by default. */
+int pf2_set_graphs(pf2_handle h, int on);
 
 /* Wall / body nodes of the projection fix-up (wallsAndBodyDoFs,
  * cylinderDyM/pfem2particle.h:253): vertex index, DoF index, boundary values
@@ -162,6 +184,47 @@ int pf2_synchronize(pf2_handle h);
 int pf2_set_profiling(pf2_handle h, int on);
 int pf2_kernel_times(pf2_handle h, double* total_ms, long long* launches, int cap);
 const char* pf2_kernel_name(int i);
+
+/* ------------------------------------------------------------------------
+ * Serving loop (no counterpart in the reference, which runs one simulation
+ * per process).  Independent simulations are submitted with their mesh,
+ * parameters, a fixed nodal field V and a step count; each is seeded with
+ * the lattice and then advanced n_steps times by pf2_step_with_field.
+ * Admission is first come, first served while the simulation's device
+ * footprint fits the HBM budget, which the server reserves once at creation
+ * and hands out in ranges.  Single-threaded: calls on one server must not
+ * overlap. */
+typedef struct pf2_server_s* pf2_server;
+typedef struct pf2_server_config {
+    unsigned long long hbm_budget;  /* bytes reserved for all simulations (0: 80% of free HBM) */
+    int max_active;                 /* simulations in flight at once (0: limited by HBM only) */
+    int steps_per_chunk;            /* steps issued per simulation between completion events (0: 8) */
+    int chunks_ahead;               /* chunks kept queued per simulation (0: 2) */
+    int use_graphs;                 /* replay each simulation's step as a HIP graph */
+} pf2_server_config;
+
+const char* pf2_server_last_error(void);
+int pf2_server_create(const pf2_server_config* cfg, pf2_server* out);
+int pf2_server_destroy(pf2_server s);
+/* Copies every input; V is dim x n_dofs, component-major.  Fails with
+ * PF2_E_CAPACITY when the simulation alone exceeds the budget. */
+int pf2_server_submit(pf2_server s, const pf2_mesh* mesh, const pf2_params* params, const double* V, int n_steps,
+                      long long* request_id);
+/* Retires finished simulations, admits pending ones, issues steps; never
+ * waits for the GPU.  n_pending / n_active may be NULL. */
+int pf2_server_poll(pf2_server s, int* n_pending, int* n_active);
+/* Polls until every submitted simulation has finished. */
+int pf2_server_drain(pf2_server s);
+/* Result of a finished simulation: the last projected field (dim x n_dofs),
+ * the final particle count and the latency from submission to completion.
+ * PF2_E_STATE while it runs; the simulation's error code if it failed. */
+int pf2_server_result(pf2_server s, long long request_id, double* V_out, long long* n_particles, double* latency_ms);
+/* [0] finished, [1] failed, [2] p50 latency ms, [3] p99 latency ms,
+ * [4] particle-updates, [5] seconds from first submission to last
+ * completion, [6] particle-updates/s over that span, [7] peak bytes in use,
+ * [8] most simulations active at once, [9] mean latency ms.  Returns the
+ * number of entries written (at most cap). */
+int pf2_server_stats(pf2_server s, double* out, int cap);
 
 #ifdef __cplusplus
 }

@@ -49,8 +49,16 @@ EXPORTS = [
     "pf2_step", "pf2_set_walls", "pf2_sort_particles_into_subdomains_and_cells", "pf2_check_empty_cells",
     "pf2_update_vertices", "pf2_initialize_maps", "pf2_n_global_particles", "pf2_download_particles",
     "pf2_cell_counts", "pf2_particle_counter", "pf2_stats", "pf2_synchronize", "pf2_set_profiling",
-    "pf2_kernel_times", "pf2_kernel_name", "pf2_step_info",
+    "pf2_kernel_times", "pf2_kernel_name", "pf2_step_info", "pf2_create_in", "pf2_estimate_bytes",
+    "pf2_memory_bytes", "pf2_step_with_field", "pf2_set_graphs",
+    "pf2_server_last_error", "pf2_server_create", "pf2_server_destroy", "pf2_server_submit", "pf2_server_poll",
+    "pf2_server_drain", "pf2_server_result", "pf2_server_stats",
 ]
+
+
+class _ServerConfig(ctypes.Structure):
+    _fields_ = [("hbm_budget", ctypes.c_ulonglong), ("max_active", ctypes.c_int),
+                ("steps_per_chunk", ctypes.c_int), ("chunks_ahead", ctypes.c_int), ("use_graphs", ctypes.c_int)]
 
 
 def load_library(path=LIB_PATH):
@@ -90,7 +98,22 @@ def load_library(path=LIB_PATH):
     lib.pf2_set_profiling.argtypes = [vp, ip]
     lib.pf2_kernel_times.argtypes = [vp, vp, vp, ip]
     lib.pf2_step_info.argtypes = [vp, vp, ip]
-    _ = dp
+    ull, ll = ctypes.c_ulonglong, ctypes.c_longlong
+    lib.pf2_create_in.argtypes = [ctypes.POINTER(_Mesh), ctypes.POINTER(_Params), ll, vp, ull, vp,
+                                  ctypes.POINTER(vp)]
+    lib.pf2_estimate_bytes.argtypes = [ctypes.POINTER(_Mesh), ctypes.POINTER(_Params), ll, ctypes.POINTER(ull)]
+    lib.pf2_memory_bytes.argtypes = [vp, ctypes.POINTER(ull)]
+    lib.pf2_step_with_field.argtypes = [vp, vp]
+    lib.pf2_set_graphs.argtypes = [vp, ip]
+    lib.pf2_server_last_error.restype = ctypes.c_char_p
+    lib.pf2_server_create.argtypes = [ctypes.POINTER(_ServerConfig), ctypes.POINTER(vp)]
+    lib.pf2_server_destroy.argtypes = [vp]
+    lib.pf2_server_submit.argtypes = [vp, ctypes.POINTER(_Mesh), ctypes.POINTER(_Params), vp, ip,
+                                      ctypes.POINTER(ll)]
+    lib.pf2_server_poll.argtypes = [vp, ctypes.POINTER(ip), ctypes.POINTER(ip)]
+    lib.pf2_server_drain.argtypes = [vp]
+    lib.pf2_server_result.argtypes = [vp, ll, vp, ctypes.POINTER(ll), ctypes.POINTER(dp)]
+    lib.pf2_server_stats.argtypes = [vp, vp, ip]
     _lib = lib
     return lib
 
@@ -104,6 +127,23 @@ def _f64(a, shape=None):
     if shape is not None and a.shape != shape:
         a = a.reshape(shape)
     return a
+
+
+def _params_of(dim, dt, substeps, max_per_part, quantities, precision=FP64, keep_velocity_ext=False):
+    q = list(quantities) + [1] * (3 - len(quantities))
+    return _Params(dt, substeps, max_per_part, (ctypes.c_int * 3)(*q[:3]), precision, 1 if keep_velocity_ext else 0)
+
+
+def estimate_bytes(m, dt, substeps, max_per_part, quantities, precision=FP64, keep_velocity_ext=False, capacity=0):
+    """Device bytes a simulation of this mesh and parameters takes (upper bound)."""
+    lib = load_library()
+    p = _params_of(m.dim, dt, substeps, max_per_part, quantities, precision, keep_velocity_ext)
+    ms = _Mesh(m.dim, m.n_cells, m.n_vertices, m.n_dofs, None, None, None)
+    out = ctypes.c_ulonglong()
+    rc = lib.pf2_estimate_bytes(ctypes.byref(ms), ctypes.byref(p), capacity, ctypes.byref(out))
+    if rc != 0:
+        raise Pf2Error(f"pf2 error {rc}: {lib.pf2_last_error().decode()}")
+    return out.value
 
 
 class ParticleStage:
@@ -204,6 +244,19 @@ class ParticleStage:
         out = np.empty((self.dim, self.mesh.n_dofs)) if download else None
         self._check(self._lib.pf2_step(self._h, _ptr(out)))
         return out
+
+    def step_with_field(self, V_device_ptr):
+        """One step refreshing V from a device field (component-major) given
+        by its address, e.g. a torch tensor's data_ptr(); no host sync."""
+        self._check(self._lib.pf2_step_with_field(self._h, ctypes.c_void_p(V_device_ptr)))
+
+    def set_graphs(self, on=True):
+        self._check(self._lib.pf2_set_graphs(self._h, 1 if on else 0))
+
+    def memory_bytes(self):
+        out = ctypes.c_ulonglong()
+        self._check(self._lib.pf2_memory_bytes(self._h, ctypes.byref(out)))
+        return out.value
 
     def check_empty_cells(self):
         self._check(self._lib.pf2_check_empty_cells(self._h))
@@ -318,3 +371,74 @@ class Pfem2Solver:
 
     def distribute_particle_velocities_to_grid(self):
         return self.stage.distribute_particle_velocities_to_grid(download=True)
+
+
+class Server:
+    """Serving loop for independent simulations on one GPU (pf2_server_*).
+
+    submit() copies a request (mesh, parameters, fixed field V, step count);
+    poll() admits, issues and retires without waiting for the GPU; result()
+    returns the last projected field, the final particle count and the
+    latency from submission to completion."""
+
+    def __init__(self, hbm_budget=0, max_active=0, steps_per_chunk=0, chunks_ahead=0, use_graphs=True):
+        self._lib = load_library()
+        cfg = _ServerConfig(hbm_budget, max_active, steps_per_chunk, chunks_ahead, 1 if use_graphs else 0)
+        h = ctypes.c_void_p()
+        self._check(self._lib.pf2_server_create(ctypes.byref(cfg), ctypes.byref(h)))
+        self._s = h
+        self._dims = {}
+
+    def _check(self, rc):
+        if rc < 0:
+            raise Pf2Error(f"pf2 server error {rc}: {self._lib.pf2_server_last_error().decode()}")
+
+    def close(self):
+        if getattr(self, "_s", None):
+            self._lib.pf2_server_destroy(self._s)
+            self._s = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit(self, m, V, n_steps, dt, substeps, max_per_part, quantities, precision=FP64):
+        vertices = _f64(m.vertices)
+        cells = np.ascontiguousarray(m.cells, dtype=np.int32)
+        V = _f64(V, (m.dim, m.n_dofs))
+        ms = _Mesh(m.dim, m.n_cells, m.n_vertices, m.n_dofs, _ptr(vertices), _ptr(cells), None)
+        p = _params_of(m.dim, dt, substeps, max_per_part, quantities, precision)
+        rid = ctypes.c_longlong()
+        self._check(self._lib.pf2_server_submit(self._s, ctypes.byref(ms), ctypes.byref(p), _ptr(V), n_steps,
+                                                ctypes.byref(rid)))
+        self._dims[rid.value] = (m.dim, m.n_dofs)
+        return rid.value
+
+    def poll(self):
+        npend, nact = ctypes.c_int(), ctypes.c_int()
+        self._check(self._lib.pf2_server_poll(self._s, ctypes.byref(npend), ctypes.byref(nact)))
+        return npend.value, nact.value
+
+    def drain(self):
+        self._check(self._lib.pf2_server_drain(self._s))
+
+    def result(self, rid):
+        """(V, n_particles, latency_ms), or None while the request runs."""
+        dim, nd = self._dims[rid]
+        V = np.empty((dim, nd))
+        n = ctypes.c_longlong()
+        lat = ctypes.c_double()
+        rc = self._lib.pf2_server_result(self._s, rid, _ptr(V), ctypes.byref(n), ctypes.byref(lat))
+        if rc == -3:
+            return None
+        self._check(rc)
+        return V, n.value, lat.value
+
+    def stats(self):
+        out = np.zeros(10)
+        self._check(self._lib.pf2_server_stats(self._s, _ptr(out), 10))
+        keys = ("finished", "failed", "p50_ms", "p99_ms", "particle_updates", "span_s", "updates_per_s",
+                "peak_bytes", "max_active", "mean_ms")
+        return dict(zip(keys, out.tolist()))

@@ -12,7 +12,8 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB = os.path.join(ROOT, "pfem2_amd", "libpfem2_amd.so")
 ORACLE_LIB = os.path.join(ROOT, "oracle", "build", "libpf2_oracle.so")
 
-HIP_SOURCES = [os.path.join(ROOT, "pfem2_amd", "csrc", "pf2_lib.hip")]
+HIP_SOURCES = [os.path.join(ROOT, "pfem2_amd", "csrc", "pf2_lib.hip"),
+               os.path.join(ROOT, "pfem2_amd", "csrc", "pf2_server.hip")]
 HIP_FLAGS = [
     "-O3", "-std=c++17", "--offload-arch=gfx950", "-fPIC", "-shared",
     # IEEE evaluation order: no contraction into FMA, so FP64 results

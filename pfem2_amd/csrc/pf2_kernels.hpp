@@ -264,6 +264,9 @@ __device__ int relocate(const DevMesh<DIM>& m, int c, const double (*P)[DIM], co
     for (int d = 0; d < DIM; ++d) t[d] *= rn;
     double best = vdot<DIM>(vn, t);
     int cvk = 0;
+    double dir[DIM];
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) dir[d] = P[0][d];
 #pragma unroll
     for (int k = 1; k < NV; ++k) {
 #pragma unroll
@@ -275,11 +278,12 @@ __device__ int relocate(const DevMesh<DIM>& m, int c, const double (*P)[DIM], co
         if (a > best) {
             cvk = k;
             best = a;
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) dir[d] = P[k][d];
         }
     }
-    double dir[DIM];
 #pragma unroll
-    for (int d = 0; d < DIM; ++d) dir[d] = x[d] - P[cvk][d];
+    for (int d = 0; d < DIM; ++d) dir[d] = x[d] - dir[d];
     rn = 1.0 / vnorm<DIM>(dir);
 #pragma unroll
     for (int d = 0; d < DIM; ++d) dir[d] *= rn;
@@ -287,36 +291,28 @@ __device__ int relocate(const DevMesh<DIM>& m, int c, const double (*P)[DIM], co
     const int gv = m.cv[(size_t)c * NV + cvk];
     const int s1 = m.star_off[gv + 1];
     const int n = s1 - s0;
-    if (n <= MAXSTAR) {
+    if (n <= 32) {
         // candidates in decreasing key order (ties: star order), first
-        // containing cell wins -- the reference's sorted loop, stopping early
-        double key[MAXSTAR];
-        int scell[MAXSTAR];
-#pragma unroll
-        for (int e = 0; e < MAXSTAR; ++e) {
-            if (e < n) {
+        // containing cell wins -- the reference's sorted loop, stopping early.
+        // Each pass re-reads the star (cache hits) instead of holding the keys
+        // in registers, which keeps the search kernels' occupancy up.
+        unsigned tried = 0;
+        for (int it = 0; it < n; ++it) {
+            int best = -1;
+            double bk = 0.0;
+            for (int e = 0; e < n; ++e) {
+                if ((tried >> e) & 1u) continue;
                 double sd[DIM];
 #pragma unroll
                 for (int d = 0; d < DIM; ++d) sd[d] = m.star_dir[(size_t)(s0 + e) * DIM + d];
-                key[e] = vdot<DIM>(sd, dir);
-                scell[e] = m.star_cell[s0 + e];
-            } else {
-                key[e] = 0.0;
-                scell[e] = -1;
-            }
-        }
-        unsigned tried = 0;
-        for (int it = 0; it < n; ++it) {
-            int best = -1, cc = -1;
-            double bk = 0.0;
-#pragma unroll
-            for (int e = 0; e < MAXSTAR; ++e)
-                if (e < n && !((tried >> e) & 1u) && (best < 0 || key[e] > bk)) {
+                const double key = vdot<DIM>(sd, dir);
+                if (best < 0 || key > bk) {
                     best = e;
-                    bk = key[e];
-                    cc = scell[e];
+                    bk = key;
                 }
+            }
             tried |= 1u << best;
+            const int cc = m.star_cell[s0 + best];
             double Q[NV][DIM], r[DIM];
             load_vertices<DIM>(m, cc, Q);
             if (inverse_map<DIM>(Q, x, r, m.ftol[cc], true) && inside_unit<DIM>(r)) {
@@ -519,7 +515,7 @@ __device__ __forceinline__ void park_particle(const DevParticles& A, const DevWo
 // k_move: [correction] + substeps until done or the first failed own-cell test.
 //   correct_particles_velocities  TurekBenchmark/pfem2particle.cpp:792-811
 template <int DIM>
-__global__ void __launch_bounds__(256) k_move(DevMesh<DIM> m, const double* __restrict__ V,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DIM == 2 ? 3 : 1))) k_move(DevMesh<DIM> m, const double* __restrict__ V,
                                               const double* __restrict__ Vold, DevParticles A, DevParams pp,
                                               DevWork w, DevQueues qu, int do_correct, int do_move) {
     constexpr int NV = Q1<DIM>::NV;
@@ -564,7 +560,7 @@ __global__ void __launch_bounds__(256) k_move(DevMesh<DIM> m, const double* __re
 // `round`, then the remaining substeps; a particle that needs another search
 // is queued for round + 1, or, in the last round, searched inline.
 template <int DIM>
-__global__ void __launch_bounds__(256) k_search_resume(DevMesh<DIM> m, const double* __restrict__ V,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DIM == 2 ? 3 : 1))) k_search_resume(DevMesh<DIM> m, const double* __restrict__ V,
                                                        DevParticles A, DevParams pp, DevWork w, DevQueues qu,
                                                        int round) {
     constexpr int NV = Q1<DIM>::NV;

@@ -45,12 +45,33 @@ const char* kKernelNames[K_NKERNELS] = {"k_move", "k_group", "k_cell_count", "k_
                                         "k_seed", "misc", "k_move32", "k_cell_write32", "k_cell_partials32",
                                         "k_search_resume", "k_search_resume32", "k_queue_prefix"};
 
+// bytes obtained through dalloc on this thread (a handle's footprint is the
+// difference across its creation)
+thread_local size_t g_alloc_bytes = 0;
+
+// While a handle is created inside caller-provided device memory, dalloc
+// carves 256-byte aligned pieces out of it instead of calling hipMalloc.
+struct Arena {
+    char* base = nullptr;
+    size_t size = 0, used = 0;
+};
+thread_local Arena g_arena;
+
 template <typename T>
 int dalloc(T** p, size_t n) {
     *p = nullptr;
     if (n == 0) n = 1;
-    hipError_t e = hipMalloc((void**)p, n * sizeof(T));
-    if (e != hipSuccess) return fail(PF2_E_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+    const size_t bytes = n * sizeof(T);
+    if (g_arena.base) {
+        const size_t at = (g_arena.used + 255) & ~(size_t)255;
+        if (at + bytes > g_arena.size) return fail(PF2_E_CAPACITY, "device memory block too small for the handle");
+        *p = (T*)(g_arena.base + at);
+        g_arena.used = at + bytes;
+    } else {
+        hipError_t e = hipMalloc((void**)p, bytes);
+        if (e != hipSuccess) return fail(PF2_E_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+    }
+    g_alloc_bytes += bytes;
     return 0;
 }
 
@@ -127,6 +148,14 @@ struct pf2_handle_s {
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_pending;
     double k_ms[K_NKERNELS] = {};
     long long k_n[K_NKERNELS] = {};
+    // HIP graphs of one step, keyed by the buffer parity (cur, cur_off) and
+    // by whether the step first refreshes V from a device field
+    bool graphs = false;
+    hipGraphExec_t gexec[8] = {};
+    const double* g_field = nullptr;   // device field baked into the refreshing graphs
+    size_t bytes = 0;                  // device memory held by the handle
+    bool in_block = false;             // created inside caller memory: nothing of it is freed here
+    bool own_stream = true;
 };
 
 namespace {
@@ -343,6 +372,7 @@ int alloc_particles(pf2_handle_s* h) {
 }
 
 void free_particles(pf2_handle_s* h) {
+    if (h->in_block) return;
     for (int b = 0; b < 2; ++b) {
         PBufs& pb = h->buf[b];
         hipFree(pb.cell); hipFree(pb.id); hipFree(pb.x); hipFree(pb.xi); hipFree(pb.v); hipFree(pb.vext);
@@ -591,6 +621,95 @@ int from_mixed(pf2_handle_s* h) {
     return 0;
 }
 
+// Device bytes pf2_create requests for a mesh of these sizes (the same list
+// of allocations as pf2_create and alloc_particles), plus 1 MiB for the scan
+// state whose size the scan library fixes at run time.
+size_t estimate_bytes(int dim, long long nc, long long nvt, long long nd, long long cap, const pf2_params* p) {
+    const long long nv = 1LL << dim, nck = nc * nv;
+    size_t b = 0;
+    b += nvt * dim * 8 + nck * 4 * 2 + (nvt + 1) * 4 + nck * 4 + nck * dim * 8;       // X, cv, cd, star
+    b += (nd + 1) * 4 + nck * 4 + nc * 8 + nck * dim * 8 + nck * 4 + nck * dim * 8 * 2;  // dof map, ftol, cellX/star/V/dV
+    b += nd * dim * 8 * 3;                                                              // V, V_old, staging
+    b += (nc + 1) * 4 * 8 + nc * 8 + nc * 4 + nc * nv * (dim + 1) * 8 + 16;             // per-cell work, partials
+    b += (size_t)(MOVE_ROUNDS + 2) * (QSHARDS + 1) * QSTRIDE * 4 + (size_t)(MOVE_ROUNDS + 2) * (QSHARDS + 2) * 4;
+    b += 4 + (size_t)STAT_SHARDS * NSTAT * 8;
+    b += nck * dim * 4 * 2 + nc * geo32_floats(dim) * 4;                                 // FP32 fields, geometry
+    long long per = 4 + 4 + 3 * dim * 8;
+    if (p->keep_velocity_ext) per += dim * 8;
+    if (p->precision == PF2_FP32_MIXED) per += 2 * dim * 4;
+    b += 2 * cap * per;                                                                 // two particle buffers
+    const long long shard_cap = 2 * ((cap + QSHARDS - 1) / QSHARDS) + 64;
+    const long long qlen = QSHARDS * shard_cap + cap;
+    b += cap * 4 * 4 + 2 * qlen * 4 + cap * 4 + cap * dim * 8;                          // work, queues, parking
+    return b + (1u << 20);
+}
+
+void invalidate_graphs(pf2_handle_s* h) {
+    bool any = false;
+    for (auto g : h->gexec) any = any || g;
+    if (!any) return;
+    hipStreamSynchronize(h->stream);
+    for (auto& g : h->gexec)
+        if (g) {
+            hipGraphExecDestroy(g);
+            g = nullptr;
+        }
+}
+
+// one step: optional refresh of V from a device field (component-major),
+// then correct, move (with reseed) and project -- no host synchronisation
+int step_body(pf2_handle_s* h, const double* field) {
+    if (field) {
+        if (int r = upload_field(h, field, h->dV, hipMemcpyDeviceToDevice)) return r;
+        if (int r = pack_fields(h)) return r;
+    }
+    if (int r = (h->dim == 2 ? run_move<2>(h, 1, 1) : run_move<3>(h, 1, 1))) return r;
+    return h->dim == 2 ? run_project<2>(h) : run_project<3>(h);
+}
+
+// the step as a HIP graph: captured once per buffer parity, replayed after.
+// The host-side effects of a step (buffer swap, FP32 state flag) are applied
+// on every replay exactly as the eager step applies them.
+int step_graph(pf2_handle_s* h, const double* field) {
+    if (field != h->g_field) {
+        for (int k = 4; k < 8; ++k)
+            if (h->gexec[k]) {
+                hipGraphExecDestroy(h->gexec[k]);
+                h->gexec[k] = nullptr;
+            }
+        h->g_field = field;
+    }
+    const int key = (field ? 4 : 0) + h->cur * 2 + h->cur_off;
+    if (!h->gexec[key]) {
+        HIPC(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+        const int r = step_body(h, field);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(h->stream, &g);
+        if (r) {
+            if (g) hipGraphDestroy(g);
+            return r;
+        }
+        HIPC(e);
+        const hipError_t ei = hipGraphInstantiate(&h->gexec[key], g, nullptr, nullptr, 0);
+        hipGraphDestroy(g);
+        HIPC(ei);
+        // the capture already applied the step's host-side effects; undo
+        // them so that the replay below applies them once
+        h->cur = 1 - h->cur;
+        h->cur_off = 1 - h->cur_off;
+    }
+    HIPC(hipGraphLaunch(h->gexec[key], h->stream));
+    h->cur = 1 - h->cur;
+    h->cur_off = 1 - h->cur_off;
+    if (mixed(h)) h->fp64_valid = false;
+    return 0;
+}
+
+int run_step(pf2_handle_s* h, const double* field) {
+    if (h->graphs && !h->profiling) return step_graph(h, field);
+    return step_body(h, field);
+}
+
 }  // namespace
 
 // ============================================================================
@@ -599,6 +718,15 @@ extern "C" {
 const char* pf2_last_error(void) { return g_err.c_str(); }
 
 int pf2_create(const pf2_mesh* mesh, const pf2_params* params, long long capacity, pf2_handle* out) {
+    return pf2_create_in(mesh, params, capacity, nullptr, 0, nullptr, out);
+}
+
+int pf2_create_in(const pf2_mesh* mesh, const pf2_params* params, long long capacity, void* block,
+                  unsigned long long block_bytes, void* stream, pf2_handle* out) {
+    struct ArenaScope {
+        ArenaScope(void* b, size_t n) { g_arena = Arena{(char*)b, b ? n : 0, 0}; }
+        ~ArenaScope() { g_arena = Arena{}; }
+    } arena_scope(block, block_bytes);
     if (!mesh || !out) return fail(PF2_E_ARG, "mesh/out is NULL");
     *out = nullptr;
     if (mesh->dim != 2 && mesh->dim != 3) return fail(PF2_E_ARG, "dim must be 2 or 3");
@@ -655,7 +783,12 @@ int pf2_create(const pf2_mesh* mesh, const pf2_params* params, long long capacit
         pf2_destroy(h);
         return r;
     };
-    if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess)
+    const size_t bytes0 = g_alloc_bytes;
+    h->in_block = block != nullptr;
+    if (stream) {
+        h->stream = (hipStream_t)stream;
+        h->own_stream = false;
+    } else if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess)
         return bail(fail(PF2_E_HIP, "hipStreamCreate failed"));
     const int nc = h->n_cells, nvt = h->n_vertices, nd = h->n_dofs, dim = h->dim;
     if (dalloc(&h->dX, (size_t)nvt * dim) || dalloc(&h->dcv, nck) || dalloc(&h->dcd, nck) ||
@@ -710,6 +843,7 @@ int pf2_create(const pf2_mesh* mesh, const pf2_params* params, long long capacit
     else launch_geo<3>(h->stream, dev_mesh<3>(h), h->dgeo32, h->dftol, h->dcellX);
     if (pack_fields(h)) return bail(PF2_E_HIP);
     if (hipStreamSynchronize(h->stream) != hipSuccess) return bail(fail(PF2_E_HIP, "initialisation sync failed"));
+    h->bytes = g_alloc_bytes - bytes0;
     *out = h;
     return 0;
 }
@@ -717,13 +851,19 @@ int pf2_create(const pf2_mesh* mesh, const pf2_params* params, long long capacit
 int pf2_destroy(pf2_handle h) {
     if (!h) return 0;
     if (h->stream) hipStreamSynchronize(h->stream);
+    invalidate_graphs(h);
     free_particles(h);
     void* ptrs[] = {h->dX, h->dcv, h->dcd, h->dstar_off, h->dstar_cell, h->dstar_dir, h->ddof_off, h->ddof_cell,
-                    h->dftol, h->dcellX, h->dcell_star, h->dcellV, h->dcellDV, h->dV, h->dVold, h->dtmp, h->dV32, h->dVold32, h->dgeo32, h->off[0], h->off[1], h->arr_cnt,
-                    h->arr_off, h->arr_fill, h->final_cnt, h->spawn_cnt, h->spawn_off, h->hitmask, h->nsurv,
-                    h->partial, h->counters, h->qcount, h->qprefix, h->pcounter, h->stats, h->scan_tmp, h->w_dof, h->w_mask, h->w_fix,
-                    h->w_vwall, h->w_cells, h->w_bc, h->w_val};
-    for (void* p : ptrs)
+                    h->dftol, h->dcellX, h->dcell_star, h->dcellV, h->dcellDV, h->dV, h->dVold, h->dtmp, h->dV32,
+                    h->dVold32, h->dgeo32, h->off[0], h->off[1], h->arr_cnt, h->arr_off, h->arr_fill, h->final_cnt,
+                    h->spawn_cnt, h->spawn_off, h->hitmask, h->nsurv, h->partial, h->counters, h->qcount, h->qprefix,
+                    h->pcounter, h->stats, h->scan_tmp};
+    if (!h->in_block)
+        for (void* p : ptrs)
+            if (p) hipFree(p);
+    // wall lists are allocated after creation, always with hipMalloc
+    void* wptrs[] = {h->w_dof, h->w_mask, h->w_fix, h->w_vwall, h->w_cells, h->w_bc, h->w_val};
+    for (void* p : wptrs)
         if (p) hipFree(p);
     for (auto& e : h->ev_free) {
         hipEventDestroy(e.first);
@@ -733,13 +873,14 @@ int pf2_destroy(pf2_handle h) {
         hipEventDestroy(e.second.first);
         hipEventDestroy(e.second.second);
     }
-    if (h->stream) hipStreamDestroy(h->stream);
+    if (h->stream && h->own_stream) hipStreamDestroy(h->stream);
     delete h;
     return 0;
 }
 
 int pf2_set_params(pf2_handle h, const pf2_params* params) {
     if (!h) return fail(PF2_E_ARG, "handle is NULL");
+    invalidate_graphs(h);
     if (int r = check_params(h->dim, params)) return r;
     if (params->precision != h->params.precision)
         return fail(PF2_E_STATE, "precision is fixed at pf2_create");
@@ -776,6 +917,7 @@ int pf2_download_field(pf2_handle h, double* V) {
 
 int pf2_seed_particles(pf2_handle h) {
     if (!h) return fail(PF2_E_ARG, "handle is NULL");
+    invalidate_graphs(h);
     long long n = 0;
     if (int r = get_count(h, &n)) return r;
     if (n != 0) return fail(PF2_E_STATE, "seed_particles on a handle that already holds particles");
@@ -832,6 +974,7 @@ int pf2_download_particles(pf2_handle h, int* cell, uint32_t* id, double* x, dou
 
 int pf2_insert_particles(pf2_handle h, int n, const int* cell, const double* xi, const double* v) {
     if (!h) return fail(PF2_E_ARG, "handle is NULL");
+    invalidate_graphs(h);
     if (n < 0 || (n > 0 && (!cell || !xi))) return fail(PF2_E_ARG, "invalid particle arrays");
     const int dim = h->dim;
     for (int i = 0; i < n; ++i) {
@@ -938,12 +1081,44 @@ int pf2_distribute_particle_velocities_to_grid(pf2_handle h, double* V_out) {
 
 int pf2_step(pf2_handle h, double* V_out) {
     if (!h) return fail(PF2_E_ARG, "handle is NULL");
-    if (int r = (h->dim == 2 ? run_move<2>(h, 1, 1) : run_move<3>(h, 1, 1))) return r;
-    return pf2_distribute_particle_velocities_to_grid(h, V_out);
+    if (int r = run_step(h, nullptr)) return r;
+    if (V_out) return download_field(h, V_out);
+    return 0;
+}
+
+int pf2_step_with_field(pf2_handle h, const double* V_device) {
+    if (!h) return fail(PF2_E_ARG, "handle is NULL");
+    if (!V_device) return fail(PF2_E_ARG, "V_device is NULL");
+    return run_step(h, V_device);
+}
+
+int pf2_set_graphs(pf2_handle h, int on) {
+    if (!h) return fail(PF2_E_ARG, "handle is NULL");
+    HIPC(hipStreamSynchronize(h->stream));
+    invalidate_graphs(h);
+    h->graphs = on != 0;
+    return 0;
+}
+
+int pf2_memory_bytes(pf2_handle h, unsigned long long* bytes) {
+    if (!h || !bytes) return fail(PF2_E_ARG, "handle/bytes is NULL");
+    *bytes = h->bytes;
+    return 0;
+}
+
+int pf2_estimate_bytes(const pf2_mesh* mesh, const pf2_params* params, long long capacity,
+                       unsigned long long* bytes) {
+    if (!mesh || !bytes) return fail(PF2_E_ARG, "mesh/bytes is NULL");
+    if (mesh->dim != 2 && mesh->dim != 3) return fail(PF2_E_ARG, "dim must be 2 or 3");
+    if (int r = check_params(mesh->dim, params)) return r;
+    const long long cap = capacity > 0 ? capacity : default_capacity(mesh->dim, mesh->n_cells, params);
+    *bytes = estimate_bytes(mesh->dim, mesh->n_cells, mesh->n_vertices, mesh->n_dofs, cap, params);
+    return 0;
 }
 
 int pf2_set_walls(pf2_handle h, int n, const int* vertex, const int* dof, const double* bc, const int* mask) {
     if (!h) return fail(PF2_E_ARG, "handle is NULL");
+    invalidate_graphs(h);
     hipFree(h->w_dof); hipFree(h->w_mask); hipFree(h->w_fix); hipFree(h->w_vwall); hipFree(h->w_cells);
     hipFree(h->w_bc); hipFree(h->w_val);
     h->w_dof = h->w_mask = h->w_fix = h->w_vwall = h->w_cells = nullptr;

@@ -326,7 +326,7 @@ __global__ void __launch_bounds__(256) k_move32(DevMesh<DIM> m, const float* __r
 // FP64 relocation of queued particles (the reference rule), then the
 // remaining FP32 substeps
 template <int DIM>
-__global__ void __launch_bounds__(256) k_search_resume32(DevMesh<DIM> m, const float* __restrict__ V32,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DIM == 2 ? 4 : 1))) k_search_resume32(DevMesh<DIM> m, const float* __restrict__ V32,
                                                          const float* __restrict__ geo, const int* __restrict__ pcell,
                                                          float* __restrict__ pxi, double* __restrict__ pvext,
                                                          long long cap, DevParams pp, DevWork w, DevQueues qu,
