@@ -3,6 +3,13 @@ import sys
 
 import pytest
 
+try:
+    # Load torch's HIP runtime before the library: the process then holds one
+    # runtime, shared by the library and the torch tensors some tests use.
+    import torch  # noqa: F401
+except ImportError:
+    pass
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
