@@ -95,7 +95,7 @@ struct DevMesh {
     const int* dof_cell;   // packed: cell * NV + local vertex
     const double* ftol;    // per cell: tolerance of the clearly-outside test (-1: none)
     const double* cellX;   // per cell: its vertex coordinates packed [NV][DIM] (copies of X)
-    const int* cell_star;  // per cell and local vertex: offset of that vertex's star
+    const int2* cell_star; // per cell and local vertex: [begin, end) of that vertex's star
     const double* cellV;   // per cell: V at its DoFs [NV][DIM] (refreshed when V changes)
     const double* cellDV;  // per cell: V - V_old at its DoFs [NV][DIM]
 };
@@ -192,35 +192,24 @@ __device__ __forceinline__ void interp(const double (*Fc)[DIM], const double* xi
 
 template <int DIM>
 __device__ __forceinline__ int part_key(const DevParams& pp, const double* xi) {
-    int key = 0, stride = 1;
+    // bits 0-15: part index with the phantom value q_d (the grouping key);
+    // bits 16-23: 1 + ordinal of an in-range part, 0 for a phantom part
+    int key = 0, stride = 1, ord = 0;
+    bool in = true;
 #pragma unroll
     for (int d = 0; d < DIM; ++d) {
         const double h = 1.0 / pp.q[d];
         const unsigned num = (unsigned)(xi[d] / h);
         key += (int)num * stride;
         stride *= pp.q[d] + 1;
+        in = in && (int)num < pp.q[d];
+        ord = ord * pp.q[d] + (int)num;
     }
-    return key;
+    return key | ((in ? ord + 1 : 0) << 16);
 }
 
-// ordinal of an in-range part in seeding order (first direction outermost), -1 for phantom parts
-template <int DIM>
-__device__ __forceinline__ int part_ordinal(const DevParams& pp, int key) {
-    int num[3];
-    int k = key;
-#pragma unroll
-    for (int d = 0; d < DIM; ++d) {
-        num[d] = k % (pp.q[d] + 1);
-        k /= pp.q[d] + 1;
-    }
-    int o = 0;
-#pragma unroll
-    for (int d = 0; d < DIM; ++d) {
-        if (num[d] >= pp.q[d]) return -1;
-        o = o * pp.q[d] + num[d];
-    }
-    return o;
-}
+// in-range part ordinal of a packed part key, -1 for a phantom part
+__device__ __forceinline__ int part_ordinal(int pkey) { return (pkey >> 16) - 1; }
 
 template <int DIM>
 __device__ __forceinline__ void lattice_point(const DevParams& pp, int ordinal, double* xi) {
@@ -246,12 +235,33 @@ __device__ __forceinline__ void lattice_point(const DevParams& pp, int ordinal, 
 // that is the containing cell with the largest key, earliest star position on
 // ties, which is what the loop below selects.  Returns -1 if no star cell
 // contains the point.
-constexpr int MAXSTAR = 16;
+struct NoPrefetch {
+    __device__ void load(int) {}
+};
 
+// nodal velocities of a candidate cell (FP64 path)
 template <int DIM>
+struct PrefetchV64 {
+    const double* cellV;
+    double Vc[Q1<DIM>::NV][DIM];
+    __device__ void load(int cc) {
+        const double* cf = cellV + (size_t)cc * Q1<DIM>::NV * DIM;
+#pragma unroll
+        for (int k = 0; k < Q1<DIM>::NV; ++k)
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) Vc[k][d] = cf[k * DIM + d];
+    }
+};
+
+// `Q` returns the vertices of the cell found; `pre.load(cell)` is called for
+// each candidate before its containment test, so that a caller can have the
+// candidate's other data loaded alongside its vertices (the data of the
+// last candidate loaded is the found cell's).
+template <int DIM, class Pre>
 __device__ int relocate(const DevMesh<DIM>& m, int c, const double (*P)[DIM], const double* xi_old,
-                        const double* vext, const double* x, double* xi_out) {
+                        const double* vext, const double* x, double* xi_out, double (*Q)[DIM], Pre& pre) {
     constexpr int NV = Q1<DIM>::NV;
+    constexpr int CH = 4;   // star entries read together
     double xold[DIM], vn[DIM], t[DIM];
     forward<DIM>(P, xi_old, xold);
     const double rv = 1.0 / vnorm<DIM>(vext);
@@ -287,34 +297,43 @@ __device__ int relocate(const DevMesh<DIM>& m, int c, const double (*P)[DIM], co
     rn = 1.0 / vnorm<DIM>(dir);
 #pragma unroll
     for (int d = 0; d < DIM; ++d) dir[d] *= rn;
-    const int s0 = m.cell_star[(size_t)c * NV + cvk];
-    const int gv = m.cv[(size_t)c * NV + cvk];
-    const int s1 = m.star_off[gv + 1];
-    const int n = s1 - s0;
+    const int2 se = m.cell_star[(size_t)c * NV + cvk];
+    const int s0 = se.x, n = se.y - se.x;
     if (n <= 32) {
         // candidates in decreasing key order (ties: star order), first
         // containing cell wins -- the reference's sorted loop, stopping early.
-        // Each pass re-reads the star (cache hits) instead of holding the keys
-        // in registers, which keeps the search kernels' occupancy up.
+        // Each pass re-reads the star (cache hits, CH entries at a time)
+        // instead of holding the keys in registers.
         unsigned tried = 0;
         for (int it = 0; it < n; ++it) {
-            int best = -1;
+            int best_e = -1, cc = -1;
             double bk = 0.0;
-            for (int e = 0; e < n; ++e) {
-                if ((tried >> e) & 1u) continue;
-                double sd[DIM];
+            for (int e0 = 0; e0 < n; e0 += CH) {
+                double key[CH];
+                int cell[CH];
 #pragma unroll
-                for (int d = 0; d < DIM; ++d) sd[d] = m.star_dir[(size_t)(s0 + e) * DIM + d];
-                const double key = vdot<DIM>(sd, dir);
-                if (best < 0 || key > bk) {
-                    best = e;
-                    bk = key;
+                for (int u = 0; u < CH; ++u) {
+                    const int e = e0 + u < n ? e0 + u : n - 1;
+                    double sd[DIM];
+#pragma unroll
+                    for (int d = 0; d < DIM; ++d) sd[d] = m.star_dir[(size_t)(s0 + e) * DIM + d];
+                    key[u] = vdot<DIM>(sd, dir);
+                    cell[u] = m.star_cell[s0 + e];
+                }
+#pragma unroll
+                for (int u = 0; u < CH; ++u) {
+                    const int e = e0 + u;
+                    if (e < n && !((tried >> e) & 1u) && (best_e < 0 || key[u] > bk)) {
+                        best_e = e;
+                        bk = key[u];
+                        cc = cell[u];
+                    }
                 }
             }
-            tried |= 1u << best;
-            const int cc = m.star_cell[s0 + best];
-            double Q[NV][DIM], r[DIM];
+            tried |= 1u << best_e;
+            double r[DIM];
             load_vertices<DIM>(m, cc, Q);
+            pre.load(cc);
             if (inverse_map<DIM>(Q, x, r, m.ftol[cc], true) && inside_unit<DIM>(r)) {
 #pragma unroll
                 for (int d = 0; d < DIM; ++d) xi_out[d] = r[d];
@@ -323,16 +342,17 @@ __device__ int relocate(const DevMesh<DIM>& m, int c, const double (*P)[DIM], co
         }
         return -1;
     }
+    // stars of more than 32 cells: the containing cell with the largest key
     int bestc = -1;
     double bestkey = 0.0;
-    for (int e = s0; e < s1; ++e) {
+    for (int e = s0; e < s0 + n; ++e) {
         double sd[DIM];
 #pragma unroll
         for (int d = 0; d < DIM; ++d) sd[d] = m.star_dir[(size_t)e * DIM + d];
         const double key = vdot<DIM>(sd, dir);
         if (bestc >= 0 && !(key > bestkey)) continue;
         const int cc = m.star_cell[e];
-        double Q[NV][DIM], r[DIM];
+        double r[DIM];
         load_vertices<DIM>(m, cc, Q);
         if (inverse_map<DIM>(Q, x, r, m.ftol[cc], true) && inside_unit<DIM>(r)) {
             bestc = cc;
@@ -340,6 +360,10 @@ __device__ int relocate(const DevMesh<DIM>& m, int c, const double (*P)[DIM], co
 #pragma unroll
             for (int d = 0; d < DIM; ++d) xi_out[d] = r[d];
         }
+    }
+    if (bestc >= 0) {
+        load_vertices<DIM>(m, bestc, Q);
+        pre.load(bestc);
     }
     return bestc;
 }
@@ -515,7 +539,7 @@ __device__ __forceinline__ void park_particle(const DevParticles& A, const DevWo
 // k_move: [correction] + substeps until done or the first failed own-cell test.
 //   correct_particles_velocities  TurekBenchmark/pfem2particle.cpp:792-811
 template <int DIM>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DIM == 2 ? 3 : 1))) k_move(DevMesh<DIM> m, const double* __restrict__ V,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DIM == 2 ? 3 : 2))) k_move(DevMesh<DIM> m, const double* __restrict__ V,
                                               const double* __restrict__ Vold, DevParticles A, DevParams pp,
                                               DevWork w, DevQueues qu, int do_correct, int do_move) {
     constexpr int NV = Q1<DIM>::NV;
@@ -560,7 +584,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DIM ==
 // `round`, then the remaining substeps; a particle that needs another search
 // is queued for round + 1, or, in the last round, searched inline.
 template <int DIM>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DIM == 2 ? 3 : 1))) k_search_resume(DevMesh<DIM> m, const double* __restrict__ V,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DIM == 2 ? 3 : 2))) k_search_resume(DevMesh<DIM> m, const double* __restrict__ V,
                                                        DevParticles A, DevParams pp, DevWork w, DevQueues qu,
                                                        int round) {
     constexpr int NV = Q1<DIM>::NV;
@@ -585,11 +609,13 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DIM ==
                 xi[d] = A.xi[(size_t)d * A.cap + i];
                 vext[d] = qu.park_vext[(size_t)d * A.cap + i];
             }
+            double P[NV][DIM];
+            load_vertices<DIM>(m, c, P);
             for (;;) {
-                double P[NV][DIM], r[DIM];
-                load_vertices<DIM>(m, c, P);
+                double Q[NV][DIM], r[DIM];
                 ++searches;
-                const int cn = relocate<DIM>(m, c, P, xi, vext, x, r);
+                PrefetchV64<DIM> pre{m.cellV};
+                const int cn = relocate<DIM>(m, c, P, xi, vext, x, r, Q, pre);
                 if (cn < 0) {
                     c = -1;
                     ++lost;
@@ -600,9 +626,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DIM ==
 #pragma unroll
                 for (int d = 0; d < DIM; ++d) xi[d] = r[d];
                 ++s;
-                double Vc[NV][DIM];
-                load_cell<DIM>(m, c, P, Vc);
-                st = advance64<DIM>(m, V, pp, A, i, c, x, xi, vext, s, P, Vc, m.ftol[c]);
+#pragma unroll
+                for (int k = 0; k < NV; ++k)
+#pragma unroll
+                    for (int d = 0; d < DIM; ++d) P[k][d] = Q[k][d];
+                st = advance64<DIM>(m, V, pp, A, i, c, x, xi, vext, s, P, pre.Vc, m.ftol[c]);
                 if (st == 0 || !last) break;
             }
         }
@@ -678,12 +706,14 @@ template <int DIM>
 __global__ void __launch_bounds__(256) k_cell_count(int n_cells, const int* __restrict__ old_off,
                                                     const uint32_t* __restrict__ pid, DevParams pp, DevWork w,
                                                     int reseed) {
-    // rows padded by one word so that the groups of a wavefront reading the
-    // same column hit different LDS banks
-    __shared__ int s_idx[GROUPS][GCAP + 1];
-    __shared__ int s_key[GROUPS][GCAP + 1];
-    __shared__ uint32_t s_id[GROUPS][GCAP + 1];
-    __shared__ unsigned char s_surv[GROUPS][GCAP + 4];
+    // Staged rows: candidate index; (part key << 32 | id), all ones for a
+    // candidate that left; id of a survivor, all ones otherwise.  Rows are
+    // padded to a multiple of 4 entries with those fill values, so the scans
+    // below read 16 bytes at a time, and their strides put the four groups of
+    // a wavefront on different LDS banks.
+    __shared__ int s_idx[GROUPS][GCAP + 4];
+    __shared__ __align__(16) unsigned long long s_kid[GROUPS][GCAP + 2];
+    __shared__ __align__(16) uint32_t s_sid[GROUPS][GCAP + 4];
     const int g = threadIdx.x / GS, gl = threadIdx.x % GS;
     const unsigned long long all = pp.nparts_in >= 64 ? ~0ull : ((1ull << pp.nparts_in) - 1);
     int deleted = 0;
@@ -693,33 +723,46 @@ __global__ void __launch_bounds__(256) k_cell_count(int n_cells, const int* __re
         const int n_st = cv ? old_off[c + 1] - old_off[c] : 0;
         const int k = cv ? n_st + w.arr_cnt[c] : 0;
         const bool small = k <= GCAP;
+        const int kr = (k + 3) & ~3;
         unsigned long long hit = 0;
         int nvalid = 0, nsurv = 0;
         if (small)
-            for (int j = gl; j < k; j += GS) {
-                const int idx = candidate(w, old_off, c, n_st, j);
-                const bool valid = w.dest[idx] == c;
-                s_idx[g][j] = valid ? idx : -1;
-                s_key[g][j] = valid ? w.pkey[idx] : -1;
-                s_id[g][j] = pid[idx];
+            for (int j = gl; j < kr; j += GS) {
+                int idx = -1;
+                unsigned long long kid = ~0ull;
+                if (j < k) {
+                    const int cidx = candidate(w, old_off, c, n_st, j);
+                    if (w.dest[cidx] == c) {
+                        idx = cidx;
+                        kid = ((unsigned long long)(uint32_t)w.pkey[cidx] << 32) | pid[cidx];
+                    }
+                }
+                s_idx[g][j] = idx;
+                s_kid[g][j] = kid;
+                s_sid[g][j] = ~0u;
             }
         wave_sync();
         if (small)
             for (int j = gl; j < k; j += GS) {
-                int surv = 0;
-                if (s_idx[g][j] >= 0) {
-                    ++nvalid;
-                    const int key = s_key[g][j];
-                    const uint32_t id = s_id[g][j];
-                    const int o = part_ordinal<DIM>(pp, key);
-                    if (o >= 0) hit |= 1ull << o;
-                    int rank = 0;
-                    if (reseed)
-                        for (int t = 0; t < k; ++t) rank += (s_key[g][t] == key && s_id[g][t] < id) ? 1 : 0;
-                    surv = rank < pp.max_per_part || !reseed;
-                    nsurv += surv;
+                if (s_idx[g][j] < 0) continue;
+                ++nvalid;
+                const unsigned long long me = s_kid[g][j];
+                const int key = (int)(me >> 32);
+                const int o = part_ordinal(key);
+                if (o >= 0) hit |= 1ull << o;
+                // rank = particles of the same part with a smaller id
+                int rank = 0;
+                if (reseed) {
+                    const unsigned long long lo = me & 0xffffffff00000000ull;
+                    for (int t = 0; t < kr; t += 2) {
+                        const ulonglong2 v = *reinterpret_cast<const ulonglong2*>(&s_kid[g][t]);
+                        rank += (v.x >= lo && v.x < me) ? 1 : 0;
+                        rank += (v.y >= lo && v.y < me) ? 1 : 0;
+                    }
                 }
-                s_surv[g][j] = (unsigned char)surv;
+                const bool surv = rank < pp.max_per_part || !reseed;
+                nsurv += surv ? 1 : 0;
+                if (surv) s_sid[g][j] = (uint32_t)me;
             }
         else
             // large cell: the same census straight from global memory
@@ -729,7 +772,7 @@ __global__ void __launch_bounds__(256) k_cell_count(int n_cells, const int* __re
                 ++nvalid;
                 const int key = w.pkey[idx];
                 const uint32_t id = pid[idx];
-                const int o = part_ordinal<DIM>(pp, key);
+                const int o = part_ordinal(key);
                 if (o >= 0) hit |= 1ull << o;
                 int rank = 0;
                 if (reseed)
@@ -743,18 +786,22 @@ __global__ void __launch_bounds__(256) k_cell_count(int n_cells, const int* __re
             }
         if (__ballot(!small) != 0) __threadfence();
         wave_sync();
-        // positions of the survivors in ascending id order
+        // positions of the survivors in ascending id order (ids are below
+        // 2^32 - 1, the fill value)
         if (small)
             for (int j = gl; j < k; j += GS) {
                 const int idx = s_idx[g][j];
                 if (idx < 0) continue;
-                if (!s_surv[g][j]) {
+                const uint32_t id = s_sid[g][j];
+                if (id == ~0u) {
                     w.spos[idx] = -1;
                     continue;
                 }
-                const uint32_t id = s_id[g][j];
                 int pos = 0;
-                for (int t = 0; t < k; ++t) pos += (s_surv[g][t] && s_id[g][t] < id) ? 1 : 0;
+                for (int t = 0; t < kr; t += 4) {
+                    const uint4 v = *reinterpret_cast<const uint4*>(&s_sid[g][t]);
+                    pos += (v.x < id ? 1 : 0) + (v.y < id ? 1 : 0) + (v.z < id ? 1 : 0) + (v.w < id ? 1 : 0);
+                }
                 w.spos[idx] = pos;
             }
         else
@@ -1162,7 +1209,9 @@ __global__ void __launch_bounds__(256) k_resort(DevMesh<DIM> m, DevParticles A, 
             for (int d = 0; d < DIM; ++d) xi[d] = r[d];
         } else {
             ++searches;
-            c = relocate<DIM>(m, c, P, xi, vext, x, r);
+            double Q[NV][DIM];
+            NoPrefetch pre;
+            c = relocate<DIM>(m, c, P, xi, vext, x, r, Q, pre);
             if (c < 0) ++lost;
             else
 #pragma unroll

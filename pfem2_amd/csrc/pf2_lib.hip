@@ -98,7 +98,7 @@ struct pf2_handle_s {
     hipStream_t stream = nullptr;
     // host copies
     std::vector<double> X;
-    std::vector<int> cv, cd, star_off, star_cell, dof_off, dof_cell, cell_star;
+    std::vector<int> cv, cd, star_off, star_cell, dof_off, dof_cell, cell_star;   // cell_star: pairs
     std::vector<double> star_dir;
     // device mesh
     double* dX = nullptr;
@@ -107,7 +107,7 @@ struct pf2_handle_s {
     int *ddof_off = nullptr, *ddof_cell = nullptr;
     double* dftol = nullptr;
     double* dcellX = nullptr;
-    int* dcell_star = nullptr;
+    int2* dcell_star = nullptr;
     double *dcellV = nullptr, *dcellDV = nullptr;
     // fields (node-major)
     double *dV = nullptr, *dVold = nullptr, *dtmp = nullptr;
@@ -628,7 +628,7 @@ size_t estimate_bytes(int dim, long long nc, long long nvt, long long nd, long l
     const long long nv = 1LL << dim, nck = nc * nv;
     size_t b = 0;
     b += nvt * dim * 8 + nck * 4 * 2 + (nvt + 1) * 4 + nck * 4 + nck * dim * 8;       // X, cv, cd, star
-    b += (nd + 1) * 4 + nck * 4 + nc * 8 + nck * dim * 8 + nck * 4 + nck * dim * 8 * 2;  // dof map, ftol, cellX/star/V/dV
+    b += (nd + 1) * 4 + nck * 4 + nc * 8 + nck * dim * 8 + nck * 8 + nck * dim * 8 * 2;  // dof map, ftol, cellX/star/V/dV
     b += nd * dim * 8 * 3;                                                              // V, V_old, staging
     b += (nc + 1) * 4 * 8 + nc * 8 + nc * 4 + nc * nv * (dim + 1) * 8 + 16;             // per-cell work, partials
     b += (size_t)(MOVE_ROUNDS + 2) * (QSHARDS + 1) * QSTRIDE * 4 + (size_t)(MOVE_ROUNDS + 2) * (QSHARDS + 2) * 4;
@@ -767,8 +767,11 @@ int pf2_create_in(const pf2_mesh* mesh, const pf2_params* params, long long capa
             for (int k = 0; k < h->nv; ++k) h->star_cell[fill[h->cv[(size_t)c * h->nv + k]]++] = c;
     }
     compute_star_dirs(h);
-    h->cell_star.resize(nck);
-    for (size_t i = 0; i < nck; ++i) h->cell_star[i] = h->star_off[h->cv[i]];
+    h->cell_star.resize(2 * nck);
+    for (size_t i = 0; i < nck; ++i) {
+        h->cell_star[2 * i] = h->star_off[h->cv[i]];
+        h->cell_star[2 * i + 1] = h->star_off[h->cv[i] + 1];
+    }
     // DoF -> (cell, local vertex), ascending cells
     h->dof_off.assign(h->n_dofs + 1, 0);
     for (size_t i = 0; i < nck; ++i) h->dof_off[h->cd[i] + 1]++;

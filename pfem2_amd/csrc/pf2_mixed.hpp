@@ -208,6 +208,16 @@ __device__ __forceinline__ void load_cell32(const float* __restrict__ geo, const
         for (int d = 0; d < DIM; ++d) Fc[k][d] = cellV32[((size_t)c * NV + k) * DIM + d];
 }
 
+// FP32 coefficients and nodal velocities of a candidate cell
+template <int DIM>
+struct PrefetchCell32 {
+    const float* geo;
+    const float* V32;
+    float g[DIM == 2 ? 6 : 21];
+    float Vc[Q1<DIM>::NV][DIM];
+    __device__ void load(int cc) { load_cell32<DIM>(geo, V32, cc, g, Vc); }
+};
+
 template <int DIM>
 __device__ __forceinline__ int part_key32(const DevParams& pp, const float* xi) {
     double xd[DIM];
@@ -326,7 +336,7 @@ __global__ void __launch_bounds__(256) k_move32(DevMesh<DIM> m, const float* __r
 // FP64 relocation of queued particles (the reference rule), then the
 // remaining FP32 substeps
 template <int DIM>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DIM == 2 ? 4 : 1))) k_search_resume32(DevMesh<DIM> m, const float* __restrict__ V32,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DIM == 2 ? 4 : 2))) k_search_resume32(DevMesh<DIM> m, const float* __restrict__ V32,
                                                          const float* __restrict__ geo, const int* __restrict__ pcell,
                                                          float* __restrict__ pxi, double* __restrict__ pvext,
                                                          long long cap, DevParams pp, DevWork w, DevQueues qu,
@@ -353,10 +363,13 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DIM ==
                 xi[d] = pxi[(size_t)d * cap + i];
                 u[d] = (float)qu.park_vext[(size_t)d * cap + i];
             }
+            // 2-D: the found cell's vertices and FP32 data are loaded with the
+            // candidate and reused; 3-D reloads them (register budget)
+            double P[NV][DIM];
+            load_vertices<DIM>(m, c, P);
             for (;;) {
                 ++searches;
-                double P[NV][DIM], xid[DIM], x[DIM], vext[DIM], r[DIM];
-                load_vertices<DIM>(m, c, P);
+                double Q[NV][DIM], xid[DIM], x[DIM], vext[DIM], r[DIM];
 #pragma unroll
                 for (int d = 0; d < DIM; ++d) {
                     xid[d] = (double)xi[d];
@@ -365,25 +378,50 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DIM ==
                 forward<DIM>(P, xid, x);
 #pragma unroll
                 for (int d = 0; d < DIM; ++d) x[d] = x[d] + (double)(u[d] * h32);
-                const int cn = relocate<DIM>(m, c, P, xid, vext, x, r);
+                int cn;
+                if constexpr (DIM == 2) {
+                    PrefetchCell32<DIM> pre{geo, V32};
+                    cn = relocate<DIM>(m, c, P, xid, vext, x, r, Q, pre);
+                    if (cn >= 0) {
+#pragma unroll
+                        for (int d = 0; d < DIM; ++d) {
+                            const float t = (float)r[d];
+                            xi[d] = t < 0.0f ? 0.0f : (t > 1.0f ? 1.0f : t);
+                        }
+                        ++s;
+                        st = advance32<DIM>(pp, h32, i, pvext, cap, xi, u, s, pre.g, pre.Vc);
+                    }
+                } else {
+                    NoPrefetch pre;
+                    cn = relocate<DIM>(m, c, P, xid, vext, x, r, Q, pre);
+                    if (cn >= 0) {
+#pragma unroll
+                        for (int d = 0; d < DIM; ++d) {
+                            const float t = (float)r[d];
+                            xi[d] = t < 0.0f ? 0.0f : (t > 1.0f ? 1.0f : t);
+                        }
+                        float g[NG], Vc[NV][DIM];
+                        load_cell32<DIM>(geo, V32, cn, g, Vc);
+                        ++s;
+                        st = advance32<DIM>(pp, h32, i, pvext, cap, xi, u, s, g, Vc);
+                    }
+                }
                 if (cn < 0) {
                     c = -1;
                     ++lost;
                     st = 0;
                     break;
                 }
-                // reference points are kept inside [0,1] after rounding to FP32
-#pragma unroll
-                for (int d = 0; d < DIM; ++d) {
-                    const float t = (float)r[d];
-                    xi[d] = t < 0.0f ? 0.0f : (t > 1.0f ? 1.0f : t);
-                }
                 c = cn;
-                ++s;
-                float g[NG], Vc[NV][DIM];
-                load_cell32<DIM>(geo, V32, c, g, Vc);
-                st = advance32<DIM>(pp, h32, i, pvext, cap, xi, u, s, g, Vc);
                 if (st == 0 || !last) break;
+                if constexpr (DIM == 2) {
+#pragma unroll
+                    for (int k = 0; k < NV; ++k)
+#pragma unroll
+                        for (int d = 0; d < DIM; ++d) P[k][d] = Q[k][d];
+                } else {
+                    load_vertices<DIM>(m, c, P);
+                }
             }
         }
         if (!last) park32<DIM>(pxi, cap, w, qu, round + 1, valid && st == 1, i, c, xi, u, s);
