@@ -121,10 +121,13 @@ def kernel_bytes(name, info, m, dim=2):
             + q[0] * (dim * wx * (2 if name == "k_move" else 1) + 4 + 4 + 4 + 8 * dim)
         mesh = nc * nv * 4 + nd * dim * wv * 2 + (m.n_vertices * dim * 8 if name == "k_move" else nc * 24)
         return reads + writes + mesh
-    if name in ("k_search_resume", "k_search_resume32"):
-        wx = 8 if name == "k_search_resume" else 4
-        per = 4 + 4 + 4 + 4 + dim * wx * (2 if name == "k_search_resume" else 1) + 8 * dim + 8 + dim * wx
-        return int(sum(q) * per / max(len(q), 1))
+    if name.startswith("k_search_resume"):
+        # one launch per relocation round; round r handles q[r] particles
+        mm = re.match(r"(k_search_resume(?:32)?)(?:_r(\d))?$", name)
+        base, r = mm.group(1), int(mm.group(2) or 0)
+        wx = 8 if base == "k_search_resume" else 4
+        per = 8 + 4 + 4 + dim * wx * (2 if base == "k_search_resume" else 1) + 8 * dim + 8 + dim * wx
+        return int(q[r] * per) if r < len(q) else None
     if name == "k_group":
         return n0 * 8 + arr * 8 + nc * 4
     if name == "k_cell_count":
@@ -465,11 +468,12 @@ def channel_main(args, torch):
     st.close()
     o = _oracle(W)
     cpu_s, upd = 0.0, 0
+    Vo = None
     for _ in range(steps):
         upd += o.n_global_particles()
         cpu_s += o.time_steps(1)
+        Vo = o.field()          # the projection of this step
         o.upload_fields(W.V, None)
-    Vo = o.field()
     gpu_rate = 0.5 * (n0 + n1) * steps / gpu_s
     out = base_line(args, W, {"updates_per_s": gpu_rate, "ms_per_step": gpu_s * 1e3 / steps, "particles": n1,
                               "p50_ms": None, "p99_ms": None}, "f64")

@@ -135,6 +135,7 @@ struct DevWork {
     int* nsurv;         // per cell: survivors
     double* partial;    // per cell: NV * (DIM + 1) projection sums
     int* counters;      // [0] n_particles, [2] overflow flags, [3] count before the last regroup
+    int* big;           // [0] number of cells listed for k_cell_big, then the cells
     uint32_t* particle_counter;
     unsigned long long* stats;   // STAT_SHARDS rows: lost, deleted, spawned, searches, zero-weight
 };
@@ -262,6 +263,12 @@ __device__ int relocate(const DevMesh<DIM>& m, int c, const double (*P)[DIM], co
                         const double* vext, const double* x, double* xi_out, double (*Q)[DIM], Pre& pre) {
     constexpr int NV = Q1<DIM>::NV;
     constexpr int CH = 4;   // star entries read together
+    // 2-D: the star ranges of all four vertices are read with no dependence
+    // on the vertex choice
+    int2 cs[DIM == 2 ? NV : 1];
+    if constexpr (DIM == 2)
+#pragma unroll
+        for (int k = 0; k < NV; ++k) cs[k] = m.cell_star[(size_t)c * NV + k];
     double xold[DIM], vn[DIM], t[DIM];
     forward<DIM>(P, xi_old, xold);
     const double rv = 1.0 / vnorm<DIM>(vext);
@@ -274,6 +281,7 @@ __device__ int relocate(const DevMesh<DIM>& m, int c, const double (*P)[DIM], co
     for (int d = 0; d < DIM; ++d) t[d] *= rn;
     double best = vdot<DIM>(vn, t);
     int cvk = 0;
+    int2 sel = cs[0];
     double dir[DIM];
 #pragma unroll
     for (int d = 0; d < DIM; ++d) dir[d] = P[0][d];
@@ -288,6 +296,7 @@ __device__ int relocate(const DevMesh<DIM>& m, int c, const double (*P)[DIM], co
         if (a > best) {
             cvk = k;
             best = a;
+            if constexpr (DIM == 2) sel = cs[k];
 #pragma unroll
             for (int d = 0; d < DIM; ++d) dir[d] = P[k][d];
         }
@@ -297,7 +306,9 @@ __device__ int relocate(const DevMesh<DIM>& m, int c, const double (*P)[DIM], co
     rn = 1.0 / vnorm<DIM>(dir);
 #pragma unroll
     for (int d = 0; d < DIM; ++d) dir[d] *= rn;
-    const int2 se = m.cell_star[(size_t)c * NV + cvk];
+    int2 se;
+    if constexpr (DIM == 2) se = sel;
+    else se = m.cell_star[(size_t)c * NV + cvk];
     const int s0 = se.x, n = se.y - se.x;
     if (n <= 32) {
         // candidates in decreasing key order (ties: star order), first
@@ -390,7 +401,7 @@ constexpr int QSHARDS = 256;
 constexpr int QSTRIDE = 32;   // ints between shard counters
 
 struct DevQueues {
-    int* data[2];         // ping-pong buffers: QSHARDS * shard_cap + overflow
+    int2* data[2];        // ping-pong buffers of (particle, cell): QSHARDS * shard_cap + overflow
     int* count;           // (MOVE_ROUNDS + 2) rounds x (QSHARDS + 1) counters, QSTRIDE apart
     int* prefix;          // per round: QSHARDS + 2 ints (exclusive prefix, overflow start, total)
     int shard_cap;
@@ -403,7 +414,7 @@ __device__ __forceinline__ int* queue_count(const DevQueues& qu, int round, int 
 }
 
 // append `value` for the lanes with pred set (all lanes of the wavefront call)
-__device__ __forceinline__ void queue_push(const DevQueues& qu, int round, bool pred, int value) {
+__device__ __forceinline__ void queue_push(const DevQueues& qu, int round, bool pred, int2 value) {
     const unsigned long long mask = __ballot(pred);
     if (mask == 0) return;
     const int lane = threadIdx.x & (WAVE - 1);
@@ -414,7 +425,7 @@ __device__ __forceinline__ void queue_push(const DevQueues& qu, int round, bool 
     base = __shfl(base, leader);
     const int slot = base + __popcll(mask & ((1ull << lane) - 1));
     const bool spill = pred && slot >= qu.shard_cap;
-    int* data = qu.data[round & 1];
+    int2* data = qu.data[round & 1];
     if (pred && !spill) data[(size_t)sh * qu.shard_cap + slot] = value;
     const unsigned long long smask = __ballot(spill);
     if (smask == 0) return;
@@ -461,8 +472,8 @@ __device__ __forceinline__ int queue_prefix(const DevQueues& qu, int round, int*
 }
 
 // particle index of queue entry j
-__device__ __forceinline__ int queue_get(const DevQueues& qu, int round, const int* s_pref, int j) {
-    const int* data = qu.data[round & 1];
+__device__ __forceinline__ int2 queue_get(const DevQueues& qu, int round, const int* s_pref, int j) {
+    const int2* data = qu.data[round & 1];
     if (j >= s_pref[QSHARDS]) return data[(size_t)QSHARDS * qu.shard_cap + (j - s_pref[QSHARDS])];
     int lo = 0, hi = QSHARDS - 1;   // last shard with s_pref <= j
     while (lo < hi) {
@@ -524,9 +535,8 @@ template <int DIM>
 __device__ __forceinline__ void park_particle(const DevParticles& A, const DevWork& w, const DevQueues& qu,
                                               int round_out, bool park, int i, int c, const double* x,
                                               const double* xi, const double* vext, int s) {
-    queue_push(qu, round_out, park, i);
+    queue_push(qu, round_out, park, make_int2(i, c));
     if (!park) return;
-    w.dest[i] = c;
     qu.park_s[i] = s;
 #pragma unroll
     for (int d = 0; d < DIM; ++d) {
@@ -597,8 +607,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DIM ==
     for (int base = blockIdx.x * blockDim.x + (threadIdx.x - lane); base < n; base += stride) {
         const int j = base + lane;
         const bool valid = j < n;
-        const int i = valid ? queue_get(qu, round, s_pref, j) : -1;
-        int c = valid ? w.dest[i] : -1;
+        const int2 e = valid ? queue_get(qu, round, s_pref, j) : make_int2(-1, -1);
+        const int i = e.x;
+        int c = e.y;
         int s = valid ? qu.park_s[i] : 0;
         int st = 0;
         double x[DIM], xi[DIM], vext[DIM];
@@ -764,27 +775,11 @@ __global__ void __launch_bounds__(256) k_cell_count(int n_cells, const int* __re
                 nsurv += surv ? 1 : 0;
                 if (surv) s_sid[g][j] = (uint32_t)me;
             }
-        else
-            // large cell: the same census straight from global memory
-            for (int j = gl; j < k; j += GS) {
-                const int idx = candidate(w, old_off, c, n_st, j);
-                if (w.dest[idx] != c) continue;
-                ++nvalid;
-                const int key = w.pkey[idx];
-                const uint32_t id = pid[idx];
-                const int o = part_ordinal(key);
-                if (o >= 0) hit |= 1ull << o;
-                int rank = 0;
-                if (reseed)
-                    for (int t = 0; t < k; ++t) {
-                        const int it = candidate(w, old_off, c, n_st, t);
-                        rank += (w.dest[it] == c && w.pkey[it] == key && pid[it] < id) ? 1 : 0;
-                    }
-                const int surv = rank < pp.max_per_part || !reseed;
-                nsurv += surv;
-                w.spos[idx] = surv ? 0 : -1;
-            }
-        if (__ballot(!small) != 0) __threadfence();
+        else if (cv && gl == 0) {
+            // more than GCAP candidates: census by k_cell_big
+            const int slot = atomicAdd(&w.big[0], 1);
+            w.big[1 + slot] = c;
+        }
         wave_sync();
         // positions of the survivors in ascending id order (ids are below
         // 2^32 - 1, the fill value)
@@ -804,23 +799,11 @@ __global__ void __launch_bounds__(256) k_cell_count(int n_cells, const int* __re
                 }
                 w.spos[idx] = pos;
             }
-        else
-            for (int j = gl; j < k; j += GS) {
-                const int idx = candidate(w, old_off, c, n_st, j);
-                if (w.dest[idx] != c || load_agent(&w.spos[idx]) < 0) continue;
-                const uint32_t id = pid[idx];
-                int pos = 0;
-                for (int t = 0; t < k; ++t) {
-                    const int it = candidate(w, old_off, c, n_st, t);
-                    pos += (w.dest[it] == c && load_agent(&w.spos[it]) >= 0 && pid[it] < id) ? 1 : 0;
-                }
-                w.spos[idx] = pos;
-            }
         wave_sync();
         hit = group_or(hit);
         nvalid = group_sum(nvalid);
         nsurv = group_sum(nsurv);
-        if (cv && gl == 0) {
+        if (cv && small && gl == 0) {
             const int spawns = reseed ? __popcll(all & ~hit) : 0;
             w.hitmask[c] = reseed ? hit : all;
             w.nsurv[c] = nsurv;
@@ -831,6 +814,114 @@ __global__ void __launch_bounds__(256) k_cell_count(int n_cells, const int* __re
     }
     deleted = wave_sum(deleted);
     if ((threadIdx.x & (WAVE - 1)) == 0 && deleted) stat_add(w.stats, 1, (unsigned long long)deleted);
+}
+
+// The census of k_cell_count for the cells it listed (more than GCAP
+// candidates): one 256-thread block per cell, candidates staged in LDS.
+// A cell with more than BIGCAP candidates sets overflow flag 2 and keeps no
+// particle.
+constexpr int BIGCAP = 2048;
+
+template <int DIM>
+__global__ void __launch_bounds__(256) k_cell_big(const int* __restrict__ old_off, const uint32_t* __restrict__ pid,
+                                                  DevParams pp, DevWork w, int reseed) {
+    __shared__ int s_idx[BIGCAP];
+    __shared__ __align__(16) unsigned long long s_kid[BIGCAP];
+    __shared__ __align__(16) uint32_t s_sid[BIGCAP];
+    __shared__ unsigned long long s_hit;
+    __shared__ int s_nvalid, s_nsurv;
+    const int tid = threadIdx.x;
+    const unsigned long long all = pp.nparts_in >= 64 ? ~0ull : ((1ull << pp.nparts_in) - 1);
+    const int nbig = w.big[0];
+    int deleted = 0;
+    for (int b = blockIdx.x; b < nbig; b += gridDim.x) {
+        const int c = w.big[1 + b];
+        const int n_st = old_off[c + 1] - old_off[c];
+        const int k = n_st + w.arr_cnt[c];
+        if (k > BIGCAP) {
+            for (int j = tid; j < k; j += blockDim.x) w.spos[candidate(w, old_off, c, n_st, j)] = -1;
+            if (tid == 0) {
+                atomicOr(&w.counters[2], 2);
+                w.hitmask[c] = all;
+                w.nsurv[c] = 0;
+                w.final_cnt[c] = 0;
+                w.spawn_cnt[c] = 0;
+            }
+            continue;
+        }
+        const int kr = (k + 3) & ~3;
+        if (tid == 0) {
+            s_hit = 0;
+            s_nvalid = 0;
+            s_nsurv = 0;
+        }
+        for (int j = tid; j < kr; j += blockDim.x) {
+            int idx = -1;
+            unsigned long long kid = ~0ull;
+            if (j < k) {
+                const int cidx = candidate(w, old_off, c, n_st, j);
+                if (w.dest[cidx] == c) {
+                    idx = cidx;
+                    kid = ((unsigned long long)(uint32_t)w.pkey[cidx] << 32) | pid[cidx];
+                }
+            }
+            s_idx[j] = idx;
+            s_kid[j] = kid;
+            s_sid[j] = ~0u;
+        }
+        __syncthreads();
+        unsigned long long hit = 0;
+        int nvalid = 0, nsurv = 0;
+        for (int j = tid; j < k; j += blockDim.x) {
+            if (s_idx[j] < 0) continue;
+            ++nvalid;
+            const unsigned long long me = s_kid[j];
+            const int o = part_ordinal((int)(me >> 32));
+            if (o >= 0) hit |= 1ull << o;
+            int rank = 0;
+            if (reseed) {
+                const unsigned long long lo = me & 0xffffffff00000000ull;
+                for (int t = 0; t < kr; t += 2) {
+                    const ulonglong2 v = *reinterpret_cast<const ulonglong2*>(&s_kid[t]);
+                    rank += (v.x >= lo && v.x < me) ? 1 : 0;
+                    rank += (v.y >= lo && v.y < me) ? 1 : 0;
+                }
+            }
+            const bool surv = rank < pp.max_per_part || !reseed;
+            nsurv += surv ? 1 : 0;
+            if (surv) s_sid[j] = (uint32_t)me;
+        }
+        if (hit) atomicOr(&s_hit, hit);
+        if (nvalid) atomicAdd(&s_nvalid, nvalid);
+        if (nsurv) atomicAdd(&s_nsurv, nsurv);
+        __syncthreads();
+        for (int j = tid; j < k; j += blockDim.x) {
+            const int idx = s_idx[j];
+            if (idx < 0) continue;
+            const uint32_t id = s_sid[j];
+            if (id == ~0u) {
+                w.spos[idx] = -1;
+                continue;
+            }
+            int pos = 0;
+            for (int t = 0; t < kr; t += 4) {
+                const uint4 v = *reinterpret_cast<const uint4*>(&s_sid[t]);
+                pos += (v.x < id ? 1 : 0) + (v.y < id ? 1 : 0) + (v.z < id ? 1 : 0) + (v.w < id ? 1 : 0);
+            }
+            w.spos[idx] = pos;
+        }
+        if (tid == 0) {
+            const unsigned long long hm = s_hit;
+            const int spawns = reseed ? __popcll(all & ~hm) : 0;
+            w.hitmask[c] = reseed ? hm : all;
+            w.nsurv[c] = s_nsurv;
+            w.final_cnt[c] = s_nsurv + spawns;
+            w.spawn_cnt[c] = spawns;
+            deleted += s_nvalid - s_nsurv;
+        }
+        __syncthreads();
+    }
+    if (tid == 0 && deleted) stat_add(w.stats, 1, (unsigned long long)deleted);
 }
 
 // ---------------------------------------------------------------------------
@@ -928,6 +1019,7 @@ __global__ void k_finalize(int n_cells, DevWork w, int* queue_count) {
     *w.particle_counter += (uint32_t)spawned;
     w.counters[3] = w.counters[0];   // count before the step (diagnostics)
     w.counters[0] = total;
+    w.big[0] = 0;
 }
 
 // ---------------------------------------------------------------------------

@@ -38,12 +38,15 @@ int fail(int code, const std::string& msg) {
 
 enum KernelId {
     K_MOVE = 0, K_GROUP, K_CELL_COUNT, K_CELL_WRITE, K_SCAN, K_FINALIZE, K_PARTIALS, K_NODE, K_WALL,
-    K_SEED, K_MISC, K_MOVE32, K_CELL_WRITE32, K_PARTIALS32, K_SEARCH, K_SEARCH32, K_QPREFIX, K_NKERNELS
+    K_SEED, K_MISC, K_MOVE32, K_CELL_WRITE32, K_PARTIALS32, K_SEARCH, K_SEARCH32, K_QPREFIX,
+    K_SEARCH_R1, K_SEARCH_R2, K_SEARCH32_R1, K_SEARCH32_R2, K_CELL_BIG, K_NKERNELS
 };
 const char* kKernelNames[K_NKERNELS] = {"k_move", "k_group", "k_cell_count", "k_cell_write", "scan",
                                         "k_finalize", "k_cell_partials", "k_node", "wall_fixup",
                                         "k_seed", "misc", "k_move32", "k_cell_write32", "k_cell_partials32",
-                                        "k_search_resume", "k_search_resume32", "k_queue_prefix"};
+                                        "k_search_resume", "k_search_resume32", "k_queue_prefix",
+                                        "k_search_resume_r1", "k_search_resume_r2", "k_search_resume32_r1",
+                                        "k_search_resume32_r2", "k_cell_big"};
 
 // bytes obtained through dalloc on this thread (a handle's footprint is the
 // difference across its creation)
@@ -128,10 +131,11 @@ struct pf2_handle_s {
     int* nsurv = nullptr;
     double* partial = nullptr;
     int* counters = nullptr;
+    int* big = nullptr;
     int* qcount = nullptr;
     int* qprefix = nullptr;
     long long shard_cap = 0;
-    int* queue[2] = {nullptr, nullptr};
+    int2* queue[2] = {nullptr, nullptr};
     double* park_vext = nullptr;
     int* park_s = nullptr;
     uint32_t* pcounter = nullptr;
@@ -299,6 +303,7 @@ DevWork dev_work(pf2_handle_s* h) {
     w.counters = h->counters;
     w.particle_counter = h->pcounter;
     w.stats = h->stats;
+    w.big = h->big;
     return w;
 }
 
@@ -381,7 +386,8 @@ void free_particles(pf2_handle_s* h) {
     }
     hipFree(h->dest); hipFree(h->pkey); hipFree(h->spos); hipFree(h->arr_list);
     hipFree(h->queue[0]); hipFree(h->queue[1]); hipFree(h->park_s); hipFree(h->park_vext);
-    h->dest = h->pkey = h->spos = h->arr_list = h->queue[0] = h->queue[1] = h->park_s = nullptr;
+    h->dest = h->pkey = h->spos = h->arr_list = h->park_s = nullptr;
+    h->queue[0] = h->queue[1] = nullptr;
     h->park_vext = nullptr;
 }
 
@@ -397,7 +403,7 @@ int get_count(pf2_handle_s* h, long long* n) {
     HIPC(hipMemcpyAsync(cnt, h->counters, sizeof(cnt), hipMemcpyDeviceToHost, h->stream));
     HIPC(hipStreamSynchronize(h->stream));
     if (cnt[2] & 1) return fail(PF2_E_CAPACITY, "particle capacity exceeded");
-    if (cnt[2] & 2) return fail(PF2_E_CAPACITY, "more than 1024 particles in one cell");
+    if (cnt[2] & 2) return fail(PF2_E_CAPACITY, "more than 2048 particles stayed in or arrived at one cell");
     *n = cnt[0];
     h->n_est = cnt[0];
     return 0;
@@ -421,6 +427,11 @@ int regroup(pf2_handle_s* h, int reseed) {
         Timed t(h, K_CELL_COUNT);
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cell_count<DIM>), dim3(cell_grid(nc)), dim3(256), 0, h->stream, nc,
                            old_off, h->buf[h->cur].id, pp, w, reseed);
+    }
+    {
+        Timed t(h, K_CELL_BIG);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cell_big<DIM>), dim3(256), dim3(256), 0, h->stream, old_off,
+                           h->buf[h->cur].id, pp, w, reseed);
     }
     if (int r = scan(h, h->final_cnt, w.new_off, nc + 1)) return r;
     if (int r = scan(h, h->spawn_cnt, h->spawn_off, nc + 1)) return r;
@@ -467,7 +478,7 @@ int run_move(pf2_handle_s* h, int do_correct, int do_move) {
                     Timed tq(h, K_QPREFIX);
                     hipLaunchKernelGGL(k_queue_prefix, dim3(1), dim3(QSHARDS), 0, h->stream, qu, r);
                 }
-                Timed t(h, K_SEARCH32);
+                Timed t(h, r == 0 ? K_SEARCH32 : (r == 1 ? K_SEARCH32_R1 : K_SEARCH32_R2));
                 launch_search_resume32<DIM>(h->stream, r == 0 ? qgrid : qgrid / 4 + 1, dev_mesh<DIM>(h), h->dV32,
                                             h->dgeo32, b.cell, b.xi32, b.vext, h->cap, pp, w, qu, r);
             }
@@ -483,7 +494,7 @@ int run_move(pf2_handle_s* h, int do_correct, int do_move) {
                     Timed tq(h, K_QPREFIX);
                     hipLaunchKernelGGL(k_queue_prefix, dim3(1), dim3(QSHARDS), 0, h->stream, qu, r);
                 }
-                Timed t(h, K_SEARCH);
+                Timed t(h, r == 0 ? K_SEARCH : (r == 1 ? K_SEARCH_R1 : K_SEARCH_R2));
                 hipLaunchKernelGGL(HIP_KERNEL_NAME(k_search_resume<DIM>), dim3(r == 0 ? qgrid : qgrid / 4 + 1),
                                    dim3(256), 0, h->stream, dev_mesh<DIM>(h), h->dV, dev_particles(h, h->cur), pp, w,
                                    qu, r);
@@ -630,7 +641,7 @@ size_t estimate_bytes(int dim, long long nc, long long nvt, long long nd, long l
     b += nvt * dim * 8 + nck * 4 * 2 + (nvt + 1) * 4 + nck * 4 + nck * dim * 8;       // X, cv, cd, star
     b += (nd + 1) * 4 + nck * 4 + nc * 8 + nck * dim * 8 + nck * 8 + nck * dim * 8 * 2;  // dof map, ftol, cellX/star/V/dV
     b += nd * dim * 8 * 3;                                                              // V, V_old, staging
-    b += (nc + 1) * 4 * 8 + nc * 8 + nc * 4 + nc * nv * (dim + 1) * 8 + 16;             // per-cell work, partials
+    b += (nc + 1) * 4 * 9 + nc * 8 + nc * 4 + nc * nv * (dim + 1) * 8 + 16;             // per-cell work, partials
     b += (size_t)(MOVE_ROUNDS + 2) * (QSHARDS + 1) * QSTRIDE * 4 + (size_t)(MOVE_ROUNDS + 2) * (QSHARDS + 2) * 4;
     b += 4 + (size_t)STAT_SHARDS * NSTAT * 8;
     b += nck * dim * 4 * 2 + nc * geo32_floats(dim) * 4;                                 // FP32 fields, geometry
@@ -640,7 +651,7 @@ size_t estimate_bytes(int dim, long long nc, long long nvt, long long nd, long l
     b += 2 * cap * per;                                                                 // two particle buffers
     const long long shard_cap = 2 * ((cap + QSHARDS - 1) / QSHARDS) + 64;
     const long long qlen = QSHARDS * shard_cap + cap;
-    b += cap * 4 * 4 + 2 * qlen * 4 + cap * 4 + cap * dim * 8;                          // work, queues, parking
+    b += cap * 4 * 4 + 2 * qlen * 8 + cap * 4 + cap * dim * 8;                          // work, queues, parking
     return b + (1u << 20);
 }
 
@@ -803,7 +814,7 @@ int pf2_create_in(const pf2_mesh* mesh, const pf2_params* params, long long capa
         dalloc(&h->off[1], nc + 1) || dalloc(&h->arr_cnt, nc + 1) || dalloc(&h->arr_off, nc + 1) ||
         dalloc(&h->arr_fill, nc + 1) || dalloc(&h->final_cnt, nc + 1) || dalloc(&h->spawn_cnt, nc + 1) ||
         dalloc(&h->spawn_off, nc + 1) || dalloc(&h->hitmask, nc) || dalloc(&h->nsurv, nc) ||
-        dalloc(&h->partial, (size_t)nc * h->nv * (dim + 1)) || dalloc(&h->counters, 4) ||
+        dalloc(&h->partial, (size_t)nc * h->nv * (dim + 1)) || dalloc(&h->counters, 4) || dalloc(&h->big, nc + 1) ||
         dalloc(&h->qcount, (size_t)(MOVE_ROUNDS + 2) * (QSHARDS + 1) * QSTRIDE) ||
         dalloc(&h->qprefix, (size_t)(MOVE_ROUNDS + 2) * (QSHARDS + 2)) ||
         dalloc(&h->pcounter, 1) || dalloc(&h->stats, STAT_SHARDS * NSTAT))
@@ -834,6 +845,7 @@ int pf2_create_in(const pf2_mesh* mesh, const pf2_params* params, long long capa
         hipMemsetAsync(h->final_cnt, 0, sizeof(int) * (nc + 1), h->stream) ||
         hipMemsetAsync(h->spawn_cnt, 0, sizeof(int) * (nc + 1), h->stream) ||
         hipMemsetAsync(h->counters, 0, sizeof(int) * 4, h->stream) ||
+        hipMemsetAsync(h->big, 0, sizeof(int) * (nc + 1), h->stream) ||
         hipMemsetAsync(h->qcount, 0, sizeof(int) * (MOVE_ROUNDS + 2) * (QSHARDS + 1) * QSTRIDE, h->stream) ||
         hipMemsetAsync(h->pcounter, 0, sizeof(uint32_t), h->stream) ||
         hipMemsetAsync(h->stats, 0, sizeof(unsigned long long) * STAT_SHARDS * NSTAT, h->stream))
@@ -859,7 +871,7 @@ int pf2_destroy(pf2_handle h) {
     void* ptrs[] = {h->dX, h->dcv, h->dcd, h->dstar_off, h->dstar_cell, h->dstar_dir, h->ddof_off, h->ddof_cell,
                     h->dftol, h->dcellX, h->dcell_star, h->dcellV, h->dcellDV, h->dV, h->dVold, h->dtmp, h->dV32,
                     h->dVold32, h->dgeo32, h->off[0], h->off[1], h->arr_cnt, h->arr_off, h->arr_fill, h->final_cnt,
-                    h->spawn_cnt, h->spawn_off, h->hitmask, h->nsurv, h->partial, h->counters, h->qcount, h->qprefix,
+                    h->spawn_cnt, h->spawn_off, h->hitmask, h->nsurv, h->partial, h->counters, h->big, h->qcount, h->qprefix,
                     h->pcounter, h->stats, h->scan_tmp};
     if (!h->in_block)
         for (void* p : ptrs)

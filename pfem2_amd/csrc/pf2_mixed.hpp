@@ -279,9 +279,8 @@ template <int DIM>
 __device__ __forceinline__ void park32(float* __restrict__ pxi, long long cap, const DevWork& w, const DevQueues& qu,
                                        int round_out, bool park, int i, int c, const float* xi, const float* u,
                                        int s) {
-    queue_push(qu, round_out, park, i);
+    queue_push(qu, round_out, park, make_int2(i, c));
     if (!park) return;
-    w.dest[i] = c;
     qu.park_s[i] = s;
 #pragma unroll
     for (int d = 0; d < DIM; ++d) {
@@ -352,8 +351,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DIM ==
     for (int base = blockIdx.x * blockDim.x + (threadIdx.x - lane); base < n; base += gridDim.x * blockDim.x) {
         const int j = base + lane;
         const bool valid = j < n;
-        const int i = valid ? queue_get(qu, round, s_pref, j) : -1;
-        int c = valid ? w.dest[i] : -1;
+        const int2 e = valid ? queue_get(qu, round, s_pref, j) : make_int2(-1, -1);
+        const int i = e.x;
+        int c = e.y;
         int s = valid ? qu.park_s[i] : 0;
         int st = 0;
         float xi[DIM], u[DIM];
