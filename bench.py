@@ -101,9 +101,11 @@ def moving_workload():
         X[:, 1] += A * math.sin(omega * step * dt) * w
         return X
 
-    return Workload("moving", m, V, [5, 5], 0.001, 10, 2, keep_vext=True, motion=motion,
+    # dt = 0.0005: the default workload's per-substep Courant number at half its cell size
+    return Workload("moving", m, V, [5, 5], 0.0005, 10, 2, keep_vext=True, motion=motion,
                     desc="~400k graded quads around a cylinder oscillating vertically (amplitude 0.1 D, 2 Hz), "
-                         "25 particles/cell, per step: correct+move+project, vertex update, re-sort, reseed")
+                         "25 particles/cell, dt=0.0005, per step: correct+move+project, vertex update, re-sort, "
+                         "reseed")
 
 
 def kernel_bytes(name, info, m, dim=2):
@@ -416,6 +418,15 @@ def base_line(args, W, r, dtype):
     }
 
 
+def only_main(args, W, torch):
+    """One precision path alone (for profiler runs)."""
+    from pfem2_amd import FP32_MIXED, FP64
+    r = run_path(args, W, FP64 if args.only == "fp64" else FP32_MIXED, torch)
+    print(f"# {W.name} {args.only} {r['updates_per_s']:.3e} updates/s, {r['ms_per_step']:.3f} ms/step",
+          file=sys.stderr, flush=True)
+    return {"path": args.only, "workload": W.name, **r, "roofline": roofline(r, W.m, traffic=False)}
+
+
 def workload_main(args, W, torch, with_cpu=True, with_accuracy=True):
     from pfem2_amd import FP32_MIXED, FP64
     print(f"# {W.name}: mesh {W.m.n_cells} cells {W.m.n_vertices} vertices", file=sys.stderr, flush=True)
@@ -506,6 +517,8 @@ def main():
     ap.add_argument("--serve-budget-gib", type=float, default=200.0)
     ap.add_argument("--channel-steps", type=int, default=100)
     ap.add_argument("--copy-calibration", action="store_true")
+    ap.add_argument("--only", choices=["fp64", "fp32"], default=None,
+                    help="run one precision path of the workload and print its measurements only")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (every box has one GPU); "
@@ -525,6 +538,11 @@ def main():
         return
     if args.workload == "channel":
         print(json.dumps(channel_main(args, torch)))
+        return
+    if args.only:
+        W = {"cylinder": lambda: cylinder_workload(args), "pipe3d": pipe_workload, "moving": moving_workload}[
+            args.workload]()
+        print(json.dumps(only_main(args, W, torch)))
         return
     if args.workload == "pipe3d":
         print(json.dumps(workload_main(args, pipe_workload(), torch, with_cpu=True, with_accuracy=False)))

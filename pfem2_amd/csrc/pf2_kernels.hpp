@@ -676,7 +676,8 @@ __device__ __forceinline__ int candidate(const DevWork& w, const int* old_off, i
 // of dependent loads; several cells per wavefront hide that latency).
 constexpr int GS = 16;
 constexpr int GROUPS = 256 / GS;     // groups per 256-thread block
-constexpr int GCAP = 64;             // candidates staged in LDS per group
+constexpr int GCAP = 64;             // candidates staged in LDS per group (lattices of up to 16 parts)
+constexpr int GCAP_L = 128;          // the same for larger lattices
 
 __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -712,8 +713,9 @@ __device__ __forceinline__ unsigned long long group_or(unsigned long long v) {
 // every in-range part that holds no particle is reseeded.  With reseed = 0
 // (relocation only) every particle that found a cell survives.
 // A cell's candidates (stayers of its old segment, then arrivals) are staged
-// in LDS when there are at most GCAP of them, else read from global memory.
-template <int DIM>
+// in LDS when there are at most CAP of them (GCAP or GCAP_L), else listed
+// for k_cell_big.
+template <int DIM, int CAP>
 __global__ void __launch_bounds__(256) k_cell_count(int n_cells, const int* __restrict__ old_off,
                                                     const uint32_t* __restrict__ pid, DevParams pp, DevWork w,
                                                     int reseed) {
@@ -722,9 +724,9 @@ __global__ void __launch_bounds__(256) k_cell_count(int n_cells, const int* __re
     // padded to a multiple of 4 entries with those fill values, so the scans
     // below read 16 bytes at a time, and their strides put the four groups of
     // a wavefront on different LDS banks.
-    __shared__ int s_idx[GROUPS][GCAP + 4];
-    __shared__ __align__(16) unsigned long long s_kid[GROUPS][GCAP + 2];
-    __shared__ __align__(16) uint32_t s_sid[GROUPS][GCAP + 4];
+    __shared__ int s_idx[GROUPS][CAP + 4];
+    __shared__ __align__(16) unsigned long long s_kid[GROUPS][CAP + 2];
+    __shared__ __align__(16) uint32_t s_sid[GROUPS][CAP + 4];
     const int g = threadIdx.x / GS, gl = threadIdx.x % GS;
     const unsigned long long all = pp.nparts_in >= 64 ? ~0ull : ((1ull << pp.nparts_in) - 1);
     int deleted = 0;
@@ -733,7 +735,7 @@ __global__ void __launch_bounds__(256) k_cell_count(int n_cells, const int* __re
         const bool cv = c < n_cells;
         const int n_st = cv ? old_off[c + 1] - old_off[c] : 0;
         const int k = cv ? n_st + w.arr_cnt[c] : 0;
-        const bool small = k <= GCAP;
+        const bool small = k <= CAP;
         const int kr = (k + 3) & ~3;
         unsigned long long hit = 0;
         int nvalid = 0, nsurv = 0;
@@ -743,7 +745,8 @@ __global__ void __launch_bounds__(256) k_cell_count(int n_cells, const int* __re
                 unsigned long long kid = ~0ull;
                 if (j < k) {
                     const int cidx = candidate(w, old_off, c, n_st, j);
-                    if (w.dest[cidx] == c) {
+                    // arrivals are listed for c because their destination is c
+                    if (j >= n_st || w.dest[cidx] == c) {
                         idx = cidx;
                         kid = ((unsigned long long)(uint32_t)w.pkey[cidx] << 32) | pid[cidx];
                     }
@@ -776,7 +779,7 @@ __global__ void __launch_bounds__(256) k_cell_count(int n_cells, const int* __re
                 if (surv) s_sid[g][j] = (uint32_t)me;
             }
         else if (cv && gl == 0) {
-            // more than GCAP candidates: census by k_cell_big
+            // more than CAP candidates: census by k_cell_big
             const int slot = atomicAdd(&w.big[0], 1);
             w.big[1 + slot] = c;
         }
@@ -816,7 +819,7 @@ __global__ void __launch_bounds__(256) k_cell_count(int n_cells, const int* __re
     if ((threadIdx.x & (WAVE - 1)) == 0 && deleted) stat_add(w.stats, 1, (unsigned long long)deleted);
 }
 
-// The census of k_cell_count for the cells it listed (more than GCAP
+// The census of k_cell_count for the cells it listed (more than CAP
 // candidates): one 256-thread block per cell, candidates staged in LDS.
 // A cell with more than BIGCAP candidates sets overflow flag 2 and keeps no
 // particle.
@@ -860,7 +863,7 @@ __global__ void __launch_bounds__(256) k_cell_big(const int* __restrict__ old_of
             unsigned long long kid = ~0ull;
             if (j < k) {
                 const int cidx = candidate(w, old_off, c, n_st, j);
-                if (w.dest[cidx] == c) {
+                if (j >= n_st || w.dest[cidx] == c) {
                     idx = cidx;
                     kid = ((unsigned long long)(uint32_t)w.pkey[cidx] << 32) | pid[cidx];
                 }
@@ -946,7 +949,8 @@ __device__ __forceinline__ void cell_write_one(const DevMesh<DIM>& m, const doub
     }
     for (int j = gl; j < k; j += GS) {
         const int idx = candidate(w, old_off, c, n_st, j);
-        if (w.dest[idx] != c) continue;
+        // arrivals are in this cell's list because their destination is c
+        if (j < n_st && w.dest[idx] != c) continue;
         const int pos = w.spos[idx];
         if (pos < 0) continue;
         const long long o = out0 + pos;
@@ -1336,6 +1340,35 @@ __global__ void k_keys(DevParticles A, DevParams pp, DevWork w) {
 // per-cell copies of V and V - V_old at the cell's DoFs (one load level in
 // the particle kernels); V - V_old is the same per-node difference the
 // correction forms (TurekBenchmark/pfem2particle.cpp:806-807)
+// V refreshed from a component-major device field: the node-major copy and
+// the per-cell copies of V and V - V_old (FP32 copies when V32 is given)
+template <int DIM>
+__global__ void k_refresh(DevMesh<DIM> m, const double* __restrict__ Vcm, double* __restrict__ V,
+                          const double* __restrict__ Vold, double* __restrict__ cellV, double* __restrict__ cellDV,
+                          float* __restrict__ V32, float* __restrict__ DV32) {
+    constexpr int NV = Q1<DIM>::NV;
+    const long long nck = (long long)m.n_cells * NV, nd = m.n_dofs;
+    const long long n = nck > nd ? nck : nd;
+    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x) {
+        if (t < nd)
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) V[(size_t)t * DIM + d] = Vcm[(size_t)d * nd + t];
+        if (t < nck) {
+            const int dof = m.cd[t];
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) {
+                const double a = Vcm[(size_t)d * nd + dof], b = Vold[(size_t)dof * DIM + d];
+                cellV[(size_t)t * DIM + d] = a;
+                cellDV[(size_t)t * DIM + d] = a - b;
+                if (V32) {
+                    V32[(size_t)t * DIM + d] = (float)a;
+                    DV32[(size_t)t * DIM + d] = (float)(a - b);
+                }
+            }
+        }
+    }
+}
+
 template <int DIM>
 __global__ void k_pack_fields(DevMesh<DIM> m, const double* __restrict__ V, const double* __restrict__ Vold,
                               double* __restrict__ cellV, double* __restrict__ cellDV) {

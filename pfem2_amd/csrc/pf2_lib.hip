@@ -85,10 +85,12 @@ struct PBufs {
     double* xi = nullptr;
     double* v = nullptr;
     double* vext = nullptr;
-    // FP32-mixed storage
-    float* xi32 = nullptr;
-    float* v32 = nullptr;
+    // FP32-mixed storage: one record per particle (xi, v)
+    float4* s32a = nullptr;
+    float2* s32b = nullptr;   // 3-D only
 };
+
+pf2::Rec32 rec32(const PBufs& b) { return pf2::Rec32{b.s32a, b.s32b}; }
 
 }  // namespace
 
@@ -159,6 +161,7 @@ struct pf2_handle_s {
     const double* g_field = nullptr;   // device field baked into the refreshing graphs
     size_t bytes = 0;                  // device memory held by the handle
     bool in_block = false;             // created inside caller memory: nothing of it is freed here
+    bool packed = true;                // per-cell copies of V, V - V_old are current
     bool own_stream = true;
 };
 
@@ -365,7 +368,7 @@ int alloc_particles(pf2_handle_s* h) {
             dalloc(&pb.xi, h->cap * h->dim) || dalloc(&pb.v, h->cap * h->dim))
             return PF2_E_HIP;
         if (h->params.keep_velocity_ext && dalloc(&pb.vext, h->cap * h->dim)) return PF2_E_HIP;
-        if (mixed(h) && (dalloc(&pb.xi32, h->cap * h->dim) || dalloc(&pb.v32, h->cap * h->dim))) return PF2_E_HIP;
+        if (mixed(h) && (dalloc(&pb.s32a, h->cap) || (h->dim == 3 && dalloc(&pb.s32b, h->cap)))) return PF2_E_HIP;
     }
     h->shard_cap = 2 * ((h->cap + QSHARDS - 1) / QSHARDS) + 64;
     const long long qlen = QSHARDS * h->shard_cap + h->cap;
@@ -381,7 +384,7 @@ void free_particles(pf2_handle_s* h) {
     for (int b = 0; b < 2; ++b) {
         PBufs& pb = h->buf[b];
         hipFree(pb.cell); hipFree(pb.id); hipFree(pb.x); hipFree(pb.xi); hipFree(pb.v); hipFree(pb.vext);
-        hipFree(pb.xi32); hipFree(pb.v32);
+        hipFree(pb.s32a); hipFree(pb.s32b);
         pb = PBufs();
     }
     hipFree(h->dest); hipFree(h->pkey); hipFree(h->spos); hipFree(h->arr_list);
@@ -425,12 +428,16 @@ int regroup(pf2_handle_s* h, int reseed) {
     }
     {
         Timed t(h, K_CELL_COUNT);
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cell_count<DIM>), dim3(cell_grid(nc)), dim3(256), 0, h->stream, nc,
-                           old_off, h->buf[h->cur].id, pp, w, reseed);
+        if (pp.nparts_in <= 16)
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cell_count<DIM, GCAP>), dim3(cell_grid(nc)), dim3(256), 0,
+                               h->stream, nc, old_off, h->buf[h->cur].id, pp, w, reseed);
+        else
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cell_count<DIM, GCAP_L>), dim3(cell_grid(nc)), dim3(256), 0,
+                               h->stream, nc, old_off, h->buf[h->cur].id, pp, w, reseed);
     }
     {
         Timed t(h, K_CELL_BIG);
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cell_big<DIM>), dim3(256), dim3(256), 0, h->stream, old_off,
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cell_big<DIM>), dim3(1024), dim3(256), 0, h->stream, old_off,
                            h->buf[h->cur].id, pp, w, reseed);
     }
     if (int r = scan(h, h->final_cnt, w.new_off, nc + 1)) return r;
@@ -438,9 +445,9 @@ int regroup(pf2_handle_s* h, int reseed) {
     if (mixed(h)) {
         Timed t(h, K_CELL_WRITE32);
         launch_cell_write32<DIM>(h->stream, cell_grid(nc), dev_mesh<DIM>(h), h->dV, old_off, h->buf[h->cur].id,
-                                 h->buf[h->cur].xi32, h->buf[h->cur].v32, h->buf[1 - h->cur].cell,
-                                 h->buf[1 - h->cur].id, h->buf[1 - h->cur].xi32, h->buf[1 - h->cur].v32,
-                                 h->buf[h->cur].vext, h->buf[1 - h->cur].vext, h->cap, pp, w);
+                                 rec32(h->buf[h->cur]), h->buf[1 - h->cur].cell, h->buf[1 - h->cur].id,
+                                 rec32(h->buf[1 - h->cur]), h->buf[h->cur].vext, h->buf[1 - h->cur].vext, h->cap, pp,
+                                 w);
     } else {
         Timed t(h, K_CELL_WRITE);
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cell_write<DIM>), dim3(cell_grid(nc)), dim3(256), 0, h->stream,
@@ -469,8 +476,8 @@ int run_move(pf2_handle_s* h, int do_correct, int do_move) {
         PBufs& b = h->buf[h->cur];
         {
             Timed t(h, K_MOVE32);
-            launch_move32<DIM>(h->stream, grid, dev_mesh<DIM>(h), h->dV32, h->dVold32, h->dgeo32, b.cell, b.xi32,
-                               b.v32, b.vext, h->cap, pp, w, qu, do_correct, do_move);
+            launch_move32<DIM>(h->stream, grid, dev_mesh<DIM>(h), h->dV32, h->dVold32, h->dgeo32, b.cell, rec32(b),
+                               b.vext, h->cap, pp, w, qu, do_correct, do_move);
         }
         if (do_move)
             for (int r = 0; r <= MOVE_ROUNDS; ++r) {
@@ -480,7 +487,7 @@ int run_move(pf2_handle_s* h, int do_correct, int do_move) {
                 }
                 Timed t(h, r == 0 ? K_SEARCH32 : (r == 1 ? K_SEARCH32_R1 : K_SEARCH32_R2));
                 launch_search_resume32<DIM>(h->stream, r == 0 ? qgrid : qgrid / 4 + 1, dev_mesh<DIM>(h), h->dV32,
-                                            h->dgeo32, b.cell, b.xi32, b.vext, h->cap, pp, w, qu, r);
+                                            h->dgeo32, b.cell, rec32(b), b.vext, h->cap, pp, w, qu, r);
             }
     } else {
         {
@@ -511,8 +518,7 @@ int run_project(pf2_handle_s* h) {
     DevWork w = dev_work(h);
     if (mixed(h)) {
         Timed t(h, K_PARTIALS32);
-        launch_partials32<DIM>(h->stream, cell_grid(nc), nc, h->off[h->cur_off], h->buf[h->cur].xi32, h->buf[h->cur].v32, h->cap,
-                               h->partial);
+        launch_partials32<DIM>(h->stream, cell_grid(nc), nc, h->off[h->cur_off], rec32(h->buf[h->cur]), h->partial);
     } else {
         Timed t(h, K_PARTIALS);
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cell_partials<DIM>), dim3(cell_grid(nc)), dim3(256), 0, h->stream, nc,
@@ -539,8 +545,7 @@ int run_project(pf2_handle_s* h) {
                            h->stream, wl, h->dV);
         if (mixed(h)) {
             h->fp64_valid = false;
-            launch_wall_particles32<DIM>(h->stream, dev_mesh<DIM>(h), wl, h->off[h->cur_off], h->buf[h->cur].xi32,
-                                         h->buf[h->cur].v32, h->cap);
+            launch_wall_particles32<DIM>(h->stream, dev_mesh<DIM>(h), wl, h->off[h->cur_off], rec32(h->buf[h->cur]));
         } else {
             hipLaunchKernelGGL(HIP_KERNEL_NAME(k_wall_particles<DIM>), dim3((h->n_wall_cells + 3) / 4), dim3(256), 0,
                                h->stream, dev_mesh<DIM>(h), wl, h->off[h->cur_off], dev_particles(h, h->cur));
@@ -548,10 +553,7 @@ int run_project(pf2_handle_s* h) {
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_wall_set<DIM>), dim3(grid_for(h->n_walls, 256)), dim3(256), 0, h->stream,
                            wl, h->dV, h->dVold);
     }
-    {
-        Timed t(h, K_MISC);
-        if (int r = pack_fields(h)) return r;
-    }
+    h->packed = false;   // V and V_old changed: refreshed before their next use
     HIPC(hipGetLastError());
     return 0;
 }
@@ -588,6 +590,7 @@ int upload_field(pf2_handle_s* h, const double* src, double* dst, hipMemcpyKind 
 // per-cell copies of V and V - V_old (FP64, and FP32 for the mixed path);
 // called whenever V or V_old change
 int pack_fields(pf2_handle_s* h) {
+    h->packed = true;
     const long long n = (long long)h->n_cells * h->nv;
     const int g = grid_for(n, 256, 4096);
     if (h->dim == 2) {
@@ -607,12 +610,35 @@ int pack_fields(pf2_handle_s* h) {
     return 0;
 }
 
+int ensure_packed(pf2_handle_s* h) {
+    if (h->packed) return 0;
+    Timed t(h, K_MISC);
+    return pack_fields(h);
+}
+
+// V from a device field (component-major) into the node-major copy and the
+// per-cell copies, in one pass
+int refresh_field(pf2_handle_s* h, const double* field) {
+    const long long n = std::max((long long)h->n_cells * h->nv, (long long)h->n_dofs);
+    const int g = grid_for(n, 256, 4096);
+    Timed t(h, K_MISC);
+    if (h->dim == 2)
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_refresh<2>), dim3(g), dim3(256), 0, h->stream, dev_mesh<2>(h), field,
+                           h->dV, h->dVold, h->dcellV, h->dcellDV, mixed(h) ? h->dV32 : nullptr, h->dVold32);
+    else
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_refresh<3>), dim3(g), dim3(256), 0, h->stream, dev_mesh<3>(h), field,
+                           h->dV, h->dVold, h->dcellV, h->dcellDV, mixed(h) ? h->dV32 : nullptr, h->dVold32);
+    HIPC(hipGetLastError());
+    h->packed = true;
+    return 0;
+}
+
 // FP64 state <-> FP32-mixed state (downloads and inserts go through FP64)
 int to_mixed(pf2_handle_s* h) {
     if (!mixed(h)) return 0;
     h->fp64_valid = true;
-    launch_state_to32(h->stream, h->dim, h->buf[h->cur].xi, h->buf[h->cur].v, h->buf[h->cur].xi32,
-                      h->buf[h->cur].v32, h->cap, h->counters, h->n_est);
+    launch_state_to32(h->stream, h->dim, h->buf[h->cur].xi, h->buf[h->cur].v, rec32(h->buf[h->cur]), h->cap,
+                      h->counters, h->n_est);
     HIPC(hipGetLastError());
     return 0;
 }
@@ -620,12 +646,12 @@ int to_mixed(pf2_handle_s* h) {
 int from_mixed(pf2_handle_s* h) {
     if (!mixed(h) || h->fp64_valid) return 0;
     if (h->dim == 2)
-        launch_state_from32<2>(h->stream, dev_mesh<2>(h), h->buf[h->cur].cell, h->buf[h->cur].xi32,
-                               h->buf[h->cur].v32, h->buf[h->cur].x, h->buf[h->cur].xi, h->buf[h->cur].v, h->cap,
+        launch_state_from32<2>(h->stream, dev_mesh<2>(h), h->buf[h->cur].cell, rec32(h->buf[h->cur]),
+                               h->buf[h->cur].x, h->buf[h->cur].xi, h->buf[h->cur].v, h->cap,
                                h->counters, h->n_est);
     else
-        launch_state_from32<3>(h->stream, dev_mesh<3>(h), h->buf[h->cur].cell, h->buf[h->cur].xi32,
-                               h->buf[h->cur].v32, h->buf[h->cur].x, h->buf[h->cur].xi, h->buf[h->cur].v, h->cap,
+        launch_state_from32<3>(h->stream, dev_mesh<3>(h), h->buf[h->cur].cell, rec32(h->buf[h->cur]),
+                               h->buf[h->cur].x, h->buf[h->cur].xi, h->buf[h->cur].v, h->cap,
                                h->counters, h->n_est);
     HIPC(hipGetLastError());
     h->fp64_valid = true;
@@ -670,8 +696,12 @@ void invalidate_graphs(pf2_handle_s* h) {
 // one step: optional refresh of V from a device field (component-major),
 // then correct, move (with reseed) and project -- no host synchronisation
 int step_body(pf2_handle_s* h, const double* field) {
+    // the copies are refreshed unconditionally, so that a captured graph
+    // holds the refresh whatever the state at capture
     if (field) {
-        if (int r = upload_field(h, field, h->dV, hipMemcpyDeviceToDevice)) return r;
+        if (int r = refresh_field(h, field)) return r;
+    } else {
+        Timed t(h, K_MISC);
         if (int r = pack_fields(h)) return r;
     }
     if (int r = (h->dim == 2 ? run_move<2>(h, 1, 1) : run_move<3>(h, 1, 1))) return r;
@@ -933,6 +963,7 @@ int pf2_download_field(pf2_handle h, double* V) {
 int pf2_seed_particles(pf2_handle h) {
     if (!h) return fail(PF2_E_ARG, "handle is NULL");
     invalidate_graphs(h);
+    if (int r = ensure_packed(h)) return r;
     long long n = 0;
     if (int r = get_count(h, &n)) return r;
     if (n != 0) return fail(PF2_E_STATE, "seed_particles on a handle that already holds particles");
@@ -990,6 +1021,7 @@ int pf2_download_particles(pf2_handle h, int* cell, uint32_t* id, double* x, dou
 int pf2_insert_particles(pf2_handle h, int n, const int* cell, const double* xi, const double* v) {
     if (!h) return fail(PF2_E_ARG, "handle is NULL");
     invalidate_graphs(h);
+    if (int r = ensure_packed(h)) return r;
     if (n < 0 || (n > 0 && (!cell || !xi))) return fail(PF2_E_ARG, "invalid particle arrays");
     const int dim = h->dim;
     for (int i = 0; i < n; ++i) {
@@ -1079,11 +1111,13 @@ int pf2_insert_particles(pf2_handle h, int n, const int* cell, const double* xi,
 
 int pf2_correct_particles_velocities(pf2_handle h) {
     if (!h) return fail(PF2_E_ARG, "handle is NULL");
+    if (int r = ensure_packed(h)) return r;
     return h->dim == 2 ? run_move<2>(h, 1, 0) : run_move<3>(h, 1, 0);
 }
 
 int pf2_move_particles(pf2_handle h) {
     if (!h) return fail(PF2_E_ARG, "handle is NULL");
+    if (int r = ensure_packed(h)) return r;
     return h->dim == 2 ? run_move<2>(h, 0, 1) : run_move<3>(h, 0, 1);
 }
 
@@ -1200,7 +1234,8 @@ int pf2_sort_particles_into_subdomains_and_cells(pf2_handle h) {
     HIPC(hipGetLastError());
     const int saved = h->params.precision;
     h->params.precision = PF2_FP64;   // regroup the FP64 copy, then convert back
-    int r = h->dim == 2 ? regroup<2>(h, 0) : regroup<3>(h, 0);
+    int r = ensure_packed(h);
+    if (!r) r = h->dim == 2 ? regroup<2>(h, 0) : regroup<3>(h, 0);
     h->params.precision = saved;
     if (r) return r;
     long long n;
@@ -1225,7 +1260,8 @@ int pf2_check_empty_cells(pf2_handle h) {
     HIPC(hipGetLastError());
     const int saved = h->params.precision;
     h->params.precision = PF2_FP64;
-    int r = h->dim == 2 ? regroup<2>(h, 1) : regroup<3>(h, 1);
+    int r = ensure_packed(h);
+    if (!r) r = h->dim == 2 ? regroup<2>(h, 1) : regroup<3>(h, 1);
     h->params.precision = saved;
     if (r) return r;
     long long n;

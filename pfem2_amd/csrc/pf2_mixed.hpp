@@ -2,7 +2,7 @@
 //
 // A particle is stored as (cell, id, xi in FP32, v in FP32): 8 + 4*DIM*2
 // bytes instead of 8 + 8*DIM*3 in the FP64 path (there is no absolute
-// position; it is F_K(xi) on demand).  Within a substep the displacement
+// position; it is F_K(xi) on demand); xi and v sit together in one record.  Within a substep the displacement
 // h*u is applied in reference coordinates of the current cell by a short
 // FP32 Newton solve of F_K(xi + delta) - F_K(xi) = h*u, written with the
 // exact multilinear difference so no absolute coordinates enter the FP32
@@ -17,6 +17,86 @@
 namespace pf2 {
 
 inline int geo32_floats(int dim) { return dim == 2 ? 6 : 21; }
+
+// FP32 particle state, one record per particle, so that a scattered access
+// touches one cache line (2-D) or two (3-D):
+//   2-D  a = {xi0, xi1, v0, v1}
+//   3-D  a = {xi0, xi1, xi2, v0}, b = {v1, v2}
+struct Rec32 {
+    float4* a;
+    float2* b;   // 3-D only
+};
+
+template <int DIM>
+__device__ __forceinline__ void rec_load(const Rec32& r, long long i, float* xi, float* v) {
+    const float4 q = r.a[i];
+    if constexpr (DIM == 2) {
+        xi[0] = q.x;
+        xi[1] = q.y;
+        v[0] = q.z;
+        v[1] = q.w;
+    } else {
+        const float2 t = r.b[i];
+        xi[0] = q.x;
+        xi[1] = q.y;
+        xi[2] = q.z;
+        v[0] = q.w;
+        v[1] = t.x;
+        v[2] = t.y;
+    }
+}
+
+template <int DIM>
+__device__ __forceinline__ void rec_load_xi(const Rec32& r, long long i, float* xi) {
+    if constexpr (DIM == 2) {
+        const float2 q = reinterpret_cast<const float2*>(r.a + i)[0];
+        xi[0] = q.x;
+        xi[1] = q.y;
+    } else {
+        const float4 q = r.a[i];
+        xi[0] = q.x;
+        xi[1] = q.y;
+        xi[2] = q.z;
+    }
+}
+
+template <int DIM>
+__device__ __forceinline__ void rec_store(const Rec32& r, long long i, const float* xi, const float* v) {
+    if constexpr (DIM == 2) {
+        r.a[i] = make_float4(xi[0], xi[1], v[0], v[1]);
+    } else {
+        r.a[i] = make_float4(xi[0], xi[1], xi[2], v[0]);
+        r.b[i] = make_float2(v[1], v[2]);
+    }
+}
+
+template <int DIM>
+__device__ __forceinline__ void rec_store_xi(const Rec32& r, long long i, const float* xi) {
+    if constexpr (DIM == 2) {
+        reinterpret_cast<float2*>(r.a + i)[0] = make_float2(xi[0], xi[1]);
+    } else {
+        float* p = reinterpret_cast<float*>(r.a + i);
+        p[0] = xi[0];
+        p[1] = xi[1];
+        p[2] = xi[2];
+    }
+}
+
+template <int DIM>
+__device__ __forceinline__ void rec_store_v(const Rec32& r, long long i, const float* v) {
+    if constexpr (DIM == 2) {
+        reinterpret_cast<float2*>(r.a + i)[1] = make_float2(v[0], v[1]);
+    } else {
+        reinterpret_cast<float*>(r.a + i)[3] = v[0];
+        r.b[i] = make_float2(v[1], v[2]);
+    }
+}
+
+template <int DIM>
+__device__ __forceinline__ void rec_copy(const Rec32& src, long long i, const Rec32& dst, long long o) {
+    dst.a[o] = src.a[i];
+    if constexpr (DIM == 3) dst.b[o] = src.b[i];
+}
 
 // multilinear coefficients of F_K relative to vertex 0, in FP32
 //   2-D: F = x0 + a xi0 + b xi1 + c xi0 xi1
@@ -265,36 +345,33 @@ __device__ __forceinline__ int advance32(const DevParams& pp, float h32, int i, 
 }
 
 template <int DIM>
-__device__ __forceinline__ void finish32(const DevParams& pp, const int* __restrict__ pcell, float* __restrict__ pxi,
-                                         long long cap, const DevWork& w, bool valid, int i, int c, const float* xi) {
+__device__ __forceinline__ void finish32(const DevParams& pp, const int* __restrict__ pcell, const Rec32& R,
+                                         const DevWork& w, bool valid, int i, int c, const float* xi) {
     if (!valid) return;
     if (c >= 0 && c != pcell[i]) atomicAdd(&w.arr_cnt[c], 1);
-#pragma unroll
-    for (int d = 0; d < DIM; ++d) pxi[(size_t)d * cap + i] = xi[d];
+    rec_store_xi<DIM>(R, i, xi);
     w.dest[i] = c;
     if (c >= 0) w.pkey[i] = part_key32<DIM>(pp, xi);
 }
 
 template <int DIM>
-__device__ __forceinline__ void park32(float* __restrict__ pxi, long long cap, const DevWork& w, const DevQueues& qu,
+__device__ __forceinline__ void park32(const Rec32& R, long long cap, const DevWork& w, const DevQueues& qu,
                                        int round_out, bool park, int i, int c, const float* xi, const float* u,
                                        int s) {
     queue_push(qu, round_out, park, make_int2(i, c));
     if (!park) return;
     qu.park_s[i] = s;
+    rec_store_xi<DIM>(R, i, xi);
 #pragma unroll
-    for (int d = 0; d < DIM; ++d) {
-        pxi[(size_t)d * cap + i] = xi[d];
-        qu.park_vext[(size_t)d * cap + i] = (double)u[d];
-    }
+    for (int d = 0; d < DIM; ++d) qu.park_vext[(size_t)d * cap + i] = (double)u[d];
 }
 
 template <int DIM>
 __global__ void __launch_bounds__(256) k_move32(DevMesh<DIM> m, const float* __restrict__ V32,
                                                 const float* __restrict__ dV32, const float* __restrict__ geo,
-                                                const int* __restrict__ pcell, float* __restrict__ pxi,
-                                                float* __restrict__ pv, double* __restrict__ pvext, long long cap,
-                                                DevParams pp, DevWork w, DevQueues qu, int do_correct, int do_move) {
+                                                const int* __restrict__ pcell, Rec32 R, double* __restrict__ pvext,
+                                                long long cap, DevParams pp, DevWork w, DevQueues qu, int do_correct,
+                                                int do_move) {
     constexpr int NV = Q1<DIM>::NV;
     constexpr int NG = DIM == 2 ? 6 : 21;
     const int n = w.counters[0];
@@ -304,20 +381,18 @@ __global__ void __launch_bounds__(256) k_move32(DevMesh<DIM> m, const float* __r
         const int i = base + lane;
         const bool valid = i < n;
         const int c = valid ? pcell[i] : 0;
-        float xi[DIM];
+        float xi[DIM], v[DIM];
 #pragma unroll
-        for (int d = 0; d < DIM; ++d) xi[d] = valid ? pxi[(size_t)d * cap + i] : 0.5f;
+        for (int d = 0; d < DIM; ++d) xi[d] = v[d] = 0.5f;
+        if (valid) rec_load<DIM>(R, i, xi, v);
         if (do_correct && valid) {
-            float v[DIM], N[NV];
-#pragma unroll
-            for (int d = 0; d < DIM; ++d) v[d] = pv[(size_t)d * cap + i];
+            float N[NV];
             shape32<DIM>(xi, N);
 #pragma unroll
             for (int k = 0; k < NV; ++k)
 #pragma unroll
                 for (int d = 0; d < DIM; ++d) v[d] = __fmaf_rn(N[k], dV32[((size_t)c * NV + k) * DIM + d], v[d]);
-#pragma unroll
-            for (int d = 0; d < DIM; ++d) pv[(size_t)d * cap + i] = v[d];
+            rec_store_v<DIM>(R, i, v);
         }
         if (!do_move) continue;
         int st = 0, s = 0;
@@ -327,8 +402,8 @@ __global__ void __launch_bounds__(256) k_move32(DevMesh<DIM> m, const float* __r
             load_cell32<DIM>(geo, V32, c, g, Vc);
             st = advance32<DIM>(pp, h32, i, pvext, cap, xi, u, s, g, Vc);
         }
-        park32<DIM>(pxi, cap, w, qu, 0, valid && st == 1, i, c, xi, u, s);
-        finish32<DIM>(pp, pcell, pxi, cap, w, valid && st == 0, i, c, xi);
+        park32<DIM>(R, cap, w, qu, 0, valid && st == 1, i, c, xi, u, s);
+        finish32<DIM>(pp, pcell, R, w, valid && st == 0, i, c, xi);
     }
 }
 
@@ -337,7 +412,7 @@ __global__ void __launch_bounds__(256) k_move32(DevMesh<DIM> m, const float* __r
 template <int DIM>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DIM == 2 ? 4 : 2))) k_search_resume32(DevMesh<DIM> m, const float* __restrict__ V32,
                                                          const float* __restrict__ geo, const int* __restrict__ pcell,
-                                                         float* __restrict__ pxi, double* __restrict__ pvext,
+                                                         Rec32 R, double* __restrict__ pvext,
                                                          long long cap, DevParams pp, DevWork w, DevQueues qu,
                                                          int round) {
     constexpr int NV = Q1<DIM>::NV;
@@ -358,11 +433,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DIM ==
         int st = 0;
         float xi[DIM], u[DIM];
         if (valid) {
+            rec_load_xi<DIM>(R, i, xi);
 #pragma unroll
-            for (int d = 0; d < DIM; ++d) {
-                xi[d] = pxi[(size_t)d * cap + i];
-                u[d] = (float)qu.park_vext[(size_t)d * cap + i];
-            }
+            for (int d = 0; d < DIM; ++d) u[d] = (float)qu.park_vext[(size_t)d * cap + i];
             // 2-D: the found cell's vertices and FP32 data are loaded with the
             // candidate and reused; 3-D reloads them (register budget)
             double P[NV][DIM];
@@ -424,8 +497,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DIM ==
                 }
             }
         }
-        if (!last) park32<DIM>(pxi, cap, w, qu, round + 1, valid && st == 1, i, c, xi, u, s);
-        finish32<DIM>(pp, pcell, pxi, cap, w, valid && st == 0, i, c, xi);
+        if (!last) park32<DIM>(R, cap, w, qu, round + 1, valid && st == 1, i, c, xi, u, s);
+        finish32<DIM>(pp, pcell, R, w, valid && st == 0, i, c, xi);
     }
     wave_stat(w.stats, 0, lost);
     wave_stat(w.stats, 3, searches);
@@ -433,29 +506,27 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DIM ==
 
 template <int DIM>
 void launch_move32(hipStream_t s, int grid, DevMesh<DIM> m, const float* V32, const float* dV32, const float* geo,
-                   int* pcell, float* pxi, float* pv, double* pvext, long long cap, DevParams pp, DevWork w,
+                   int* pcell, Rec32 R, double* pvext, long long cap, DevParams pp, DevWork w,
                    DevQueues qu, int do_correct, int do_move) {
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_move32<DIM>), dim3(grid), dim3(256), 0, s, m, V32, dV32, geo, pcell, pxi, pv,
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_move32<DIM>), dim3(grid), dim3(256), 0, s, m, V32, dV32, geo, pcell, R,
                        pvext, cap, pp, w, qu, do_correct, do_move);
 }
 
 template <int DIM>
 void launch_search_resume32(hipStream_t s, int grid, DevMesh<DIM> m, const float* V32, const float* geo,
-                            const int* pcell, float* pxi, double* pvext, long long cap, DevParams pp, DevWork w,
+                            const int* pcell, Rec32 R, double* pvext, long long cap, DevParams pp, DevWork w,
                             DevQueues qu, int round) {
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_search_resume32<DIM>), dim3(grid), dim3(256), 0, s, m, V32, geo, pcell, pxi,
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_search_resume32<DIM>), dim3(grid), dim3(256), 0, s, m, V32, geo, pcell, R,
                        pvext, cap, pp, w, qu, round);
 }
 
 template <int DIM>
 __device__ __forceinline__ void cell_write32_one(const DevMesh<DIM>& m, const double* __restrict__ V,
                                                  const int* __restrict__ old_off, const uint32_t* __restrict__ aid,
-                                                 const float* __restrict__ axi, const float* __restrict__ av,
-                                                 int* __restrict__ bcell, uint32_t* __restrict__ bid,
-                                                 float* __restrict__ bxi, float* __restrict__ bv,
-                                                 const double* __restrict__ avext, double* __restrict__ bvext,
-                                                 long long cap, const DevParams& pp, const DevWork& w, int c,
-                                                 int gl) {
+                                                 const Rec32& A, int* __restrict__ bcell, uint32_t* __restrict__ bid,
+                                                 const Rec32& B, const double* __restrict__ avext,
+                                                 double* __restrict__ bvext, long long cap, const DevParams& pp,
+                                                 const DevWork& w, int c, int gl) {
     constexpr int NV = Q1<DIM>::NV;
     const int n_st = old_off[c + 1] - old_off[c];
     const int k = n_st + w.arr_cnt[c];
@@ -468,17 +539,14 @@ __device__ __forceinline__ void cell_write32_one(const DevMesh<DIM>& m, const do
     }
     for (int j = gl; j < k; j += GS) {
         const int idx = candidate(w, old_off, c, n_st, j);
-        if (w.dest[idx] != c) continue;
+        // arrivals are in this cell's list because their destination is c
+        if (j < n_st && w.dest[idx] != c) continue;
         const int pos = w.spos[idx];
         if (pos < 0) continue;
         const long long o = out0 + pos;
         bcell[o] = c;
         bid[o] = aid[idx];
-#pragma unroll
-        for (int d = 0; d < DIM; ++d) {
-            bxi[(size_t)d * cap + o] = axi[(size_t)d * cap + idx];
-            bv[(size_t)d * cap + o] = av[(size_t)d * cap + idx];
-        }
+        rec_copy<DIM>(A, idx, B, o);
         if (pp.keep_vext)
 #pragma unroll
             for (int d = 0; d < DIM; ++d) bvext[(size_t)d * cap + o] = avext[(size_t)d * cap + idx];
@@ -498,12 +566,14 @@ __device__ __forceinline__ void cell_write32_one(const DevMesh<DIM>& m, const do
             const long long q = out0 + nsurv + r;
             bcell[q] = c;
             bid[q] = id0 + (uint32_t)r + 1u;
+            float xf[DIM], vf[DIM];
 #pragma unroll
             for (int d = 0; d < DIM; ++d) {
-                bxi[(size_t)d * cap + q] = (float)xi[d];
-                bv[(size_t)d * cap + q] = (float)v[d];
+                xf[d] = (float)xi[d];
+                vf[d] = (float)v[d];
                 if (pp.keep_vext) bvext[(size_t)d * cap + q] = 0.0;
             }
+            rec_store<DIM>(B, q, xf, vf);
         }
     }
     if (gl == 0) {
@@ -515,30 +585,28 @@ __device__ __forceinline__ void cell_write32_one(const DevMesh<DIM>& m, const do
 template <int DIM>
 __global__ void __launch_bounds__(256) k_cell_write32(DevMesh<DIM> m, const double* __restrict__ V,
                                                      const int* __restrict__ old_off, const uint32_t* __restrict__ aid,
-                                                     const float* __restrict__ axi, const float* __restrict__ av,
-                                                     int* __restrict__ bcell, uint32_t* __restrict__ bid,
-                                                     float* __restrict__ bxi, float* __restrict__ bv,
-                                                     const double* __restrict__ avext, double* __restrict__ bvext,
-                                                     long long cap, DevParams pp, DevWork w) {
+                                                     Rec32 A, int* __restrict__ bcell, uint32_t* __restrict__ bid,
+                                                     Rec32 B, const double* __restrict__ avext,
+                                                     double* __restrict__ bvext, long long cap, DevParams pp,
+                                                     DevWork w) {
     const int g = threadIdx.x / GS, gl = threadIdx.x % GS;
     for (int c = blockIdx.x * GROUPS + g; c < m.n_cells; c += gridDim.x * GROUPS)
-        cell_write32_one<DIM>(m, V, old_off, aid, axi, av, bcell, bid, bxi, bv, avext, bvext, cap, pp, w, c, gl);
+        cell_write32_one<DIM>(m, V, old_off, aid, A, bcell, bid, B, avext, bvext, cap, pp, w, c, gl);
 }
 
 template <int DIM>
 void launch_cell_write32(hipStream_t s, int grid, DevMesh<DIM> m, const double* V, const int* old_off, const uint32_t* aid,
-                         const float* axi, const float* av, int* bcell, uint32_t* bid, float* bxi, float* bv,
-                         const double* avext, double* bvext, long long cap, DevParams pp, DevWork w) {
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cell_write32<DIM>), dim3(grid), dim3(256), 0, s, m, V, old_off, aid, axi,
-                       av, bcell, bid, bxi, bv, avext, bvext, cap, pp, w);
+                         Rec32 A, int* bcell, uint32_t* bid, Rec32 B, const double* avext, double* bvext, long long cap,
+                         DevParams pp, DevWork w) {
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cell_write32<DIM>), dim3(grid), dim3(256), 0, s, m, V, old_off, aid, A,
+                       bcell, bid, B, avext, bvext, cap, pp, w);
 }
 
 // per-cell projection sums in FP32 with Kahan compensation, ascending id order
 // (groups of lanes per cell as in k_cell_partials)
 template <int DIM>
-__global__ void __launch_bounds__(256) k_cell_partials32(int n_cells, const int* __restrict__ off,
-                                                         const float* __restrict__ pxi, const float* __restrict__ pv,
-                                                         long long cap, double* __restrict__ partial) {
+__global__ void __launch_bounds__(256) k_cell_partials32(int n_cells, const int* __restrict__ off, Rec32 R,
+                                                         double* __restrict__ partial) {
     constexpr int NV = Q1<DIM>::NV;
     constexpr int NACC = PartialsShape<DIM>::NACC;
     constexpr int PGS = PartialsShape<DIM>::PGS;
@@ -560,10 +628,12 @@ __global__ void __launch_bounds__(256) k_cell_partials32(int n_cells, const int*
             const int j0 = b + t * PGS;
             const int j = j0 + gl;
             if (j < e) {
+                float xi[DIM], v[DIM];
+                rec_load<DIM>(R, j, xi, v);
 #pragma unroll
                 for (int d = 0; d < DIM; ++d) {
-                    s_xi[g][gl][d] = pxi[(size_t)d * cap + j];
-                    s_v[g][gl][d] = pv[(size_t)d * cap + j];
+                    s_xi[g][gl][d] = xi[d];
+                    s_v[g][gl][d] = v[d];
                 }
             }
             wave_sync();
@@ -598,15 +668,12 @@ __global__ void __launch_bounds__(256) k_cell_partials32(int n_cells, const int*
 }
 
 template <int DIM>
-void launch_partials32(hipStream_t s, int grid, int n_cells, const int* off, const float* pxi, const float* pv, long long cap,
-                       double* partial) {
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cell_partials32<DIM>), dim3(grid), dim3(256), 0, s, n_cells, off,
-                       pxi, pv, cap, partial);
+void launch_partials32(hipStream_t s, int grid, int n_cells, const int* off, Rec32 R, double* partial) {
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cell_partials32<DIM>), dim3(grid), dim3(256), 0, s, n_cells, off, R, partial);
 }
 
 template <int DIM>
-__global__ void k_wall_particles32(DevMesh<DIM> m, DevWalls wl, const int* __restrict__ off,
-                                   const float* __restrict__ pxi, float* __restrict__ pv, long long cap) {
+__global__ void k_wall_particles32(DevMesh<DIM> m, DevWalls wl, const int* __restrict__ off, Rec32 R) {
     constexpr int NV = Q1<DIM>::NV;
     const int lane = threadIdx.x & (WAVE - 1);
     const int t = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
@@ -632,11 +699,7 @@ __global__ void k_wall_particles32(DevMesh<DIM> m, DevWalls wl, const int* __res
     if (nw == 0) return;
     for (int j = off[c] + lane; j < off[c + 1]; j += WAVE) {
         float xi[DIM], v[DIM], N[NV];
-#pragma unroll
-        for (int d = 0; d < DIM; ++d) {
-            xi[d] = pxi[(size_t)d * cap + j];
-            v[d] = pv[(size_t)d * cap + j];
-        }
+        rec_load<DIM>(R, j, xi, v);
         shape32<DIM>(xi, N);
         for (int q = 0; q < nw; ++q) {
             const int wi = widx[q];
@@ -648,16 +711,14 @@ __global__ void k_wall_particles32(DevMesh<DIM> m, DevWalls wl, const int* __res
             for (int d = 0; d < DIM; ++d)
                 if ((wl.mask[wi] >> d) & 1) v[d] = __fmaf_rn(nk, (float)(wl.bc[wi * 3 + d] - wl.val[wi * 3 + d]), v[d]);
         }
-#pragma unroll
-        for (int d = 0; d < DIM; ++d) pv[(size_t)d * cap + j] = v[d];
+        rec_store_v<DIM>(R, j, v);
     }
 }
 
 template <int DIM>
-void launch_wall_particles32(hipStream_t s, DevMesh<DIM> m, DevWalls wl, const int* off, const float* pxi, float* pv,
-                             long long cap) {
+void launch_wall_particles32(hipStream_t s, DevMesh<DIM> m, DevWalls wl, const int* off, Rec32 R) {
     hipLaunchKernelGGL(HIP_KERNEL_NAME(k_wall_particles32<DIM>), dim3((wl.n_cells + 3) / 4), dim3(256), 0, s, m, wl,
-                       off, pxi, pv, cap);
+                       off, R);
 }
 
 // per-cell FP32 copies of V and V - V_old at the cell's DoFs
@@ -677,39 +738,46 @@ __global__ void k_pack_fields32(DevMesh<DIM> m, const double* __restrict__ V, co
     }
 }
 
-__global__ void k_state_to32(int dim, const double* __restrict__ xi, const double* __restrict__ v,
-                             float* __restrict__ xi32, float* __restrict__ v32, long long cap,
+template <int DIM>
+__global__ void k_state_to32(const double* __restrict__ xi, const double* __restrict__ v, Rec32 R, long long cap,
                              const int* __restrict__ counters) {
     const long long n = counters[0];
-    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
-        for (int d = 0; d < dim; ++d) {
-            xi32[(size_t)d * cap + i] = (float)xi[(size_t)d * cap + i];
-            v32[(size_t)d * cap + i] = (float)v[(size_t)d * cap + i];
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        float xf[DIM], vf[DIM];
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) {
+            xf[d] = (float)xi[(size_t)d * cap + i];
+            vf[d] = (float)v[(size_t)d * cap + i];
         }
+        rec_store<DIM>(R, i, xf, vf);
+    }
 }
 
-inline void launch_state_to32(hipStream_t s, int dim, const double* xi, const double* v, float* xi32, float* v32,
-                              long long cap, const int* counters, long long n_est) {
+inline void launch_state_to32(hipStream_t s, int dim, const double* xi, const double* v, Rec32 R, long long cap,
+                              const int* counters, long long n_est) {
     long long g = (n_est + 255) / 256;
     if (g < 1) g = 1;
     if (g > 4096) g = 4096;
-    hipLaunchKernelGGL(k_state_to32, dim3((int)g), dim3(256), 0, s, dim, xi, v, xi32, v32, cap, counters);
+    if (dim == 2) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_state_to32<2>), dim3((int)g), dim3(256), 0, s, xi, v, R, cap, counters);
+    else hipLaunchKernelGGL(HIP_KERNEL_NAME(k_state_to32<3>), dim3((int)g), dim3(256), 0, s, xi, v, R, cap, counters);
 }
 
 // FP64 view of the FP32 state: xi, v widened, location mapped from (cell, xi)
 template <int DIM>
-__global__ void k_state_from32(DevMesh<DIM> m, const int* __restrict__ pcell, const float* __restrict__ xi32,
-                               const float* __restrict__ v32, double* __restrict__ x, double* __restrict__ xi,
-                               double* __restrict__ v, long long cap, const int* __restrict__ counters) {
+__global__ void k_state_from32(DevMesh<DIM> m, const int* __restrict__ pcell, Rec32 R, double* __restrict__ x,
+                               double* __restrict__ xi, double* __restrict__ v, long long cap,
+                               const int* __restrict__ counters) {
     constexpr int NV = Q1<DIM>::NV;
     const long long n = counters[0];
     for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
         double xd[DIM], P[NV][DIM], xx[DIM];
+        float xf[DIM], vf[DIM];
+        rec_load<DIM>(R, i, xf, vf);
 #pragma unroll
         for (int d = 0; d < DIM; ++d) {
-            xd[d] = (double)xi32[(size_t)d * cap + i];
+            xd[d] = (double)xf[d];
             xi[(size_t)d * cap + i] = xd[d];
-            v[(size_t)d * cap + i] = (double)v32[(size_t)d * cap + i];
+            v[(size_t)d * cap + i] = (double)vf[d];
         }
         load_vertices<DIM>(m, pcell[i], P);
         forward<DIM>(P, xd, xx);
@@ -719,13 +787,13 @@ __global__ void k_state_from32(DevMesh<DIM> m, const int* __restrict__ pcell, co
 }
 
 template <int DIM>
-void launch_state_from32(hipStream_t s, DevMesh<DIM> m, const int* pcell, const float* xi32, const float* v32,
-                         double* x, double* xi, double* v, long long cap, const int* counters, long long n_est) {
+void launch_state_from32(hipStream_t s, DevMesh<DIM> m, const int* pcell, Rec32 R, double* x, double* xi, double* v,
+                         long long cap, const int* counters, long long n_est) {
     long long g = (n_est + 255) / 256;
     if (g < 1) g = 1;
     if (g > 4096) g = 4096;
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_state_from32<DIM>), dim3((int)g), dim3(256), 0, s, m, pcell, xi32, v32, x, xi,
-                       v, cap, counters);
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_state_from32<DIM>), dim3((int)g), dim3(256), 0, s, m, pcell, R, x, xi, v,
+                       cap, counters);
 }
 
 }  // namespace pf2

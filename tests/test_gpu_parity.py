@@ -178,24 +178,26 @@ def test_moving_mesh_resort():
         assert np.array_equal(g.cell_counts(), o.cell_counts())
 
 
-def test_crowded_cells_bitwise():
-    """Cells holding more candidates than the per-group LDS rows (64) go
-    through the block-per-cell census; results stay bitwise equal."""
+@pytest.mark.parametrize("q", [[4, 4], [5, 5]])
+def test_crowded_cells_bitwise(q):
+    """Cells holding more candidates than the per-group LDS rows (64 for up
+    to 16 lattice parts, 128 above) go through the block-per-cell census;
+    results stay bitwise equal."""
     m = M.quad_2d(10, 6, (0.0, 0.0), (2.0, 1.2), jitter=0.15, seed=30)
     V = M.uniform_field(m, [0.4, 0.1]) + 0.1 * M.fourier_perturbation(m.vertices, [2.0, 1.2], seed=31)
-    cell, xi = M.jittered_lattice(m, [4, 4], 0.05, 32)
+    cell, xi = M.jittered_lattice(m, q, 0.05, 32)
     rng = np.random.default_rng(33)
     extra_cells = np.concatenate([np.full(150, 5), np.full(300, 17), np.full(90, 33)]).astype(np.int32)
     extra_xi = rng.uniform(0.0, 1.0, size=(len(extra_cells), 2))
     cell = np.concatenate([cell, extra_cells])
     xi = np.concatenate([xi, extra_xi])
     order = np.argsort(cell, kind="stable")
-    g = ParticleStage(m, 0.05, 5, 3, [4, 4], FP64, capacity=20000)
-    o = Oracle(m, 0.05, 5, 3, [4, 4])
+    g = ParticleStage(m, 0.05, 5, 3, q, FP64, capacity=20000)
+    o = Oracle(m, 0.05, 5, 3, q)
     for s in (g, o):
         s.upload_fields(V, V)
         s.insert_particles(cell[order], xi[order])
-    _run_steps(g, o, lambda s: V, 3, "crowded")
+    _run_steps(g, o, lambda s: V, 3, f"crowded q={q}")
 
 
 def test_overfull_cell_reports_capacity_error():
