@@ -391,7 +391,11 @@ __device__ int relocate(const DevMesh<DIM>& m, int c, const double (*P)[DIM], co
 // particles then run in full wavefronts (k_search_resume), which continue
 // the remaining substeps and queue again if needed.  After MOVE_ROUNDS
 // rounds the last kernel finishes every remaining particle inline.
-constexpr int MOVE_ROUNDS = 2;
+#ifndef PF2_MOVE_ROUNDS
+#define PF2_MOVE_ROUNDS 1
+#endif
+constexpr int MOVE_ROUNDS = PF2_MOVE_ROUNDS;
+static_assert(MOVE_ROUNDS >= 1 && MOVE_ROUNDS <= 2, "one or two parking rounds");
 
 // Queues of parked particles, one per round, ping-ponging between two
 // buffers.  A queue is QSHARDS shards of shard_cap entries plus an overflow

@@ -221,6 +221,13 @@ int cell_grid(int n_cells) {
 
 int pack_fields(pf2_handle_s* h);
 
+// search grid per round: the first round sees most parked particles; later
+// rounds see fewer, the last one loops every remaining particle to the end
+int round_grid(int qgrid, int r) {
+    if (r == 0) return qgrid;
+    return r == MOVE_ROUNDS ? qgrid / 2 + 1 : qgrid / 4 + 1;
+}
+
 int grid_for(long long n, int block, int cap_blocks = 65535 * 4) {
     long long g = (n + block - 1) / block;
     if (g < 1) g = 1;
@@ -486,7 +493,7 @@ int run_move(pf2_handle_s* h, int do_correct, int do_move) {
                     hipLaunchKernelGGL(k_queue_prefix, dim3(1), dim3(QSHARDS), 0, h->stream, qu, r);
                 }
                 Timed t(h, r == 0 ? K_SEARCH32 : (r == 1 ? K_SEARCH32_R1 : K_SEARCH32_R2));
-                launch_search_resume32<DIM>(h->stream, r == 0 ? qgrid : qgrid / 4 + 1, dev_mesh<DIM>(h), h->dV32,
+                launch_search_resume32<DIM>(h->stream, round_grid(qgrid, r), dev_mesh<DIM>(h), h->dV32,
                                             h->dgeo32, b.cell, rec32(b), b.vext, h->cap, pp, w, qu, r);
             }
     } else {
@@ -502,7 +509,7 @@ int run_move(pf2_handle_s* h, int do_correct, int do_move) {
                     hipLaunchKernelGGL(k_queue_prefix, dim3(1), dim3(QSHARDS), 0, h->stream, qu, r);
                 }
                 Timed t(h, r == 0 ? K_SEARCH : (r == 1 ? K_SEARCH_R1 : K_SEARCH_R2));
-                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_search_resume<DIM>), dim3(r == 0 ? qgrid : qgrid / 4 + 1),
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_search_resume<DIM>), dim3(round_grid(qgrid, r)),
                                    dim3(256), 0, h->stream, dev_mesh<DIM>(h), h->dV, dev_particles(h, h->cur), pp, w,
                                    qu, r);
             }
@@ -1338,7 +1345,7 @@ int pf2_step_info(pf2_handle h, long long* out, int cap) {
     int arrivals = 0;
     HIPC(hipMemcpyAsync(&arrivals, h->arr_off + h->n_cells, 4, hipMemcpyDeviceToHost, h->stream));
     HIPC(hipStreamSynchronize(h->stream));
-    long long vals[6];
+    long long vals[6] = {};
     vals[0] = cnt[3];
     vals[1] = cnt[0];
     for (int r = 0; r <= MOVE_ROUNDS; ++r) vals[2 + r] = pref[(size_t)r * (QSHARDS + 2) + QSHARDS + 1];
