@@ -151,6 +151,13 @@ int pf2_check_empty_cells(pf2_handle h);
  * directions are kept unless recompute_maps != 0, as in the reference where
  * initialize_maps runs once. */
 int pf2_update_vertices(pf2_handle h, const double* vertices, int recompute_maps);
+/* The moving-mesh sequence of cylinderDyM/cylinderDyM.cc:1157-1160 in one
+ * call: transform_grid (new vertex positions), then
+ * sort_particles_into_subdomains_and_cells and check_empty_cells.  The
+ * relocation and the reseed share one census; survivors, ids, counts and
+ * statistics equal those of the three calls.  The FP32-mixed path keeps its
+ * FP32 state (locations from the mesh before the move, relocation in FP64). */
+int pf2_move_mesh(pf2_handle h, const double* vertices, int recompute_maps);
 /* pfem2ParticleHandler::initialize_maps (TurekBenchmark/pfem2particle.cpp:215-219). */
 int pf2_initialize_maps(pf2_handle h);
 

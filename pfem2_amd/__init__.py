@@ -50,7 +50,7 @@ EXPORTS = [
     "pf2_update_vertices", "pf2_initialize_maps", "pf2_n_global_particles", "pf2_download_particles",
     "pf2_cell_counts", "pf2_particle_counter", "pf2_stats", "pf2_synchronize", "pf2_set_profiling",
     "pf2_kernel_times", "pf2_kernel_name", "pf2_step_info", "pf2_create_in", "pf2_estimate_bytes",
-    "pf2_memory_bytes", "pf2_step_with_field", "pf2_set_graphs",
+    "pf2_memory_bytes", "pf2_step_with_field", "pf2_set_graphs", "pf2_move_mesh",
     "pf2_server_last_error", "pf2_server_create", "pf2_server_destroy", "pf2_server_submit", "pf2_server_poll",
     "pf2_server_drain", "pf2_server_result", "pf2_server_stats",
 ]
@@ -90,6 +90,7 @@ def load_library(path=LIB_PATH):
     lib.pf2_step.argtypes = [vp, vp]
     lib.pf2_set_walls.argtypes = [vp, ip, vp, vp, vp, vp]
     lib.pf2_update_vertices.argtypes = [vp, vp, ip]
+    lib.pf2_move_mesh.argtypes = [vp, vp, ip]
     lib.pf2_n_global_particles.argtypes = [vp, ctypes.POINTER(ctypes.c_longlong)]
     lib.pf2_download_particles.argtypes = [vp, vp, vp, vp, vp, vp]
     lib.pf2_cell_counts.argtypes = [vp, vp]
@@ -271,6 +272,12 @@ class ParticleStage:
     def update_vertices(self, vertices, recompute_maps=False):
         self._vertices = _f64(vertices, (self.mesh.n_vertices, self.dim))
         self._check(self._lib.pf2_update_vertices(self._h, _ptr(self._vertices), 1 if recompute_maps else 0))
+
+    def move_mesh(self, vertices, recompute_maps=False):
+        """transform_grid + sort_particles_into_subdomains_and_cells +
+        check_empty_cells in one pass."""
+        self._vertices = _f64(vertices, (self.mesh.n_vertices, self.dim))
+        self._check(self._lib.pf2_move_mesh(self._h, _ptr(self._vertices), 1 if recompute_maps else 0))
 
     # -- pfem2ParticleHandler methods --
     def sort_particles_into_subdomains_and_cells(self):

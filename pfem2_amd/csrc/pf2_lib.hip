@@ -1293,6 +1293,57 @@ int pf2_update_vertices(pf2_handle h, const double* vertices, int recompute_maps
     return 0;
 }
 
+int pf2_move_mesh(pf2_handle h, const double* vertices, int recompute_maps) {
+    if (!h || !vertices) return fail(PF2_E_ARG, "NULL argument");
+    if (!h->params.keep_velocity_ext)
+        return fail(PF2_E_STATE, "relocation without advection needs keep_velocity_ext");
+    if (int r = ensure_packed(h)) return r;
+    PBufs& b = h->buf[h->cur];
+    const int grid = grid_for(h->n_est, 256, 4096);
+    if (mixed(h)) {
+        // locations on the mesh before the move, in FP64
+        Timed t(h, K_MISC);
+        if (h->dim == 2)
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pos32<2>), dim3(grid), dim3(256), 0, h->stream, dev_mesh<2>(h), b.cell,
+                               rec32(b), b.x, h->cap, h->counters);
+        else
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pos32<3>), dim3(grid), dim3(256), 0, h->stream, dev_mesh<3>(h), b.cell,
+                               rec32(b), b.x, h->cap, h->counters);
+        HIPC(hipGetLastError());
+    }
+    h->X.assign(vertices, vertices + (size_t)h->n_vertices * h->dim);
+    HIPC(hipMemcpyAsync(h->dX, h->X.data(), h->X.size() * 8, hipMemcpyHostToDevice, h->stream));
+    if (recompute_maps) {
+        compute_star_dirs(h);
+        if (int r = upload_star_dirs(h)) return r;
+    }
+    if (h->dim == 2) launch_geo<2>(h->stream, dev_mesh<2>(h), h->dgeo32, h->dftol, h->dcellX);
+    else launch_geo<3>(h->stream, dev_mesh<3>(h), h->dgeo32, h->dftol, h->dcellX);
+    DevParams pp = dev_params(h);
+    DevWork w = dev_work(h);
+    {
+        Timed t(h, K_MOVE);
+        if (mixed(h)) {
+            h->fp64_valid = false;
+            if (h->dim == 2)
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_resort32<2>), dim3(grid), dim3(256), 0, h->stream,
+                                   dev_mesh<2>(h), b.cell, b.x, b.vext, rec32(b), h->cap, pp, w);
+            else
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_resort32<3>), dim3(grid), dim3(256), 0, h->stream,
+                                   dev_mesh<3>(h), b.cell, b.x, b.vext, rec32(b), h->cap, pp, w);
+        } else if (h->dim == 2)
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_resort<2>), dim3(grid), dim3(256), 0, h->stream, dev_mesh<2>(h),
+                               dev_particles(h, h->cur), pp, w);
+        else
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_resort<3>), dim3(grid), dim3(256), 0, h->stream, dev_mesh<3>(h),
+                               dev_particles(h, h->cur), pp, w);
+    }
+    HIPC(hipGetLastError());
+    // relocation and reseed in one census: the same survivors, ids and
+    // counts as the re-sort's census followed by check_empty_cells'
+    return h->dim == 2 ? regroup<2>(h, 1) : regroup<3>(h, 1);
+}
+
 int pf2_initialize_maps(pf2_handle h) {
     if (!h) return fail(PF2_E_ARG, "handle is NULL");
     compute_star_dirs(h);

@@ -796,4 +796,76 @@ void launch_state_from32(hipStream_t s, DevMesh<DIM> m, const int* pcell, Rec32 
                        cap, counters);
 }
 
+// FP64 location F_K(xi) of every FP32-state particle on the current
+// geometry (before the vertices move)
+template <int DIM>
+__global__ void k_pos32(DevMesh<DIM> m, const int* __restrict__ pcell, Rec32 R, double* __restrict__ x,
+                        long long cap, const int* __restrict__ counters) {
+    constexpr int NV = Q1<DIM>::NV;
+    const long long n = counters[0];
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        float xf[DIM];
+        double xd[DIM], P[NV][DIM], xx[DIM];
+        rec_load_xi<DIM>(R, i, xf);
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) xd[d] = (double)xf[d];
+        load_vertices<DIM>(m, pcell[i], P);
+        forward<DIM>(P, xd, xx);
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) x[(size_t)d * cap + i] = xx[d];
+    }
+}
+
+// k_resort for the FP32 state: own-cell test and relocation of the fixed
+// location x on the moved mesh in FP64, reference point stored in FP32
+template <int DIM>
+__global__ void __launch_bounds__(256) k_resort32(DevMesh<DIM> m, const int* __restrict__ pcell,
+                                                  const double* __restrict__ px, const double* __restrict__ pvext,
+                                                  Rec32 R, long long cap, DevParams pp, DevWork w) {
+    constexpr int NV = Q1<DIM>::NV;
+    const int n = w.counters[0];
+    const int lane = threadIdx.x & (WAVE - 1);
+    int lost = 0, searches = 0;
+    for (int base = blockIdx.x * blockDim.x + (threadIdx.x - lane); base < n; base += gridDim.x * blockDim.x) {
+        const int i = base + lane;
+        const bool valid = i < n;
+        const int c0 = valid ? pcell[i] : -1;
+        int c = c0;
+        if (valid) {
+            double x[DIM], xi[DIM], vext[DIM], r[DIM], P[NV][DIM];
+            float xf[DIM];
+            rec_load_xi<DIM>(R, i, xf);
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) {
+                xi[d] = (double)xf[d];
+                x[d] = px[(size_t)d * cap + i];
+                vext[d] = pvext ? pvext[(size_t)d * cap + i] : 0.0;
+            }
+            load_vertices<DIM>(m, c, P);
+            bool found = inverse_map<DIM>(P, x, r, m.ftol[c], false) && inside_unit<DIM>(r);
+            if (!found) {
+                ++searches;
+                double Q[NV][DIM];
+                NoPrefetch pre;
+                c = relocate<DIM>(m, c, P, xi, vext, x, r, Q, pre);
+                found = c >= 0;
+                if (!found) ++lost;
+            }
+            if (found) {
+#pragma unroll
+                for (int d = 0; d < DIM; ++d) {
+                    const float t = (float)r[d];
+                    xf[d] = t < 0.0f ? 0.0f : (t > 1.0f ? 1.0f : t);
+                }
+                rec_store_xi<DIM>(R, i, xf);
+                w.pkey[i] = part_key32<DIM>(pp, xf);
+            }
+            w.dest[i] = c;
+        }
+        if (valid && c >= 0 && c != c0) atomicAdd(&w.arr_cnt[c], 1);
+    }
+    wave_stat(w.stats, 0, lost);
+    wave_stat(w.stats, 3, searches);
+}
+
 }  // namespace pf2

@@ -101,3 +101,39 @@ def test_fp32_mixed_fixed_point_and_determinism():
     assert np.array_equal(a["cell"], b["cell"]) and np.array_equal(a["id"], b["id"])
     assert np.max(np.abs(a["xi"] - b["xi"])) < 1e-7
     assert np.max(np.abs(a["v"] - b["v"])) < 1e-6 * np.max(np.abs(a["v"]))
+
+
+def test_fp32_mixed_moving_mesh():
+    """FP32 state through the fused mesh move (locations from the mesh before
+    the move, FP64 relocation) against the FP64 oracle's three calls."""
+    m = M.cylinder_channel(0.1, seed=37)
+    V = M.cylinder_flow(m, 1.0, eps=0.01, seed=38)
+    g = ParticleStage(m, 0.002, 8, 2, [4, 4], FP32_MIXED, keep_velocity_ext=True)
+    o = Oracle(m, 0.002, 8, 2, [4, 4])
+    cell, xi = M.jittered_lattice(m, [4, 4], 0.05, 39)
+    for st in (g, o):
+        st.upload_fields(V, V)
+        st.insert_particles(cell, xi)
+    n0 = len(cell)
+    X0 = m.vertices.copy()
+    d = np.hypot(X0[:, 0] - 0.2, X0[:, 1] - 0.2)
+    w = np.clip(1.0 - (d - 0.05) / 0.15, 0.0, 1.0)
+    for s in range(6):
+        g.step()
+        Vo = o.step()
+        X = X0.copy()
+        X[:, 1] += 0.003 * np.sin(0.5 * (s + 1)) * w
+        g.move_mesh(X)
+        o.update_vertices(X)
+        o.sort_particles_into_subdomains_and_cells()
+        o.check_empty_cells()
+        g.upload_fields(V, None)
+        o.upload_fields(V, None)
+    Vg = g.step(download=True)
+    Vo = o.step()
+    err = field_errors(g, o, m, Vg, Vo, n0)
+    print("moving fp32", err, flush=True)
+    assert err["V_rel_l2"] <= 1e-5
+    assert err["x_rel"] <= 1e-5
+    assert err["owner_mismatch"] <= 1e-3
+    assert err["count_mismatch"] <= 1e-2

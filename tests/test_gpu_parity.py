@@ -156,9 +156,11 @@ def test_walls_fixup():
     _run_steps(g, o, lambda s: V, 4, "walls")
 
 
-def test_moving_mesh_resort():
+@pytest.mark.parametrize("fused", [False, True])
+def test_moving_mesh_resort(fused):
     """cylinderDyM order: step, then move vertices, re-sort without advection,
-    reseed empty parts (cylinderDyM/cylinderDyM.cc:1147-1160)."""
+    reseed empty parts (cylinderDyM/cylinderDyM.cc:1147-1160); the fused
+    pf2_move_mesh gives the same bits as the three calls."""
     m = M.cylinder_channel(0.08, seed=26)
     V = M.cylinder_flow(m, 1.0, eps=0.02, seed=27)
     g, o = _pair(m, V, 0.004, 5, 3, [3, 3], seed=28, keep_vext=True)
@@ -170,12 +172,20 @@ def test_moving_mesh_resort():
         w = np.clip(1.0 - (d - 0.05) / 0.15, 0.0, 1.0)        # displacement decays away from the cylinder
         X = X0.copy()
         X[:, 1] += 0.004 * np.sin(0.7 * (s + 1)) * w
-        for st in (g, o):
-            st.update_vertices(X, recompute_maps=(s == 1))
-            st.sort_particles_into_subdomains_and_cells()
-            st.check_empty_cells()
+        if fused:
+            g.move_mesh(X, recompute_maps=(s == 1))
+        else:
+            g.update_vertices(X, recompute_maps=(s == 1))
+            g.sort_particles_into_subdomains_and_cells()
+            g.check_empty_cells()
+        o.update_vertices(X, recompute_maps=(s == 1))
+        o.sort_particles_into_subdomains_and_cells()
+        o.check_empty_cells()
         assert_particles_match(g.particles(), o.particles(), REL, f"moving mesh {s}")
         assert np.array_equal(g.cell_counts(), o.cell_counts())
+        assert g.particle_counter() == o.particle_counter()
+        sg, so = g.stats(), o.stats()
+        assert sg["lost"] == so["lost"] and sg["spawned"] == so["spawned"] and sg["deleted"] == so["deleted"]
 
 
 @pytest.mark.parametrize("q", [[4, 4], [5, 5]])
