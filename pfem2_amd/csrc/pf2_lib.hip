@@ -221,6 +221,13 @@ int cell_grid(int n_cells) {
 
 int pack_fields(pf2_handle_s* h);
 
+// bound of the per-cell particle count after a census with reseed
+long long fuse_bound(const pf2_handle_s* h) {
+    long long b = h->params.max_per_part;
+    for (int d = 0; d < h->dim; ++d) b *= h->params.quantities[d] + 1;
+    return b;
+}
+
 // search grid per round: the first round sees most parked particles; later
 // rounds see fewer, the last one loops every remaining particle to the end
 int round_grid(int qgrid, int r) {
@@ -422,7 +429,7 @@ int get_count(pf2_handle_s* h, long long* n) {
 // Regroup after k_move / k_resort / k_keys filled dest and pkey:
 // arrivals, census, survivors + reseed into the other buffer, swap.
 template <int DIM>
-int regroup(pf2_handle_s* h, int reseed) {
+int regroup(pf2_handle_s* h, int reseed, bool fuse_sums = false) {
     const int nc = h->n_cells;
     DevParams pp = dev_params(h);
     DevWork w = dev_work(h);
@@ -454,7 +461,7 @@ int regroup(pf2_handle_s* h, int reseed) {
         launch_cell_write32<DIM>(h->stream, cell_grid(nc), dev_mesh<DIM>(h), h->dV, old_off, h->buf[h->cur].id,
                                  rec32(h->buf[h->cur]), h->buf[1 - h->cur].cell, h->buf[1 - h->cur].id,
                                  rec32(h->buf[1 - h->cur]), h->buf[h->cur].vext, h->buf[1 - h->cur].vext, h->cap, pp,
-                                 w);
+                                 w, fuse_sums, h->partial);
     } else {
         Timed t(h, K_CELL_WRITE);
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cell_write<DIM>), dim3(cell_grid(nc)), dim3(256), 0, h->stream,
@@ -472,7 +479,7 @@ int regroup(pf2_handle_s* h, int reseed) {
 }
 
 template <int DIM>
-int run_move(pf2_handle_s* h, int do_correct, int do_move) {
+int run_move(pf2_handle_s* h, int do_correct, int do_move, bool fuse_sums = false) {
     DevParams pp = dev_params(h);
     DevWork w = dev_work(h);
     DevQueues qu = dev_queues(h);
@@ -516,14 +523,16 @@ int run_move(pf2_handle_s* h, int do_correct, int do_move) {
     }
     HIPC(hipGetLastError());
     if (!do_move) return 0;
-    return regroup<DIM>(h, 1);
+    return regroup<DIM>(h, 1, fuse_sums);
 }
 
 template <int DIM>
-int run_project(pf2_handle_s* h) {
+int run_project(pf2_handle_s* h, bool sums_done = false) {
     const int nc = h->n_cells;
     DevWork w = dev_work(h);
-    if (mixed(h)) {
+    if (sums_done) {
+        // the projection sums came with the regroup (k_cell_write32<2, true>)
+    } else if (mixed(h)) {
         Timed t(h, K_PARTIALS32);
         launch_partials32<DIM>(h->stream, cell_grid(nc), nc, h->off[h->cur_off], rec32(h->buf[h->cur]), h->partial);
     } else {
@@ -711,8 +720,11 @@ int step_body(pf2_handle_s* h, const double* field) {
         Timed t(h, K_MISC);
         if (int r = pack_fields(h)) return r;
     }
-    if (int r = (h->dim == 2 ? run_move<2>(h, 1, 1) : run_move<3>(h, 1, 1))) return r;
-    return h->dim == 2 ? run_project<2>(h) : run_project<3>(h);
+    // 2-D FP32-mixed steps whose per-cell final count is bounded by the
+    // LDS rows compute the projection sums while regrouping
+    const bool fuse = h->dim == 2 && mixed(h) && fuse_bound(h) <= FCAP;
+    if (int r = (h->dim == 2 ? run_move<2>(h, 1, 1, fuse) : run_move<3>(h, 1, 1))) return r;
+    return h->dim == 2 ? run_project<2>(h, fuse) : run_project<3>(h);
 }
 
 // the step as a HIP graph: captured once per buffer parity, replayed after.

@@ -529,13 +529,54 @@ void launch_search_resume32(hipStream_t s, int grid, DevMesh<DIM> m, const float
                        pvext, cap, pp, w, qu, round);
 }
 
+// rows of the fused projection sums: a cell's final particles, 2-D records
+constexpr int FCAP = 64;
+
+// Kahan-compensated FP32 projection sums of one cell over n staged records
+// in order (lane gl < NACC owns one sum), as k_cell_partials32 computes them
 template <int DIM>
+__device__ __forceinline__ double cell_sums32(const float4* __restrict__ rec, int n, int gl) {
+    constexpr int NV = Q1<DIM>::NV;
+    const int kk = gl / (DIM + 1);
+    const int comp = gl - kk * (DIM + 1);
+    float sum = 0.0f, cmp = 0.0f;
+    for (int q = 0; q < n; ++q) {
+        const float4 r = rec[q];
+        const float xi[2] = {r.x, r.y};
+        const float v[2] = {r.z, r.w};
+        float N[NV];
+        shape32<DIM>(xi, N);
+        float nk = N[0];
+#pragma unroll
+        for (int t = 1; t < NV; ++t)
+            if (t == kk) nk = N[t];
+        float term = nk;
+        if (comp < DIM) {
+            float vc = v[0];
+#pragma unroll
+            for (int d = 1; d < DIM; ++d)
+                if (d == comp) vc = v[d];
+            term = nk * vc;
+        }
+        const float y = term - cmp;
+        const float tt = sum + y;
+        cmp = (tt - sum) - y;
+        sum = tt;
+    }
+    return (double)sum - (double)cmp;
+}
+
+// FUSE (2-D, final count per cell <= FCAP): the group also stages the cell's
+// final records in LDS and writes its projection sums, so no separate
+// partials pass re-reads the particles
+template <int DIM, bool FUSE>
 __device__ __forceinline__ void cell_write32_one(const DevMesh<DIM>& m, const double* __restrict__ V,
                                                  const int* __restrict__ old_off, const uint32_t* __restrict__ aid,
                                                  const Rec32& A, int* __restrict__ bcell, uint32_t* __restrict__ bid,
                                                  const Rec32& B, const double* __restrict__ avext,
                                                  double* __restrict__ bvext, long long cap, const DevParams& pp,
-                                                 const DevWork& w, int c, int gl) {
+                                                 const DevWork& w, int c, int gl, float4* __restrict__ srec,
+                                                 double* __restrict__ partial) {
     constexpr int NV = Q1<DIM>::NV;
     const int n_st = old_off[c + 1] - old_off[c];
     const int k = n_st + w.arr_cnt[c];
@@ -555,7 +596,13 @@ __device__ __forceinline__ void cell_write32_one(const DevMesh<DIM>& m, const do
         const long long o = out0 + pos;
         bcell[o] = c;
         bid[o] = aid[idx];
-        rec_copy<DIM>(A, idx, B, o);
+        if constexpr (FUSE) {
+            const float4 r = A.a[idx];
+            B.a[o] = r;
+            srec[pos] = r;
+        } else {
+            rec_copy<DIM>(A, idx, B, o);
+        }
         if (pp.keep_vext)
 #pragma unroll
             for (int d = 0; d < DIM; ++d) bvext[(size_t)d * cap + o] = avext[(size_t)d * cap + idx];
@@ -583,32 +630,47 @@ __device__ __forceinline__ void cell_write32_one(const DevMesh<DIM>& m, const do
                 if (pp.keep_vext) bvext[(size_t)d * cap + q] = 0.0;
             }
             rec_store<DIM>(B, q, xf, vf);
+            if constexpr (FUSE) srec[nsurv + r] = make_float4(xf[0], xf[1], vf[0], vf[1]);
         }
     }
     if (gl == 0) {
         w.arr_cnt[c] = 0;
         w.arr_fill[c] = 0;
     }
+    if constexpr (FUSE) {
+        constexpr int NACC = PartialsShape<DIM>::NACC;
+        wave_sync();
+        if (gl < NACC) partial[(size_t)c * NACC + gl] = cell_sums32<DIM>(srec, nfinal, gl);
+        wave_sync();
+    }
 }
 
-template <int DIM>
+template <int DIM, bool FUSE>
 __global__ void __launch_bounds__(256) k_cell_write32(DevMesh<DIM> m, const double* __restrict__ V,
                                                      const int* __restrict__ old_off, const uint32_t* __restrict__ aid,
                                                      Rec32 A, int* __restrict__ bcell, uint32_t* __restrict__ bid,
                                                      Rec32 B, const double* __restrict__ avext,
                                                      double* __restrict__ bvext, long long cap, DevParams pp,
-                                                     DevWork w) {
+                                                     DevWork w, double* __restrict__ partial) {
+    __shared__ float4 s_rec[FUSE ? GROUPS : 1][FUSE ? FCAP : 1];
     const int g = threadIdx.x / GS, gl = threadIdx.x % GS;
     for (int c = blockIdx.x * GROUPS + g; c < m.n_cells; c += gridDim.x * GROUPS)
-        cell_write32_one<DIM>(m, V, old_off, aid, A, bcell, bid, B, avext, bvext, cap, pp, w, c, gl);
+        cell_write32_one<DIM, FUSE>(m, V, old_off, aid, A, bcell, bid, B, avext, bvext, cap, pp, w, c, gl,
+                                    s_rec[FUSE ? g : 0], partial);
 }
 
+// fuse = the projection sums are computed here (2-D; final counts per cell
+// bounded by FCAP, i.e. MAX * prod(q_d + 1) <= FCAP and a census with reseed)
 template <int DIM>
 void launch_cell_write32(hipStream_t s, int grid, DevMesh<DIM> m, const double* V, const int* old_off, const uint32_t* aid,
                          Rec32 A, int* bcell, uint32_t* bid, Rec32 B, const double* avext, double* bvext, long long cap,
-                         DevParams pp, DevWork w) {
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cell_write32<DIM>), dim3(grid), dim3(256), 0, s, m, V, old_off, aid, A,
-                       bcell, bid, B, avext, bvext, cap, pp, w);
+                         DevParams pp, DevWork w, bool fuse, double* partial) {
+    if (DIM == 2 && fuse)
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cell_write32<DIM, DIM == 2>), dim3(grid), dim3(256), 0, s, m, V, old_off,
+                           aid, A, bcell, bid, B, avext, bvext, cap, pp, w, partial);
+    else
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cell_write32<DIM, false>), dim3(grid), dim3(256), 0, s, m, V, old_off,
+                           aid, A, bcell, bid, B, avext, bvext, cap, pp, w, partial);
 }
 
 // per-cell projection sums in FP32 with Kahan compensation, ascending id order
