@@ -158,6 +158,13 @@ int pf2_update_vertices(pf2_handle h, const double* vertices, int recompute_maps
  * statistics equal those of the three calls.  The FP32-mixed path keeps its
  * FP32 state (locations from the mesh before the move, relocation in FP64). */
 int pf2_move_mesh(pf2_handle h, const double* vertices, int recompute_maps);
+/* A prescribed mesh motion X(t) = X0 + a(t) W, for the moving-mesh case
+ * without a host elasticity solve: pf2_set_motion stores the current
+ * vertices as X0 and the displacement shape W (n_vertices x dim, host);
+ * pf2_move_mesh_scaled(h, a) is pf2_move_mesh with the vertices X0 + a W
+ * formed on the device.  Needs keep_velocity_ext (which reserves X0, W). */
+int pf2_set_motion(pf2_handle h, const double* W);
+int pf2_move_mesh_scaled(pf2_handle h, double a);
 /* pfem2ParticleHandler::initialize_maps (TurekBenchmark/pfem2particle.cpp:215-219). */
 int pf2_initialize_maps(pf2_handle h);
 
@@ -217,6 +224,12 @@ int pf2_server_destroy(pf2_server s);
  * PF2_E_CAPACITY when the simulation alone exceeds the budget. */
 int pf2_server_submit(pf2_server s, const pf2_mesh* mesh, const pf2_params* params, const double* V, int n_steps,
                       long long* request_id);
+/* A moving-mesh request: after each step k (1-based) the mesh moves to
+ * X0 + amplitude * sin(omega * k * dt + phase) * W (pf2_move_mesh_scaled);
+ * W is n_vertices x dim.  keep_velocity_ext is set for it. */
+int pf2_server_submit_moving(pf2_server s, const pf2_mesh* mesh, const pf2_params* params, const double* V,
+                             int n_steps, const double* W, double amplitude, double omega, double phase,
+                             long long* request_id);
 /* Retires finished simulations, admits pending ones, issues steps; never
  * waits for the GPU.  n_pending / n_active may be NULL. */
 int pf2_server_poll(pf2_server s, int* n_pending, int* n_active);

@@ -50,7 +50,8 @@ EXPORTS = [
     "pf2_update_vertices", "pf2_initialize_maps", "pf2_n_global_particles", "pf2_download_particles",
     "pf2_cell_counts", "pf2_particle_counter", "pf2_stats", "pf2_synchronize", "pf2_set_profiling",
     "pf2_kernel_times", "pf2_kernel_name", "pf2_step_info", "pf2_create_in", "pf2_estimate_bytes",
-    "pf2_memory_bytes", "pf2_step_with_field", "pf2_set_graphs", "pf2_move_mesh",
+    "pf2_memory_bytes", "pf2_step_with_field", "pf2_set_graphs", "pf2_move_mesh", "pf2_set_motion",
+    "pf2_move_mesh_scaled", "pf2_server_submit_moving",
     "pf2_server_last_error", "pf2_server_create", "pf2_server_destroy", "pf2_server_submit", "pf2_server_poll",
     "pf2_server_drain", "pf2_server_result", "pf2_server_stats",
 ]
@@ -91,6 +92,8 @@ def load_library(path=LIB_PATH):
     lib.pf2_set_walls.argtypes = [vp, ip, vp, vp, vp, vp]
     lib.pf2_update_vertices.argtypes = [vp, vp, ip]
     lib.pf2_move_mesh.argtypes = [vp, vp, ip]
+    lib.pf2_set_motion.argtypes = [vp, vp]
+    lib.pf2_move_mesh_scaled.argtypes = [vp, dp]
     lib.pf2_n_global_particles.argtypes = [vp, ctypes.POINTER(ctypes.c_longlong)]
     lib.pf2_download_particles.argtypes = [vp, vp, vp, vp, vp, vp]
     lib.pf2_cell_counts.argtypes = [vp, vp]
@@ -111,6 +114,8 @@ def load_library(path=LIB_PATH):
     lib.pf2_server_destroy.argtypes = [vp]
     lib.pf2_server_submit.argtypes = [vp, ctypes.POINTER(_Mesh), ctypes.POINTER(_Params), vp, ip,
                                       ctypes.POINTER(ll)]
+    lib.pf2_server_submit_moving.argtypes = [vp, ctypes.POINTER(_Mesh), ctypes.POINTER(_Params), vp, ip, vp, dp, dp,
+                                             dp, ctypes.POINTER(ll)]
     lib.pf2_server_poll.argtypes = [vp, ctypes.POINTER(ip), ctypes.POINTER(ip)]
     lib.pf2_server_drain.argtypes = [vp]
     lib.pf2_server_result.argtypes = [vp, ll, vp, ctypes.POINTER(ll), ctypes.POINTER(dp)]
@@ -279,6 +284,15 @@ class ParticleStage:
         self._vertices = _f64(vertices, (self.mesh.n_vertices, self.dim))
         self._check(self._lib.pf2_move_mesh(self._h, _ptr(self._vertices), 1 if recompute_maps else 0))
 
+    def set_motion(self, W):
+        """Prescribed motion X0 + a W: the current vertices become X0."""
+        self._W = _f64(W, (self.mesh.n_vertices, self.dim))
+        self._check(self._lib.pf2_set_motion(self._h, _ptr(self._W)))
+
+    def move_mesh_scaled(self, a):
+        """move_mesh with the vertices X0 + a W formed on the device."""
+        self._check(self._lib.pf2_move_mesh_scaled(self._h, float(a)))
+
     # -- pfem2ParticleHandler methods --
     def sort_particles_into_subdomains_and_cells(self):
         self._check(self._lib.pf2_sort_particles_into_subdomains_and_cells(self._h))
@@ -420,6 +434,23 @@ class Server:
         rid = ctypes.c_longlong()
         self._check(self._lib.pf2_server_submit(self._s, ctypes.byref(ms), ctypes.byref(p), _ptr(V), n_steps,
                                                 ctypes.byref(rid)))
+        self._dims[rid.value] = (m.dim, m.n_dofs)
+        return rid.value
+
+    def submit_moving(self, m, V, n_steps, dt, substeps, max_per_part, quantities, W, amplitude, omega, phase=0.0,
+                      precision=FP64):
+        """A request whose mesh moves to X0 + amplitude sin(omega k dt + phase) W
+        after each step k."""
+        vertices = _f64(m.vertices)
+        cells = np.ascontiguousarray(m.cells, dtype=np.int32)
+        V = _f64(V, (m.dim, m.n_dofs))
+        W = _f64(W, (m.n_vertices, m.dim))
+        ms = _Mesh(m.dim, m.n_cells, m.n_vertices, m.n_dofs, _ptr(vertices), _ptr(cells), None)
+        p = _params_of(m.dim, dt, substeps, max_per_part, quantities, precision, True)
+        rid = ctypes.c_longlong()
+        self._check(self._lib.pf2_server_submit_moving(self._s, ctypes.byref(ms), ctypes.byref(p), _ptr(V), n_steps,
+                                                       _ptr(W), float(amplitude), float(omega), float(phase),
+                                                       ctypes.byref(rid)))
         self._dims[rid.value] = (m.dim, m.n_dofs)
         return rid.value
 

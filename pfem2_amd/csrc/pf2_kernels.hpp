@@ -1344,6 +1344,13 @@ __global__ void k_keys(DevParticles A, DevParams pp, DevWork w) {
 // per-cell copies of V and V - V_old at the cell's DoFs (one load level in
 // the particle kernels); V - V_old is the same per-node difference the
 // correction forms (TurekBenchmark/pfem2particle.cpp:806-807)
+// prescribed mesh motion: X = X0 + a W
+__global__ void k_displace(const double* __restrict__ X0, const double* __restrict__ W, double a,
+                           double* __restrict__ X, long long n) {
+    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x)
+        X[t] = X0[t] + a * W[t];
+}
+
 // V refreshed from a component-major device field: the node-major copy and
 // the per-cell copies of V and V - V_old (FP32 copies when V32 is given)
 template <int DIM>

@@ -162,6 +162,9 @@ struct pf2_handle_s {
     size_t bytes = 0;                  // device memory held by the handle
     bool in_block = false;             // created inside caller memory: nothing of it is freed here
     bool packed = true;                // per-cell copies of V, V - V_old are current
+    // prescribed mesh motion X = X0 + a W (handles with keep_velocity_ext)
+    double *dX0 = nullptr, *dW = nullptr;
+    bool x_host_stale = false;         // the device vertices moved without the host copy
     bool own_stream = true;
 };
 
@@ -688,7 +691,10 @@ size_t estimate_bytes(int dim, long long nc, long long nvt, long long nd, long l
     b += 4 + (size_t)STAT_SHARDS * NSTAT * 8;
     b += nck * dim * 4 * 2 + nc * geo32_floats(dim) * 4;                                 // FP32 fields, geometry
     long long per = 4 + 4 + 3 * dim * 8;
-    if (p->keep_velocity_ext) per += dim * 8;
+    if (p->keep_velocity_ext) {
+        per += dim * 8;
+        b += 2 * nvt * dim * 8;    // rest positions and displacement shape of a prescribed motion
+    }
     if (p->precision == PF2_FP32_MIXED) per += 2 * dim * 4;
     b += 2 * cap * per;                                                                 // two particle buffers
     const long long shard_cap = 2 * ((cap + QSHARDS - 1) / QSHARDS) + 64;
@@ -759,9 +765,12 @@ int step_graph(pf2_handle_s* h, const double* field) {
         h->cur_off = 1 - h->cur_off;
     }
     HIPC(hipGraphLaunch(h->gexec[key], h->stream));
+    // host-side effects of an eager step: buffers swapped, FP32 state ahead
+    // of its FP64 copy, per-cell field copies stale after the projection
     h->cur = 1 - h->cur;
     h->cur_off = 1 - h->cur_off;
     if (mixed(h)) h->fp64_valid = false;
+    h->packed = false;
     return 0;
 }
 
@@ -874,6 +883,9 @@ int pf2_create_in(const pf2_mesh* mesh, const pf2_params* params, long long capa
             return bail(PF2_E_HIP);
     }
     if (alloc_particles(h)) return bail(PF2_E_HIP);
+    if (h->params.keep_velocity_ext &&
+        (dalloc(&h->dX0, (size_t)nvt * dim) || dalloc(&h->dW, (size_t)nvt * dim)))
+        return bail(PF2_E_HIP);
     auto up = [&](void* d, const void* s, size_t bytes) {
         return hipMemcpyAsync(d, s, bytes, hipMemcpyHostToDevice, h->stream);
     };
@@ -919,6 +931,7 @@ int pf2_destroy(pf2_handle h) {
     free_particles(h);
     void* ptrs[] = {h->dX, h->dcv, h->dcd, h->dstar_off, h->dstar_cell, h->dstar_dir, h->ddof_off, h->ddof_cell,
                     h->dftol, h->dcellX, h->dcell_star, h->dcellV, h->dcellDV, h->dV, h->dVold, h->dtmp, h->dV32,
+                    h->dX0, h->dW,
                     h->dVold32, h->dgeo32, h->off[0], h->off[1], h->arr_cnt, h->arr_off, h->arr_fill, h->final_cnt,
                     h->spawn_cnt, h->spawn_off, h->hitmask, h->nsurv, h->partial, h->counters, h->big, h->qcount, h->qprefix,
                     h->pcounter, h->stats, h->scan_tmp};
@@ -1305,8 +1318,9 @@ int pf2_update_vertices(pf2_handle h, const double* vertices, int recompute_maps
     return 0;
 }
 
-int pf2_move_mesh(pf2_handle h, const double* vertices, int recompute_maps) {
-    if (!h || !vertices) return fail(PF2_E_ARG, "NULL argument");
+// transform_grid + re-sort + check_empty_cells with one census; the new
+// vertices come from the host (vertices) or from X0 + a W on the device
+int move_mesh_impl(pf2_handle_s* h, const double* vertices, int recompute_maps, const double* scale) {
     if (!h->params.keep_velocity_ext)
         return fail(PF2_E_STATE, "relocation without advection needs keep_velocity_ext");
     if (int r = ensure_packed(h)) return r;
@@ -1323,8 +1337,17 @@ int pf2_move_mesh(pf2_handle h, const double* vertices, int recompute_maps) {
                                rec32(b), b.x, h->cap, h->counters);
         HIPC(hipGetLastError());
     }
-    h->X.assign(vertices, vertices + (size_t)h->n_vertices * h->dim);
-    HIPC(hipMemcpyAsync(h->dX, h->X.data(), h->X.size() * 8, hipMemcpyHostToDevice, h->stream));
+    if (vertices) {
+        h->X.assign(vertices, vertices + (size_t)h->n_vertices * h->dim);
+        HIPC(hipMemcpyAsync(h->dX, h->X.data(), h->X.size() * 8, hipMemcpyHostToDevice, h->stream));
+        h->x_host_stale = false;
+    } else {
+        const long long n = (long long)h->n_vertices * h->dim;
+        hipLaunchKernelGGL(k_displace, dim3(grid_for(n, 256, 4096)), dim3(256), 0, h->stream, h->dX0, h->dW, *scale,
+                           h->dX, n);
+        HIPC(hipGetLastError());
+        h->x_host_stale = true;
+    }
     if (recompute_maps) {
         compute_star_dirs(h);
         if (int r = upload_star_dirs(h)) return r;
@@ -1356,8 +1379,34 @@ int pf2_move_mesh(pf2_handle h, const double* vertices, int recompute_maps) {
     return h->dim == 2 ? regroup<2>(h, 1) : regroup<3>(h, 1);
 }
 
+int pf2_move_mesh(pf2_handle h, const double* vertices, int recompute_maps) {
+    if (!h || !vertices) return fail(PF2_E_ARG, "NULL argument");
+    return move_mesh_impl(h, vertices, recompute_maps, nullptr);
+}
+
+int pf2_set_motion(pf2_handle h, const double* W) {
+    if (!h || !W) return fail(PF2_E_ARG, "NULL argument");
+    if (!h->dX0) return fail(PF2_E_STATE, "a prescribed motion needs keep_velocity_ext");
+    const size_t n = (size_t)h->n_vertices * h->dim;
+    HIPC(hipMemcpyAsync(h->dX0, h->dX, n * 8, hipMemcpyDeviceToDevice, h->stream));
+    HIPC(hipMemcpyAsync(h->dW, W, n * 8, hipMemcpyHostToDevice, h->stream));
+    HIPC(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+int pf2_move_mesh_scaled(pf2_handle h, double a) {
+    if (!h) return fail(PF2_E_ARG, "handle is NULL");
+    if (!h->dX0) return fail(PF2_E_STATE, "a prescribed motion needs keep_velocity_ext");
+    return move_mesh_impl(h, nullptr, 0, &a);
+}
+
 int pf2_initialize_maps(pf2_handle h) {
     if (!h) return fail(PF2_E_ARG, "handle is NULL");
+    if (h->x_host_stale) {
+        HIPC(hipMemcpyAsync(h->X.data(), h->dX, h->X.size() * 8, hipMemcpyDeviceToHost, h->stream));
+        HIPC(hipStreamSynchronize(h->stream));
+        h->x_host_stale = false;
+    }
     compute_star_dirs(h);
     if (int r = upload_star_dirs(h)) return r;
     HIPC(hipStreamSynchronize(h->stream));

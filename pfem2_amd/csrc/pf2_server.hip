@@ -43,6 +43,10 @@ struct Request {
     std::vector<int> cv, cd;
     pf2_params params{};
     int n_steps = 0;
+    // prescribed mesh motion X0 + amp sin(omega k dt + phase) W after step k
+    bool moving = false;
+    std::vector<double> W;
+    double amp = 0, omega = 0, phase = 0;
     unsigned long long bytes = 0;   // footprint reserved in the block (handle + field)
     // while active
     pf2_handle h = nullptr;
@@ -173,6 +177,8 @@ void release(pf2_server_s* s, Request* r) {
     r->cd.shrink_to_fit();
     r->V.clear();
     r->V.shrink_to_fit();
+    r->W.clear();
+    r->W.shrink_to_fit();
 }
 
 void mark_failed(pf2_server_s* s, Request* r, int code, const std::string& msg) {
@@ -230,7 +236,8 @@ int admit(pf2_server_s* s, Request* r) {
     }
     long long n = 0;
     if ((rc = pf2_upload_fields(r->h, r->V.data(), r->V.data())) || (rc = pf2_seed_particles(r->h)) ||
-        (rc = pf2_n_global_particles(r->h, &n)) || (rc = pf2_set_graphs(r->h, s->cfg.use_graphs))) {
+        (rc = pf2_n_global_particles(r->h, &n)) || (rc = pf2_set_graphs(r->h, s->cfg.use_graphs)) ||
+        (r->moving && (rc = pf2_set_motion(r->h, r->W.data())))) {
         mark_failed(s, r, rc, pf2_last_error());
         return 0;
     }
@@ -248,11 +255,17 @@ int issue(pf2_server_s* s, Request* r) {
     const int depth = s->cfg.chunks_ahead > 0 ? s->cfg.chunks_ahead : 2;
     while ((int)r->inflight.size() < depth && r->issued < r->n_steps) {
         const int k = std::min(chunk, r->n_steps - r->issued);
-        for (int i = 0; i < k; ++i)
-            if (int rc = pf2_step_with_field(r->h, r->dfield)) {
+        for (int i = 0; i < k; ++i) {
+            int rc = pf2_step_with_field(r->h, r->dfield);
+            if (!rc && r->moving) {
+                const double t = (double)(r->issued + i + 1) * r->params.dt;
+                rc = pf2_move_mesh_scaled(r->h, r->amp * std::sin(r->omega * t + r->phase));
+            }
+            if (rc) {
                 mark_failed(s, r, rc, pf2_last_error());
                 return 1;
             }
+        }
         r->issued += k;
         hipEvent_t e = get_event(s);
         if (!e || hipEventRecord(e, r->stream) != hipSuccess) {
@@ -347,8 +360,8 @@ int pf2_server_destroy(pf2_server s) {
     return 0;
 }
 
-int pf2_server_submit(pf2_server s, const pf2_mesh* mesh, const pf2_params* params, const double* V, int n_steps,
-                      long long* request_id) {
+static int submit_impl(pf2_server_s* s, const pf2_mesh* mesh, const pf2_params* params, const double* V, int n_steps,
+                const double* W, double amp, double omega, double phase, long long* request_id) {
     if (!s || !mesh || !params || !V || !request_id) return sfail(PF2_E_ARG, "NULL argument");
     if (n_steps < 0) return sfail(PF2_E_ARG, "n_steps must be >= 0");
     if (mesh->dim != 2 && mesh->dim != 3) return sfail(PF2_E_ARG, "dim must be 2 or 3");
@@ -367,6 +380,14 @@ int pf2_server_submit(pf2_server s, const pf2_mesh* mesh, const pf2_params* para
     r->V.assign(V, V + (size_t)mesh->n_dofs * mesh->dim);
     r->params = *params;
     r->n_steps = n_steps;
+    if (W) {
+        r->moving = true;
+        r->params.keep_velocity_ext = 1;
+        r->W.assign(W, W + (size_t)mesh->n_vertices * mesh->dim);
+        r->amp = amp;
+        r->omega = omega;
+        r->phase = phase;
+    }
     r->bytes = footprint(r);
     if (r->bytes == 0) {
         delete r;
@@ -383,6 +404,18 @@ int pf2_server_submit(pf2_server s, const pf2_mesh* mesh, const pf2_params* para
     s->pending.push_back(r);
     *request_id = r->id;
     return 0;
+}
+
+int pf2_server_submit(pf2_server s, const pf2_mesh* mesh, const pf2_params* params, const double* V, int n_steps,
+                      long long* request_id) {
+    return submit_impl(s, mesh, params, V, n_steps, nullptr, 0, 0, 0, request_id);
+}
+
+int pf2_server_submit_moving(pf2_server s, const pf2_mesh* mesh, const pf2_params* params, const double* V,
+                             int n_steps, const double* W, double amplitude, double omega, double phase,
+                             long long* request_id) {
+    if (!W) return sfail(PF2_E_ARG, "W is NULL");
+    return submit_impl(s, mesh, params, V, n_steps, W, amplitude, omega, phase, request_id);
 }
 
 int pf2_server_poll(pf2_server s, int* n_pending, int* n_active) {

@@ -161,3 +161,72 @@ def test_footprint_estimate_bounds_handle(precision):
         got = st.memory_bytes()
         st.close()
         assert got <= est <= got + (2 << 20)
+
+
+def _moving_case():
+    m = M.cylinder_channel(0.06, seed=81)
+    V = M.cylinder_flow(m, 1.0, eps=0.02, seed=82)
+    d = np.hypot(m.vertices[:, 0] - 0.2, m.vertices[:, 1] - 0.2)
+    w = np.clip(1.0 - (d - 0.05) / 0.1, 0.0, 1.0)
+    W = np.zeros_like(m.vertices)
+    W[:, 1] = w
+    return m, V, W
+
+
+@pytest.mark.parametrize("precision", [FP64, FP32_MIXED])
+def test_prescribed_motion_equals_host_vertices(precision):
+    """pf2_move_mesh_scaled(a) = pf2_move_mesh(X0 + a W) bit for bit."""
+    m, V, W = _moving_case()
+    runs = []
+    for scaled in (False, True):
+        st = ParticleStage(m, 0.003, 6, 2, [3, 3], precision, keep_velocity_ext=True)
+        st.upload_fields(V, V)
+        st.seed_particles()
+        if scaled:
+            st.set_motion(W)
+        X0 = m.vertices.copy()
+        for k in range(1, 5):
+            st.step()
+            a = 0.004 * np.sin(0.9 * k)
+            if scaled:
+                st.move_mesh_scaled(a)
+            else:
+                st.move_mesh(X0 + a * W)
+            st.upload_fields(V, None)
+        p = st.particles()
+        runs.append((st.step(download=True), p["cell"], p["id"], p["xi"], p["v"]))
+        st.close()
+    for a, b in zip(*runs):
+        assert np.array_equal(a, b, equal_nan=True)
+
+
+def test_server_moving_request_alone_equals_mix_equals_direct():
+    m, V, W = _moving_case()
+    dt, S, MAX, q, steps = 0.003, 6, 2, [3, 3], 7
+    amp, omega, phase = 0.004, 150.0, 0.3
+    st = ParticleStage(m, dt, S, MAX, q, FP32_MIXED, keep_velocity_ext=True)
+    st.upload_fields(V, V)
+    st.seed_particles()
+    st.set_motion(W)
+    dV = _device_field(V)
+    for k in range(1, steps + 1):
+        st.step_with_field(dV.data_ptr())
+        st.move_mesh_scaled(amp * np.sin(omega * k * dt + phase))
+    ref_V, ref_n = st.download_field(), st.n_global_particles()
+    st.close()
+    res = []
+    for mix in (False, True):
+        srv = Server(hbm_budget=4 << 30, steps_per_chunk=3, use_graphs=True)
+        ids = []
+        if mix:
+            for mk, Vk, pk, sk in _requests()[:3]:
+                ids.append(srv.submit(mk, Vk, sk, *pk))
+        rid = srv.submit_moving(m, V, steps, dt, S, MAX, q, W, amp, omega, phase, precision=FP32_MIXED)
+        srv.drain()
+        st_ = srv.stats()
+        assert st_["failed"] == 0 and st_["finished"] == len(ids) + 1
+        res.append(srv.result(rid))
+        srv.close()
+    for Vr, nr, _ in res:
+        assert nr == ref_n
+        assert np.array_equal(Vr, ref_V, equal_nan=True)
