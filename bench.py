@@ -15,8 +15,9 @@ resident in HBM before the timed region; steps are replayed from HIP graphs.
 
 The default run also serves the request stream of the latency metric
 (256 independent simulations, Poisson arrivals, 2k-200k cells, 9/16/25
-particles per cell, 50-500 steps; admission by HBM footprint) and reports its
-p50/p99 completion latency under "serving".
+particles per cell, 50-500 steps, plus 8 moving-mesh simulations of ~400k
+cells; admission by HBM footprint) and reports its p50/p99 completion
+latency under "serving".
 
 Other workloads (--workload): channel (200x20 quads, FP64, with a host FEM
 stand-in between particle steps, against the CPU oracle over the whole run), pipe3d (823k hexahedra in a pipe, 2x2x2 lattice),
@@ -110,10 +111,12 @@ def moving_workload():
         return Vs
 
     # dt = 0.0005: the default workload's per-substep Courant number at half its cell size
-    return Workload("moving", m, V, [5, 5], 0.0005, 10, 2, keep_vext=True, motion=motion, field=field,
-                    desc="~400k graded quads around a cylinder oscillating vertically (amplitude 0.1 D, 2 Hz), "
-                         "flow blended into the wall velocity near it, 25 particles/cell, dt=0.0005, per step: "
-                         "correct+move+project, then the mesh move (vertex update, re-sort, reseed)")
+    wl = Workload("moving", m, V, [5, 5], 0.0005, 10, 2, keep_vext=True, motion=motion, field=field,
+                  desc="~400k graded quads around a cylinder oscillating vertically (amplitude 0.1 D, 2 Hz), "
+                       "flow blended into the wall velocity near it, 25 particles/cell, dt=0.0005, per step: "
+                       "correct+move+project, then the mesh move (vertex update, re-sort, reseed)")
+    wl.blend, wl.amp, wl.omega = w, A, omega     # X(t) = X0 + amp sin(omega t) (0, blend)
+    return wl
 
 
 def kernel_bytes(name, info, m, dim=2):
@@ -359,7 +362,10 @@ def roofline(res, m, traffic=True):
 
 def serve(args):
     """The request stream: independent simulations with Poisson arrivals,
-    served by pf2_server (FP32-mixed, graph replay, admission by footprint)."""
+    served by pf2_server (FP32-mixed, graph replay, admission by footprint).
+    Moving-mesh requests (~400k quads around a cylinder oscillating with a
+    seeded phase, 25 particles per cell) arrive within the first half of the
+    stream, so that the memory in use climbs toward the budget."""
     from pfem2_amd import FP32_MIXED, Server
     from pfem2_amd import mesh as M
     rng = np.random.default_rng(SEEDS["serve"])
@@ -374,9 +380,22 @@ def serve(args):
         m = M.channel_2d(nx, ny, 5.0, 1.0)
         V = M.poiseuille_2d(m, umax, 1.0, 0.01, seed=1000 + i)
         dt = 0.5 * min(5.0 / nx, 1.0 / ny) / umax
-        reqs.append((m, V, steps, dt, q))
-    arrivals = np.cumsum(rng.exponential(1.0 / args.serve_rate, len(reqs)))
-    print(f"# serving {len(reqs)} requests over ~{arrivals[-1]:.1f} s", file=sys.stderr, flush=True)
+        reqs.append({"m": m, "V": V, "steps": steps, "dt": dt, "q": q, "moving": None})
+    arrivals = list(np.cumsum(rng.exponential(1.0 / args.serve_rate, len(reqs))))
+    if args.serve_moving > 0:
+        W0 = moving_workload()
+        Wd = np.zeros_like(W0.m.vertices)
+        Wd[:, 1] = W0.blend
+        span = arrivals[-1]
+        for j in range(args.serve_moving):
+            reqs.append({"m": W0.m, "V": W0.V, "steps": int(rng.integers(40, 101)), "dt": W0.dt, "q": 5,
+                         "moving": (Wd, W0.amp, W0.omega, float(rng.uniform(0.0, 2.0 * math.pi)))})
+            arrivals.append(float(rng.uniform(0.0, 0.5 * span)))
+    order = np.argsort(arrivals, kind="stable")
+    reqs = [reqs[k] for k in order]
+    arrivals = [arrivals[k] for k in order]
+    print(f"# serving {len(reqs)} requests ({args.serve_moving} moving) over ~{arrivals[-1]:.1f} s", file=sys.stderr,
+          flush=True)
     srv = Server(hbm_budget=int(args.serve_budget_gib * (1 << 30)), use_graphs=True)
     ids = []
     t0 = time.perf_counter()
@@ -384,8 +403,13 @@ def serve(args):
     while True:
         now = time.perf_counter() - t0
         while i < len(reqs) and arrivals[i] <= now:
-            m, V, steps, dt, q = reqs[i]
-            ids.append(srv.submit(m, V, steps, dt, 10, 2, [q, q], FP32_MIXED))
+            r = reqs[i]
+            if r["moving"] is None:
+                ids.append(srv.submit(r["m"], r["V"], r["steps"], r["dt"], 10, 2, [r["q"], r["q"]], FP32_MIXED))
+            else:
+                Wd, amp, omega, phase = r["moving"]
+                ids.append(srv.submit_moving(r["m"], r["V"], r["steps"], r["dt"], 10, 2, [r["q"], r["q"]], Wd, amp,
+                                             omega, phase, FP32_MIXED))
             i += 1
         npend, nact = srv.poll()
         if i == len(reqs) and npend == 0 and nact == 0:
@@ -394,18 +418,27 @@ def serve(args):
             time.sleep(max(0.0, min(arrivals[i] - now, 0.001)))
     wall = time.perf_counter() - t0
     st = srv.stats()
-    lat = [srv.result(r)[2] for r in ids]
+    lat = np.array([srv.result(r)[2] for r in ids])
     srv.close()
-    cells = [r[0].n_cells for r in reqs]
-    return {"requests": len(reqs), "finished": int(st["finished"]), "failed": int(st["failed"]),
-            "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"], "mean_ms": st["mean_ms"],
-            "max_ms": float(max(lat)), "updates_per_s": st["updates_per_s"],
-            "particle_updates": st["particle_updates"], "wall_s": wall,
-            "peak_hbm_gib": st["peak_bytes"] / 2 ** 30, "max_active": int(st["max_active"]),
-            "arrival_rate_per_s": args.serve_rate, "budget_gib": args.serve_budget_gib,
-            "cells_mean": float(np.mean(cells)), "precision": "fp32-mixed",
-            "stream": f"{len(reqs)} requests, Poisson arrivals ({args.serve_rate}/s), cells log-uniform 2k-200k "
-                      "(channel quads), lattice 3/4/5, Umax U[0.2,2], 50-500 steps, 10 substeps, MAX=2"}
+    mov = np.array([r["moving"] is not None for r in reqs])
+    cells = [r["m"].n_cells for r in reqs]
+    out = {"requests": len(reqs), "moving_requests": int(mov.sum()), "finished": int(st["finished"]),
+           "failed": int(st["failed"]), "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"], "mean_ms": st["mean_ms"],
+           "max_ms": float(lat.max()), "updates_per_s": st["updates_per_s"],
+           "particle_updates": st["particle_updates"], "wall_s": wall,
+           "peak_hbm_gib": st["peak_bytes"] / 2 ** 30, "max_active": int(st["max_active"]),
+           "arrival_rate_per_s": args.serve_rate, "budget_gib": args.serve_budget_gib,
+           "cells_mean": float(np.mean(cells)), "precision": "fp32-mixed",
+           "stream": f"{args.serve_requests} static requests, Poisson arrivals ({args.serve_rate}/s), cells "
+                     "log-uniform 2k-200k (channel quads), lattice 3/4/5, Umax U[0.2,2], 50-500 steps; "
+                     f"{int(mov.sum())} moving-mesh requests (407k quads, 5x5 lattice, 40-100 steps, oscillating "
+                     "cylinder) arriving in the first half; 10 substeps, MAX=2"}
+    if mov.any():
+        out["static_p50_ms"] = float(np.percentile(lat[~mov], 50))
+        out["static_p99_ms"] = float(np.percentile(lat[~mov], 99))
+        out["moving_p50_ms"] = float(np.percentile(lat[mov], 50))
+        out["moving_p99_ms"] = float(np.percentile(lat[mov], 99))
+    return out
 
 
 def copy_calibration(torch, mib=512):
@@ -548,6 +581,7 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=256)
     ap.add_argument("--serve-rate", type=float, default=10.0, help="Poisson arrival rate, requests/s")
     ap.add_argument("--serve-budget-gib", type=float, default=200.0)
+    ap.add_argument("--serve-moving", type=int, default=8, help="moving-mesh requests mixed into the stream")
     ap.add_argument("--channel-steps", type=int, default=100)
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--only", choices=["fp64", "fp32"], default=None,
