@@ -127,10 +127,13 @@ struct DevWork {
     int* arr_off;       // per cell: exclusive scan of arr_cnt
     int* arr_fill;      // per cell: fill counter
     int* arr_list;      // arrivals grouped by destination cell
-    int* final_cnt;     // per cell (n_cells + 1 for the scan)
-    int* spawn_cnt;
-    int* new_off;       // n_cells + 1
-    int* spawn_off;     // n_cells + 1
+    // per cell (n_cells + 1 for the scan): final count << 32 | spawns, and
+    // its exclusive scan (particle offset << 32 | spawn offset): one 64-bit
+    // scan gives both offsets (the spawn total stays below 2^32, so the low
+    // halves never carry)
+    unsigned long long* fs_cnt;
+    unsigned long long* fs_off;
+    int* new_off;       // n_cells + 1: particle offsets, written from fs_off by the cell writes
     unsigned long long* hitmask;   // per cell: in-range parts that hold a particle
     int* nsurv;         // per cell: survivors
     double* partial;    // per cell: NV * (DIM + 1) projection sums
@@ -413,17 +416,54 @@ struct DevQueues {
     int* park_s;          // substep index of a parked particle
 };
 
+// Loops of wavefronts over n items (particles or queue entries): the
+// wavefront handles items base + lane for base = begin, begin + step, ...
+// below end, and appends to queue shard `shard`.  Shards follow the item
+// order, so a queue keeps the order of the particles (stored by cell) up to
+// the order within a shard, and the searches of one wavefront touch nearby
+// cells.
+struct WaveLoop {
+    int begin, end, step, shard;
+};
+
+// grid-stride: wavefront k takes chunks k, k + W, ... of WAVE items
+__device__ __forceinline__ WaveLoop stride_loop(int n) {
+    const long long wpb = blockDim.x / WAVE, nw = gridDim.x * wpb;
+    const long long k = blockIdx.x * wpb + threadIdx.x / WAVE;
+    WaveLoop L;
+    L.begin = (int)(k * WAVE);
+    L.end = n;
+    L.step = (int)(nw * WAVE);
+    L.shard = (int)(k * QSHARDS / nw);
+    return L;
+}
+
+// one contiguous range of whole workgroup-sized chunks per workgroup, the
+// wavefronts of a workgroup interleaved within it
+__device__ __forceinline__ WaveLoop range_loop(int n) {
+    const long long g = gridDim.x, b = blockIdx.x;
+    const long long per = ((n + g - 1) / g + blockDim.x - 1) / blockDim.x * blockDim.x;
+    const long long b0 = b * per < n ? b * per : n;
+    const long long wpb = blockDim.x / WAVE, wv = threadIdx.x / WAVE;
+    WaveLoop L;
+    L.begin = (int)(b0 + wv * WAVE);
+    L.end = (int)(b0 + per < n ? b0 + per : n);
+    L.step = blockDim.x;
+    L.shard = (int)((b * wpb + wv) * QSHARDS / (g * wpb));
+    return L;
+}
+
 __device__ __forceinline__ int* queue_count(const DevQueues& qu, int round, int shard) {
     return qu.count + ((size_t)round * (QSHARDS + 1) + shard) * QSTRIDE;
 }
 
-// append `value` for the lanes with pred set (all lanes of the wavefront call)
-__device__ __forceinline__ void queue_push(const DevQueues& qu, int round, bool pred, int2 value) {
+// append `value` to shard `sh` for the lanes with pred set (all lanes of the
+// wavefront call)
+__device__ __forceinline__ void queue_push(const DevQueues& qu, int round, bool pred, int2 value, int sh) {
     const unsigned long long mask = __ballot(pred);
     if (mask == 0) return;
     const int lane = threadIdx.x & (WAVE - 1);
     const int leader = __ffsll((long long)mask) - 1;
-    const int sh = wave_shard() & (QSHARDS - 1);
     int base = 0;
     if (lane == leader) base = atomicAdd(queue_count(qu, round, sh), __popcll(mask));
     base = __shfl(base, leader);
@@ -537,9 +577,9 @@ __device__ __forceinline__ void finish_particle(const DevParams& pp, const DevPa
 // park a particle for the search of the next round (all lanes call)
 template <int DIM>
 __device__ __forceinline__ void park_particle(const DevParticles& A, const DevWork& w, const DevQueues& qu,
-                                              int round_out, bool park, int i, int c, const double* x,
-                                              const double* xi, const double* vext, int s) {
-    queue_push(qu, round_out, park, make_int2(i, c));
+                                              int round_out, int shard, bool park, int i, int c,
+                                              const double* x, const double* xi, const double* vext, int s) {
+    queue_push(qu, round_out, park, make_int2(i, c), shard);
     if (!park) return;
     qu.park_s[i] = s;
 #pragma unroll
@@ -559,10 +599,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DIM ==
     constexpr int NV = Q1<DIM>::NV;
     const int n = w.counters[0];
     const int lane = threadIdx.x & (WAVE - 1);
-    const int stride = gridDim.x * blockDim.x;
-    for (int base = blockIdx.x * blockDim.x + (threadIdx.x - lane); base < n; base += stride) {
+    const WaveLoop L = stride_loop(n);
+    for (int base = L.begin; base < L.end; base += L.step) {
         const int i = base + lane;
-        const bool valid = i < n;
+        const bool valid = i < L.end;
         const int c = valid ? A.cell[i] : 0;
         double x[DIM], xi[DIM], vext[DIM];
 #pragma unroll
@@ -589,7 +629,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DIM ==
             load_cell<DIM>(m, c, P, Vc);
             st = advance64<DIM>(m, V, pp, A, i, c, x, xi, vext, s, P, Vc, m.ftol[c]);
         }
-        park_particle<DIM>(A, w, qu, 0, valid && st == 1, i, c, x, xi, vext, s);
+        park_particle<DIM>(A, w, qu, 0, L.shard, valid && st == 1, i, c, x, xi, vext, s);
         finish_particle<DIM>(pp, A, w, valid && st == 0, i, c, x, xi, true);
     }
 }
@@ -606,11 +646,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DIM ==
     const int n = queue_prefix(qu, round, s_pref);
     const bool last = round == MOVE_ROUNDS;
     const int lane = threadIdx.x & (WAVE - 1);
-    const int stride = gridDim.x * blockDim.x;
+    const WaveLoop L = range_loop(n);
     int lost = 0, searches = 0;
-    for (int base = blockIdx.x * blockDim.x + (threadIdx.x - lane); base < n; base += stride) {
+    for (int base = L.begin; base < L.end; base += L.step) {
         const int j = base + lane;
-        const bool valid = j < n;
+        const bool valid = j < L.end;
         const int2 e = valid ? queue_get(qu, round, s_pref, j) : make_int2(-1, -1);
         const int i = e.x;
         int c = e.y;
@@ -649,7 +689,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DIM ==
                 if (st == 0 || !last) break;
             }
         }
-        if (!last) park_particle<DIM>(A, w, qu, round + 1, valid && st == 1, i, c, x, xi, vext, s);
+        if (!last) park_particle<DIM>(A, w, qu, round + 1, L.shard, valid && st == 1, i, c, x, xi, vext, s);
         finish_particle<DIM>(pp, A, w, valid && st == 0, i, c, x, xi, true);
     }
     wave_stat(w.stats, 0, lost);
@@ -814,8 +854,7 @@ __global__ void __launch_bounds__(256) k_cell_count(int n_cells, const int* __re
             const int spawns = reseed ? __popcll(all & ~hit) : 0;
             w.hitmask[c] = reseed ? hit : all;
             w.nsurv[c] = nsurv;
-            w.final_cnt[c] = nsurv + spawns;
-            w.spawn_cnt[c] = spawns;
+            w.fs_cnt[c] = ((unsigned long long)(nsurv + spawns) << 32) | (unsigned)spawns;
             deleted += nvalid - nsurv;
         }
     }
@@ -851,8 +890,7 @@ __global__ void __launch_bounds__(256) k_cell_big(const int* __restrict__ old_of
                 atomicOr(&w.counters[2], 2);
                 w.hitmask[c] = all;
                 w.nsurv[c] = 0;
-                w.final_cnt[c] = 0;
-                w.spawn_cnt[c] = 0;
+                w.fs_cnt[c] = 0;
             }
             continue;
         }
@@ -922,8 +960,7 @@ __global__ void __launch_bounds__(256) k_cell_big(const int* __restrict__ old_of
             const int spawns = reseed ? __popcll(all & ~hm) : 0;
             w.hitmask[c] = reseed ? hm : all;
             w.nsurv[c] = s_nsurv;
-            w.final_cnt[c] = s_nsurv + spawns;
-            w.spawn_cnt[c] = spawns;
+            w.fs_cnt[c] = ((unsigned long long)(s_nsurv + spawns) << 32) | (unsigned)spawns;
             deleted += s_nvalid - s_nsurv;
         }
         __syncthreads();
@@ -944,9 +981,11 @@ __device__ __forceinline__ void cell_write_one(const DevMesh<DIM>& m, const doub
     constexpr int NV = Q1<DIM>::NV;
     const int n_st = old_off[c + 1] - old_off[c];
     const int k = n_st + w.arr_cnt[c];
-    const long long out0 = w.new_off[c];
+    const unsigned long long fo = w.fs_off[c];
+    const long long out0 = (long long)(fo >> 32);
     const int nsurv = w.nsurv[c];
-    const int nfinal = w.final_cnt[c];
+    const int nfinal = (int)(w.fs_cnt[c] >> 32);
+    if (gl == 0) w.new_off[c] = (int)out0;
     if (out0 + nfinal > B.cap) {
         if (gl == 0) atomicOr(&w.counters[2], 1);
         return;
@@ -974,7 +1013,7 @@ __device__ __forceinline__ void cell_write_one(const DevMesh<DIM>& m, const doub
     if (spawns > 0) {
         const unsigned long long all = pp.nparts_in >= 64 ? ~0ull : ((1ull << pp.nparts_in) - 1);
         const unsigned long long empty = all & ~w.hitmask[c];
-        const uint32_t id0 = *w.particle_counter + (uint32_t)w.spawn_off[c];
+        const uint32_t id0 = *w.particle_counter + (uint32_t)fo;
         for (int o = gl; o < pp.nparts_in; o += GS) {
             if (!((empty >> o) & 1)) continue;
             const int r = __popcll(empty & ((1ull << o) - 1));
@@ -1021,8 +1060,10 @@ __global__ void k_finalize(int n_cells, DevWork w, int* queue_count) {
         if (t == 0) queue_count[((size_t)r * (QSHARDS + 1) + QSHARDS) * QSTRIDE] = 0;
     }
     if (t != 0) return;
-    const int total = w.new_off[n_cells];
-    const int spawned = w.spawn_off[n_cells];
+    const unsigned long long fo = w.fs_off[n_cells];
+    const int total = (int)(fo >> 32);
+    const int spawned = (int)(uint32_t)fo;
+    w.new_off[n_cells] = total;
     w.stats[2] += (unsigned long long)spawned;
     *w.particle_counter += (uint32_t)spawned;
     w.counters[3] = w.counters[0];   // count before the step (diagnostics)

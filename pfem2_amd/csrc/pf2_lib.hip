@@ -128,7 +128,7 @@ struct pf2_handle_s {
     // work
     int *dest = nullptr, *pkey = nullptr, *spos = nullptr;
     int *arr_cnt = nullptr, *arr_off = nullptr, *arr_fill = nullptr, *arr_list = nullptr;
-    int *final_cnt = nullptr, *spawn_cnt = nullptr, *spawn_off = nullptr;
+    unsigned long long *fs_cnt = nullptr, *fs_off = nullptr;
     unsigned long long* hitmask = nullptr;
     int* nsurv = nullptr;
     double* partial = nullptr;
@@ -313,10 +313,9 @@ DevWork dev_work(pf2_handle_s* h) {
     w.arr_off = h->arr_off;
     w.arr_fill = h->arr_fill;
     w.arr_list = h->arr_list;
-    w.final_cnt = h->final_cnt;
-    w.spawn_cnt = h->spawn_cnt;
+    w.fs_cnt = h->fs_cnt;
+    w.fs_off = h->fs_off;
     w.new_off = h->off[1 - h->cur_off];
-    w.spawn_off = h->spawn_off;
     w.hitmask = h->hitmask;
     w.nsurv = h->nsurv;
     w.partial = h->partial;
@@ -411,7 +410,8 @@ void free_particles(pf2_handle_s* h) {
     h->park_vext = nullptr;
 }
 
-int scan(pf2_handle_s* h, const int* in, int* out, int n) {
+template <class T>
+int scan(pf2_handle_s* h, const T* in, T* out, int n) {
     Timed t(h, K_SCAN);
     size_t bytes = h->scan_tmp_bytes;
     HIPC(hipcub::DeviceScan::ExclusiveSum(h->scan_tmp, bytes, in, out, n, h->stream));
@@ -457,8 +457,7 @@ int regroup(pf2_handle_s* h, int reseed, bool fuse_sums = false) {
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cell_big<DIM>), dim3(1024), dim3(256), 0, h->stream, old_off,
                            h->buf[h->cur].id, pp, w, reseed);
     }
-    if (int r = scan(h, h->final_cnt, w.new_off, nc + 1)) return r;
-    if (int r = scan(h, h->spawn_cnt, h->spawn_off, nc + 1)) return r;
+    if (int r = scan(h, h->fs_cnt, h->fs_off, nc + 1)) return r;
     if (mixed(h)) {
         Timed t(h, K_CELL_WRITE32);
         launch_cell_write32<DIM>(h->stream, cell_grid(nc), dev_mesh<DIM>(h), h->dV, old_off, h->buf[h->cur].id,
@@ -686,7 +685,7 @@ size_t estimate_bytes(int dim, long long nc, long long nvt, long long nd, long l
     b += nvt * dim * 8 + nck * 4 * 2 + (nvt + 1) * 4 + nck * 4 + nck * dim * 8;       // X, cv, cd, star
     b += (nd + 1) * 4 + nck * 4 + nc * 8 + nck * dim * 8 + nck * 8 + nck * dim * 8 * 2;  // dof map, ftol, cellX/star/V/dV
     b += nd * dim * 8 * 3;                                                              // V, V_old, staging
-    b += (nc + 1) * 4 * 9 + nc * 8 + nc * 4 + nc * nv * (dim + 1) * 8 + 16;             // per-cell work, partials
+    b += (nc + 1) * 4 * 10 + nc * 8 + nc * 4 + nc * nv * (dim + 1) * 8 + 16;            // per-cell work, partials
     b += (size_t)(MOVE_ROUNDS + 2) * (QSHARDS + 1) * QSTRIDE * 4 + (size_t)(MOVE_ROUNDS + 2) * (QSHARDS + 2) * 4;
     b += 4 + (size_t)STAT_SHARDS * NSTAT * 8;
     b += nck * dim * 4 * 2 + nc * geo32_floats(dim) * 4;                                 // FP32 fields, geometry
@@ -870,8 +869,7 @@ int pf2_create_in(const pf2_mesh* mesh, const pf2_params* params, long long capa
         dalloc(&h->dcellV, nck * dim) || dalloc(&h->dcellDV, nck * dim) || dalloc(&h->dV, (size_t)nd * dim) ||
         dalloc(&h->dVold, (size_t)nd * dim) || dalloc(&h->dtmp, (size_t)nd * dim) || dalloc(&h->off[0], nc + 1) ||
         dalloc(&h->off[1], nc + 1) || dalloc(&h->arr_cnt, nc + 1) || dalloc(&h->arr_off, nc + 1) ||
-        dalloc(&h->arr_fill, nc + 1) || dalloc(&h->final_cnt, nc + 1) || dalloc(&h->spawn_cnt, nc + 1) ||
-        dalloc(&h->spawn_off, nc + 1) || dalloc(&h->hitmask, nc) || dalloc(&h->nsurv, nc) ||
+        dalloc(&h->arr_fill, nc + 1) || dalloc(&h->fs_cnt, nc + 1) || dalloc(&h->fs_off, nc + 1) || dalloc(&h->hitmask, nc) || dalloc(&h->nsurv, nc) ||
         dalloc(&h->partial, (size_t)nc * h->nv * (dim + 1)) || dalloc(&h->counters, 4) || dalloc(&h->big, nc + 1) ||
         dalloc(&h->qcount, (size_t)(MOVE_ROUNDS + 2) * (QSHARDS + 1) * QSTRIDE) ||
         dalloc(&h->qprefix, (size_t)(MOVE_ROUNDS + 2) * (QSHARDS + 2)) ||
@@ -903,17 +901,18 @@ int pf2_create_in(const pf2_mesh* mesh, const pf2_params* params, long long capa
         hipMemsetAsync(h->off[1], 0, sizeof(int) * (nc + 1), h->stream) ||
         hipMemsetAsync(h->arr_cnt, 0, sizeof(int) * (nc + 1), h->stream) ||
         hipMemsetAsync(h->arr_fill, 0, sizeof(int) * (nc + 1), h->stream) ||
-        hipMemsetAsync(h->final_cnt, 0, sizeof(int) * (nc + 1), h->stream) ||
-        hipMemsetAsync(h->spawn_cnt, 0, sizeof(int) * (nc + 1), h->stream) ||
+        hipMemsetAsync(h->fs_cnt, 0, sizeof(unsigned long long) * (nc + 1), h->stream) ||
         hipMemsetAsync(h->counters, 0, sizeof(int) * 4, h->stream) ||
         hipMemsetAsync(h->big, 0, sizeof(int) * (nc + 1), h->stream) ||
         hipMemsetAsync(h->qcount, 0, sizeof(int) * (MOVE_ROUNDS + 2) * (QSHARDS + 1) * QSTRIDE, h->stream) ||
         hipMemsetAsync(h->pcounter, 0, sizeof(uint32_t), h->stream) ||
         hipMemsetAsync(h->stats, 0, sizeof(unsigned long long) * STAT_SHARDS * NSTAT, h->stream))
         return bail(fail(PF2_E_HIP, "initialisation failed"));
-    if (hipcub::DeviceScan::ExclusiveSum(nullptr, h->scan_tmp_bytes, h->final_cnt, h->off[0], nc + 1, h->stream) !=
-        hipSuccess)
+    size_t tb32 = 0, tb64 = 0;
+    if (hipcub::DeviceScan::ExclusiveSum(nullptr, tb32, h->arr_cnt, h->arr_off, nc + 1, h->stream) != hipSuccess ||
+        hipcub::DeviceScan::ExclusiveSum(nullptr, tb64, h->fs_cnt, h->fs_off, nc + 1, h->stream) != hipSuccess)
         return bail(fail(PF2_E_HIP, "scan sizing failed"));
+    h->scan_tmp_bytes = tb32 > tb64 ? tb32 : tb64;
     if (dalloc((char**)&h->scan_tmp, h->scan_tmp_bytes)) return bail(PF2_E_HIP);
     if (h->dim == 2) launch_geo<2>(h->stream, dev_mesh<2>(h), h->dgeo32, h->dftol, h->dcellX);
     else launch_geo<3>(h->stream, dev_mesh<3>(h), h->dgeo32, h->dftol, h->dcellX);
@@ -932,8 +931,8 @@ int pf2_destroy(pf2_handle h) {
     void* ptrs[] = {h->dX, h->dcv, h->dcd, h->dstar_off, h->dstar_cell, h->dstar_dir, h->ddof_off, h->ddof_cell,
                     h->dftol, h->dcellX, h->dcell_star, h->dcellV, h->dcellDV, h->dV, h->dVold, h->dtmp, h->dV32,
                     h->dX0, h->dW,
-                    h->dVold32, h->dgeo32, h->off[0], h->off[1], h->arr_cnt, h->arr_off, h->arr_fill, h->final_cnt,
-                    h->spawn_cnt, h->spawn_off, h->hitmask, h->nsurv, h->partial, h->counters, h->big, h->qcount, h->qprefix,
+                    h->dVold32, h->dgeo32, h->off[0], h->off[1], h->arr_cnt, h->arr_off, h->arr_fill, h->fs_cnt,
+                    h->fs_off, h->hitmask, h->nsurv, h->partial, h->counters, h->big, h->qcount, h->qprefix,
                     h->pcounter, h->stats, h->scan_tmp};
     if (!h->in_block)
         for (void* p : ptrs)

@@ -365,9 +365,9 @@ __device__ __forceinline__ void finish32(const DevParams& pp, const int* __restr
 
 template <int DIM>
 __device__ __forceinline__ void park32(const Rec32& R, long long cap, const DevWork& w, const DevQueues& qu,
-                                       int round_out, bool park, int i, int c, const float* xi, const float* u,
-                                       int s) {
-    queue_push(qu, round_out, park, make_int2(i, c));
+                                       int round_out, int shard, bool park, int i, int c, const float* xi,
+                                       const float* u, int s) {
+    queue_push(qu, round_out, park, make_int2(i, c), shard);
     if (!park) return;
     qu.park_s[i] = s;
     rec_store_xi<DIM>(R, i, xi);
@@ -386,9 +386,10 @@ __global__ void __launch_bounds__(256) k_move32(DevMesh<DIM> m, const float* __r
     const int n = w.counters[0];
     const float h32 = (float)pp.h;
     const int lane = threadIdx.x & (WAVE - 1);
-    for (int base = blockIdx.x * blockDim.x + (threadIdx.x - lane); base < n; base += gridDim.x * blockDim.x) {
+    const WaveLoop L = stride_loop(n);
+    for (int base = L.begin; base < L.end; base += L.step) {
         const int i = base + lane;
-        const bool valid = i < n;
+        const bool valid = i < L.end;
         const int c = valid ? pcell[i] : 0;
         float xi[DIM], v[DIM];
 #pragma unroll
@@ -411,7 +412,7 @@ __global__ void __launch_bounds__(256) k_move32(DevMesh<DIM> m, const float* __r
             load_cell32<DIM>(geo, V32, c, g, Vc);
             st = advance32<DIM>(pp, h32, i, pvext, cap, xi, u, s, g, Vc);
         }
-        park32<DIM>(R, cap, w, qu, 0, valid && st == 1, i, c, xi, u, s);
+        park32<DIM>(R, cap, w, qu, 0, L.shard, valid && st == 1, i, c, xi, u, s);
         finish32<DIM>(pp, pcell, R, w, valid && st == 0, i, c, xi);
     }
 }
@@ -431,10 +432,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DIM ==
     const bool last = round == MOVE_ROUNDS;
     const float h32 = (float)pp.h;
     const int lane = threadIdx.x & (WAVE - 1);
+    const WaveLoop L = range_loop(n);
     int lost = 0, searches = 0;
-    for (int base = blockIdx.x * blockDim.x + (threadIdx.x - lane); base < n; base += gridDim.x * blockDim.x) {
+    for (int base = L.begin; base < L.end; base += L.step) {
         const int j = base + lane;
-        const bool valid = j < n;
+        const bool valid = j < L.end;
         const int2 e = valid ? queue_get(qu, round, s_pref, j) : make_int2(-1, -1);
         const int i = e.x;
         int c = e.y;
@@ -506,7 +508,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DIM ==
                 }
             }
         }
-        if (!last) park32<DIM>(R, cap, w, qu, round + 1, valid && st == 1, i, c, xi, u, s);
+        if (!last) park32<DIM>(R, cap, w, qu, round + 1, L.shard, valid && st == 1, i, c, xi, u, s);
         finish32<DIM>(pp, pcell, R, w, valid && st == 0, i, c, xi);
     }
     wave_stat(w.stats, 0, lost);
@@ -580,9 +582,11 @@ __device__ __forceinline__ void cell_write32_one(const DevMesh<DIM>& m, const do
     constexpr int NV = Q1<DIM>::NV;
     const int n_st = old_off[c + 1] - old_off[c];
     const int k = n_st + w.arr_cnt[c];
-    const long long out0 = w.new_off[c];
+    const unsigned long long fo = w.fs_off[c];
+    const long long out0 = (long long)(fo >> 32);
     const int nsurv = w.nsurv[c];
-    const int nfinal = w.final_cnt[c];
+    const int nfinal = (int)(w.fs_cnt[c] >> 32);
+    if (gl == 0) w.new_off[c] = (int)out0;
     if (out0 + nfinal > cap) {
         if (gl == 0) atomicOr(&w.counters[2], 1);
         return;
@@ -611,7 +615,7 @@ __device__ __forceinline__ void cell_write32_one(const DevMesh<DIM>& m, const do
     if (spawns > 0) {
         const unsigned long long all = pp.nparts_in >= 64 ? ~0ull : ((1ull << pp.nparts_in) - 1);
         const unsigned long long empty = all & ~w.hitmask[c];
-        const uint32_t id0 = *w.particle_counter + (uint32_t)w.spawn_off[c];
+        const uint32_t id0 = *w.particle_counter + (uint32_t)fo;
         for (int o = gl; o < pp.nparts_in; o += GS) {
             if (!((empty >> o) & 1)) continue;
             const int r = __popcll(empty & ((1ull << o) - 1));
