@@ -82,6 +82,44 @@ __device__ __forceinline__ void wave_stat(unsigned long long* stats, int k, int 
     if ((threadIdx.x & (WAVE - 1)) == 0 && t) stat_add(stats, k, (unsigned long long)t);
 }
 
+// Arrivals (particles that changed cell) of the lanes with pred set: particle
+// i takes a slot of cell c's row, so the census finds a cell's arrivals with
+// no grouping pass; arrivals past the row go to the cell's overflow list (the
+// census then handles the cell in k_cell_big).  Lanes with the same cell
+// share one atomic, and the atomics of all cells of the wavefront are issued
+// together, so the wavefront waits for one returning atomic.  All lanes of
+// the wavefront call.
+template <class W>
+__device__ __forceinline__ void add_arrival(const W& w, bool pred, int c, int i) {
+    const int lane = threadIdx.x & (WAVE - 1);
+    unsigned long long todo = __ballot(pred);
+    if (todo == 0) return;
+    int leader = lane, off = 0, cnt = 0;
+    while (todo) {
+        const int l = __ffsll((long long)todo) - 1;
+        const int key = __shfl(c, l);
+        const unsigned long long same = __ballot(pred && c == key) & todo;
+        if ((same >> lane) & 1) {
+            leader = l;
+            off = __popcll(same & ((1ull << lane) - 1));
+        }
+        if (lane == l) cnt = __popcll(same);
+        todo &= ~same;
+    }
+    int base = 0;
+    if (cnt) base = atomicAdd(&w.arr_cnt[c], cnt);
+    base = __shfl(base, leader);
+    if (!pred) return;
+    const int slot = base + off;
+    if (slot < w.aslots) {
+        w.arr_slot[(size_t)c * w.aslots + slot] = i;
+        return;
+    }
+    const int e = atomicAdd(&w.counters[4], 1);
+    const int next = atomicExch(&w.ovf_head[c], e);
+    w.ovf[e] = make_int2(i, next);
+}
+
 template <int DIM>
 struct DevMesh {
     int n_cells, n_vertices, n_dofs;
@@ -124,9 +162,10 @@ struct DevWork {
     int* pkey;          // per particle: part key of the final position
     int* spos;          // per particle: position among the cell's survivors, -1 = dropped
     int* arr_cnt;       // per cell: arrivals
-    int* arr_off;       // per cell: exclusive scan of arr_cnt
-    int* arr_fill;      // per cell: fill counter
-    int* arr_list;      // arrivals grouped by destination cell
+    int* arr_slot;      // per cell: aslots arrival slots (particle indices)
+    int aslots;
+    int2* ovf;          // arrivals past a cell's slots: (particle, next entry) lists
+    int* ovf_head;      // per cell: first overflow entry or -1
     // per cell (n_cells + 1 for the scan): final count << 32 | spawns, and
     // its exclusive scan (particle offset << 32 | spawn offset): one 64-bit
     // scan gives both offsets (the spawn total stays below 2^32, so the low
@@ -137,7 +176,8 @@ struct DevWork {
     unsigned long long* hitmask;   // per cell: in-range parts that hold a particle
     int* nsurv;         // per cell: survivors
     double* partial;    // per cell: NV * (DIM + 1) projection sums
-    int* counters;      // [0] n_particles, [2] overflow flags, [3] count before the last regroup
+    int* counters;      // [0] n_particles, [1] arrivals of the last regroup, [2] overflow flags,
+                        // [3] count before the last regroup, [4] overflow entries in use
     int* big;           // [0] number of cells listed for k_cell_big, then the cells
     uint32_t* particle_counter;
     unsigned long long* stats;   // STAT_SHARDS rows: lost, deleted, spawned, searches, zero-weight
@@ -560,9 +600,8 @@ template <int DIM>
 __device__ __forceinline__ void finish_particle(const DevParams& pp, const DevParticles& A, const DevWork& w,
                                                 bool valid, int i, int c, const double* x, const double* xi,
                                                 bool write_state) {
+    add_arrival(w, valid && c >= 0 && c != A.cell[valid ? i : 0], c, i);
     if (!valid) return;
-    // arrival count: one non-returning atomic per lane (addresses are spread over cells)
-    if (c >= 0 && c != A.cell[i]) atomicAdd(&w.arr_cnt[c], 1);
     if (write_state) {
 #pragma unroll
         for (int d = 0; d < DIM; ++d) {
@@ -696,22 +735,22 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DIM ==
     wave_stat(w.stats, 3, searches);
 }
 
-// particles that changed cell, grouped by destination cell
-__global__ void k_group(const int* __restrict__ pcell, DevWork w) {
-    const int n = w.counters[0];
-    const int lane = threadIdx.x & (WAVE - 1);
-    for (int base = blockIdx.x * blockDim.x + (threadIdx.x - lane); base < n; base += gridDim.x * blockDim.x) {
-        const int i = base + lane;
-        const int c = i < n ? w.dest[i] : -1;
-        const bool arrive = c >= 0 && c != pcell[i];
-        const int slot = wave_inc_by_key(w.arr_fill, arrive, c);
-        if (arrive) w.arr_list[w.arr_off[c] + slot] = i;
-    }
+// candidate j of cell c: stayers of the old segment first, then the
+// arrivals in the cell's slots (j < n_st + min(arrivals, aslots))
+__device__ __forceinline__ int candidate(const DevWork& w, const int* old_off, int c, int n_st, int j) {
+    return j < n_st ? old_off[c] + j : w.arr_slot[(size_t)c * w.aslots + (j - n_st)];
 }
 
-// candidate j of cell c: stayers of the old segment first, then arrivals
-__device__ __forceinline__ int candidate(const DevWork& w, const int* old_off, int c, int n_st, int j) {
-    return j < n_st ? old_off[c] + j : w.arr_list[w.arr_off[c] + (j - n_st)];
+// candidates beyond the slots: the cell's overflow list, entry t handled by
+// lane t % stride
+template <class F>
+__device__ __forceinline__ void for_overflow(const DevWork& w, int c, int lane, int stride, F f) {
+    int t = 0;
+    for (int e = w.ovf_head[c]; e >= 0; ++t) {
+        const int2 v = w.ovf[e];
+        if (t % stride == lane) f(t, v.x);
+        e = v.y;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -773,13 +812,15 @@ __global__ void __launch_bounds__(256) k_cell_count(int n_cells, const int* __re
     __shared__ __align__(16) uint32_t s_sid[GROUPS][CAP + 4];
     const int g = threadIdx.x / GS, gl = threadIdx.x % GS;
     const unsigned long long all = pp.nparts_in >= 64 ? ~0ull : ((1ull << pp.nparts_in) - 1);
-    int deleted = 0;
+    int deleted = 0, arrivals = 0;
     for (int cb = blockIdx.x * GROUPS; cb < n_cells; cb += gridDim.x * GROUPS) {
         const int c = cb + g;
         const bool cv = c < n_cells;
         const int n_st = cv ? old_off[c + 1] - old_off[c] : 0;
-        const int k = cv ? n_st + w.arr_cnt[c] : 0;
-        const bool small = k <= CAP;
+        const int na = cv ? w.arr_cnt[c] : 0;
+        const int k = n_st + na;
+        const bool small = k <= CAP && na <= w.aslots;
+        if (cv && gl == 0) arrivals += na;
         const int kr = (k + 3) & ~3;
         unsigned long long hit = 0;
         int nvalid = 0, nsurv = 0;
@@ -860,6 +901,7 @@ __global__ void __launch_bounds__(256) k_cell_count(int n_cells, const int* __re
     }
     deleted = wave_sum(deleted);
     if ((threadIdx.x & (WAVE - 1)) == 0 && deleted) stat_add(w.stats, 1, (unsigned long long)deleted);
+    wave_stat(w.stats, 5, arrivals);
 }
 
 // The census of k_cell_count for the cells it listed (more than CAP
@@ -883,9 +925,12 @@ __global__ void __launch_bounds__(256) k_cell_big(const int* __restrict__ old_of
     for (int b = blockIdx.x; b < nbig; b += gridDim.x) {
         const int c = w.big[1 + b];
         const int n_st = old_off[c + 1] - old_off[c];
-        const int k = n_st + w.arr_cnt[c];
+        const int na = w.arr_cnt[c];
+        const int k = n_st + na;
+        const int kin = n_st + (na < w.aslots ? na : w.aslots);   // candidates in the old segment and slots
         if (k > BIGCAP) {
-            for (int j = tid; j < k; j += blockDim.x) w.spos[candidate(w, old_off, c, n_st, j)] = -1;
+            for (int j = tid; j < kin; j += blockDim.x) w.spos[candidate(w, old_off, c, n_st, j)] = -1;
+            for_overflow(w, c, tid, blockDim.x, [&](int, int idx) { w.spos[idx] = -1; });
             if (tid == 0) {
                 atomicOr(&w.counters[2], 2);
                 w.hitmask[c] = all;
@@ -901,6 +946,7 @@ __global__ void __launch_bounds__(256) k_cell_big(const int* __restrict__ old_of
             s_nsurv = 0;
         }
         for (int j = tid; j < kr; j += blockDim.x) {
+            if (j >= kin && j < k) continue;   // overflow entries, staged below
             int idx = -1;
             unsigned long long kid = ~0ull;
             if (j < k) {
@@ -914,6 +960,11 @@ __global__ void __launch_bounds__(256) k_cell_big(const int* __restrict__ old_of
             s_kid[j] = kid;
             s_sid[j] = ~0u;
         }
+        for_overflow(w, c, tid, blockDim.x, [&](int t, int cidx) {
+            s_idx[kin + t] = cidx;
+            s_kid[kin + t] = ((unsigned long long)(uint32_t)w.pkey[cidx] << 32) | pid[cidx];
+            s_sid[kin + t] = ~0u;
+        });
         __syncthreads();
         unsigned long long hit = 0;
         int nvalid = 0, nsurv = 0;
@@ -980,7 +1031,8 @@ __device__ __forceinline__ void cell_write_one(const DevMesh<DIM>& m, const doub
                                                int gl) {
     constexpr int NV = Q1<DIM>::NV;
     const int n_st = old_off[c + 1] - old_off[c];
-    const int k = n_st + w.arr_cnt[c];
+    const int na = w.arr_cnt[c];
+    const int kin = n_st + (na < w.aslots ? na : w.aslots);
     const unsigned long long fo = w.fs_off[c];
     const long long out0 = (long long)(fo >> 32);
     const int nsurv = w.nsurv[c];
@@ -990,12 +1042,9 @@ __device__ __forceinline__ void cell_write_one(const DevMesh<DIM>& m, const doub
         if (gl == 0) atomicOr(&w.counters[2], 1);
         return;
     }
-    for (int j = gl; j < k; j += GS) {
-        const int idx = candidate(w, old_off, c, n_st, j);
-        // arrivals are in this cell's list because their destination is c
-        if (j < n_st && w.dest[idx] != c) continue;
+    auto put = [&](int idx) {
         const int pos = w.spos[idx];
-        if (pos < 0) continue;
+        if (pos < 0) return;
         const long long o = out0 + pos;
         B.cell[o] = c;
         B.id[o] = A.id[idx];
@@ -1008,7 +1057,14 @@ __device__ __forceinline__ void cell_write_one(const DevMesh<DIM>& m, const doub
         if (pp.keep_vext)
 #pragma unroll
             for (int d = 0; d < DIM; ++d) B.vext[(size_t)d * B.cap + o] = A.vext[(size_t)d * A.cap + idx];
+    };
+    for (int j = gl; j < kin; j += GS) {
+        const int idx = candidate(w, old_off, c, n_st, j);
+        // arrivals are in this cell's slots because their destination is c
+        if (j < n_st && w.dest[idx] != c) continue;
+        put(idx);
     }
+    for_overflow(w, c, gl, GS, [&](int, int idx) { put(idx); });
     const int spawns = nfinal - nsurv;
     if (spawns > 0) {
         const unsigned long long all = pp.nparts_in >= 64 ? ~0ull : ((1ull << pp.nparts_in) - 1);
@@ -1038,7 +1094,7 @@ __device__ __forceinline__ void cell_write_one(const DevMesh<DIM>& m, const doub
     }
     if (gl == 0) {
         w.arr_cnt[c] = 0;
-        w.arr_fill[c] = 0;
+        w.ovf_head[c] = -1;
     }
 }
 
@@ -1054,12 +1110,22 @@ __global__ void __launch_bounds__(256) k_cell_write(DevMesh<DIM> m, const double
 // totals after the scans: particle count, id counter, statistics; resets the
 // queue counters for the next step (launched with QSHARDS threads)
 __global__ void k_finalize(int n_cells, DevWork w, int* queue_count) {
+    static_assert(QSHARDS == STAT_SHARDS, "one thread per stat row");
+    __shared__ int s_arr;
     const int t = threadIdx.x;
     for (int r = 0; r <= MOVE_ROUNDS + 1; ++r) {
         queue_count[((size_t)r * (QSHARDS + 1) + t) * QSTRIDE] = 0;
         if (t == 0) queue_count[((size_t)r * (QSHARDS + 1) + QSHARDS) * QSTRIDE] = 0;
     }
+    // arrivals of this regroup (diagnostics): the stat rows' slot 5, cleared
+    if (t == 0) s_arr = 0;
+    __syncthreads();
+    const int a = (int)w.stats[(size_t)t * NSTAT + 5];
+    w.stats[(size_t)t * NSTAT + 5] = 0;
+    if (a) atomicAdd(&s_arr, a);
+    __syncthreads();
     if (t != 0) return;
+    w.counters[1] = s_arr;
     const unsigned long long fo = w.fs_off[n_cells];
     const int total = (int)(fo >> 32);
     const int spawned = (int)(uint32_t)fo;
@@ -1068,6 +1134,7 @@ __global__ void k_finalize(int n_cells, DevWork w, int* queue_count) {
     *w.particle_counter += (uint32_t)spawned;
     w.counters[3] = w.counters[0];   // count before the step (diagnostics)
     w.counters[0] = total;
+    w.counters[4] = 0;
     w.big[0] = 0;
 }
 
@@ -1363,7 +1430,7 @@ __global__ void __launch_bounds__(256) k_resort(DevMesh<DIM> m, DevParticles A, 
         w.dest[i] = c;
         if (c >= 0) w.pkey[i] = part_key<DIM>(pp, xi);
         }
-        if (valid && c >= 0 && c != c0) atomicAdd(&w.arr_cnt[c], 1);
+        add_arrival(w, valid && c >= 0 && c != c0, c, i);
     }
     wave_stat(w.stats, 0, lost);
     wave_stat(w.stats, 3, searches);

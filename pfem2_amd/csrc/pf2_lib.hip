@@ -127,7 +127,9 @@ struct pf2_handle_s {
     int cur_off = 0;
     // work
     int *dest = nullptr, *pkey = nullptr, *spos = nullptr;
-    int *arr_cnt = nullptr, *arr_off = nullptr, *arr_fill = nullptr, *arr_list = nullptr;
+    int *arr_cnt = nullptr, *arr_slot = nullptr, *ovf_head = nullptr;
+    int2* ovf = nullptr;
+    int aslots = 0;
     unsigned long long *fs_cnt = nullptr, *fs_off = nullptr;
     unsigned long long* hitmask = nullptr;
     int* nsurv = nullptr;
@@ -310,9 +312,10 @@ DevWork dev_work(pf2_handle_s* h) {
     w.pkey = h->pkey;
     w.spos = h->spos;
     w.arr_cnt = h->arr_cnt;
-    w.arr_off = h->arr_off;
-    w.arr_fill = h->arr_fill;
-    w.arr_list = h->arr_list;
+    w.arr_slot = h->arr_slot;
+    w.aslots = h->aslots;
+    w.ovf = h->ovf;
+    w.ovf_head = h->ovf_head;
     w.fs_cnt = h->fs_cnt;
     w.fs_off = h->fs_off;
     w.new_off = h->off[1 - h->cur_off];
@@ -377,6 +380,14 @@ long long default_capacity(int dim, int n_cells, const pf2_params* p) {
     return per * (long long)n_cells;
 }
 
+// arrival slots per cell: the census row capacity of the lattice, so that a
+// cell with more arrivals is one the census hands to k_cell_big anyway
+int arrival_slots(int dim, const pf2_params* p) {
+    long long parts = 1;
+    for (int d = 0; d < dim; ++d) parts *= p->quantities[d];
+    return parts <= 16 ? GCAP : GCAP_L;
+}
+
 int alloc_particles(pf2_handle_s* h) {
     for (int b = 0; b < 2; ++b) {
         PBufs& pb = h->buf[b];
@@ -389,7 +400,7 @@ int alloc_particles(pf2_handle_s* h) {
     h->shard_cap = 2 * ((h->cap + QSHARDS - 1) / QSHARDS) + 64;
     const long long qlen = QSHARDS * h->shard_cap + h->cap;
     if (dalloc(&h->dest, h->cap) || dalloc(&h->pkey, h->cap) || dalloc(&h->spos, h->cap) ||
-        dalloc(&h->arr_list, h->cap) || dalloc(&h->queue[0], qlen) || dalloc(&h->queue[1], qlen) ||
+        dalloc(&h->ovf, h->cap) || dalloc(&h->queue[0], qlen) || dalloc(&h->queue[1], qlen) ||
         dalloc(&h->park_s, h->cap) || dalloc(&h->park_vext, h->cap * h->dim))
         return PF2_E_HIP;
     return 0;
@@ -403,9 +414,10 @@ void free_particles(pf2_handle_s* h) {
         hipFree(pb.s32a); hipFree(pb.s32b);
         pb = PBufs();
     }
-    hipFree(h->dest); hipFree(h->pkey); hipFree(h->spos); hipFree(h->arr_list);
+    hipFree(h->dest); hipFree(h->pkey); hipFree(h->spos); hipFree(h->ovf);
     hipFree(h->queue[0]); hipFree(h->queue[1]); hipFree(h->park_s); hipFree(h->park_vext);
-    h->dest = h->pkey = h->spos = h->arr_list = h->park_s = nullptr;
+    h->dest = h->pkey = h->spos = h->park_s = nullptr;
+    h->ovf = nullptr;
     h->queue[0] = h->queue[1] = nullptr;
     h->park_vext = nullptr;
 }
@@ -437,12 +449,6 @@ int regroup(pf2_handle_s* h, int reseed, bool fuse_sums = false) {
     DevParams pp = dev_params(h);
     DevWork w = dev_work(h);
     const int* old_off = h->off[h->cur_off];
-    if (int r = scan(h, h->arr_cnt, h->arr_off, nc + 1)) return r;
-    {
-        Timed t(h, K_GROUP);
-        hipLaunchKernelGGL(k_group, dim3(grid_for(h->n_est, 256, 4096)), dim3(256), 0, h->stream,
-                           h->buf[h->cur].cell, w);
-    }
     {
         Timed t(h, K_CELL_COUNT);
         if (pp.nparts_in <= 16)
@@ -685,7 +691,8 @@ size_t estimate_bytes(int dim, long long nc, long long nvt, long long nd, long l
     b += nvt * dim * 8 + nck * 4 * 2 + (nvt + 1) * 4 + nck * 4 + nck * dim * 8;       // X, cv, cd, star
     b += (nd + 1) * 4 + nck * 4 + nc * 8 + nck * dim * 8 + nck * 8 + nck * dim * 8 * 2;  // dof map, ftol, cellX/star/V/dV
     b += nd * dim * 8 * 3;                                                              // V, V_old, staging
-    b += (nc + 1) * 4 * 10 + nc * 8 + nc * 4 + nc * nv * (dim + 1) * 8 + 16;            // per-cell work, partials
+    b += (nc + 1) * 4 * 9 + nc * 4 * (1 + arrival_slots(dim, p));                      // per-cell work, arrival slots
+    b += nc * 8 + nc * 4 + nc * nv * (dim + 1) * 8 + 32;                                 // census, partials, counters
     b += (size_t)(MOVE_ROUNDS + 2) * (QSHARDS + 1) * QSTRIDE * 4 + (size_t)(MOVE_ROUNDS + 2) * (QSHARDS + 2) * 4;
     b += 4 + (size_t)STAT_SHARDS * NSTAT * 8;
     b += nck * dim * 4 * 2 + nc * geo32_floats(dim) * 4;                                 // FP32 fields, geometry
@@ -698,7 +705,7 @@ size_t estimate_bytes(int dim, long long nc, long long nvt, long long nd, long l
     b += 2 * cap * per;                                                                 // two particle buffers
     const long long shard_cap = 2 * ((cap + QSHARDS - 1) / QSHARDS) + 64;
     const long long qlen = QSHARDS * shard_cap + cap;
-    b += cap * 4 * 4 + 2 * qlen * 8 + cap * 4 + cap * dim * 8;                          // work, queues, parking
+    b += cap * 4 * 5 + 2 * qlen * 8 + cap * 4 + cap * dim * 8;                          // work, overflow, queues, parking
     return b + (1u << 20);
 }
 
@@ -809,6 +816,7 @@ int pf2_create_in(const pf2_mesh* mesh, const pf2_params* params, long long capa
     h->n_dofs = mesh->n_dofs;
     h->params = *params;
     h->cap = capacity > 0 ? capacity : default_capacity(h->dim, h->n_cells, params);
+    h->aslots = arrival_slots(h->dim, params);
     if (h->cap > 0x7fffffffLL) {
         delete h;
         return fail(PF2_E_ARG, "capacity above 2^31-1 particles");
@@ -868,9 +876,9 @@ int pf2_create_in(const pf2_mesh* mesh, const pf2_params* params, long long capa
         dalloc(&h->ddof_cell, h->dof_cell.size()) || dalloc(&h->dftol, nc) || dalloc(&h->dcellX, nck * dim) || dalloc(&h->dcell_star, nck) ||
         dalloc(&h->dcellV, nck * dim) || dalloc(&h->dcellDV, nck * dim) || dalloc(&h->dV, (size_t)nd * dim) ||
         dalloc(&h->dVold, (size_t)nd * dim) || dalloc(&h->dtmp, (size_t)nd * dim) || dalloc(&h->off[0], nc + 1) ||
-        dalloc(&h->off[1], nc + 1) || dalloc(&h->arr_cnt, nc + 1) || dalloc(&h->arr_off, nc + 1) ||
-        dalloc(&h->arr_fill, nc + 1) || dalloc(&h->fs_cnt, nc + 1) || dalloc(&h->fs_off, nc + 1) || dalloc(&h->hitmask, nc) || dalloc(&h->nsurv, nc) ||
-        dalloc(&h->partial, (size_t)nc * h->nv * (dim + 1)) || dalloc(&h->counters, 4) || dalloc(&h->big, nc + 1) ||
+        dalloc(&h->off[1], nc + 1) || dalloc(&h->arr_cnt, nc + 1) ||
+        dalloc(&h->arr_slot, (size_t)nc * h->aslots) || dalloc(&h->ovf_head, nc) || dalloc(&h->fs_cnt, nc + 1) || dalloc(&h->fs_off, nc + 1) || dalloc(&h->hitmask, nc) || dalloc(&h->nsurv, nc) ||
+        dalloc(&h->partial, (size_t)nc * h->nv * (dim + 1)) || dalloc(&h->counters, 8) || dalloc(&h->big, nc + 1) ||
         dalloc(&h->qcount, (size_t)(MOVE_ROUNDS + 2) * (QSHARDS + 1) * QSTRIDE) ||
         dalloc(&h->qprefix, (size_t)(MOVE_ROUNDS + 2) * (QSHARDS + 2)) ||
         dalloc(&h->pcounter, 1) || dalloc(&h->stats, STAT_SHARDS * NSTAT))
@@ -900,19 +908,17 @@ int pf2_create_in(const pf2_mesh* mesh, const pf2_params* params, long long capa
         hipMemsetAsync(h->off[0], 0, sizeof(int) * (nc + 1), h->stream) ||
         hipMemsetAsync(h->off[1], 0, sizeof(int) * (nc + 1), h->stream) ||
         hipMemsetAsync(h->arr_cnt, 0, sizeof(int) * (nc + 1), h->stream) ||
-        hipMemsetAsync(h->arr_fill, 0, sizeof(int) * (nc + 1), h->stream) ||
+        hipMemsetAsync(h->ovf_head, 0xff, sizeof(int) * nc, h->stream) ||
         hipMemsetAsync(h->fs_cnt, 0, sizeof(unsigned long long) * (nc + 1), h->stream) ||
-        hipMemsetAsync(h->counters, 0, sizeof(int) * 4, h->stream) ||
+        hipMemsetAsync(h->counters, 0, sizeof(int) * 8, h->stream) ||
         hipMemsetAsync(h->big, 0, sizeof(int) * (nc + 1), h->stream) ||
         hipMemsetAsync(h->qcount, 0, sizeof(int) * (MOVE_ROUNDS + 2) * (QSHARDS + 1) * QSTRIDE, h->stream) ||
         hipMemsetAsync(h->pcounter, 0, sizeof(uint32_t), h->stream) ||
         hipMemsetAsync(h->stats, 0, sizeof(unsigned long long) * STAT_SHARDS * NSTAT, h->stream))
         return bail(fail(PF2_E_HIP, "initialisation failed"));
-    size_t tb32 = 0, tb64 = 0;
-    if (hipcub::DeviceScan::ExclusiveSum(nullptr, tb32, h->arr_cnt, h->arr_off, nc + 1, h->stream) != hipSuccess ||
-        hipcub::DeviceScan::ExclusiveSum(nullptr, tb64, h->fs_cnt, h->fs_off, nc + 1, h->stream) != hipSuccess)
+    if (hipcub::DeviceScan::ExclusiveSum(nullptr, h->scan_tmp_bytes, h->fs_cnt, h->fs_off, nc + 1, h->stream) !=
+        hipSuccess)
         return bail(fail(PF2_E_HIP, "scan sizing failed"));
-    h->scan_tmp_bytes = tb32 > tb64 ? tb32 : tb64;
     if (dalloc((char**)&h->scan_tmp, h->scan_tmp_bytes)) return bail(PF2_E_HIP);
     if (h->dim == 2) launch_geo<2>(h->stream, dev_mesh<2>(h), h->dgeo32, h->dftol, h->dcellX);
     else launch_geo<3>(h->stream, dev_mesh<3>(h), h->dgeo32, h->dftol, h->dcellX);
@@ -931,7 +937,7 @@ int pf2_destroy(pf2_handle h) {
     void* ptrs[] = {h->dX, h->dcv, h->dcd, h->dstar_off, h->dstar_cell, h->dstar_dir, h->ddof_off, h->ddof_cell,
                     h->dftol, h->dcellX, h->dcell_star, h->dcellV, h->dcellDV, h->dV, h->dVold, h->dtmp, h->dV32,
                     h->dX0, h->dW,
-                    h->dVold32, h->dgeo32, h->off[0], h->off[1], h->arr_cnt, h->arr_off, h->arr_fill, h->fs_cnt,
+                    h->dVold32, h->dgeo32, h->off[0], h->off[1], h->arr_cnt, h->arr_slot, h->ovf_head, h->fs_cnt,
                     h->fs_off, h->hitmask, h->nsurv, h->partial, h->counters, h->big, h->qcount, h->qprefix,
                     h->pcounter, h->stats, h->scan_tmp};
     if (!h->in_block)
@@ -1453,9 +1459,8 @@ int pf2_step_info(pf2_handle h, long long* out, int cap) {
     std::vector<int> offs(2);
     HIPC(hipMemcpyAsync(pref.data(), h->qprefix, pref.size() * 4, hipMemcpyDeviceToHost, h->stream));
     HIPC(hipMemcpyAsync(cnt, h->counters, sizeof(cnt), hipMemcpyDeviceToHost, h->stream));
-    int arrivals = 0;
-    HIPC(hipMemcpyAsync(&arrivals, h->arr_off + h->n_cells, 4, hipMemcpyDeviceToHost, h->stream));
     HIPC(hipStreamSynchronize(h->stream));
+    const int arrivals = cnt[1];
     long long vals[6] = {};
     vals[0] = cnt[3];
     vals[1] = cnt[0];

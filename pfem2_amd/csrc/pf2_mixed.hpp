@@ -356,8 +356,8 @@ __device__ __forceinline__ int advance32(const DevParams& pp, float h32, int i, 
 template <int DIM>
 __device__ __forceinline__ void finish32(const DevParams& pp, const int* __restrict__ pcell, const Rec32& R,
                                          const DevWork& w, bool valid, int i, int c, const float* xi) {
+    add_arrival(w, valid && c >= 0 && c != pcell[valid ? i : 0], c, i);
     if (!valid) return;
-    if (c >= 0 && c != pcell[i]) atomicAdd(&w.arr_cnt[c], 1);
     rec_store_xi<DIM>(R, i, xi);
     w.dest[i] = c;
     if (c >= 0) w.pkey[i] = part_key32<DIM>(pp, xi);
@@ -581,7 +581,8 @@ __device__ __forceinline__ void cell_write32_one(const DevMesh<DIM>& m, const do
                                                  double* __restrict__ partial) {
     constexpr int NV = Q1<DIM>::NV;
     const int n_st = old_off[c + 1] - old_off[c];
-    const int k = n_st + w.arr_cnt[c];
+    const int na = w.arr_cnt[c];
+    const int kin = n_st + (na < w.aslots ? na : w.aslots);
     const unsigned long long fo = w.fs_off[c];
     const long long out0 = (long long)(fo >> 32);
     const int nsurv = w.nsurv[c];
@@ -591,12 +592,9 @@ __device__ __forceinline__ void cell_write32_one(const DevMesh<DIM>& m, const do
         if (gl == 0) atomicOr(&w.counters[2], 1);
         return;
     }
-    for (int j = gl; j < k; j += GS) {
-        const int idx = candidate(w, old_off, c, n_st, j);
-        // arrivals are in this cell's list because their destination is c
-        if (j < n_st && w.dest[idx] != c) continue;
+    auto put = [&](int idx) {
         const int pos = w.spos[idx];
-        if (pos < 0) continue;
+        if (pos < 0) return;
         const long long o = out0 + pos;
         bcell[o] = c;
         bid[o] = aid[idx];
@@ -610,7 +608,14 @@ __device__ __forceinline__ void cell_write32_one(const DevMesh<DIM>& m, const do
         if (pp.keep_vext)
 #pragma unroll
             for (int d = 0; d < DIM; ++d) bvext[(size_t)d * cap + o] = avext[(size_t)d * cap + idx];
+    };
+    for (int j = gl; j < kin; j += GS) {
+        const int idx = candidate(w, old_off, c, n_st, j);
+        // arrivals are in this cell's slots because their destination is c
+        if (j < n_st && w.dest[idx] != c) continue;
+        put(idx);
     }
+    for_overflow(w, c, gl, GS, [&](int, int idx) { put(idx); });
     const int spawns = nfinal - nsurv;
     if (spawns > 0) {
         const unsigned long long all = pp.nparts_in >= 64 ? ~0ull : ((1ull << pp.nparts_in) - 1);
@@ -639,7 +644,7 @@ __device__ __forceinline__ void cell_write32_one(const DevMesh<DIM>& m, const do
     }
     if (gl == 0) {
         w.arr_cnt[c] = 0;
-        w.arr_fill[c] = 0;
+        w.ovf_head[c] = -1;
     }
     if constexpr (FUSE) {
         constexpr int NACC = PartialsShape<DIM>::NACC;
@@ -937,7 +942,7 @@ __global__ void __launch_bounds__(256) k_resort32(DevMesh<DIM> m, const int* __r
             }
             w.dest[i] = c;
         }
-        if (valid && c >= 0 && c != c0) atomicAdd(&w.arr_cnt[c], 1);
+        add_arrival(w, valid && c >= 0 && c != c0, c, i);
     }
     wave_stat(w.stats, 0, lost);
     wave_stat(w.stats, 3, searches);
