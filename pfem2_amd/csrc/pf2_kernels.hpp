@@ -520,9 +520,12 @@ __device__ __forceinline__ void queue_push(const DevQueues& qu, int round, bool 
     if (spill) data[(size_t)QSHARDS * qu.shard_cap + ob + __popcll(smask & ((1ull << lane) - 1))] = value;
 }
 
-// exclusive prefix of the valid entries per shard, then the overflow start
-// and the total; one block of QSHARDS threads per round
-__global__ void k_queue_prefix(DevQueues qu, int round) {
+// Exclusive prefix of the valid entries per shard, then the overflow start
+// and the total, computed by each workgroup of the search kernel from the
+// shard counters (blockDim.x == QSHARDS: one thread per shard; the counters
+// are a few cache lines, so the redundant work costs less than a launch).
+// Workgroup 0 also stores it for diagnostics.
+__device__ __forceinline__ int queue_prefix(const DevQueues& qu, int round, int* s_pref) {
     __shared__ int s_wave[QSHARDS / WAVE];
     const int t = threadIdx.x;
     const int lane = t & (WAVE - 1), wv = t / WAVE;
@@ -538,20 +541,18 @@ __global__ void k_queue_prefix(DevQueues qu, int round) {
     __syncthreads();
     int wbase = 0;
     for (int q = 0; q < wv; ++q) wbase += s_wave[q];
-    int* pref = qu.prefix + (size_t)round * (QSHARDS + 2);
-    pref[t] = wbase + incl - v;
+    s_pref[t] = wbase + incl - v;
     if (t == QSHARDS - 1) {
         const int tot = wbase + incl;
-        pref[QSHARDS] = tot;
-        pref[QSHARDS + 1] = tot + *queue_count(qu, round, QSHARDS);
+        s_pref[QSHARDS] = tot;
+        s_pref[QSHARDS + 1] = tot + *queue_count(qu, round, QSHARDS);
     }
-}
-
-// the prefix of a round into LDS (blockDim.x >= QSHARDS + 2 is not required)
-__device__ __forceinline__ int queue_prefix(const DevQueues& qu, int round, int* s_pref) {
-    const int* pref = qu.prefix + (size_t)round * (QSHARDS + 2);
-    for (int t = threadIdx.x; t < QSHARDS + 2; t += blockDim.x) s_pref[t] = pref[t];
     __syncthreads();
+    if (blockIdx.x == 0) {
+        int* pref = qu.prefix + (size_t)round * (QSHARDS + 2);
+        pref[t] = s_pref[t];
+        if (t < 2) pref[QSHARDS + t] = s_pref[QSHARDS + t];
+    }
     return s_pref[QSHARDS + 1];
 }
 

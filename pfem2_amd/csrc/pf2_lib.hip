@@ -503,10 +503,6 @@ int run_move(pf2_handle_s* h, int do_correct, int do_move, bool fuse_sums = fals
         }
         if (do_move)
             for (int r = 0; r <= MOVE_ROUNDS; ++r) {
-                {
-                    Timed tq(h, K_QPREFIX);
-                    hipLaunchKernelGGL(k_queue_prefix, dim3(1), dim3(QSHARDS), 0, h->stream, qu, r);
-                }
                 Timed t(h, r == 0 ? K_SEARCH32 : (r == 1 ? K_SEARCH32_R1 : K_SEARCH32_R2));
                 launch_search_resume32<DIM>(h->stream, round_grid(qgrid, r), dev_mesh<DIM>(h), h->dV32,
                                             h->dgeo32, b.cell, rec32(b), b.vext, h->cap, pp, w, qu, r);
@@ -519,10 +515,6 @@ int run_move(pf2_handle_s* h, int do_correct, int do_move, bool fuse_sums = fals
         }
         if (do_move)
             for (int r = 0; r <= MOVE_ROUNDS; ++r) {
-                {
-                    Timed tq(h, K_QPREFIX);
-                    hipLaunchKernelGGL(k_queue_prefix, dim3(1), dim3(QSHARDS), 0, h->stream, qu, r);
-                }
                 Timed t(h, r == 0 ? K_SEARCH : (r == 1 ? K_SEARCH_R1 : K_SEARCH_R2));
                 hipLaunchKernelGGL(HIP_KERNEL_NAME(k_search_resume<DIM>), dim3(round_grid(qgrid, r)),
                                    dim3(256), 0, h->stream, dev_mesh<DIM>(h), h->dV, dev_particles(h, h->cur), pp, w,
