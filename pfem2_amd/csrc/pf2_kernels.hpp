@@ -134,6 +134,7 @@ struct DevMesh {
     const double* ftol;    // per cell: tolerance of the clearly-outside test (-1: none)
     const double* cellX;   // per cell: its vertex coordinates packed [NV][DIM] (copies of X)
     const int2* cell_star; // per cell and local vertex: [begin, end) of that vertex's star
+    const int4* cell_nbr;  // 2-D: per cell, the neighbours across faces 0..3 (-1 on the boundary)
     const double* cellV;   // per cell: V at its DoFs [NV][DIM] (refreshed when V changes)
     const double* cellDV;  // per cell: V - V_old at its DoFs [NV][DIM]
 };
@@ -301,17 +302,41 @@ struct PrefetchV64 {
 // each candidate before its containment test, so that a caller can have the
 // candidate's other data loaded alongside its vertices (the data of the
 // last candidate loaded is the found cell's).
-template <int DIM, class Pre>
+// 2-D face through which the step from xold to x most likely left the cell:
+// the linearised reference displacement J(xi_old)^-1 (x - xold) added to
+// xi_old, the component furthest outside [0, 1] (faces 0..3: xi0 = 0, xi0 = 1,
+// xi1 = 0, xi1 = 1), -1 if none.  Only a guess: the caller verifies it.
+__device__ __forceinline__ int exit_face_2d(const double (*P)[2], const double* xi_old, const double* xold,
+                                            const double* x) {
+    double J[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        J[i][0] = (1.0 - xi_old[1]) * (P[1][i] - P[0][i]) + xi_old[1] * (P[3][i] - P[2][i]);
+        J[i][1] = (1.0 - xi_old[0]) * (P[2][i] - P[0][i]) + xi_old[0] * (P[3][i] - P[1][i]);
+    }
+    const double d0 = x[0] - xold[0], d1 = x[1] - xold[1];
+    const double det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+    const double e0 = xi_old[0] + (J[1][1] * d0 - J[0][1] * d1) / det;
+    const double e1 = xi_old[1] + (J[0][0] * d1 - J[1][0] * d0) / det;
+    const double o0 = e0 < 0.0 ? -e0 : e0 - 1.0;
+    const double o1 = e1 < 0.0 ? -e1 : e1 - 1.0;
+    if (o0 > 0.0 && o0 >= o1) return e0 < 0.0 ? 0 : 1;
+    if (o1 > 0.0) return e1 < 0.0 ? 2 : 3;
+    return -1;
+}
+
+// margin in reference coordinates beyond which a point inside one cell is
+// inside no other cell under the containment test (the Newton acceptance
+// radius is 1e-10 of a cell diameter)
+constexpr double NBR_MARGIN = 1e-6;
+
+template <int DIM, class Pre, bool SHORTCUT = true>
 __device__ int relocate(const DevMesh<DIM>& m, int c, const double (*P)[DIM], const double* xi_old,
                         const double* vext, const double* x, double* xi_out, double (*Q)[DIM], Pre& pre) {
     constexpr int NV = Q1<DIM>::NV;
     constexpr int CH = 4;   // star entries read together
-    // 2-D: the star ranges of all four vertices are read with no dependence
-    // on the vertex choice
-    int2 cs[DIM == 2 ? NV : 1];
-    if constexpr (DIM == 2)
-#pragma unroll
-        for (int k = 0; k < NV; ++k) cs[k] = m.cell_star[(size_t)c * NV + k];
+    int4 nb;
+    if constexpr (DIM == 2 && SHORTCUT) nb = m.cell_nbr[c];
     double xold[DIM], vn[DIM], t[DIM];
     forward<DIM>(P, xi_old, xold);
     const double rv = 1.0 / vnorm<DIM>(vext);
@@ -324,7 +349,6 @@ __device__ int relocate(const DevMesh<DIM>& m, int c, const double (*P)[DIM], co
     for (int d = 0; d < DIM; ++d) t[d] *= rn;
     double best = vdot<DIM>(vn, t);
     int cvk = 0;
-    int2 sel = cs[0];
     double dir[DIM];
 #pragma unroll
     for (int d = 0; d < DIM; ++d) dir[d] = P[0][d];
@@ -339,9 +363,34 @@ __device__ int relocate(const DevMesh<DIM>& m, int c, const double (*P)[DIM], co
         if (a > best) {
             cvk = k;
             best = a;
-            if constexpr (DIM == 2) sel = cs[k];
 #pragma unroll
             for (int d = 0; d < DIM; ++d) dir[d] = P[k][d];
+        }
+    }
+    // the selected vertex's star range, loaded alongside the shortcut's cell
+    const int2 se = m.cell_star[(size_t)c * NV + cvk];
+    if constexpr (DIM == 2 && SHORTCUT) {
+        // Face-neighbour shortcut.  The neighbour g across the exit face is in
+        // the selected vertex's star when that vertex lies on the face; if g
+        // contains x at least NBR_MARGIN inside, no other cell contains x,
+        // so g is the first containing cell of the star in any order: the
+        // result of the loop below, with the same inverse map.
+        const int f = exit_face_2d(P, xi_old, xold, x);
+        if (f >= 0) {
+            const int fa = f == 1 ? 1 : (f == 3 ? 2 : 0);
+            const int fb = f == 0 ? 2 : (f == 2 ? 1 : 3);
+            const int g = f == 0 ? nb.x : (f == 1 ? nb.y : (f == 2 ? nb.z : nb.w));
+            if (g >= 0 && (cvk == fa || cvk == fb)) {
+                double r[DIM];
+                load_vertices<DIM>(m, g, Q);
+                pre.load(g);
+                if (inverse_map<DIM>(Q, x, r, m.ftol[g], true) && r[0] >= NBR_MARGIN && r[0] <= 1.0 - NBR_MARGIN &&
+                    r[1] >= NBR_MARGIN && r[1] <= 1.0 - NBR_MARGIN) {
+#pragma unroll
+                    for (int d = 0; d < DIM; ++d) xi_out[d] = r[d];
+                    return g;
+                }
+            }
         }
     }
 #pragma unroll
@@ -349,9 +398,6 @@ __device__ int relocate(const DevMesh<DIM>& m, int c, const double (*P)[DIM], co
     rn = 1.0 / vnorm<DIM>(dir);
 #pragma unroll
     for (int d = 0; d < DIM; ++d) dir[d] *= rn;
-    int2 se;
-    if constexpr (DIM == 2) se = sel;
-    else se = m.cell_star[(size_t)c * NV + cvk];
     const int s0 = se.x, n = se.y - se.x;
     if (n <= 32) {
         // candidates in decreasing key order (ties: star order), first

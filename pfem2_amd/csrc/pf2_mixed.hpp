@@ -419,8 +419,11 @@ __global__ void __launch_bounds__(256) k_move32(DevMesh<DIM> m, const float* __r
 
 // FP64 relocation of queued particles (the reference rule), then the
 // remaining FP32 substeps
-template <int DIM>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DIM == 2 ? 4 : 2))) k_search_resume32(DevMesh<DIM> m, const float* __restrict__ V32,
+// (LAST: the last round, which loops every particle to its end; the
+// face-neighbour shortcut is left to the earlier round, whose register
+// budget it fits)
+template <int DIM, bool LAST>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DIM == 2 ? (LAST ? 4 : 3) : 2))) k_search_resume32(DevMesh<DIM> m, const float* __restrict__ V32,
                                                          const float* __restrict__ geo, const int* __restrict__ pcell,
                                                          Rec32 R, double* __restrict__ pvext,
                                                          long long cap, DevParams pp, DevWork w, DevQueues qu,
@@ -429,7 +432,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DIM ==
     constexpr int NG = DIM == 2 ? 6 : 21;
     __shared__ int s_pref[QSHARDS + 2];
     const int n = queue_prefix(qu, round, s_pref);
-    const bool last = round == MOVE_ROUNDS;
+    constexpr bool last = LAST;
     const float h32 = (float)pp.h;
     const int lane = threadIdx.x & (WAVE - 1);
     const WaveLoop L = range_loop(n);
@@ -465,7 +468,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DIM ==
                 int cn;
                 if constexpr (DIM == 2) {
                     PrefetchCell32<DIM> pre{geo, V32};
-                    cn = relocate<DIM>(m, c, P, xid, vext, x, r, Q, pre);
+                    cn = relocate<DIM, PrefetchCell32<DIM>, !LAST>(m, c, P, xid, vext, x, r, Q, pre);
                     if (cn >= 0) {
 #pragma unroll
                         for (int d = 0; d < DIM; ++d) {
@@ -527,8 +530,12 @@ template <int DIM>
 void launch_search_resume32(hipStream_t s, int grid, DevMesh<DIM> m, const float* V32, const float* geo,
                             const int* pcell, Rec32 R, double* pvext, long long cap, DevParams pp, DevWork w,
                             DevQueues qu, int round) {
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_search_resume32<DIM>), dim3(grid), dim3(256), 0, s, m, V32, geo, pcell, R,
-                       pvext, cap, pp, w, qu, round);
+    if (round == MOVE_ROUNDS)
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_search_resume32<DIM, true>), dim3(grid), dim3(256), 0, s, m, V32, geo,
+                           pcell, R, pvext, cap, pp, w, qu, round);
+    else
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_search_resume32<DIM, false>), dim3(grid), dim3(256), 0, s, m, V32, geo,
+                           pcell, R, pvext, cap, pp, w, qu, round);
 }
 
 // rows of the fused projection sums: a cell's final particles, 2-D records
