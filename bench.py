@@ -331,14 +331,15 @@ def measured_traffic(kernel):
 
 
 def roofline(res, m, traffic=True):
-    """Roofline of the kernel with the largest share of the step.  The
+    """Roofline of the kernel with the largest share of the step.  The FP64
     relocation rounds are one kernel launched once per round: its bytes and
     duration per launch are averaged over the rounds, as the profiler's
-    per-kernel figures are."""
+    per-kernel figures are.  The FP32 rounds are two kernels (the last round
+    is an instantiation of its own) and stay apart."""
     per_step = res["kernels_ms_per_step"]
     fam = {}
     for key, v in per_step.items():
-        f = re.sub(r"_r\d$", "", key)
+        f = re.sub(r"_r\d$", "", key) if key.startswith("k_search_resume_r") else key
         fam.setdefault(f, []).append(key)
     dom = max(fam, key=lambda f: sum(per_step[k] for k in fam[f]))
     members = fam[dom]
