@@ -134,7 +134,8 @@ struct DevMesh {
     const double* ftol;    // per cell: tolerance of the clearly-outside test (-1: none)
     const double* cellX;   // per cell: its vertex coordinates packed [NV][DIM] (copies of X)
     const int2* cell_star; // per cell and local vertex: [begin, end) of that vertex's star
-    const int4* cell_nbr;  // 2-D: per cell, the neighbours across faces 0..3 (-1 on the boundary)
+    const int4* cell_nbr;  // per cell, the neighbours across faces 0..2*DIM-1 (-1 on the boundary or past
+                           // the faces): one int4 in 2-D, two in 3-D
     const double* cellV;   // per cell: V at its DoFs [NV][DIM] (refreshed when V changes)
     const double* cellDV;  // per cell: V - V_old at its DoFs [NV][DIM]
 };
@@ -302,27 +303,46 @@ struct PrefetchV64 {
 // each candidate before its containment test, so that a caller can have the
 // candidate's other data loaded alongside its vertices (the data of the
 // last candidate loaded is the found cell's).
-// 2-D face through which the step from xold to x most likely left the cell:
-// the linearised reference displacement J(xi_old)^-1 (x - xold) added to
-// xi_old, the component furthest outside [0, 1] (faces 0..3: xi0 = 0, xi0 = 1,
-// xi1 = 0, xi1 = 1), -1 if none.  Only a guess: the caller verifies it.
-__device__ __forceinline__ int exit_face_2d(const double (*P)[2], const double* xi_old, const double* xold,
-                                            const double* x) {
-    double J[2][2];
+// Face through which the step from xold to x most likely left the cell: the
+// linearised reference displacement J(xi_old)^-1 (x - xold) added to xi_old,
+// the component furthest outside [0, 1] (face 2a + s is xi_a = s), -1 if
+// none.  Only a guess: the caller verifies it.
+template <int DIM>
+__device__ __forceinline__ int exit_face(const double (*P)[DIM], const double* xi_old, const double* xold,
+                                         const double* x) {
+    constexpr int NV = Q1<DIM>::NV;
+    double J[DIM][DIM], Ji[DIM][DIM], dx[DIM], dxi[DIM];
 #pragma unroll
-    for (int i = 0; i < 2; ++i) {
-        J[i][0] = (1.0 - xi_old[1]) * (P[1][i] - P[0][i]) + xi_old[1] * (P[3][i] - P[2][i]);
-        J[i][1] = (1.0 - xi_old[0]) * (P[2][i] - P[0][i]) + xi_old[0] * (P[3][i] - P[1][i]);
+    for (int i = 0; i < DIM; ++i)
+#pragma unroll
+        for (int j = 0; j < DIM; ++j) J[i][j] = 0.0;
+#pragma unroll
+    for (int k = 0; k < NV; ++k)
+#pragma unroll
+        for (int j = 0; j < DIM; ++j) {
+            double w = ((k >> j) & 1) ? 1.0 : -1.0;
+#pragma unroll
+            for (int d = 0; d < DIM; ++d)
+                if (d != j) w *= ((k >> d) & 1) ? xi_old[d] : 1.0 - xi_old[d];
+#pragma unroll
+            for (int i = 0; i < DIM; ++i) J[i][j] += w * P[k][i];
+        }
+    invert<DIM>(J, Ji);
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) dx[d] = x[d] - xold[d];
+    matvec<DIM>(Ji, dx, dxi);
+    int f = -1;
+    double worst = 0.0;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+        const double e = xi_old[d] + dxi[d];
+        const double o = e < 0.0 ? -e : e - 1.0;
+        if (o > worst) {
+            worst = o;
+            f = 2 * d + (e < 0.0 ? 0 : 1);
+        }
     }
-    const double d0 = x[0] - xold[0], d1 = x[1] - xold[1];
-    const double det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
-    const double e0 = xi_old[0] + (J[1][1] * d0 - J[0][1] * d1) / det;
-    const double e1 = xi_old[1] + (J[0][0] * d1 - J[1][0] * d0) / det;
-    const double o0 = e0 < 0.0 ? -e0 : e0 - 1.0;
-    const double o1 = e1 < 0.0 ? -e1 : e1 - 1.0;
-    if (o0 > 0.0 && o0 >= o1) return e0 < 0.0 ? 0 : 1;
-    if (o1 > 0.0) return e1 < 0.0 ? 2 : 3;
-    return -1;
+    return f;
 }
 
 // margin in reference coordinates beyond which a point inside one cell is
@@ -335,8 +355,17 @@ __device__ int relocate(const DevMesh<DIM>& m, int c, const double (*P)[DIM], co
                         const double* vext, const double* x, double* xi_out, double (*Q)[DIM], Pre& pre) {
     constexpr int NV = Q1<DIM>::NV;
     constexpr int CH = 4;   // star entries read together
-    int4 nb;
-    if constexpr (DIM == 2 && SHORTCUT) nb = m.cell_nbr[c];
+    int nb[2 * DIM];
+    if constexpr (SHORTCUT) {
+#pragma unroll
+        for (int h = 0; h < DIM - 1; ++h) {
+            const int4 q = m.cell_nbr[(size_t)c * (DIM - 1) + h];
+            nb[4 * h] = q.x;
+            nb[4 * h + 1] = q.y;
+            if (4 * h + 2 < 2 * DIM) nb[4 * h + 2] = q.z;
+            if (4 * h + 3 < 2 * DIM) nb[4 * h + 3] = q.w;
+        }
+    }
     double xold[DIM], vn[DIM], t[DIM];
     forward<DIM>(P, xi_old, xold);
     const double rv = 1.0 / vnorm<DIM>(vext);
@@ -369,23 +398,26 @@ __device__ int relocate(const DevMesh<DIM>& m, int c, const double (*P)[DIM], co
     }
     // the selected vertex's star range, loaded alongside the shortcut's cell
     const int2 se = m.cell_star[(size_t)c * NV + cvk];
-    if constexpr (DIM == 2 && SHORTCUT) {
+    if constexpr (SHORTCUT) {
         // Face-neighbour shortcut.  The neighbour g across the exit face is in
         // the selected vertex's star when that vertex lies on the face; if g
         // contains x at least NBR_MARGIN inside, no other cell contains x,
         // so g is the first containing cell of the star in any order: the
         // result of the loop below, with the same inverse map.
-        const int f = exit_face_2d(P, xi_old, xold, x);
-        if (f >= 0) {
-            const int fa = f == 1 ? 1 : (f == 3 ? 2 : 0);
-            const int fb = f == 0 ? 2 : (f == 2 ? 1 : 3);
-            const int g = f == 0 ? nb.x : (f == 1 ? nb.y : (f == 2 ? nb.z : nb.w));
-            if (g >= 0 && (cvk == fa || cvk == fb)) {
+        const int f = exit_face<DIM>(P, xi_old, xold, x);
+        if (f >= 0 && ((cvk >> (f >> 1)) & 1) == (f & 1)) {
+            int g = nb[0];
+#pragma unroll
+            for (int q = 1; q < 2 * DIM; ++q)
+                if (f == q) g = nb[q];
+            if (g >= 0) {
                 double r[DIM];
                 load_vertices<DIM>(m, g, Q);
                 pre.load(g);
-                if (inverse_map<DIM>(Q, x, r, m.ftol[g], true) && r[0] >= NBR_MARGIN && r[0] <= 1.0 - NBR_MARGIN &&
-                    r[1] >= NBR_MARGIN && r[1] <= 1.0 - NBR_MARGIN) {
+                bool in = inverse_map<DIM>(Q, x, r, m.ftol[g], true);
+#pragma unroll
+                for (int d = 0; d < DIM; ++d) in = in && r[d] >= NBR_MARGIN && r[d] <= 1.0 - NBR_MARGIN;
+                if (in) {
 #pragma unroll
                     for (int d = 0; d < DIM; ++d) xi_out[d] = r[d];
                     return g;
@@ -756,7 +788,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DIM ==
                 double Q[NV][DIM], r[DIM];
                 ++searches;
                 PrefetchV64<DIM> pre{m.cellV};
-                const int cn = relocate<DIM>(m, c, P, xi, vext, x, r, Q, pre);
+                // (3-D: the shortcut's registers push this kernel into heavy spilling)
+                const int cn = relocate<DIM, PrefetchV64<DIM>, DIM == 2>(m, c, P, xi, vext, x, r, Q, pre);
                 if (cn < 0) {
                     c = -1;
                     ++lost;

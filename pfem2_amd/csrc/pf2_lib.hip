@@ -113,7 +113,7 @@ struct pf2_handle_s {
     double* dftol = nullptr;
     double* dcellX = nullptr;
     int2* dcell_star = nullptr;
-    std::vector<int> cell_nbr;   // 2-D: face neighbours, 4 per cell
+    std::vector<int> cell_nbr;   // face neighbours, 4 (2-D) or 8 (3-D, 6 used) per cell
     int4* dcell_nbr = nullptr;
     double *dcellV = nullptr, *dcellDV = nullptr;
     // fields (node-major)
@@ -684,7 +684,7 @@ size_t estimate_bytes(int dim, long long nc, long long nvt, long long nd, long l
     const long long nv = 1LL << dim, nck = nc * nv;
     size_t b = 0;
     b += nvt * dim * 8 + nck * 4 * 2 + (nvt + 1) * 4 + nck * 4 + nck * dim * 8;       // X, cv, cd, star
-    b += dim == 2 ? nc * 16 : 0;                                                         // face neighbours
+    b += nc * 16 * (dim - 1);                                                            // face neighbours
     b += (nd + 1) * 4 + nck * 4 + nc * 8 + nck * dim * 8 + nck * 8 + nck * dim * 8 * 2;  // dof map, ftol, cellX/star/V/dV
     b += nd * dim * 8 * 3;                                                              // V, V_old, staging
     b += (nc + 1) * 4 * 9 + nc * 4 * (1 + arrival_slots(dim, p));                      // per-cell work, arrival slots
@@ -844,20 +844,29 @@ int pf2_create_in(const pf2_mesh* mesh, const pf2_params* params, long long capa
         h->cell_star[2 * i] = h->star_off[h->cv[i]];
         h->cell_star[2 * i + 1] = h->star_off[h->cv[i] + 1];
     }
-    if (h->dim == 2) {
-        // neighbour across each face (faces xi0 = 0, xi0 = 1, xi1 = 0, xi1 = 1:
-        // local vertices {0,2}, {1,3}, {0,1}, {2,3}): the other cell of the
-        // first vertex's star that also holds the second vertex
-        static const int fv[4][2] = {{0, 2}, {1, 3}, {0, 1}, {2, 3}};
-        h->cell_nbr.assign((size_t)4 * h->n_cells, -1);
+    {
+        // neighbour across each face (face 2a + s: the local vertices whose
+        // bit a is s): the other cell of the face's first vertex's star that
+        // holds all the face's vertices; -1 on the boundary
+        const int nv = h->nv, nf = 2 * h->dim, stride = h->dim == 2 ? 4 : 8;
+        h->cell_nbr.assign((size_t)stride * h->n_cells, -1);
         for (int c = 0; c < h->n_cells; ++c)
-            for (int f = 0; f < 4; ++f) {
-                const int a = h->cv[(size_t)c * 4 + fv[f][0]], b = h->cv[(size_t)c * 4 + fv[f][1]];
-                for (int e = h->star_off[a]; e < h->star_off[a + 1] && h->cell_nbr[(size_t)c * 4 + f] < 0; ++e) {
+            for (int f = 0; f < nf; ++f) {
+                int fvtx[4], nfv = 0;
+                for (int k = 0; k < nv; ++k)
+                    if (((k >> (f >> 1)) & 1) == (f & 1)) fvtx[nfv++] = h->cv[(size_t)c * nv + k];
+                int& out = h->cell_nbr[(size_t)c * stride + f];
+                for (int e = h->star_off[fvtx[0]]; e < h->star_off[fvtx[0] + 1] && out < 0; ++e) {
                     const int s = h->star_cell[e];
                     if (s == c) continue;
-                    for (int k = 0; k < 4; ++k)
-                        if (h->cv[(size_t)s * 4 + k] == b) h->cell_nbr[(size_t)c * 4 + f] = s;
+                    int hits = 0;
+                    for (int q = 1; q < nfv; ++q)
+                        for (int k = 0; k < nv; ++k)
+                            if (h->cv[(size_t)s * nv + k] == fvtx[q]) {
+                                ++hits;
+                                break;
+                            }
+                    if (hits == nfv - 1) out = s;
                 }
             }
     }
@@ -887,7 +896,7 @@ int pf2_create_in(const pf2_mesh* mesh, const pf2_params* params, long long capa
         dalloc(&h->dstar_off, nvt + 1) || dalloc(&h->dstar_cell, h->star_cell.size()) ||
         dalloc(&h->dstar_dir, h->star_dir.size()) || dalloc(&h->ddof_off, nd + 1) ||
         dalloc(&h->ddof_cell, h->dof_cell.size()) || dalloc(&h->dftol, nc) || dalloc(&h->dcellX, nck * dim) || dalloc(&h->dcell_star, nck) ||
-        (h->dim == 2 && dalloc(&h->dcell_nbr, (size_t)h->n_cells)) ||
+        dalloc(&h->dcell_nbr, (size_t)h->n_cells * (h->dim - 1)) ||
         dalloc(&h->dcellV, nck * dim) || dalloc(&h->dcellDV, nck * dim) || dalloc(&h->dV, (size_t)nd * dim) ||
         dalloc(&h->dVold, (size_t)nd * dim) || dalloc(&h->dtmp, (size_t)nd * dim) || dalloc(&h->off[0], nc + 1) ||
         dalloc(&h->off[1], nc + 1) || dalloc(&h->arr_cnt, nc + 1) ||
@@ -916,7 +925,7 @@ int pf2_create_in(const pf2_mesh* mesh, const pf2_params* params, long long capa
         up(h->ddof_off, h->dof_off.data(), h->dof_off.size() * 4) ||
         up(h->ddof_cell, h->dof_cell.data(), h->dof_cell.size() * 4) ||
         up(h->dcell_star, h->cell_star.data(), h->cell_star.size() * 4) ||
-        (h->dim == 2 && up(h->dcell_nbr, h->cell_nbr.data(), h->cell_nbr.size() * 4)))
+        up(h->dcell_nbr, h->cell_nbr.data(), h->cell_nbr.size() * 4))
         return bail(fail(PF2_E_HIP, "mesh upload failed"));
     if (hipMemsetAsync(h->dV, 0, sizeof(double) * nd * dim, h->stream) ||
         hipMemsetAsync(h->dVold, 0, sizeof(double) * nd * dim, h->stream) ||
