@@ -1540,7 +1540,8 @@ __global__ void k_displace(const double* __restrict__ X0, const double* __restri
 }
 
 // V refreshed from a component-major device field: the node-major copy and
-// the per-cell copies of V and V - V_old (FP32 copies when V32 is given)
+// the per-cell copies of V and V - V_old (FP32 copies when V32 is given; the
+// FP64 V - V_old, read only by the FP64 correction, when cellDV is given)
 template <int DIM>
 __global__ void k_refresh(DevMesh<DIM> m, const double* __restrict__ Vcm, double* __restrict__ V,
                           const double* __restrict__ Vold, double* __restrict__ cellV, double* __restrict__ cellDV,
@@ -1558,7 +1559,7 @@ __global__ void k_refresh(DevMesh<DIM> m, const double* __restrict__ Vcm, double
             for (int d = 0; d < DIM; ++d) {
                 const double a = Vcm[(size_t)d * nd + dof], b = Vold[(size_t)dof * DIM + d];
                 cellV[(size_t)t * DIM + d] = a;
-                cellDV[(size_t)t * DIM + d] = a - b;
+                if (cellDV) cellDV[(size_t)t * DIM + d] = a - b;
                 if (V32) {
                     V32[(size_t)t * DIM + d] = (float)a;
                     DV32[(size_t)t * DIM + d] = (float)(a - b);
@@ -1579,7 +1580,7 @@ __global__ void k_pack_fields(DevMesh<DIM> m, const double* __restrict__ V, cons
         for (int d = 0; d < DIM; ++d) {
             const double a = V[(size_t)dof * DIM + d], b = Vold[(size_t)dof * DIM + d];
             cellV[(size_t)t * DIM + d] = a;
-            cellDV[(size_t)t * DIM + d] = a - b;
+            if (cellDV) cellDV[(size_t)t * DIM + d] = a - b;
         }
     }
 }

@@ -614,13 +614,13 @@ int pack_fields(pf2_handle_s* h) {
     const int g = grid_for(n, 256, 4096);
     if (h->dim == 2) {
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pack_fields<2>), dim3(g), dim3(256), 0, h->stream, dev_mesh<2>(h), h->dV,
-                           h->dVold, h->dcellV, h->dcellDV);
+                           h->dVold, h->dcellV, mixed(h) ? nullptr : h->dcellDV);
         if (mixed(h))
             hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pack_fields32<2>), dim3(g), dim3(256), 0, h->stream, dev_mesh<2>(h),
                                h->dV, h->dVold, h->dV32, h->dVold32);
     } else {
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pack_fields<3>), dim3(g), dim3(256), 0, h->stream, dev_mesh<3>(h), h->dV,
-                           h->dVold, h->dcellV, h->dcellDV);
+                           h->dVold, h->dcellV, mixed(h) ? nullptr : h->dcellDV);
         if (mixed(h))
             hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pack_fields32<3>), dim3(g), dim3(256), 0, h->stream, dev_mesh<3>(h),
                                h->dV, h->dVold, h->dV32, h->dVold32);
@@ -643,10 +643,10 @@ int refresh_field(pf2_handle_s* h, const double* field) {
     Timed t(h, K_MISC);
     if (h->dim == 2)
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_refresh<2>), dim3(g), dim3(256), 0, h->stream, dev_mesh<2>(h), field,
-                           h->dV, h->dVold, h->dcellV, h->dcellDV, mixed(h) ? h->dV32 : nullptr, h->dVold32);
+                           h->dV, h->dVold, h->dcellV, mixed(h) ? nullptr : h->dcellDV, mixed(h) ? h->dV32 : nullptr, h->dVold32);
     else
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_refresh<3>), dim3(g), dim3(256), 0, h->stream, dev_mesh<3>(h), field,
-                           h->dV, h->dVold, h->dcellV, h->dcellDV, mixed(h) ? h->dV32 : nullptr, h->dVold32);
+                           h->dV, h->dVold, h->dcellV, mixed(h) ? nullptr : h->dcellDV, mixed(h) ? h->dV32 : nullptr, h->dVold32);
     HIPC(hipGetLastError());
     h->packed = true;
     return 0;

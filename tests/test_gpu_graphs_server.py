@@ -66,6 +66,41 @@ def test_graph_replay_matches_eager(kind, precision):
             assert np.array_equal(xa, xb, equal_nan=True)
 
 
+def _run_plain(m, V, prm, precision, graphs, steps, checks, uploads):
+    """pf2_step (no field): the node pass refreshes the per-cell copies of V
+    for the next step; an upload in between makes them stale again."""
+    dt, S, MAX, q = prm
+    st = ParticleStage(m, dt, S, MAX, q, precision)
+    st.upload_fields(V, V)
+    st.seed_particles()
+    st.set_graphs(graphs)
+    out = []
+    for s in range(steps):
+        if s in uploads:
+            st.upload_fields(0.9 * V, V)
+        Vp = st.step(download=True)
+        if s in checks:
+            p = st.particles()
+            out.append((Vp, p["cell"], p["id"], p["xi"], p["v"], st.cell_counts()))
+    st.close()
+    return out
+
+
+@pytest.mark.parametrize("kind", ["cyl", "box3d"])
+@pytest.mark.parametrize("precision", [FP64, FP32_MIXED])
+def test_graph_replay_of_plain_step_matches_eager(kind, precision):
+    m, V, prm = _case(kind)
+    checks = {0, 1, 3, 4, 6}
+    uploads = {3, 5}
+    print(f"plain step graph replay case={kind} precision={precision}", flush=True)
+    a = _run_plain(m, V, prm, precision, False, 7, checks, uploads)
+    b = _run_plain(m, V, prm, precision, True, 7, checks, uploads)
+    assert len(a) == len(b) == len(checks)
+    for ra, rb in zip(a, b):
+        for xa, xb in zip(ra, rb):
+            assert np.array_equal(xa, xb, equal_nan=True)
+
+
 def _direct(m, V, prm, steps):
     dt, S, MAX, q = prm
     st = ParticleStage(m, dt, S, MAX, q, FP64)
