@@ -132,6 +132,50 @@ def test_separate_calls_equal_fused_step(precision):
         g2.upload_fields(V, None)
 
 
+@pytest.mark.parametrize("precision", [FP64, FP32_MIXED])
+def test_step_from_empty_state(precision):
+    """No particles at all: the move has nothing to do, the census reseeds
+    every part of every cell (ids continue the counter from 0) and the
+    projection uses the new particles; equal to the oracle (FP64 bitwise,
+    FP32-mixed to FP32 rounding of the reseeded lattice points)."""
+    m = M.quad_2d(12, 7, (0.0, 0.0), (1.2, 0.7), jitter=0.2, seed=60)
+    V = M.uniform_field(m, [0.3, -0.1]) + 0.05 * M.fourier_perturbation(m.vertices, [1.2, 0.7], seed=61)
+    g = ParticleStage(m, 0.01, 4, 2, [3, 3], precision)
+    o = Oracle(m, 0.01, 4, 2, [3, 3])
+    for s in (g, o):
+        s.upload_fields(V, V)
+    assert g.n_global_particles() == 0
+    Vg = g.step(download=True)
+    Vo = o.step()
+    assert g.n_global_particles() == o.n_global_particles() == m.n_cells * 9
+    assert g.particle_counter() == o.particle_counter() == m.n_cells * 9
+    pg, po = g.particles(), o.particles()
+    assert np.array_equal(pg["cell"], po["cell"]) and np.array_equal(pg["id"], po["id"])
+    tol = 0.0 if precision == FP64 else 1e-6
+    assert np.max(np.abs(pg["xi"] - po["xi"])) <= tol
+    assert field_maxrel(Vg, Vo) <= (REL if precision == FP64 else 1e-5)
+
+
+def test_points_on_faces_and_vertices():
+    """Particles exactly on cell faces and vertices (xi in {0, 1/2, 1}):
+    the part index of xi = 1 is the extra index q (the phantom part of the
+    reference's census), own-cell tests at the boundary of the unit square,
+    relocations that start on a shared face; bitwise equal to the oracle."""
+    m = M.quad_2d(8, 6, (0.0, 0.0), (1.6, 1.2), jitter=0.15, seed=62)
+    V = M.uniform_field(m, [0.25, 0.15])
+    vals = np.array([0.0, 0.5, 1.0])
+    gx, gy = np.meshgrid(vals, vals, indexing="ij")
+    pts = np.stack([gx.ravel(), gy.ravel()], axis=1)
+    cell = np.repeat(np.arange(m.n_cells, dtype=np.int32), len(pts))
+    xi = np.tile(pts, (m.n_cells, 1))
+    g = ParticleStage(m, 0.02, 5, 2, [2, 2], FP64)
+    o = Oracle(m, 0.02, 5, 2, [2, 2])
+    for s in (g, o):
+        s.upload_fields(V, V)
+        s.insert_particles(cell, xi)
+    _run_steps(g, o, lambda s: V, 3, "faces and vertices")
+
+
 def test_run_twice_bitwise():
     m = M.cylinder_channel(0.1, seed=21)
     V = M.cylinder_flow(m, 1.0, eps=0.02, seed=22)
