@@ -141,8 +141,6 @@ def kernel_bytes(name, info, m, dim=2):
         wx = 8 if base == "k_search_resume" else 4
         per = 8 + 4 + 4 + dim * wx * (2 if base == "k_search_resume" else 1) + 8 * dim + 8 + dim * wx
         return int(q[r] * per) if r < len(q) else None
-    if name == "k_group":
-        return n0 * 8 + arr * 8 + nc * 4
     if name == "k_cell_count":
         return (n0 + arr) * 12 + arr * 4 + n0 * 4 + nc * (8 + 4 + 4 + 20)
     if name in ("k_cell_write", "k_cell_write32"):

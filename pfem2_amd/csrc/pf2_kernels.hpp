@@ -13,13 +13,14 @@
 //                  own-cell test; such particles are parked in a queue.
 //   k_search_resume  MOVE_ROUNDS + 1 rounds over the queue: vertex-star
 //                  search in full wavefronts, then the remaining substeps.
-//   k_group        particles that changed cell, grouped by destination cell.
-//   k_cell_count   one wavefront per cell: part counts over stayers +
+//                  A particle that ends in another cell takes a slot in that
+//                  cell's arrival row (add_arrival).
+//   k_cell_count   one 16-lane group per cell: part counts over stayers +
 //                  arrivals, survivors (MAX smallest ids per part), empty
-//                  parts to reseed, final count.
-//   k_cell_write   one wavefront per cell: survivors and new particles into
-//                  the other buffer at their final positions.
-//   k_cell_partials one wavefront per cell: per-cell projection sums in
+//                  parts to reseed, final count (k_cell_big: larger cells).
+//   k_cell_write   one 16-lane group per cell: survivors and new particles
+//                  into the other buffer at their final positions.
+//   k_cell_partials one group per cell: per-cell projection sums in
 //                  ascending id order.
 //   k_node         one thread per DoF: sum of the cell partials in ascending
 //                  cell order, then division by the weight.

@@ -36,6 +36,9 @@ int fail(int code, const std::string& msg) {
             return fail(PF2_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));        \
     } while (0)
 
+// Timing ids of pf2_kernel_times / pf2_kernel_name.  The ids of kernels that
+// no longer exist (K_GROUP, K_QPREFIX) stay reserved, so that a caller's
+// numbering does not shift; they report no launches.
 enum KernelId {
     K_MOVE = 0, K_GROUP, K_CELL_COUNT, K_CELL_WRITE, K_SCAN, K_FINALIZE, K_PARTIALS, K_NODE, K_WALL,
     K_SEED, K_MISC, K_MOVE32, K_CELL_WRITE32, K_PARTIALS32, K_SEARCH, K_SEARCH32, K_QPREFIX,
