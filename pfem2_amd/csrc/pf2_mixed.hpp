@@ -315,22 +315,58 @@ __device__ __forceinline__ int part_key32(const DevParams& pp, const float* xi) 
     return part_key<DIM>(pp, xd);
 }
 
+// multilinear coefficients of the nodal values of a cell (in place):
+//   2-D  u = c0 + c1 xi0 + c2 xi1 + c3 xi0 xi1
+//   3-D  u = c0 + c1 xi0 + c2 xi1 + c3 xi2 + c4 xi0 xi1 + c5 xi0 xi2 + c6 xi1 xi2 + c7 xi0 xi1 xi2
+// so that a substep evaluates u with DIM nested FMA chains instead of the
+// shape functions and a sum over the vertices
+template <int DIM>
+__device__ __forceinline__ void to_coeffs32(float (*c)[DIM]) {
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+        const float v0 = c[0][d], v1 = c[1][d], v2 = c[2][d], v3 = c[3][d];
+        c[1][d] = v1 - v0;
+        c[2][d] = v2 - v0;
+        c[3][d] = (v3 - v2) - (v1 - v0);
+        if constexpr (DIM == 3) {
+            const float v4 = c[4][d], v5 = c[5][d], v6 = c[6][d], v7 = c[7][d];
+            c[3][d] = v4 - v0;
+            c[4][d] = (v3 - v2) - (v1 - v0);
+            c[5][d] = (v5 - v4) - (v1 - v0);
+            c[6][d] = (v6 - v4) - (v2 - v0);
+            c[7][d] = ((v7 - v6) - (v5 - v4)) - ((v3 - v2) - (v1 - v0));
+        }
+    }
+}
+
+template <int DIM>
+__device__ __forceinline__ float eval_coeffs32(const float (*c)[DIM], int d, const float* xi) {
+    if constexpr (DIM == 2)
+        return __fmaf_rn(__fmaf_rn(c[3][d], xi[0], c[2][d]), xi[1], __fmaf_rn(c[1][d], xi[0], c[0][d]));
+    else
+        return __fmaf_rn(__fmaf_rn(__fmaf_rn(c[7][d], xi[0], c[6][d]), xi[1], __fmaf_rn(c[5][d], xi[0], c[3][d])),
+                         xi[2],
+                         __fmaf_rn(__fmaf_rn(c[4][d], xi[0], c[2][d]), xi[1], __fmaf_rn(c[1][d], xi[0], c[0][d])));
+}
+
 // FP32 substeps until done (0) or the reference point leaves the unit cell (1);
-// u (the last velocity) is returned for the relocation
+// u (the last velocity) is returned for the relocation.  Vc: the cell's nodal
+// velocities.
 template <int DIM>
 __device__ __forceinline__ int advance32(const DevParams& pp, float h32, int i, double* __restrict__ pvext,
                                          long long cap, float* xi, float* u, int& s, const float* g,
                                          const float (*Vc)[DIM]) {
     constexpr int NV = Q1<DIM>::NV;
+    float cf[NV][DIM];
+#pragma unroll
+    for (int k = 0; k < NV; ++k)
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) cf[k][d] = Vc[k][d];
+    to_coeffs32<DIM>(cf);
     for (; s < pp.substeps; ++s) {
-        float N[NV], dx[DIM], dl[DIM];
-        shape32<DIM>(xi, N);
+        float dx[DIM], dl[DIM];
 #pragma unroll
-        for (int d = 0; d < DIM; ++d) u[d] = 0.0f;
-#pragma unroll
-        for (int k = 0; k < NV; ++k)
-#pragma unroll
-            for (int d = 0; d < DIM; ++d) u[d] = __fmaf_rn(N[k], Vc[k][d], u[d]);
+        for (int d = 0; d < DIM; ++d) u[d] = eval_coeffs32<DIM>(cf, d, xi);
 #pragma unroll
         for (int d = 0; d < DIM; ++d) dx[d] = u[d] * h32;
         if (pp.keep_vext && s == pp.substeps - 1 && pvext)
