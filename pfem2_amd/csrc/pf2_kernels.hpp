@@ -834,6 +834,7 @@ __device__ __forceinline__ void for_overflow(const DevWork& w, int c, int lane, 
     }
 }
 
+
 // ---------------------------------------------------------------------------
 // Cell kernels.  A cell is handled by a group of GS = 16 lanes, so a
 // wavefront works on four cells at once (the per-cell work is a short chain
@@ -842,6 +843,30 @@ constexpr int GS = 16;
 constexpr int GROUPS = 256 / GS;     // groups per 256-thread block
 constexpr int GCAP = 64;             // candidates staged in LDS per group (lattices of up to 16 parts)
 constexpr int GCAP_L = 128;          // the same for larger lattices
+
+// put(particle, position) for each survivor of cell c as the census left
+// them: listed in id order in the cell's slot row when they fit, else by
+// the position the census stored per candidate (the old segment's stayers,
+// the slots and the overflow list)
+template <class F>
+__device__ __forceinline__ void survivors_of(const DevWork& w, const int* old_off, int c, int n_st, int kin,
+                                             int nsurv, int gl, F put) {
+    if (nsurv <= w.aslots) {
+        for (int t = gl; t < nsurv; t += GS) put(w.arr_slot[(size_t)c * w.aslots + t], t);
+        return;
+    }
+    for (int j = gl; j < kin; j += GS) {
+        const int idx = candidate(w, old_off, c, n_st, j);
+        // arrivals are in this cell's slots because their destination is c
+        if (j < n_st && w.dest[idx] != c) continue;
+        const int pos = w.spos[idx];
+        if (pos >= 0) put(idx, pos);
+    }
+    for_overflow(w, c, gl, GS, [&](int, int idx) {
+        const int pos = w.spos[idx];
+        if (pos >= 0) put(idx, pos);
+    });
+}
 
 __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -950,15 +975,19 @@ __global__ void __launch_bounds__(256) k_cell_count(int n_cells, const int* __re
             w.big[1 + slot] = c;
         }
         wave_sync();
+        nsurv = group_sum(nsurv);
         // positions of the survivors in ascending id order (ids are below
-        // 2^32 - 1, the fill value)
+        // 2^32 - 1, the fill value): the survivors in that order replace the
+        // cell's arrival slots (already staged) when they fit, else each
+        // candidate gets its position (-1: dropped)
+        const bool listed = nsurv <= w.aslots;
         if (small)
             for (int j = gl; j < k; j += GS) {
                 const int idx = s_idx[g][j];
                 if (idx < 0) continue;
                 const uint32_t id = s_sid[g][j];
                 if (id == ~0u) {
-                    w.spos[idx] = -1;
+                    if (!listed) w.spos[idx] = -1;
                     continue;
                 }
                 int pos = 0;
@@ -966,12 +995,12 @@ __global__ void __launch_bounds__(256) k_cell_count(int n_cells, const int* __re
                     const uint4 v = *reinterpret_cast<const uint4*>(&s_sid[g][t]);
                     pos += (v.x < id ? 1 : 0) + (v.y < id ? 1 : 0) + (v.z < id ? 1 : 0) + (v.w < id ? 1 : 0);
                 }
-                w.spos[idx] = pos;
+                if (listed) w.arr_slot[(size_t)c * w.aslots + pos] = idx;
+                else w.spos[idx] = pos;
             }
         wave_sync();
         hit = group_or(hit);
         nvalid = group_sum(nvalid);
-        nsurv = group_sum(nsurv);
         if (cv && small && gl == 0) {
             const int spawns = reseed ? __popcll(all & ~hit) : 0;
             w.hitmask[c] = reseed ? hit : all;
@@ -1072,12 +1101,14 @@ __global__ void __launch_bounds__(256) k_cell_big(const int* __restrict__ old_of
         if (nvalid) atomicAdd(&s_nvalid, nvalid);
         if (nsurv) atomicAdd(&s_nsurv, nsurv);
         __syncthreads();
+        // survivors listed in the cell's (staged) arrival slots when they fit
+        const bool listed = s_nsurv <= w.aslots;
         for (int j = tid; j < k; j += blockDim.x) {
             const int idx = s_idx[j];
             if (idx < 0) continue;
             const uint32_t id = s_sid[j];
             if (id == ~0u) {
-                w.spos[idx] = -1;
+                if (!listed) w.spos[idx] = -1;
                 continue;
             }
             int pos = 0;
@@ -1085,7 +1116,8 @@ __global__ void __launch_bounds__(256) k_cell_big(const int* __restrict__ old_of
                 const uint4 v = *reinterpret_cast<const uint4*>(&s_sid[t]);
                 pos += (v.x < id ? 1 : 0) + (v.y < id ? 1 : 0) + (v.z < id ? 1 : 0) + (v.w < id ? 1 : 0);
             }
-            w.spos[idx] = pos;
+            if (listed) w.arr_slot[(size_t)c * w.aslots + pos] = idx;
+            else w.spos[idx] = pos;
         }
         if (tid == 0) {
             const unsigned long long hm = s_hit;
@@ -1123,9 +1155,7 @@ __device__ __forceinline__ void cell_write_one(const DevMesh<DIM>& m, const doub
         if (gl == 0) atomicOr(&w.counters[2], 1);
         return;
     }
-    auto put = [&](int idx) {
-        const int pos = w.spos[idx];
-        if (pos < 0) return;
+    auto put = [&](int idx, int pos) {
         const long long o = out0 + pos;
         B.cell[o] = c;
         B.id[o] = A.id[idx];
@@ -1139,13 +1169,7 @@ __device__ __forceinline__ void cell_write_one(const DevMesh<DIM>& m, const doub
 #pragma unroll
             for (int d = 0; d < DIM; ++d) B.vext[(size_t)d * B.cap + o] = A.vext[(size_t)d * A.cap + idx];
     };
-    for (int j = gl; j < kin; j += GS) {
-        const int idx = candidate(w, old_off, c, n_st, j);
-        // arrivals are in this cell's slots because their destination is c
-        if (j < n_st && w.dest[idx] != c) continue;
-        put(idx);
-    }
-    for_overflow(w, c, gl, GS, [&](int, int idx) { put(idx); });
+    survivors_of(w, old_off, c, n_st, kin, nsurv, gl, put);
     const int spawns = nfinal - nsurv;
     if (spawns > 0) {
         const unsigned long long all = pp.nparts_in >= 64 ? ~0ull : ((1ull << pp.nparts_in) - 1);

@@ -635,9 +635,7 @@ __device__ __forceinline__ void cell_write32_one(const DevMesh<DIM>& m, const do
         if (gl == 0) atomicOr(&w.counters[2], 1);
         return;
     }
-    auto put = [&](int idx) {
-        const int pos = w.spos[idx];
-        if (pos < 0) return;
+    auto put = [&](int idx, int pos) {
         const long long o = out0 + pos;
         bcell[o] = c;
         bid[o] = aid[idx];
@@ -652,13 +650,7 @@ __device__ __forceinline__ void cell_write32_one(const DevMesh<DIM>& m, const do
 #pragma unroll
             for (int d = 0; d < DIM; ++d) bvext[(size_t)d * cap + o] = avext[(size_t)d * cap + idx];
     };
-    for (int j = gl; j < kin; j += GS) {
-        const int idx = candidate(w, old_off, c, n_st, j);
-        // arrivals are in this cell's slots because their destination is c
-        if (j < n_st && w.dest[idx] != c) continue;
-        put(idx);
-    }
-    for_overflow(w, c, gl, GS, [&](int, int idx) { put(idx); });
+    survivors_of(w, old_off, c, n_st, kin, nsurv, gl, put);
     const int spawns = nfinal - nsurv;
     if (spawns > 0) {
         const unsigned long long all = pp.nparts_in >= 64 ? ~0ull : ((1ull << pp.nparts_in) - 1);
