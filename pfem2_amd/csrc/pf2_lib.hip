@@ -224,9 +224,13 @@ struct Timed {
 };
 
 // cell kernels: 16 (or 8) cells per 256-thread block, blocks looping over cells
-int cell_grid(int n_cells) {
+// workgroups of a cell kernel (16 cells per workgroup, grid-stride beyond
+// `cap` workgroups).  The census runs with a cap of 8192 (one pass over the
+// ~100k cells of the cylinder workload: each wavefront's chain of dependent
+// reads runs once), the write and sum passes with 4096 (measured fastest)
+int cell_grid(int n_cells, int cap = 4096) {
     int g = (n_cells + 15) / 16;
-    return g < 2048 ? (g < 1 ? 1 : g) : 2048;
+    return g < cap ? (g < 1 ? 1 : g) : cap;
 }
 
 int pack_fields(pf2_handle_s* h);
@@ -458,10 +462,10 @@ int regroup(pf2_handle_s* h, int reseed, bool fuse_sums = false) {
     {
         Timed t(h, K_CELL_COUNT);
         if (pp.nparts_in <= 16)
-            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cell_count<DIM, GCAP>), dim3(cell_grid(nc)), dim3(256), 0,
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cell_count<DIM, GCAP>), dim3(cell_grid(nc, 8192)), dim3(256), 0,
                                h->stream, nc, old_off, h->buf[h->cur].id, pp, w, reseed);
         else
-            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cell_count<DIM, GCAP_L>), dim3(cell_grid(nc)), dim3(256), 0,
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cell_count<DIM, GCAP_L>), dim3(cell_grid(nc, 8192)), dim3(256), 0,
                                h->stream, nc, old_off, h->buf[h->cur].id, pp, w, reseed);
     }
     {
