@@ -501,8 +501,11 @@ int run_move(pf2_handle_s* h, int do_correct, int do_move, bool fuse_sums = fals
     DevParams pp = dev_params(h);
     DevWork w = dev_work(h);
     DevQueues qu = dev_queues(h);
-    const int grid = grid_for(h->n_est, 256, 256 * 16);
-    const int qgrid = grid_for(h->n_est / 2 + 1, 256, 256 * 8);
+    // the move: up to 8192 workgroups, about one wavefront-sized chunk per
+    // wavefront on the cylinder workload (a smaller grid looping over
+    // several chunks was 1% slower); the searches: up to 2048
+    const int grid = grid_for(h->n_est, 256, 8192);
+    const int qgrid = grid_for(h->n_est / 2 + 1, 256, 2048);
     if (mixed(h)) {
         h->fp64_valid = false;
         PBufs& b = h->buf[h->cur];
