@@ -406,9 +406,15 @@ __device__ __forceinline__ void park32(const Rec32& R, long long cap, const DevW
     queue_push(qu, round_out, park, make_int2(i, c), shard);
     if (!park) return;
     qu.park_s[i] = s;
-    rec_store_xi<DIM>(R, i, xi);
+    if constexpr (DIM == 2) {
+        // 2-D: reference point and velocity_ext in one 16-byte record (the
+        // FP64 parking array has room for it), read back by the search
+        reinterpret_cast<float4*>(qu.park_vext)[i] = make_float4(xi[0], xi[1], u[0], u[1]);
+    } else {
+        rec_store_xi<DIM>(R, i, xi);
 #pragma unroll
-    for (int d = 0; d < DIM; ++d) qu.park_vext[(size_t)d * cap + i] = (double)u[d];
+        for (int d = 0; d < DIM; ++d) qu.park_vext[(size_t)d * cap + i] = (double)u[d];
+    }
 }
 
 template <int DIM>
@@ -484,9 +490,17 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DIM ==
         int st = 0;
         float xi[DIM], u[DIM];
         if (valid) {
-            rec_load_xi<DIM>(R, i, xi);
+            if constexpr (DIM == 2) {
+                const float4 pk = reinterpret_cast<const float4*>(qu.park_vext)[i];
+                xi[0] = pk.x;
+                xi[1] = pk.y;
+                u[0] = pk.z;
+                u[1] = pk.w;
+            } else {
+                rec_load_xi<DIM>(R, i, xi);
 #pragma unroll
-            for (int d = 0; d < DIM; ++d) u[d] = (float)qu.park_vext[(size_t)d * cap + i];
+                for (int d = 0; d < DIM; ++d) u[d] = (float)qu.park_vext[(size_t)d * cap + i];
+            }
             // 2-D: the found cell's vertices and FP32 data are loaded with the
             // candidate and reused; 3-D reloads them (register budget)
             double P[NV][DIM];
