@@ -258,6 +258,9 @@ __device__ __forceinline__ void mv32(const float (*M)[DIM], const float* v, floa
 #ifndef PF2_LS_TOL2
 #define PF2_LS_TOL2 1e-12f
 #endif
+#ifndef PF2_LS_TOL
+#define PF2_LS_TOL 1e-6f
+#endif
 template <int DIM>
 __device__ __forceinline__ bool local_step(const float* g, const float* xi, const float* dx, float* dl) {
     if constexpr (DIM == 2) {
@@ -266,11 +269,11 @@ __device__ __forceinline__ bool local_step(const float* g, const float* xi, cons
         // with e = J^-1 dx and w = J^-1 c (three instructions per iteration
         // instead of nine).  The change of a fourth iteration is the
         // residual J^-1 (dx - F(xi + delta) + F(xi)); the step returns that
-        // fourth iterate when the change is within 1e-6 |e|.
+        // fourth iterate when the change is within 1e-6 |e| (1-norms).  No
+        // early exit: with det <= 0 the result is discarded.
         const float J00 = __fmaf_rn(g[4], xi[1], g[0]), J10 = __fmaf_rn(g[5], xi[1], g[1]);
         const float J01 = __fmaf_rn(g[4], xi[0], g[2]), J11 = __fmaf_rn(g[5], xi[0], g[3]);
         const float det = __fmaf_rn(J00, J11, -(J01 * J10));
-        if (!(det > 0.0f)) return false;
         const float r = __builtin_amdgcn_rcpf(det);
         const float e0 = __fmaf_rn(J11, dx[0], -(J01 * dx[1])) * r;
         const float e1 = __fmaf_rn(J00, dx[1], -(J10 * dx[0])) * r;
@@ -285,11 +288,10 @@ __device__ __forceinline__ bool local_step(const float* g, const float* xi, cons
         }
         const float p = d0 * d1;
         const float n0 = __fmaf_rn(-w0, p, e0), n1 = __fmaf_rn(-w1, p, e1);
-        const float r0 = n0 - d0, r1 = n1 - d1;
-        const float r2 = __fmaf_rn(r1, r1, r0 * r0), e2 = __fmaf_rn(e1, e1, e0 * e0);
+        const float r1n = fabsf(n0 - d0) + fabsf(n1 - d1), e1n = fabsf(e0) + fabsf(e1);
         dl[0] = n0;
         dl[1] = n1;
-        return r2 <= __fmaf_rn(PF2_LS_TOL2, e2, 1e-36f);
+        return (det > 0.0f) & (r1n <= __fmaf_rn(PF2_LS_TOL, e1n, 1e-30f));
     }
     float J[DIM][DIM], Ji[DIM][DIM];
     jac32<DIM>(g, xi, J);
@@ -406,7 +408,15 @@ __device__ __forceinline__ int advance32(const DevParams& pp, float h32, int i, 
             for (int d = 0; d < DIM; ++d) pvext[(size_t)d * cap + i] = (double)u[d];
         bool in = false;
         float e[DIM];
-        if (local_step<DIM>(g, xi, dx, dl)) {
+        if constexpr (DIM == 2) {
+            // one exit per substep (the 2-D step has no early exit)
+            in = local_step<DIM>(g, xi, dx, dl);
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) {
+                e[d] = xi[d] + dl[d];
+                in = in & (e[d] >= 0.0f) & (e[d] <= 1.0f);
+            }
+        } else if (local_step<DIM>(g, xi, dx, dl)) {
             in = true;
 #pragma unroll
             for (int d = 0; d < DIM; ++d) {
