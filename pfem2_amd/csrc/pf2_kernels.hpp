@@ -156,6 +156,7 @@ struct DevParams {
     int substeps;
     int max_per_part;
     int q[3];
+    double hq[3];       // 1.0 / q[d] (the part width, correctly rounded)
     int nparts_in;      // q0*q1[*q2]
     int keep_vext;
 };
@@ -245,7 +246,7 @@ __device__ __forceinline__ int part_key(const DevParams& pp, const double* xi) {
     bool in = true;
 #pragma unroll
     for (int d = 0; d < DIM; ++d) {
-        const double h = 1.0 / pp.q[d];
+        const double h = pp.hq[d];
         const unsigned num = (unsigned)(xi[d] / h);
         key += (int)num * stride;
         stride *= pp.q[d] + 1;
@@ -269,7 +270,7 @@ __device__ __forceinline__ void lattice_point(const DevParams& pp, int ordinal, 
     }
 #pragma unroll
     for (int d = 0; d < DIM; ++d) {
-        const double h = 1.0 / pp.q[d];
+        const double h = pp.hq[d];
         xi[d] = (num[d] + 1.0 / 2) * h;
     }
 }
@@ -545,10 +546,15 @@ struct WaveLoop {
     int begin, end, step, shard;
 };
 
+// Both loops work in 32-bit unsigned arithmetic on wavefront-uniform values
+// (grids of at most 2^16 workgroups, n < 2^31), which the compiler keeps in
+// scalar registers: a 64-bit division per wavefront costs more vector
+// instructions than a substep.
+
 // grid-stride: wavefront k takes chunks k, k + W, ... of WAVE items
 __device__ __forceinline__ WaveLoop stride_loop(int n) {
-    const long long wpb = blockDim.x / WAVE, nw = gridDim.x * wpb;
-    const long long k = blockIdx.x * wpb + threadIdx.x / WAVE;
+    const unsigned wpb = blockDim.x / WAVE, nw = gridDim.x * wpb;
+    const unsigned k = blockIdx.x * wpb + (unsigned)__builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
     WaveLoop L;
     L.begin = (int)(k * WAVE);
     L.end = n;
@@ -560,13 +566,14 @@ __device__ __forceinline__ WaveLoop stride_loop(int n) {
 // one contiguous range of whole workgroup-sized chunks per workgroup, the
 // wavefronts of a workgroup interleaved within it
 __device__ __forceinline__ WaveLoop range_loop(int n) {
-    const long long g = gridDim.x, b = blockIdx.x;
-    const long long per = ((n + g - 1) / g + blockDim.x - 1) / blockDim.x * blockDim.x;
-    const long long b0 = b * per < n ? b * per : n;
-    const long long wpb = blockDim.x / WAVE, wv = threadIdx.x / WAVE;
+    const unsigned g = gridDim.x, b = blockIdx.x, un = (unsigned)n;
+    const unsigned per = ((un + g - 1) / g + blockDim.x - 1) / blockDim.x * blockDim.x;
+    const unsigned long long b0l = (unsigned long long)b * per;
+    const unsigned b0 = b0l < un ? (unsigned)b0l : un;
+    const unsigned wpb = blockDim.x / WAVE, wv = (unsigned)__builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
     WaveLoop L;
     L.begin = (int)(b0 + wv * WAVE);
-    L.end = (int)(b0 + per < n ? b0 + per : n);
+    L.end = (int)(un - b0 > per ? b0 + per : un);
     L.step = blockDim.x;
     L.shard = (int)((b * wpb + wv) * QSHARDS / (g * wpb));
     return L;

@@ -311,6 +311,7 @@ DevParams dev_params(pf2_handle_s* h) {
     p.max_per_part = h->params.max_per_part;
     p.nparts_in = 1;
     for (int d = 0; d < 3; ++d) p.q[d] = d < h->dim ? h->params.quantities[d] : 1;
+    for (int d = 0; d < 3; ++d) p.hq[d] = 1.0 / p.q[d];
     for (int d = 0; d < h->dim; ++d) p.nparts_in *= p.q[d];
     p.keep_vext = h->params.keep_velocity_ext;
     return p;
