@@ -471,7 +471,7 @@ int regroup(pf2_handle_s* h, int reseed, bool fuse_sums = false) {
     }
     {
         Timed t(h, K_CELL_BIG);
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cell_big<DIM>), dim3(1024), dim3(256), 0, h->stream, old_off,
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cell_big<DIM>), dim3(256), dim3(256), 0, h->stream, old_off,
                            h->buf[h->cur].id, pp, w, reseed);
     }
     if (int r = scan(h, h->fs_cnt, h->fs_off, nc + 1)) return r;
