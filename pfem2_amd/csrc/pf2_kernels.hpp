@@ -915,11 +915,12 @@ template <int DIM, int CAP>
 __global__ void __launch_bounds__(256) k_cell_count(int n_cells, const int* __restrict__ old_off,
                                                     const uint32_t* __restrict__ pid, DevParams pp, DevWork w,
                                                     int reseed) {
-    // Staged rows: candidate index; (part key << 32 | id), all ones for a
-    // candidate that left; id of a survivor, all ones otherwise.  Rows are
-    // padded to a multiple of 4 entries with those fill values, so the scans
-    // below read 16 bytes at a time, and their strides put the four groups of
-    // a wavefront on different LDS banks.
+    // Staged rows, compacted to the cell's valid candidates (stayers whose
+    // destination is c, then the arrivals): candidate index; part key << 32 |
+    // id; id of a survivor, all ones otherwise.  Rows are padded to a
+    // multiple of 4 entries with fill values (key all ones, id all ones), so
+    // the scans below read 16 bytes at a time, and their strides put the four
+    // groups of a wavefront on different LDS banks.
     __shared__ int s_idx[GROUPS][CAP + 4];
     __shared__ __align__(16) unsigned long long s_kid[GROUPS][CAP + 2];
     __shared__ __align__(16) uint32_t s_sid[GROUPS][CAP + 4];
@@ -934,11 +935,13 @@ __global__ void __launch_bounds__(256) k_cell_count(int n_cells, const int* __re
         const int k = n_st + na;
         const bool small = k <= CAP && na <= w.aslots;
         if (cv && gl == 0) arrivals += na;
-        const int kr = (k + 3) & ~3;
         unsigned long long hit = 0;
-        int nvalid = 0, nsurv = 0;
-        if (small)
-            for (int j = gl; j < kr; j += GS) {
+        int nv = 0, nsurv = 0;   // nv: valid candidates, the same in all lanes of the group
+        if (small) {
+            // the group's lanes in its wavefront's ballot start at bit sh
+            const int sh = (threadIdx.x & (WAVE - 1)) & ~(GS - 1);
+            for (int j0 = 0; j0 < k; j0 += GS) {
+                const int j = j0 + gl;
                 int idx = -1;
                 unsigned long long kid = ~0ull;
                 if (j < k) {
@@ -949,32 +952,41 @@ __global__ void __launch_bounds__(256) k_cell_count(int n_cells, const int* __re
                         kid = ((unsigned long long)(uint32_t)w.pkey[cidx] << 32) | pid[cidx];
                     }
                 }
-                s_idx[g][j] = idx;
-                s_kid[g][j] = kid;
-                s_sid[g][j] = ~0u;
+                const unsigned mask = (unsigned)(__ballot(idx >= 0) >> sh) & 0xffffu;
+                if (idx >= 0) {
+                    const int at = nv + __popc(mask & ((1u << gl) - 1u));
+                    s_idx[g][at] = idx;
+                    s_kid[g][at] = kid;
+                    s_sid[g][at] = ~0u;
+                }
+                nv += __popc(mask);
             }
+            if (gl < ((nv + 3) & ~3) - nv) {
+                s_kid[g][nv + gl] = ~0ull;
+                s_sid[g][nv + gl] = ~0u;
+            }
+        }
+        const int nvr = (nv + 3) & ~3;
         wave_sync();
         if (small)
-            for (int j = gl; j < k; j += GS) {
-                if (s_idx[g][j] < 0) continue;
-                ++nvalid;
+            for (int j = gl; j < nv; j += GS) {
                 const unsigned long long me = s_kid[g][j];
-                const int key = (int)(me >> 32);
-                const int o = part_ordinal(key);
+                const uint32_t mk = (uint32_t)(me >> 32), mi = (uint32_t)me;
+                const int o = part_ordinal((int)mk);
                 if (o >= 0) hit |= 1ull << o;
-                // rank = particles of the same part with a smaller id
+                // rank = particles of the same part with a smaller id (fill
+                // entries carry no valid key)
                 int rank = 0;
                 if (reseed) {
-                    const unsigned long long lo = me & 0xffffffff00000000ull;
-                    for (int t = 0; t < kr; t += 2) {
+                    for (int t = 0; t < nvr; t += 2) {
                         const ulonglong2 v = *reinterpret_cast<const ulonglong2*>(&s_kid[g][t]);
-                        rank += (v.x >= lo && v.x < me) ? 1 : 0;
-                        rank += (v.y >= lo && v.y < me) ? 1 : 0;
+                        rank += ((uint32_t)(v.x >> 32) == mk && (uint32_t)v.x < mi) ? 1 : 0;
+                        rank += ((uint32_t)(v.y >> 32) == mk && (uint32_t)v.y < mi) ? 1 : 0;
                     }
                 }
                 const bool surv = rank < pp.max_per_part || !reseed;
                 nsurv += surv ? 1 : 0;
-                if (surv) s_sid[g][j] = (uint32_t)me;
+                if (surv) s_sid[g][j] = mi;
             }
         else if (cv && gl == 0) {
             // more than CAP candidates: census by k_cell_big
@@ -989,16 +1001,15 @@ __global__ void __launch_bounds__(256) k_cell_count(int n_cells, const int* __re
         // candidate gets its position (-1: dropped)
         const bool listed = nsurv <= w.aslots;
         if (small)
-            for (int j = gl; j < k; j += GS) {
+            for (int j = gl; j < nv; j += GS) {
                 const int idx = s_idx[g][j];
-                if (idx < 0) continue;
                 const uint32_t id = s_sid[g][j];
                 if (id == ~0u) {
                     if (!listed) w.spos[idx] = -1;
                     continue;
                 }
                 int pos = 0;
-                for (int t = 0; t < kr; t += 4) {
+                for (int t = 0; t < nvr; t += 4) {
                     const uint4 v = *reinterpret_cast<const uint4*>(&s_sid[g][t]);
                     pos += (v.x < id ? 1 : 0) + (v.y < id ? 1 : 0) + (v.z < id ? 1 : 0) + (v.w < id ? 1 : 0);
                 }
@@ -1007,13 +1018,12 @@ __global__ void __launch_bounds__(256) k_cell_count(int n_cells, const int* __re
             }
         wave_sync();
         hit = group_or(hit);
-        nvalid = group_sum(nvalid);
         if (cv && small && gl == 0) {
             const int spawns = reseed ? __popcll(all & ~hit) : 0;
             w.hitmask[c] = reseed ? hit : all;
             w.nsurv[c] = nsurv;
             w.fs_cnt[c] = ((unsigned long long)(nsurv + spawns) << 32) | (unsigned)spawns;
-            deleted += nvalid - nsurv;
+            deleted += nv - nsurv;
         }
     }
     deleted = wave_sum(deleted);
@@ -1333,13 +1343,29 @@ __global__ void k_node(DevMesh<DIM> m, const double* __restrict__ partial, doubl
         double acc[DIM], wsum = 0.0;
 #pragma unroll
         for (int d = 0; d < DIM; ++d) acc[d] = 0.0;
-        for (int e = m.dof_off[i]; e < m.dof_off[i + 1]; ++e) {
-            const int ck = m.dof_cell[e];
-            const int c = ck / NV, k = ck - c * NV;
-            const double* p = partial + (size_t)c * NACC + k * (DIM + 1);
+        // the node's cells in chunks of NV: the chunk's cell indices, then
+        // its partials, are loaded together (two dependent levels per chunk
+        // instead of two per cell); the sums keep the cell order
+        const int e0 = m.dof_off[i], e1 = m.dof_off[i + 1];
+        for (int e = e0; e < e1; e += NV) {
+            int ck[NV];
 #pragma unroll
-            for (int d = 0; d < DIM; ++d) acc[d] += p[d];
-            wsum += p[DIM];
+            for (int q = 0; q < NV; ++q) ck[q] = e + q < e1 ? m.dof_cell[e + q] : -1;
+            double pv[NV][DIM + 1];
+#pragma unroll
+            for (int q = 0; q < NV; ++q) {
+                const int c = ck[q] >= 0 ? ck[q] / NV : 0, k = ck[q] >= 0 ? ck[q] - c * NV : 0;
+                const double* p = partial + (size_t)c * NACC + k * (DIM + 1);
+#pragma unroll
+                for (int d = 0; d <= DIM; ++d) pv[q][d] = ck[q] >= 0 ? p[d] : 0.0;
+            }
+#pragma unroll
+            for (int q = 0; q < NV; ++q)
+                if (ck[q] >= 0) {
+#pragma unroll
+                    for (int d = 0; d < DIM; ++d) acc[d] += pv[q][d];
+                    wsum += pv[q][DIM];
+                }
         }
         if (wsum == 0.0) stat_add(stats, 4, 1ull);
 #pragma unroll
