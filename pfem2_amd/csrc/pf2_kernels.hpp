@@ -329,7 +329,28 @@ __device__ __forceinline__ int exit_face(const double (*P)[DIM], const double* x
 #pragma unroll
             for (int i = 0; i < DIM; ++i) J[i][j] += w * P[k][i];
         }
-    invert<DIM>(J, Ji);
+    // J^-1 = adj(J) / det: the face is chosen from values scaled by det, so
+    // no division (the choice is only a guess; a cell with det <= 0 gets an
+    // arbitrary one, which the caller's containment test rejects)
+    if constexpr (DIM == 2) {
+        Ji[0][0] = J[1][1];
+        Ji[0][1] = -J[0][1];
+        Ji[1][0] = -J[1][0];
+        Ji[1][1] = J[0][0];
+    } else {
+        Ji[0][0] = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+        Ji[0][1] = J[0][2] * J[2][1] - J[0][1] * J[2][2];
+        Ji[0][2] = J[0][1] * J[1][2] - J[0][2] * J[1][1];
+        Ji[1][0] = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+        Ji[1][1] = J[0][0] * J[2][2] - J[0][2] * J[2][0];
+        Ji[1][2] = J[0][2] * J[1][0] - J[0][0] * J[1][2];
+        Ji[2][0] = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+        Ji[2][1] = J[0][1] * J[2][0] - J[0][0] * J[2][1];
+        Ji[2][2] = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+    }
+    double dt = 0.0;
+#pragma unroll
+    for (int j = 0; j < DIM; ++j) dt += J[0][j] * Ji[j][0];
 #pragma unroll
     for (int d = 0; d < DIM; ++d) dx[d] = x[d] - xold[d];
     matvec<DIM>(Ji, dx, dxi);
@@ -337,8 +358,8 @@ __device__ __forceinline__ int exit_face(const double (*P)[DIM], const double* x
     double worst = 0.0;
 #pragma unroll
     for (int d = 0; d < DIM; ++d) {
-        const double e = xi_old[d] + dxi[d];
-        const double o = e < 0.0 ? -e : e - 1.0;
+        const double e = xi_old[d] * dt + dxi[d];   // det * (reference coordinate after the move)
+        const double o = e < 0.0 ? -e : e - dt;
         if (o > worst) {
             worst = o;
             f = 2 * d + (e < 0.0 ? 0 : 1);
