@@ -1306,6 +1306,12 @@ __global__ void __launch_bounds__(256) k_cell_partials(int n_cells, const int* _
     const int g = threadIdx.x / PGS, gl = threadIdx.x % PGS;
     const int kk = gl / (DIM + 1);
     const int comp = gl - kk * (DIM + 1);
+    double ft[DIM], fs[DIM];
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+        ft[d] = ((kk >> d) & 1) ? 1.0 : -1.0;
+        fs[d] = ((kk >> d) & 1) ? 0.0 : 1.0;
+    }
     for (int cb = blockIdx.x * NG; cb < n_cells; cb += gridDim.x * NG) {
         const int c = cb + g;
         const bool cv = c < n_cells;
@@ -1328,14 +1334,14 @@ __global__ void __launch_bounds__(256) k_cell_partials(int n_cells, const int* _
             const int cnt = (e - j0) < PGS ? (e - j0) : PGS;
             if (gl < NACC)
                 for (int q = 0; q < cnt; ++q) {
-                    double xi[DIM], N[NV];
+                    // N_k of this lane's vertex alone: factor d is ft_d xi_d +
+                    // fs_d (xi_d, or 1 - xi_d with one rounding), multiplied
+                    // in shape()'s order, so the value is shape()'s N[k] (up
+                    // to the sign of a zero factor, which no sum here sees:
+                    // acc starts at +0)
+                    double nk = fma(ft[0], s_xi[g][q][0], fs[0]);
 #pragma unroll
-                    for (int d = 0; d < DIM; ++d) xi[d] = s_xi[g][q][d];
-                    shape<DIM>(xi, N);
-                    double nk = N[0];
-#pragma unroll
-                    for (int r = 1; r < NV; ++r)
-                        if (r == kk) nk = N[r];
+                    for (int d = 1; d < DIM; ++d) nk *= fma(ft[d], s_xi[g][q][d], fs[d]);
                     if (comp < DIM) {
                         double vc = s_v[g][q][0];
 #pragma unroll

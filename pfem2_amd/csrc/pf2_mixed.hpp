@@ -162,6 +162,27 @@ __device__ __forceinline__ void shape32(const float* xi, float* N) {
     }
 }
 
+// N_k alone, for a lane with a fixed vertex k: factor d is t_d xi_d + s_d
+// (xi_d, or 1 - xi_d with one rounding), multiplied in shape32's order, so
+// the value is shape32's N[k]
+template <int DIM>
+__device__ __forceinline__ void shape_one32_coeffs(int k, float* t, float* s) {
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+        const bool one = (k >> d) & 1;
+        t[d] = one ? 1.0f : -1.0f;
+        s[d] = one ? 0.0f : 1.0f;
+    }
+}
+
+template <int DIM>
+__device__ __forceinline__ float shape_one32(const float* t, const float* s, const float* xi) {
+    float val = __fmaf_rn(t[0], xi[0], s[0]);
+#pragma unroll
+    for (int d = 1; d < DIM; ++d) val *= __fmaf_rn(t[d], xi[d], s[d]);
+    return val;
+}
+
 // J(xi) (Jacobian of F_K at the current reference point) and the nonlinear
 // part Q of the local difference:
 //   F(xi + delta) - F(xi) = J(xi) delta + Q(delta)
@@ -644,25 +665,19 @@ __device__ __forceinline__ double cell_sums32(const float4* __restrict__ rec, in
     constexpr int NV = Q1<DIM>::NV;
     const int kk = gl / (DIM + 1);
     const int comp = gl - kk * (DIM + 1);
+    float ft[DIM], fs[DIM];
+    shape_one32_coeffs<DIM>(kk, ft, fs);
     float sum = 0.0f, cmp = 0.0f;
     for (int q = 0; q < n; ++q) {
         const float4 r = rec[q];
         const float xi[2] = {r.x, r.y};
         const float v[2] = {r.z, r.w};
-        float N[NV];
-        shape32<DIM>(xi, N);
-        float nk = N[0];
+        const float nk = shape_one32<DIM>(ft, fs, xi);
+        float vc = 1.0f;   // the weight term (comp == DIM) is N_k itself
 #pragma unroll
-        for (int t = 1; t < NV; ++t)
-            if (t == kk) nk = N[t];
-        float term = nk;
-        if (comp < DIM) {
-            float vc = v[0];
-#pragma unroll
-            for (int d = 1; d < DIM; ++d)
-                if (d == comp) vc = v[d];
-            term = nk * vc;
-        }
+        for (int d = 0; d < DIM; ++d)
+            if (d == comp) vc = v[d];
+        const float term = nk * vc;
         const float y = term - cmp;
         const float tt = sum + y;
         cmp = (tt - sum) - y;
@@ -791,6 +806,8 @@ __global__ void __launch_bounds__(256) k_cell_partials32(int n_cells, const int*
     const int g = threadIdx.x / PGS, gl = threadIdx.x % PGS;
     const int kk = gl / (DIM + 1);
     const int comp = gl - kk * (DIM + 1);
+    float ft[DIM], fs[DIM];
+    shape_one32_coeffs<DIM>(kk, ft, fs);
     for (int cb = blockIdx.x * NG; cb < n_cells; cb += gridDim.x * NG) {
         const int c = cb + g;
         const bool cv = c < n_cells;
@@ -815,22 +832,15 @@ __global__ void __launch_bounds__(256) k_cell_partials32(int n_cells, const int*
             const int cnt = (e - j0) < PGS ? (e - j0) : PGS;
             if (gl < NACC)
                 for (int q = 0; q < cnt; ++q) {
-                    float xi[DIM], N[NV];
+                    float xi[DIM];
 #pragma unroll
                     for (int d = 0; d < DIM; ++d) xi[d] = s_xi[g][q][d];
-                    shape32<DIM>(xi, N);
-                    float nk = N[0];
+                    const float nk = shape_one32<DIM>(ft, fs, xi);
+                    float vc = 1.0f;   // the weight term (comp == DIM) is N_k itself
 #pragma unroll
-                    for (int r = 1; r < NV; ++r)
-                        if (r == kk) nk = N[r];
-                    float term = nk;
-                    if (comp < DIM) {
-                        float vc = s_v[g][q][0];
-#pragma unroll
-                        for (int d = 1; d < DIM; ++d)
-                            if (d == comp) vc = s_v[g][q][d];
-                        term = nk * vc;
-                    }
+                    for (int d = 0; d < DIM; ++d)
+                        if (d == comp) vc = s_v[g][q][d];
+                    const float term = nk * vc;
                     const float y = term - cmp;
                     const float tt = sum + y;
                     cmp = (tt - sum) - y;
