@@ -137,8 +137,10 @@ struct DevMesh {
     const int2* cell_star; // per cell and local vertex: [begin, end) of that vertex's star
     const int4* cell_nbr;  // per cell, the neighbours across faces 0..2*DIM-1 (-1 on the boundary or past
                            // the faces): one int4 in 2-D, two in 3-D
-    const double* cellV;   // per cell: V at its DoFs [NV][DIM] (refreshed when V changes)
-    const double* cellDV;  // per cell: V - V_old at its DoFs [NV][DIM]
+    const double* cellV;   // per cell: V at its DoFs [NV][DIM] (refreshed when V changes); null for
+                           // FP32-mixed handles, whose particle kernels read FP32 copies
+    const double* cellDV;  // per cell: V - V_old at its DoFs [NV][DIM] (FP64 handles)
+    const double* Vn;      // V, node-major [n_dofs][DIM]
 };
 
 struct DevParticles {
@@ -187,19 +189,40 @@ struct DevWork {
     unsigned long long* stats;   // STAT_SHARDS rows: lost, deleted, spawned, searches, zero-weight
 };
 
-// vertex coordinates and nodal velocities of cell c, from the packed per-cell copies
-template <int DIM>
+// vertex coordinates and nodal velocities of cell c, from the packed per-cell
+// copies, or (GATHER: FP32-mixed handles, which keep no FP64 velocity copy)
+// with the velocities gathered from V through the cell's DoF map (the same
+// values).  The choice is a template argument so that the kernels of FP64
+// handles carry no second path.
+template <int DIM, bool GATHER = false>
 __device__ __forceinline__ void load_cell(const DevMesh<DIM>& m, int c, double (*P)[DIM], double (*Fc)[DIM]) {
     constexpr int NV = Q1<DIM>::NV;
     const double* cx = m.cellX + (size_t)c * NV * DIM;
-    const double* cf = m.cellV + (size_t)c * NV * DIM;
 #pragma unroll
     for (int k = 0; k < NV; ++k)
 #pragma unroll
-        for (int d = 0; d < DIM; ++d) {
-            P[k][d] = cx[k * DIM + d];
-            Fc[k][d] = cf[k * DIM + d];
+        for (int d = 0; d < DIM; ++d) P[k][d] = cx[k * DIM + d];
+    if constexpr (GATHER) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) {
+            const double* vf = m.Vn + (size_t)m.cd[(size_t)c * NV + k] * DIM;
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) Fc[k][d] = vf[d];
         }
+    } else {
+        const double* cf = m.cellV + (size_t)c * NV * DIM;
+#pragma unroll
+        for (int k = 0; k < NV; ++k)
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) Fc[k][d] = cf[k * DIM + d];
+    }
+}
+
+// the same, choosing by whether the handle keeps the FP64 copy (seeding)
+template <int DIM>
+__device__ __forceinline__ void load_cell_any(const DevMesh<DIM>& m, int c, double (*P)[DIM], double (*Fc)[DIM]) {
+    if (m.cellV) load_cell<DIM, false>(m, c, P, Fc);
+    else load_cell<DIM, true>(m, c, P, Fc);
 }
 
 // vertex coordinates of cell c (packed per-cell copy: one dependent load level)
@@ -1175,7 +1198,7 @@ __global__ void __launch_bounds__(256) k_cell_big(const int* __restrict__ old_of
 // New particles: reference point at the part centre, location mapped,
 // velocity interpolated from V (cylinder2D/pfem2particle.cpp:533-549), ids
 // continuing the counter in ascending cell, then part, order.
-template <int DIM>
+template <int DIM, bool GATHER>
 __device__ __forceinline__ void cell_write_one(const DevMesh<DIM>& m, const double* __restrict__ V,
                                                const int* __restrict__ old_off, const DevParticles& A,
                                                const DevParticles& B, const DevParams& pp, const DevWork& w, int c,
@@ -1218,7 +1241,7 @@ __device__ __forceinline__ void cell_write_one(const DevMesh<DIM>& m, const doub
             const int r = __popcll(empty & ((1ull << o) - 1));
             double xi[DIM], P[NV][DIM], Vc[NV][DIM], x[DIM], v[DIM];
             lattice_point<DIM>(pp, o, xi);
-            load_cell<DIM>(m, c, P, Vc);
+            load_cell<DIM, GATHER>(m, c, P, Vc);
             forward<DIM>(P, xi, x);
             interp<DIM>(Vc, xi, v);
             const long long q = out0 + nsurv + r;
@@ -1241,13 +1264,14 @@ __device__ __forceinline__ void cell_write_one(const DevMesh<DIM>& m, const doub
     }
 }
 
-template <int DIM>
+// GATHER: an FP32-mixed handle's FP64 regroup (no FP64 per-cell velocity copy)
+template <int DIM, bool GATHER>
 __global__ void __launch_bounds__(256) k_cell_write(DevMesh<DIM> m, const double* __restrict__ V,
                                                     const int* __restrict__ old_off, DevParticles A,
                                                     DevParticles B, DevParams pp, DevWork w) {
     const int g = threadIdx.x / GS, gl = threadIdx.x % GS;
     for (int c = blockIdx.x * GROUPS + g; c < m.n_cells; c += gridDim.x * GROUPS)
-        cell_write_one<DIM>(m, V, old_off, A, B, pp, w, c, gl);
+        cell_write_one<DIM, GATHER>(m, V, old_off, A, B, pp, w, c, gl);
 }
 
 // totals after the scans: particle count, id counter, statistics; resets the
@@ -1512,7 +1536,7 @@ __global__ void k_seed(DevMesh<DIM> m, const double* __restrict__ V, DevParticle
         const int o = (int)(i - (long long)c * pp.nparts_in);
         double xi[DIM], P[NV][DIM], Vc[NV][DIM], x[DIM], v[DIM];
         lattice_point<DIM>(pp, o, xi);
-        load_cell<DIM>(m, c, P, Vc);
+        load_cell_any<DIM>(m, c, P, Vc);
         forward<DIM>(P, xi, x);
         interp<DIM>(Vc, xi, v);
         A.cell[i] = c;
@@ -1540,7 +1564,7 @@ __global__ void k_init_particles(DevMesh<DIM> m, const double* __restrict__ V, D
         double xi[DIM], P[NV][DIM], Vc[NV][DIM], x[DIM], v[DIM];
 #pragma unroll
         for (int d = 0; d < DIM; ++d) xi[d] = A.xi[(size_t)d * A.cap + i];
-        load_cell<DIM>(m, c, P, Vc);
+        load_cell_any<DIM>(m, c, P, Vc);
         forward<DIM>(P, xi, x);
 #pragma unroll
         for (int d = 0; d < DIM; ++d) A.x[(size_t)d * A.cap + i] = x[d];
@@ -1643,7 +1667,7 @@ __global__ void k_refresh(DevMesh<DIM> m, const double* __restrict__ Vcm, double
 #pragma unroll
             for (int d = 0; d < DIM; ++d) {
                 const double a = Vcm[(size_t)d * nd + dof], b = Vold[(size_t)dof * DIM + d];
-                cellV[(size_t)t * DIM + d] = a;
+                if (cellV) cellV[(size_t)t * DIM + d] = a;
                 if (cellDV) cellDV[(size_t)t * DIM + d] = a - b;
                 if (V32) {
                     V32[(size_t)t * DIM + d] = (float)a;
@@ -1664,7 +1688,7 @@ __global__ void k_pack_fields(DevMesh<DIM> m, const double* __restrict__ V, cons
 #pragma unroll
         for (int d = 0; d < DIM; ++d) {
             const double a = V[(size_t)dof * DIM + d], b = Vold[(size_t)dof * DIM + d];
-            cellV[(size_t)t * DIM + d] = a;
+            if (cellV) cellV[(size_t)t * DIM + d] = a;
             if (cellDV) cellDV[(size_t)t * DIM + d] = a - b;
         }
     }

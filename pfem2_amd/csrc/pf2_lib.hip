@@ -274,8 +274,9 @@ DevMesh<DIM> dev_mesh(pf2_handle_s* h) {
     m.cellX = h->dcellX;
     m.cell_star = h->dcell_star;
     m.cell_nbr = h->dcell_nbr;
-    m.cellV = h->dcellV;
+    m.cellV = h->dcellV;     // null for FP32-mixed handles
     m.cellDV = h->dcellDV;
+    m.Vn = h->dV;
     return m;
 }
 
@@ -477,15 +478,20 @@ int regroup(pf2_handle_s* h, int reseed, bool fuse_sums = false) {
     if (int r = scan(h, h->fs_cnt, h->fs_off, nc + 1)) return r;
     if (mixed(h)) {
         Timed t(h, K_CELL_WRITE32);
-        launch_cell_write32<DIM>(h->stream, cell_grid(nc), dev_mesh<DIM>(h), h->dV, old_off, h->buf[h->cur].id,
+        launch_cell_write32<DIM>(h->stream, cell_grid(nc), dev_mesh<DIM>(h), h->dV32, old_off, h->buf[h->cur].id,
                                  rec32(h->buf[h->cur]), h->buf[1 - h->cur].cell, h->buf[1 - h->cur].id,
                                  rec32(h->buf[1 - h->cur]), h->buf[h->cur].vext, h->buf[1 - h->cur].vext, h->cap, pp,
                                  w, fuse_sums, h->partial);
     } else {
         Timed t(h, K_CELL_WRITE);
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cell_write<DIM>), dim3(cell_grid(nc)), dim3(256), 0, h->stream,
-                           dev_mesh<DIM>(h), h->dV, old_off, dev_particles(h, h->cur), dev_particles(h, 1 - h->cur),
-                           pp, w);
+        if (h->dcellV)
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cell_write<DIM, false>), dim3(cell_grid(nc)), dim3(256), 0,
+                               h->stream, dev_mesh<DIM>(h), h->dV, old_off, dev_particles(h, h->cur),
+                               dev_particles(h, 1 - h->cur), pp, w);
+        else
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cell_write<DIM, true>), dim3(cell_grid(nc)), dim3(256), 0,
+                               h->stream, dev_mesh<DIM>(h), h->dV, old_off, dev_particles(h, h->cur),
+                               dev_particles(h, 1 - h->cur), pp, w);
     }
     {
         Timed t(h, K_FINALIZE);
@@ -617,21 +623,23 @@ int upload_field(pf2_handle_s* h, const double* src, double* dst, hipMemcpyKind 
     return 0;
 }
 
-// per-cell copies of V and V - V_old (FP64, and FP32 for the mixed path);
-// called whenever V or V_old change
+// per-cell copies of V and V - V_old (FP64 handles: FP64 copies; FP32-mixed
+// handles: FP32 copies only); called whenever V or V_old change
 int pack_fields(pf2_handle_s* h) {
     h->packed = true;
     const long long n = (long long)h->n_cells * h->nv;
     const int g = grid_for(n, 256, 4096);
     if (h->dim == 2) {
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pack_fields<2>), dim3(g), dim3(256), 0, h->stream, dev_mesh<2>(h), h->dV,
-                           h->dVold, h->dcellV, mixed(h) ? nullptr : h->dcellDV);
+        if (h->dcellV)
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pack_fields<2>), dim3(g), dim3(256), 0, h->stream, dev_mesh<2>(h), h->dV,
+                               h->dVold, h->dcellV, mixed(h) ? nullptr : h->dcellDV);
         if (mixed(h))
             hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pack_fields32<2>), dim3(g), dim3(256), 0, h->stream, dev_mesh<2>(h),
                                h->dV, h->dVold, h->dV32, h->dVold32);
     } else {
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pack_fields<3>), dim3(g), dim3(256), 0, h->stream, dev_mesh<3>(h), h->dV,
-                           h->dVold, h->dcellV, mixed(h) ? nullptr : h->dcellDV);
+        if (h->dcellV)
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pack_fields<3>), dim3(g), dim3(256), 0, h->stream, dev_mesh<3>(h), h->dV,
+                               h->dVold, h->dcellV, mixed(h) ? nullptr : h->dcellDV);
         if (mixed(h))
             hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pack_fields32<3>), dim3(g), dim3(256), 0, h->stream, dev_mesh<3>(h),
                                h->dV, h->dVold, h->dV32, h->dVold32);
@@ -696,7 +704,8 @@ size_t estimate_bytes(int dim, long long nc, long long nvt, long long nd, long l
     size_t b = 0;
     b += nvt * dim * 8 + nck * 4 * 2 + (nvt + 1) * 4 + nck * 4 + nck * dim * 8;       // X, cv, cd, star
     b += nc * 16 * (dim - 1);                                                            // face neighbours
-    b += (nd + 1) * 4 + nck * 4 + nc * 8 + nck * dim * 8 + nck * 8 + nck * dim * 8 * 2;  // dof map, ftol, cellX/star/V/dV
+    b += (nd + 1) * 4 + nck * 4 + nc * 8 + nck * dim * 8 + nck * 8;                    // dof map, ftol, cellX/star
+    if (p->precision != PF2_FP32_MIXED) b += nck * dim * 8 * 2;                          // FP64 cell V / dV copies
     b += nd * dim * 8 * 3;                                                              // V, V_old, staging
     b += (nc + 1) * 4 * 9 + nc * 4 * (1 + arrival_slots(dim, p));                      // per-cell work, arrival slots
     b += nc * 8 + nc * 4 + nc * nv * (dim + 1) * 8 + 32;                                 // census, partials, counters
@@ -908,7 +917,8 @@ int pf2_create_in(const pf2_mesh* mesh, const pf2_params* params, long long capa
         dalloc(&h->dstar_dir, h->star_dir.size()) || dalloc(&h->ddof_off, nd + 1) ||
         dalloc(&h->ddof_cell, h->dof_cell.size()) || dalloc(&h->dftol, nc) || dalloc(&h->dcellX, nck * dim) || dalloc(&h->dcell_star, nck) ||
         dalloc(&h->dcell_nbr, (size_t)h->n_cells * (h->dim - 1)) ||
-        dalloc(&h->dcellV, nck * dim) || dalloc(&h->dcellDV, nck * dim) || dalloc(&h->dV, (size_t)nd * dim) ||
+        (!mixed(h) && (dalloc(&h->dcellV, nck * dim) || dalloc(&h->dcellDV, nck * dim))) ||
+        dalloc(&h->dV, (size_t)nd * dim) ||
         dalloc(&h->dVold, (size_t)nd * dim) || dalloc(&h->dtmp, (size_t)nd * dim) || dalloc(&h->off[0], nc + 1) ||
         dalloc(&h->off[1], nc + 1) || dalloc(&h->arr_cnt, nc + 1) ||
         dalloc(&h->arr_slot, (size_t)nc * h->aslots) || dalloc(&h->ovf_head, nc) || dalloc(&h->fs_cnt, nc + 1) || dalloc(&h->fs_off, nc + 1) || dalloc(&h->hitmask, nc) || dalloc(&h->nsurv, nc) ||

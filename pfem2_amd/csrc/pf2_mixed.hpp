@@ -643,7 +643,7 @@ __device__ __forceinline__ double cell_sums32(const float4* __restrict__ rec, in
 // final records in LDS and writes its projection sums, so no separate
 // partials pass re-reads the particles
 template <int DIM, bool FUSE>
-__device__ __forceinline__ void cell_write32_one(const DevMesh<DIM>& m, const double* __restrict__ V,
+__device__ __forceinline__ void cell_write32_one(const DevMesh<DIM>& m, const float* __restrict__ V32c,
                                                  const int* __restrict__ old_off, const uint32_t* __restrict__ aid,
                                                  const Rec32& A, int* __restrict__ bcell, uint32_t* __restrict__ bid,
                                                  const Rec32& B, const double* __restrict__ avext,
@@ -687,9 +687,14 @@ __device__ __forceinline__ void cell_write32_one(const DevMesh<DIM>& m, const do
         for (int o = gl; o < pp.nparts_in; o += GS) {
             if (!((empty >> o) & 1)) continue;
             const int r = __popcll(empty & ((1ull << o) - 1));
-            double xi[DIM], P[NV][DIM], Vc[NV][DIM], v[DIM];
+            // velocity from the cell's FP32 copy of V (this handle keeps no
+            // FP64 per-cell copy), interpolated in FP64
+            double xi[DIM], Vc[NV][DIM], v[DIM];
             lattice_point<DIM>(pp, o, xi);
-            load_cell<DIM>(m, c, P, Vc);
+#pragma unroll
+            for (int k = 0; k < NV; ++k)
+#pragma unroll
+                for (int d = 0; d < DIM; ++d) Vc[k][d] = (double)V32c[((size_t)c * NV + k) * DIM + d];
             interp<DIM>(Vc, xi, v);
             const long long q = out0 + nsurv + r;
             bcell[q] = c;
@@ -718,7 +723,7 @@ __device__ __forceinline__ void cell_write32_one(const DevMesh<DIM>& m, const do
 }
 
 template <int DIM, bool FUSE>
-__global__ void __launch_bounds__(256) k_cell_write32(DevMesh<DIM> m, const double* __restrict__ V,
+__global__ void __launch_bounds__(256) k_cell_write32(DevMesh<DIM> m, const float* __restrict__ V32c,
                                                      const int* __restrict__ old_off, const uint32_t* __restrict__ aid,
                                                      Rec32 A, int* __restrict__ bcell, uint32_t* __restrict__ bid,
                                                      Rec32 B, const double* __restrict__ avext,
@@ -727,21 +732,21 @@ __global__ void __launch_bounds__(256) k_cell_write32(DevMesh<DIM> m, const doub
     __shared__ float4 s_rec[FUSE ? GROUPS : 1][FUSE ? FCAP : 1];
     const int g = threadIdx.x / GS, gl = threadIdx.x % GS;
     for (int c = blockIdx.x * GROUPS + g; c < m.n_cells; c += gridDim.x * GROUPS)
-        cell_write32_one<DIM, FUSE>(m, V, old_off, aid, A, bcell, bid, B, avext, bvext, cap, pp, w, c, gl,
+        cell_write32_one<DIM, FUSE>(m, V32c, old_off, aid, A, bcell, bid, B, avext, bvext, cap, pp, w, c, gl,
                                     s_rec[FUSE ? g : 0], partial);
 }
 
 // fuse = the projection sums are computed here (2-D; final counts per cell
 // bounded by FCAP, i.e. MAX * prod(q_d + 1) <= FCAP and a census with reseed)
 template <int DIM>
-void launch_cell_write32(hipStream_t s, int grid, DevMesh<DIM> m, const double* V, const int* old_off, const uint32_t* aid,
+void launch_cell_write32(hipStream_t s, int grid, DevMesh<DIM> m, const float* V32c, const int* old_off, const uint32_t* aid,
                          Rec32 A, int* bcell, uint32_t* bid, Rec32 B, const double* avext, double* bvext, long long cap,
                          DevParams pp, DevWork w, bool fuse, double* partial) {
     if (DIM == 2 && fuse)
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cell_write32<DIM, DIM == 2>), dim3(grid), dim3(256), 0, s, m, V, old_off,
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cell_write32<DIM, DIM == 2>), dim3(grid), dim3(256), 0, s, m, V32c, old_off,
                            aid, A, bcell, bid, B, avext, bvext, cap, pp, w, partial);
     else
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cell_write32<DIM, false>), dim3(grid), dim3(256), 0, s, m, V, old_off,
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cell_write32<DIM, false>), dim3(grid), dim3(256), 0, s, m, V32c, old_off,
                            aid, A, bcell, bid, B, avext, bvext, cap, pp, w, partial);
 }
 
