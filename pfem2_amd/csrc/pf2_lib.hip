@@ -551,7 +551,7 @@ int run_project(pf2_handle_s* h, bool sums_done = false) {
     const int nc = h->n_cells;
     DevWork w = dev_work(h);
     if (sums_done) {
-        // the projection sums came with the regroup (k_cell_write32<2, true>)
+        // the projection sums came with the regroup (k_cell_write32<DIM, true>)
     } else if (mixed(h)) {
         Timed t(h, K_PARTIALS32);
         launch_partials32<DIM>(h->stream, cell_grid(nc), nc, h->off[h->cur_off], rec32(h->buf[h->cur]), h->partial);
@@ -750,9 +750,9 @@ int step_body(pf2_handle_s* h, const double* field) {
     }
     // 2-D FP32-mixed steps whose per-cell final count is bounded by the
     // LDS rows compute the projection sums while regrouping
-    const bool fuse = h->dim == 2 && mixed(h) && fuse_bound(h) <= FCAP;
-    if (int r = (h->dim == 2 ? run_move<2>(h, 1, 1, fuse) : run_move<3>(h, 1, 1))) return r;
-    return h->dim == 2 ? run_project<2>(h, fuse) : run_project<3>(h);
+    const bool fuse = mixed(h) && fuse_bound(h) <= FCAP;
+    if (int r = (h->dim == 2 ? run_move<2>(h, 1, 1, fuse) : run_move<3>(h, 1, 1, fuse))) return r;
+    return h->dim == 2 ? run_project<2>(h, fuse) : run_project<3>(h, fuse);
 }
 
 // the step as a HIP graph: captured once per buffer parity, replayed after.

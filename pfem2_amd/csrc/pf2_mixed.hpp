@@ -608,23 +608,38 @@ void launch_search_resume32(hipStream_t s, int grid, DevMesh<DIM> m, const float
                            pcell, R, pvext, cap, pp, w, qu, round);
 }
 
-// rows of the fused projection sums: a cell's final particles, 2-D records
+// rows of the fused projection sums: a cell's final particles (records; in
+// 3-D the second part of each record in a row of its own)
 constexpr int FCAP = 64;
 
 // Kahan-compensated FP32 projection sums of one cell over n staged records
-// in order (lane gl < NACC owns one sum), as k_cell_partials32 computes them
+// in order (accumulator a < NACC: vertex a / (DIM + 1), component a % (DIM +
+// 1), the last one the weight), as k_cell_partials32 computes them
 template <int DIM>
-__device__ __forceinline__ double cell_sums32(const float4* __restrict__ rec, int n, int gl) {
-    constexpr int NV = Q1<DIM>::NV;
-    const int kk = gl / (DIM + 1);
-    const int comp = gl - kk * (DIM + 1);
+__device__ __forceinline__ double cell_sums32(const float4* __restrict__ rec, const float2* __restrict__ recb, int n,
+                                              int a) {
+    const int kk = a / (DIM + 1);
+    const int comp = a - kk * (DIM + 1);
     float ft[DIM], fs[DIM];
     shape_one32_coeffs<DIM>(kk, ft, fs);
     float sum = 0.0f, cmp = 0.0f;
     for (int q = 0; q < n; ++q) {
         const float4 r = rec[q];
-        const float xi[2] = {r.x, r.y};
-        const float v[2] = {r.z, r.w};
+        float xi[DIM], v[DIM];
+        if constexpr (DIM == 2) {
+            xi[0] = r.x;
+            xi[1] = r.y;
+            v[0] = r.z;
+            v[1] = r.w;
+        } else {
+            const float2 rb = recb[q];
+            xi[0] = r.x;
+            xi[1] = r.y;
+            xi[2] = r.z;
+            v[0] = r.w;
+            v[1] = rb.x;
+            v[2] = rb.y;
+        }
         const float nk = shape_one32<DIM>(ft, fs, xi);
         float vc = 1.0f;   // the weight term (comp == DIM) is N_k itself
 #pragma unroll
@@ -639,9 +654,10 @@ __device__ __forceinline__ double cell_sums32(const float4* __restrict__ rec, in
     return (double)sum - (double)cmp;
 }
 
-// FUSE (2-D, final count per cell <= FCAP): the group also stages the cell's
+// FUSE (final count per cell <= FCAP): the group also stages the cell's
 // final records in LDS and writes its projection sums, so no separate
-// partials pass re-reads the particles
+// partials pass re-reads the particles (3-D: each lane of the group owns two
+// of the 32 sums)
 template <int DIM, bool FUSE>
 __device__ __forceinline__ void cell_write32_one(const DevMesh<DIM>& m, const float* __restrict__ V32c,
                                                  const int* __restrict__ old_off, const uint32_t* __restrict__ aid,
@@ -649,7 +665,7 @@ __device__ __forceinline__ void cell_write32_one(const DevMesh<DIM>& m, const fl
                                                  const Rec32& B, const double* __restrict__ avext,
                                                  double* __restrict__ bvext, long long cap, const DevParams& pp,
                                                  const DevWork& w, int c, int gl, float4* __restrict__ srec,
-                                                 double* __restrict__ partial) {
+                                                 float2* __restrict__ srecb, double* __restrict__ partial) {
     constexpr int NV = Q1<DIM>::NV;
     const int n_st = old_off[c + 1] - old_off[c];
     const int na = w.arr_cnt[c];
@@ -671,6 +687,11 @@ __device__ __forceinline__ void cell_write32_one(const DevMesh<DIM>& m, const fl
             const float4 r = A.a[idx];
             B.a[o] = r;
             srec[pos] = r;
+            if constexpr (DIM == 3) {
+                const float2 rb = A.b[idx];
+                B.b[o] = rb;
+                srecb[pos] = rb;
+            }
         } else {
             rec_copy<DIM>(A, idx, B, o);
         }
@@ -707,7 +728,14 @@ __device__ __forceinline__ void cell_write32_one(const DevMesh<DIM>& m, const fl
                 if (pp.keep_vext) bvext[(size_t)d * cap + q] = 0.0;
             }
             rec_store<DIM>(B, q, xf, vf);
-            if constexpr (FUSE) srec[nsurv + r] = make_float4(xf[0], xf[1], vf[0], vf[1]);
+            if constexpr (FUSE) {
+                if constexpr (DIM == 2) {
+                    srec[nsurv + r] = make_float4(xf[0], xf[1], vf[0], vf[1]);
+                } else {
+                    srec[nsurv + r] = make_float4(xf[0], xf[1], xf[2], vf[0]);
+                    srecb[nsurv + r] = make_float2(vf[1], vf[2]);
+                }
+            }
         }
     }
     if (gl == 0) {
@@ -717,7 +745,7 @@ __device__ __forceinline__ void cell_write32_one(const DevMesh<DIM>& m, const fl
     if constexpr (FUSE) {
         constexpr int NACC = PartialsShape<DIM>::NACC;
         wave_sync();
-        if (gl < NACC) partial[(size_t)c * NACC + gl] = cell_sums32<DIM>(srec, nfinal, gl);
+        for (int a = gl; a < NACC; a += GS) partial[(size_t)c * NACC + a] = cell_sums32<DIM>(srec, srecb, nfinal, a);
         wave_sync();
     }
 }
@@ -729,21 +757,23 @@ __global__ void __launch_bounds__(256) k_cell_write32(DevMesh<DIM> m, const floa
                                                      Rec32 B, const double* __restrict__ avext,
                                                      double* __restrict__ bvext, long long cap, DevParams pp,
                                                      DevWork w, double* __restrict__ partial) {
+    constexpr bool FB = FUSE && DIM == 3;
     __shared__ float4 s_rec[FUSE ? GROUPS : 1][FUSE ? FCAP : 1];
+    __shared__ float2 s_recb[FB ? GROUPS : 1][FB ? FCAP : 1];
     const int g = threadIdx.x / GS, gl = threadIdx.x % GS;
     for (int c = blockIdx.x * GROUPS + g; c < m.n_cells; c += gridDim.x * GROUPS)
         cell_write32_one<DIM, FUSE>(m, V32c, old_off, aid, A, bcell, bid, B, avext, bvext, cap, pp, w, c, gl,
-                                    s_rec[FUSE ? g : 0], partial);
+                                    s_rec[FUSE ? g : 0], s_recb[FB ? g : 0], partial);
 }
 
-// fuse = the projection sums are computed here (2-D; final counts per cell
+// fuse = the projection sums are computed here (final counts per cell
 // bounded by FCAP, i.e. MAX * prod(q_d + 1) <= FCAP and a census with reseed)
 template <int DIM>
 void launch_cell_write32(hipStream_t s, int grid, DevMesh<DIM> m, const float* V32c, const int* old_off, const uint32_t* aid,
                          Rec32 A, int* bcell, uint32_t* bid, Rec32 B, const double* avext, double* bvext, long long cap,
                          DevParams pp, DevWork w, bool fuse, double* partial) {
-    if (DIM == 2 && fuse)
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cell_write32<DIM, DIM == 2>), dim3(grid), dim3(256), 0, s, m, V32c, old_off,
+    if (fuse)
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cell_write32<DIM, true>), dim3(grid), dim3(256), 0, s, m, V32c, old_off,
                            aid, A, bcell, bid, B, avext, bvext, cap, pp, w, partial);
     else
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cell_write32<DIM, false>), dim3(grid), dim3(256), 0, s, m, V32c, old_off,
