@@ -615,14 +615,22 @@ constexpr int FCAP = 64;
 // Kahan-compensated FP32 projection sums of one cell over n staged records
 // in order (accumulator a < NACC: vertex a / (DIM + 1), component a % (DIM +
 // 1), the last one the weight), as k_cell_partials32 computes them
-template <int DIM>
-__device__ __forceinline__ double cell_sums32(const float4* __restrict__ rec, const float2* __restrict__ recb, int n,
-                                              int a) {
-    const int kk = a / (DIM + 1);
-    const int comp = a - kk * (DIM + 1);
-    float ft[DIM], fs[DIM];
-    shape_one32_coeffs<DIM>(kk, ft, fs);
-    float sum = 0.0f, cmp = 0.0f;
+template <int DIM, int NA>
+__device__ __forceinline__ void cell_sums32(const float4* __restrict__ rec, const float2* __restrict__ recb, int n,
+                                            int a0, double* out) {
+    // NA accumulators a0, a0 + GS, ... in one pass over the records (their
+    // Kahan chains are independent, so they overlap)
+    float ft[NA][DIM], fs[NA][DIM], sum[NA], cmp[NA];
+    int comp[NA];
+#pragma unroll
+    for (int u = 0; u < NA; ++u) {
+        const int a = a0 + u * GS;
+        const int kk = a / (DIM + 1);
+        comp[u] = a - kk * (DIM + 1);
+        shape_one32_coeffs<DIM>(kk, ft[u], fs[u]);
+        sum[u] = 0.0f;
+        cmp[u] = 0.0f;
+    }
     for (int q = 0; q < n; ++q) {
         const float4 r = rec[q];
         float xi[DIM], v[DIM];
@@ -640,18 +648,22 @@ __device__ __forceinline__ double cell_sums32(const float4* __restrict__ rec, co
             v[1] = rb.x;
             v[2] = rb.y;
         }
-        const float nk = shape_one32<DIM>(ft, fs, xi);
-        float vc = 1.0f;   // the weight term (comp == DIM) is N_k itself
 #pragma unroll
-        for (int d = 0; d < DIM; ++d)
-            if (d == comp) vc = v[d];
-        const float term = nk * vc;
-        const float y = term - cmp;
-        const float tt = sum + y;
-        cmp = (tt - sum) - y;
-        sum = tt;
+        for (int u = 0; u < NA; ++u) {
+            const float nk = shape_one32<DIM>(ft[u], fs[u], xi);
+            float vc = 1.0f;   // the weight term (comp == DIM) is N_k itself
+#pragma unroll
+            for (int d = 0; d < DIM; ++d)
+                if (d == comp[u]) vc = v[d];
+            const float term = nk * vc;
+            const float y = term - cmp[u];
+            const float tt = sum[u] + y;
+            cmp[u] = (tt - sum[u]) - y;
+            sum[u] = tt;
+        }
     }
-    return (double)sum - (double)cmp;
+#pragma unroll
+    for (int u = 0; u < NA; ++u) out[u] = (double)sum[u] - (double)cmp[u];
 }
 
 // FUSE (final count per cell <= FCAP): the group also stages the cell's
@@ -745,7 +757,13 @@ __device__ __forceinline__ void cell_write32_one(const DevMesh<DIM>& m, const fl
     if constexpr (FUSE) {
         constexpr int NACC = PartialsShape<DIM>::NACC;
         wave_sync();
-        for (int a = gl; a < NACC; a += GS) partial[(size_t)c * NACC + a] = cell_sums32<DIM>(srec, srecb, nfinal, a);
+        constexpr int NA = (NACC + GS - 1) / GS;   // sums per lane: 1 in 2-D (12 lanes), 2 in 3-D
+        if (gl < NACC) {
+            double out[NA];
+            cell_sums32<DIM, NA>(srec, srecb, nfinal, gl, out);
+#pragma unroll
+            for (int u = 0; u < NA; ++u) partial[(size_t)c * NACC + gl + u * GS] = out[u];
+        }
         wave_sync();
     }
 }
