@@ -39,16 +39,22 @@ def field_errors(g, o, m, Vg, Vo, n0):
             "count_mismatch": float(np.mean(cnt_g != cnt_o)), "n_gpu": len(pg["id"]), "n_oracle": len(po["id"])}
 
 
-@pytest.mark.parametrize("name", ["cylinder", "channel"])
+@pytest.mark.parametrize("name", ["cylinder", "channel", "pipe3d"])
 def test_fp32_mixed_error_vs_fp64_oracle(name):
     if name == "cylinder":
         m = M.cylinder_channel(0.12, seed=31)
         V = M.cylinder_flow(m, 1.0, eps=0.01, seed=32)
         dt, S, MAX, q = 0.002, 10, 2, [4, 4]
-    else:
+    elif name == "channel":
         m = M.channel_2d(100, 10, 5.0, 1.0)
         V = M.poiseuille_2d(m, 1.0, 1.0, 0.01, seed=33)
         dt, S, MAX, q = 0.01, 10, 3, [4, 4]
+    else:
+        # hexahedra: the 3-D local step, the fused 3-D projection sums and
+        # spawns interpolated from the FP32 per-cell copy of V
+        m = M.pipe_3d(12, 12, 16, 0.5, 1.0)
+        V = M.pipe_flow(m, 1.0, 0.5, eps=0.05, seed=35)
+        dt, S, MAX, q = 0.01, 10, 2, [2, 2, 2]
     g = ParticleStage(m, dt, S, MAX, q, FP32_MIXED)
     o = Oracle(m, dt, S, MAX, q)
     cell, xi = M.jittered_lattice(m, q, 0.05, 34)

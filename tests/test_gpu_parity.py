@@ -107,15 +107,22 @@ def test_pipe_steps_3d():
     _run_steps(g, o, lambda s: V, 3, "pipe3d")
 
 
+@pytest.mark.parametrize("kind", ["cyl", "pipe3d"])
 @pytest.mark.parametrize("precision", [FP64, FP32_MIXED])
-def test_separate_calls_equal_fused_step(precision):
-    """pf2_step (which, in the 2-D FP32-mixed path, computes the projection
-    sums while regrouping) equals the three separate calls bit for bit."""
-    m = M.cylinder_channel(0.08, seed=18)
-    V = M.cylinder_flow(m, 1.0, eps=0.02, seed=19)
-    cell, xi = M.jittered_lattice(m, [3, 3], 0.05, 20)
-    g1 = ParticleStage(m, 0.004, 5, 3, [3, 3], precision)
-    g2 = ParticleStage(m, 0.004, 5, 3, [3, 3], precision)
+def test_separate_calls_equal_fused_step(precision, kind):
+    """pf2_step (which, in the FP32-mixed path, computes the projection sums
+    while regrouping) equals the three separate calls bit for bit."""
+    if kind == "cyl":
+        m = M.cylinder_channel(0.08, seed=18)
+        V = M.cylinder_flow(m, 1.0, eps=0.02, seed=19)
+        q, dt, MAX = [3, 3], 0.004, 3
+    else:
+        m = M.pipe_3d(8, 8, 10, 0.5, 1.0)
+        V = M.pipe_flow(m, 1.0, 0.5, eps=0.05, seed=19)
+        q, dt, MAX = [2, 2, 2], 0.02, 2
+    cell, xi = M.jittered_lattice(m, q, 0.05, 20)
+    g1 = ParticleStage(m, dt, 5, MAX, q, precision)
+    g2 = ParticleStage(m, dt, 5, MAX, q, precision)
     for g in (g1, g2):
         g.upload_fields(V, V)
         g.insert_particles(cell, xi)
