@@ -397,7 +397,7 @@ __device__ __forceinline__ int exit_face(const double (*P)[DIM], const double* x
 constexpr double NBR_MARGIN = 1e-6;
 
 template <int DIM, class Pre, bool SHORTCUT = true>
-__device__ int relocate(const DevMesh<DIM>& m, int c, const double (*P)[DIM], const double* xi_old,
+__device__ __forceinline__ int relocate(const DevMesh<DIM>& m, int c, const double (*P)[DIM], const double* xi_old,
                         const double* vext, const double* x, double* xi_out, double (*Q)[DIM], Pre& pre) {
     constexpr int NV = Q1<DIM>::NV;
     constexpr int CH = 4;   // star entries read together
