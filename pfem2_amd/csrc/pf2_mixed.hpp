@@ -481,11 +481,11 @@ __global__ void __launch_bounds__(256) k_move32(DevMesh<DIM> m, const float* __r
 // particles cross several cells per step and the face-neighbour shortcut
 // misses more often there: measured 4-8% slower with it.  It runs few
 // wavefronts, so it takes the full register budget (2 per SIMD, no spills).
-// The first round is latency-bound with many wavefronts: 4 per SIMD (128
-// registers, 20 bytes per lane spilled) runs it 12% faster than 3 without
-// spills.)
+// The first round is latency-bound with many wavefronts: in 2-D 4 per SIMD
+// (128 registers, 8 bytes per lane spilled) runs it 12% faster than 3
+// without spills; in 3-D 3 per SIMD (212 bytes spilled) 3% faster than 2.)
 template <int DIM, bool LAST>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DIM == 2 ? (LAST ? 2 : 4) : 2))) k_search_resume32(DevMesh<DIM> m, const float* __restrict__ V32,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LAST ? 2 : (DIM == 2 ? 4 : 3)))) k_search_resume32(DevMesh<DIM> m, const float* __restrict__ V32,
                                                          const float* __restrict__ geo, const int* __restrict__ pcell,
                                                          Rec32 R, double* __restrict__ pvext,
                                                          long long cap, DevParams pp, DevWork w, DevQueues qu,
