@@ -748,8 +748,8 @@ int step_body(pf2_handle_s* h, const double* field) {
         Timed t(h, K_MISC);
         if (int r = pack_fields(h)) return r;
     }
-    // 2-D FP32-mixed steps whose per-cell final count is bounded by the
-    // LDS rows compute the projection sums while regrouping
+    // FP32-mixed steps whose per-cell final count is bounded by the LDS rows
+    // compute the projection sums while regrouping
     const bool fuse = mixed(h) && fuse_bound(h) <= FCAP;
     if (int r = (h->dim == 2 ? run_move<2>(h, 1, 1, fuse) : run_move<3>(h, 1, 1, fuse))) return r;
     return h->dim == 2 ? run_project<2>(h, fuse) : run_project<3>(h, fuse);
