@@ -109,9 +109,13 @@ def test_fp32_mixed_fixed_point_and_determinism():
     assert np.max(np.abs(a["v"] - b["v"])) < 1e-6 * np.max(np.abs(a["v"]))
 
 
-def test_fp32_mixed_moving_mesh():
-    """FP32 state through the fused mesh move (locations from the mesh before
-    the move, FP64 relocation) against the FP64 oracle's three calls."""
+@pytest.mark.parametrize("fused", [True, False])
+def test_fp32_mixed_moving_mesh(fused):
+    """FP32 state through the mesh move (locations from the mesh before the
+    move, FP64 relocation) against the FP64 oracle's three calls: the fused
+    pf2_move_mesh, or the same three calls, which regroup the handle's FP64
+    copy (new particles take V through the DoF map: an FP32-mixed handle
+    keeps no FP64 per-cell velocity copy)."""
     m = M.cylinder_channel(0.1, seed=37)
     V = M.cylinder_flow(m, 1.0, eps=0.01, seed=38)
     g = ParticleStage(m, 0.002, 8, 2, [4, 4], FP32_MIXED, keep_velocity_ext=True)
@@ -129,7 +133,12 @@ def test_fp32_mixed_moving_mesh():
         Vo = o.step()
         X = X0.copy()
         X[:, 1] += 0.003 * np.sin(0.5 * (s + 1)) * w
-        g.move_mesh(X)
+        if fused:
+            g.move_mesh(X)
+        else:
+            g.update_vertices(X)
+            g.sort_particles_into_subdomains_and_cells()
+            g.check_empty_cells()
         o.update_vertices(X)
         o.sort_particles_into_subdomains_and_cells()
         o.check_empty_cells()
