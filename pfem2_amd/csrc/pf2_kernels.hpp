@@ -1275,8 +1275,9 @@ __global__ void __launch_bounds__(256) k_cell_write(DevMesh<DIM> m, const double
 }
 
 // totals after the scans: particle count, id counter, statistics; resets the
-// queue counters for the next step (launched with QSHARDS threads)
-__global__ void k_finalize(int n_cells, DevWork w, int* queue_count) {
+// queue counters for the next step (one workgroup of QSHARDS threads: k_finalize,
+// or the last workgroup of k_node in a fused step)
+__device__ __forceinline__ void finalize_block(int n_cells, const DevWork& w, int* queue_count) {
     static_assert(QSHARDS == STAT_SHARDS, "one thread per stat row");
     __shared__ int s_arr;
     const int t = threadIdx.x;
@@ -1303,6 +1304,10 @@ __global__ void k_finalize(int n_cells, DevWork w, int* queue_count) {
     w.counters[0] = total;
     w.counters[4] = 0;
     w.big[0] = 0;
+}
+
+__global__ void __launch_bounds__(QSHARDS) k_finalize(int n_cells, DevWork w, int* queue_count) {
+    finalize_block(n_cells, w, queue_count);
 }
 
 // ---------------------------------------------------------------------------
@@ -1385,12 +1390,20 @@ __global__ void __launch_bounds__(256) k_cell_partials(int n_cells, const int* _
 // k_node: node sums over the cells that contain the DoF (ascending cell
 // order, TurekBenchmark/pfem2particle.cpp:895-909) divided by the weight.
 // Writes V and V_old (both hold the projected field afterwards).
-template <int DIM>
-__global__ void k_node(DevMesh<DIM> m, const double* __restrict__ partial, double* __restrict__ V,
-                       double* __restrict__ Vold, unsigned long long* stats) {
+template <int DIM, bool FIN>
+__global__ void __launch_bounds__(256) k_node(DevMesh<DIM> m, const double* __restrict__ partial,
+                                              double* __restrict__ V, double* __restrict__ Vold,
+                                              unsigned long long* stats, DevWork w, int* queue_count) {
     constexpr int NV = Q1<DIM>::NV;
     constexpr int NACC = NV * (DIM + 1);
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < m.n_dofs; i += gridDim.x * blockDim.x) {
+    // FIN (a step whose regroup left its bookkeeping for later): the last
+    // workgroup runs it; it depends on the cell write only, as the node sums do
+    if (FIN && blockIdx.x == gridDim.x - 1) {
+        finalize_block(m.n_cells, w, queue_count);
+        return;
+    }
+    const int nb = gridDim.x - (FIN ? 1 : 0);
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < m.n_dofs; i += nb * blockDim.x) {
         double acc[DIM], wsum = 0.0;
 #pragma unroll
         for (int d = 0; d < DIM; ++d) acc[d] = 0.0;

@@ -456,7 +456,7 @@ int get_count(pf2_handle_s* h, long long* n) {
 // Regroup after k_move / k_resort / k_keys filled dest and pkey:
 // arrivals, census, survivors + reseed into the other buffer, swap.
 template <int DIM>
-int regroup(pf2_handle_s* h, int reseed, bool fuse_sums = false) {
+int regroup(pf2_handle_s* h, int reseed, bool fuse_sums = false, bool defer_finalize = false) {
     const int nc = h->n_cells;
     DevParams pp = dev_params(h);
     DevWork w = dev_work(h);
@@ -493,7 +493,7 @@ int regroup(pf2_handle_s* h, int reseed, bool fuse_sums = false) {
                                h->stream, dev_mesh<DIM>(h), h->dV, old_off, dev_particles(h, h->cur),
                                dev_particles(h, 1 - h->cur), pp, w);
     }
-    {
+    if (!defer_finalize) {
         Timed t(h, K_FINALIZE);
         hipLaunchKernelGGL(k_finalize, dim3(1), dim3(QSHARDS), 0, h->stream, nc, w, h->qcount);
     }
@@ -543,7 +543,9 @@ int run_move(pf2_handle_s* h, int do_correct, int do_move, bool fuse_sums = fals
     }
     HIPC(hipGetLastError());
     if (!do_move) return 0;
-    return regroup<DIM>(h, 1, fuse_sums);
+    // a fused step (projection sums in the cell write) leaves the
+    // bookkeeping to the node pass's last workgroup
+    return regroup<DIM>(h, 1, fuse_sums, fuse_sums);
 }
 
 template <int DIM>
@@ -562,8 +564,17 @@ int run_project(pf2_handle_s* h, bool sums_done = false) {
     }
     {
         Timed t(h, K_NODE);
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_node<DIM>), dim3(grid_for(h->n_dofs, 256)), dim3(256), 0, h->stream,
-                           dev_mesh<DIM>(h), h->partial, h->dV, h->dVold, h->stats);
+        if (sums_done) {
+            // also the regroup's deferred bookkeeping (k_finalize's work); the
+            // regroup has swapped the offset buffers, so its new offsets are
+            // now the current ones
+            DevWork wf = w;
+            wf.new_off = h->off[h->cur_off];
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_node<DIM, true>), dim3(grid_for(h->n_dofs, 256) + 1), dim3(256), 0,
+                               h->stream, dev_mesh<DIM>(h), h->partial, h->dV, h->dVold, h->stats, wf, h->qcount);
+        } else
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_node<DIM, false>), dim3(grid_for(h->n_dofs, 256)), dim3(256), 0,
+                               h->stream, dev_mesh<DIM>(h), h->partial, h->dV, h->dVold, h->stats, w, h->qcount);
     }
     if (h->n_walls > 0) {
         Timed t(h, K_WALL);
