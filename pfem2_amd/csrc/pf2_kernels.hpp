@@ -894,6 +894,7 @@ constexpr int GS = 16;
 constexpr int GROUPS = 256 / GS;     // groups per 256-thread block
 constexpr int GCAP = 64;             // candidates staged in LDS per group (lattices of up to 16 parts)
 constexpr int GCAP_L = 128;          // the same for larger lattices
+constexpr int FCAP = 64;             // final particles per cell staged for the fused projection sums
 
 // put(particle, position) for each survivor of cell c as the census left
 // them: listed in id order in the cell's slot row when they fit, else by
@@ -1198,11 +1199,44 @@ __global__ void __launch_bounds__(256) k_cell_big(const int* __restrict__ old_of
 // New particles: reference point at the part centre, location mapped,
 // velocity interpolated from V (cylinder2D/pfem2particle.cpp:533-549), ids
 // continuing the counter in ascending cell, then part, order.
-template <int DIM, bool GATHER>
+// FP64 projection sum a of one cell over n staged particles in order, the
+// expressions of k_cell_partials (accumulator a: vertex a / (DIM + 1),
+// component a % (DIM + 1), the last one the weight)
+template <int DIM>
+__device__ __forceinline__ double cell_sums64(const double* __restrict__ sxi, const double* __restrict__ sv, int n,
+                                              int a) {
+    const int kk = a / (DIM + 1);
+    const int comp = a - kk * (DIM + 1);
+    double ft[DIM], fs[DIM];
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+        ft[d] = ((kk >> d) & 1) ? 1.0 : -1.0;
+        fs[d] = ((kk >> d) & 1) ? 0.0 : 1.0;
+    }
+    double acc = 0.0;
+    for (int q = 0; q < n; ++q) {
+        double nk = fma(ft[0], sxi[q * DIM], fs[0]);
+#pragma unroll
+        for (int d = 1; d < DIM; ++d) nk *= fma(ft[d], sxi[q * DIM + d], fs[d]);
+        if (comp < DIM) {
+            double vc = sv[q * DIM];
+#pragma unroll
+            for (int d = 1; d < DIM; ++d)
+                if (d == comp) vc = sv[q * DIM + d];
+            acc += nk * vc;
+        } else {
+            acc += nk;
+        }
+    }
+    return acc;
+}
+
+template <int DIM, bool GATHER, bool FUSE>
 __device__ __forceinline__ void cell_write_one(const DevMesh<DIM>& m, const double* __restrict__ V,
                                                const int* __restrict__ old_off, const DevParticles& A,
                                                const DevParticles& B, const DevParams& pp, const DevWork& w, int c,
-                                               int gl) {
+                                               int gl, double* __restrict__ sxi, double* __restrict__ sv,
+                                               double* __restrict__ partial) {
     constexpr int NV = Q1<DIM>::NV;
     const int n_st = old_off[c + 1] - old_off[c];
     const int na = w.arr_cnt[c];
@@ -1222,9 +1256,14 @@ __device__ __forceinline__ void cell_write_one(const DevMesh<DIM>& m, const doub
         B.id[o] = A.id[idx];
 #pragma unroll
         for (int d = 0; d < DIM; ++d) {
+            const double xi = A.xi[(size_t)d * A.cap + idx], v = A.v[(size_t)d * A.cap + idx];
             B.x[(size_t)d * B.cap + o] = A.x[(size_t)d * A.cap + idx];
-            B.xi[(size_t)d * B.cap + o] = A.xi[(size_t)d * A.cap + idx];
-            B.v[(size_t)d * B.cap + o] = A.v[(size_t)d * A.cap + idx];
+            B.xi[(size_t)d * B.cap + o] = xi;
+            B.v[(size_t)d * B.cap + o] = v;
+            if constexpr (FUSE) {
+                sxi[pos * DIM + d] = xi;
+                sv[pos * DIM + d] = v;
+            }
         }
         if (pp.keep_vext)
 #pragma unroll
@@ -1252,6 +1291,10 @@ __device__ __forceinline__ void cell_write_one(const DevMesh<DIM>& m, const doub
                 B.x[(size_t)d * B.cap + q] = x[d];
                 B.xi[(size_t)d * B.cap + q] = xi[d];
                 B.v[(size_t)d * B.cap + q] = v[d];
+                if constexpr (FUSE) {
+                    sxi[(nsurv + r) * DIM + d] = xi[d];
+                    sv[(nsurv + r) * DIM + d] = v[d];
+                }
             }
             if (pp.keep_vext)
 #pragma unroll
@@ -1262,16 +1305,28 @@ __device__ __forceinline__ void cell_write_one(const DevMesh<DIM>& m, const doub
         w.arr_cnt[c] = 0;
         w.ovf_head[c] = -1;
     }
+    if constexpr (FUSE) {
+        constexpr int NACC = NV * (DIM + 1);
+        wave_sync();
+        for (int a = gl; a < NACC; a += GS) partial[(size_t)c * NACC + a] = cell_sums64<DIM>(sxi, sv, nfinal, a);
+        wave_sync();
+    }
 }
 
-// GATHER: an FP32-mixed handle's FP64 regroup (no FP64 per-cell velocity copy)
-template <int DIM, bool GATHER>
+// GATHER: an FP32-mixed handle's FP64 regroup (no FP64 per-cell velocity copy).
+// FUSE (final count per cell <= FCAP): the group stages the cell's final
+// particles in LDS and writes its projection sums, as k_cell_partials does.
+template <int DIM, bool GATHER, bool FUSE>
 __global__ void __launch_bounds__(256) k_cell_write(DevMesh<DIM> m, const double* __restrict__ V,
                                                     const int* __restrict__ old_off, DevParticles A,
-                                                    DevParticles B, DevParams pp, DevWork w) {
+                                                    DevParticles B, DevParams pp, DevWork w,
+                                                    double* __restrict__ partial) {
+    __shared__ double s_xi[FUSE ? GROUPS : 1][FUSE ? FCAP * DIM : 1];
+    __shared__ double s_v[FUSE ? GROUPS : 1][FUSE ? FCAP * DIM : 1];
     const int g = threadIdx.x / GS, gl = threadIdx.x % GS;
     for (int c = blockIdx.x * GROUPS + g; c < m.n_cells; c += gridDim.x * GROUPS)
-        cell_write_one<DIM, GATHER>(m, V, old_off, A, B, pp, w, c, gl);
+        cell_write_one<DIM, GATHER, FUSE>(m, V, old_off, A, B, pp, w, c, gl, s_xi[FUSE ? g : 0], s_v[FUSE ? g : 0],
+                                          partial);
 }
 
 // totals after the scans: particle count, id counter, statistics; resets the

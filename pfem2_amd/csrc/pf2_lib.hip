@@ -484,14 +484,19 @@ int regroup(pf2_handle_s* h, int reseed, bool fuse_sums = false, bool defer_fina
                                  w, fuse_sums, h->partial);
     } else {
         Timed t(h, K_CELL_WRITE);
-        if (h->dcellV)
-            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cell_write<DIM, false>), dim3(cell_grid(nc)), dim3(256), 0,
+        // (an FP32-mixed handle's FP64 regroup gathers V and never fuses)
+        if (!h->dcellV)
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cell_write<DIM, true, false>), dim3(cell_grid(nc)), dim3(256), 0,
                                h->stream, dev_mesh<DIM>(h), h->dV, old_off, dev_particles(h, h->cur),
-                               dev_particles(h, 1 - h->cur), pp, w);
+                               dev_particles(h, 1 - h->cur), pp, w, h->partial);
+        else if (fuse_sums)
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cell_write<DIM, false, true>), dim3(cell_grid(nc)), dim3(256), 0,
+                               h->stream, dev_mesh<DIM>(h), h->dV, old_off, dev_particles(h, h->cur),
+                               dev_particles(h, 1 - h->cur), pp, w, h->partial);
         else
-            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cell_write<DIM, true>), dim3(cell_grid(nc)), dim3(256), 0,
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cell_write<DIM, false, false>), dim3(cell_grid(nc)), dim3(256), 0,
                                h->stream, dev_mesh<DIM>(h), h->dV, old_off, dev_particles(h, h->cur),
-                               dev_particles(h, 1 - h->cur), pp, w);
+                               dev_particles(h, 1 - h->cur), pp, w, h->partial);
     }
     if (!defer_finalize) {
         Timed t(h, K_FINALIZE);
@@ -553,7 +558,8 @@ int run_project(pf2_handle_s* h, bool sums_done = false) {
     const int nc = h->n_cells;
     DevWork w = dev_work(h);
     if (sums_done) {
-        // the projection sums came with the regroup (k_cell_write32<DIM, true>)
+        // the projection sums came with the regroup (k_cell_write<DIM, false, true>,
+        // k_cell_write32<DIM, true>)
     } else if (mixed(h)) {
         Timed t(h, K_PARTIALS32);
         launch_partials32<DIM>(h->stream, cell_grid(nc), nc, h->off[h->cur_off], rec32(h->buf[h->cur]), h->partial);
@@ -759,9 +765,9 @@ int step_body(pf2_handle_s* h, const double* field) {
         Timed t(h, K_MISC);
         if (int r = pack_fields(h)) return r;
     }
-    // FP32-mixed steps whose per-cell final count is bounded by the LDS rows
-    // compute the projection sums while regrouping
-    const bool fuse = mixed(h) && fuse_bound(h) <= FCAP;
+    // steps whose per-cell final count is bounded by the LDS rows compute
+    // the projection sums while regrouping
+    const bool fuse = fuse_bound(h) <= FCAP;
     if (int r = (h->dim == 2 ? run_move<2>(h, 1, 1, fuse) : run_move<3>(h, 1, 1, fuse))) return r;
     return h->dim == 2 ? run_project<2>(h, fuse) : run_project<3>(h, fuse);
 }

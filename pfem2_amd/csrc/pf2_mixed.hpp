@@ -608,9 +608,9 @@ void launch_search_resume32(hipStream_t s, int grid, DevMesh<DIM> m, const float
                            pcell, R, pvext, cap, pp, w, qu, round);
 }
 
-// rows of the fused projection sums: a cell's final particles (records; in
-// 3-D the second part of each record in a row of its own)
-constexpr int FCAP = 64;
+// rows of the fused projection sums (FCAP, pf2_kernels.hpp): a cell's final
+// particles (records; in 3-D the second part of each record in a row of its
+// own)
 
 // Kahan-compensated FP32 projection sums of one cell over n staged records
 // in order (accumulator a < NACC: vertex a / (DIM + 1), component a % (DIM +

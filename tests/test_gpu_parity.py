@@ -110,8 +110,8 @@ def test_pipe_steps_3d():
 @pytest.mark.parametrize("kind", ["cyl", "pipe3d"])
 @pytest.mark.parametrize("precision", [FP64, FP32_MIXED])
 def test_separate_calls_equal_fused_step(precision, kind):
-    """pf2_step (which, in the FP32-mixed path, computes the projection sums
-    while regrouping) equals the three separate calls bit for bit."""
+    """pf2_step (which computes the projection sums while regrouping) equals
+    the three separate calls bit for bit."""
     if kind == "cyl":
         m = M.cylinder_channel(0.08, seed=18)
         V = M.cylinder_flow(m, 1.0, eps=0.02, seed=19)
