@@ -1199,36 +1199,52 @@ __global__ void __launch_bounds__(256) k_cell_big(const int* __restrict__ old_of
 // New particles: reference point at the part centre, location mapped,
 // velocity interpolated from V (cylinder2D/pfem2particle.cpp:533-549), ids
 // continuing the counter in ascending cell, then part, order.
-// FP64 projection sum a of one cell over n staged particles in order, the
+// FP64 projection sums of one cell over n staged particles in order, the
 // expressions of k_cell_partials (accumulator a: vertex a / (DIM + 1),
-// component a % (DIM + 1), the last one the weight)
-template <int DIM>
-__device__ __forceinline__ double cell_sums64(const double* __restrict__ sxi, const double* __restrict__ sv, int n,
-                                              int a) {
-    const int kk = a / (DIM + 1);
-    const int comp = a - kk * (DIM + 1);
-    double ft[DIM], fs[DIM];
+// component a % (DIM + 1), the last one the weight); NA accumulators a0,
+// a0 + GS, ... in one pass (their chains are independent, so they overlap)
+template <int DIM, int NA>
+__device__ __forceinline__ void cell_sums64(const double* __restrict__ sxi, const double* __restrict__ sv, int n,
+                                            int a0, double* out) {
+    double ft[NA][DIM], fs[NA][DIM], acc[NA];
+    int comp[NA];
 #pragma unroll
-    for (int d = 0; d < DIM; ++d) {
-        ft[d] = ((kk >> d) & 1) ? 1.0 : -1.0;
-        fs[d] = ((kk >> d) & 1) ? 0.0 : 1.0;
+    for (int u = 0; u < NA; ++u) {
+        const int a = a0 + u * GS;
+        const int kk = a / (DIM + 1);
+        comp[u] = a - kk * (DIM + 1);
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) {
+            ft[u][d] = ((kk >> d) & 1) ? 1.0 : -1.0;
+            fs[u][d] = ((kk >> d) & 1) ? 0.0 : 1.0;
+        }
+        acc[u] = 0.0;
     }
-    double acc = 0.0;
     for (int q = 0; q < n; ++q) {
-        double nk = fma(ft[0], sxi[q * DIM], fs[0]);
+        double xi[DIM], v[DIM];
 #pragma unroll
-        for (int d = 1; d < DIM; ++d) nk *= fma(ft[d], sxi[q * DIM + d], fs[d]);
-        if (comp < DIM) {
-            double vc = sv[q * DIM];
+        for (int d = 0; d < DIM; ++d) {
+            xi[d] = sxi[q * DIM + d];
+            v[d] = sv[q * DIM + d];
+        }
 #pragma unroll
-            for (int d = 1; d < DIM; ++d)
-                if (d == comp) vc = sv[q * DIM + d];
-            acc += nk * vc;
-        } else {
-            acc += nk;
+        for (int u = 0; u < NA; ++u) {
+            double nk = fma(ft[u][0], xi[0], fs[u][0]);
+#pragma unroll
+            for (int d = 1; d < DIM; ++d) nk *= fma(ft[u][d], xi[d], fs[u][d]);
+            if (comp[u] < DIM) {
+                double vc = v[0];
+#pragma unroll
+                for (int d = 1; d < DIM; ++d)
+                    if (d == comp[u]) vc = v[d];
+                acc[u] += nk * vc;
+            } else {
+                acc[u] += nk;
+            }
         }
     }
-    return acc;
+#pragma unroll
+    for (int u = 0; u < NA; ++u) out[u] = acc[u];
 }
 
 template <int DIM, bool GATHER, bool FUSE>
@@ -1308,7 +1324,13 @@ __device__ __forceinline__ void cell_write_one(const DevMesh<DIM>& m, const doub
     if constexpr (FUSE) {
         constexpr int NACC = NV * (DIM + 1);
         wave_sync();
-        for (int a = gl; a < NACC; a += GS) partial[(size_t)c * NACC + a] = cell_sums64<DIM>(sxi, sv, nfinal, a);
+        constexpr int NA = (NACC + GS - 1) / GS;   // sums per lane: 1 in 2-D (12 lanes), 2 in 3-D
+        if (gl < NACC) {
+            double out[NA];
+            cell_sums64<DIM, NA>(sxi, sv, nfinal, gl, out);
+#pragma unroll
+            for (int u = 0; u < NA; ++u) partial[(size_t)c * NACC + gl + u * GS] = out[u];
+        }
         wave_sync();
     }
 }
