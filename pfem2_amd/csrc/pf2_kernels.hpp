@@ -189,6 +189,15 @@ struct DevWork {
     unsigned long long* stats;   // STAT_SHARDS rows: lost, deleted, spawned, searches, zero-weight
 };
 
+// counters[2] bits 0 and 1 (particle capacity exceeded, a cell past the
+// census limit) are sticky: the regroup that set them left the particle
+// buffers incomplete and the new offsets beyond the capacity.  Every later
+// kernel that reads particles or offsets then returns at once, so steps
+// already queued (graph replays, chunks issued ahead) touch no memory out of
+// bounds; the host reports PF2_E_CAPACITY at its next synchronisation.
+constexpr int DEAD_FLAGS = 3;
+__device__ __forceinline__ bool state_dead(const int* counters) { return (counters[2] & DEAD_FLAGS) != 0; }
+
 // vertex coordinates and nodal velocities of cell c, from the packed per-cell
 // copies, or (GATHER: FP32-mixed handles, which keep no FP64 velocity copy)
 // with the velocities gathered from V through the cell's DoF map (the same
@@ -767,7 +776,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DIM ==
                                               const double* __restrict__ Vold, DevParticles A, DevParams pp,
                                               DevWork w, DevQueues qu, int do_correct, int do_move) {
     constexpr int NV = Q1<DIM>::NV;
-    const int n = w.counters[0];
+    const int n = state_dead(w.counters) ? 0 : w.counters[0];
     const int lane = threadIdx.x & (WAVE - 1);
     const WaveLoop L = stride_loop(n);
     for (int base = L.begin; base < L.end; base += L.step) {
@@ -969,6 +978,7 @@ __global__ void __launch_bounds__(256) k_cell_count(int n_cells, const int* __re
     __shared__ int s_idx[GROUPS][CAP + 4];
     __shared__ __align__(16) unsigned long long s_kid[GROUPS][CAP + 2];
     __shared__ __align__(16) uint32_t s_sid[GROUPS][CAP + 4];
+    if (state_dead(w.counters)) return;
     const int g = threadIdx.x / GS, gl = threadIdx.x % GS;
     const unsigned long long all = pp.nparts_in >= 64 ? ~0ull : ((1ull << pp.nparts_in) - 1);
     int deleted = 0, arrivals = 0;
@@ -1090,6 +1100,7 @@ __global__ void __launch_bounds__(256) k_cell_big(const int* __restrict__ old_of
     __shared__ __align__(16) uint32_t s_sid[BIGCAP];
     __shared__ unsigned long long s_hit;
     __shared__ int s_nvalid, s_nsurv;
+    if (state_dead(w.counters)) return;
     const int tid = threadIdx.x;
     const unsigned long long all = pp.nparts_in >= 64 ? ~0ull : ((1ull << pp.nparts_in) - 1);
     const int nbig = w.big[0];
@@ -1345,6 +1356,7 @@ __global__ void __launch_bounds__(256) k_cell_write(DevMesh<DIM> m, const double
                                                     double* __restrict__ partial) {
     __shared__ double s_xi[FUSE ? GROUPS : 1][FUSE ? FCAP * DIM : 1];
     __shared__ double s_v[FUSE ? GROUPS : 1][FUSE ? FCAP * DIM : 1];
+    if (state_dead(w.counters)) return;
     const int g = threadIdx.x / GS, gl = threadIdx.x % GS;
     for (int c = blockIdx.x * GROUPS + g; c < m.n_cells; c += gridDim.x * GROUPS)
         cell_write_one<DIM, GATHER, FUSE>(m, V, old_off, A, B, pp, w, c, gl, s_xi[FUSE ? g : 0], s_v[FUSE ? g : 0],
@@ -1372,8 +1384,10 @@ __device__ __forceinline__ void finalize_block(int n_cells, const DevWork& w, in
     if (t != 0) return;
     w.counters[1] = s_arr;
     const unsigned long long fo = w.fs_off[n_cells];
-    const int total = (int)(fo >> 32);
-    const int spawned = (int)(uint32_t)fo;
+    // a dead state publishes no particles (its total may exceed the capacity)
+    const bool dead = state_dead(w.counters);
+    const int total = dead ? 0 : (int)(fo >> 32);
+    const int spawned = dead ? 0 : (int)(uint32_t)fo;
     w.new_off[n_cells] = total;
     w.stats[2] += (unsigned long long)spawned;
     *w.particle_counter += (uint32_t)spawned;
@@ -1402,7 +1416,8 @@ struct PartialsShape {
 
 template <int DIM>
 __global__ void __launch_bounds__(256) k_cell_partials(int n_cells, const int* __restrict__ off, DevParticles A,
-                                                       double* __restrict__ partial) {
+                                                       double* __restrict__ partial, const int* __restrict__ counters) {
+    if (state_dead(counters)) return;
     constexpr int NV = Q1<DIM>::NV;
     constexpr int NACC = PartialsShape<DIM>::NACC;
     constexpr int PGS = PartialsShape<DIM>::PGS;
@@ -1479,6 +1494,7 @@ __global__ void __launch_bounds__(256) k_node(DevMesh<DIM> m, const double* __re
         finalize_block(m.n_cells, w, queue_count);
         return;
     }
+    if (state_dead(w.counters)) return;
     const int nb = gridDim.x - (FIN ? 1 : 0);
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < m.n_dofs; i += nb * blockDim.x) {
         double acc[DIM], wsum = 0.0;
@@ -1530,6 +1546,7 @@ struct DevWalls {
     const int* vwall;       // per vertex: wall index or -1
     int n_cells;            // cells adjacent to a wall node
     const int* cells;
+    const int* counters;    // the handle's counters (dead-state flags)
 };
 
 template <int DIM>
@@ -1555,7 +1572,7 @@ __global__ void k_wall_particles(DevMesh<DIM> m, DevWalls wl, const int* __restr
     constexpr int NV = Q1<DIM>::NV;
     const int lane = threadIdx.x & (WAVE - 1);
     const int t = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
-    if (t >= wl.n_cells) return;
+    if (t >= wl.n_cells || state_dead(wl.counters)) return;
     const int c = wl.cells[t];
     int order[NV], widx[NV], nw = 0;
 #pragma unroll
@@ -1674,7 +1691,7 @@ __global__ void k_init_particles(DevMesh<DIM> m, const double* __restrict__ V, D
 template <int DIM>
 __global__ void __launch_bounds__(256) k_resort(DevMesh<DIM> m, DevParticles A, DevParams pp, DevWork w) {
     constexpr int NV = Q1<DIM>::NV;
-    const int n = w.counters[0];
+    const int n = state_dead(w.counters) ? 0 : w.counters[0];
     const int lane = threadIdx.x & (WAVE - 1);
     int lost = 0, searches = 0;
     for (int base = blockIdx.x * blockDim.x + (threadIdx.x - lane); base < n; base += gridDim.x * blockDim.x) {
@@ -1718,7 +1735,7 @@ __global__ void __launch_bounds__(256) k_resort(DevMesh<DIM> m, DevParticles A, 
 // dest / key of every particle for a reseed pass without movement
 template <int DIM>
 __global__ void k_keys(DevParticles A, DevParams pp, DevWork w) {
-    const int n = w.counters[0];
+    const int n = state_dead(w.counters) ? 0 : w.counters[0];
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         double xi[DIM];
 #pragma unroll

@@ -442,12 +442,19 @@ int scan(pf2_handle_s* h, const T* in, T* out, int n) {
     return 0;
 }
 
+// the sticky overflow flags (counters[2]) of a synchronised read; after
+// either, the handle holds no valid particle state
+int flag_error(int flags) {
+    if (flags & 1) return fail(PF2_E_CAPACITY, "particle capacity exceeded");
+    if (flags & 2) return fail(PF2_E_CAPACITY, "more than 2048 particles stayed in or arrived at one cell");
+    return 0;
+}
+
 int get_count(pf2_handle_s* h, long long* n) {
     int cnt[3];
     HIPC(hipMemcpyAsync(cnt, h->counters, sizeof(cnt), hipMemcpyDeviceToHost, h->stream));
     HIPC(hipStreamSynchronize(h->stream));
-    if (cnt[2] & 1) return fail(PF2_E_CAPACITY, "particle capacity exceeded");
-    if (cnt[2] & 2) return fail(PF2_E_CAPACITY, "more than 2048 particles stayed in or arrived at one cell");
+    if (int r = flag_error(cnt[2])) return r;
     *n = cnt[0];
     h->n_est = cnt[0];
     return 0;
@@ -455,8 +462,11 @@ int get_count(pf2_handle_s* h, long long* n) {
 
 // Regroup after k_move / k_resort / k_keys filled dest and pkey:
 // arrivals, census, survivors + reseed into the other buffer, swap.
+// fp64_state: regroup the FP64 copy of an FP32-mixed handle's state (the
+// re-sort and check_empty_cells run in FP64).
 template <int DIM>
-int regroup(pf2_handle_s* h, int reseed, bool fuse_sums = false, bool defer_finalize = false) {
+int regroup(pf2_handle_s* h, int reseed, bool fuse_sums = false, bool defer_finalize = false,
+            bool fp64_state = false) {
     const int nc = h->n_cells;
     DevParams pp = dev_params(h);
     DevWork w = dev_work(h);
@@ -476,7 +486,7 @@ int regroup(pf2_handle_s* h, int reseed, bool fuse_sums = false, bool defer_fina
                            h->buf[h->cur].id, pp, w, reseed);
     }
     if (int r = scan(h, h->fs_cnt, h->fs_off, nc + 1)) return r;
-    if (mixed(h)) {
+    if (mixed(h) && !fp64_state) {
         Timed t(h, K_CELL_WRITE32);
         launch_cell_write32<DIM>(h->stream, cell_grid(nc), dev_mesh<DIM>(h), h->dV32, old_off, h->buf[h->cur].id,
                                  rec32(h->buf[h->cur]), h->buf[1 - h->cur].cell, h->buf[1 - h->cur].id,
@@ -562,11 +572,12 @@ int run_project(pf2_handle_s* h, bool sums_done = false) {
         // k_cell_write32<DIM, true>)
     } else if (mixed(h)) {
         Timed t(h, K_PARTIALS32);
-        launch_partials32<DIM>(h->stream, cell_grid(nc), nc, h->off[h->cur_off], rec32(h->buf[h->cur]), h->partial);
+        launch_partials32<DIM>(h->stream, cell_grid(nc), nc, h->off[h->cur_off], rec32(h->buf[h->cur]), h->partial,
+                               h->counters);
     } else {
         Timed t(h, K_PARTIALS);
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cell_partials<DIM>), dim3(cell_grid(nc)), dim3(256), 0, h->stream, nc,
-                           h->off[h->cur_off], dev_particles(h, h->cur), h->partial);
+                           h->off[h->cur_off], dev_particles(h, h->cur), h->partial, h->counters);
     }
     {
         Timed t(h, K_NODE);
@@ -594,6 +605,7 @@ int run_project(pf2_handle_s* h, bool sums_done = false) {
         wl.vwall = h->w_vwall;
         wl.n_cells = h->n_wall_cells;
         wl.cells = h->w_cells;
+        wl.counters = h->counters;
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_wall_flags<DIM>), dim3(grid_for(h->n_walls, 256)), dim3(256), 0,
                            h->stream, wl, h->dV);
         if (mixed(h)) {
@@ -620,9 +632,11 @@ int download_field(pf2_handle_s* h, double* V) {
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_to_comp_major<3>), dim3(grid_for(n, 256)), dim3(256), 0, h->stream, h->dV,
                            h->dtmp, n);
     HIPC(hipGetLastError());
+    int flags = 0;
     HIPC(hipMemcpyAsync(V, h->dtmp, sizeof(double) * (size_t)n * h->dim, hipMemcpyDeviceToHost, h->stream));
+    HIPC(hipMemcpyAsync(&flags, h->counters + 2, sizeof(int), hipMemcpyDeviceToHost, h->stream));
     HIPC(hipStreamSynchronize(h->stream));
-    return 0;
+    return flag_error(flags);
 }
 
 int upload_field(pf2_handle_s* h, const double* src, double* dst, hipMemcpyKind kind) {
@@ -1331,11 +1345,9 @@ int pf2_sort_particles_into_subdomains_and_cells(pf2_handle h) {
                                h->stream, dev_mesh<3>(h), dev_particles(h, h->cur), pp, w);
     }
     HIPC(hipGetLastError());
-    const int saved = h->params.precision;
-    h->params.precision = PF2_FP64;   // regroup the FP64 copy, then convert back
+    // regroup the FP64 copy, then convert back
     int r = ensure_packed(h);
-    if (!r) r = h->dim == 2 ? regroup<2>(h, 0) : regroup<3>(h, 0);
-    h->params.precision = saved;
+    if (!r) r = h->dim == 2 ? regroup<2>(h, 0, false, false, true) : regroup<3>(h, 0, false, false, true);
     if (r) return r;
     long long n;
     if (int q = get_count(h, &n)) return q;
@@ -1357,11 +1369,8 @@ int pf2_check_empty_cells(pf2_handle h) {
                                h->stream, dev_particles(h, h->cur), pp, w);
     }
     HIPC(hipGetLastError());
-    const int saved = h->params.precision;
-    h->params.precision = PF2_FP64;
     int r = ensure_packed(h);
-    if (!r) r = h->dim == 2 ? regroup<2>(h, 1) : regroup<3>(h, 1);
-    h->params.precision = saved;
+    if (!r) r = h->dim == 2 ? regroup<2>(h, 1, false, false, true) : regroup<3>(h, 1, false, false, true);
     if (r) return r;
     long long n;
     if (int q = get_count(h, &n)) return q;
@@ -1487,6 +1496,8 @@ int pf2_n_global_particles(pf2_handle h, long long* n) {
 
 int pf2_cell_counts(pf2_handle h, int* counts) {
     if (!h || !counts) return fail(PF2_E_ARG, "NULL argument");
+    long long n;
+    if (int r = get_count(h, &n)) return r;
     std::vector<int> off(h->n_cells + 1);
     HIPC(hipMemcpyAsync(off.data(), h->off[h->cur_off], off.size() * 4, hipMemcpyDeviceToHost, h->stream));
     HIPC(hipStreamSynchronize(h->stream));

@@ -441,7 +441,7 @@ __global__ void __launch_bounds__(256) k_move32(DevMesh<DIM> m, const float* __r
                                                 int do_move) {
     constexpr int NV = Q1<DIM>::NV;
     constexpr int NG = DIM == 2 ? 6 : 21;
-    const int n = w.counters[0];
+    const int n = state_dead(w.counters) ? 0 : w.counters[0];
     const float h32 = (float)pp.h;
     const int lane = threadIdx.x & (WAVE - 1);
     const WaveLoop L = stride_loop(n);
@@ -778,6 +778,7 @@ __global__ void __launch_bounds__(256) k_cell_write32(DevMesh<DIM> m, const floa
     constexpr bool FB = FUSE && DIM == 3;
     __shared__ float4 s_rec[FUSE ? GROUPS : 1][FUSE ? FCAP : 1];
     __shared__ float2 s_recb[FB ? GROUPS : 1][FB ? FCAP : 1];
+    if (state_dead(w.counters)) return;
     const int g = threadIdx.x / GS, gl = threadIdx.x % GS;
     for (int c = blockIdx.x * GROUPS + g; c < m.n_cells; c += gridDim.x * GROUPS)
         cell_write32_one<DIM, FUSE>(m, V32c, old_off, aid, A, bcell, bid, B, avext, bvext, cap, pp, w, c, gl,
@@ -802,7 +803,8 @@ void launch_cell_write32(hipStream_t s, int grid, DevMesh<DIM> m, const float* V
 // (groups of lanes per cell as in k_cell_partials)
 template <int DIM>
 __global__ void __launch_bounds__(256) k_cell_partials32(int n_cells, const int* __restrict__ off, Rec32 R,
-                                                         double* __restrict__ partial) {
+                                                         double* __restrict__ partial, const int* __restrict__ counters) {
+    if (state_dead(counters)) return;
     constexpr int NV = Q1<DIM>::NV;
     constexpr int NACC = PartialsShape<DIM>::NACC;
     constexpr int PGS = PartialsShape<DIM>::PGS;
@@ -859,8 +861,10 @@ __global__ void __launch_bounds__(256) k_cell_partials32(int n_cells, const int*
 }
 
 template <int DIM>
-void launch_partials32(hipStream_t s, int grid, int n_cells, const int* off, Rec32 R, double* partial) {
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cell_partials32<DIM>), dim3(grid), dim3(256), 0, s, n_cells, off, R, partial);
+void launch_partials32(hipStream_t s, int grid, int n_cells, const int* off, Rec32 R, double* partial,
+                       const int* counters) {
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cell_partials32<DIM>), dim3(grid), dim3(256), 0, s, n_cells, off, R, partial,
+                       counters);
 }
 
 template <int DIM>
@@ -868,7 +872,7 @@ __global__ void k_wall_particles32(DevMesh<DIM> m, DevWalls wl, const int* __res
     constexpr int NV = Q1<DIM>::NV;
     const int lane = threadIdx.x & (WAVE - 1);
     const int t = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
-    if (t >= wl.n_cells) return;
+    if (t >= wl.n_cells || state_dead(wl.counters)) return;
     const int c = wl.cells[t];
     int order[NV], widx[NV], nw = 0;
 #pragma unroll
@@ -932,7 +936,7 @@ __global__ void k_pack_fields32(DevMesh<DIM> m, const double* __restrict__ V, co
 template <int DIM>
 __global__ void k_state_to32(const double* __restrict__ xi, const double* __restrict__ v, Rec32 R, long long cap,
                              const int* __restrict__ counters) {
-    const long long n = counters[0];
+    const long long n = state_dead(counters) ? 0 : counters[0];
     for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
         float xf[DIM], vf[DIM];
 #pragma unroll
@@ -959,7 +963,7 @@ __global__ void k_state_from32(DevMesh<DIM> m, const int* __restrict__ pcell, Re
                                double* __restrict__ xi, double* __restrict__ v, long long cap,
                                const int* __restrict__ counters) {
     constexpr int NV = Q1<DIM>::NV;
-    const long long n = counters[0];
+    const long long n = state_dead(counters) ? 0 : counters[0];
     for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
         double xd[DIM], P[NV][DIM], xx[DIM];
         float xf[DIM], vf[DIM];
@@ -993,7 +997,7 @@ template <int DIM>
 __global__ void k_pos32(DevMesh<DIM> m, const int* __restrict__ pcell, Rec32 R, double* __restrict__ x,
                         long long cap, const int* __restrict__ counters) {
     constexpr int NV = Q1<DIM>::NV;
-    const long long n = counters[0];
+    const long long n = state_dead(counters) ? 0 : counters[0];
     for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
         float xf[DIM];
         double xd[DIM], P[NV][DIM], xx[DIM];
@@ -1014,7 +1018,7 @@ __global__ void __launch_bounds__(256) k_resort32(DevMesh<DIM> m, const int* __r
                                                   const double* __restrict__ px, const double* __restrict__ pvext,
                                                   Rec32 R, long long cap, DevParams pp, DevWork w) {
     constexpr int NV = Q1<DIM>::NV;
-    const int n = w.counters[0];
+    const int n = state_dead(w.counters) ? 0 : w.counters[0];
     const int lane = threadIdx.x & (WAVE - 1);
     int lost = 0, searches = 0;
     for (int base = blockIdx.x * blockDim.x + (threadIdx.x - lane); base < n; base += gridDim.x * blockDim.x) {

@@ -1,14 +1,16 @@
 """GPU (HIP, FP64 path) versus the CPU oracle on identical seeded inputs.
 
 Bar: cell ownership, ids and per-cell counts bit-exact; x, xi, v and the
-projected nodal field within 1e-12 relative (the kernels are written to give
-the oracle's bits; the report records whether they are bitwise equal).
+projected nodal field bitwise equal to the oracle's (the kernels evaluate the
+oracle's expressions in its order, without contraction).  The bar named for
+this path is 1e-12 relative; bitwise equality is stricter and is what the
+tests assert.
 """
 import numpy as np
 import pytest
 
 from oracle_binding import Oracle
-from parity_util import assert_particles_match, compare_particles, field_maxrel
+from parity_util import assert_particles_match, bitwise_equal, compare_particles, field_maxrel
 from pfem2_amd import FP32_MIXED, FP64, ParticleStage
 from pfem2_amd import mesh as M
 
@@ -38,12 +40,12 @@ def _run_steps(g, o, V_of_step, steps, where):
         Vg = g.step(download=True)
         Vo = o.step()
         pg, po = g.particles(), o.particles()
-        rep = assert_particles_match(pg, po, REL, f"{where} step {s}")
+        rep = assert_particles_match(pg, po, REL, f"{where} step {s}", bitwise=True)
         assert np.array_equal(g.cell_counts(), o.cell_counts()), f"{where} step {s}: cell counts"
         fr = field_maxrel(Vg, Vo)
-        assert fr <= REL, f"{where} step {s}: projected field differs by {fr}"
         rep["field_maxrel"] = fr
-        rep["field_bitwise"] = bool(np.array_equal(np.nan_to_num(Vg), np.nan_to_num(Vo)))
+        rep["field_bitwise"] = bitwise_equal(Vg, Vo)
+        assert rep["field_bitwise"], f"{where} step {s}: projected field not bitwise equal (max rel {fr})"
         reps.append(rep)
         Vn = V_of_step(s)
         g.upload_fields(Vn, None)
@@ -158,9 +160,12 @@ def test_step_from_empty_state(precision):
     assert g.particle_counter() == o.particle_counter() == m.n_cells * 9
     pg, po = g.particles(), o.particles()
     assert np.array_equal(pg["cell"], po["cell"]) and np.array_equal(pg["id"], po["id"])
-    tol = 0.0 if precision == FP64 else 1e-6
-    assert np.max(np.abs(pg["xi"] - po["xi"])) <= tol
-    assert field_maxrel(Vg, Vo) <= (REL if precision == FP64 else 1e-5)
+    if precision == FP64:
+        assert bitwise_equal(pg["xi"], po["xi"]) and bitwise_equal(pg["v"], po["v"])
+        assert bitwise_equal(Vg, Vo)
+    else:
+        assert np.max(np.abs(pg["xi"] - po["xi"])) <= 1e-6
+        assert field_maxrel(Vg, Vo) <= 1e-5
 
 
 def test_points_on_faces_and_vertices():
@@ -196,8 +201,8 @@ def test_run_twice_bitwise():
             Vp = g.step(download=True)
         outs.append((g.particles(), Vp))
         g.close()
-    assert_particles_match(outs[0][0], outs[1][0], 0.0, "run twice")
-    assert np.array_equal(outs[0][1], outs[1][1])
+    assert_particles_match(outs[0][0], outs[1][0], 0.0, "run twice", bitwise=True)
+    assert bitwise_equal(outs[0][1], outs[1][1])
 
 
 def test_walls_fixup():
@@ -241,7 +246,7 @@ def test_moving_mesh_resort(fused):
         o.update_vertices(X, recompute_maps=(s == 1))
         o.sort_particles_into_subdomains_and_cells()
         o.check_empty_cells()
-        assert_particles_match(g.particles(), o.particles(), REL, f"moving mesh {s}")
+        assert_particles_match(g.particles(), o.particles(), REL, f"moving mesh {s}", bitwise=True)
         assert np.array_equal(g.cell_counts(), o.cell_counts())
         assert g.particle_counter() == o.particle_counter()
         sg, so = g.stats(), o.stats()
@@ -284,3 +289,39 @@ def test_overfull_cell_reports_capacity_error():
     g.step()
     with pytest.raises(Pf2Error, match="2048"):
         g.n_global_particles()
+
+
+@pytest.mark.parametrize("graphs", [False, True])
+@pytest.mark.parametrize("precision", [FP64, FP32_MIXED])
+def test_capacity_overflow_is_sticky(precision, graphs):
+    """A reseed that needs more particles than the capacity: 64 particles in
+    cell 0 of a 4x4 mesh with capacity 64; the census keeps 3 per part of
+    cell 0 (12) and spawns 4 in each of the 15 empty cells (60), 72 > 64.
+    The write skips the cells that do not fit and flags the overflow; two
+    more steps queued before the host looks (eager or graph replays) do no
+    work, and every synchronising call reports the capacity error."""
+    from pfem2_amd import Pf2Error
+    m = M.quad_2d(4, 4, (0.0, 0.0), (1.0, 1.0))
+    V = M.uniform_field(m, [0.0, 0.0])
+    rng = np.random.default_rng(35)
+    g = ParticleStage(m, 0.01, 2, 3, [2, 2], precision, capacity=64)
+    g.upload_fields(V, V)
+    g.insert_particles(np.zeros(64, dtype=np.int32), rng.uniform(0.05, 0.95, size=(64, 2)))
+    assert g.n_global_particles() == 64
+    g.set_graphs(graphs)
+    for _ in range(3):
+        g.step()
+    with pytest.raises(Pf2Error, match="capacity"):
+        g.n_global_particles()
+    with pytest.raises(Pf2Error, match="capacity"):
+        g.cell_counts()
+    with pytest.raises(Pf2Error, match="capacity"):
+        g.step(download=True)
+    g.close()
+    # the library stays usable: a fresh handle with room steps normally
+    h = ParticleStage(m, 0.01, 2, 3, [2, 2], precision)
+    h.upload_fields(V, V)
+    h.seed_particles()
+    h.step()
+    assert h.n_global_particles() == m.n_cells * 4
+    h.close()
