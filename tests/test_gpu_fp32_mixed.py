@@ -152,3 +152,34 @@ def test_fp32_mixed_moving_mesh(fused):
     assert err["x_rel"] <= 1e-5
     assert err["owner_mismatch"] <= 1e-3
     assert err["count_mismatch"] <= 1e-2
+
+
+def test_fp32_mixed_check_empty_cells_then_correct_and_move():
+    """After a projection the per-cell FP32 copies of V and V - V_old are
+    stale; check_empty_cells (an FP64 regroup on an FP32-mixed handle) must
+    refresh them before it regroups, so that the correction and the move
+    that follow without an upload use the projected field (V - V_old = 0),
+    not the field uploaded before the step."""
+    m = M.cylinder_channel(0.12, seed=40)
+    V = M.cylinder_flow(m, 1.0, eps=0.01, seed=41)
+    V2 = M.cylinder_flow(m, 1.3, eps=0.02, seed=42)
+    g = ParticleStage(m, 0.002, 8, 2, [3, 3], FP32_MIXED)
+    o = Oracle(m, 0.002, 8, 2, [3, 3])
+    cell, xi = M.jittered_lattice(m, [3, 3], 0.05, 43)
+    for s in (g, o):
+        s.upload_fields(V2, V)
+        s.insert_particles(cell, xi)
+    n0 = len(cell)
+    g.step()
+    o.step()
+    for s in (g, o):
+        s.check_empty_cells()
+        s.correct_particles_velocities()
+        s.move_particles()
+    Vg = g.distribute_particle_velocities_to_grid()
+    Vo = o.distribute_particle_velocities_to_grid()
+    err = field_errors(g, o, m, Vg, Vo, n0)
+    print("check_empty_cells fp32", err, flush=True)
+    assert err["V_rel_l2"] <= 1e-5
+    assert err["x_rel"] <= 1e-5
+    assert err["v_p9999"] <= 1e-5
