@@ -1,32 +1,33 @@
 """Benchmark of the PFEM2 particle stage on one MI355X.
 
-Default workload (the metric's configuration that fits one GPU): 2-D channel
-with a circular obstacle (Turek-Schaefer geometry 2.2 x 0.41, cylinder
-r = 0.05), graded quad mesh of ~100k cells with seeded vertex jitter, 16
-particles per cell (4x4 lattice, +-0.05 seeded jitter, ~1.6M particles at the
-start), velocity = potential flow around the cylinder + 1% seeded Fourier
-perturbation, dt = 0.001, 10 advection substeps, MAX = 2 particles per cell
-part.
+Value (the largest configuration of the metric that fits one GPU): the
+moving-mesh cylinder channel, ~400k graded quads around a cylinder that
+oscillates across the channel (amplitude 0.1 D), 25 particles per cell
+(5x5 lattice, +-0.05 seeded jitter, ~10.2M particles at the start), the
+flow around the cylinder blended into the wall's velocity near it,
+dt = 0.0005, 10 advection substeps, MAX = 2, on the FP64 path (the
+reference's arithmetic, bitwise equal to the CPU oracle).  One step =
+correct -> move (10 substeps of advection + relocation, then reseed/delete)
+-> project, for every particle, after the FEM stand-in (the step's nodal V,
+a device-resident field, copied into the stage), then the mesh move (vertex
+update, re-sort of every particle, reseed).  All inputs are resident in HBM
+before the timed region.
 
-One step = correct -> move (10 substeps of advection + relocation, then
-reseed/delete) -> project, for every particle, after the FEM stand-in (the
-new nodal V, a device-resident field, copied into the stage).  All inputs are
-resident in HBM before the timed region; steps are replayed from HIP graphs.
+The line also carries the same workload on the FP32-mixed path, the
+~100k-cell cylinder channel (16 particles per cell, dt = 0.001) in both
+precisions with the FP32-mixed path's per-field error against FP64 and its
+targets (met or not), the CPU oracle on a sample of the value's workload
+(one core, and one simulation per host core at once), and the request
+stream of the latency metric (256 independent simulations, Poisson
+arrivals, 2k-200k cells, 9/16/25 particles per cell, 50-500 steps, plus 8
+moving-mesh simulations of ~400k cells; admission by HBM footprint) with
+its p50/p99 completion latency under "serving".
 
-The default run also serves the request stream of the latency metric
-(256 independent simulations, Poisson arrivals, 2k-200k cells, 9/16/25
-particles per cell, 50-500 steps, plus 8 moving-mesh simulations of ~400k
-cells; admission by HBM footprint) and reports its p50/p99 completion
-latency under "serving".
-
-Other workloads (--workload): channel (200x20 quads, FP64, with a host FEM
-stand-in between particle steps, against the CPU oracle over the whole run), pipe3d (823k hexahedra in a pipe, 2x2x2 lattice),
-moving (~400k quads around an oscillating cylinder, 25 particles per cell,
-mesh motion + re-sort every step), serve (the request stream alone).
-
-Prints one JSON line.  `value` is the FP32-mixed path's whole-job
-throughput; "paths" holds the FP64 path (bitwise equal to the CPU oracle)
-and the per-field error of the FP32-mixed path against it.
+Other workloads (--workload): cylinder (the 100k-cell channel alone),
+channel (200x20 quads, FP64, with a host FEM stand-in between particle
+steps, against the CPU oracle over the whole run), pipe3d (826k hexahedra
+in a pipe, 2x2x2 lattice), moving (the value's workload alone), serve (the
+request stream alone).  Prints one JSON line.
 """
 import argparse
 import json
@@ -43,6 +44,9 @@ sys.path.insert(0, ROOT)
 
 SEEDS = {"mesh": 2024, "field": 7, "particles": 11, "serve": 99, "pipe": 31, "moving": 57}
 HBM_PEAK_GBS = 8000.0
+# committed rocprofv3 --pmc summaries (profiles/hbm_traffic_<name>.txt) per workload and precision
+PROFILES = {("moving", 0): "moving400k_fp64", ("cylinder", 0): "cylinder100k_fp64",
+            ("cylinder", 1): "cylinder100k"}
 DTYPE32 = "fp32-mixed (f32 reference coords/velocities, f64 relocation and node sums)"
 
 
@@ -84,12 +88,12 @@ def pipe_workload():
                          "Fourier perturbation, 2x2x2 lattice +-0.05 jitter, dt=0.01, 10 substeps, MAX=2")
 
 
-def moving_workload():
+def moving_workload(scale=2.0):
     """~400k quads around a cylinder oscillating across the channel
     (amplitude 0.1 D, 2 Hz); the displacement blends from rigid at the
     cylinder to zero at 3 radii."""
     from pfem2_amd import mesh as M
-    m = M.cylinder_channel(2.0, seed=SEEDS["moving"])
+    m = M.cylinder_channel(scale, seed=SEEDS["moving"])
     V = M.cylinder_flow(m, 1.0, eps=0.01, seed=SEEDS["field"])
     X0 = m.vertices.copy()
     c = np.array([0.2, 0.2])
@@ -202,7 +206,7 @@ def run_path(args, W, precision, torch):
     st.set_graphs(Fs is None)
     for _ in range(args.warmup):
         step()
-    st.synchronize()
+    n_start = st.n_global_particles()     # synchronises: the count entering the timed window
     t0 = time.perf_counter()
     for _ in range(args.steps):
         step()
@@ -210,7 +214,9 @@ def run_path(args, W, precision, torch):
     t1 = time.perf_counter()
     n1 = st.n_global_particles()
     ms = (t1 - t0) * 1e3 / args.steps
-    updates = 0.5 * (n0 + n1)
+    # particles stepped per timed step: the counts entering and leaving the
+    # timed window, averaged (the count changes by reseeding and losses only)
+    updates = 0.5 * (n_start + n1)
     lat = []
     for _ in range(max(args.steps, 20)):
         a = time.perf_counter()
@@ -237,7 +243,8 @@ def run_path(args, W, precision, torch):
         "updates_per_s": updates / (ms * 1e-3),
         "ms_per_step": ms,
         "ms_per_step_eager": ms_eager,
-        "particles_start": int(n0),
+        "particles_inserted": int(n0),
+        "particles_start": int(n_start),
         "particles": int(n1),
         "p50_ms": float(np.percentile(lat, 50)),
         "p99_ms": float(np.percentile(lat, 99)),
@@ -297,29 +304,108 @@ def _oracle(W):
     return o
 
 
-def cpu_baseline(W, seconds=10.0):
-    """Oracle (CPU restatement, 1 thread) on a bounded sample of the workload:
-    the same mesh and particles, stepped until about `seconds` of CPU time."""
-    o = _oracle(W)
-    done, t, upd = 0, 0.0, 0.0
-    while t < seconds and done < 50:
+def cpu_steps(W, o, seconds, max_steps=50):
+    """Full steps of workload W on one oracle until about `seconds` of CPU
+    time (at least one step).  A step is what the GPU step is: correct +
+    move + project (timed inside the oracle) and, for a moving mesh, the
+    vertex update, re-sort and reseed (timed here).  Returns (particle-updates,
+    seconds, steps)."""
+    done, t, upd = 0, 0.0, 0
+    while (done == 0 or t < seconds) and done < max_steps:
+        if W.field is not None:
+            o.upload_fields(W.field(done, W.dt), None)
         n = o.n_global_particles()
-        dt = o.time_steps(1)
-        o.upload_fields(W.V, None)
-        t += dt
+        t += o.time_steps(1)
+        if W.motion is not None:
+            X = W.motion(done + 1, W.dt)
+            a = time.perf_counter()
+            o.update_vertices(X)
+            o.sort_particles_into_subdomains_and_cells()
+            o.check_empty_cells()
+            t += time.perf_counter() - a
+        elif W.field is None:
+            o.upload_fields(W.V, None)
         upd += n
         done += 1
-    return {"value": upd / t, "unit": "particle-updates/s", "cores": 1, "kind": "port",
-            "sample": f"{done} full steps of the same workload ({W.m.n_cells} cells, ~{int(upd / done)} particles), "
-                      f"{t:.1f} s single-threaded"}
+    return upd, t, done
 
 
-def measured_traffic(kernel):
-    """HBM bytes per launch of `kernel` for the default workload from the
-    committed rocprofv3 --pmc summary (FETCH_SIZE and WRITE_SIZE scaled by a
-    plain copy kernel of known size), or None."""
-    path = os.path.join(ROOT, "profiles", "hbm_traffic_cylinder100k.txt")
-    if not os.path.exists(path):
+def host_threads():
+    """Host cores this process may use, at most 16 (a one-GPU box's share)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def cpu_baseline(W, seconds=10.0, threads=None):
+    """The oracle (the CPU restatement of the reference's particle stage,
+    FP64, built -O3) on a bounded sample of workload W: one simulation on one
+    core, then `threads` independent simulations at once (the reference runs
+    one MPI rank per core; the oracle releases the interpreter lock, so the
+    simulations run in parallel threads), each stepped for about `seconds`."""
+    import threading
+    o = _oracle(W)
+    upd1, t1, k1 = cpu_steps(W, o, seconds)
+    o.close()
+    threads = threads or host_threads()
+    res = [None] * threads
+    ready = threading.Barrier(threads + 1)
+
+    def run(j):
+        oj = _oracle(W)
+        ready.wait()
+        res[j] = cpu_steps(W, oj, seconds, max_steps=k1)
+        oj.close()
+
+    ts = [threading.Thread(target=run, args=(j,)) for j in range(threads)]
+    for t in ts:
+        t.start()
+    ready.wait()
+    a = time.perf_counter()
+    for t in ts:
+        t.join()
+    wall = time.perf_counter() - a
+    updn = sum(r[0] for r in res)
+    desc = f"{W.name} workload ({W.m.n_cells} cells, ~{int(upd1 / k1)} particles)"
+    return {"value": updn / wall, "unit": "particle-updates/s", "cores": threads, "kind": "port",
+            "sample": f"{threads} independent simulations of the {desc} at once, {k1} full step(s) each "
+                      f"(step = correct+move+project{', mesh move' if W.motion else ''}), {wall:.1f} s wall",
+            "single_core": {"value": upd1 / t1, "cores": 1,
+                            "sample": f"one simulation, {k1} full step(s), {t1:.1f} s"}}
+
+
+def b_alg(W, n_particles, precision):
+    """Compulsory HBM bytes per particle-update: particle state read and
+    written once, nodal V, V_old and the projection (f64), vertex
+    coordinates (twice on a moving mesh: read and written), the cell DoF
+    map and the particle offsets."""
+    from pfem2_amd import FP64
+    m, d = W.m, W.m.dim
+    wx, wv = (8, 8) if precision == FP64 else (4, 4)
+    ndc = getattr(m, "dofs_per_cell", 1 << d)
+    b = 2 * (8 + d * (wx + wv))
+    b += m.n_dofs / n_particles * 3 * d * 8
+    b += m.n_vertices / n_particles * d * 8 * (2 if W.motion is not None else 1)
+    b += m.n_cells / n_particles * (ndc * 4 + 8)
+    return b
+
+
+def stage_roofline(W, r, precision):
+    """The whole particle stage against HBM: B_alg x particle-updates/s."""
+    b = b_alg(W, 0.5 * (r["particles_start"] + r["particles"]), precision)
+    ach = b * r["updates_per_s"] / 1e9
+    return {"bound": "hbm", "bytes_per_update": b, "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": ach / HBM_PEAK_GBS}
+
+
+def measured_traffic(kernel, profile):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
+    summary of the workload (profiles/hbm_traffic_<profile>.txt: FETCH_SIZE
+    and WRITE_SIZE scaled by a plain copy kernel of known size), or None."""
+    path = os.path.join(ROOT, "profiles", f"hbm_traffic_{profile}.txt")
+    if not profile or not os.path.exists(path):
         return None
     for line in open(path):
         mm = re.match(r"\s*kernel=(\S+)\s+hbm_bytes_per_launch=(\d+)", line)
@@ -328,7 +414,7 @@ def measured_traffic(kernel):
     return None
 
 
-def roofline(res, m, traffic=True):
+def roofline(res, m, profile=None):
     """Roofline of the kernel with the largest share of the step.  The FP64
     relocation rounds are one kernel launched once per round: its bytes and
     duration per launch are averaged over the rounds, as the profiler's
@@ -354,7 +440,7 @@ def roofline(res, m, traffic=True):
                            "share_of_step": per_step[key] / sum(per_step.values())}
     return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS if achieved else None,
-            "traffic": measured_traffic(dom) if traffic else None,
+            "traffic": measured_traffic(dom, profile),
             "kernel": dom, "launches_per_step": len(members), "bytes_per_launch": b, "avg_launch_ms": dur * 1e3,
             "all_kernels": others}
 
@@ -465,7 +551,7 @@ def base_line(args, W, r, dtype):
         "vs_baseline": None,
         "dtype": dtype,
         "data": "synthetic",
-        "config": {"workload": W.desc, "cells": W.m.n_cells, "particles": r["particles"], "dt": W.dt,
+        "config": {"workload": f"{W.name}: {W.desc}", "cells": W.m.n_cells, "particles": r["particles"], "dt": W.dt,
                    "substeps": W.substeps, "max_per_part": W.max_per_part, "seeds": SEEDS},
         "p50_ms": r["p50_ms"],
         "p99_ms": r["p99_ms"],
@@ -478,37 +564,64 @@ def only_main(args, W, torch):
     r = run_path(args, W, FP64 if args.only == "fp64" else FP32_MIXED, torch)
     print(f"# {W.name} {args.only} {r['updates_per_s']:.3e} updates/s, {r['ms_per_step']:.3f} ms/step",
           file=sys.stderr, flush=True)
-    return {"path": args.only, "workload": W.name, **r, "roofline": roofline(r, W.m, traffic=False)}
+    return {"path": args.only, "workload": W.name, **r, "roofline": roofline(r, W.m)}
 
 
-def workload_main(args, W, torch, with_cpu=True, with_accuracy=True):
+def path_summary(W, r, precision):
+    return {"updates_per_s": r["updates_per_s"], "ms_per_step": r["ms_per_step"],
+            "ms_per_step_eager": r["ms_per_step_eager"], "p50_ms": r["p50_ms"], "p99_ms": r["p99_ms"],
+            "particles_start": r["particles_start"], "particles_end": r["particles"],
+            "kernels_ms_per_step": r["kernels_ms_per_step"], "roofline": roofline(r, W.m),
+            "stage_roofline": stage_roofline(W, r, precision), "stats": r["stats"], "step_info": r["step_info"],
+            "memory_gib": r["memory_bytes"] / 2 ** 30}
+
+
+def run_both(args, W, torch):
     from pfem2_amd import FP32_MIXED, FP64
     print(f"# {W.name}: mesh {W.m.n_cells} cells {W.m.n_vertices} vertices", file=sys.stderr, flush=True)
     r64 = run_path(args, W, FP64, torch)
-    print(f"# fp64 {r64['updates_per_s']:.3e} updates/s, {r64['ms_per_step']:.3f} ms/step "
+    print(f"# {W.name} fp64 {r64['updates_per_s']:.3e} updates/s, {r64['ms_per_step']:.3f} ms/step "
           f"(eager {r64['ms_per_step_eager']:.3f})", file=sys.stderr, flush=True)
     r32 = run_path(args, W, FP32_MIXED, torch)
-    print(f"# fp32-mixed {r32['updates_per_s']:.3e} updates/s, {r32['ms_per_step']:.3f} ms/step "
+    print(f"# {W.name} fp32-mixed {r32['updates_per_s']:.3e} updates/s, {r32['ms_per_step']:.3f} ms/step "
           f"(eager {r32['ms_per_step_eager']:.3f})", file=sys.stderr, flush=True)
+    return r64, r32
+
+
+FP32_TARGETS = {"error_per_field": 1e-5, "speedup_over_fp64": 3.0, "hbm_frac": 0.6}
+
+
+def fp32_targets(acc, speedup, frac):
+    """The FP32-mixed path's contract, met or not, per item."""
+    out = {"speedup_over_fp64": {"value": speedup, "target": FP32_TARGETS["speedup_over_fp64"],
+                                 "met": speedup >= FP32_TARGETS["speedup_over_fp64"]},
+           "stage_hbm_frac": {"value": frac, "target": FP32_TARGETS["hbm_frac"],
+                              "met": frac >= FP32_TARGETS["hbm_frac"]}}
+    if acc:
+        for k in ("V_rel_l2", "V_max_rel", "x_max_rel", "v_max_rel"):
+            out[k] = {"value": acc[k], "target": FP32_TARGETS["error_per_field"],
+                      "met": acc[k] <= FP32_TARGETS["error_per_field"]}
+    return out
+
+
+def workload_main(args, W, torch, with_cpu=True, with_accuracy=True):
+    """Both precision paths of one workload; the FP64 path (the reference's
+    arithmetic, bitwise equal to the oracle) is the line's value."""
+    from pfem2_amd import FP32_MIXED, FP64
+    r64, r32 = run_both(args, W, torch)
     acc = accuracy(W, args.accuracy_steps) if with_accuracy and args.accuracy_steps > 0 else None
     if acc:
         print(f"# fp32-mixed vs fp64: {acc}", file=sys.stderr, flush=True)
     cpu = cpu_baseline(W, args.cpu_seconds) if with_cpu and not args.no_cpu else None
-    out = base_line(args, W, r32, DTYPE32)
-    out["roofline"] = roofline(r32, W.m, traffic=W.name == "cylinder")
+    out = base_line(args, W, r64, "f64")
+    out["roofline"] = roofline(r64, W.m, profile=PROFILES.get((W.name, FP64)))
+    out["stage_roofline"] = stage_roofline(W, r64, FP64)
     out["cpu_baseline"] = cpu
-    out["paths"] = {
-        "fp64": {"updates_per_s": r64["updates_per_s"], "ms_per_step": r64["ms_per_step"],
-                 "ms_per_step_eager": r64["ms_per_step_eager"], "p50_ms": r64["p50_ms"], "p99_ms": r64["p99_ms"],
-                 "kernels_ms_per_step": r64["kernels_ms_per_step"], "roofline": roofline(r64, W.m, False),
-                 "stats": r64["stats"], "step_info": r64["step_info"], "memory_gib": r64["memory_bytes"] / 2 ** 30},
-        "fp32_mixed": {"updates_per_s": r32["updates_per_s"], "ms_per_step": r32["ms_per_step"],
-                       "ms_per_step_eager": r32["ms_per_step_eager"],
-                       "kernels_ms_per_step": r32["kernels_ms_per_step"], "stats": r32["stats"],
-                       "step_info": r32["step_info"], "error_vs_fp64": acc,
-                       "memory_gib": r32["memory_bytes"] / 2 ** 30},
-        "speedup_fp32_over_fp64": r32["updates_per_s"] / r64["updates_per_s"],
-    }
+    s32 = path_summary(W, r32, FP32_MIXED)
+    s32["error_vs_fp64"] = acc
+    speed = r32["updates_per_s"] / r64["updates_per_s"]
+    s32["targets"] = fp32_targets(acc, speed, s32["stage_roofline"]["frac"])
+    out["paths"] = {"fp64": path_summary(W, r64, FP64), "fp32_mixed": s32, "speedup_fp32_over_fp64": speed}
     return out
 
 
@@ -576,7 +689,8 @@ def main():
     ap.add_argument("--accuracy-steps", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
-    ap.add_argument("--workload", choices=["cylinder", "channel", "pipe3d", "moving", "serve"], default="cylinder")
+    ap.add_argument("--workload", choices=["default", "cylinder", "channel", "pipe3d", "moving", "serve"],
+                    default="default")
     ap.add_argument("--serve-requests", type=int, default=256)
     ap.add_argument("--serve-rate", type=float, default=10.0, help="Poisson arrival rate, requests/s")
     ap.add_argument("--serve-budget-gib", type=float, default=200.0)
@@ -611,12 +725,21 @@ def main():
         print(json.dumps(only_main(args, W, torch)))
         return
     if args.workload == "pipe3d":
-        print(json.dumps(workload_main(args, pipe_workload(), torch, with_cpu=True, with_accuracy=False)))
+        print(json.dumps(workload_main(args, pipe_workload(), torch, with_accuracy=False)))
         return
+    if args.workload == "cylinder":
+        print(json.dumps(workload_main(args, cylinder_workload(args), torch)))
+        return
+    # default: the largest single-GPU configuration (the moving-mesh cylinder,
+    # ~400k quads, ~10M particles) with the FP64 path as the value, then the
+    # 100k-cell cylinder channel in both precisions with the FP32-mixed
+    # path's error against FP64, then the request stream
+    out = workload_main(args, moving_workload(), torch, with_accuracy=False)
     if args.workload == "moving":
-        print(json.dumps(workload_main(args, moving_workload(), torch, with_cpu=False, with_accuracy=False)))
+        print(json.dumps(out))
         return
-    out = workload_main(args, cylinder_workload(args), torch)
+    cyl = workload_main(args, cylinder_workload(args), torch, with_cpu=False)
+    out["cylinder100k"] = {"workload": cyl["config"]["workload"], "cells": cyl["config"]["cells"], **cyl["paths"]}
     if not args.no_serve:
         s = serve(args)
         print(f"# serving: p50 {s['p50_ms']:.1f} ms p99 {s['p99_ms']:.1f} ms, {s['updates_per_s']:.3e} updates/s, "

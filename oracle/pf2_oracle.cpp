@@ -482,7 +482,7 @@ void lattice_point(const Oracle& o, const int* ijk, double* xi) {
 // returns -1 when the reference rule cannot place it (particle is deleted)
 int locate(Oracle& o, int c, Particle& p) {
     const int dim = o.dim;
-    double P[MAXV * 3], r[3];
+    double P[MAXV * 3] = {}, r[3];
     gather_cell_vertices(o, c, P);
     if (inverse_map(o, P, p.x, r, &o.n_newton, false) && inside_unit(dim, r)) {
         for (int d = 0; d < dim; ++d) p.xi[d] = r[d];

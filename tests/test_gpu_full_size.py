@@ -1,0 +1,140 @@
+"""Parity at the sizes of the benchmark configurations: the FP64 path is
+bitwise equal to the oracle (cells, ids, counts, id counter, x, xi, v and the
+projected field), the FP32-mixed path is within its per-field error.
+
+  channel 200x20   4,000 quads, 4x4 lattice (64k particles), dt 0.01,
+                   10 substeps, MAX 3, 100 steps with the host FEM stand-in
+                   (fractional step + pressure solve) between particle steps
+  cylinder 100k    101,785 graded quads, 4x4 lattice (1.63M particles),
+                   dt 0.001, 10 substeps, MAX 2: 2 FP64 steps; 10 FP32-mixed
+                   steps against 10 oracle steps
+  pipe 64x64x256   826,368 hexahedra, 2x2x2 lattice (6.6M particles),
+                   dt 0.01, 10 substeps, MAX 2: one FP64 step
+  moving 400k      407,140 quads around an oscillating cylinder, 5x5
+                   lattice (10.2M particles), dt 0.0005: one FP64 step, then
+                   the mesh move (vertex update, re-sort, reseed) fused on the
+                   GPU against the oracle's three calls
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle_binding import Oracle
+from parity_util import assert_particles_match, bitwise_equal
+from pfem2_amd import FP32_MIXED, FP64, ParticleStage
+from pfem2_amd import mesh as M
+
+pytestmark = pytest.mark.gpu
+
+
+def _pair(m, V, dt, S, MAX, q, seed, keep_vext=False, precision=FP64):
+    g = ParticleStage(m, dt, S, MAX, q, precision, keep_velocity_ext=keep_vext)
+    o = Oracle(m, dt, S, MAX, q)
+    cell, xi = M.jittered_lattice(m, q, 0.05, seed)
+    for s in (g, o):
+        s.upload_fields(V, V)
+        s.insert_particles(cell, xi)
+    return g, o, len(cell)
+
+
+def _assert_same(g, o, Vg, Vo, where):
+    rep = assert_particles_match(g.particles(), o.particles(), where=where, bitwise=True)
+    assert np.array_equal(g.cell_counts(), o.cell_counts()), f"{where}: cell counts"
+    assert g.particle_counter() == o.particle_counter(), f"{where}: id counter"
+    sg, so = g.stats(), o.stats()
+    for k in ("lost", "deleted", "spawned"):
+        assert sg[k] == so[k], f"{where}: {k} {sg[k]} != {so[k]}"
+    if Vg is not None:
+        assert bitwise_equal(Vg, Vo), f"{where}: projected field not bitwise equal"
+    print(where, rep["n_gpu"], "particles bitwise equal", flush=True)
+
+
+def test_channel_200x20_100_steps_with_host_fem():
+    from pfem2_amd.fem import ChannelFEM
+    m = M.channel_2d(200, 20, 10.0, 1.0)
+    V = M.poiseuille_2d(m, 1.0, 1.0, 0.01, seed=7)
+    g, o, _ = _pair(m, V, 0.01, 10, 3, [4, 4], seed=11)
+    fg, fo = ChannelFEM(m, 0.01), ChannelFEM(m, 0.01)
+    for s in range(100):
+        Vg = g.step(download=True)
+        Vo = o.step()
+        assert bitwise_equal(Vg, Vo), f"channel step {s}: projected field"
+        g.upload_fields(fg.step(Vg), None)
+        o.upload_fields(fo.step(Vo), None)
+    _assert_same(g, o, None, None, "channel 200x20 after 100 steps")
+    g.close()
+
+
+def _cylinder100k():
+    m = M.cylinder_channel(1.0, seed=2024)
+    V = M.cylinder_flow(m, 1.0, eps=0.01, seed=7)
+    assert m.n_cells == 101785
+    return m, V
+
+
+def test_cylinder_100k_fp64_two_steps():
+    m, V = _cylinder100k()
+    g, o, n0 = _pair(m, V, 0.001, 10, 2, [4, 4], seed=11)
+    assert n0 == 16 * m.n_cells
+    for s in range(2):
+        Vg = g.step(download=True)
+        Vo = o.step()
+        _assert_same(g, o, Vg, Vo, f"cylinder 100k step {s}")
+        g.upload_fields(V, None)
+        o.upload_fields(V, None)
+    g.close()
+
+
+def test_cylinder_100k_fp32_mixed_ten_steps():
+    from test_gpu_fp32_mixed import field_errors
+    m, V = _cylinder100k()
+    g, o, n0 = _pair(m, V, 0.001, 10, 2, [4, 4], seed=11, precision=FP32_MIXED)
+    for _ in range(10):
+        Vg = g.step(download=True)
+        Vo = o.step()
+        g.upload_fields(V, None)
+        o.upload_fields(V, None)
+    err = field_errors(g, o, m, Vg, Vo, n0)
+    print("cylinder 100k fp32-mixed", err, flush=True)
+    assert err["V_rel_l2"] <= 1e-5
+    assert err["x_rel"] <= 1e-5
+    assert err["v_p9999"] <= 1e-5
+    assert err["owner_mismatch"] <= 1e-4
+    assert err["count_mismatch"] <= 1e-3
+    g.close()
+
+
+def test_pipe_64x64x256_fp64_one_step():
+    m = M.pipe_3d(64, 64, 256)
+    V = M.pipe_flow(m, 1.0, 0.5, 0.01, seed=31)
+    assert m.n_cells == 826368
+    g, o, _ = _pair(m, V, 0.01, 10, 2, [2, 2, 2], seed=11)
+    Vg = g.step(download=True)
+    Vo = o.step()
+    _assert_same(g, o, Vg, Vo, "pipe 64x64x256 step 0")
+    g.close()
+
+
+def test_moving_400k_fp64_step_and_mesh_move():
+    m = M.cylinder_channel(2.0, seed=57)
+    V = M.cylinder_flow(m, 1.0, eps=0.01, seed=7)
+    X0 = m.vertices.copy()
+    r = np.linalg.norm(X0 - np.array([0.2, 0.2]), axis=1)
+    w = 0.5 * (1.0 + np.cos(math.pi * np.clip((r - 0.05) / 0.10, 0.0, 1.0)))
+    A, omega, dt = 0.01, 2.0 * math.pi * 2.0, 0.0005
+    Vs = V * (1.0 - w)
+    Vs[1] += w * A * omega
+    g, o, n0 = _pair(m, Vs, dt, 10, 2, [5, 5], seed=11, keep_vext=True)
+    assert n0 == 25 * m.n_cells > 10_000_000
+    Vg = g.step(download=True)
+    Vo = o.step()
+    _assert_same(g, o, Vg, Vo, "moving 400k step 0")
+    X = X0.copy()
+    X[:, 1] += A * math.sin(omega * dt) * w
+    g.move_mesh(X)
+    o.update_vertices(X)
+    o.sort_particles_into_subdomains_and_cells()
+    o.check_empty_cells()
+    _assert_same(g, o, None, None, "moving 400k after the mesh move")
+    g.close()
