@@ -121,6 +121,20 @@ int pf2_move_particles(pf2_handle h);
  * cylinderDyM/pfem2particle.cpp:960-992.  V_out (host, component-major) may
  * be NULL. */
 int pf2_distribute_particle_velocities_to_grid(pf2_handle h, double* V_out);
+/* No counterpart in the reference.  pf2_distribute_particle_velocities_to_grid
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc): the
+ * per-cell projection sums go to a static device array of
+ * PF2_STATIC_SCRATCH_BYTES per GPU instead of the handle's buffer, and the
+ * projected field is written to V_out_device (device memory, dim x n_dofs,
+ * component-major; NULL: kept resident only), with no host synchronisation.
+ * Results equal those of pf2_distribute_particle_velocities_to_grid.
+ * Needs n_cells * 2^dim * (dim + 1) * 8 <= PF2_STATIC_SCRATCH_BYTES
+ * (PF2_E_CAPACITY otherwise) and the handle's device current.  All calls on
+ * one GPU share the array: calls that overlap on one GPU (other streams,
+ * other threads) race. */
+#define PF2_STATIC_SCRATCH_BYTES (64ull << 20)
+int pf2_distribute_particle_velocities_to_grid_static(pf2_handle h, double* V_out_device);
 /* correct -> move -> distribute in one call (the driver sequence of
  * cylinder2D/cylinder2D.cc:865-867); V_out may be NULL. */
 int pf2_step(pf2_handle h, double* V_out);

@@ -46,7 +46,7 @@ EXPORTS = [
     "pf2_last_error", "pf2_create", "pf2_destroy", "pf2_set_params", "pf2_upload_fields",
     "pf2_upload_fields_device", "pf2_download_field", "pf2_seed_particles", "pf2_insert_particles",
     "pf2_correct_particles_velocities", "pf2_move_particles", "pf2_distribute_particle_velocities_to_grid",
-    "pf2_step", "pf2_set_walls", "pf2_sort_particles_into_subdomains_and_cells", "pf2_check_empty_cells",
+    "pf2_distribute_particle_velocities_to_grid_static", "pf2_step", "pf2_set_walls", "pf2_sort_particles_into_subdomains_and_cells", "pf2_check_empty_cells",
     "pf2_update_vertices", "pf2_initialize_maps", "pf2_n_global_particles", "pf2_download_particles",
     "pf2_cell_counts", "pf2_particle_counter", "pf2_stats", "pf2_synchronize", "pf2_set_profiling",
     "pf2_kernel_times", "pf2_kernel_name", "pf2_step_info", "pf2_create_in", "pf2_estimate_bytes",
@@ -88,6 +88,7 @@ def load_library(path=LIB_PATH):
               "pf2_synchronize"):
         getattr(lib, f).argtypes = [vp]
     lib.pf2_distribute_particle_velocities_to_grid.argtypes = [vp, vp]
+    lib.pf2_distribute_particle_velocities_to_grid_static.argtypes = [vp, vp]
     lib.pf2_step.argtypes = [vp, vp]
     lib.pf2_set_walls.argtypes = [vp, ip, vp, vp, vp, vp]
     lib.pf2_update_vertices.argtypes = [vp, vp, ip]
@@ -245,6 +246,14 @@ class ParticleStage:
         out = np.empty((self.dim, self.mesh.n_dofs)) if download else None
         self._check(self._lib.pf2_distribute_particle_velocities_to_grid(self._h, _ptr(out)))
         return out
+
+    def distribute_particle_velocities_to_grid_static(self, V_device_ptr=0):
+        """The projection with its per-cell sums in the library's static
+        device array (no allocation); the field goes to device memory at
+        V_device_ptr (dim x n_dofs, component-major; 0: kept resident only).
+        No host synchronisation."""
+        self._check(self._lib.pf2_distribute_particle_velocities_to_grid_static(
+            self._h, ctypes.c_void_p(V_device_ptr) if V_device_ptr else None))
 
     def step(self, download=False):
         out = np.empty((self.dim, self.mesh.n_dofs)) if download else None

@@ -13,6 +13,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -173,6 +174,7 @@ struct pf2_handle_s {
     double *dX0 = nullptr, *dW = nullptr;
     bool x_host_stale = false;         // the device vertices moved without the host copy
     bool own_stream = true;
+    int device = 0;                    // the device current at creation
 };
 
 namespace {
@@ -563,21 +565,25 @@ int run_move(pf2_handle_s* h, int do_correct, int do_move, bool fuse_sums = fals
     return regroup<DIM>(h, 1, fuse_sums, fuse_sums);
 }
 
+// partial: where the per-cell projection sums go (the handle's own buffer
+// unless a caller supplies another of the same size)
 template <int DIM>
-int run_project(pf2_handle_s* h, bool sums_done = false) {
+int run_project(pf2_handle_s* h, bool sums_done = false, double* partial = nullptr) {
     const int nc = h->n_cells;
     DevWork w = dev_work(h);
+    if (!partial) partial = h->partial;
+    w.partial = partial;
     if (sums_done) {
         // the projection sums came with the regroup (k_cell_write<DIM, false, true>,
         // k_cell_write32<DIM, true>)
     } else if (mixed(h)) {
         Timed t(h, K_PARTIALS32);
-        launch_partials32<DIM>(h->stream, cell_grid(nc), nc, h->off[h->cur_off], rec32(h->buf[h->cur]), h->partial,
+        launch_partials32<DIM>(h->stream, cell_grid(nc), nc, h->off[h->cur_off], rec32(h->buf[h->cur]), partial,
                                h->counters);
     } else {
         Timed t(h, K_PARTIALS);
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cell_partials<DIM>), dim3(cell_grid(nc)), dim3(256), 0, h->stream, nc,
-                           h->off[h->cur_off], dev_particles(h, h->cur), h->partial, h->counters);
+                           h->off[h->cur_off], dev_particles(h, h->cur), partial, h->counters);
     }
     {
         Timed t(h, K_NODE);
@@ -588,10 +594,10 @@ int run_project(pf2_handle_s* h, bool sums_done = false) {
             DevWork wf = w;
             wf.new_off = h->off[h->cur_off];
             hipLaunchKernelGGL(HIP_KERNEL_NAME(k_node<DIM, true>), dim3(grid_for(h->n_dofs, 256) + 1), dim3(256), 0,
-                               h->stream, dev_mesh<DIM>(h), h->partial, h->dV, h->dVold, h->stats, wf, h->qcount);
+                               h->stream, dev_mesh<DIM>(h), partial, h->dV, h->dVold, h->stats, wf, h->qcount);
         } else
             hipLaunchKernelGGL(HIP_KERNEL_NAME(k_node<DIM, false>), dim3(grid_for(h->n_dofs, 256)), dim3(256), 0,
-                               h->stream, dev_mesh<DIM>(h), h->partial, h->dV, h->dVold, h->stats, w, h->qcount);
+                               h->stream, dev_mesh<DIM>(h), partial, h->dV, h->dVold, h->stats, w, h->qcount);
     }
     if (h->n_walls > 0) {
         Timed t(h, K_WALL);
@@ -832,6 +838,49 @@ int run_step(pf2_handle_s* h, const double* field) {
     return step_body(h, field);
 }
 
+// ---------------------------------------------------------------------------
+// Projection sums of pf2_distribute_particle_velocities_to_grid_static: a
+// file-scope device array instead of the handle's buffer, for callers that
+// may not allocate.  Every GPU holds its own copy at its own address; the
+// host learns it once per device with hipGetSymbolAddress and passes that
+// address to the kernels (the array's name is not a device address in host
+// code).  Calls on one GPU share the array, so such calls must not overlap.
+constexpr size_t STATIC_SCRATCH_BYTES = (size_t)PF2_STATIC_SCRATCH_BYTES;
+__device__ __align__(256) double scratch[STATIC_SCRATCH_BYTES / sizeof(double)];
+
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[MAX_DEVICES] = {};
+
+// the device address of `scratch` on the current device (looked up once)
+int scratch_address(void** out) {
+    int dev = 0;
+    HIPC(hipGetDevice(&dev));
+    if (dev < 0 || dev >= MAX_DEVICES) return fail(PF2_E_HIP, "device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        HIPC(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return fail(PF2_E_HIP, "hipGetSymbolAddress returned no address");
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return 0;
+}
+
+// field (node-major) -> caller device memory, component-major
+int field_to_device(pf2_handle_s* h, double* V_out) {
+    const int n = h->n_dofs;
+    if (h->dim == 2)
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_to_comp_major<2>), dim3(grid_for(n, 256)), dim3(256), 0, h->stream, h->dV,
+                           V_out, n);
+    else
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_to_comp_major<3>), dim3(grid_for(n, 256)), dim3(256), 0, h->stream, h->dV,
+                           V_out, n);
+    HIPC(hipGetLastError());
+    return 0;
+}
+
 }  // namespace
 
 // ============================================================================
@@ -937,6 +986,7 @@ int pf2_create_in(const pf2_mesh* mesh, const pf2_params* params, long long capa
     };
     const size_t bytes0 = g_alloc_bytes;
     h->in_block = block != nullptr;
+    if (hipGetDevice(&h->device) != hipSuccess) return bail(fail(PF2_E_HIP, "hipGetDevice failed"));
     if (stream) {
         h->stream = (hipStream_t)stream;
         h->own_stream = false;
@@ -1239,6 +1289,26 @@ int pf2_distribute_particle_velocities_to_grid(pf2_handle h, double* V_out) {
     if (int r = (h->dim == 2 ? run_project<2>(h) : run_project<3>(h))) return r;
     if (V_out) return download_field(h, V_out);
     return 0;
+}
+
+int pf2_distribute_particle_velocities_to_grid_static(pf2_handle h, double* V_out_device) {
+    if (!h) return fail(PF2_E_ARG, "handle is NULL");
+    const size_t need = (size_t)h->n_cells * h->nv * (h->dim + 1) * sizeof(double);
+    if (need > STATIC_SCRATCH_BYTES)
+        return fail(PF2_E_CAPACITY, "projection sums of " + std::to_string(need) + " bytes exceed the static scratch of " +
+                                        std::to_string(STATIC_SCRATCH_BYTES) + " bytes");
+    int cur = 0;
+    HIPC(hipGetDevice(&cur));
+    if (cur != h->device) return fail(PF2_E_STATE, "the handle's device is not the current device");
+    void* sp = nullptr;
+    if (int r = scratch_address(&sp)) return r;
+    // no per-kernel timing here: its events would be created on demand
+    const bool prof = h->profiling;
+    h->profiling = false;
+    int r = h->dim == 2 ? run_project<2>(h, false, (double*)sp) : run_project<3>(h, false, (double*)sp);
+    if (!r && V_out_device) r = field_to_device(h, V_out_device);
+    h->profiling = prof;
+    return r;
 }
 
 int pf2_step(pf2_handle h, double* V_out) {
