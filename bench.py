@@ -445,9 +445,10 @@ def roofline(res, m, profile=None):
             "all_kernels": others}
 
 
-def serve(args):
+def serve(args, rate=None, batched=None):
     """The request stream: independent simulations with Poisson arrivals,
-    served by pf2_server (FP32-mixed, graph replay, admission by footprint).
+    served by pf2_server (FP32-mixed, admission by footprint; batched: every
+    step advances all active simulations with one launch per kernel).
     Moving-mesh requests (~400k quads around a cylinder oscillating with a
     seeded phase, 25 particles per cell) arrive within the first half of the
     stream, so that the memory in use climbs toward the budget."""
@@ -466,7 +467,9 @@ def serve(args):
         V = M.poiseuille_2d(m, umax, 1.0, 0.01, seed=1000 + i)
         dt = 0.5 * min(5.0 / nx, 1.0 / ny) / umax
         reqs.append({"m": m, "V": V, "steps": steps, "dt": dt, "q": q, "moving": None})
-    arrivals = list(np.cumsum(rng.exponential(1.0 / args.serve_rate, len(reqs))))
+    rate = args.serve_rate if rate is None else rate
+    batched = (not args.serve_unbatched) if batched is None else batched
+    arrivals = list(np.cumsum(rng.exponential(1.0 / rate, len(reqs))))
     if args.serve_moving > 0:
         W0 = moving_workload()
         Wd = np.zeros_like(W0.m.vertices)
@@ -481,7 +484,7 @@ def serve(args):
     arrivals = [arrivals[k] for k in order]
     print(f"# serving {len(reqs)} requests ({args.serve_moving} moving) over ~{arrivals[-1]:.1f} s", file=sys.stderr,
           flush=True)
-    srv = Server(hbm_budget=int(args.serve_budget_gib * (1 << 30)), use_graphs=True)
+    srv = Server(hbm_budget=int(args.serve_budget_gib * (1 << 30)), use_graphs=True, batched=batched)
     ids = []
     t0 = time.perf_counter()
     i = 0
@@ -512,9 +515,9 @@ def serve(args):
            "max_ms": float(lat.max()), "updates_per_s": st["updates_per_s"],
            "particle_updates": st["particle_updates"], "wall_s": wall,
            "peak_hbm_gib": st["peak_bytes"] / 2 ** 30, "max_active": int(st["max_active"]),
-           "arrival_rate_per_s": args.serve_rate, "budget_gib": args.serve_budget_gib,
+           "arrival_rate_per_s": rate, "budget_gib": args.serve_budget_gib, "batched": bool(batched),
            "cells_mean": float(np.mean(cells)), "precision": "fp32-mixed",
-           "stream": f"{args.serve_requests} static requests, Poisson arrivals ({args.serve_rate}/s), cells "
+           "stream": f"{args.serve_requests} static requests, Poisson arrivals ({rate}/s), cells "
                      "log-uniform 2k-200k (channel quads), lattice 3/4/5, Umax U[0.2,2], 50-500 steps; "
                      f"{int(mov.sum())} moving-mesh requests (407k quads, 5x5 lattice, 40-100 steps, oscillating "
                      "cylinder) arriving in the first half; 10 substeps, MAX=2"}
@@ -689,12 +692,16 @@ def main():
     ap.add_argument("--accuracy-steps", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
-    ap.add_argument("--workload", choices=["default", "cylinder", "channel", "pipe3d", "moving", "serve"],
-                    default="default")
+    ap.add_argument("--workload", choices=["default", "cylinder", "channel", "pipe3d", "moving", "serve",
+                                           "serve-sweep"], default="default")
     ap.add_argument("--serve-requests", type=int, default=256)
     ap.add_argument("--serve-rate", type=float, default=10.0, help="Poisson arrival rate, requests/s")
     ap.add_argument("--serve-budget-gib", type=float, default=200.0)
     ap.add_argument("--serve-moving", type=int, default=8, help="moving-mesh requests mixed into the stream")
+    ap.add_argument("--serve-unbatched", action="store_true",
+                    help="serve with one step graph per simulation instead of batched steps")
+    ap.add_argument("--serve-rates", default="5,10,20,40,80,160",
+                    help="arrival rates of --workload serve-sweep (requests/s)")
     ap.add_argument("--channel-steps", type=int, default=100)
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--only", choices=["fp64", "fp32"], default=None,
@@ -707,6 +714,22 @@ def main():
     import torch
     if args.copy_calibration:
         copy_calibration(torch)
+        return
+    if args.workload == "serve-sweep":
+        # p50/p99 latency and throughput against the offered load, batched
+        # and per-simulation serving
+        rows = []
+        for rate in [float(x) for x in args.serve_rates.split(",")]:
+            for bt in (True, False):
+                r = serve(args, rate=rate, batched=bt)
+                print(f"# rate {rate}/s batched={bt}: p50 {r['p50_ms']:.1f} ms p99 {r['p99_ms']:.1f} ms "
+                      f"{r['updates_per_s']:.3e} updates/s, max active {r['max_active']}", file=sys.stderr, flush=True)
+                rows.append({k: r[k] for k in ("arrival_rate_per_s", "batched", "p50_ms", "p99_ms", "updates_per_s",
+                                               "max_active", "peak_hbm_gib", "finished", "failed", "wall_s")})
+        out = {"metric": "p50/p99 sim latency against offered load", "value": None, "unit": "ms",
+               "n_gpus": 1, "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": DTYPE32,
+               "data": "synthetic", "config": {"workload": rows and r["stream"]}, "sweep": rows}
+        print(json.dumps(out))
         return
     if args.workload == "serve":
         s = serve(args)

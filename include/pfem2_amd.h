@@ -43,15 +43,22 @@ extern "C" {
 typedef struct pf2_handle_s* pf2_handle;
 
 /* Triangulation + DoFHandler of the reference's pfem2Solver
- * (cylinder2D/pfem2particle.h:170-175; TurekBenchmark/pfem2particle.h:196-205). */
+ * (cylinder2D/pfem2particle.h:170-175; TurekBenchmark/pfem2particle.h:196-205).
+ * The reference's velocity field is FE_Q<dim>(1) (cylinder2D/pfem2particle.cpp:473);
+ * degree 2 (FE_Q<dim>(2) on the same Q1 geometry) extends it: cell_dofs then
+ * lists 3^dim DoFs per cell in deal.II's order (vertices, lines, [faces,]
+ * interior; line and face numbering of deal.II's GeometryInfo). */
 typedef struct pf2_mesh {
     int dim;                    /* 2 or 3 */
     int n_cells;
     int n_vertices;
-    int n_dofs;                 /* Q1 velocity DoFs per component */
+    int n_dofs;                 /* velocity DoFs per component */
     const double* vertices;
     const int* cell_vertices;
-    const int* cell_dofs;       /* NULL: same as cell_vertices */
+    const int* cell_dofs;       /* degree 1: NULL means the same as cell_vertices;
+                                   degree 2: n_cells x 3^dim, required */
+    int degree;                 /* velocity field degree: 1 (0 also means 1) or 2;
+                                   degree 2 runs on PF2_FP64 handles without walls */
 } pf2_mesh;
 
 /* Compile-time knobs of the reference made run-time
@@ -142,6 +149,19 @@ int pf2_step(pf2_handle h, double* V_out);
  * (component-major): V is refreshed from V_device, then correct -> move ->
  * distribute.  No host synchronisation. */
 int pf2_step_with_field(pf2_handle h, const double* V_device);
+/* Batched steps (no counterpart in the reference): pf2_batch_step does
+ * pf2_step_with_field(hs[i], fields[i]) for n simulations with one launch per
+ * kernel of the step for all of them, on the batch's stream (ordered after
+ * each simulation's earlier work, and each simulation's later work after
+ * it).  Each simulation's results equal those of its own step, bit for bit.
+ * The simulations share one dimension and precision, have degree-1 fields
+ * and no walls, and appear once each.  A batch holds up to max_sims
+ * simulations per step; stream NULL creates one.  Calls on one batch must
+ * not overlap. */
+typedef struct pf2_batch_s* pf2_batch;
+int pf2_batch_create(int max_sims, void* stream, pf2_batch* out);
+int pf2_batch_step(pf2_batch b, int n, const pf2_handle* hs, const double* const* fields);
+int pf2_batch_destroy(pf2_batch b);
 /* Capture pf2_step / pf2_step_with_field (when no V_out is requested and
  * profiling is off) as HIP graphs, one per buffer parity, replayed on later
  * steps; results are identical to the eager launches.  pf2_set_params,
@@ -181,6 +201,32 @@ int pf2_set_motion(pf2_handle h, const double* W);
 int pf2_move_mesh_scaled(pf2_handle h, double a);
 /* pfem2ParticleHandler::initialize_maps (TurekBenchmark/pfem2particle.cpp:215-219). */
 int pf2_initialize_maps(pf2_handle h);
+
+/* pfem2Solver::reinterpolate_fields (cylinderDyM/pfem2particle.cpp:1144-1250),
+ * called before the mesh moves (cylinderDyM/cylinderDyM.cc:1157): the
+ * resident V and n_extra further nodal scalar fields on the same DoFs
+ * (extra: n_extra x n_dofs, host, updated in place, e.g. the pressure) take
+ * the values of the current mesh's fields at the vertices displaced by U
+ * (n_vertices x dim).  Vertices on a boundary face keep their values.  The
+ * search starts in the vertex's first cell and tries the rest of its star
+ * (ascending), accepting reference coordinates within 1e-5 of the unit
+ * cell.  V_old is not changed.  Q1 fields. */
+int pf2_reinterpolate_fields(pf2_handle h, const double* U, int n_extra, double* extra);
+
+/* pfem2Solver::calculate_loads (TurekBenchmark/pfem2particle.cpp:920-970;
+ * pressure part: cylinder2D/pfem2particle.cpp:691-735), 2-D Q1: force
+ * coefficients over the boundary faces of a patch, given as (cell, face
+ * number) pairs in the reference's loop order (cells ascending, faces
+ * ascending).  V: velocity (2 x n_dofs, host, component-major; NULL: the
+ * resident V), P: pressure (n_dofs).  out5 = {Cx_nu, Cx_p, Cy_nu, Cy_p,
+ * P[dof_a] - P[dof_e]} with C = 2 F / (rho u_mean^2 diam); a negative DoF
+ * drops its term of the pressure difference. */
+typedef struct pf2_loads_params {
+    double mu, rho, u_mean, diam;
+    int dof_a, dof_e;
+} pf2_loads_params;
+int pf2_calculate_loads(pf2_handle h, const double* V, const double* P, int n_faces, const int* faces,
+                        const pf2_loads_params* lp, double* out5);
 
 /* pfem2ParticleHandler::n_global_particles (cylinder2D/pfem2particle.cpp:186-189). */
 int pf2_n_global_particles(pf2_handle h, long long* n);
@@ -229,6 +275,8 @@ typedef struct pf2_server_config {
     int steps_per_chunk;            /* steps issued per simulation between completion events (0: 8) */
     int chunks_ahead;               /* chunks kept queued per simulation (0: 2) */
     int use_graphs;                 /* replay each simulation's step as a HIP graph */
+    int batched;                    /* step all active simulations of one dimension and precision
+                                       together with pf2_batch_step (degree-1 fields) */
 } pf2_server_config;
 
 const char* pf2_server_last_error(void);

@@ -36,8 +36,9 @@ struct Mesh {
     int dim = 2;
     std::vector<double> vertices;     // n_vertices x dim
     std::vector<int> cell_vertices;   // n_cells x 2^dim, deal.II vertex order
-    std::vector<int> cell_dofs;       // empty: same as cell_vertices
+    std::vector<int> cell_dofs;       // empty: same as cell_vertices (degree 1)
     int n_dofs = 0;                   // 0: one DoF per vertex
+    int degree = 1;                   // velocity field degree (2: cell_dofs has 3^dim DoFs per cell)
     int n_cells() const { return (int)(cell_vertices.size() >> dim); }
     int n_vertices() const { return (int)(vertices.size() / dim); }
 };
@@ -135,6 +136,7 @@ public:
         m.vertices = mesh_.vertices.data();
         m.cell_vertices = mesh_.cell_vertices.data();
         m.cell_dofs = mesh_.cell_dofs.empty() ? nullptr : mesh_.cell_dofs.data();
+        m.degree = mesh_.degree;
         check(pf2_create(&m, &p, 0, &particle_handler.h_));
         particle_handler.dim_ = mesh_.dim;
         particle_handler.n_cells_ = m.n_cells;

@@ -22,6 +22,8 @@
 //                                    special case bc = 0, both components enforced)
 //   * moving-mesh re-sort           cylinderDyM/cylinderDyM.cc:1157-1160,
 //                                   cylinderDyM/pfem2particle.cpp:1252-1273
+//   * field reinterpolation         cylinderDyM/pfem2particle.cpp:1144-1250
+//   * surface loads                 TurekBenchmark/pfem2particle.cpp:920-970
 //
 // deal.II primitives (deal.II >= 9.0 is not vendored; the API used implies
 // 9.2).  Restated from their published definitions:
@@ -39,6 +41,13 @@
 //     vertex_to_cell_centers_directions (normalised vertex-average centre minus
 //     vertex).
 //   * Tensor division by a scalar multiplies by the reciprocal.
+// Velocity fields of degree 2 (FE_Q<dim>(2) on the Q1 geometry) extend the
+// reference, whose fields are FE_Q<dim>(1) (cylinder2D/pfem2particle.cpp:473):
+// 3^dim DoFs per cell in deal.II's order (vertices, lines, [faces,] interior),
+// shape values products of the 1-D quadratic Lagrange polynomials on the
+// nodes 0, 1, 1/2.  Every loop over "the cell's vertices" of the velocity
+// correction, advection, reseed interpolation and projection runs over the
+// cell's DoFs instead; geometry stays Q1.
 // Agreement with deal.II itself below the last ulp is not pinned (deal.II is
 // absent).  All floating-point expressions are written without fused
 // multiply-add; the build uses -ffp-contract=off.
@@ -68,7 +77,8 @@
 
 namespace {
 
-constexpr int MAXV = 8;
+constexpr int MAXV = 8;    // vertices per cell
+constexpr int MAXD = 27;   // velocity DoFs per cell
 
 struct Particle {
     double x[3];
@@ -88,10 +98,12 @@ struct Wall {
 struct Oracle {
     int dim = 2;
     int nvpc = 4;
+    int degree = 1;                 // velocity field degree
+    int ndpc = 4;                   // velocity DoFs per cell
     int n_cells = 0, n_vertices = 0, n_dofs = 0;
     std::vector<double> X;          // n_vertices * dim
     std::vector<int> cell_vertices; // n_cells * nvpc
-    std::vector<int> cell_dofs;     // n_cells * nvpc
+    std::vector<int> cell_dofs;     // n_cells * ndpc
     // vertex star: cells around each vertex in ascending order, and the
     // normalised direction (cell centre - vertex) of each entry
     std::vector<int> star_off, star_cell;
@@ -122,6 +134,36 @@ inline void shape_values(int dim, const double* xi, double* N) {
             const double t = xi[d];
             val *= ((k >> d) & 1) ? t : (1.0 - t);
         }
+        N[k] = val;
+    }
+}
+
+// Degree-2 nodes: unit coordinate index per direction, 0 -> 0, 1 -> 1,
+// 2 -> 1/2.  2-D lines x=0, x=1, y=0, y=1; 3-D lines of the face z=0, of
+// z=1, then those parallel to z; faces x=0, x=1, y=0, y=1, z=0, z=1.
+const int Q2N2[9][2] = {{0, 0}, {1, 0}, {0, 1}, {1, 1}, {0, 2}, {1, 2}, {2, 0}, {2, 1}, {2, 2}};
+const int Q2N3[27][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {1, 1, 0}, {0, 0, 1}, {1, 0, 1}, {0, 1, 1}, {1, 1, 1},
+                         {0, 2, 0}, {1, 2, 0}, {2, 0, 0}, {2, 1, 0}, {0, 2, 1}, {1, 2, 1}, {2, 0, 1}, {2, 1, 1},
+                         {0, 0, 2}, {1, 0, 2}, {0, 1, 2}, {1, 1, 2},
+                         {0, 2, 2}, {1, 2, 2}, {2, 0, 2}, {2, 1, 2}, {2, 2, 0}, {2, 2, 1}, {2, 2, 2}};
+
+// 1-D quadratic Lagrange polynomial of the node at 0 (a = 0), 1 (a = 1) or 1/2 (a = 2)
+inline double lagrange2(int a, double t) {
+    if (a == 0) return (1.0 - t) * (1.0 - 2.0 * t);
+    if (a == 1) return t * (2.0 * t - 1.0);
+    return (4.0 * t) * (1.0 - t);
+}
+
+// shape values of the velocity DoFs of a cell at xi, in DoF order
+inline void field_shape_values(int dim, int degree, const double* xi, double* N) {
+    if (degree == 1) {
+        shape_values(dim, xi, N);
+        return;
+    }
+    const int nd = dim == 2 ? 9 : 27;
+    for (int k = 0; k < nd; ++k) {
+        double val = 1.0;
+        for (int d = 0; d < dim; ++d) val *= lagrange2(dim == 2 ? Q2N2[k][d] : Q2N3[k][d], xi[d]);
         N[k] = val;
     }
 }
@@ -385,8 +427,9 @@ bool clearly_outside(int dim, const double* P, const double* p) {
 // (used for the own-cell test, where the point is usually inside).  Both
 // orders give the same decision because the test is result-preserving.
 bool inverse_map(const Oracle& o, const double* P, const double* p, double* out, uint64_t* counter,
-                 bool filter_first) {
-    if (o.filter && (filter_first || o.dim == 3) && clearly_outside(o.dim, P, p)) return false;
+                 bool filter_first, bool use_filter = true) {
+    const bool filter = o.filter && use_filter;
+    if (filter && (filter_first || o.dim == 3) && clearly_outside(o.dim, P, p)) return false;
     if (o.dim == 2) {
         double r[2];
         if (inverse_closed_form_2d(P, p, r)) {
@@ -397,9 +440,40 @@ bool inverse_map(const Oracle& o, const double* P, const double* p, double* out,
                 return true;
             }
         }
-        if (o.filter && !filter_first && clearly_outside(o.dim, P, p)) return false;
+        if (filter && !filter_first && clearly_outside(o.dim, P, p)) return false;
     }
     return inverse_newton(o.dim, P, p, out, counter);
+}
+
+// GeometryInfo::is_inside_unit_cell(p, eps): -eps <= xi_d <= 1 + eps
+inline bool inside_unit_eps(int dim, const double* xi, double eps) {
+    for (int d = 0; d < dim; ++d)
+        if (!(xi[d] >= -eps && xi[d] <= 1.0 + eps)) return false;
+    return true;
+}
+
+// vertices on a face that belongs to one cell only (face->at_boundary())
+std::vector<char> boundary_vertices(const Oracle& o) {
+    const int nf = 2 * o.dim;
+    std::vector<std::vector<int>> keys;
+    for (int c = 0; c < o.n_cells; ++c)
+        for (int f = 0; f < nf; ++f) {
+            std::vector<int> key;
+            for (int k = 0; k < o.nvpc; ++k)
+                if (((k >> (f >> 1)) & 1) == (f & 1)) key.push_back(o.cell_vertices[(size_t)c * o.nvpc + k]);
+            std::sort(key.begin(), key.end());
+            keys.push_back(key);
+        }
+    std::sort(keys.begin(), keys.end());
+    std::vector<char> bnd(o.n_vertices, 0);
+    for (size_t i = 0; i < keys.size();) {
+        size_t j = i;
+        while (j < keys.size() && keys[j] == keys[i]) ++j;
+        if (j - i == 1)
+            for (int v : keys[i]) bnd[v] = 1;
+        i = j;
+    }
+    return bnd;
 }
 
 inline bool inside_unit(int dim, const double* xi) {
@@ -450,11 +524,11 @@ void compute_star_dirs(Oracle& o) {
 // ---- particle stage -------------------------------------------------------
 
 void interp(const Oracle& o, const std::vector<double>& F, int c, const double* xi, double* out) {
-    double N[MAXV];
-    shape_values(o.dim, xi, N);
+    double N[MAXD];
+    field_shape_values(o.dim, o.degree, xi, N);
     for (int d = 0; d < o.dim; ++d) out[d] = 0.0;
-    for (int k = 0; k < o.nvpc; ++k) {
-        const int dof = o.cell_dofs[(size_t)c * o.nvpc + k];
+    for (int k = 0; k < o.ndpc; ++k) {
+        const int dof = o.cell_dofs[(size_t)c * o.ndpc + k];
         for (int d = 0; d < o.dim; ++d) out[d] += N[k] * F[(size_t)d * o.n_dofs + dof];
     }
 }
@@ -616,17 +690,21 @@ void check_cell_for_empty_parts(Oracle& o, int c) {
 extern "C" {
 
 void* orc_create(int dim, int n_cells, int n_vertices, const double* vertices, const int* cell_vertices,
-                 int n_dofs, const int* cell_dofs) {
+                 int n_dofs, const int* cell_dofs, int degree) {
     if (dim != 2 && dim != 3) return nullptr;
+    if (degree != 1 && degree != 2) return nullptr;
+    if (degree == 2 && !cell_dofs) return nullptr;
     Oracle* o = new Oracle();
     o->dim = dim;
     o->nvpc = 1 << dim;
+    o->degree = degree;
+    o->ndpc = degree == 1 ? o->nvpc : (dim == 2 ? 9 : 27);
     o->n_cells = n_cells;
     o->n_vertices = n_vertices;
     o->n_dofs = n_dofs;
     o->X.assign(vertices, vertices + (size_t)n_vertices * dim);
     o->cell_vertices.assign(cell_vertices, cell_vertices + (size_t)n_cells * o->nvpc);
-    if (cell_dofs) o->cell_dofs.assign(cell_dofs, cell_dofs + (size_t)n_cells * o->nvpc);
+    if (cell_dofs) o->cell_dofs.assign(cell_dofs, cell_dofs + (size_t)n_cells * o->ndpc);
     else o->cell_dofs = o->cell_vertices;
     o->V.assign((size_t)dim * n_dofs, 0.0);
     o->Vold.assign((size_t)dim * n_dofs, 0.0);
@@ -692,12 +770,12 @@ void orc_insert(void* h, int n, const int* cell, const double* xi, const double*
 // correct_particles_velocities (TurekBenchmark/pfem2particle.cpp:792-811)
 void orc_correct(void* h) {
     Oracle& o = *static_cast<Oracle*>(h);
-    double N[MAXV];
+    double N[MAXD];
     for (int c = 0; c < o.n_cells; ++c)
         for (Particle& p : o.cells[c]) {
-            shape_values(o.dim, p.xi, N);
-            for (int k = 0; k < o.nvpc; ++k) {
-                const int dof = o.cell_dofs[(size_t)c * o.nvpc + k];
+            field_shape_values(o.dim, o.degree, p.xi, N);
+            for (int k = 0; k < o.ndpc; ++k) {
+                const int dof = o.cell_dofs[(size_t)c * o.ndpc + k];
                 for (int d = 0; d < o.dim; ++d)
                     p.v[d] = p.v[d] + N[k] * (o.V[(size_t)d * o.n_dofs + dof] - o.Vold[(size_t)d * o.n_dofs + dof]);
             }
@@ -737,21 +815,21 @@ void orc_project(void* h, double* Vout) {
     Oracle& o = *static_cast<Oracle*>(h);
     const int dim = o.dim;
     std::vector<double> nodeV((size_t)dim * o.n_dofs, 0.0), nodeW(o.n_dofs, 0.0);
-    double N[MAXV];
+    double N[MAXD];
     for (int c = 0; c < o.n_cells; ++c) {
-        double lv[MAXV][3], lw[MAXV];
-        for (int k = 0; k < o.nvpc; ++k) {
+        double lv[MAXD][3], lw[MAXD];
+        for (int k = 0; k < o.ndpc; ++k) {
             lw[k] = 0.0;
             for (int d = 0; d < dim; ++d) lv[k][d] = 0.0;
         }
-        for (int k = 0; k < o.nvpc; ++k)
+        for (int k = 0; k < o.ndpc; ++k)
             for (const Particle& p : o.cells[c]) {
-                shape_values(dim, p.xi, N);
+                field_shape_values(dim, o.degree, p.xi, N);
                 for (int d = 0; d < dim; ++d) lv[k][d] += N[k] * p.v[d];
                 lw[k] += N[k];
             }
-        for (int k = 0; k < o.nvpc; ++k) {
-            const int dof = o.cell_dofs[(size_t)c * o.nvpc + k];
+        for (int k = 0; k < o.ndpc; ++k) {
+            const int dof = o.cell_dofs[(size_t)c * o.ndpc + k];
             for (int d = 0; d < dim; ++d) nodeV[(size_t)d * o.n_dofs + dof] += lv[k][d];
             nodeW[dof] += lw[k];
         }
@@ -824,6 +902,155 @@ void orc_update_vertices(void* h, const double* vertices, int recompute_maps) {
 
 void orc_initialize_maps(void* h) { compute_star_dirs(*static_cast<Oracle*>(h)); }
 
+// pfem2Solver::reinterpolate_fields (cylinderDyM/pfem2particle.cpp:1144-1250),
+// run before the mesh moves: every vertex DoF of V (and of the n_extra
+// further scalar fields, n_extra x n_dofs, e.g. the pressure) takes the
+// field of the current mesh at the vertex displaced by U (n_vertices x dim).
+// Vertices in ascending order (the std::map of moving nodes); a boundary
+// vertex keeps its values.  The search starts in the vertex's first cell
+// (the smallest index of its star, where the reference's node records its
+// cell), then tries the other cells of the star in ascending order; a cell
+// holds the point when its inverse map lies in the unit cell widened by
+// 1e-5.  A failed inverse map leaves the reference coordinates as they
+// were (the reference catches the exception and keeps localCoords): in the
+// first cell that is the initial (0, 0[, 0]), which counts as inside.  A
+// vertex that no cell holds keeps its values, and so does a DoF that no
+// vertex carries (a vertex of no cell has no DoF).  Q1 fields.
+void orc_reinterpolate_fields(void* h, const double* U, int n_extra, double* extra) {
+    Oracle& o = *static_cast<Oracle*>(h);
+    const int dim = o.dim, nd = o.n_dofs;
+    const std::vector<char> bnd = boundary_vertices(o);
+    std::vector<double> nV((size_t)dim * nd, 0.0), nE((size_t)n_extra * nd, 0.0), w(nd, 0.0);
+    double P[MAXV * 3], r[3], N[MAXV];
+    for (int v = 0; v < o.n_vertices; ++v) {
+        if (o.star_off[v] == o.star_off[v + 1]) continue;
+        const int c0 = o.star_cell[o.star_off[v]];
+        int k0 = 0;
+        while (o.cell_vertices[(size_t)c0 * o.nvpc + k0] != v) ++k0;
+        const int dof = o.cell_dofs[(size_t)c0 * o.nvpc + k0];
+        auto keep = [&]() {
+            for (int d = 0; d < dim; ++d) nV[(size_t)d * nd + dof] += o.V[(size_t)d * nd + dof];
+            for (int e = 0; e < n_extra; ++e) nE[(size_t)e * nd + dof] += extra[(size_t)e * nd + dof];
+            w[dof] += 1;
+        };
+        if (bnd[v]) {
+            keep();
+            continue;
+        }
+        double p[3];
+        for (int d = 0; d < dim; ++d) p[d] = o.X[(size_t)v * dim + d] + U[(size_t)v * dim + d];
+        double xi[3] = {0.0, 0.0, 0.0};
+        int cell = c0;
+        bool found = false;
+        gather_cell_vertices(o, c0, P);
+        if (inverse_map(o, P, p, r, &o.n_newton, false, false))
+            for (int d = 0; d < dim; ++d) xi[d] = r[d];
+        if (inside_unit_eps(dim, xi, 1e-5)) found = true;
+        else
+            for (int e = o.star_off[v]; e < o.star_off[v + 1]; ++e) {
+                const int c = o.star_cell[e];
+                if (c == c0) continue;
+                gather_cell_vertices(o, c, P);
+                if (inverse_map(o, P, p, r, &o.n_newton, false, false))
+                    for (int d = 0; d < dim; ++d) xi[d] = r[d];
+                if (inside_unit_eps(dim, xi, 1e-5)) {
+                    cell = c;
+                    found = true;
+                    break;
+                }
+            }
+        if (!found) {
+            keep();
+            continue;
+        }
+        shape_values(dim, xi, N);
+        for (int k = 0; k < o.nvpc; ++k) {
+            const int q = o.cell_dofs[(size_t)cell * o.nvpc + k];
+            for (int d = 0; d < dim; ++d) nV[(size_t)d * nd + dof] += N[k] * o.V[(size_t)d * nd + q];
+            for (int e = 0; e < n_extra; ++e) nE[(size_t)e * nd + dof] += N[k] * extra[(size_t)e * nd + q];
+        }
+        w[dof] += 1;
+    }
+    for (int i = 0; i < nd; ++i) {
+        if (w[i] == 0.0) {
+            for (int d = 0; d < dim; ++d) nV[(size_t)d * nd + i] = o.V[(size_t)d * nd + i];
+            for (int e = 0; e < n_extra; ++e) nE[(size_t)e * nd + i] = extra[(size_t)e * nd + i];
+            continue;
+        }
+        for (int d = 0; d < dim; ++d) nV[(size_t)d * nd + i] /= w[i];
+        for (int e = 0; e < n_extra; ++e) nE[(size_t)e * nd + i] /= w[i];
+    }
+    o.V = nV;
+    if (n_extra > 0) std::memcpy(extra, nE.data(), sizeof(double) * nE.size());
+}
+
+// pfem2Solver::calculate_loads (TurekBenchmark/pfem2particle.cpp:920-970), 2-D:
+// over the given boundary faces (cell, face number) in order, two Gauss
+// points per face: pressure and the normal derivative of the tangential
+// velocity from the Q1 fields V (2 x n_dofs) and P (n_dofs); forces
+// Fx_nu += mu dVt/dn n_y JxW, Fx_p -= p n_x JxW, Fy_nu -= mu dVt/dn n_x JxW,
+// Fy_p -= p n_y JxW; out = {Cx_nu, Cx_p, Cy_nu, Cy_p} = 2 F / (rho u_mean^2 diam)
+// and the pressure difference P[dof_a] - P[dof_e] (a negative index: the
+// term is absent).  FEFaceValues of MappingQ1 restated: face points (0, s),
+// (1, s), (s, 0), (s, 1) for faces 0-3, s = 1/2 -+ 1/(2 sqrt 3), weight 1/2;
+// tangent J t_f with t_f = (0,-1), (0,1), (1,0), (-1,0); boundary form
+// (t_y, -t_x), its length times the weight is JxW, the unit normal is it
+// divided by that length; gradients J^-T grad_xi N.
+void orc_calculate_loads(void* h, const double* V, const double* Pf, int n_faces, const int* faces, double mu,
+                         double rho, double u_mean, double diam, int dof_a, int dof_e, double* out) {
+    Oracle& o = *static_cast<Oracle*>(h);
+    const int nd = o.n_dofs;
+    const double g = 0.5 / std::sqrt(3.0);
+    const double sq[2] = {0.5 - g, 0.5 + g};
+    const double tf[4][2] = {{0.0, -1.0}, {0.0, 1.0}, {1.0, 0.0}, {-1.0, 0.0}};
+    double Fx_nu = 0.0, Fx_p = 0.0, Fy_nu = 0.0, Fy_p = 0.0;
+    double P[MAXV * 3], N[MAXV], G[MAXV * 3];
+    for (int f = 0; f < n_faces; ++f) {
+        const int c = faces[2 * f], fn = faces[2 * f + 1];
+        gather_cell_vertices(o, c, P);
+        for (int q = 0; q < 2; ++q) {
+            double xi[2];
+            xi[0] = fn == 0 ? 0.0 : (fn == 1 ? 1.0 : sq[q]);
+            xi[1] = fn == 2 ? 0.0 : (fn == 3 ? 1.0 : sq[q]);
+            shape_values(2, xi, N);
+            shape_grads(2, xi, G);
+            // J[i][j] = d x_i / d xi_j, sums over vertices from zero
+            double J[2][2] = {{0.0, 0.0}, {0.0, 0.0}};
+            for (int k = 0; k < 4; ++k)
+                for (int i = 0; i < 2; ++i)
+                    for (int j = 0; j < 2; ++j) J[i][j] += P[k * 2 + i] * G[k * 2 + j];
+            const double t0 = J[0][0] * tf[fn][0] + J[0][1] * tf[fn][1];
+            const double t1 = J[1][0] * tf[fn][0] + J[1][1] * tf[fn][1];
+            const double len = std::sqrt(t1 * t1 + t0 * t0);
+            const double nx = t1 / len, ny = -t0 / len;
+            const double JxW = len * 0.5;
+            const double det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+            const double inv = 1.0 / det;
+            double pv = 0.0, dvt = 0.0;
+            for (int k = 0; k < 4; ++k) {
+                const int dof = o.cell_dofs[(size_t)c * 4 + k];
+                const double gx = (J[1][1] * G[k * 2] - J[1][0] * G[k * 2 + 1]) * inv;
+                const double gy = (J[0][0] * G[k * 2 + 1] - J[0][1] * G[k * 2]) * inv;
+                pv += Pf[dof] * N[k];
+                dvt += (V[dof] * ny - V[(size_t)nd + dof] * nx) * (gx * nx + gy * ny);
+            }
+            Fx_nu += mu * dvt * ny * JxW;
+            Fx_p -= pv * nx * JxW;
+            Fy_nu -= mu * dvt * nx * JxW;
+            Fy_p -= pv * ny * JxW;
+        }
+    }
+    const double s = rho * u_mean * u_mean * diam;
+    out[0] = 2.0 * Fx_nu / s;
+    out[1] = 2.0 * Fx_p / s;
+    out[2] = 2.0 * Fy_nu / s;
+    out[3] = 2.0 * Fy_p / s;
+    double pd = 0.0;
+    if (dof_a >= 0) pd += Pf[dof_a];
+    if (dof_e >= 0) pd -= Pf[dof_e];
+    out[4] = pd;
+}
+
 long long orc_count(void* h) {
     Oracle& o = *static_cast<Oracle*>(h);
     long long n = 0;
@@ -889,6 +1116,9 @@ int orc_clearly_outside(int dim, const double* P, const double* p) { return clea
 void orc_forward_map(int dim, const double* P, const double* xi, double* x) { forward_map(dim, P, xi, x); }
 
 void orc_shape_values(int dim, const double* xi, double* N) { shape_values(dim, xi, N); }
+void orc_field_shape_values(int dim, int degree, const double* xi, double* N) {
+    field_shape_values(dim, degree, xi, N);
+}
 
 // star of a vertex: writes up to cap entries, returns the star size
 int orc_star(void* h, int vertex, int* cells, double* dirs, int cap) {

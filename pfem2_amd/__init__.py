@@ -32,7 +32,23 @@ class Pf2Error(RuntimeError):
 class _Mesh(ctypes.Structure):
     _fields_ = [("dim", ctypes.c_int), ("n_cells", ctypes.c_int), ("n_vertices", ctypes.c_int),
                 ("n_dofs", ctypes.c_int), ("vertices", ctypes.c_void_p), ("cell_vertices", ctypes.c_void_p),
-                ("cell_dofs", ctypes.c_void_p)]
+                ("cell_dofs", ctypes.c_void_p), ("degree", ctypes.c_int)]
+
+
+def _mesh_arrays(m):
+    """Contiguous vertex, cell-vertex and (degree 2) cell-DoF arrays of a
+    mesh object; the caller keeps them alive while the struct is in use."""
+    vertices = _f64(m.vertices)
+    cells = np.ascontiguousarray(m.cells, dtype=np.int32)
+    deg = getattr(m, "degree", 1)
+    cd = np.ascontiguousarray(m.cell_dofs, dtype=np.int32) if deg == 2 else None
+    return vertices, cells, cd
+
+
+def _mesh_struct(m, arrays):
+    vertices, cells, cd = arrays
+    return _Mesh(m.dim, m.n_cells, m.n_vertices, m.n_dofs, _ptr(vertices), _ptr(cells), _ptr(cd),
+                 getattr(m, "degree", 1))
 
 
 class _Params(ctypes.Structure):
@@ -46,7 +62,9 @@ EXPORTS = [
     "pf2_last_error", "pf2_create", "pf2_destroy", "pf2_set_params", "pf2_upload_fields",
     "pf2_upload_fields_device", "pf2_download_field", "pf2_seed_particles", "pf2_insert_particles",
     "pf2_correct_particles_velocities", "pf2_move_particles", "pf2_distribute_particle_velocities_to_grid",
-    "pf2_distribute_particle_velocities_to_grid_static", "pf2_step", "pf2_set_walls", "pf2_sort_particles_into_subdomains_and_cells", "pf2_check_empty_cells",
+    "pf2_distribute_particle_velocities_to_grid_static", "pf2_step", "pf2_set_walls",
+    "pf2_sort_particles_into_subdomains_and_cells", "pf2_check_empty_cells", "pf2_reinterpolate_fields",
+    "pf2_calculate_loads", "pf2_batch_create", "pf2_batch_step", "pf2_batch_destroy",
     "pf2_update_vertices", "pf2_initialize_maps", "pf2_n_global_particles", "pf2_download_particles",
     "pf2_cell_counts", "pf2_particle_counter", "pf2_stats", "pf2_synchronize", "pf2_set_profiling",
     "pf2_kernel_times", "pf2_kernel_name", "pf2_step_info", "pf2_create_in", "pf2_estimate_bytes",
@@ -57,9 +75,15 @@ EXPORTS = [
 ]
 
 
+class _LoadsParams(ctypes.Structure):
+    _fields_ = [("mu", ctypes.c_double), ("rho", ctypes.c_double), ("u_mean", ctypes.c_double),
+                ("diam", ctypes.c_double), ("dof_a", ctypes.c_int), ("dof_e", ctypes.c_int)]
+
+
 class _ServerConfig(ctypes.Structure):
     _fields_ = [("hbm_budget", ctypes.c_ulonglong), ("max_active", ctypes.c_int),
-                ("steps_per_chunk", ctypes.c_int), ("chunks_ahead", ctypes.c_int), ("use_graphs", ctypes.c_int)]
+                ("steps_per_chunk", ctypes.c_int), ("chunks_ahead", ctypes.c_int), ("use_graphs", ctypes.c_int),
+                ("batched", ctypes.c_int)]
 
 
 def load_library(path=LIB_PATH):
@@ -92,6 +116,11 @@ def load_library(path=LIB_PATH):
     lib.pf2_step.argtypes = [vp, vp]
     lib.pf2_set_walls.argtypes = [vp, ip, vp, vp, vp, vp]
     lib.pf2_update_vertices.argtypes = [vp, vp, ip]
+    lib.pf2_reinterpolate_fields.argtypes = [vp, vp, ip, vp]
+    lib.pf2_batch_create.argtypes = [ip, vp, ctypes.POINTER(vp)]
+    lib.pf2_batch_step.argtypes = [vp, ip, vp, vp]
+    lib.pf2_batch_destroy.argtypes = [vp]
+    lib.pf2_calculate_loads.argtypes = [vp, vp, vp, ip, vp, ctypes.POINTER(_LoadsParams), vp]
     lib.pf2_move_mesh.argtypes = [vp, vp, ip]
     lib.pf2_set_motion.argtypes = [vp, vp]
     lib.pf2_move_mesh_scaled.argtypes = [vp, dp]
@@ -145,7 +174,7 @@ def estimate_bytes(m, dt, substeps, max_per_part, quantities, precision=FP64, ke
     """Device bytes a simulation of this mesh and parameters takes (upper bound)."""
     lib = load_library()
     p = _params_of(m.dim, dt, substeps, max_per_part, quantities, precision, keep_velocity_ext)
-    ms = _Mesh(m.dim, m.n_cells, m.n_vertices, m.n_dofs, None, None, None)
+    ms = _Mesh(m.dim, m.n_cells, m.n_vertices, m.n_dofs, None, None, None, getattr(m, "degree", 1))
     out = ctypes.c_ulonglong()
     rc = lib.pf2_estimate_bytes(ctypes.byref(ms), ctypes.byref(p), capacity, ctypes.byref(out))
     if rc != 0:
@@ -168,12 +197,11 @@ class ParticleStage:
         self._lib = load_library()
         self.mesh = m
         self.dim = m.dim
-        self._vertices = _f64(m.vertices)
-        self._cells = np.ascontiguousarray(m.cells, dtype=np.int32)
+        self._arrays = _mesh_arrays(m)
         q = list(quantities) + [1] * (3 - len(quantities))
         self._params = _Params(dt, substeps, max_per_part, (ctypes.c_int * 3)(*q[:3]), precision,
                                1 if keep_velocity_ext else 0)
-        ms = _Mesh(m.dim, m.n_cells, m.n_vertices, m.n_dofs, _ptr(self._vertices), _ptr(self._cells), None)
+        ms = _mesh_struct(m, self._arrays)
         h = ctypes.c_void_p()
         self._check(self._lib.pf2_create(ctypes.byref(ms), ctypes.byref(self._params), capacity, ctypes.byref(h)))
         self._h = h
@@ -306,6 +334,30 @@ class ParticleStage:
     def sort_particles_into_subdomains_and_cells(self):
         self._check(self._lib.pf2_sort_particles_into_subdomains_and_cells(self._h))
 
+    def reinterpolate_fields(self, U, extra=None):
+        """reinterpolate_fields: V (resident) and the extra nodal scalar
+        fields (n_extra x n_dofs, returned updated) at the vertices displaced
+        by U (n_vertices x dim), on the mesh before it moves."""
+        U = _f64(U, (self.mesh.n_vertices, self.dim))
+        if extra is None:
+            self._check(self._lib.pf2_reinterpolate_fields(self._h, _ptr(U), 0, None))
+            return None
+        E = np.array(extra, dtype=np.float64, copy=True).reshape(-1, self.mesh.n_dofs)
+        self._check(self._lib.pf2_reinterpolate_fields(self._h, _ptr(U), E.shape[0], _ptr(E)))
+        return E
+
+    def calculate_loads(self, P, faces, mu, rho, u_mean, diam, dof_a=-1, dof_e=-1, V=None):
+        """calculate_loads over boundary faces (cell, face number):
+        {Cx_nu, Cx_p, Cy_nu, Cy_p, P[dof_a] - P[dof_e]}; V None: the resident V."""
+        P = _f64(P, (self.mesh.n_dofs,))
+        faces = np.ascontiguousarray(faces, dtype=np.int32).reshape(-1, 2)
+        Vv = None if V is None else _f64(V, (self.dim, self.mesh.n_dofs))
+        lp = _LoadsParams(mu, rho, u_mean, diam, dof_a, dof_e)
+        out = np.zeros(5)
+        self._check(self._lib.pf2_calculate_loads(self._h, _ptr(Vv), _ptr(P), len(faces), _ptr(faces),
+                                                  ctypes.byref(lp), _ptr(out)))
+        return out
+
     def initialize_maps(self):
         self._check(self._lib.pf2_initialize_maps(self._h))
 
@@ -366,6 +418,41 @@ class ParticleStage:
         return {self._lib.pf2_kernel_name(i).decode(): (float(ms[i]), int(n[i])) for i in range(k) if n[i] > 0}
 
 
+class Batch:
+    """Steps of several ParticleStage simulations (one dimension and
+    precision, Q1 fields, no walls) with one launch per kernel for all of
+    them (pf2_batch_step); each one's results equal its own step's."""
+
+    def __init__(self, max_sims):
+        self._lib = load_library()
+        b = ctypes.c_void_p()
+        rc = self._lib.pf2_batch_create(max_sims, None, ctypes.byref(b))
+        if rc != 0:
+            raise Pf2Error(f"pf2 error {rc}: {self._lib.pf2_last_error().decode()}")
+        self._b = b
+
+    def step(self, stages, field_ptrs):
+        """One step of every stage, stage i refreshing V from the device
+        field at field_ptrs[i] (component-major, e.g. a tensor's data_ptr())."""
+        n = len(stages)
+        hs = (ctypes.c_void_p * n)(*[st._h for st in stages])
+        fs = (ctypes.c_void_p * n)(*field_ptrs)
+        rc = self._lib.pf2_batch_step(self._b, n, hs, fs)
+        if rc != 0:
+            raise Pf2Error(f"pf2 error {rc}: {self._lib.pf2_last_error().decode()}")
+
+    def close(self):
+        if getattr(self, "_b", None):
+            self._lib.pf2_batch_destroy(self._b)
+            self._b = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class Pfem2Solver:
     """Mirror of the reference's pfem2Solver particle interface
     (cylinder2D/pfem2particle.h:115-192): the driver sets `solution` (nodal V,
@@ -409,11 +496,15 @@ class Server:
     submit() copies a request (mesh, parameters, fixed field V, step count);
     poll() admits, issues and retires without waiting for the GPU; result()
     returns the last projected field, the final particle count and the
-    latency from submission to completion."""
+    latency from submission to completion.  batched: all active simulations
+    of one dimension and precision advance together, one launch per kernel
+    of the step (pf2_batch_step); otherwise each replays its own step graph
+    on its own stream."""
 
-    def __init__(self, hbm_budget=0, max_active=0, steps_per_chunk=0, chunks_ahead=0, use_graphs=True):
+    def __init__(self, hbm_budget=0, max_active=0, steps_per_chunk=0, chunks_ahead=0, use_graphs=True, batched=True):
         self._lib = load_library()
-        cfg = _ServerConfig(hbm_budget, max_active, steps_per_chunk, chunks_ahead, 1 if use_graphs else 0)
+        cfg = _ServerConfig(hbm_budget, max_active, steps_per_chunk, chunks_ahead, 1 if use_graphs else 0,
+                            1 if batched else 0)
         h = ctypes.c_void_p()
         self._check(self._lib.pf2_server_create(ctypes.byref(cfg), ctypes.byref(h)))
         self._s = h
@@ -435,10 +526,9 @@ class Server:
             pass
 
     def submit(self, m, V, n_steps, dt, substeps, max_per_part, quantities, precision=FP64):
-        vertices = _f64(m.vertices)
-        cells = np.ascontiguousarray(m.cells, dtype=np.int32)
+        arrays = _mesh_arrays(m)
         V = _f64(V, (m.dim, m.n_dofs))
-        ms = _Mesh(m.dim, m.n_cells, m.n_vertices, m.n_dofs, _ptr(vertices), _ptr(cells), None)
+        ms = _mesh_struct(m, arrays)
         p = _params_of(m.dim, dt, substeps, max_per_part, quantities, precision)
         rid = ctypes.c_longlong()
         self._check(self._lib.pf2_server_submit(self._s, ctypes.byref(ms), ctypes.byref(p), _ptr(V), n_steps,
@@ -450,11 +540,10 @@ class Server:
                       precision=FP64):
         """A request whose mesh moves to X0 + amplitude sin(omega k dt + phase) W
         after each step k."""
-        vertices = _f64(m.vertices)
-        cells = np.ascontiguousarray(m.cells, dtype=np.int32)
+        arrays = _mesh_arrays(m)
         V = _f64(V, (m.dim, m.n_dofs))
         W = _f64(W, (m.n_vertices, m.dim))
-        ms = _Mesh(m.dim, m.n_cells, m.n_vertices, m.n_dofs, _ptr(vertices), _ptr(cells), None)
+        ms = _mesh_struct(m, arrays)
         p = _params_of(m.dim, dt, substeps, max_per_part, quantities, precision, True)
         rid = ctypes.c_longlong()
         self._check(self._lib.pf2_server_submit_moving(self._s, ctypes.byref(ms), ctypes.byref(p), _ptr(V), n_steps,

@@ -36,6 +36,104 @@ PF2_HD void shape(const double* xi, double* N) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// Velocity fields of degree DEG on the Q1 geometry.  DEG = 1: the vertex
+// values above.  DEG = 2 (FE_Q<dim>(2)): 3^dim nodes per cell in deal.II's
+// order -- vertices (lexicographic), then lines, then (3-D) faces, then the
+// interior.  Node k sits at unit coordinates (t(a_0), t(a_1)[, t(a_2)]) with
+// axis index a in {0, 1, 2} for t = 0, 1, 1/2, and
+//   N_k(xi) = L_{a_0}(xi_0) L_{a_1}(xi_1) [L_{a_2}(xi_2)]   (left to right)
+//   L_0(t) = (1 - t)(1 - 2t),  L_1(t) = t (2t - 1),  L_2(t) = 4t (1 - t)
+// each factor evaluated as written (two roundings and one product).
+// 2-D lines: 0 x=0, 1 x=1, 2 y=0, 3 y=1.  3-D lines: 0-3 those of the face
+// z=0, 4-7 of z=1, 8-11 parallel to z at the vertices 0-3; faces 0-5:
+// x=0, x=1, y=0, y=1, z=0, z=1.
+template <int DIM, int DEG>
+struct QF {
+    static constexpr int ND = DEG == 1 ? (1 << DIM) : (DIM == 2 ? 9 : 27);
+};
+
+// axis index of node k (of a degree-2 cell) in direction d
+PF2_HD int q2_axis(int dim, int k, int d) {
+    // 2-D: 9 nodes x 2 axes; 3-D: 27 nodes x 3 axes
+    constexpr signed char A2[9][2] = {{0, 0}, {1, 0}, {0, 1}, {1, 1}, {0, 2}, {1, 2}, {2, 0}, {2, 1}, {2, 2}};
+    constexpr signed char A3[27][3] = {
+        {0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {1, 1, 0}, {0, 0, 1}, {1, 0, 1}, {0, 1, 1}, {1, 1, 1},   // vertices
+        {0, 2, 0}, {1, 2, 0}, {2, 0, 0}, {2, 1, 0},                                           // lines z = 0
+        {0, 2, 1}, {1, 2, 1}, {2, 0, 1}, {2, 1, 1},                                           // lines z = 1
+        {0, 0, 2}, {1, 0, 2}, {0, 1, 2}, {1, 1, 2},                                           // lines along z
+        {0, 2, 2}, {1, 2, 2}, {2, 0, 2}, {2, 1, 2}, {2, 2, 0}, {2, 2, 1},                     // faces
+        {2, 2, 2}};                                                                           // interior
+    return dim == 2 ? A2[k][d] : A3[k][d];
+}
+
+// the 1-D Lagrange factor of axis index a as p(t) * r(t) with
+// p = c[0] t + c[1] and r = c[2] t + c[3]; every product c t is exact
+// (c in {+-1, +-2, 4}), so fma gives the expression of the definition
+PF2_HD void lag2_coeffs(int a, double* c) {
+    c[0] = a == 0 ? -1.0 : (a == 1 ? 1.0 : 4.0);
+    c[1] = a == 0 ? 1.0 : 0.0;
+    c[2] = a == 0 ? -2.0 : (a == 1 ? 2.0 : -1.0);
+    c[3] = a == 0 ? 1.0 : (a == 1 ? -1.0 : 1.0);
+}
+
+PF2_HD double lag2(int a, double t) {
+    if (a == 0) return (1.0 - t) * (1.0 - 2.0 * t);
+    if (a == 1) return t * (2.0 * t - 1.0);
+    return (4.0 * t) * (1.0 - t);
+}
+
+// N_k(xi) of every node, k ascending
+template <int DIM, int DEG>
+PF2_HD void shape_q(const double* xi, double* N) {
+    if constexpr (DEG == 1) {
+        shape<DIM>(xi, N);
+    } else {
+        double L[DIM][3];
+#pragma unroll
+        for (int d = 0; d < DIM; ++d)
+#pragma unroll
+            for (int a = 0; a < 3; ++a) L[d][a] = lag2(a, xi[d]);
+#pragma unroll
+        for (int k = 0; k < QF<DIM, 2>::ND; ++k) {
+            double val = 1.0;
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) val *= L[d][q2_axis(DIM, k, d)];
+            N[k] = val;
+        }
+    }
+}
+
+// N_k of one node k chosen at run time (a lane's node in the projection
+// sums): per direction the factor coefficients, then the product in
+// shape_q's order, so that the value equals shape_q's N[k]
+template <int DIM, int DEG>
+struct ShapeOne {
+    double c[DIM][4];
+    __host__ __device__ void init(int k) {
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) {
+            if constexpr (DEG == 1) {
+                const bool one = (k >> d) & 1;
+                c[d][0] = one ? 1.0 : -1.0;
+                c[d][1] = one ? 0.0 : 1.0;
+            } else {
+                lag2_coeffs(q2_axis(DIM, k, d), c[d]);
+            }
+        }
+    }
+    __host__ __device__ double factor(int d, double t) const {
+        if constexpr (DEG == 1) return fma(c[d][0], t, c[d][1]);
+        else return fma(c[d][0], t, c[d][1]) * fma(c[d][2], t, c[d][3]);
+    }
+    __host__ __device__ double operator()(const double* xi) const {
+        double v = factor(0, xi[0]);
+#pragma unroll
+        for (int d = 1; d < DIM; ++d) v *= factor(d, xi[d]);
+        return v;
+    }
+};
+
 template <int DIM>
 PF2_HD void shape_grad(const double* xi, double (*G)[DIM]) {
 #pragma unroll

@@ -124,22 +124,23 @@ __device__ __forceinline__ void add_arrival(const W& w, bool pred, int c, int i)
 template <int DIM>
 struct DevMesh {
     int n_cells, n_vertices, n_dofs;
+    int nd;                // velocity DoFs per cell: 2^DIM (Q1) or 3^DIM (Q2)
     const double* X;
     const int* cv;
-    const int* cd;
+    const int* cd;         // per cell: its ND velocity DoFs (ND = QF<DIM, DEG>::ND)
     const int* star_off;
     const int* star_cell;
     const double* star_dir;
     const int* dof_off;
-    const int* dof_cell;   // packed: cell * NV + local vertex
+    const int* dof_cell;   // packed: cell * ND + local DoF
     const double* ftol;    // per cell: tolerance of the clearly-outside test (-1: none)
     const double* cellX;   // per cell: its vertex coordinates packed [NV][DIM] (copies of X)
     const int2* cell_star; // per cell and local vertex: [begin, end) of that vertex's star
     const int4* cell_nbr;  // per cell, the neighbours across faces 0..2*DIM-1 (-1 on the boundary or past
                            // the faces): one int4 in 2-D, two in 3-D
-    const double* cellV;   // per cell: V at its DoFs [NV][DIM] (refreshed when V changes); null for
+    const double* cellV;   // per cell: V at its DoFs [ND][DIM] (refreshed when V changes); null for
                            // FP32-mixed handles, whose particle kernels read FP32 copies
-    const double* cellDV;  // per cell: V - V_old at its DoFs [NV][DIM] (FP64 handles)
+    const double* cellDV;  // per cell: V - V_old at its DoFs [ND][DIM] (FP64 handles)
     const double* Vn;      // V, node-major [n_dofs][DIM]
 };
 
@@ -203,9 +204,10 @@ __device__ __forceinline__ bool state_dead(const int* counters) { return (counte
 // with the velocities gathered from V through the cell's DoF map (the same
 // values).  The choice is a template argument so that the kernels of FP64
 // handles carry no second path.
-template <int DIM, bool GATHER = false>
+template <int DIM, bool GATHER = false, int DEG = 1>
 __device__ __forceinline__ void load_cell(const DevMesh<DIM>& m, int c, double (*P)[DIM], double (*Fc)[DIM]) {
     constexpr int NV = Q1<DIM>::NV;
+    constexpr int ND = QF<DIM, DEG>::ND;
     const double* cx = m.cellX + (size_t)c * NV * DIM;
 #pragma unroll
     for (int k = 0; k < NV; ++k)
@@ -213,25 +215,25 @@ __device__ __forceinline__ void load_cell(const DevMesh<DIM>& m, int c, double (
         for (int d = 0; d < DIM; ++d) P[k][d] = cx[k * DIM + d];
     if constexpr (GATHER) {
 #pragma unroll
-        for (int k = 0; k < NV; ++k) {
-            const double* vf = m.Vn + (size_t)m.cd[(size_t)c * NV + k] * DIM;
+        for (int k = 0; k < ND; ++k) {
+            const double* vf = m.Vn + (size_t)m.cd[(size_t)c * ND + k] * DIM;
 #pragma unroll
             for (int d = 0; d < DIM; ++d) Fc[k][d] = vf[d];
         }
     } else {
-        const double* cf = m.cellV + (size_t)c * NV * DIM;
+        const double* cf = m.cellV + (size_t)c * ND * DIM;
 #pragma unroll
-        for (int k = 0; k < NV; ++k)
+        for (int k = 0; k < ND; ++k)
 #pragma unroll
             for (int d = 0; d < DIM; ++d) Fc[k][d] = cf[k * DIM + d];
     }
 }
 
 // the same, choosing by whether the handle keeps the FP64 copy (seeding)
-template <int DIM>
+template <int DIM, int DEG = 1>
 __device__ __forceinline__ void load_cell_any(const DevMesh<DIM>& m, int c, double (*P)[DIM], double (*Fc)[DIM]) {
-    if (m.cellV) load_cell<DIM, false>(m, c, P, Fc);
-    else load_cell<DIM, true>(m, c, P, Fc);
+    if (m.cellV) load_cell<DIM, false, DEG>(m, c, P, Fc);
+    else load_cell<DIM, true, DEG>(m, c, P, Fc);
 }
 
 // vertex coordinates of cell c (packed per-cell copy: one dependent load level)
@@ -257,15 +259,17 @@ __device__ __forceinline__ void gather_vertices(const DevMesh<DIM>& m, int c, do
     }
 }
 
-// u = sum_k N_k * F_k accumulated from zero (velocity at a reference point)
-template <int DIM>
+// u = sum_k N_k * F_k accumulated from zero over the cell's DoFs in order
+// (velocity at a reference point)
+template <int DIM, int DEG = 1>
 __device__ __forceinline__ void interp(const double (*Fc)[DIM], const double* xi, double* u) {
-    double N[Q1<DIM>::NV];
-    shape<DIM>(xi, N);
+    constexpr int ND = QF<DIM, DEG>::ND;
+    double N[ND];
+    shape_q<DIM, DEG>(xi, N);
 #pragma unroll
     for (int d = 0; d < DIM; ++d) u[d] = 0.0;
 #pragma unroll
-    for (int k = 0; k < Q1<DIM>::NV; ++k)
+    for (int k = 0; k < ND; ++k)
 #pragma unroll
         for (int d = 0; d < DIM; ++d) u[d] += N[k] * Fc[k][d];
 }
@@ -320,14 +324,15 @@ struct NoPrefetch {
 };
 
 // nodal velocities of a candidate cell (FP64 path)
-template <int DIM>
+template <int DIM, int DEG = 1>
 struct PrefetchV64 {
+    static constexpr int ND = QF<DIM, DEG>::ND;
     const double* cellV;
-    double Vc[Q1<DIM>::NV][DIM];
+    double Vc[ND][DIM];
     __device__ void load(int cc) {
-        const double* cf = cellV + (size_t)cc * Q1<DIM>::NV * DIM;
+        const double* cf = cellV + (size_t)cc * ND * DIM;
 #pragma unroll
-        for (int k = 0; k < Q1<DIM>::NV; ++k)
+        for (int k = 0; k < ND; ++k)
 #pragma unroll
             for (int d = 0; d < DIM; ++d) Vc[k][d] = cf[k * DIM + d];
     }
@@ -599,15 +604,23 @@ struct WaveLoop {
     int begin, end, step, shard;
 };
 
+// The workgroups a kernel body loops over: the hardware grid, or, in a
+// batched launch that steps several simulations at once, the simulation's
+// own range of the launch's workgroups (index within it and its size).
+struct Grid {
+    unsigned bid, nblk;
+};
+__device__ __forceinline__ Grid hw_grid() { return Grid{blockIdx.x, gridDim.x}; }
+
 // Both loops work in 32-bit unsigned arithmetic on wavefront-uniform values
 // (grids of at most 2^16 workgroups, n < 2^31), which the compiler keeps in
 // scalar registers: a 64-bit division per wavefront costs more vector
 // instructions than a substep.
 
 // grid-stride: wavefront k takes chunks k, k + W, ... of WAVE items
-__device__ __forceinline__ WaveLoop stride_loop(int n) {
-    const unsigned wpb = blockDim.x / WAVE, nw = gridDim.x * wpb;
-    const unsigned k = blockIdx.x * wpb + (unsigned)__builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
+__device__ __forceinline__ WaveLoop stride_loop(int n, Grid G = hw_grid()) {
+    const unsigned wpb = blockDim.x / WAVE, nw = G.nblk * wpb;
+    const unsigned k = G.bid * wpb + (unsigned)__builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
     WaveLoop L;
     L.begin = (int)(k * WAVE);
     L.end = n;
@@ -618,8 +631,8 @@ __device__ __forceinline__ WaveLoop stride_loop(int n) {
 
 // one contiguous range of whole workgroup-sized chunks per workgroup, the
 // wavefronts of a workgroup interleaved within it
-__device__ __forceinline__ WaveLoop range_loop(int n) {
-    const unsigned g = gridDim.x, b = blockIdx.x, un = (unsigned)n;
+__device__ __forceinline__ WaveLoop range_loop(int n, Grid G = hw_grid()) {
+    const unsigned g = G.nblk, b = G.bid, un = (unsigned)n;
     const unsigned per = ((un + g - 1) / g + blockDim.x - 1) / blockDim.x * blockDim.x;
     const unsigned long long b0l = (unsigned long long)b * per;
     const unsigned b0 = b0l < un ? (unsigned)b0l : un;
@@ -664,7 +677,7 @@ __device__ __forceinline__ void queue_push(const DevQueues& qu, int round, bool 
 // shard counters (blockDim.x == QSHARDS: one thread per shard; the counters
 // are a few cache lines, so the redundant work costs less than a launch).
 // Workgroup 0 also stores it for diagnostics.
-__device__ __forceinline__ int queue_prefix(const DevQueues& qu, int round, int* s_pref) {
+__device__ __forceinline__ int queue_prefix(const DevQueues& qu, int round, int* s_pref, Grid G = hw_grid()) {
     __shared__ int s_wave[QSHARDS / WAVE];
     const int t = threadIdx.x;
     const int lane = t & (WAVE - 1), wv = t / WAVE;
@@ -687,7 +700,7 @@ __device__ __forceinline__ int queue_prefix(const DevQueues& qu, int round, int*
         s_pref[QSHARDS + 1] = tot + *queue_count(qu, round, QSHARDS);
     }
     __syncthreads();
-    if (blockIdx.x == 0) {
+    if (G.bid == 0) {
         int* pref = qu.prefix + (size_t)round * (QSHARDS + 2);
         pref[t] = s_pref[t];
         if (t < 2) pref[QSHARDS + t] = s_pref[QSHARDS + t];
@@ -709,13 +722,13 @@ __device__ __forceinline__ int2 queue_get(const DevQueues& qu, int round, const 
 }
 
 // runs substeps until done (returns 0) or the own-cell test fails (returns 1)
-template <int DIM>
+template <int DIM, int DEG = 1>
 __device__ __forceinline__ int advance64(const DevMesh<DIM>& m, const double* __restrict__ V, const DevParams& pp,
                                          const DevParticles& A, int i, int c, double* x, double* xi, double* vext,
                                          int& s, const double (*P)[DIM], const double (*Vc)[DIM], double tol) {
     for (; s < pp.substeps; ++s) {
         double u[DIM], r[DIM];
-        interp<DIM>(Vc, xi, u);
+        interp<DIM, DEG>(Vc, xi, u);
 #pragma unroll
         for (int d = 0; d < DIM; ++d) vext[d] = u[d];
 #pragma unroll
@@ -771,14 +784,15 @@ __device__ __forceinline__ void park_particle(const DevParticles& A, const DevWo
 
 // k_move: [correction] + substeps until done or the first failed own-cell test.
 //   correct_particles_velocities  TurekBenchmark/pfem2particle.cpp:792-811
-template <int DIM>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DIM == 2 ? 3 : 2))) k_move(DevMesh<DIM> m, const double* __restrict__ V,
-                                              const double* __restrict__ Vold, DevParticles A, DevParams pp,
-                                              DevWork w, DevQueues qu, int do_correct, int do_move) {
+template <int DIM, int DEG = 1>
+__device__ __forceinline__ void move_body(const Grid& G, const DevMesh<DIM>& m, const double* __restrict__ V,
+                                          const double* __restrict__ Vold, const DevParticles& A, const DevParams& pp,
+                                          const DevWork& w, const DevQueues& qu, int do_correct, int do_move) {
     constexpr int NV = Q1<DIM>::NV;
+    constexpr int ND = QF<DIM, DEG>::ND;
     const int n = state_dead(w.counters) ? 0 : w.counters[0];
     const int lane = threadIdx.x & (WAVE - 1);
-    const WaveLoop L = stride_loop(n);
+    const WaveLoop L = stride_loop(n, G);
     for (int base = L.begin; base < L.end; base += L.step) {
         const int i = base + lane;
         const bool valid = i < L.end;
@@ -787,13 +801,13 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DIM ==
 #pragma unroll
         for (int d = 0; d < DIM; ++d) xi[d] = valid ? A.xi[(size_t)d * A.cap + i] : 0.5;
         if (do_correct && valid) {
-            double v[DIM], N[NV];
+            double v[DIM], N[ND];
 #pragma unroll
             for (int d = 0; d < DIM; ++d) v[d] = A.v[(size_t)d * A.cap + i];
-            shape<DIM>(xi, N);
-            const double* dv = m.cellDV + (size_t)c * NV * DIM;
+            shape_q<DIM, DEG>(xi, N);
+            const double* dv = m.cellDV + (size_t)c * ND * DIM;
 #pragma unroll
-            for (int k = 0; k < NV; ++k)
+            for (int k = 0; k < ND; ++k)
 #pragma unroll
                 for (int d = 0; d < DIM; ++d) v[d] = v[d] + N[k] * dv[k * DIM + d];
 #pragma unroll
@@ -804,28 +818,35 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DIM ==
         if (valid) {
 #pragma unroll
             for (int d = 0; d < DIM; ++d) x[d] = A.x[(size_t)d * A.cap + i];
-            double P[NV][DIM], Vc[NV][DIM];
-            load_cell<DIM>(m, c, P, Vc);
-            st = advance64<DIM>(m, V, pp, A, i, c, x, xi, vext, s, P, Vc, m.ftol[c]);
+            double P[NV][DIM], Vc[ND][DIM];
+            load_cell<DIM, false, DEG>(m, c, P, Vc);
+            st = advance64<DIM, DEG>(m, V, pp, A, i, c, x, xi, vext, s, P, Vc, m.ftol[c]);
         }
         park_particle<DIM>(A, w, qu, 0, L.shard, valid && st == 1, i, c, x, xi, vext, s);
         finish_particle<DIM>(pp, A, w, valid && st == 0, i, c, x, xi, true);
     }
 }
 
+template <int DIM, int DEG = 1>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DIM == 2 ? 3 : 2))) k_move(DevMesh<DIM> m, const double* __restrict__ V,
+                                              const double* __restrict__ Vold, DevParticles A, DevParams pp,
+                                              DevWork w, DevQueues qu, int do_correct, int do_move) {
+    move_body<DIM, DEG>(hw_grid(), m, V, Vold, A, pp, w, qu, do_correct, do_move);
+}
+
 // k_search_resume: vertex-star search of the particles queued in round
 // `round`, then the remaining substeps; a particle that needs another search
 // is queued for round + 1, or, in the last round, searched inline.
-template <int DIM>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DIM == 2 ? 3 : 2))) k_search_resume(DevMesh<DIM> m, const double* __restrict__ V,
-                                                       DevParticles A, DevParams pp, DevWork w, DevQueues qu,
-                                                       int round) {
+template <int DIM, int DEG = 1>
+__device__ __forceinline__ void search_body(const Grid& G, const DevMesh<DIM>& m, const double* __restrict__ V,
+                                            const DevParticles& A, const DevParams& pp, const DevWork& w,
+                                            const DevQueues& qu, int round) {
     constexpr int NV = Q1<DIM>::NV;
     __shared__ int s_pref[QSHARDS + 2];
-    const int n = queue_prefix(qu, round, s_pref);
+    const int n = queue_prefix(qu, round, s_pref, G);
     const bool last = round == MOVE_ROUNDS;
     const int lane = threadIdx.x & (WAVE - 1);
-    const WaveLoop L = range_loop(n);
+    const WaveLoop L = range_loop(n, G);
     int lost = 0, searches = 0;
     for (int base = L.begin; base < L.end; base += L.step) {
         const int j = base + lane;
@@ -848,9 +869,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DIM ==
             for (;;) {
                 double Q[NV][DIM], r[DIM];
                 ++searches;
-                PrefetchV64<DIM> pre{m.cellV};
+                PrefetchV64<DIM, DEG> pre{m.cellV};
                 // (3-D: the shortcut's registers push this kernel into heavy spilling)
-                const int cn = relocate<DIM, PrefetchV64<DIM>, DIM == 2>(m, c, P, xi, vext, x, r, Q, pre);
+                const int cn = relocate<DIM, PrefetchV64<DIM, DEG>, DIM == 2>(m, c, P, xi, vext, x, r, Q, pre);
                 if (cn < 0) {
                     c = -1;
                     ++lost;
@@ -865,7 +886,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DIM ==
                 for (int k = 0; k < NV; ++k)
 #pragma unroll
                     for (int d = 0; d < DIM; ++d) P[k][d] = Q[k][d];
-                st = advance64<DIM>(m, V, pp, A, i, c, x, xi, vext, s, P, pre.Vc, m.ftol[c]);
+                st = advance64<DIM, DEG>(m, V, pp, A, i, c, x, xi, vext, s, P, pre.Vc, m.ftol[c]);
                 if (st == 0 || !last) break;
             }
         }
@@ -874,6 +895,13 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DIM ==
     }
     wave_stat(w.stats, 0, lost);
     wave_stat(w.stats, 3, searches);
+}
+
+template <int DIM, int DEG = 1>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DIM == 2 ? 3 : 2))) k_search_resume(DevMesh<DIM> m, const double* __restrict__ V,
+                                                       DevParticles A, DevParams pp, DevWork w, DevQueues qu,
+                                                       int round) {
+    search_body<DIM, DEG>(hw_grid(), m, V, A, pp, w, qu, round);
 }
 
 // candidate j of cell c: stayers of the old segment first, then the
@@ -966,9 +994,7 @@ __device__ __forceinline__ unsigned long long group_or(unsigned long long v) {
 // in LDS when there are at most CAP of them (GCAP or GCAP_L), else listed
 // for k_cell_big.
 template <int DIM, int CAP>
-__global__ void __launch_bounds__(256) k_cell_count(int n_cells, const int* __restrict__ old_off,
-                                                    const uint32_t* __restrict__ pid, DevParams pp, DevWork w,
-                                                    int reseed) {
+__device__ __forceinline__ void cell_count_body(const Grid& G, int n_cells, const int* __restrict__ old_off, const uint32_t* __restrict__ pid, const DevParams& pp, const DevWork& w, int reseed) {
     // Staged rows, compacted to the cell's valid candidates (stayers whose
     // destination is c, then the arrivals): candidate index; part key << 32 |
     // id; id of a survivor, all ones otherwise.  Rows are padded to a
@@ -982,7 +1008,7 @@ __global__ void __launch_bounds__(256) k_cell_count(int n_cells, const int* __re
     const int g = threadIdx.x / GS, gl = threadIdx.x % GS;
     const unsigned long long all = pp.nparts_in >= 64 ? ~0ull : ((1ull << pp.nparts_in) - 1);
     int deleted = 0, arrivals = 0;
-    for (int cb = blockIdx.x * GROUPS; cb < n_cells; cb += gridDim.x * GROUPS) {
+    for (int cb = G.bid * GROUPS; cb < n_cells; cb += G.nblk * GROUPS) {
         const int c = cb + g;
         const bool cv = c < n_cells;
         const int n_st = cv ? old_off[c + 1] - old_off[c] : 0;
@@ -1086,6 +1112,13 @@ __global__ void __launch_bounds__(256) k_cell_count(int n_cells, const int* __re
     wave_stat(w.stats, 5, arrivals);
 }
 
+template <int DIM, int CAP>
+__global__ void __launch_bounds__(256) k_cell_count(int n_cells, const int* __restrict__ old_off,
+                                                    const uint32_t* __restrict__ pid, DevParams pp, DevWork w,
+                                                    int reseed) {
+    cell_count_body<DIM, CAP>(hw_grid(), n_cells, old_off, pid, pp, w, reseed);
+}
+
 // The census of k_cell_count for the cells it listed (more than CAP
 // candidates): one 256-thread block per cell, candidates staged in LDS.
 // A cell with more than BIGCAP candidates sets overflow flag 2 and keeps no
@@ -1093,8 +1126,7 @@ __global__ void __launch_bounds__(256) k_cell_count(int n_cells, const int* __re
 constexpr int BIGCAP = 2048;
 
 template <int DIM>
-__global__ void __launch_bounds__(256) k_cell_big(const int* __restrict__ old_off, const uint32_t* __restrict__ pid,
-                                                  DevParams pp, DevWork w, int reseed) {
+__device__ __forceinline__ void cell_big_body(const Grid& G, const int* __restrict__ old_off, const uint32_t* __restrict__ pid, const DevParams& pp, const DevWork& w, int reseed) {
     __shared__ int s_idx[BIGCAP];
     __shared__ __align__(16) unsigned long long s_kid[BIGCAP];
     __shared__ __align__(16) uint32_t s_sid[BIGCAP];
@@ -1105,7 +1137,7 @@ __global__ void __launch_bounds__(256) k_cell_big(const int* __restrict__ old_of
     const unsigned long long all = pp.nparts_in >= 64 ? ~0ull : ((1ull << pp.nparts_in) - 1);
     const int nbig = w.big[0];
     int deleted = 0;
-    for (int b = blockIdx.x; b < nbig; b += gridDim.x) {
+    for (int b = G.bid; b < nbig; b += G.nblk) {
         const int c = w.big[1 + b];
         const int n_st = old_off[c + 1] - old_off[c];
         const int na = w.arr_cnt[c];
@@ -1205,6 +1237,12 @@ __global__ void __launch_bounds__(256) k_cell_big(const int* __restrict__ old_of
     if (tid == 0 && deleted) stat_add(w.stats, 1, (unsigned long long)deleted);
 }
 
+template <int DIM>
+__global__ void __launch_bounds__(256) k_cell_big(const int* __restrict__ old_off, const uint32_t* __restrict__ pid,
+                                                  DevParams pp, DevWork w, int reseed) {
+    cell_big_body<DIM>(hw_grid(), old_off, pid, pp, w, reseed);
+}
+
 // ---------------------------------------------------------------------------
 // k_cell_write: survivors and reseeded particles into buffer B.
 // New particles: reference point at the part centre, location mapped,
@@ -1214,21 +1252,19 @@ __global__ void __launch_bounds__(256) k_cell_big(const int* __restrict__ old_of
 // expressions of k_cell_partials (accumulator a: vertex a / (DIM + 1),
 // component a % (DIM + 1), the last one the weight); NA accumulators a0,
 // a0 + GS, ... in one pass (their chains are independent, so they overlap)
-template <int DIM, int NA>
+template <int DIM, int NA, int DEG = 1>
 __device__ __forceinline__ void cell_sums64(const double* __restrict__ sxi, const double* __restrict__ sv, int n,
                                             int a0, double* out) {
-    double ft[NA][DIM], fs[NA][DIM], acc[NA];
+    constexpr int NACC = QF<DIM, DEG>::ND * (DIM + 1);
+    ShapeOne<DIM, DEG> sh[NA];
+    double acc[NA];
     int comp[NA];
 #pragma unroll
     for (int u = 0; u < NA; ++u) {
         const int a = a0 + u * GS;
-        const int kk = a / (DIM + 1);
+        const int kk = (a < NACC ? a : 0) / (DIM + 1);
         comp[u] = a - kk * (DIM + 1);
-#pragma unroll
-        for (int d = 0; d < DIM; ++d) {
-            ft[u][d] = ((kk >> d) & 1) ? 1.0 : -1.0;
-            fs[u][d] = ((kk >> d) & 1) ? 0.0 : 1.0;
-        }
+        sh[u].init(kk);
         acc[u] = 0.0;
     }
     for (int q = 0; q < n; ++q) {
@@ -1240,9 +1276,7 @@ __device__ __forceinline__ void cell_sums64(const double* __restrict__ sxi, cons
         }
 #pragma unroll
         for (int u = 0; u < NA; ++u) {
-            double nk = fma(ft[u][0], xi[0], fs[u][0]);
-#pragma unroll
-            for (int d = 1; d < DIM; ++d) nk *= fma(ft[u][d], xi[d], fs[u][d]);
+            const double nk = sh[u](xi);
             if (comp[u] < DIM) {
                 double vc = v[0];
 #pragma unroll
@@ -1258,13 +1292,14 @@ __device__ __forceinline__ void cell_sums64(const double* __restrict__ sxi, cons
     for (int u = 0; u < NA; ++u) out[u] = acc[u];
 }
 
-template <int DIM, bool GATHER, bool FUSE>
+template <int DIM, bool GATHER, bool FUSE, int DEG = 1>
 __device__ __forceinline__ void cell_write_one(const DevMesh<DIM>& m, const double* __restrict__ V,
                                                const int* __restrict__ old_off, const DevParticles& A,
                                                const DevParticles& B, const DevParams& pp, const DevWork& w, int c,
                                                int gl, double* __restrict__ sxi, double* __restrict__ sv,
                                                double* __restrict__ partial) {
     constexpr int NV = Q1<DIM>::NV;
+    constexpr int ND = QF<DIM, DEG>::ND;
     const int n_st = old_off[c + 1] - old_off[c];
     const int na = w.arr_cnt[c];
     const int kin = n_st + (na < w.aslots ? na : w.aslots);
@@ -1305,11 +1340,11 @@ __device__ __forceinline__ void cell_write_one(const DevMesh<DIM>& m, const doub
         for (int o = gl; o < pp.nparts_in; o += GS) {
             if (!((empty >> o) & 1)) continue;
             const int r = __popcll(empty & ((1ull << o) - 1));
-            double xi[DIM], P[NV][DIM], Vc[NV][DIM], x[DIM], v[DIM];
+            double xi[DIM], P[NV][DIM], Vc[ND][DIM], x[DIM], v[DIM];
             lattice_point<DIM>(pp, o, xi);
-            load_cell<DIM, GATHER>(m, c, P, Vc);
+            load_cell<DIM, GATHER, DEG>(m, c, P, Vc);
             forward<DIM>(P, xi, x);
-            interp<DIM>(Vc, xi, v);
+            interp<DIM, DEG>(Vc, xi, v);
             const long long q = out0 + nsurv + r;
             B.cell[q] = c;
             B.id[q] = id0 + (uint32_t)r + 1u;
@@ -1333,14 +1368,16 @@ __device__ __forceinline__ void cell_write_one(const DevMesh<DIM>& m, const doub
         w.ovf_head[c] = -1;
     }
     if constexpr (FUSE) {
-        constexpr int NACC = NV * (DIM + 1);
+        constexpr int NACC = ND * (DIM + 1);
         wave_sync();
-        constexpr int NA = (NACC + GS - 1) / GS;   // sums per lane: 1 in 2-D (12 lanes), 2 in 3-D
+        // sums per lane: Q1 1 in 2-D (12 lanes), 2 in 3-D; Q2 2 in 2-D, 7 in 3-D
+        constexpr int NA = (NACC + GS - 1) / GS;
         if (gl < NACC) {
             double out[NA];
-            cell_sums64<DIM, NA>(sxi, sv, nfinal, gl, out);
+            cell_sums64<DIM, NA, DEG>(sxi, sv, nfinal, gl, out);
 #pragma unroll
-            for (int u = 0; u < NA; ++u) partial[(size_t)c * NACC + gl + u * GS] = out[u];
+            for (int u = 0; u < NA; ++u)
+                if (gl + u * GS < NACC) partial[(size_t)c * NACC + gl + u * GS] = out[u];
         }
         wave_sync();
     }
@@ -1349,18 +1386,23 @@ __device__ __forceinline__ void cell_write_one(const DevMesh<DIM>& m, const doub
 // GATHER: an FP32-mixed handle's FP64 regroup (no FP64 per-cell velocity copy).
 // FUSE (final count per cell <= FCAP): the group stages the cell's final
 // particles in LDS and writes its projection sums, as k_cell_partials does.
-template <int DIM, bool GATHER, bool FUSE>
-__global__ void __launch_bounds__(256) k_cell_write(DevMesh<DIM> m, const double* __restrict__ V,
-                                                    const int* __restrict__ old_off, DevParticles A,
-                                                    DevParticles B, DevParams pp, DevWork w,
-                                                    double* __restrict__ partial) {
+template <int DIM, bool GATHER, bool FUSE, int DEG = 1>
+__device__ __forceinline__ void cell_write_body(const Grid& G, const DevMesh<DIM>& m, const double* __restrict__ V, const int* __restrict__ old_off, const DevParticles& A, const DevParticles& B, const DevParams& pp, const DevWork& w, double* __restrict__ partial) {
     __shared__ double s_xi[FUSE ? GROUPS : 1][FUSE ? FCAP * DIM : 1];
     __shared__ double s_v[FUSE ? GROUPS : 1][FUSE ? FCAP * DIM : 1];
     if (state_dead(w.counters)) return;
     const int g = threadIdx.x / GS, gl = threadIdx.x % GS;
-    for (int c = blockIdx.x * GROUPS + g; c < m.n_cells; c += gridDim.x * GROUPS)
-        cell_write_one<DIM, GATHER, FUSE>(m, V, old_off, A, B, pp, w, c, gl, s_xi[FUSE ? g : 0], s_v[FUSE ? g : 0],
-                                          partial);
+    for (int c = G.bid * GROUPS + g; c < m.n_cells; c += G.nblk * GROUPS)
+        cell_write_one<DIM, GATHER, FUSE, DEG>(m, V, old_off, A, B, pp, w, c, gl, s_xi[FUSE ? g : 0],
+                                               s_v[FUSE ? g : 0], partial);
+}
+
+template <int DIM, bool GATHER, bool FUSE, int DEG = 1>
+__global__ void __launch_bounds__(256) k_cell_write(DevMesh<DIM> m, const double* __restrict__ V,
+                                                    const int* __restrict__ old_off, DevParticles A,
+                                                    DevParticles B, DevParams pp, DevWork w,
+                                                    double* __restrict__ partial) {
+    cell_write_body<DIM, GATHER, FUSE, DEG>(hw_grid(), m, V, old_off, A, B, pp, w, partial);
 }
 
 // totals after the scans: particle count, id counter, statistics; resets the
@@ -1407,40 +1449,46 @@ __global__ void __launch_bounds__(QSHARDS) k_finalize(int n_cells, DevWork w, in
 // sum_p N_k(xi_p) v_p and sum_p N_k(xi_p), particles in ascending id order.
 // A group of PGS lanes per cell stages PGS particles at a time in LDS; lane
 // a = k*(DIM+1) + comp of the group then adds its terms in order.
-template <int DIM>
+// Q1: one lane per accumulator (12 or 32 lanes per cell); Q2: 27 (2-D, one
+// lane each) or 108 accumulators (3-D: a group of 64 lanes, two each)
+template <int DIM, int DEG = 1>
 struct PartialsShape {
-    static constexpr int NACC = Q1<DIM>::NV * (DIM + 1);
-    static constexpr int PGS = NACC <= 16 ? 16 : 32;
+    static constexpr int NACC = QF<DIM, DEG>::ND * (DIM + 1);
+    static constexpr int PGS = NACC <= 16 ? 16 : (NACC <= 32 ? 32 : 64);
+    static constexpr int NA = (NACC + PGS - 1) / PGS;   // accumulators per lane
     static constexpr int NG = 256 / PGS;
 };
 
-template <int DIM>
-__global__ void __launch_bounds__(256) k_cell_partials(int n_cells, const int* __restrict__ off, DevParticles A,
-                                                       double* __restrict__ partial, const int* __restrict__ counters) {
+template <int DIM, int DEG = 1>
+__device__ __forceinline__ void partials_body(const Grid& G, int n_cells, const int* __restrict__ off, const DevParticles& A, double* __restrict__ partial, const int* __restrict__ counters) {
     if (state_dead(counters)) return;
-    constexpr int NV = Q1<DIM>::NV;
-    constexpr int NACC = PartialsShape<DIM>::NACC;
-    constexpr int PGS = PartialsShape<DIM>::PGS;
-    constexpr int NG = PartialsShape<DIM>::NG;
+    constexpr int NACC = PartialsShape<DIM, DEG>::NACC;
+    constexpr int PGS = PartialsShape<DIM, DEG>::PGS;
+    constexpr int NG = PartialsShape<DIM, DEG>::NG;
+    constexpr int NA = PartialsShape<DIM, DEG>::NA;
     __shared__ double s_xi[NG][PGS][DIM];
     __shared__ double s_v[NG][PGS][DIM];
     const int g = threadIdx.x / PGS, gl = threadIdx.x % PGS;
-    const int kk = gl / (DIM + 1);
-    const int comp = gl - kk * (DIM + 1);
-    double ft[DIM], fs[DIM];
+    // lane gl holds accumulators gl, gl + PGS, ... (a = k * (DIM + 1) + comp)
+    ShapeOne<DIM, DEG> sh[NA];
+    int comp[NA];
 #pragma unroll
-    for (int d = 0; d < DIM; ++d) {
-        ft[d] = ((kk >> d) & 1) ? 1.0 : -1.0;
-        fs[d] = ((kk >> d) & 1) ? 0.0 : 1.0;
+    for (int u = 0; u < NA; ++u) {
+        const int a = gl + u * PGS;
+        const int kk = (a < NACC ? a : 0) / (DIM + 1);
+        comp[u] = a - kk * (DIM + 1);
+        sh[u].init(kk);
     }
-    for (int cb = blockIdx.x * NG; cb < n_cells; cb += gridDim.x * NG) {
+    for (int cb = G.bid * NG; cb < n_cells; cb += G.nblk * NG) {
         const int c = cb + g;
         const bool cv = c < n_cells;
         const int b = cv ? off[c] : 0, e = cv ? off[c + 1] : 0;
         int trips = (e - b + PGS - 1) / PGS;
 #pragma unroll
         for (int o = 1; o < WAVE; o <<= 1) trips = max(trips, __shfl_xor(trips, o));
-        double acc = 0.0;
+        double acc[NA];
+#pragma unroll
+        for (int u = 0; u < NA; ++u) acc[u] = 0.0;
         for (int t = 0; t < trips; ++t) {
             const int j0 = b + t * PGS;
             const int j = j0 + gl;
@@ -1455,48 +1503,60 @@ __global__ void __launch_bounds__(256) k_cell_partials(int n_cells, const int* _
             const int cnt = (e - j0) < PGS ? (e - j0) : PGS;
             if (gl < NACC)
                 for (int q = 0; q < cnt; ++q) {
-                    // N_k of this lane's vertex alone: factor d is ft_d xi_d +
-                    // fs_d (xi_d, or 1 - xi_d with one rounding), multiplied
-                    // in shape()'s order, so the value is shape()'s N[k] (up
-                    // to the sign of a zero factor, which no sum here sees:
-                    // acc starts at +0)
-                    double nk = fma(ft[0], s_xi[g][q][0], fs[0]);
+                    // N_k of this lane's node alone: per direction the
+                    // factor of the node's axis, multiplied in shape_q's
+                    // order, so the value is shape_q's N[k] (up to the sign
+                    // of a zero factor, which no sum here sees: acc starts
+                    // at +0)
+                    double xi[DIM];
 #pragma unroll
-                    for (int d = 1; d < DIM; ++d) nk *= fma(ft[d], s_xi[g][q][d], fs[d]);
-                    if (comp < DIM) {
-                        double vc = s_v[g][q][0];
+                    for (int d = 0; d < DIM; ++d) xi[d] = s_xi[g][q][d];
 #pragma unroll
-                        for (int d = 1; d < DIM; ++d)
-                            if (d == comp) vc = s_v[g][q][d];
-                        acc += nk * vc;
-                    } else {
-                        acc += nk;
+                    for (int u = 0; u < NA; ++u) {
+                        const double nk = sh[u](xi);
+                        if (comp[u] < DIM) {
+                            double vc = s_v[g][q][0];
+#pragma unroll
+                            for (int d = 1; d < DIM; ++d)
+                                if (d == comp[u]) vc = s_v[g][q][d];
+                            acc[u] += nk * vc;
+                        } else {
+                            acc[u] += nk;
+                        }
                     }
                 }
             wave_sync();
         }
-        if (cv && gl < NACC) partial[(size_t)c * NACC + gl] = acc;
+        if (cv)
+#pragma unroll
+            for (int u = 0; u < NA; ++u)
+                if (gl + u * PGS < NACC) partial[(size_t)c * NACC + gl + u * PGS] = acc[u];
     }
+}
+
+template <int DIM, int DEG = 1>
+__global__ void __launch_bounds__(256) k_cell_partials(int n_cells, const int* __restrict__ off, DevParticles A,
+                                                       double* __restrict__ partial, const int* __restrict__ counters) {
+    partials_body<DIM, DEG>(hw_grid(), n_cells, off, A, partial, counters);
 }
 
 // k_node: node sums over the cells that contain the DoF (ascending cell
 // order, TurekBenchmark/pfem2particle.cpp:895-909) divided by the weight.
 // Writes V and V_old (both hold the projected field afterwards).
-template <int DIM, bool FIN>
-__global__ void __launch_bounds__(256) k_node(DevMesh<DIM> m, const double* __restrict__ partial,
-                                              double* __restrict__ V, double* __restrict__ Vold,
-                                              unsigned long long* stats, DevWork w, int* queue_count) {
-    constexpr int NV = Q1<DIM>::NV;
-    constexpr int NACC = NV * (DIM + 1);
+template <int DIM, bool FIN, int DEG = 1>
+__device__ __forceinline__ void node_body(const Grid& G, const DevMesh<DIM>& m, const double* __restrict__ partial, double* __restrict__ V, double* __restrict__ Vold, unsigned long long* stats, const DevWork& w, int* queue_count) {
+    constexpr int NV = Q1<DIM>::NV;   // cells per chunk
+    constexpr int ND = QF<DIM, DEG>::ND;
+    constexpr int NACC = ND * (DIM + 1);
     // FIN (a step whose regroup left its bookkeeping for later): the last
     // workgroup runs it; it depends on the cell write only, as the node sums do
-    if (FIN && blockIdx.x == gridDim.x - 1) {
+    if (FIN && G.bid == G.nblk - 1) {
         finalize_block(m.n_cells, w, queue_count);
         return;
     }
     if (state_dead(w.counters)) return;
-    const int nb = gridDim.x - (FIN ? 1 : 0);
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < m.n_dofs; i += nb * blockDim.x) {
+    const int nb = G.nblk - (FIN ? 1 : 0);
+    for (int i = G.bid * blockDim.x + threadIdx.x; i < m.n_dofs; i += nb * blockDim.x) {
         double acc[DIM], wsum = 0.0;
 #pragma unroll
         for (int d = 0; d < DIM; ++d) acc[d] = 0.0;
@@ -1511,7 +1571,7 @@ __global__ void __launch_bounds__(256) k_node(DevMesh<DIM> m, const double* __re
             double pv[NV][DIM + 1];
 #pragma unroll
             for (int q = 0; q < NV; ++q) {
-                const int c = ck[q] >= 0 ? ck[q] / NV : 0, k = ck[q] >= 0 ? ck[q] - c * NV : 0;
+                const int c = ck[q] >= 0 ? ck[q] / ND : 0, k = ck[q] >= 0 ? ck[q] - c * ND : 0;
                 const double* p = partial + (size_t)c * NACC + k * (DIM + 1);
 #pragma unroll
                 for (int d = 0; d <= DIM; ++d) pv[q][d] = ck[q] >= 0 ? p[d] : 0.0;
@@ -1532,6 +1592,13 @@ __global__ void __launch_bounds__(256) k_node(DevMesh<DIM> m, const double* __re
             Vold[(size_t)i * DIM + d] = val;
         }
     }
+}
+
+template <int DIM, bool FIN, int DEG = 1>
+__global__ void __launch_bounds__(256) k_node(DevMesh<DIM> m, const double* __restrict__ partial,
+                                              double* __restrict__ V, double* __restrict__ Vold,
+                                              unsigned long long* stats, DevWork w, int* queue_count) {
+    node_body<DIM, FIN, DEG>(hw_grid(), m, partial, V, Vold, stats, w, queue_count);
 }
 
 // ---------------------------------------------------------------------------
@@ -1632,20 +1699,21 @@ __global__ void k_wall_set(DevWalls wl, double* __restrict__ V, double* __restri
 // ---------------------------------------------------------------------------
 // seeding: q0*q1[*q2] particles per cell in lattice order, ids in cell order
 // (cylinder2D/pfem2particle.cpp:487-508, 563-580)
-template <int DIM>
+template <int DIM, int DEG = 1>
 __global__ void k_seed(DevMesh<DIM> m, const double* __restrict__ V, DevParticles A, DevParams pp,
                        uint32_t id_base, int* off) {
     constexpr int NV = Q1<DIM>::NV;
+    constexpr int ND = QF<DIM, DEG>::ND;
     const long long total = (long long)m.n_cells * pp.nparts_in;
     for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < total;
          i += (long long)gridDim.x * blockDim.x) {
         const int c = (int)(i / pp.nparts_in);
         const int o = (int)(i - (long long)c * pp.nparts_in);
-        double xi[DIM], P[NV][DIM], Vc[NV][DIM], x[DIM], v[DIM];
+        double xi[DIM], P[NV][DIM], Vc[ND][DIM], x[DIM], v[DIM];
         lattice_point<DIM>(pp, o, xi);
-        load_cell_any<DIM>(m, c, P, Vc);
+        load_cell_any<DIM, DEG>(m, c, P, Vc);
         forward<DIM>(P, xi, x);
-        interp<DIM>(Vc, xi, v);
+        interp<DIM, DEG>(Vc, xi, v);
         A.cell[i] = c;
         A.id[i] = id_base + (uint32_t)i + 1u;
 #pragma unroll
@@ -1662,21 +1730,22 @@ __global__ void k_seed(DevMesh<DIM> m, const double* __restrict__ V, DevParticle
 
 // location from (cell, reference point) and, optionally, velocity from V, for
 // particles placed by pf2_insert_particles
-template <int DIM>
+template <int DIM, int DEG = 1>
 __global__ void k_init_particles(DevMesh<DIM> m, const double* __restrict__ V, DevParticles A,
                                  const unsigned char* __restrict__ need_v, long long n, int keep_vext) {
     constexpr int NV = Q1<DIM>::NV;
+    constexpr int ND = QF<DIM, DEG>::ND;
     for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
         const int c = A.cell[i];
-        double xi[DIM], P[NV][DIM], Vc[NV][DIM], x[DIM], v[DIM];
+        double xi[DIM], P[NV][DIM], Vc[ND][DIM], x[DIM], v[DIM];
 #pragma unroll
         for (int d = 0; d < DIM; ++d) xi[d] = A.xi[(size_t)d * A.cap + i];
-        load_cell_any<DIM>(m, c, P, Vc);
+        load_cell_any<DIM, DEG>(m, c, P, Vc);
         forward<DIM>(P, xi, x);
 #pragma unroll
         for (int d = 0; d < DIM; ++d) A.x[(size_t)d * A.cap + i] = x[d];
         if (need_v[i]) {
-            interp<DIM>(Vc, xi, v);
+            interp<DIM, DEG>(Vc, xi, v);
 #pragma unroll
             for (int d = 0; d < DIM; ++d) A.v[(size_t)d * A.cap + i] = v[d];
         }
@@ -1759,13 +1828,10 @@ __global__ void k_displace(const double* __restrict__ X0, const double* __restri
 // the per-cell copies of V and V - V_old (FP32 copies when V32 is given; the
 // FP64 V - V_old, read only by the FP64 correction, when cellDV is given)
 template <int DIM>
-__global__ void k_refresh(DevMesh<DIM> m, const double* __restrict__ Vcm, double* __restrict__ V,
-                          const double* __restrict__ Vold, double* __restrict__ cellV, double* __restrict__ cellDV,
-                          float* __restrict__ V32, float* __restrict__ DV32) {
-    constexpr int NV = Q1<DIM>::NV;
-    const long long nck = (long long)m.n_cells * NV, nd = m.n_dofs;
+__device__ __forceinline__ void refresh_body(const Grid& G, const DevMesh<DIM>& m, const double* __restrict__ Vcm, double* __restrict__ V, const double* __restrict__ Vold, double* __restrict__ cellV, double* __restrict__ cellDV, float* __restrict__ V32, float* __restrict__ DV32) {
+    const long long nck = (long long)m.n_cells * m.nd, nd = m.n_dofs;
     const long long n = nck > nd ? nck : nd;
-    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x) {
+    for (long long t = G.bid * (long long)blockDim.x + threadIdx.x; t < n; t += (long long)G.nblk * blockDim.x) {
         if (t < nd)
 #pragma unroll
             for (int d = 0; d < DIM; ++d) V[(size_t)t * DIM + d] = Vcm[(size_t)d * nd + t];
@@ -1786,10 +1852,16 @@ __global__ void k_refresh(DevMesh<DIM> m, const double* __restrict__ Vcm, double
 }
 
 template <int DIM>
+__global__ void k_refresh(DevMesh<DIM> m, const double* __restrict__ Vcm, double* __restrict__ V,
+                          const double* __restrict__ Vold, double* __restrict__ cellV, double* __restrict__ cellDV,
+                          float* __restrict__ V32, float* __restrict__ DV32) {
+    refresh_body<DIM>(hw_grid(), m, Vcm, V, Vold, cellV, cellDV, V32, DV32);
+}
+
+template <int DIM>
 __global__ void k_pack_fields(DevMesh<DIM> m, const double* __restrict__ V, const double* __restrict__ Vold,
                               double* __restrict__ cellV, double* __restrict__ cellDV) {
-    constexpr int NV = Q1<DIM>::NV;
-    const long long n = (long long)m.n_cells * NV;
+    const long long n = (long long)m.n_cells * m.nd;
     for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x) {
         const int dof = m.cd[t];
 #pragma unroll

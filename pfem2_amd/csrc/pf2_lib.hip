@@ -18,6 +18,8 @@
 #include <vector>
 
 #include "../../include/pfem2_amd.h"
+#include "pf2_batch.hpp"
+#include "pf2_fem.hpp"
 #include "pf2_kernels.hpp"
 #include "pf2_mixed.hpp"
 
@@ -100,6 +102,7 @@ pf2::Rec32 rec32(const PBufs& b) { return pf2::Rec32{b.s32a, b.s32b}; }
 
 struct pf2_handle_s {
     int dim = 2, nv = 4;
+    int deg = 1, nd = 4;   // velocity field degree and its DoFs per cell
     int n_cells = 0, n_vertices = 0, n_dofs = 0;
     pf2_params params{};
     long long cap = 0;
@@ -116,6 +119,8 @@ struct pf2_handle_s {
     int *ddof_off = nullptr, *ddof_cell = nullptr;
     double* dftol = nullptr;
     double* dcellX = nullptr;
+    unsigned char* dvbnd = nullptr;   // per vertex: on a boundary face
+    int* dvdof = nullptr;             // per vertex: its velocity DoF (-1: in no cell)
     int2* dcell_star = nullptr;
     std::vector<int> cell_nbr;   // face neighbours, 4 (2-D) or 8 (3-D, 6 used) per cell
     int4* dcell_nbr = nullptr;
@@ -152,6 +157,7 @@ struct pf2_handle_s {
     unsigned long long* stats = nullptr;
     void* scan_tmp = nullptr;
     size_t scan_tmp_bytes = 0;
+    unsigned long long* scan_tiles = nullptr;   // tile sums of the batched offset scan
     // walls
     int n_walls = 0, n_wall_cells = 0;
     int *w_dof = nullptr, *w_mask = nullptr, *w_fix = nullptr, *w_vwall = nullptr, *w_cells = nullptr;
@@ -264,6 +270,7 @@ DevMesh<DIM> dev_mesh(pf2_handle_s* h) {
     m.n_cells = h->n_cells;
     m.n_vertices = h->n_vertices;
     m.n_dofs = h->n_dofs;
+    m.nd = h->nd;
     m.X = h->dX;
     m.cv = h->dcv;
     m.cd = h->dcd;
@@ -501,8 +508,16 @@ int regroup(pf2_handle_s* h, int reseed, bool fuse_sums = false, bool defer_fina
             hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cell_write<DIM, true, false>), dim3(cell_grid(nc)), dim3(256), 0,
                                h->stream, dev_mesh<DIM>(h), h->dV, old_off, dev_particles(h, h->cur),
                                dev_particles(h, 1 - h->cur), pp, w, h->partial);
+        else if (fuse_sums && h->deg == 2)
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cell_write<DIM, false, true, 2>), dim3(cell_grid(nc)), dim3(256), 0,
+                               h->stream, dev_mesh<DIM>(h), h->dV, old_off, dev_particles(h, h->cur),
+                               dev_particles(h, 1 - h->cur), pp, w, h->partial);
         else if (fuse_sums)
             hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cell_write<DIM, false, true>), dim3(cell_grid(nc)), dim3(256), 0,
+                               h->stream, dev_mesh<DIM>(h), h->dV, old_off, dev_particles(h, h->cur),
+                               dev_particles(h, 1 - h->cur), pp, w, h->partial);
+        else if (h->deg == 2)
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cell_write<DIM, false, false, 2>), dim3(cell_grid(nc)), dim3(256), 0,
                                h->stream, dev_mesh<DIM>(h), h->dV, old_off, dev_particles(h, h->cur),
                                dev_particles(h, 1 - h->cur), pp, w, h->partial);
         else
@@ -547,15 +562,26 @@ int run_move(pf2_handle_s* h, int do_correct, int do_move, bool fuse_sums = fals
     } else {
         {
             Timed t(h, K_MOVE);
-            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_move<DIM>), dim3(grid), dim3(256), 0, h->stream, dev_mesh<DIM>(h),
-                               h->dV, h->dVold, dev_particles(h, h->cur), pp, w, qu, do_correct, do_move);
+            if (h->deg == 2)
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_move<DIM, 2>), dim3(grid), dim3(256), 0, h->stream,
+                                   dev_mesh<DIM>(h), h->dV, h->dVold, dev_particles(h, h->cur), pp, w, qu, do_correct,
+                                   do_move);
+            else
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_move<DIM>), dim3(grid), dim3(256), 0, h->stream,
+                                   dev_mesh<DIM>(h), h->dV, h->dVold, dev_particles(h, h->cur), pp, w, qu, do_correct,
+                                   do_move);
         }
         if (do_move)
             for (int r = 0; r <= MOVE_ROUNDS; ++r) {
                 Timed t(h, r == 0 ? K_SEARCH : (r == 1 ? K_SEARCH_R1 : K_SEARCH_R2));
-                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_search_resume<DIM>), dim3(round_grid(qgrid, r)),
-                                   dim3(256), 0, h->stream, dev_mesh<DIM>(h), h->dV, dev_particles(h, h->cur), pp, w,
-                                   qu, r);
+                if (h->deg == 2)
+                    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_search_resume<DIM, 2>), dim3(round_grid(qgrid, r)),
+                                       dim3(256), 0, h->stream, dev_mesh<DIM>(h), h->dV, dev_particles(h, h->cur), pp,
+                                       w, qu, r);
+                else
+                    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_search_resume<DIM>), dim3(round_grid(qgrid, r)),
+                                       dim3(256), 0, h->stream, dev_mesh<DIM>(h), h->dV, dev_particles(h, h->cur), pp,
+                                       w, qu, r);
             }
     }
     HIPC(hipGetLastError());
@@ -582,8 +608,12 @@ int run_project(pf2_handle_s* h, bool sums_done = false, double* partial = nullp
                                h->counters);
     } else {
         Timed t(h, K_PARTIALS);
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cell_partials<DIM>), dim3(cell_grid(nc)), dim3(256), 0, h->stream, nc,
-                           h->off[h->cur_off], dev_particles(h, h->cur), partial, h->counters);
+        if (h->deg == 2)
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cell_partials<DIM, 2>), dim3(cell_grid(nc)), dim3(256), 0,
+                               h->stream, nc, h->off[h->cur_off], dev_particles(h, h->cur), partial, h->counters);
+        else
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cell_partials<DIM>), dim3(cell_grid(nc)), dim3(256), 0, h->stream,
+                               nc, h->off[h->cur_off], dev_particles(h, h->cur), partial, h->counters);
     }
     {
         Timed t(h, K_NODE);
@@ -593,9 +623,18 @@ int run_project(pf2_handle_s* h, bool sums_done = false, double* partial = nullp
             // now the current ones
             DevWork wf = w;
             wf.new_off = h->off[h->cur_off];
-            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_node<DIM, true>), dim3(grid_for(h->n_dofs, 256) + 1), dim3(256), 0,
-                               h->stream, dev_mesh<DIM>(h), partial, h->dV, h->dVold, h->stats, wf, h->qcount);
-        } else
+            if (h->deg == 2)
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_node<DIM, true, 2>), dim3(grid_for(h->n_dofs, 256) + 1),
+                                   dim3(256), 0, h->stream, dev_mesh<DIM>(h), partial, h->dV, h->dVold, h->stats, wf,
+                                   h->qcount);
+            else
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_node<DIM, true>), dim3(grid_for(h->n_dofs, 256) + 1),
+                                   dim3(256), 0, h->stream, dev_mesh<DIM>(h), partial, h->dV, h->dVold, h->stats, wf,
+                                   h->qcount);
+        } else if (h->deg == 2)
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_node<DIM, false, 2>), dim3(grid_for(h->n_dofs, 256)), dim3(256), 0,
+                               h->stream, dev_mesh<DIM>(h), partial, h->dV, h->dVold, h->stats, w, h->qcount);
+        else
             hipLaunchKernelGGL(HIP_KERNEL_NAME(k_node<DIM, false>), dim3(grid_for(h->n_dofs, 256)), dim3(256), 0,
                                h->stream, dev_mesh<DIM>(h), partial, h->dV, h->dVold, h->stats, w, h->qcount);
     }
@@ -664,7 +703,7 @@ int upload_field(pf2_handle_s* h, const double* src, double* dst, hipMemcpyKind 
 // handles: FP32 copies only); called whenever V or V_old change
 int pack_fields(pf2_handle_s* h) {
     h->packed = true;
-    const long long n = (long long)h->n_cells * h->nv;
+    const long long n = (long long)h->n_cells * h->nd;
     const int g = grid_for(n, 256, 4096);
     if (h->dim == 2) {
         if (h->dcellV)
@@ -694,7 +733,7 @@ int ensure_packed(pf2_handle_s* h) {
 // V from a device field (component-major) into the node-major copy and the
 // per-cell copies, in one pass
 int refresh_field(pf2_handle_s* h, const double* field) {
-    const long long n = std::max((long long)h->n_cells * h->nv, (long long)h->n_dofs);
+    const long long n = std::max((long long)h->n_cells * h->nd, (long long)h->n_dofs);
     const int g = grid_for(n, 256, 4096);
     Timed t(h, K_MISC);
     if (h->dim == 2)
@@ -736,18 +775,20 @@ int from_mixed(pf2_handle_s* h) {
 // Device bytes pf2_create requests for a mesh of these sizes (the same list
 // of allocations as pf2_create and alloc_particles), plus 1 MiB for the scan
 // state whose size the scan library fixes at run time.
-size_t estimate_bytes(int dim, long long nc, long long nvt, long long nd, long long cap, const pf2_params* p) {
+size_t estimate_bytes(int dim, long long nc, long long nvt, long long nd, long long cap, const pf2_params* p,
+                      int deg) {
     const long long nv = 1LL << dim, nck = nc * nv;
+    const long long ndc = nc * (deg == 2 ? (dim == 2 ? 9 : 27) : nv);   // velocity DoFs of all cells
     size_t b = 0;
-    b += nvt * dim * 8 + nck * 4 * 2 + (nvt + 1) * 4 + nck * 4 + nck * dim * 8;       // X, cv, cd, star
-    b += nc * 16 * (dim - 1);                                                            // face neighbours
-    b += (nd + 1) * 4 + nck * 4 + nc * 8 + nck * dim * 8 + nck * 8;                    // dof map, ftol, cellX/star
-    if (p->precision != PF2_FP32_MIXED) b += nck * dim * 8 * 2;                          // FP64 cell V / dV copies
+    b += nvt * dim * 8 + nck * 4 + ndc * 4 + (nvt + 1) * 4 + nck * 4 + nck * dim * 8;  // X, cv, cd, star
+    b += nc * 16 * (dim - 1) + nvt * 5;                                                  // face neighbours, vertex flags
+    b += (nd + 1) * 4 + ndc * 4 + nc * 8 + nck * dim * 8 + nck * 8;                    // dof map, ftol, cellX/star
+    if (p->precision != PF2_FP32_MIXED) b += ndc * dim * 8 * 2;                          // FP64 cell V / dV copies
     b += nd * dim * 8 * 3;                                                              // V, V_old, staging
     b += (nc + 1) * 4 * 9 + nc * 4 * (1 + arrival_slots(dim, p));                      // per-cell work, arrival slots
-    b += nc * 8 + nc * 4 + nc * nv * (dim + 1) * 8 + 32;                                 // census, partials, counters
+    b += nc * 8 + nc * 4 + ndc * (dim + 1) * 8 + 32;                                     // census, partials, counters
     b += (size_t)(MOVE_ROUNDS + 2) * (QSHARDS + 1) * QSTRIDE * 4 + (size_t)(MOVE_ROUNDS + 2) * (QSHARDS + 2) * 4;
-    b += 4 + (size_t)STAT_SHARDS * NSTAT * 8;
+    b += 4 + (size_t)STAT_SHARDS * NSTAT * 8 + ((nc + 1 + SCAN_TILE - 1) / SCAN_TILE) * 8;
     b += nck * dim * 4 * 2 + nc * geo32_floats(dim) * 4;                                 // FP32 fields, geometry
     long long per = 4 + 4 + 3 * dim * 8;
     if (p->keep_velocity_ext) {
@@ -839,6 +880,144 @@ int run_step(pf2_handle_s* h, const double* field) {
 }
 
 // ---------------------------------------------------------------------------
+// Batched steps (pf2_batch_*): one launch per kernel of the step for all the
+// simulations of a batch.  The table of simulations and their workgroup
+// ranges is staged in pinned host memory (a ring of BATCH_STAGES buffers,
+// so that a caller may queue that many steps before the host waits) and
+// copied to the device before each step.
+}  // namespace
+
+constexpr int BATCH_STAGES = 4;
+
+struct pf2_batch_s {
+    int max_sims = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    char* dtab = nullptr;
+    size_t tab_bytes = 0;
+    char* htab[BATCH_STAGES] = {};
+    hipEvent_t hev[BATCH_STAGES] = {};         // the staging copy of each buffer has run
+    int slot = 0;
+    std::vector<hipEvent_t> ev;               // per simulation: its stream's work so far
+    hipEvent_t done = nullptr;
+};
+
+namespace {
+
+size_t batch_table_bytes(int max_sims) {
+    const size_t simb = std::max(sizeof(BatchSim<2>), sizeof(BatchSim<3>));
+    return simb * max_sims + (size_t)BK_NKINDS * (max_sims + 1) * sizeof(int);
+}
+
+template <int DIM>
+void batch_fill(pf2_handle_s* h, const double* field, BatchSim<DIM>& S, int* grid) {
+    S.m = dev_mesh<DIM>(h);
+    S.pp = dev_params(h);
+    S.w = dev_work(h);
+    S.qu = dev_queues(h);
+    S.A = dev_particles(h, h->cur);
+    S.B = dev_particles(h, 1 - h->cur);
+    S.RA = rec32(h->buf[h->cur]);
+    S.RB = rec32(h->buf[1 - h->cur]);
+    S.old_off = h->off[h->cur_off];
+    S.field = field;
+    S.V = h->dV;
+    S.Vold = h->dVold;
+    S.cellV = h->dcellV;
+    S.cellDV = mixed(h) ? nullptr : h->dcellDV;
+    S.V32 = mixed(h) ? h->dV32 : nullptr;
+    S.dV32 = h->dVold32;
+    S.geo = h->dgeo32;
+    S.partial = h->partial;
+    S.qcount = h->qcount;
+    S.tiles = h->scan_tiles;
+    const int nc = h->n_cells;
+    S.ntiles = (nc + 1 + SCAN_TILE - 1) / SCAN_TILE;
+    // each kernel's grid as the simulation's own step would launch it
+    const int qgrid = grid_for(h->n_est / 2 + 1, 256, 2048);
+    grid[BK_REFRESH] = grid_for(std::max((long long)nc * h->nd, (long long)h->n_dofs), 256, 4096);
+    grid[BK_MOVE] = grid_for(h->n_est, 256, 8192);
+    for (int r = 0; r <= MOVE_ROUNDS; ++r) grid[BK_SEARCH0 + r] = round_grid(qgrid, r);
+    const bool small = S.pp.nparts_in <= 16;
+    grid[BK_COUNT] = small ? cell_grid(nc, 8192) : 0;
+    grid[BK_COUNT_L] = small ? 0 : cell_grid(nc, 8192);
+    grid[BK_BIG] = 32;
+    grid[BK_TILES] = S.ntiles;
+    grid[BK_WRITE] = cell_grid(nc);
+    grid[BK_PARTIALS] = cell_grid(nc);
+    grid[BK_NODE] = grid_for(h->n_dofs, 256);
+}
+
+template <int DIM>
+int batch_step(pf2_batch_s* b, int n, const pf2_handle* hs, const double* const* fields) {
+    const size_t simb = sizeof(BatchSim<DIM>);
+    const int slot = b->slot;
+    HIPC(hipEventSynchronize(b->hev[slot]));   // this staging buffer's last copy has run
+    char* ht = b->htab[slot];
+    BatchSim<DIM>* sims = reinterpret_cast<BatchSim<DIM>*>(ht);
+    int* starts = reinterpret_cast<int*>(ht + simb * n);
+    int tot[BK_NKINDS] = {};
+    for (int i = 0; i < n; ++i) {
+        int grid[BK_NKINDS];
+        batch_fill<DIM>(hs[i], fields[i], sims[i], grid);
+        for (int k = 0; k < BK_NKINDS; ++k) {
+            starts[(size_t)k * (n + 1) + i] = tot[k];
+            tot[k] += grid[k];
+        }
+    }
+    for (int k = 0; k < BK_NKINDS; ++k) starts[(size_t)k * (n + 1) + n] = tot[k];
+    // after every simulation's earlier work
+    for (int i = 0; i < n; ++i) {
+        HIPC(hipEventRecord(b->ev[i], hs[i]->stream));
+        HIPC(hipStreamWaitEvent(b->stream, b->ev[i], 0));
+    }
+    const size_t bytes = simb * n + (size_t)BK_NKINDS * (n + 1) * sizeof(int);
+    HIPC(hipMemcpyAsync(b->dtab, ht, bytes, hipMemcpyHostToDevice, b->stream));
+    HIPC(hipEventRecord(b->hev[slot], b->stream));
+    b->slot = (slot + 1) % BATCH_STAGES;
+    BatchView<DIM> bv{reinterpret_cast<const BatchSim<DIM>*>(b->dtab), reinterpret_cast<const int*>(b->dtab + simb * n),
+                      n};
+    const bool mx = mixed(hs[0]);
+    hipStream_t st = b->stream;
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(kb_refresh<DIM>), dim3(tot[BK_REFRESH]), dim3(256), 0, st, bv);
+    if (mx) hipLaunchKernelGGL(HIP_KERNEL_NAME(kb_move32<DIM>), dim3(tot[BK_MOVE]), dim3(256), 0, st, bv);
+    else hipLaunchKernelGGL(HIP_KERNEL_NAME(kb_move<DIM>), dim3(tot[BK_MOVE]), dim3(256), 0, st, bv);
+    for (int r = 0; r <= MOVE_ROUNDS; ++r) {
+        const dim3 g(tot[BK_SEARCH0 + r]);
+        if (!mx) hipLaunchKernelGGL(HIP_KERNEL_NAME(kb_search<DIM>), g, dim3(256), 0, st, bv, r);
+        else if (r == MOVE_ROUNDS) hipLaunchKernelGGL(HIP_KERNEL_NAME(kb_search32<DIM, true>), g, dim3(256), 0, st, bv, r);
+        else hipLaunchKernelGGL(HIP_KERNEL_NAME(kb_search32<DIM, false>), g, dim3(256), 0, st, bv, r);
+    }
+    if (tot[BK_COUNT])
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(kb_count<DIM, GCAP>), dim3(tot[BK_COUNT]), dim3(256), 0, st, bv);
+    if (tot[BK_COUNT_L])
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(kb_count<DIM, GCAP_L>), dim3(tot[BK_COUNT_L]), dim3(256), 0, st, bv);
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(kb_big<DIM>), dim3(tot[BK_BIG]), dim3(256), 0, st, bv);
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(kb_scan_tiles<DIM>), dim3(tot[BK_TILES]), dim3(256), 0, st, bv);
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(kb_scan_tops<DIM>), dim3(n), dim3(256), 0, st, bv);
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(kb_scan_apply<DIM>), dim3(tot[BK_TILES]), dim3(256), 0, st, bv);
+    if (mx) hipLaunchKernelGGL(HIP_KERNEL_NAME(kb_write32<DIM>), dim3(tot[BK_WRITE]), dim3(256), 0, st, bv);
+    else hipLaunchKernelGGL(HIP_KERNEL_NAME(kb_write<DIM>), dim3(tot[BK_WRITE]), dim3(256), 0, st, bv);
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(kb_finalize<DIM>), dim3(n), dim3(QSHARDS), 0, st, bv);
+    if (mx) hipLaunchKernelGGL(HIP_KERNEL_NAME(kb_partials32<DIM>), dim3(tot[BK_PARTIALS]), dim3(256), 0, st, bv);
+    else hipLaunchKernelGGL(HIP_KERNEL_NAME(kb_partials<DIM>), dim3(tot[BK_PARTIALS]), dim3(256), 0, st, bv);
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(kb_node<DIM>), dim3(tot[BK_NODE]), dim3(256), 0, st, bv);
+    HIPC(hipGetLastError());
+    // every simulation's later work after the batch
+    HIPC(hipEventRecord(b->done, st));
+    for (int i = 0; i < n; ++i) HIPC(hipStreamWaitEvent(hs[i]->stream, b->done, 0));
+    // the host-side effects of each simulation's own step
+    for (int i = 0; i < n; ++i) {
+        pf2_handle_s* h = hs[i];
+        h->cur = 1 - h->cur;
+        h->cur_off = 1 - h->cur_off;
+        if (mixed(h)) h->fp64_valid = false;
+        h->packed = false;
+    }
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
 // Projection sums of pf2_distribute_particle_velocities_to_grid_static: a
 // file-scope device array instead of the handle's buffer, for callers that
 // may not allocate.  Every GPU holds its own copy at its own address; the
@@ -904,9 +1083,15 @@ int pf2_create_in(const pf2_mesh* mesh, const pf2_params* params, long long capa
     if (mesh->n_cells < 1 || mesh->n_vertices < 1 || mesh->n_dofs < 1 || !mesh->vertices || !mesh->cell_vertices)
         return fail(PF2_E_ARG, "empty mesh");
     if (int r = check_params(mesh->dim, params)) return r;
+    if (mesh->degree != 0 && mesh->degree != 1 && mesh->degree != 2) return fail(PF2_E_ARG, "degree must be 1 or 2");
+    if (mesh->degree == 2 && !mesh->cell_dofs) return fail(PF2_E_ARG, "a degree-2 field needs cell_dofs");
+    if (mesh->degree == 2 && params->precision == PF2_FP32_MIXED)
+        return fail(PF2_E_ARG, "degree-2 fields run on the FP64 path (FP32-mixed handles take Q1 fields)");
     pf2_handle_s* h = new pf2_handle_s();
     h->dim = mesh->dim;
     h->nv = 1 << mesh->dim;
+    h->deg = mesh->degree == 2 ? 2 : 1;
+    h->nd = h->deg == 2 ? (mesh->dim == 2 ? 9 : 27) : h->nv;
     h->n_cells = mesh->n_cells;
     h->n_vertices = mesh->n_vertices;
     h->n_dofs = mesh->n_dofs;
@@ -918,14 +1103,21 @@ int pf2_create_in(const pf2_mesh* mesh, const pf2_params* params, long long capa
         return fail(PF2_E_ARG, "capacity above 2^31-1 particles");
     }
     const size_t nck = (size_t)h->n_cells * h->nv;
+    const size_t ndc = (size_t)h->n_cells * h->nd;   // velocity DoF entries of all cells
     h->X.assign(mesh->vertices, mesh->vertices + (size_t)h->n_vertices * h->dim);
     h->cv.assign(mesh->cell_vertices, mesh->cell_vertices + nck);
     h->cd.assign(mesh->cell_dofs ? mesh->cell_dofs : mesh->cell_vertices,
-                 (mesh->cell_dofs ? mesh->cell_dofs : mesh->cell_vertices) + nck);
+                 (mesh->cell_dofs ? mesh->cell_dofs : mesh->cell_vertices) + ndc);
     for (size_t i = 0; i < nck; ++i) {
-        if (h->cv[i] < 0 || h->cv[i] >= h->n_vertices || h->cd[i] < 0 || h->cd[i] >= h->n_dofs) {
+        if (h->cv[i] < 0 || h->cv[i] >= h->n_vertices) {
             delete h;
-            return fail(PF2_E_ARG, "cell vertex/dof index out of range");
+            return fail(PF2_E_ARG, "cell vertex index out of range");
+        }
+    }
+    for (size_t i = 0; i < ndc; ++i) {
+        if (h->cd[i] < 0 || h->cd[i] >= h->n_dofs) {
+            delete h;
+            return fail(PF2_E_ARG, "cell dof index out of range");
         }
     }
     // vertex star (GridTools::vertex_to_cell_map: std::set => ascending cells)
@@ -970,15 +1162,32 @@ int pf2_create_in(const pf2_mesh* mesh, const pf2_params* params, long long capa
                 }
             }
     }
-    // DoF -> (cell, local vertex), ascending cells
+    // per vertex: on a face with no neighbour (face->at_boundary()), and its DoF
+    std::vector<unsigned char> vbnd(h->n_vertices, 0);
+    std::vector<int> vdof(h->n_vertices, -1);
+    {
+        const int nv = h->nv, nf = 2 * h->dim, stride = h->dim == 2 ? 4 : 8;
+        for (int c = 0; c < h->n_cells; ++c)
+            for (int f = 0; f < nf; ++f)
+                if (h->cell_nbr[(size_t)c * stride + f] < 0)
+                    for (int k = 0; k < nv; ++k)
+                        if (((k >> (f >> 1)) & 1) == (f & 1)) vbnd[h->cv[(size_t)c * nv + k]] = 1;
+        for (int v = 0; v < h->n_vertices; ++v) {
+            if (h->star_off[v] == h->star_off[v + 1]) continue;
+            const int c0 = h->star_cell[h->star_off[v]];
+            for (int k = 0; k < nv; ++k)
+                if (h->cv[(size_t)c0 * nv + k] == v) vdof[v] = h->cd[(size_t)c0 * h->nd + k];
+        }
+    }
+    // DoF -> (cell, local DoF), ascending cells
     h->dof_off.assign(h->n_dofs + 1, 0);
-    for (size_t i = 0; i < nck; ++i) h->dof_off[h->cd[i] + 1]++;
+    for (size_t i = 0; i < ndc; ++i) h->dof_off[h->cd[i] + 1]++;
     for (int v = 0; v < h->n_dofs; ++v) h->dof_off[v + 1] += h->dof_off[v];
     h->dof_cell.assign(h->dof_off[h->n_dofs], 0);
     {
         std::vector<int> fill(h->dof_off.begin(), h->dof_off.end() - 1);
         for (int c = 0; c < h->n_cells; ++c)
-            for (int k = 0; k < h->nv; ++k) h->dof_cell[fill[h->cd[(size_t)c * h->nv + k]]++] = c * h->nv + k;
+            for (int k = 0; k < h->nd; ++k) h->dof_cell[fill[h->cd[(size_t)c * h->nd + k]]++] = c * h->nd + k;
     }
     auto bail = [&](int r) {
         pf2_destroy(h);
@@ -993,20 +1202,22 @@ int pf2_create_in(const pf2_mesh* mesh, const pf2_params* params, long long capa
     } else if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess)
         return bail(fail(PF2_E_HIP, "hipStreamCreate failed"));
     const int nc = h->n_cells, nvt = h->n_vertices, nd = h->n_dofs, dim = h->dim;
-    if (dalloc(&h->dX, (size_t)nvt * dim) || dalloc(&h->dcv, nck) || dalloc(&h->dcd, nck) ||
+    if (dalloc(&h->dX, (size_t)nvt * dim) || dalloc(&h->dcv, nck) || dalloc(&h->dcd, ndc) ||
         dalloc(&h->dstar_off, nvt + 1) || dalloc(&h->dstar_cell, h->star_cell.size()) ||
         dalloc(&h->dstar_dir, h->star_dir.size()) || dalloc(&h->ddof_off, nd + 1) ||
         dalloc(&h->ddof_cell, h->dof_cell.size()) || dalloc(&h->dftol, nc) || dalloc(&h->dcellX, nck * dim) || dalloc(&h->dcell_star, nck) ||
-        dalloc(&h->dcell_nbr, (size_t)h->n_cells * (h->dim - 1)) ||
-        (!mixed(h) && (dalloc(&h->dcellV, nck * dim) || dalloc(&h->dcellDV, nck * dim))) ||
+        dalloc(&h->dcell_nbr, (size_t)h->n_cells * (h->dim - 1)) || dalloc(&h->dvbnd, nvt) ||
+        dalloc(&h->dvdof, nvt) ||
+        (!mixed(h) && (dalloc(&h->dcellV, ndc * dim) || dalloc(&h->dcellDV, ndc * dim))) ||
         dalloc(&h->dV, (size_t)nd * dim) ||
         dalloc(&h->dVold, (size_t)nd * dim) || dalloc(&h->dtmp, (size_t)nd * dim) || dalloc(&h->off[0], nc + 1) ||
         dalloc(&h->off[1], nc + 1) || dalloc(&h->arr_cnt, nc + 1) ||
         dalloc(&h->arr_slot, (size_t)nc * h->aslots) || dalloc(&h->ovf_head, nc) || dalloc(&h->fs_cnt, nc + 1) || dalloc(&h->fs_off, nc + 1) || dalloc(&h->hitmask, nc) || dalloc(&h->nsurv, nc) ||
-        dalloc(&h->partial, (size_t)nc * h->nv * (dim + 1)) || dalloc(&h->counters, 8) || dalloc(&h->big, nc + 1) ||
+        dalloc(&h->partial, ndc * (dim + 1)) || dalloc(&h->counters, 8) || dalloc(&h->big, nc + 1) ||
         dalloc(&h->qcount, (size_t)(MOVE_ROUNDS + 2) * (QSHARDS + 1) * QSTRIDE) ||
         dalloc(&h->qprefix, (size_t)(MOVE_ROUNDS + 2) * (QSHARDS + 2)) ||
-        dalloc(&h->pcounter, 1) || dalloc(&h->stats, STAT_SHARDS * NSTAT))
+        dalloc(&h->pcounter, 1) || dalloc(&h->stats, STAT_SHARDS * NSTAT) ||
+        dalloc(&h->scan_tiles, (size_t)(nc + 1 + SCAN_TILE - 1) / SCAN_TILE))
         return bail(PF2_E_HIP);
     if (mixed(h) || true) {
         if (dalloc(&h->dV32, nck * dim) || dalloc(&h->dVold32, nck * dim) ||
@@ -1021,13 +1232,14 @@ int pf2_create_in(const pf2_mesh* mesh, const pf2_params* params, long long capa
         return hipMemcpyAsync(d, s, bytes, hipMemcpyHostToDevice, h->stream);
     };
     if (up(h->dX, h->X.data(), h->X.size() * 8) || up(h->dcv, h->cv.data(), nck * 4) ||
-        up(h->dcd, h->cd.data(), nck * 4) || up(h->dstar_off, h->star_off.data(), h->star_off.size() * 4) ||
+        up(h->dcd, h->cd.data(), ndc * 4) || up(h->dstar_off, h->star_off.data(), h->star_off.size() * 4) ||
         up(h->dstar_cell, h->star_cell.data(), h->star_cell.size() * 4) ||
         up(h->dstar_dir, h->star_dir.data(), h->star_dir.size() * 8) ||
         up(h->ddof_off, h->dof_off.data(), h->dof_off.size() * 4) ||
         up(h->ddof_cell, h->dof_cell.data(), h->dof_cell.size() * 4) ||
         up(h->dcell_star, h->cell_star.data(), h->cell_star.size() * 4) ||
-        up(h->dcell_nbr, h->cell_nbr.data(), h->cell_nbr.size() * 4))
+        up(h->dcell_nbr, h->cell_nbr.data(), h->cell_nbr.size() * 4) || up(h->dvbnd, vbnd.data(), nvt) ||
+        up(h->dvdof, vdof.data(), (size_t)nvt * 4))
         return bail(fail(PF2_E_HIP, "mesh upload failed"));
     if (hipMemsetAsync(h->dV, 0, sizeof(double) * nd * dim, h->stream) ||
         hipMemsetAsync(h->dVold, 0, sizeof(double) * nd * dim, h->stream) ||
@@ -1061,11 +1273,11 @@ int pf2_destroy(pf2_handle h) {
     invalidate_graphs(h);
     free_particles(h);
     void* ptrs[] = {h->dX, h->dcv, h->dcd, h->dstar_off, h->dstar_cell, h->dstar_dir, h->ddof_off, h->ddof_cell,
-                    h->dftol, h->dcellX, h->dcell_star, h->dcell_nbr, h->dcellV, h->dcellDV, h->dV, h->dVold, h->dtmp, h->dV32,
+                    h->dftol, h->dcellX, h->dcell_star, h->dcell_nbr, h->dvbnd, h->dvdof, h->dcellV, h->dcellDV, h->dV, h->dVold, h->dtmp, h->dV32,
                     h->dX0, h->dW,
                     h->dVold32, h->dgeo32, h->off[0], h->off[1], h->arr_cnt, h->arr_slot, h->ovf_head, h->fs_cnt,
                     h->fs_off, h->hitmask, h->nsurv, h->partial, h->counters, h->big, h->qcount, h->qprefix,
-                    h->pcounter, h->stats, h->scan_tmp};
+                    h->pcounter, h->stats, h->scan_tmp, h->scan_tiles};
     if (!h->in_block)
         for (void* p : ptrs)
             if (p) hipFree(p);
@@ -1138,12 +1350,19 @@ int pf2_seed_particles(pf2_handle h) {
     HIPC(hipStreamSynchronize(h->stream));
     {
         Timed t(h, K_SEED);
-        if (h->dim == 2)
-            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_seed<2>), dim3(grid_for(total, 256)), dim3(256), 0, h->stream,
-                               dev_mesh<2>(h), h->dV, dev_particles(h, h->cur), pp, base, h->off[h->cur_off]);
+        const int g = grid_for(total, 256);
+        if (h->dim == 2 && h->deg == 2)
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_seed<2, 2>), dim3(g), dim3(256), 0, h->stream, dev_mesh<2>(h), h->dV,
+                               dev_particles(h, h->cur), pp, base, h->off[h->cur_off]);
+        else if (h->dim == 2)
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_seed<2>), dim3(g), dim3(256), 0, h->stream, dev_mesh<2>(h), h->dV,
+                               dev_particles(h, h->cur), pp, base, h->off[h->cur_off]);
+        else if (h->deg == 2)
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_seed<3, 2>), dim3(g), dim3(256), 0, h->stream, dev_mesh<3>(h), h->dV,
+                               dev_particles(h, h->cur), pp, base, h->off[h->cur_off]);
         else
-            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_seed<3>), dim3(grid_for(total, 256)), dim3(256), 0, h->stream,
-                               dev_mesh<3>(h), h->dV, dev_particles(h, h->cur), pp, base, h->off[h->cur_off]);
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_seed<3>), dim3(g), dim3(256), 0, h->stream, dev_mesh<3>(h), h->dV,
+                               dev_particles(h, h->cur), pp, base, h->off[h->cur_off]);
     }
     HIPC(hipGetLastError());
     const int cnt = (int)total;
@@ -1251,14 +1470,20 @@ int pf2_insert_particles(pf2_handle h, int n, const int* cell, const double* xi,
     HIPC(hipMemcpyAsync(h->off[h->cur_off], offh.data(), (h->n_cells + 1) * 4, hipMemcpyHostToDevice, h->stream));
     {
         Timed t(h, K_SEED);
-        if (dim == 2)
-            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_init_particles<2>), dim3(grid_for(N, 256)), dim3(256), 0, h->stream,
-                               dev_mesh<2>(h), h->dV, dev_particles(h, h->cur), dflags, N,
-                               h->params.keep_velocity_ext);
+        const int g = grid_for(N, 256);
+        const int kv = h->params.keep_velocity_ext;
+        if (dim == 2 && h->deg == 2)
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_init_particles<2, 2>), dim3(g), dim3(256), 0, h->stream,
+                               dev_mesh<2>(h), h->dV, dev_particles(h, h->cur), dflags, N, kv);
+        else if (dim == 2)
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_init_particles<2>), dim3(g), dim3(256), 0, h->stream,
+                               dev_mesh<2>(h), h->dV, dev_particles(h, h->cur), dflags, N, kv);
+        else if (h->deg == 2)
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_init_particles<3, 2>), dim3(g), dim3(256), 0, h->stream,
+                               dev_mesh<3>(h), h->dV, dev_particles(h, h->cur), dflags, N, kv);
         else
-            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_init_particles<3>), dim3(grid_for(N, 256)), dim3(256), 0, h->stream,
-                               dev_mesh<3>(h), h->dV, dev_particles(h, h->cur), dflags, N,
-                               h->params.keep_velocity_ext);
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_init_particles<3>), dim3(g), dim3(256), 0, h->stream,
+                               dev_mesh<3>(h), h->dV, dev_particles(h, h->cur), dflags, N, kv);
     }
     HIPC(hipGetLastError());
     const int cnt = (int)N;
@@ -1293,7 +1518,7 @@ int pf2_distribute_particle_velocities_to_grid(pf2_handle h, double* V_out) {
 
 int pf2_distribute_particle_velocities_to_grid_static(pf2_handle h, double* V_out_device) {
     if (!h) return fail(PF2_E_ARG, "handle is NULL");
-    const size_t need = (size_t)h->n_cells * h->nv * (h->dim + 1) * sizeof(double);
+    const size_t need = (size_t)h->n_cells * h->nd * (h->dim + 1) * sizeof(double);
     if (need > STATIC_SCRATCH_BYTES)
         return fail(PF2_E_CAPACITY, "projection sums of " + std::to_string(need) + " bytes exceed the static scratch of " +
                                         std::to_string(STATIC_SCRATCH_BYTES) + " bytes");
@@ -1309,6 +1534,68 @@ int pf2_distribute_particle_velocities_to_grid_static(pf2_handle h, double* V_ou
     if (!r && V_out_device) r = field_to_device(h, V_out_device);
     h->profiling = prof;
     return r;
+}
+
+int pf2_batch_create(int max_sims, void* stream, pf2_batch* out) {
+    if (!out || max_sims < 1) return fail(PF2_E_ARG, "invalid batch size");
+    *out = nullptr;
+    pf2_batch_s* b = new pf2_batch_s();
+    b->max_sims = max_sims;
+    b->tab_bytes = batch_table_bytes(max_sims);
+    auto bail = [&](const char* msg) {
+        pf2_batch_destroy(b);
+        return fail(PF2_E_HIP, msg);
+    };
+    if (stream) {
+        b->stream = (hipStream_t)stream;
+    } else {
+        if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) return bail("stream creation failed");
+        b->own_stream = true;
+    }
+    if (hipMalloc((void**)&b->dtab, b->tab_bytes) != hipSuccess) return bail("batch table allocation failed");
+    for (auto& t : b->htab)
+        if (hipHostMalloc((void**)&t, b->tab_bytes, hipHostMallocDefault) != hipSuccess)
+            return bail("batch table allocation failed");
+    b->ev.assign(max_sims, nullptr);
+    for (auto& e : b->ev)
+        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return bail("event creation failed");
+    for (auto& e : b->hev)
+        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess || hipEventRecord(e, b->stream) != hipSuccess)
+            return bail("event creation failed");
+    if (hipEventCreateWithFlags(&b->done, hipEventDisableTiming) != hipSuccess) return bail("event creation failed");
+    *out = b;
+    return 0;
+}
+
+int pf2_batch_destroy(pf2_batch b) {
+    if (!b) return 0;
+    if (b->stream) hipStreamSynchronize(b->stream);
+    hipFree(b->dtab);
+    for (auto t : b->htab)
+        if (t) hipHostFree(t);
+    for (auto e : b->ev)
+        if (e) hipEventDestroy(e);
+    for (auto e : b->hev)
+        if (e) hipEventDestroy(e);
+    if (b->done) hipEventDestroy(b->done);
+    if (b->own_stream && b->stream) hipStreamDestroy(b->stream);
+    delete b;
+    return 0;
+}
+
+int pf2_batch_step(pf2_batch b, int n, const pf2_handle* hs, const double* const* fields) {
+    if (!b || !hs || !fields || n < 1) return fail(PF2_E_ARG, "NULL argument");
+    if (n > b->max_sims) return fail(PF2_E_ARG, "more simulations than the batch holds");
+    for (int i = 0; i < n; ++i) {
+        if (!hs[i] || !fields[i]) return fail(PF2_E_ARG, "NULL handle or field");
+        if (hs[i]->dim != hs[0]->dim || mixed(hs[i]) != mixed(hs[0]))
+            return fail(PF2_E_ARG, "a batch takes simulations of one dimension and precision");
+        if (hs[i]->deg != 1 || hs[i]->n_walls > 0)
+            return fail(PF2_E_STATE, "a batch takes degree-1 fields without walls");
+        for (int j = 0; j < i; ++j)
+            if (hs[j] == hs[i]) return fail(PF2_E_ARG, "a simulation appears twice in the batch");
+    }
+    return hs[0]->dim == 2 ? batch_step<2>(b, n, hs, fields) : batch_step<3>(b, n, hs, fields);
 }
 
 int pf2_step(pf2_handle h, double* V_out) {
@@ -1344,7 +1631,8 @@ int pf2_estimate_bytes(const pf2_mesh* mesh, const pf2_params* params, long long
     if (mesh->dim != 2 && mesh->dim != 3) return fail(PF2_E_ARG, "dim must be 2 or 3");
     if (int r = check_params(mesh->dim, params)) return r;
     const long long cap = capacity > 0 ? capacity : default_capacity(mesh->dim, mesh->n_cells, params);
-    *bytes = estimate_bytes(mesh->dim, mesh->n_cells, mesh->n_vertices, mesh->n_dofs, cap, params);
+    *bytes = estimate_bytes(mesh->dim, mesh->n_cells, mesh->n_vertices, mesh->n_dofs, cap, params,
+                            mesh->degree == 2 ? 2 : 1);
     return 0;
 }
 
@@ -1357,6 +1645,7 @@ int pf2_set_walls(pf2_handle h, int n, const int* vertex, const int* dof, const 
     h->w_bc = h->w_val = nullptr;
     h->n_walls = h->n_wall_cells = 0;
     if (n <= 0) return 0;
+    if (h->deg != 1) return fail(PF2_E_STATE, "the wall fix-up acts on Q1 vertex DoFs (degree-1 fields)");
     if (!vertex || !dof || !bc || !mask) return fail(PF2_E_ARG, "wall arrays are NULL");
     struct W { int v, dof, mask; double bc[3]; };
     std::vector<W> ws(n);
@@ -1544,6 +1833,85 @@ int pf2_move_mesh_scaled(pf2_handle h, double a) {
     if (!h) return fail(PF2_E_ARG, "handle is NULL");
     if (!h->dX0) return fail(PF2_E_STATE, "a prescribed motion needs keep_velocity_ext");
     return move_mesh_impl(h, nullptr, 0, &a);
+}
+
+int pf2_reinterpolate_fields(pf2_handle h, const double* U, int n_extra, double* extra) {
+    if (!h || !U) return fail(PF2_E_ARG, "NULL argument");
+    if (n_extra < 0 || (n_extra > 0 && !extra)) return fail(PF2_E_ARG, "invalid extra fields");
+    if (h->deg != 1) return fail(PF2_E_STATE, "field reinterpolation acts on Q1 fields");
+    const size_t nvt = h->n_vertices, nd = h->n_dofs, dim = h->dim, ne = (size_t)n_extra * nd;
+    double *dU = nullptr, *dVn = nullptr, *dE = nullptr;
+    int r = 0;
+    if (dalloc(&dU, nvt * dim) || dalloc(&dVn, nd * dim) || dalloc(&dE, 2 * ne)) r = PF2_E_HIP;
+    auto done = [&](int rc) {
+        hipStreamSynchronize(h->stream);
+        hipFree(dU);
+        hipFree(dVn);
+        hipFree(dE);
+        return rc;
+    };
+    if (r) return done(r);
+    if (hipMemcpyAsync(dU, U, nvt * dim * 8, hipMemcpyHostToDevice, h->stream) ||
+        hipMemcpyAsync(dVn, h->dV, nd * dim * 8, hipMemcpyDeviceToDevice, h->stream) ||
+        (ne && hipMemcpyAsync(dE, extra, ne * 8, hipMemcpyHostToDevice, h->stream)) ||
+        (ne && hipMemcpyAsync(dE + ne, dE, ne * 8, hipMemcpyDeviceToDevice, h->stream)))
+        return done(fail(PF2_E_HIP, "reinterpolation upload failed"));
+    const int g = grid_for((long long)nvt, 256);
+    if (h->dim == 2)
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_reinterp<2>), dim3(g), dim3(256), 0, h->stream, dev_mesh<2>(h), h->dvbnd,
+                           h->dvdof, dU, h->dV, dVn, dE, dE + ne, n_extra);
+    else
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_reinterp<3>), dim3(g), dim3(256), 0, h->stream, dev_mesh<3>(h), h->dvbnd,
+                           h->dvdof, dU, h->dV, dVn, dE, dE + ne, n_extra);
+    if (hipGetLastError() != hipSuccess || hipMemcpyAsync(h->dV, dVn, nd * dim * 8, hipMemcpyDeviceToDevice, h->stream) ||
+        (ne && hipMemcpyAsync(extra, dE + ne, ne * 8, hipMemcpyDeviceToHost, h->stream)))
+        return done(fail(PF2_E_HIP, "reinterpolation failed"));
+    h->packed = false;   // V changed: the per-cell copies are refreshed before their next use
+    return done(0);
+}
+
+int pf2_calculate_loads(pf2_handle h, const double* V, const double* P, int n_faces, const int* faces,
+                        const pf2_loads_params* lp, double* out5) {
+    if (!h || !P || !lp || !out5 || (n_faces > 0 && !faces) || n_faces < 0) return fail(PF2_E_ARG, "NULL argument");
+    if (h->dim != 2 || h->deg != 1) return fail(PF2_E_STATE, "surface loads are computed for 2-D Q1 fields");
+    for (int f = 0; f < n_faces; ++f)
+        if (faces[2 * f] < 0 || faces[2 * f] >= h->n_cells || faces[2 * f + 1] < 0 || faces[2 * f + 1] > 3)
+            return fail(PF2_E_ARG, "face (cell, face number) out of range");
+    if (lp->dof_a >= h->n_dofs || lp->dof_e >= h->n_dofs) return fail(PF2_E_ARG, "pressure DoF out of range");
+    const size_t nd = h->n_dofs;
+    double *dV = nullptr, *dP = nullptr, *dT = nullptr, *dO = nullptr;
+    int* dF = nullptr;
+    int r = 0;
+    if (dalloc(&dV, 2 * nd) || dalloc(&dP, nd) || dalloc(&dT, (size_t)8 * n_faces + 8) || dalloc(&dO, 8) ||
+        dalloc(&dF, (size_t)2 * n_faces + 2))
+        r = PF2_E_HIP;
+    auto done = [&](int rc) {
+        hipStreamSynchronize(h->stream);
+        hipFree(dV);
+        hipFree(dP);
+        hipFree(dT);
+        hipFree(dO);
+        hipFree(dF);
+        return rc;
+    };
+    if (r) return done(r);
+    if (V) {
+        if (hipMemcpyAsync(dV, V, 2 * nd * 8, hipMemcpyHostToDevice, h->stream)) return done(fail(PF2_E_HIP, "upload"));
+    } else {
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_to_comp_major<2>), dim3(grid_for((long long)nd, 256)), dim3(256), 0,
+                           h->stream, h->dV, dV, (int)nd);
+    }
+    if (hipMemcpyAsync(dP, P, nd * 8, hipMemcpyHostToDevice, h->stream) ||
+        (n_faces && hipMemcpyAsync(dF, faces, (size_t)n_faces * 8, hipMemcpyHostToDevice, h->stream)))
+        return done(fail(PF2_E_HIP, "upload"));
+    if (n_faces)
+        hipLaunchKernelGGL(k_loads_terms, dim3(grid_for(2LL * n_faces, 256)), dim3(256), 0, h->stream, dev_mesh<2>(h),
+                           dF, n_faces, dV, dP, lp->mu, dT);
+    hipLaunchKernelGGL(k_loads_sum, dim3(1), dim3(64), 0, h->stream, dT, 2 * n_faces, dP, lp->rho, lp->u_mean,
+                       lp->diam, lp->dof_a, lp->dof_e, dO);
+    if (hipGetLastError() != hipSuccess || hipMemcpyAsync(out5, dO, 5 * 8, hipMemcpyDeviceToHost, h->stream))
+        return done(fail(PF2_E_HIP, "loads failed"));
+    return done(0);
 }
 
 int pf2_initialize_maps(pf2_handle h) {

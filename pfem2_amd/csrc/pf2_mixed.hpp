@@ -434,17 +434,13 @@ __device__ __forceinline__ void park32(const Rec32& R, long long cap, const DevW
 }
 
 template <int DIM>
-__global__ void __launch_bounds__(256) k_move32(DevMesh<DIM> m, const float* __restrict__ V32,
-                                                const float* __restrict__ dV32, const float* __restrict__ geo,
-                                                const int* __restrict__ pcell, Rec32 R, double* __restrict__ pvext,
-                                                long long cap, DevParams pp, DevWork w, DevQueues qu, int do_correct,
-                                                int do_move) {
+__device__ __forceinline__ void move32_body(const Grid& G, const DevMesh<DIM>& m, const float* __restrict__ V32, const float* __restrict__ dV32, const float* __restrict__ geo, const int* __restrict__ pcell, const Rec32& R, double* __restrict__ pvext, long long cap, const DevParams& pp, const DevWork& w, const DevQueues& qu, int do_correct, int do_move) {
     constexpr int NV = Q1<DIM>::NV;
     constexpr int NG = DIM == 2 ? 6 : 21;
     const int n = state_dead(w.counters) ? 0 : w.counters[0];
     const float h32 = (float)pp.h;
     const int lane = threadIdx.x & (WAVE - 1);
-    const WaveLoop L = stride_loop(n);
+    const WaveLoop L = stride_loop(n, G);
     for (int base = L.begin; base < L.end; base += L.step) {
         const int i = base + lane;
         const bool valid = i < L.end;
@@ -475,6 +471,15 @@ __global__ void __launch_bounds__(256) k_move32(DevMesh<DIM> m, const float* __r
     }
 }
 
+template <int DIM>
+__global__ void __launch_bounds__(256) k_move32(DevMesh<DIM> m, const float* __restrict__ V32,
+                                                const float* __restrict__ dV32, const float* __restrict__ geo,
+                                                const int* __restrict__ pcell, Rec32 R, double* __restrict__ pvext,
+                                                long long cap, DevParams pp, DevWork w, DevQueues qu, int do_correct,
+                                                int do_move) {
+    move32_body<DIM>(hw_grid(), m, V32, dV32, geo, pcell, R, pvext, cap, pp, w, qu, do_correct, do_move);
+}
+
 // FP64 relocation of queued particles (the reference rule), then the
 // remaining FP32 substeps
 // (LAST: the last round, which loops every particle to its end.  Its
@@ -485,19 +490,15 @@ __global__ void __launch_bounds__(256) k_move32(DevMesh<DIM> m, const float* __r
 // (128 registers, 8 bytes per lane spilled) runs it 12% faster than 3
 // without spills; in 3-D 3 per SIMD (212 bytes spilled) 3% faster than 2.)
 template <int DIM, bool LAST>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LAST ? 2 : (DIM == 2 ? 4 : 3)))) k_search_resume32(DevMesh<DIM> m, const float* __restrict__ V32,
-                                                         const float* __restrict__ geo, const int* __restrict__ pcell,
-                                                         Rec32 R, double* __restrict__ pvext,
-                                                         long long cap, DevParams pp, DevWork w, DevQueues qu,
-                                                         int round) {
+__device__ __forceinline__ void search32_body(const Grid& G, const DevMesh<DIM>& m, const float* __restrict__ V32, const float* __restrict__ geo, const int* __restrict__ pcell, const Rec32& R, double* __restrict__ pvext, long long cap, const DevParams& pp, const DevWork& w, const DevQueues& qu, int round) {
     constexpr int NV = Q1<DIM>::NV;
     constexpr int NG = DIM == 2 ? 6 : 21;
     __shared__ int s_pref[QSHARDS + 2];
-    const int n = queue_prefix(qu, round, s_pref);
+    const int n = queue_prefix(qu, round, s_pref, G);
     constexpr bool last = LAST;
     const float h32 = (float)pp.h;
     const int lane = threadIdx.x & (WAVE - 1);
-    const WaveLoop L = range_loop(n);
+    const WaveLoop L = range_loop(n, G);
     int lost = 0, searches = 0;
     for (int base = L.begin; base < L.end; base += L.step) {
         const int j = base + lane;
@@ -586,6 +587,15 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LAST ?
     }
     wave_stat(w.stats, 0, lost);
     wave_stat(w.stats, 3, searches);
+}
+
+template <int DIM, bool LAST>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LAST ? 2 : (DIM == 2 ? 4 : 3)))) k_search_resume32(DevMesh<DIM> m, const float* __restrict__ V32,
+                                                         const float* __restrict__ geo, const int* __restrict__ pcell,
+                                                         Rec32 R, double* __restrict__ pvext,
+                                                         long long cap, DevParams pp, DevWork w, DevQueues qu,
+                                                         int round) {
+    search32_body<DIM, LAST>(hw_grid(), m, V32, geo, pcell, R, pvext, cap, pp, w, qu, round);
 }
 
 template <int DIM>
@@ -769,20 +779,25 @@ __device__ __forceinline__ void cell_write32_one(const DevMesh<DIM>& m, const fl
 }
 
 template <int DIM, bool FUSE>
+__device__ __forceinline__ void cell_write32_body(const Grid& G, const DevMesh<DIM>& m, const float* __restrict__ V32c, const int* __restrict__ old_off, const uint32_t* __restrict__ aid, const Rec32& A, int* __restrict__ bcell, uint32_t* __restrict__ bid, const Rec32& B, const double* __restrict__ avext, double* __restrict__ bvext, long long cap, const DevParams& pp, const DevWork& w, double* __restrict__ partial) {
+    constexpr bool FB = FUSE && DIM == 3;
+    __shared__ float4 s_rec[FUSE ? GROUPS : 1][FUSE ? FCAP : 1];
+    __shared__ float2 s_recb[FB ? GROUPS : 1][FB ? FCAP : 1];
+    if (state_dead(w.counters)) return;
+    const int g = threadIdx.x / GS, gl = threadIdx.x % GS;
+    for (int c = G.bid * GROUPS + g; c < m.n_cells; c += G.nblk * GROUPS)
+        cell_write32_one<DIM, FUSE>(m, V32c, old_off, aid, A, bcell, bid, B, avext, bvext, cap, pp, w, c, gl,
+                                    s_rec[FUSE ? g : 0], s_recb[FB ? g : 0], partial);
+}
+
+template <int DIM, bool FUSE>
 __global__ void __launch_bounds__(256) k_cell_write32(DevMesh<DIM> m, const float* __restrict__ V32c,
                                                      const int* __restrict__ old_off, const uint32_t* __restrict__ aid,
                                                      Rec32 A, int* __restrict__ bcell, uint32_t* __restrict__ bid,
                                                      Rec32 B, const double* __restrict__ avext,
                                                      double* __restrict__ bvext, long long cap, DevParams pp,
                                                      DevWork w, double* __restrict__ partial) {
-    constexpr bool FB = FUSE && DIM == 3;
-    __shared__ float4 s_rec[FUSE ? GROUPS : 1][FUSE ? FCAP : 1];
-    __shared__ float2 s_recb[FB ? GROUPS : 1][FB ? FCAP : 1];
-    if (state_dead(w.counters)) return;
-    const int g = threadIdx.x / GS, gl = threadIdx.x % GS;
-    for (int c = blockIdx.x * GROUPS + g; c < m.n_cells; c += gridDim.x * GROUPS)
-        cell_write32_one<DIM, FUSE>(m, V32c, old_off, aid, A, bcell, bid, B, avext, bvext, cap, pp, w, c, gl,
-                                    s_rec[FUSE ? g : 0], s_recb[FB ? g : 0], partial);
+    cell_write32_body<DIM, FUSE>(hw_grid(), m, V32c, old_off, aid, A, bcell, bid, B, avext, bvext, cap, pp, w, partial);
 }
 
 // fuse = the projection sums are computed here (final counts per cell
@@ -802,8 +817,7 @@ void launch_cell_write32(hipStream_t s, int grid, DevMesh<DIM> m, const float* V
 // per-cell projection sums in FP32 with Kahan compensation, ascending id order
 // (groups of lanes per cell as in k_cell_partials)
 template <int DIM>
-__global__ void __launch_bounds__(256) k_cell_partials32(int n_cells, const int* __restrict__ off, Rec32 R,
-                                                         double* __restrict__ partial, const int* __restrict__ counters) {
+__device__ __forceinline__ void partials32_body(const Grid& G, int n_cells, const int* __restrict__ off, const Rec32& R, double* __restrict__ partial, const int* __restrict__ counters) {
     if (state_dead(counters)) return;
     constexpr int NV = Q1<DIM>::NV;
     constexpr int NACC = PartialsShape<DIM>::NACC;
@@ -816,7 +830,7 @@ __global__ void __launch_bounds__(256) k_cell_partials32(int n_cells, const int*
     const int comp = gl - kk * (DIM + 1);
     float ft[DIM], fs[DIM];
     shape_one32_coeffs<DIM>(kk, ft, fs);
-    for (int cb = blockIdx.x * NG; cb < n_cells; cb += gridDim.x * NG) {
+    for (int cb = G.bid * NG; cb < n_cells; cb += G.nblk * NG) {
         const int c = cb + g;
         const bool cv = c < n_cells;
         const int b = cv ? off[c] : 0, e = cv ? off[c + 1] : 0;
@@ -858,6 +872,12 @@ __global__ void __launch_bounds__(256) k_cell_partials32(int n_cells, const int*
         }
         if (cv && gl < NACC) partial[(size_t)c * NACC + gl] = (double)sum - (double)cmp;
     }
+}
+
+template <int DIM>
+__global__ void __launch_bounds__(256) k_cell_partials32(int n_cells, const int* __restrict__ off, Rec32 R,
+                                                         double* __restrict__ partial, const int* __restrict__ counters) {
+    partials32_body<DIM>(hw_grid(), n_cells, off, R, partial, counters);
 }
 
 template <int DIM>

@@ -7,9 +7,14 @@
 // at creation, and places every admitted simulation in a free range of it
 // (first fit, ranges coalesced on release), so admission and retirement
 // never call hipMalloc or hipFree while other simulations run.  Each active
-// simulation owns a stream from a pool and is kept a few chunks of steps
-// ahead; a chunk is a run of graph-replayed steps followed by an event.
-// Completion is observed by polling the event of the last chunk.
+// simulation owns a stream from a pool.  Batched mode (the default of the
+// bindings): every step advances all active simulations of one dimension and
+// precision together with pf2_batch_step, one launch per kernel of the step
+// for all of them, kept a few steps ahead on the group's stream; a moving
+// mesh is moved on its simulation's stream after each step.  Otherwise each
+// simulation is kept a few chunks of graph-replayed steps ahead on its own
+// stream.  Completion is observed by polling the event of a simulation's
+// last step.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -38,7 +43,7 @@ struct Request {
     long long id = 0;
     int state = PENDING;
     // inputs, kept on the host until admission
-    int dim = 2, n_cells = 0, n_vertices = 0, n_dofs = 0;
+    int dim = 2, n_cells = 0, n_vertices = 0, n_dofs = 0, degree = 1;
     std::vector<double> X, V;
     std::vector<int> cv, cd;
     pf2_params params{};
@@ -105,8 +110,18 @@ size_t round_up(size_t n) { return (n + kAlign - 1) & ~(kAlign - 1); }
 
 }  // namespace
 
+// a batch of the simulations of one dimension and precision, and the events
+// of its steps in flight
+struct BatchGroup {
+    pf2_batch b = nullptr;
+    hipStream_t stream = nullptr;
+    std::deque<hipEvent_t> inflight;
+};
+constexpr int kBatchAhead = 3;   // batched steps queued ahead (below the batch's staging ring)
+
 struct pf2_server_s {
     pf2_server_config cfg{};
+    BatchGroup groups[2][2];   // [dim - 2][precision]
     char* block = nullptr;
     size_t block_bytes = 0;
     RangeAlloc ranges;
@@ -191,7 +206,7 @@ void mark_failed(pf2_server_s* s, Request* r, int code, const std::string& msg) 
 }
 
 pf2_mesh mesh_of(const Request* r) {
-    pf2_mesh m;
+    pf2_mesh m{};
     m.dim = r->dim;
     m.n_cells = r->n_cells;
     m.n_vertices = r->n_vertices;
@@ -199,6 +214,7 @@ pf2_mesh mesh_of(const Request* r) {
     m.vertices = r->X.data();
     m.cell_vertices = r->cv.data();
     m.cell_dofs = r->cd.data();
+    m.degree = r->degree;
     return m;
 }
 
@@ -249,18 +265,25 @@ int admit(pf2_server_s* s, Request* r) {
     return 0;
 }
 
+bool batched(const pf2_server_s* s, const Request* r) { return s->cfg.batched && r->degree == 1; }
+
+// the simulation's step k (1-based) is done: the mesh move of a moving request
+int after_step(Request* r, int k) {
+    if (!r->moving) return 0;
+    const double t = (double)k * r->params.dt;
+    return pf2_move_mesh_scaled(r->h, r->amp * std::sin(r->omega * t + r->phase));
+}
+
 // issue steps so that `depth` chunks are queued
 int issue(pf2_server_s* s, Request* r) {
+    if (batched(s, r)) return 0;   // stepped by issue_batched
     const int chunk = s->cfg.steps_per_chunk > 0 ? s->cfg.steps_per_chunk : 8;
     const int depth = s->cfg.chunks_ahead > 0 ? s->cfg.chunks_ahead : 2;
     while ((int)r->inflight.size() < depth && r->issued < r->n_steps) {
         const int k = std::min(chunk, r->n_steps - r->issued);
         for (int i = 0; i < k; ++i) {
             int rc = pf2_step_with_field(r->h, r->dfield);
-            if (!rc && r->moving) {
-                const double t = (double)(r->issued + i + 1) * r->params.dt;
-                rc = pf2_move_mesh_scaled(r->h, r->amp * std::sin(r->omega * t + r->phase));
-            }
+            if (!rc) rc = after_step(r, r->issued + i + 1);
             if (rc) {
                 mark_failed(s, r, rc, pf2_last_error());
                 return 1;
@@ -274,6 +297,65 @@ int issue(pf2_server_s* s, Request* r) {
         }
         r->inflight.push_back(e);
     }
+    return 0;
+}
+
+// Batched mode: up to kBatchAhead batched steps queued per group; each takes
+// every active simulation of the group that has steps left.  A simulation
+// whose last step was issued gets an event on its stream (after the batch
+// and its mesh move), which progress() polls.
+int issue_batched(pf2_server_s* s) {
+    for (int di = 0; di < 2; ++di)
+        for (int pr = 0; pr < 2; ++pr) {
+            BatchGroup& g = s->groups[di][pr];
+            while (!g.inflight.empty() && hipEventQuery(g.inflight.front()) == hipSuccess) {
+                s->free_events.push_back(g.inflight.front());
+                g.inflight.pop_front();
+            }
+            for (;;) {
+                std::vector<Request*> mem;
+                for (Request* r : s->active)
+                    if (r->state == ACTIVE && batched(s, r) && r->dim == di + 2 && r->params.precision == pr &&
+                        r->issued < r->n_steps)
+                        mem.push_back(r);
+                if (mem.empty() || (int)g.inflight.size() >= kBatchAhead) break;
+                if (!g.b) {
+                    if (hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking) != hipSuccess)
+                        return sfail(PF2_E_HIP, "stream creation failed");
+                    const int cap = std::max(256, s->cfg.max_active);
+                    if (int rc = pf2_batch_create(cap, g.stream, &g.b)) return sfail(rc, pf2_last_error());
+                }
+                std::vector<pf2_handle> hs;
+                std::vector<const double*> fs;
+                for (Request* r : mem) {
+                    hs.push_back(r->h);
+                    fs.push_back(r->dfield);
+                }
+                if (int rc = pf2_batch_step(g.b, (int)hs.size(), hs.data(), fs.data())) {
+                    const std::string msg = pf2_last_error();
+                    for (Request* r : mem) mark_failed(s, r, rc, msg);
+                    break;
+                }
+                hipEvent_t ge = get_event(s);
+                if (!ge || hipEventRecord(ge, g.stream) != hipSuccess) return sfail(PF2_E_HIP, "event record failed");
+                g.inflight.push_back(ge);
+                for (Request* r : mem) {
+                    r->issued += 1;
+                    if (int rc = after_step(r, r->issued)) {
+                        mark_failed(s, r, rc, pf2_last_error());
+                        continue;
+                    }
+                    if (r->issued == r->n_steps) {
+                        hipEvent_t e = get_event(s);
+                        if (!e || hipEventRecord(e, r->stream) != hipSuccess) {
+                            mark_failed(s, r, PF2_E_HIP, "event record failed");
+                            continue;
+                        }
+                        r->inflight.push_back(e);
+                    }
+                }
+            }
+        }
     return 0;
 }
 
@@ -350,6 +432,12 @@ int pf2_server_create(const pf2_server_config* cfg, pf2_server* out) {
 
 int pf2_server_destroy(pf2_server s) {
     if (!s) return 0;
+    for (auto& row : s->groups)
+        for (BatchGroup& g : row) {
+            if (g.b) pf2_batch_destroy(g.b);   // synchronises the group's stream
+            if (g.stream) hipStreamDestroy(g.stream);
+            for (hipEvent_t e : g.inflight) s->free_events.push_back(e);
+        }
     for (Request* r : s->active) release(s, r);
     s->active.clear();
     for (auto& kv : s->all) delete kv.second;
@@ -372,11 +460,17 @@ static int submit_impl(pf2_server_s* s, const pf2_mesh* mesh, const pf2_params* 
     r->n_cells = mesh->n_cells;
     r->n_vertices = mesh->n_vertices;
     r->n_dofs = mesh->n_dofs;
+    r->degree = mesh->degree == 2 ? 2 : 1;
+    if (mesh->degree == 2 && !mesh->cell_dofs) {
+        delete r;
+        return sfail(PF2_E_ARG, "a degree-2 field needs cell_dofs");
+    }
     const size_t nck = (size_t)mesh->n_cells << mesh->dim;
+    const size_t ndc = r->degree == 2 ? (size_t)mesh->n_cells * (mesh->dim == 2 ? 9 : 27) : nck;
     r->X.assign(mesh->vertices, mesh->vertices + (size_t)mesh->n_vertices * mesh->dim);
     r->cv.assign(mesh->cell_vertices, mesh->cell_vertices + nck);
     const int* cd = mesh->cell_dofs ? mesh->cell_dofs : mesh->cell_vertices;
-    r->cd.assign(cd, cd + nck);
+    r->cd.assign(cd, cd + ndc);
     r->V.assign(V, V + (size_t)mesh->n_dofs * mesh->dim);
     r->params = *params;
     r->n_steps = n_steps;
@@ -438,7 +532,9 @@ int pf2_server_poll(pf2_server s, int* n_pending, int* n_active) {
         if (admit(s, r)) break;
         s->pending.pop_front();
     }
-    // keep every active simulation a few chunks ahead
+    // keep the batches and every other active simulation a few steps ahead
+    if (s->cfg.batched)
+        if (int rc = issue_batched(s)) return rc;
     for (size_t i = 0; i < s->active.size();) {
         Request* r = s->active[i];
         issue(s, r);

@@ -2,7 +2,10 @@
 
 Meshes use deal.II's cell vertex order (lexicographic: local vertex k has
 unit coordinates (bit0(k), bit1(k)[, bit2(k)])) with positive orientation, and
-Q1 DoFs numbered like the vertices (cell_dofs == cell_vertices).
+Q1 DoFs numbered like the vertices (cell_dofs == cell_vertices).  q2(mesh)
+adds a degree-2 velocity DoF numbering on the same geometry (FE_Q<dim>(2):
+3^dim DoFs per cell in deal.II's order) with its support points; the field
+generators evaluate at `dof_points`, so they serve both degrees.
 
 Generators:
   channel_2d        structured quads on [0,L]x[0,H], cells and vertices
@@ -26,6 +29,9 @@ class Mesh:
     cells: np.ndarray               # (n_cells, 2**dim) int32
     name: str = ""
     boundary_vertices: dict = field(default_factory=dict)   # tag -> vertex indices
+    degree: int = 1                 # velocity field degree
+    cell_dofs: np.ndarray = None    # degree 2: (n_cells, 3**dim) int32
+    dof_xyz: np.ndarray = None      # degree 2: support points (n_dofs, dim)
 
     @property
     def n_cells(self):
@@ -37,10 +43,62 @@ class Mesh:
 
     @property
     def n_dofs(self):
-        return self.vertices.shape[0]
+        return self.vertices.shape[0] if self.degree == 1 else self.dof_xyz.shape[0]
+
+    @property
+    def dofs_per_cell(self):
+        return (1 << self.dim) if self.degree == 1 else 3 ** self.dim
+
+    @property
+    def dof_points(self):
+        """Support points of the velocity DoFs (the vertices for degree 1)."""
+        return self.vertices if self.degree == 1 else self.dof_xyz
 
     def cell_vertices_xyz(self):
         return self.vertices[self.cells]
+
+
+# local vertex pairs of the lines and vertex quadruples of the faces, in
+# deal.II's GeometryInfo numbering (2-D lines: x=0, x=1, y=0, y=1; 3-D lines:
+# the four of face z=0, the four of z=1, then the four parallel to z; faces:
+# x=0, x=1, y=0, y=1, z=0, z=1)
+LINES = {2: [(0, 2), (1, 3), (0, 1), (2, 3)],
+         3: [(0, 2), (1, 3), (0, 1), (2, 3), (4, 6), (5, 7), (4, 5), (6, 7), (0, 4), (1, 5), (2, 6), (3, 7)]}
+FACES3 = [(0, 2, 4, 6), (1, 3, 5, 7), (0, 1, 4, 5), (2, 3, 6, 7), (0, 1, 2, 3), (4, 5, 6, 7)]
+
+
+def q2(mesh):
+    """The mesh with a degree-2 velocity DoF numbering (FE_Q<dim>(2) on the
+    Q1 geometry).  Per cell 3^dim DoFs in deal.II's order: the vertices, one
+    per line, (3-D) one per face, one interior.  Global numbering: vertex
+    DoFs keep the vertex numbers, then the lines, faces and cell interiors
+    (each in order of their sorted vertex keys / cells).  Support points are
+    the vertex averages of the lines, faces and cells, which is where the
+    multilinear map puts the unit-cell nodes."""
+    dim, cells, X = mesh.dim, np.asarray(mesh.cells), np.asarray(mesh.vertices)
+    nc, nv = cells.shape[0], X.shape[0]
+    parts, points, base = [cells.astype(np.int64)], [X], nv
+    groups = [LINES[dim]] + ([FACES3] if dim == 3 else [])
+    bits = max(1, int(nv - 1).bit_length())
+    assert 3 * bits <= 63, "too many vertices for the packed line/face keys"
+    for grp in groups:
+        keys = np.stack([np.sort(cells[:, list(g)], axis=1) for g in grp], axis=1)   # (nc, n_ent, |g|)
+        flat = keys.reshape(-1, keys.shape[2]).astype(np.int64)
+        # a line is its two vertices; a face of a conforming hex mesh is
+        # fixed by its three smallest vertex numbers
+        packed = flat[:, 0]
+        for j in range(1, min(3, flat.shape[1])):
+            packed = (packed << bits) | flat[:, j]
+        _, first, inv = np.unique(packed, return_index=True, return_inverse=True)
+        parts.append(base + inv.reshape(nc, len(grp)))
+        points.append(X[flat[first]].mean(axis=1))
+        base += len(first)
+    parts.append(base + np.arange(nc)[:, None])
+    points.append(X[cells].mean(axis=1))
+    cd = np.concatenate(parts, axis=1).astype(np.int32)
+    assert cd.shape[1] == 3 ** dim
+    return Mesh(dim, X, cells, name=mesh.name + "-q2", boundary_vertices=mesh.boundary_vertices, degree=2,
+                cell_dofs=cd, dof_xyz=np.concatenate(points, axis=0))
 
 
 def _orient_2d(vertices, cells):
@@ -311,7 +369,7 @@ def fourier_perturbation(points, extent, seed, n_modes=8, kmax=4):
 
 
 def poiseuille_2d(mesh, umax=1.0, H=1.0, eps=0.01, seed=0):
-    x = mesh.vertices
+    x = mesh.dof_points
     V = np.zeros((2, mesh.n_dofs))
     V[0] = 4.0 * umax * x[:, 1] * (H - x[:, 1]) / H ** 2
     ext = x.max(axis=0) - x.min(axis=0)
@@ -322,8 +380,8 @@ def poiseuille_2d(mesh, umax=1.0, H=1.0, eps=0.01, seed=0):
 
 def cylinder_flow(mesh, U=1.0, center=(0.2, 0.2), radius=0.05, eps=0.01, seed=0):
     """Potential flow around a cylinder plus a seeded Fourier perturbation."""
-    X = mesh.vertices[:, 0] - center[0]
-    Y = mesh.vertices[:, 1] - center[1]
+    X = mesh.dof_points[:, 0] - center[0]
+    Y = mesh.dof_points[:, 1] - center[1]
     r2 = np.maximum(X ** 2 + Y ** 2, radius ** 2)
     R2 = radius ** 2
     V = np.zeros((2, mesh.n_dofs))
@@ -331,12 +389,12 @@ def cylinder_flow(mesh, U=1.0, center=(0.2, 0.2), radius=0.05, eps=0.01, seed=0)
     V[1] = -U * 2.0 * R2 * X * Y / r2 ** 2
     if eps:
         ext = mesh.vertices.max(axis=0) - mesh.vertices.min(axis=0)
-        V += eps * U * fourier_perturbation(mesh.vertices, ext, seed)
+        V += eps * U * fourier_perturbation(mesh.dof_points, ext, seed)
     return V
 
 
 def pipe_flow(mesh, umax=1.0, radius=0.5, eps=0.01, seed=0):
-    x = mesh.vertices
+    x = mesh.dof_points
     r2 = x[:, 0] ** 2 + x[:, 1] ** 2
     V = np.zeros((3, mesh.n_dofs))
     V[2] = umax * np.clip(1.0 - r2 / radius ** 2, 0.0, None)
@@ -366,3 +424,23 @@ def jittered_lattice(mesh, quantities, jitter=0.05, seed=0):
         rng = np.random.default_rng(seed)
         xi = xi + rng.uniform(-jitter, jitter, size=xi.shape)
     return cell, np.clip(xi, 0.0, 1.0)
+
+
+def boundary_faces(mesh, tag=None):
+    """(cell, face number) of the faces that belong to one cell only (deal.II
+    face numbering: 2-D x=0, x=1, y=0, y=1; 3-D also z=0, z=1), in the
+    reference's loop order (cells ascending, then faces); with `tag`, only
+    the faces whose vertices all carry that boundary tag."""
+    dim, cells = mesh.dim, np.asarray(mesh.cells)
+    nc, nf = cells.shape[0], 2 * dim
+    fv = [[k for k in range(1 << dim) if ((k >> (f >> 1)) & 1) == (f & 1)] for f in range(nf)]
+    keys = np.stack([np.sort(cells[:, fv[f]], axis=1) for f in range(nf)], axis=1)   # (nc, nf, 2^(dim-1))
+    flat = keys.reshape(nc * nf, -1)
+    _, inv, cnt = np.unique(flat, axis=0, return_inverse=True, return_counts=True)
+    bnd = (cnt[inv.reshape(-1)] == 1).reshape(nc, nf)
+    if tag is not None:
+        on = np.zeros(mesh.n_vertices, dtype=bool)
+        on[np.asarray(mesh.boundary_vertices[tag])] = True
+        bnd &= on[keys].all(axis=2)
+    c, f = np.nonzero(bnd)
+    return np.stack([c, f], axis=1).astype(np.int32)

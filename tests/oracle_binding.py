@@ -22,7 +22,7 @@ def load():
     lib = ctypes.CDLL(ORACLE_PATH)
     vp, ip = ctypes.c_void_p, ctypes.c_int
     lib.orc_create.restype = vp
-    lib.orc_create.argtypes = [ip, ip, ip, vp, vp, ip, vp]
+    lib.orc_create.argtypes = [ip, ip, ip, vp, vp, ip, vp, ip]
     lib.orc_destroy.argtypes = [vp]
     lib.orc_set_params.argtypes = [vp, ctypes.c_double, ip, ip, vp]
     lib.orc_set_filter.argtypes = [vp, ip]
@@ -48,6 +48,7 @@ def load():
     lib.orc_clearly_outside.argtypes = [ip, vp, vp]
     lib.orc_forward_map.argtypes = [ip, vp, vp, vp]
     lib.orc_shape_values.argtypes = [ip, vp, vp]
+    lib.orc_field_shape_values.argtypes = [ip, ip, vp, vp]
     lib.orc_star.argtypes = [vp, ip, vp, vp, ip]
     lib.orc_time_steps.restype = ctypes.c_double
     lib.orc_time_steps.argtypes = [vp, ip]
@@ -72,7 +73,12 @@ class Oracle:
         self.dim = m.dim
         self._v = _f(m.vertices)
         self._c = np.ascontiguousarray(m.cells, dtype=np.int32)
-        self.h = self.lib.orc_create(m.dim, m.n_cells, m.n_vertices, _p(self._v), _p(self._c), m.n_dofs, None)
+        deg = getattr(m, "degree", 1)
+        self._cd = np.ascontiguousarray(m.cell_dofs, dtype=np.int32) if deg == 2 else None
+        self.h = self.lib.orc_create(m.dim, m.n_cells, m.n_vertices, _p(self._v), _p(self._c), m.n_dofs,
+                                     _p(self._cd), deg)
+        if not self.h:
+            raise RuntimeError("orc_create rejected the mesh")
         q = np.ascontiguousarray((list(quantities) + [1, 1, 1])[:3], dtype=np.int32)
         self.lib.orc_set_params(self.h, dt, substeps, max_per_part, _p(q))
         self.lib.orc_set_filter(self.h, 1 if filter else 0)
@@ -227,4 +233,13 @@ def shape_values(dim, xi):
     xi = _f(xi)
     out = np.zeros(1 << dim)
     lib.orc_shape_values(dim, _p(xi), _p(out))
+    return out
+
+
+def field_shape_values(dim, degree, xi):
+    """Shape values of a cell's velocity DoFs (degree 1 or 2) at xi, DoF order."""
+    lib = load()
+    xi = _f(xi)
+    out = np.zeros((1 << dim) if degree == 1 else 3 ** dim)
+    lib.orc_field_shape_values(dim, degree, _p(xi), _p(out))
     return out

@@ -33,9 +33,11 @@ def field_errors(g, o, m, Vg, Vo, n0):
     dv = np.abs(pg["v"][a][same_cell] - po["v"][b][same_cell]) / vmax
     v_err = float(np.max(dv))
     v_p9999 = float(np.quantile(np.max(dv, axis=1), 0.9999))
+    v_p999 = float(np.quantile(np.max(dv, axis=1), 0.999))
     cnt_g, cnt_o = g.cell_counts(), o.cell_counts()
     return {"V_rel_l2": rel_l2, "V_max_rel": float(np.max(np.abs(Vg - Vo)) / np.max(np.abs(Vo))),
-            "x_rel": x_err, "v_rel": v_err, "v_p9999": v_p9999, "owner_mismatch": float(1.0 - same_cell.mean()),
+            "x_rel": x_err, "v_rel": v_err, "v_p9999": v_p9999, "v_p999": v_p999,
+            "owner_mismatch": float(1.0 - same_cell.mean()),
             "count_mismatch": float(np.mean(cnt_g != cnt_o)), "n_gpu": len(pg["id"]), "n_oracle": len(po["id"])}
 
 

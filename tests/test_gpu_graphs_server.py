@@ -126,14 +126,18 @@ def _requests():
     return out
 
 
-def test_server_alone_equals_mix_equals_direct():
+@pytest.mark.parametrize("batched", [True, False])
+def test_server_alone_equals_mix_equals_direct(batched):
+    """A request served alone, in a mix of five and run directly on a handle
+    gives the same bits; batched: the mix advances with one launch per
+    kernel of the step for all active requests."""
     reqs = _requests()
     target = 4   # the cylinder request
     m, V, prm, steps = reqs[target]
     ref_V, ref_n = _direct(m, V, prm, steps)
 
     print("server alone", flush=True)
-    srv = Server(hbm_budget=4 << 30, steps_per_chunk=3, use_graphs=True)
+    srv = Server(hbm_budget=4 << 30, steps_per_chunk=3, use_graphs=True, batched=batched)
     rid = srv.submit(m, V, steps, *prm)
     srv.drain()
     V1, n1, lat1 = srv.result(rid)
@@ -144,7 +148,7 @@ def test_server_alone_equals_mix_equals_direct():
     assert np.array_equal(V1, ref_V, equal_nan=True)
 
     print("server mix", flush=True)
-    srv = Server(hbm_budget=4 << 30, steps_per_chunk=2, chunks_ahead=2, use_graphs=True)
+    srv = Server(hbm_budget=4 << 30, steps_per_chunk=2, chunks_ahead=2, use_graphs=True, batched=batched)
     ids = []
     for k, (mk, Vk, pk, sk) in enumerate(reqs):
         ids.append(srv.submit(mk, Vk, sk, *pk))
@@ -235,7 +239,8 @@ def test_prescribed_motion_equals_host_vertices(precision):
         assert np.array_equal(a, b, equal_nan=True)
 
 
-def test_server_moving_request_alone_equals_mix_equals_direct():
+@pytest.mark.parametrize("batched", [True, False])
+def test_server_moving_request_alone_equals_mix_equals_direct(batched):
     m, V, W = _moving_case()
     dt, S, MAX, q, steps = 0.003, 6, 2, [3, 3], 7
     amp, omega, phase = 0.004, 150.0, 0.3
@@ -251,7 +256,7 @@ def test_server_moving_request_alone_equals_mix_equals_direct():
     st.close()
     res = []
     for mix in (False, True):
-        srv = Server(hbm_budget=4 << 30, steps_per_chunk=3, use_graphs=True)
+        srv = Server(hbm_budget=4 << 30, steps_per_chunk=3, use_graphs=True, batched=batched)
         ids = []
         if mix:
             for mk, Vk, pk, sk in _requests()[:3]:
