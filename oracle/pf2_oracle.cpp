@@ -59,6 +59,20 @@
 // particles with the smallest ids survive.  The projection sums follow the
 // same canonical order.
 //
+// Reference container mode (orc_set_reference_mode): the particles live in a
+// std::unordered_multimap<int, Particle*> keyed by cell, with the container
+// operations of cylinder2D/pfem2particle.cpp in their order: insertion at
+// seeding (:497-498), the re-sort's erase of lost and moved particles and
+// re-insertion of the moved ones through a second multimap (:214-445), the
+// census over equal_range with std::map part counts, the first MAX in
+// container order surviving, new particles inserted before the extra ones are
+// erased (:510-561).  Every per-particle loop visits the cell's equal_range in
+// container order.  The arithmetic is the canonical mode's, so a difference
+// between the two modes comes from the container order alone: with the same
+// standard library the order is the reference's, which measures how often the
+// canonical survivor choice departs from it (and times the reference's data
+// structures).
+
 // An extra, result-preserving test is applied before an inverse map: a point
 // that lies outside a convex quadrilateral cell by more than 1e-8 * diameter
 // (or outside a cell's bounding box by that margin in 3-D) is reported as
@@ -72,6 +86,8 @@
 #include <cstdio>
 #include <vector>
 #include <algorithm>
+#include <map>
+#include <unordered_map>
 #include <chrono>
 #include <stdexcept>
 
@@ -86,6 +102,7 @@ struct Particle {
     double v[3];
     double vext[3];
     uint32_t id;
+    uint32_t ins;   // reference container mode: insertion stamp
 };
 
 struct Wall {
@@ -122,6 +139,13 @@ struct Oracle {
     // statistics
     uint64_t n_lost = 0, n_deleted = 0, n_spawned = 0, n_moved = 0, n_newton = 0;
     std::vector<Wall> walls;
+    // reference container mode
+    bool refmode = false;
+    uint32_t n_ins = 0;
+    std::unordered_multimap<int, Particle*> pm;
+    ~Oracle() {
+        for (auto& kv : pm) delete kv.second;
+    }
 };
 
 // ---- deal.II FE_Q(1) / MappingQ1 restatement ------------------------------
@@ -554,6 +578,8 @@ void lattice_point(const Oracle& o, const int* ijk, double* xi) {
 
 // (cell, reference point) of a particle after its position changed;
 // returns -1 when the reference rule cannot place it (particle is deleted)
+int star_search(Oracle& o, int c, Particle& p, const double* P);
+
 int locate(Oracle& o, int c, Particle& p) {
     const int dim = o.dim;
     double P[MAXV * 3] = {}, r[3];
@@ -562,6 +588,13 @@ int locate(Oracle& o, int c, Particle& p) {
         for (int d = 0; d < dim; ++d) p.xi[d] = r[d];
         return c;
     }
+    return star_search(o, c, p, P);
+}
+
+// the vertex-star rule for a particle that left cell c (vertices P)
+int star_search(Oracle& o, int c, Particle& p, const double* P) {
+    const int dim = o.dim;
+    double r[3];
     ++o.n_moved;
     // closest vertex in the direction of motion (pfem2Particle::find_closest_vertex_of_cell)
     double xold[3];
@@ -683,6 +716,143 @@ void check_cell_for_empty_parts(Oracle& o, int c) {
     lst.swap(out);
 }
 
+// ---- reference container mode ---------------------------------------------
+
+using PMap = std::unordered_multimap<int, Particle*>;
+
+void ref_insert(Oracle& o, int c, Particle* p) {
+    p->ins = ++o.n_ins;
+    o.pm.insert(std::make_pair(c, p));
+}
+
+// the particles of cell c in container order: ascending id (canonical), or
+// the multimap's equal_range order (reference container mode)
+template <class F>
+void for_cell(Oracle& o, int c, F f) {
+    if (o.refmode) {
+        auto r = o.pm.equal_range(c);
+        for (auto it = r.first; it != r.second; ++it) f(*it->second);
+    } else {
+        for (Particle& p : o.cells[c]) f(p);
+    }
+}
+
+// pfem2ParticleHandler::sort_particles_into_subdomains_and_cells
+// (cylinder2D/pfem2particle.cpp:214-445): own-cell test over the container in
+// iteration order, the star rule for the particles that left, then lost
+// particles erased, moved ones erased and re-inserted from a multimap built
+// from (new cell, particle) pairs in the order they were found
+void ref_sort(Oracle& o) {
+    const int dim = o.dim;
+    std::vector<PMap::iterator> out_of_cell;
+    out_of_cell.reserve(o.pm.size());
+    for (auto it = o.pm.begin(); it != o.pm.end(); ++it) {
+        double P[MAXV * 3], r[3];
+        gather_cell_vertices(o, it->first, P);
+        Particle& p = *it->second;
+        if (inverse_map(o, P, p.x, r, &o.n_newton, false) && inside_unit(dim, r))
+            for (int d = 0; d < dim; ++d) p.xi[d] = r[d];
+        else
+            out_of_cell.push_back(it);
+    }
+    std::vector<std::pair<int, Particle*>> sorted;
+    std::vector<PMap::iterator> moved, lost;
+    for (auto it : out_of_cell) {
+        double P[MAXV * 3];
+        gather_cell_vertices(o, it->first, P);
+        const int nc = star_search(o, it->first, *it->second, P);
+        if (nc < 0) {
+            lost.push_back(it);
+            continue;
+        }
+        sorted.push_back(std::make_pair(nc, it->second));
+        moved.push_back(it);
+    }
+    PMap sorted_map;
+    sorted_map.insert(sorted.begin(), sorted.end());
+    for (auto it : lost) {
+        delete it->second;
+        o.pm.erase(it);
+        ++o.n_lost;
+    }
+    for (auto it : moved) o.pm.erase(it);
+    for (auto& kv : sorted_map) kv.second->ins = ++o.n_ins;
+    o.pm.insert(sorted_map.begin(), sorted_map.end());
+}
+
+// check_cell_for_empty_parts (cylinder2D/pfem2particle.cpp:510-561): part
+// counts over equal_range in container order (std::map keyed by the part
+// indices), the particles past MAX in that order marked, new particles
+// inserted for the empty parts, then the marked ones erased.  The reference
+// keeps iterators to the marked particles across the insertions, which a
+// rehash would invalidate; here they are found again by address after the
+// insertions (an erase leaves the order of the other elements unchanged, so
+// the container ends in the same order)
+void ref_check_empty(Oracle& o, int c) {
+    const int dim = o.dim;
+    std::map<std::vector<unsigned>, unsigned> parts;
+    std::vector<Particle*> extra;
+    auto range = o.pm.equal_range(c);
+    for (auto it = range.first; it != range.second; ++it) {
+        std::vector<unsigned> key(dim);
+        for (int d = 0; d < dim; ++d) key[d] = (unsigned)(it->second->xi[d] / (1.0 / o.quant[d]));
+        if (++parts[key] > (unsigned)o.max_per_part) extra.push_back(it->second);
+    }
+    const int qz = dim == 3 ? o.quant[2] : 1;
+    for (int i = 0; i < o.quant[0]; ++i)
+        for (int j = 0; j < o.quant[1]; ++j)
+            for (int k = 0; k < qz; ++k) {
+                std::vector<unsigned> key = {(unsigned)i, (unsigned)j};
+                if (dim == 3) key.push_back((unsigned)k);
+                if (!parts[key]) {
+                    int ijk[3] = {i, j, k};
+                    double xi[3];
+                    lattice_point(o, ijk, xi);
+                    ref_insert(o, c, new Particle(make_particle(o, c, xi)));
+                    ++o.n_spawned;
+                }
+            }
+    for (Particle* q : extra) {
+        auto r = o.pm.equal_range(c);
+        for (auto it = r.first; it != r.second; ++it)
+            if (it->second == q) {
+                o.pm.erase(it);
+                break;
+            }
+        delete q;
+        ++o.n_deleted;
+    }
+}
+
+// move_particles (cylinder2D/pfem2particle.cpp:605-647)
+void ref_move(Oracle& o) {
+    const double hstep = o.dt / o.substeps;
+    for (int s = 0; s < o.substeps; ++s) {
+        for (int c = 0; c < o.n_cells; ++c) {
+            auto range = o.pm.equal_range(c);
+            for (auto it = range.first; it != range.second; ++it) {
+                Particle& p = *it->second;
+                double u[3];
+                interp(o, o.V, c, p.xi, u);
+                for (int d = 0; d < o.dim; ++d) p.vext[d] = u[d];
+                for (int d = 0; d < o.dim; ++d) u[d] *= hstep;
+                for (int d = 0; d < o.dim; ++d) p.x[d] = p.x[d] + u[d];
+            }
+        }
+        ref_sort(o);
+    }
+    for (int c = 0; c < o.n_cells; ++c) ref_check_empty(o, c);
+}
+
+// per-cell lists in ascending id order (downloads and counts)
+std::vector<std::vector<Particle>> ref_snapshot(const Oracle& o) {
+    std::vector<std::vector<Particle>> out(o.n_cells);
+    for (auto& kv : o.pm) out[kv.first].push_back(*kv.second);
+    for (auto& l : out)
+        std::sort(l.begin(), l.end(), [](const Particle& a, const Particle& b) { return a.id < b.id; });
+    return out;
+}
+
 }  // namespace
 
 // ---- C interface (ctypes) --------------------------------------------------
@@ -749,8 +919,37 @@ void orc_seed(void* h) {
                     int ijk[3] = {i, j, k};
                     double xi[3];
                     lattice_point(o, ijk, xi);
-                    o.cells[c].push_back(make_particle(o, c, xi));
+                    if (o.refmode) ref_insert(o, c, new Particle(make_particle(o, c, xi)));
+                    else o.cells[c].push_back(make_particle(o, c, xi));
                 }
+}
+
+// reference container mode on (before any particle exists; returns 0) or off
+// reference container mode: cells whose equal_range is not in descending
+// insertion-stamp order (out[0]), pairs checked (out[1])
+void orc_ref_order_check(void* h, long long* out) {
+    Oracle& o = *static_cast<Oracle*>(h);
+    out[0] = out[1] = 0;
+    for (int c = 0; c < o.n_cells; ++c) {
+        auto r = o.pm.equal_range(c);
+        uint32_t prev = 0xffffffffu;
+        bool bad = false;
+        for (auto it = r.first; it != r.second; ++it) {
+            if (it->second->ins >= prev) bad = true;
+            prev = it->second->ins;
+            ++out[1];
+        }
+        out[0] += bad;
+    }
+}
+
+int orc_set_reference_mode(void* h, int on) {
+    Oracle& o = *static_cast<Oracle*>(h);
+    for (auto& l : o.cells)
+        if (!l.empty()) return -1;
+    if (!o.pm.empty() || !o.walls.empty()) return -1;
+    o.refmode = on != 0;
+    return 0;
 }
 
 // insert particles given by (cell, reference point); velocity interpolated
@@ -761,7 +960,8 @@ void orc_insert(void* h, int n, const int* cell, const double* xi, const double*
         Particle p = make_particle(o, cell[i], xi + (size_t)i * o.dim);
         if (v)
             for (int d = 0; d < o.dim; ++d) p.v[d] = v[(size_t)i * o.dim + d];
-        o.cells[cell[i]].push_back(p);
+        if (o.refmode) ref_insert(o, cell[i], new Particle(p));
+        else o.cells[cell[i]].push_back(p);
     }
     for (auto& lst : o.cells)
         std::sort(lst.begin(), lst.end(), [](const Particle& a, const Particle& b) { return a.id < b.id; });
@@ -772,28 +972,38 @@ void orc_correct(void* h) {
     Oracle& o = *static_cast<Oracle*>(h);
     double N[MAXD];
     for (int c = 0; c < o.n_cells; ++c)
-        for (Particle& p : o.cells[c]) {
+        for_cell(o, c, [&](Particle& p) {
             field_shape_values(o.dim, o.degree, p.xi, N);
             for (int k = 0; k < o.ndpc; ++k) {
                 const int dof = o.cell_dofs[(size_t)c * o.ndpc + k];
                 for (int d = 0; d < o.dim; ++d)
                     p.v[d] = p.v[d] + N[k] * (o.V[(size_t)d * o.n_dofs + dof] - o.Vold[(size_t)d * o.n_dofs + dof]);
             }
-        }
+        });
 }
 
 // particle_handler.sort_particles_into_subdomains_and_cells
-void orc_sort(void* h) { sort_particles(*static_cast<Oracle*>(h)); }
+void orc_sort(void* h) {
+    Oracle& o = *static_cast<Oracle*>(h);
+    if (o.refmode) ref_sort(o);
+    else sort_particles(o);
+}
 
 // check_empty_cells (cylinderDyM/pfem2particle.cpp:759-763)
 void orc_check_empty_cells(void* h) {
     Oracle& o = *static_cast<Oracle*>(h);
-    for (int c = 0; c < o.n_cells; ++c) check_cell_for_empty_parts(o, c);
+    for (int c = 0; c < o.n_cells; ++c)
+        if (o.refmode) ref_check_empty(o, c);
+        else check_cell_for_empty_parts(o, c);
 }
 
 // move_particles (TurekBenchmark/pfem2particle.cpp:813-856)
 void orc_move(void* h) {
     Oracle& o = *static_cast<Oracle*>(h);
+    if (o.refmode) {
+        ref_move(o);
+        return;
+    }
     const double hstep = o.dt / o.substeps;
     for (int s = 0; s < o.substeps; ++s) {
         for (int c = 0; c < o.n_cells; ++c)
@@ -823,11 +1033,11 @@ void orc_project(void* h, double* Vout) {
             for (int d = 0; d < dim; ++d) lv[k][d] = 0.0;
         }
         for (int k = 0; k < o.ndpc; ++k)
-            for (const Particle& p : o.cells[c]) {
+            for_cell(o, c, [&](const Particle& p) {
                 field_shape_values(dim, o.degree, p.xi, N);
                 for (int d = 0; d < dim; ++d) lv[k][d] += N[k] * p.v[d];
                 lw[k] += N[k];
-            }
+            });
         for (int k = 0; k < o.ndpc; ++k) {
             const int dof = o.cell_dofs[(size_t)c * o.ndpc + k];
             for (int d = 0; d < dim; ++d) nodeV[(size_t)d * o.n_dofs + dof] += lv[k][d];
@@ -855,11 +1065,11 @@ void orc_project(void* h, double* Vout) {
                         break;
                     }
                 if (vno < 0) continue;
-                for (Particle& p : o.cells[c]) {
+                for_cell(o, c, [&](Particle& p) {
                     shape_values(dim, p.xi, N);
                     for (int d = 0; d < dim; ++d)
                         if ((w.mask >> d) & 1) p.v[d] = p.v[d] + N[vno] * (w.bc[d] - val[d]);
-                }
+                });
             }
         for (int d = 0; d < dim; ++d)
             if ((w.mask >> d) & 1) o.V[(size_t)d * o.n_dofs + w.dof] = w.bc[d];
@@ -1053,6 +1263,7 @@ void orc_calculate_loads(void* h, const double* V, const double* Pf, int n_faces
 
 long long orc_count(void* h) {
     Oracle& o = *static_cast<Oracle*>(h);
+    if (o.refmode) return (long long)o.pm.size();
     long long n = 0;
     for (auto& l : o.cells) n += (long long)l.size();
     return n;
@@ -1065,8 +1276,9 @@ void orc_download(void* h, int* cell, uint32_t* id, double* x, double* xi, doubl
     Oracle& o = *static_cast<Oracle*>(h);
     size_t i = 0;
     const int dim = o.dim;
+    const std::vector<std::vector<Particle>> snap = o.refmode ? ref_snapshot(o) : std::vector<std::vector<Particle>>();
     for (int c = 0; c < o.n_cells; ++c)
-        for (const Particle& p : o.cells[c]) {
+        for (const Particle& p : o.refmode ? snap[c] : o.cells[c]) {
             if (cell) cell[i] = c;
             if (id) id[i] = p.id;
             for (int d = 0; d < dim; ++d) {
@@ -1080,7 +1292,7 @@ void orc_download(void* h, int* cell, uint32_t* id, double* x, double* xi, doubl
 
 void orc_cell_counts(void* h, int* counts) {
     Oracle& o = *static_cast<Oracle*>(h);
-    for (int c = 0; c < o.n_cells; ++c) counts[c] = (int)o.cells[c].size();
+    for (int c = 0; c < o.n_cells; ++c) counts[c] = (int)(o.refmode ? o.pm.count(c) : o.cells[c].size());
 }
 
 void orc_stats(void* h, unsigned long long* out /* 5 */) {

@@ -52,6 +52,9 @@ def load():
     lib.orc_star.argtypes = [vp, ip, vp, vp, ip]
     lib.orc_time_steps.restype = ctypes.c_double
     lib.orc_time_steps.argtypes = [vp, ip]
+    lib.orc_set_reference_mode.restype = ctypes.c_int
+    lib.orc_set_reference_mode.argtypes = [vp, ip]
+    lib.orc_ref_order_check.argtypes = [vp, vp]
     _lib = lib
     return lib
 
@@ -152,6 +155,20 @@ class Oracle:
 
     def initialize_maps(self):
         self.lib.orc_initialize_maps(self.h)
+
+    def set_reference_mode(self, on=True):
+        """Keep the particles in a std::unordered_multimap keyed by cell and
+        visit them in its order, as the reference's particle handler does
+        (before any particle exists)."""
+        if self.lib.orc_set_reference_mode(self.h, 1 if on else 0) != 0:
+            raise RuntimeError("reference container mode is set before seeding")
+
+    def ref_order_check(self):
+        """Reference container mode: (cells whose equal_range is not in
+        descending insertion order, particles visited)."""
+        out = np.zeros(2, dtype=np.int64)
+        self.lib.orc_ref_order_check(self.h, _p(out))
+        return int(out[0]), int(out[1])
 
     def n_global_particles(self):
         return int(self.lib.orc_count(self.h))
