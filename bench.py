@@ -123,9 +123,11 @@ def moving_workload(scale=2.0):
     return wl
 
 
-def kernel_bytes(name, info, m, dim=2):
+def kernel_bytes(name, info, m, dim=2, vext=False):
     """Algorithmic HBM bytes of one launch of a step kernel, from the work
-    counts of the step (compulsory traffic: every array element touched once)."""
+    counts of the step (compulsory traffic: every array element touched once).
+    vext: the handle keeps the last advection velocity (moving mesh), which
+    the move writes and the regroup carries along."""
     nc, nd = m.n_cells, m.n_dofs
     nv = 1 << dim
     n0, n1 = info["n_before"], info["n_after"]
@@ -137,7 +139,11 @@ def kernel_bytes(name, info, m, dim=2):
         writes = n0 * dim * wv + (n0 - q[0]) * (dim * wx * (2 if name == "k_move" else 1) + 8) \
             + q[0] * (dim * wx * (2 if name == "k_move" else 1) + 4 + 4 + 4 + 8 * dim)
         mesh = nc * nv * 4 + nd * dim * wv * 2 + (m.n_vertices * dim * 8 if name == "k_move" else nc * 24)
-        return reads + writes + mesh
+        return reads + writes + mesh + (n0 * dim * 8 if vext else 0)
+    if name == "k_resort":
+        # own-cell test of every particle after a mesh move: cell, x, xi, vext
+        # in; xi, destination and part key out
+        return n0 * (4 + 3 * dim * 8) + n0 * (dim * 8 + 8) + nc * nv * 4 + m.n_vertices * dim * 8
     if name.startswith("k_search_resume"):
         # one launch per relocation round; round r handles q[r] particles
         mm = re.match(r"(k_search_resume(?:32)?)(?:_r(\d))?$", name)
@@ -149,8 +155,8 @@ def kernel_bytes(name, info, m, dim=2):
         return (n0 + arr) * 12 + arr * 4 + n0 * 4 + nc * (8 + 4 + 4 + 20)
     if name in ("k_cell_write", "k_cell_write32"):
         wx = 8 if name == "k_cell_write" else 4
-        per_src = 4 + 4 + 4 + dim * wx * (3 if name == "k_cell_write" else 2)
-        per_dst = 4 + 4 + dim * wx * (3 if name == "k_cell_write" else 2)
+        per_src = 4 + 4 + 4 + dim * wx * (3 if name == "k_cell_write" else 2) + (dim * 8 if vext else 0)
+        per_dst = 4 + 4 + dim * wx * (3 if name == "k_cell_write" else 2) + (dim * 8 if vext else 0)
         return n1 * (per_src + per_dst) + nc * 32
     if name in ("k_cell_partials", "k_cell_partials32"):
         wx = 8 if name == "k_cell_partials" else 4
@@ -252,6 +258,7 @@ def run_path(args, W, precision, torch):
         "kernels_ms_per_step": {key: v[0] / nprof for key, v in kt.items()},
         "kernel_launches_per_step": {key: v[1] / nprof for key, v in kt.items()},
         "step_info": info,
+        "keep_vext": W.keep_vext,
         "stats": st.stats(),
         "memory_bytes": st.memory_bytes(),
     }
@@ -427,13 +434,14 @@ def roofline(res, m, profile=None):
         fam.setdefault(f, []).append(key)
     dom = max(fam, key=lambda f: sum(per_step[k] for k in fam[f]))
     members = fam[dom]
-    bs = [kernel_bytes(k, res["step_info"], m, m.dim) for k in members]
+    vext = res.get("keep_vext", False)
+    bs = [kernel_bytes(k, res["step_info"], m, m.dim, vext) for k in members]
     b = sum(x for x in bs if x) / len(members) if all(bs) else None
     dur = sum(res["kernels_ms_avg"][k] for k in members) / len(members) * 1e-3
     achieved = b / dur / 1e9 if b else None
     others = {}
     for key in per_step:
-        kb = kernel_bytes(key, res["step_info"], m, m.dim)
+        kb = kernel_bytes(key, res["step_info"], m, m.dim, vext)
         if kb:
             others[key] = {"ms_avg": res["kernels_ms_avg"][key], "bytes_per_launch": kb,
                            "GBs": kb / (res["kernels_ms_avg"][key] * 1e-3) / 1e9,

@@ -45,14 +45,14 @@ int fail(int code, const std::string& msg) {
 enum KernelId {
     K_MOVE = 0, K_GROUP, K_CELL_COUNT, K_CELL_WRITE, K_SCAN, K_FINALIZE, K_PARTIALS, K_NODE, K_WALL,
     K_SEED, K_MISC, K_MOVE32, K_CELL_WRITE32, K_PARTIALS32, K_SEARCH, K_SEARCH32, K_QPREFIX,
-    K_SEARCH_R1, K_SEARCH_R2, K_SEARCH32_R1, K_SEARCH32_R2, K_CELL_BIG, K_NKERNELS
+    K_SEARCH_R1, K_SEARCH_R2, K_SEARCH32_R1, K_SEARCH32_R2, K_CELL_BIG, K_RESORT, K_RESORT32, K_NKERNELS
 };
 const char* kKernelNames[K_NKERNELS] = {"k_move", "k_group", "k_cell_count", "k_cell_write", "scan",
                                         "k_finalize", "k_cell_partials", "k_node", "wall_fixup",
                                         "k_seed", "misc", "k_move32", "k_cell_write32", "k_cell_partials32",
                                         "k_search_resume", "k_search_resume32", "k_queue_prefix",
                                         "k_search_resume_r1", "k_search_resume_r2", "k_search_resume32_r1",
-                                        "k_search_resume32_r2", "k_cell_big"};
+                                        "k_search_resume32_r2", "k_cell_big", "k_resort", "k_resort32"};
 
 // bytes obtained through dalloc on this thread (a handle's footprint is the
 // difference across its creation)
@@ -1695,7 +1695,7 @@ int pf2_sort_particles_into_subdomains_and_cells(pf2_handle h) {
     DevParams pp = dev_params(h);
     DevWork w = dev_work(h);
     {
-        Timed t(h, K_MOVE);
+        Timed t(h, K_RESORT);
         if (h->dim == 2)
             hipLaunchKernelGGL(HIP_KERNEL_NAME(k_resort<2>), dim3(grid_for(h->n_est, 256, 4096)), dim3(256), 0,
                                h->stream, dev_mesh<2>(h), dev_particles(h, h->cur), pp, w);
@@ -1792,7 +1792,7 @@ int move_mesh_impl(pf2_handle_s* h, const double* vertices, int recompute_maps, 
     DevParams pp = dev_params(h);
     DevWork w = dev_work(h);
     {
-        Timed t(h, K_MOVE);
+        Timed t(h, mixed(h) ? K_RESORT32 : K_RESORT);
         if (mixed(h)) {
             h->fp64_valid = false;
             if (h->dim == 2)

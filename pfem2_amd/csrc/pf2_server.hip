@@ -8,10 +8,12 @@
 // (first fit, ranges coalesced on release), so admission and retirement
 // never call hipMalloc or hipFree while other simulations run.  Each active
 // simulation owns a stream from a pool.  Batched mode (the default of the
-// bindings): every step advances all active simulations of one dimension and
-// precision together with pf2_batch_step, one launch per kernel of the step
-// for all of them, kept a few steps ahead on the group's stream; a moving
-// mesh is moved on its simulation's stream after each step.  Otherwise each
+// bindings): every step advances all active simulations of one dimension,
+// precision and size class together with pf2_batch_step, one launch per
+// kernel of the step for all of them, kept a few steps ahead on the group's
+// stream; a moving mesh is moved on its simulation's stream after each step.
+// The size classes keep small simulations from waiting on large ones: the
+// groups' streams run side by side.  Otherwise each
 // simulation is kept a few chunks of graph-replayed steps ahead on its own
 // stream.  Completion is observed by polling the event of a simulation's
 // last step.
@@ -118,10 +120,13 @@ struct BatchGroup {
     std::deque<hipEvent_t> inflight;
 };
 constexpr int kBatchAhead = 3;   // batched steps queued ahead (below the batch's staging ring)
+// size classes of batched simulations by cell count: below 16k, below 128k, larger
+constexpr int kSizeClasses = 3;
+int size_class(int n_cells) { return n_cells < 16384 ? 0 : (n_cells < 131072 ? 1 : 2); }
 
 struct pf2_server_s {
     pf2_server_config cfg{};
-    BatchGroup groups[2][2];   // [dim - 2][precision]
+    BatchGroup groups[2][2][kSizeClasses];   // [dim - 2][precision][size class]
     char* block = nullptr;
     size_t block_bytes = 0;
     RangeAlloc ranges;
@@ -306,8 +311,9 @@ int issue(pf2_server_s* s, Request* r) {
 // and its mesh move), which progress() polls.
 int issue_batched(pf2_server_s* s) {
     for (int di = 0; di < 2; ++di)
-        for (int pr = 0; pr < 2; ++pr) {
-            BatchGroup& g = s->groups[di][pr];
+        for (int k = 0; k < 2 * kSizeClasses; ++k) {
+            const int pr = k / kSizeClasses, sc = k % kSizeClasses;
+            BatchGroup& g = s->groups[di][pr][sc];
             while (!g.inflight.empty() && hipEventQuery(g.inflight.front()) == hipSuccess) {
                 s->free_events.push_back(g.inflight.front());
                 g.inflight.pop_front();
@@ -316,7 +322,7 @@ int issue_batched(pf2_server_s* s) {
                 std::vector<Request*> mem;
                 for (Request* r : s->active)
                     if (r->state == ACTIVE && batched(s, r) && r->dim == di + 2 && r->params.precision == pr &&
-                        r->issued < r->n_steps)
+                        size_class(r->n_cells) == sc && r->issued < r->n_steps)
                         mem.push_back(r);
                 if (mem.empty() || (int)g.inflight.size() >= kBatchAhead) break;
                 if (!g.b) {
@@ -432,12 +438,12 @@ int pf2_server_create(const pf2_server_config* cfg, pf2_server* out) {
 
 int pf2_server_destroy(pf2_server s) {
     if (!s) return 0;
-    for (auto& row : s->groups)
-        for (BatchGroup& g : row) {
-            if (g.b) pf2_batch_destroy(g.b);   // synchronises the group's stream
-            if (g.stream) hipStreamDestroy(g.stream);
-            for (hipEvent_t e : g.inflight) s->free_events.push_back(e);
-        }
+    for (int i = 0; i < 2 * 2 * kSizeClasses; ++i) {
+        BatchGroup& g = (&s->groups[0][0][0])[i];
+        if (g.b) pf2_batch_destroy(g.b);   // synchronises the group's stream
+        if (g.stream) hipStreamDestroy(g.stream);
+        for (hipEvent_t e : g.inflight) s->free_events.push_back(e);
+    }
     for (Request* r : s->active) release(s, r);
     s->active.clear();
     for (auto& kv : s->all) delete kv.second;
