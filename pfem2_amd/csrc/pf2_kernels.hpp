@@ -932,6 +932,8 @@ constexpr int GROUPS = 256 / GS;     // groups per 256-thread block
 constexpr int GCAP = 64;             // candidates staged in LDS per group (lattices of up to 16 parts)
 constexpr int GCAP_L = 128;          // the same for larger lattices
 constexpr int FCAP = 64;             // final particles per cell staged for the fused projection sums
+constexpr int WU = 4;                // survivors per lane and pass in the regroup write
+constexpr int NKMAX = 272;           // part keys counted per group in the census (prod(q_d + 1))
 
 // put(particle, position) for each survivor of cell c as the census left
 // them: listed in id order in the cell's slot row when they fit, else by
@@ -1004,9 +1006,17 @@ __device__ __forceinline__ void cell_count_body(const Grid& G, int n_cells, cons
     __shared__ int s_idx[GROUPS][CAP + 4];
     __shared__ __align__(16) unsigned long long s_kid[GROUPS][CAP + 2];
     __shared__ __align__(16) uint32_t s_sid[GROUPS][CAP + 4];
+    // per group: the cell's particles per part key, one byte each (four to
+    // a word, counted with word atomics; counts stay below CAP <= 128)
+    __shared__ unsigned int s_cnt[GROUPS][NKMAX / 4];
     if (state_dead(w.counters)) return;
     const int g = threadIdx.x / GS, gl = threadIdx.x % GS;
     const unsigned long long all = pp.nparts_in >= 64 ? ~0ull : ((1ull << pp.nparts_in) - 1);
+    int nk = 1;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) nk *= pp.q[d] + 1;
+    const bool by_count = nk <= NKMAX;
+    for (int t = gl; t < NKMAX / 4; t += GS) s_cnt[g][t] = 0;
     int deleted = 0, arrivals = 0;
     for (int cb = G.bid * GROUPS; cb < n_cells; cb += G.nblk * GROUPS) {
         const int c = cb + g;
@@ -1017,10 +1027,13 @@ __device__ __forceinline__ void cell_count_body(const Grid& G, int n_cells, cons
         const bool small = k <= CAP && na <= w.aslots;
         if (cv && gl == 0) arrivals += na;
         unsigned long long hit = 0;
-        int nv = 0, nsurv = 0;   // nv: valid candidates, the same in all lanes of the group
+        // nv: valid candidates, nst of them stayers (the first nst staged
+        // entries, in ascending id order like the cell's segment); the same
+        // in all lanes of the group
+        int nv = 0, nst = 0, nsurv = 0;
+        // the group's lanes in its wavefront's ballot start at bit sh
+        const int sh = (threadIdx.x & (WAVE - 1)) & ~(GS - 1);
         if (small) {
-            // the group's lanes in its wavefront's ballot start at bit sh
-            const int sh = (threadIdx.x & (WAVE - 1)) & ~(GS - 1);
             for (int j0 = 0; j0 < k; j0 += GS) {
                 const int j = j0 + gl;
                 int idx = -1;
@@ -1034,6 +1047,7 @@ __device__ __forceinline__ void cell_count_body(const Grid& G, int n_cells, cons
                     }
                 }
                 const unsigned mask = (unsigned)(__ballot(idx >= 0) >> sh) & 0xffffu;
+                const unsigned smask = (unsigned)(__ballot(idx >= 0 && j < n_st) >> sh) & 0xffffu;
                 if (idx >= 0) {
                     const int at = nv + __popc(mask & ((1u << gl) - 1u));
                     s_idx[g][at] = idx;
@@ -1041,6 +1055,7 @@ __device__ __forceinline__ void cell_count_body(const Grid& G, int n_cells, cons
                     s_sid[g][at] = ~0u;
                 }
                 nv += __popc(mask);
+                nst += __popc(smask);
             }
             if (gl < ((nv + 3) & ~3) - nv) {
                 s_kid[g][nv + gl] = ~0ull;
@@ -1049,16 +1064,28 @@ __device__ __forceinline__ void cell_count_body(const Grid& G, int n_cells, cons
         }
         const int nvr = (nv + 3) & ~3;
         wave_sync();
+        // rank = particles of the same part with a smaller id, needed only
+        // in a part that holds more than MAX particles: the parts' counts
+        // first, then a scan of the row for the particles of such parts
+        const bool counted = small && reseed && by_count;
+        if (counted) {
+            for (int j = gl; j < nv; j += GS) {
+                const uint32_t key = (uint32_t)(s_kid[g][j] >> 32) & 0xffffu;
+                atomicAdd(&s_cnt[g][key >> 2], 1u << (8 * (key & 3)));
+            }
+            wave_sync();
+        }
         if (small)
             for (int j = gl; j < nv; j += GS) {
                 const unsigned long long me = s_kid[g][j];
                 const uint32_t mk = (uint32_t)(me >> 32), mi = (uint32_t)me;
                 const int o = part_ordinal((int)mk);
                 if (o >= 0) hit |= 1ull << o;
-                // rank = particles of the same part with a smaller id (fill
-                // entries carry no valid key)
+                // rank (fill entries carry no valid key)
                 int rank = 0;
-                if (reseed) {
+                const uint32_t key = mk & 0xffffu;
+                const unsigned cnt = counted ? (s_cnt[g][key >> 2] >> (8 * (key & 3))) & 0xffu : 0xffu;
+                if (reseed && cnt > (unsigned)pp.max_per_part) {
                     for (int t = 0; t < nvr; t += 2) {
                         const ulonglong2 v = *reinterpret_cast<const ulonglong2*>(&s_kid[g][t]);
                         rank += ((uint32_t)(v.x >> 32) == mk && (uint32_t)v.x < mi) ? 1 : 0;
@@ -1075,14 +1102,40 @@ __device__ __forceinline__ void cell_count_body(const Grid& G, int n_cells, cons
             w.big[1 + slot] = c;
         }
         wave_sync();
+        if (counted)   // back to zero for the next cell (the words hold only this cell's parts)
+            for (int j = gl; j < nv; j += GS) s_cnt[g][((uint32_t)(s_kid[g][j] >> 32) & 0xffffu) >> 2] = 0u;
         nsurv = group_sum(nsurv);
         // positions of the survivors in ascending id order (ids are below
         // 2^32 - 1, the fill value): the survivors in that order replace the
         // cell's arrival slots (already staged) when they fit, else each
         // candidate gets its position (-1: dropped)
         const bool listed = nsurv <= w.aslots;
+        // positions: with few arrivals, a stayer's is the surviving stayers
+        // before it plus the surviving arrivals with a smaller id; otherwise
+        // (and for the arrivals) a scan of all entries
+        const int jp = small && nv - nst <= 4 ? nst : 0;
+        if (small && jp > 0) {
+            int sbase = 0;
+            for (int j0 = 0; j0 < jp; j0 += GS) {
+                const int j = j0 + gl;
+                const bool act = j < jp;
+                const uint32_t id = act ? s_sid[g][j] : ~0u;
+                const bool sv = act && id != ~0u;
+                const unsigned mask = (unsigned)(__ballot(sv) >> sh) & 0xffffu;
+                if (sv) {
+                    int pos = sbase + __popc(mask & ((1u << gl) - 1u));
+                    for (int t = nst; t < nv; ++t) pos += s_sid[g][t] < id ? 1 : 0;
+                    const int idx = s_idx[g][j];
+                    if (listed) w.arr_slot[(size_t)c * w.aslots + pos] = idx;
+                    else w.spos[idx] = pos;
+                } else if (act && !listed) {
+                    w.spos[s_idx[g][j]] = -1;
+                }
+                sbase += __popc(mask);
+            }
+        }
         if (small)
-            for (int j = gl; j < nv; j += GS) {
+            for (int j = jp + gl; j < nv; j += GS) {
                 const int idx = s_idx[g][j];
                 const uint32_t id = s_sid[g][j];
                 if (id == ~0u) {
@@ -1331,7 +1384,57 @@ __device__ __forceinline__ void cell_write_one(const DevMesh<DIM>& m, const doub
 #pragma unroll
             for (int d = 0; d < DIM; ++d) B.vext[(size_t)d * B.cap + o] = A.vext[(size_t)d * A.cap + idx];
     };
-    survivors_of(w, old_off, c, n_st, kin, nsurv, gl, put);
+    if (nsurv <= w.aslots) {
+        // survivors listed in the slot row: WU of a lane's survivors per
+        // pass, all their loads issued before any store, so that a lane has
+        // several independent loads in flight
+        for (int t0 = 0; t0 < nsurv; t0 += GS * WU) {
+            int src[WU];
+#pragma unroll
+            for (int u = 0; u < WU; ++u) {
+                const int t = t0 + gl + u * GS;
+                src[u] = t < nsurv ? w.arr_slot[(size_t)c * w.aslots + t] : -1;
+            }
+            uint32_t pid[WU];
+            double px[WU][DIM], pxi[WU][DIM], pv[WU][DIM], pe[WU][DIM];
+#pragma unroll
+            for (int u = 0; u < WU; ++u) {
+                if (src[u] < 0) continue;
+                const int idx = src[u];
+                pid[u] = A.id[idx];
+#pragma unroll
+                for (int d = 0; d < DIM; ++d) {
+                    px[u][d] = A.x[(size_t)d * A.cap + idx];
+                    pxi[u][d] = A.xi[(size_t)d * A.cap + idx];
+                    pv[u][d] = A.v[(size_t)d * A.cap + idx];
+                    pe[u][d] = pp.keep_vext ? A.vext[(size_t)d * A.cap + idx] : 0.0;
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < WU; ++u) {
+                if (src[u] < 0) continue;
+                const int pos = t0 + gl + u * GS;
+                const long long o = out0 + pos;
+                B.cell[o] = c;
+                B.id[o] = pid[u];
+#pragma unroll
+                for (int d = 0; d < DIM; ++d) {
+                    B.x[(size_t)d * B.cap + o] = px[u][d];
+                    B.xi[(size_t)d * B.cap + o] = pxi[u][d];
+                    B.v[(size_t)d * B.cap + o] = pv[u][d];
+                    if constexpr (FUSE) {
+                        sxi[pos * DIM + d] = pxi[u][d];
+                        sv[pos * DIM + d] = pv[u][d];
+                    }
+                }
+                if (pp.keep_vext)
+#pragma unroll
+                    for (int d = 0; d < DIM; ++d) B.vext[(size_t)d * B.cap + o] = pe[u][d];
+            }
+        }
+    } else {
+        survivors_of(w, old_off, c, n_st, kin, nsurv, gl, put);
+    }
     const int spawns = nfinal - nsurv;
     if (spawns > 0) {
         const unsigned long long all = pp.nparts_in >= 64 ? ~0ull : ((1ull << pp.nparts_in) - 1);
@@ -1403,6 +1506,157 @@ __global__ void __launch_bounds__(256) k_cell_write(DevMesh<DIM> m, const double
                                                     DevParticles B, DevParams pp, DevWork w,
                                                     double* __restrict__ partial) {
     cell_write_body<DIM, GATHER, FUSE, DEG>(hw_grid(), m, V, old_off, A, B, pp, w, partial);
+}
+
+// ---------------------------------------------------------------------------
+// The regroup write in two passes (regroups without fused projection sums).
+// A cell's output segment starts at an arbitrary position, so the stores of
+// k_cell_write's 16-lane groups never cover a whole cache line and L2 fills
+// every line it writes from memory first (the write volume again in reads).
+// k_cell_map does the per-cell work (offsets, survivors, spawn slots,
+// resets) and writes one int per output position: the source particle, or
+// for a new particle -1 - (cell * 64 + lattice ordinal).  k_cell_gather then
+// walks the output positions in order and copies or creates each particle,
+// so every wavefront stores whole, consecutive lines.  Cell indices below
+// 2^25 (the map's encoding), which the host checks.
+template <int DIM>
+__device__ __forceinline__ void cell_map_body(const Grid& G, int n_cells, const int* __restrict__ old_off,
+                                              long long cap, const DevParams& pp, const DevWork& w,
+                                              int* __restrict__ omap) {
+    if (state_dead(w.counters)) return;
+    const int g = threadIdx.x / GS, gl = threadIdx.x % GS;
+    for (int c = G.bid * GROUPS + g; c < n_cells; c += G.nblk * GROUPS) {
+        const int n_st = old_off[c + 1] - old_off[c];
+        const int na = w.arr_cnt[c];
+        const int kin = n_st + (na < w.aslots ? na : w.aslots);
+        const unsigned long long fo = w.fs_off[c];
+        const long long out0 = (long long)(fo >> 32);
+        const int nsurv = w.nsurv[c];
+        const int nfinal = (int)(w.fs_cnt[c] >> 32);
+        if (gl == 0) w.new_off[c] = (int)out0;
+        if (out0 + nfinal > cap) {
+            if (gl == 0) atomicOr(&w.counters[2], 1);
+            continue;
+        }
+        if (nsurv <= w.aslots) {
+            for (int t = gl; t < nsurv; t += GS) omap[out0 + t] = w.arr_slot[(size_t)c * w.aslots + t];
+        } else {
+            survivors_of(w, old_off, c, n_st, kin, nsurv, gl, [&](int idx, int pos) { omap[out0 + pos] = idx; });
+        }
+        if (nfinal > nsurv) {
+            const unsigned long long all = pp.nparts_in >= 64 ? ~0ull : ((1ull << pp.nparts_in) - 1);
+            const unsigned long long empty = all & ~w.hitmask[c];
+            for (int o = gl; o < pp.nparts_in; o += GS) {
+                if (!((empty >> o) & 1)) continue;
+                const int r = __popcll(empty & ((1ull << o) - 1));
+                omap[out0 + nsurv + r] = -1 - (c * 64 + o);
+            }
+        }
+        if (gl == 0) {
+            w.arr_cnt[c] = 0;
+            w.ovf_head[c] = -1;
+        }
+    }
+}
+
+template <int DIM>
+__global__ void __launch_bounds__(256) k_cell_map(int n_cells, const int* __restrict__ old_off, long long cap,
+                                                  DevParams pp, DevWork w, int* __restrict__ omap) {
+    cell_map_body<DIM>(hw_grid(), n_cells, old_off, cap, pp, w, omap);
+}
+
+constexpr int GU = 4;   // output positions per thread and pass of the regroup gather
+
+// a new particle at output position o: code = cell * 64 + lattice ordinal
+template <int DIM, bool GATHER, int DEG>
+__device__ __forceinline__ void gather_spawn(const DevMesh<DIM>& m, const DevParticles& B, const DevParams& pp,
+                                             const DevWork& w, long long o, int code) {
+    constexpr int NV = Q1<DIM>::NV;
+    constexpr int ND = QF<DIM, DEG>::ND;
+    const int c = code >> 6, lo = code & 63;
+    const unsigned long long fo = w.fs_off[c];
+    const int r = (int)(o - (long long)(fo >> 32)) - w.nsurv[c];
+    double xi[DIM], P[NV][DIM], Vc[ND][DIM], x[DIM], v[DIM];
+    lattice_point<DIM>(pp, lo, xi);
+    load_cell<DIM, GATHER, DEG>(m, c, P, Vc);
+    forward<DIM>(P, xi, x);
+    interp<DIM, DEG>(Vc, xi, v);
+    B.cell[o] = c;
+    B.id[o] = *w.particle_counter + (uint32_t)fo + (uint32_t)r + 1u;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+        B.x[(size_t)d * B.cap + o] = x[d];
+        B.xi[(size_t)d * B.cap + o] = xi[d];
+        B.v[(size_t)d * B.cap + o] = v[d];
+        if (pp.keep_vext) B.vext[(size_t)d * B.cap + o] = 0.0;
+    }
+}
+
+// one output position per lane and slice, GU slices of the workgroup's
+// positions per pass, all loads of a pass issued before its stores: a
+// survivor's record is copied (its cell is its destination), a new particle
+// is created at its lattice point as k_cell_write creates it (id after the
+// cell's spawn offset, velocity interpolated from the cell's DoFs,
+// velocity_ext zero)
+template <int DIM, bool GATHER, int DEG = 1>
+__device__ __forceinline__ void cell_gather_body(const Grid& G, const DevMesh<DIM>& m, const DevParticles& A,
+                                                 const DevParticles& B, const DevParams& pp, const DevWork& w,
+                                                 const int* __restrict__ omap) {
+    if (state_dead(w.counters)) return;
+    const long long total = (long long)(w.fs_off[m.n_cells] >> 32);
+    const long long n = total < B.cap ? total : B.cap;
+    const long long bd = blockDim.x, span = (long long)G.nblk * bd * GU;
+    for (long long o0 = (long long)G.bid * bd * GU + threadIdx.x; o0 < n; o0 += span) {
+        int src[GU];
+        bool act[GU];
+#pragma unroll
+        for (int u = 0; u < GU; ++u) {
+            const long long o = o0 + u * bd;
+            act[u] = o < n;
+            src[u] = act[u] ? omap[o] : -1;
+        }
+        int pc[GU];
+        uint32_t pid[GU];
+        double px[GU][DIM], pxi[GU][DIM], pv[GU][DIM], pe[GU][DIM];
+#pragma unroll
+        for (int u = 0; u < GU; ++u) {
+            if (!act[u] || src[u] < 0) continue;
+            const int s = src[u];
+            pc[u] = w.dest[s];
+            pid[u] = A.id[s];
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) {
+                px[u][d] = A.x[(size_t)d * A.cap + s];
+                pxi[u][d] = A.xi[(size_t)d * A.cap + s];
+                pv[u][d] = A.v[(size_t)d * A.cap + s];
+                pe[u][d] = pp.keep_vext ? A.vext[(size_t)d * A.cap + s] : 0.0;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < GU; ++u) {
+            if (!act[u]) continue;
+            const long long o = o0 + u * bd;
+            if (src[u] < 0) {
+                gather_spawn<DIM, GATHER, DEG>(m, B, pp, w, o, -1 - src[u]);
+                continue;
+            }
+            B.cell[o] = pc[u];
+            B.id[o] = pid[u];
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) {
+                B.x[(size_t)d * B.cap + o] = px[u][d];
+                B.xi[(size_t)d * B.cap + o] = pxi[u][d];
+                B.v[(size_t)d * B.cap + o] = pv[u][d];
+                if (pp.keep_vext) B.vext[(size_t)d * B.cap + o] = pe[u][d];
+            }
+        }
+    }
+}
+
+template <int DIM, bool GATHER, int DEG = 1>
+__global__ void __launch_bounds__(256) k_cell_gather(DevMesh<DIM> m, DevParticles A, DevParticles B, DevParams pp,
+                                                     DevWork w, const int* __restrict__ omap) {
+    cell_gather_body<DIM, GATHER, DEG>(hw_grid(), m, A, B, pp, w, omap);
 }
 
 // totals after the scans: particle count, id counter, statistics; resets the

@@ -45,14 +45,16 @@ int fail(int code, const std::string& msg) {
 enum KernelId {
     K_MOVE = 0, K_GROUP, K_CELL_COUNT, K_CELL_WRITE, K_SCAN, K_FINALIZE, K_PARTIALS, K_NODE, K_WALL,
     K_SEED, K_MISC, K_MOVE32, K_CELL_WRITE32, K_PARTIALS32, K_SEARCH, K_SEARCH32, K_QPREFIX,
-    K_SEARCH_R1, K_SEARCH_R2, K_SEARCH32_R1, K_SEARCH32_R2, K_CELL_BIG, K_RESORT, K_RESORT32, K_NKERNELS
+    K_SEARCH_R1, K_SEARCH_R2, K_SEARCH32_R1, K_SEARCH32_R2, K_CELL_BIG, K_RESORT, K_RESORT32, K_CELL_MAP,
+    K_CELL_GATHER, K_CELL_GATHER32, K_NKERNELS
 };
 const char* kKernelNames[K_NKERNELS] = {"k_move", "k_group", "k_cell_count", "k_cell_write", "scan",
                                         "k_finalize", "k_cell_partials", "k_node", "wall_fixup",
                                         "k_seed", "misc", "k_move32", "k_cell_write32", "k_cell_partials32",
                                         "k_search_resume", "k_search_resume32", "k_queue_prefix",
                                         "k_search_resume_r1", "k_search_resume_r2", "k_search_resume32_r1",
-                                        "k_search_resume32_r2", "k_cell_big", "k_resort", "k_resort32"};
+                                        "k_search_resume32_r2", "k_cell_big", "k_resort", "k_resort32",
+                                        "k_cell_map", "k_cell_gather", "k_cell_gather32"};
 
 // bytes obtained through dalloc on this thread (a handle's footprint is the
 // difference across its creation)
@@ -138,6 +140,7 @@ struct pf2_handle_s {
     int cur_off = 0;
     // work
     int *dest = nullptr, *pkey = nullptr, *spos = nullptr;
+    int* omap = nullptr;   // regroup output map (k_cell_map -> k_cell_gather)
     int *arr_cnt = nullptr, *arr_slot = nullptr, *ovf_head = nullptr;
     int2* ovf = nullptr;
     int aslots = 0;
@@ -420,7 +423,7 @@ int alloc_particles(pf2_handle_s* h) {
     }
     h->shard_cap = 2 * ((h->cap + QSHARDS - 1) / QSHARDS) + 64;
     const long long qlen = QSHARDS * h->shard_cap + h->cap;
-    if (dalloc(&h->dest, h->cap) || dalloc(&h->pkey, h->cap) || dalloc(&h->spos, h->cap) ||
+    if (dalloc(&h->dest, h->cap) || dalloc(&h->pkey, h->cap) || dalloc(&h->spos, h->cap) || dalloc(&h->omap, h->cap) ||
         dalloc(&h->ovf, h->cap) || dalloc(&h->queue[0], qlen) || dalloc(&h->queue[1], qlen) ||
         dalloc(&h->park_s, h->cap) || dalloc(&h->park_vext, h->cap * h->dim))
         return PF2_E_HIP;
@@ -435,9 +438,9 @@ void free_particles(pf2_handle_s* h) {
         hipFree(pb.s32a); hipFree(pb.s32b);
         pb = PBufs();
     }
-    hipFree(h->dest); hipFree(h->pkey); hipFree(h->spos); hipFree(h->ovf);
+    hipFree(h->dest); hipFree(h->pkey); hipFree(h->spos); hipFree(h->omap); hipFree(h->ovf);
     hipFree(h->queue[0]); hipFree(h->queue[1]); hipFree(h->park_s); hipFree(h->park_vext);
-    h->dest = h->pkey = h->spos = h->park_s = nullptr;
+    h->dest = h->pkey = h->spos = h->omap = h->park_s = nullptr;
     h->ovf = nullptr;
     h->queue[0] = h->queue[1] = nullptr;
     h->park_vext = nullptr;
@@ -495,16 +498,54 @@ int regroup(pf2_handle_s* h, int reseed, bool fuse_sums = false, bool defer_fina
                            h->buf[h->cur].id, pp, w, reseed);
     }
     if (int r = scan(h, h->fs_cnt, h->fs_off, nc + 1)) return r;
-    if (mixed(h) && !fp64_state) {
+    const bool two_pass = !fuse_sums && h->omap && nc < (1 << 25);
+    if (two_pass) {
+        // per-cell map, then the output positions in order (pf2_kernels.hpp)
+        Timed t(h, K_CELL_MAP);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cell_map<DIM>), dim3(cell_grid(nc)), dim3(256), 0, h->stream, nc,
+                           old_off, h->cap, pp, w, h->omap);
+    }
+    if (mixed(h) && !fp64_state && two_pass) {
+        Timed t(h, K_CELL_GATHER32);
+        const dim3 g(grid_for(h->n_est, 256, 8192));
+        const PBufs &a = h->buf[h->cur], &b = h->buf[1 - h->cur];
+        if (h->deg == 2)
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cell_gather32<DIM, 2>), g, dim3(256), 0, h->stream, dev_mesh<DIM>(h),
+                               h->dV32, a.id, rec32(a), b.cell, b.id, rec32(b), a.vext, b.vext, h->cap, pp, w, h->omap);
+        else
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cell_gather32<DIM>), g, dim3(256), 0, h->stream, dev_mesh<DIM>(h),
+                               h->dV32, a.id, rec32(a), b.cell, b.id, rec32(b), a.vext, b.vext, h->cap, pp, w, h->omap);
+    } else if (mixed(h) && !fp64_state) {
         Timed t(h, K_CELL_WRITE32);
         launch_cell_write32<DIM>(h->stream, cell_grid(nc), dev_mesh<DIM>(h), h->dV32, old_off, h->buf[h->cur].id,
                                  rec32(h->buf[h->cur]), h->buf[1 - h->cur].cell, h->buf[1 - h->cur].id,
                                  rec32(h->buf[1 - h->cur]), h->buf[h->cur].vext, h->buf[1 - h->cur].vext, h->cap, pp,
-                                 w, fuse_sums, h->partial);
+                                 w, fuse_sums, h->partial, h->deg);
+    } else if (two_pass) {
+        Timed t(h, K_CELL_GATHER);
+        const dim3 g(grid_for(h->n_est, 256, 8192));
+        DevParticles a = dev_particles(h, h->cur), b = dev_particles(h, 1 - h->cur);
+        // (an FP32-mixed handle's FP64 regroup gathers V through the DoF map)
+        if (!h->dcellV && h->deg == 2)
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cell_gather<DIM, true, 2>), g, dim3(256), 0, h->stream,
+                               dev_mesh<DIM>(h), a, b, pp, w, h->omap);
+        else if (!h->dcellV)
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cell_gather<DIM, true>), g, dim3(256), 0, h->stream,
+                               dev_mesh<DIM>(h), a, b, pp, w, h->omap);
+        else if (h->deg == 2)
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cell_gather<DIM, false, 2>), g, dim3(256), 0, h->stream,
+                               dev_mesh<DIM>(h), a, b, pp, w, h->omap);
+        else
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cell_gather<DIM, false>), g, dim3(256), 0, h->stream,
+                               dev_mesh<DIM>(h), a, b, pp, w, h->omap);
     } else {
         Timed t(h, K_CELL_WRITE);
         // (an FP32-mixed handle's FP64 regroup gathers V and never fuses)
-        if (!h->dcellV)
+        if (!h->dcellV && h->deg == 2)
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cell_write<DIM, true, false, 2>), dim3(cell_grid(nc)), dim3(256), 0,
+                               h->stream, dev_mesh<DIM>(h), h->dV, old_off, dev_particles(h, h->cur),
+                               dev_particles(h, 1 - h->cur), pp, w, h->partial);
+        else if (!h->dcellV)
             hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cell_write<DIM, true, false>), dim3(cell_grid(nc)), dim3(256), 0,
                                h->stream, dev_mesh<DIM>(h), h->dV, old_off, dev_particles(h, h->cur),
                                dev_particles(h, 1 - h->cur), pp, w, h->partial);
@@ -551,13 +592,13 @@ int run_move(pf2_handle_s* h, int do_correct, int do_move, bool fuse_sums = fals
         {
             Timed t(h, K_MOVE32);
             launch_move32<DIM>(h->stream, grid, dev_mesh<DIM>(h), h->dV32, h->dVold32, h->dgeo32, b.cell, rec32(b),
-                               b.vext, h->cap, pp, w, qu, do_correct, do_move);
+                               b.vext, h->cap, pp, w, qu, do_correct, do_move, h->deg);
         }
         if (do_move)
             for (int r = 0; r <= MOVE_ROUNDS; ++r) {
                 Timed t(h, r == 0 ? K_SEARCH32 : (r == 1 ? K_SEARCH32_R1 : K_SEARCH32_R2));
                 launch_search_resume32<DIM>(h->stream, round_grid(qgrid, r), dev_mesh<DIM>(h), h->dV32,
-                                            h->dgeo32, b.cell, rec32(b), b.vext, h->cap, pp, w, qu, r);
+                                            h->dgeo32, b.cell, rec32(b), b.vext, h->cap, pp, w, qu, r, h->deg);
             }
     } else {
         {
@@ -605,7 +646,7 @@ int run_project(pf2_handle_s* h, bool sums_done = false, double* partial = nullp
     } else if (mixed(h)) {
         Timed t(h, K_PARTIALS32);
         launch_partials32<DIM>(h->stream, cell_grid(nc), nc, h->off[h->cur_off], rec32(h->buf[h->cur]), partial,
-                               h->counters);
+                               h->counters, h->deg);
     } else {
         Timed t(h, K_PARTIALS);
         if (h->deg == 2)
@@ -789,7 +830,7 @@ size_t estimate_bytes(int dim, long long nc, long long nvt, long long nd, long l
     b += nc * 8 + nc * 4 + ndc * (dim + 1) * 8 + 32;                                     // census, partials, counters
     b += (size_t)(MOVE_ROUNDS + 2) * (QSHARDS + 1) * QSTRIDE * 4 + (size_t)(MOVE_ROUNDS + 2) * (QSHARDS + 2) * 4;
     b += 4 + (size_t)STAT_SHARDS * NSTAT * 8 + ((nc + 1 + SCAN_TILE - 1) / SCAN_TILE) * 8;
-    b += nck * dim * 4 * 2 + nc * geo32_floats(dim) * 4;                                 // FP32 fields, geometry
+    b += ndc * dim * 4 * 2 + nc * geo32_floats(dim) * 4;                                 // FP32 fields, geometry
     long long per = 4 + 4 + 3 * dim * 8;
     if (p->keep_velocity_ext) {
         per += dim * 8;
@@ -799,7 +840,7 @@ size_t estimate_bytes(int dim, long long nc, long long nvt, long long nd, long l
     b += 2 * cap * per;                                                                 // two particle buffers
     const long long shard_cap = 2 * ((cap + QSHARDS - 1) / QSHARDS) + 64;
     const long long qlen = QSHARDS * shard_cap + cap;
-    b += cap * 4 * 5 + 2 * qlen * 8 + cap * 4 + cap * dim * 8;                          // work, overflow, queues, parking
+    b += cap * 4 * 6 + 2 * qlen * 8 + cap * 4 + cap * dim * 8;                          // work, map, overflow, queues, parking
     return b + (1u << 20);
 }
 
@@ -1085,8 +1126,6 @@ int pf2_create_in(const pf2_mesh* mesh, const pf2_params* params, long long capa
     if (int r = check_params(mesh->dim, params)) return r;
     if (mesh->degree != 0 && mesh->degree != 1 && mesh->degree != 2) return fail(PF2_E_ARG, "degree must be 1 or 2");
     if (mesh->degree == 2 && !mesh->cell_dofs) return fail(PF2_E_ARG, "a degree-2 field needs cell_dofs");
-    if (mesh->degree == 2 && params->precision == PF2_FP32_MIXED)
-        return fail(PF2_E_ARG, "degree-2 fields run on the FP64 path (FP32-mixed handles take Q1 fields)");
     pf2_handle_s* h = new pf2_handle_s();
     h->dim = mesh->dim;
     h->nv = 1 << mesh->dim;
@@ -1220,7 +1259,7 @@ int pf2_create_in(const pf2_mesh* mesh, const pf2_params* params, long long capa
         dalloc(&h->scan_tiles, (size_t)(nc + 1 + SCAN_TILE - 1) / SCAN_TILE))
         return bail(PF2_E_HIP);
     if (mixed(h) || true) {
-        if (dalloc(&h->dV32, nck * dim) || dalloc(&h->dVold32, nck * dim) ||
+        if (dalloc(&h->dV32, ndc * dim) || dalloc(&h->dVold32, ndc * dim) ||
             dalloc(&h->dgeo32, (size_t)nc * geo32_floats(dim)))
             return bail(PF2_E_HIP);
     }

@@ -301,28 +301,29 @@ __device__ __forceinline__ bool local_step(const float* g, const float* xi, cons
     }
 }
 
-// FP32 coefficients and nodal velocities of cell c (packed per-cell copies)
-template <int DIM>
+// FP32 coefficients and nodal velocities of cell c (packed per-cell copies,
+// QF<DIM, DEG>::ND nodes per cell)
+template <int DIM, int DEG = 1>
 __device__ __forceinline__ void load_cell32(const float* __restrict__ geo, const float* __restrict__ cellV32, int c,
                                             float* g, float (*Fc)[DIM]) {
-    constexpr int NV = Q1<DIM>::NV;
+    constexpr int ND = QF<DIM, DEG>::ND;
     constexpr int NG = DIM == 2 ? 6 : 21;
 #pragma unroll
     for (int q = 0; q < NG; ++q) g[q] = geo[(size_t)c * NG + q];
 #pragma unroll
-    for (int k = 0; k < NV; ++k)
+    for (int k = 0; k < ND; ++k)
 #pragma unroll
-        for (int d = 0; d < DIM; ++d) Fc[k][d] = cellV32[((size_t)c * NV + k) * DIM + d];
+        for (int d = 0; d < DIM; ++d) Fc[k][d] = cellV32[((size_t)c * ND + k) * DIM + d];
 }
 
 // FP32 coefficients and nodal velocities of a candidate cell
-template <int DIM>
+template <int DIM, int DEG = 1>
 struct PrefetchCell32 {
     const float* geo;
     const float* V32;
     float g[DIM == 2 ? 6 : 21];
-    float Vc[Q1<DIM>::NV][DIM];
-    __device__ void load(int cc) { load_cell32<DIM>(geo, V32, cc, g, Vc); }
+    float Vc[QF<DIM, DEG>::ND][DIM];
+    __device__ void load(int cc) { load_cell32<DIM, DEG>(geo, V32, cc, g, Vc); }
 };
 
 template <int DIM>
@@ -367,24 +368,115 @@ __device__ __forceinline__ float eval_coeffs32(const float (*c)[DIM], int d, con
                          __fmaf_rn(__fmaf_rn(c[4][d], xi[0], c[2][d]), xi[1], __fmaf_rn(c[1][d], xi[0], c[0][d])));
 }
 
+// A cell's FP32 field of degree DEG as monomial coefficients of its nodal
+// values, evaluated with nested FMAs.  DEG 1: to_coeffs32 / eval_coeffs32.
+// DEG 2: the nodal values in tensor order (index a_0 + 3 a_1 [+ 9 a_2], axis
+// index a: t = 0, 1, 1/2), then along each direction
+//   (f(0), f(1), f(1/2)) -> (c0, c1, c2),  f(t) = c0 + c1 t + c2 t^2,
+//   c1 = 4 f(1/2) - 3 f(0) - f(1),  c2 = 2 (f(0) + f(1)) - 4 f(1/2),
+// so that entry a_0 + 3 a_1 [+ 9 a_2] is the coefficient of
+// xi0^a_0 xi1^a_1 [xi2^a_2].
+template <int DIM, int DEG>
+struct Field32 {
+    static constexpr int ND = QF<DIM, DEG>::ND;
+    float c[ND][DIM];
+    __device__ __forceinline__ void init(const float (*Vc)[DIM]) {
+        if constexpr (DEG == 1) {
+#pragma unroll
+            for (int k = 0; k < ND; ++k)
+#pragma unroll
+                for (int d = 0; d < DIM; ++d) c[k][d] = Vc[k][d];
+            to_coeffs32<DIM>(c);
+        } else {
+#pragma unroll
+            for (int k = 0; k < ND; ++k) {
+                int t = 0, st = 1;
+#pragma unroll
+                for (int e = 0; e < DIM; ++e) {
+                    t += q2_axis(DIM, k, e) * st;
+                    st *= 3;
+                }
+#pragma unroll
+                for (int d = 0; d < DIM; ++d) c[t][d] = Vc[k][d];
+            }
+#pragma unroll
+            for (int e = 0, st = 1; e < DIM; ++e, st *= 3)
+#pragma unroll
+                for (int b = 0; b < ND; ++b) {
+                    if ((b / st) % 3 != 0) continue;   // first entry of a line along direction e
+#pragma unroll
+                    for (int d = 0; d < DIM; ++d) {
+                        const float f0 = c[b][d], f1 = c[b + st][d], fh = c[b + 2 * st][d];
+                        c[b + st][d] = (4.0f * fh - 3.0f * f0) - f1;
+                        c[b + 2 * st][d] = 2.0f * (f0 + f1) - 4.0f * fh;
+                    }
+                }
+        }
+    }
+    // component d at xi
+    __device__ __forceinline__ float eval(int d, const float* xi) const {
+        if constexpr (DEG == 1) {
+            return eval_coeffs32<DIM>(c, d, xi);
+        } else {
+            auto row = [&](int b) {   // c[b] + c[b+1] xi0 + c[b+2] xi0^2
+                return __fmaf_rn(__fmaf_rn(c[b + 2][d], xi[0], c[b + 1][d]), xi[0], c[b][d]);
+            };
+            auto plane = [&](int b) {   // rows b, b+3, b+6 in xi1
+                return __fmaf_rn(__fmaf_rn(row(b + 6), xi[1], row(b + 3)), xi[1], row(b));
+            };
+            if constexpr (DIM == 2) return plane(0);
+            else return __fmaf_rn(__fmaf_rn(plane(18), xi[2], plane(9)), xi[2], plane(0));
+        }
+    }
+};
+
+// N_k of one node k of a degree-DEG cell in FP32 (a lane's node in the
+// projection sums): DEG 1 as shape_one32; DEG 2 the product over the
+// directions of the 1-D Lagrange factor (c0 t + c1)(c2 t + c3) of the node's
+// axis index (lag2_coeffs)
+template <int DIM, int DEG>
+struct ShapeOne32 {
+    float c[DIM][4];
+    __device__ __forceinline__ void init(int k) {
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) {
+            if constexpr (DEG == 1) {
+                const bool one = (k >> d) & 1;
+                c[d][0] = one ? 1.0f : -1.0f;
+                c[d][1] = one ? 0.0f : 1.0f;
+            } else {
+                double q[4];
+                lag2_coeffs(q2_axis(DIM, k, d), q);
+#pragma unroll
+                for (int j = 0; j < 4; ++j) c[d][j] = (float)q[j];
+            }
+        }
+    }
+    __device__ __forceinline__ float factor(int d, float t) const {
+        if constexpr (DEG == 1) return __fmaf_rn(c[d][0], t, c[d][1]);
+        else return __fmaf_rn(c[d][0], t, c[d][1]) * __fmaf_rn(c[d][2], t, c[d][3]);
+    }
+    __device__ __forceinline__ float operator()(const float* xi) const {
+        float v = factor(0, xi[0]);
+#pragma unroll
+        for (int d = 1; d < DIM; ++d) v *= factor(d, xi[d]);
+        return v;
+    }
+};
+
 // FP32 substeps until done (0) or the reference point leaves the unit cell (1);
 // u (the last velocity) is returned for the relocation.  Vc: the cell's nodal
-// velocities.
-template <int DIM>
+// velocities (QF<DIM, DEG>::ND nodes).
+template <int DIM, int DEG = 1>
 __device__ __forceinline__ int advance32(const DevParams& pp, float h32, int i, double* __restrict__ pvext,
                                          long long cap, float* xi, float* u, int& s, const float* g,
                                          const float (*Vc)[DIM]) {
-    constexpr int NV = Q1<DIM>::NV;
-    float cf[NV][DIM];
-#pragma unroll
-    for (int k = 0; k < NV; ++k)
-#pragma unroll
-        for (int d = 0; d < DIM; ++d) cf[k][d] = Vc[k][d];
-    to_coeffs32<DIM>(cf);
+    Field32<DIM, DEG> cf;
+    cf.init(Vc);
     for (; s < pp.substeps; ++s) {
         float dx[DIM], dl[DIM];
 #pragma unroll
-        for (int d = 0; d < DIM; ++d) u[d] = eval_coeffs32<DIM>(cf, d, xi);
+        for (int d = 0; d < DIM; ++d) u[d] = cf.eval(d, xi);
 #pragma unroll
         for (int d = 0; d < DIM; ++d) dx[d] = u[d] * h32;
         if (pp.keep_vext && s == pp.substeps - 1 && pvext)
@@ -433,9 +525,10 @@ __device__ __forceinline__ void park32(const Rec32& R, long long cap, const DevW
     }
 }
 
-template <int DIM>
+template <int DIM, int DEG = 1>
 __device__ __forceinline__ void move32_body(const Grid& G, const DevMesh<DIM>& m, const float* __restrict__ V32, const float* __restrict__ dV32, const float* __restrict__ geo, const int* __restrict__ pcell, const Rec32& R, double* __restrict__ pvext, long long cap, const DevParams& pp, const DevWork& w, const DevQueues& qu, int do_correct, int do_move) {
     constexpr int NV = Q1<DIM>::NV;
+    constexpr int ND = QF<DIM, DEG>::ND;
     constexpr int NG = DIM == 2 ? 6 : 21;
     const int n = state_dead(w.counters) ? 0 : w.counters[0];
     const float h32 = (float)pp.h;
@@ -450,34 +543,47 @@ __device__ __forceinline__ void move32_body(const Grid& G, const DevMesh<DIM>& m
         for (int d = 0; d < DIM; ++d) xi[d] = v[d] = 0.5f;
         if (valid) rec_load<DIM>(R, i, xi, v);
         if (do_correct && valid) {
-            float N[NV];
-            shape32<DIM>(xi, N);
+            if constexpr (DEG == 1) {
+                float N[NV];
+                shape32<DIM>(xi, N);
 #pragma unroll
-            for (int k = 0; k < NV; ++k)
+                for (int k = 0; k < NV; ++k)
 #pragma unroll
-                for (int d = 0; d < DIM; ++d) v[d] = __fmaf_rn(N[k], dV32[((size_t)c * NV + k) * DIM + d], v[d]);
+                    for (int d = 0; d < DIM; ++d) v[d] = __fmaf_rn(N[k], dV32[((size_t)c * NV + k) * DIM + d], v[d]);
+            } else {
+                // the increment V - V_old at xi from its monomial coefficients
+                float dVc[ND][DIM];
+#pragma unroll
+                for (int k = 0; k < ND; ++k)
+#pragma unroll
+                    for (int d = 0; d < DIM; ++d) dVc[k][d] = dV32[((size_t)c * ND + k) * DIM + d];
+                Field32<DIM, DEG> f;
+                f.init(dVc);
+#pragma unroll
+                for (int d = 0; d < DIM; ++d) v[d] = v[d] + f.eval(d, xi);
+            }
             rec_store_v<DIM>(R, i, v);
         }
         if (!do_move) continue;
         int st = 0, s = 0;
         float u[DIM];
         if (valid) {
-            float g[NG], Vc[NV][DIM];
-            load_cell32<DIM>(geo, V32, c, g, Vc);
-            st = advance32<DIM>(pp, h32, i, pvext, cap, xi, u, s, g, Vc);
+            float g[NG], Vc[ND][DIM];
+            load_cell32<DIM, DEG>(geo, V32, c, g, Vc);
+            st = advance32<DIM, DEG>(pp, h32, i, pvext, cap, xi, u, s, g, Vc);
         }
         park32<DIM>(R, cap, w, qu, 0, L.shard, valid && st == 1, i, c, xi, u, s);
         finish32<DIM>(pp, pcell, R, w, valid && st == 0, i, c, xi);
     }
 }
 
-template <int DIM>
+template <int DIM, int DEG = 1>
 __global__ void __launch_bounds__(256) k_move32(DevMesh<DIM> m, const float* __restrict__ V32,
                                                 const float* __restrict__ dV32, const float* __restrict__ geo,
                                                 const int* __restrict__ pcell, Rec32 R, double* __restrict__ pvext,
                                                 long long cap, DevParams pp, DevWork w, DevQueues qu, int do_correct,
                                                 int do_move) {
-    move32_body<DIM>(hw_grid(), m, V32, dV32, geo, pcell, R, pvext, cap, pp, w, qu, do_correct, do_move);
+    move32_body<DIM, DEG>(hw_grid(), m, V32, dV32, geo, pcell, R, pvext, cap, pp, w, qu, do_correct, do_move);
 }
 
 // FP64 relocation of queued particles (the reference rule), then the
@@ -489,9 +595,10 @@ __global__ void __launch_bounds__(256) k_move32(DevMesh<DIM> m, const float* __r
 // The first round is latency-bound with many wavefronts: in 2-D 4 per SIMD
 // (128 registers, 8 bytes per lane spilled) runs it 12% faster than 3
 // without spills; in 3-D 3 per SIMD (212 bytes spilled) 3% faster than 2.)
-template <int DIM, bool LAST>
+template <int DIM, bool LAST, int DEG = 1>
 __device__ __forceinline__ void search32_body(const Grid& G, const DevMesh<DIM>& m, const float* __restrict__ V32, const float* __restrict__ geo, const int* __restrict__ pcell, const Rec32& R, double* __restrict__ pvext, long long cap, const DevParams& pp, const DevWork& w, const DevQueues& qu, int round) {
     constexpr int NV = Q1<DIM>::NV;
+    constexpr int ND = QF<DIM, DEG>::ND;
     constexpr int NG = DIM == 2 ? 6 : 21;
     __shared__ int s_pref[QSHARDS + 2];
     const int n = queue_prefix(qu, round, s_pref, G);
@@ -538,8 +645,8 @@ __device__ __forceinline__ void search32_body(const Grid& G, const DevMesh<DIM>&
                 for (int d = 0; d < DIM; ++d) x[d] = x[d] + (double)(u[d] * h32);
                 int cn;
                 if constexpr (DIM == 2) {
-                    PrefetchCell32<DIM> pre{geo, V32};
-                    cn = relocate<DIM, PrefetchCell32<DIM>, !LAST>(m, c, P, xid, vext, x, r, Q, pre);
+                    PrefetchCell32<DIM, DEG> pre{geo, V32};
+                    cn = relocate<DIM, PrefetchCell32<DIM, DEG>, !LAST>(m, c, P, xid, vext, x, r, Q, pre);
                     if (cn >= 0) {
 #pragma unroll
                         for (int d = 0; d < DIM; ++d) {
@@ -547,7 +654,7 @@ __device__ __forceinline__ void search32_body(const Grid& G, const DevMesh<DIM>&
                             xi[d] = t < 0.0f ? 0.0f : (t > 1.0f ? 1.0f : t);
                         }
                         ++s;
-                        st = advance32<DIM>(pp, h32, i, pvext, cap, xi, u, s, pre.g, pre.Vc);
+                        st = advance32<DIM, DEG>(pp, h32, i, pvext, cap, xi, u, s, pre.g, pre.Vc);
                     }
                 } else {
                     NoPrefetch pre;
@@ -558,10 +665,10 @@ __device__ __forceinline__ void search32_body(const Grid& G, const DevMesh<DIM>&
                             const float t = (float)r[d];
                             xi[d] = t < 0.0f ? 0.0f : (t > 1.0f ? 1.0f : t);
                         }
-                        float g[NG], Vc[NV][DIM];
-                        load_cell32<DIM>(geo, V32, cn, g, Vc);
+                        float g[NG], Vc[ND][DIM];
+                        load_cell32<DIM, DEG>(geo, V32, cn, g, Vc);
                         ++s;
-                        st = advance32<DIM>(pp, h32, i, pvext, cap, xi, u, s, g, Vc);
+                        st = advance32<DIM, DEG>(pp, h32, i, pvext, cap, xi, u, s, g, Vc);
                     }
                 }
                 if (cn < 0) {
@@ -589,28 +696,40 @@ __device__ __forceinline__ void search32_body(const Grid& G, const DevMesh<DIM>&
     wave_stat(w.stats, 3, searches);
 }
 
-template <int DIM, bool LAST>
+template <int DIM, bool LAST, int DEG = 1>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LAST ? 2 : (DIM == 2 ? 4 : 3)))) k_search_resume32(DevMesh<DIM> m, const float* __restrict__ V32,
                                                          const float* __restrict__ geo, const int* __restrict__ pcell,
                                                          Rec32 R, double* __restrict__ pvext,
                                                          long long cap, DevParams pp, DevWork w, DevQueues qu,
                                                          int round) {
-    search32_body<DIM, LAST>(hw_grid(), m, V32, geo, pcell, R, pvext, cap, pp, w, qu, round);
+    search32_body<DIM, LAST, DEG>(hw_grid(), m, V32, geo, pcell, R, pvext, cap, pp, w, qu, round);
 }
 
+// deg: the velocity field's degree (1 or 2)
 template <int DIM>
 void launch_move32(hipStream_t s, int grid, DevMesh<DIM> m, const float* V32, const float* dV32, const float* geo,
                    int* pcell, Rec32 R, double* pvext, long long cap, DevParams pp, DevWork w,
-                   DevQueues qu, int do_correct, int do_move) {
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_move32<DIM>), dim3(grid), dim3(256), 0, s, m, V32, dV32, geo, pcell, R,
-                       pvext, cap, pp, w, qu, do_correct, do_move);
+                   DevQueues qu, int do_correct, int do_move, int deg) {
+    if (deg == 2)
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_move32<DIM, 2>), dim3(grid), dim3(256), 0, s, m, V32, dV32, geo, pcell, R,
+                           pvext, cap, pp, w, qu, do_correct, do_move);
+    else
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_move32<DIM>), dim3(grid), dim3(256), 0, s, m, V32, dV32, geo, pcell, R,
+                           pvext, cap, pp, w, qu, do_correct, do_move);
 }
 
 template <int DIM>
 void launch_search_resume32(hipStream_t s, int grid, DevMesh<DIM> m, const float* V32, const float* geo,
                             const int* pcell, Rec32 R, double* pvext, long long cap, DevParams pp, DevWork w,
-                            DevQueues qu, int round) {
-    if (round == MOVE_ROUNDS)
+                            DevQueues qu, int round, int deg) {
+    const bool last = round == MOVE_ROUNDS;
+    if (deg == 2 && last)
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_search_resume32<DIM, true, 2>), dim3(grid), dim3(256), 0, s, m, V32, geo,
+                           pcell, R, pvext, cap, pp, w, qu, round);
+    else if (deg == 2)
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_search_resume32<DIM, false, 2>), dim3(grid), dim3(256), 0, s, m, V32,
+                           geo, pcell, R, pvext, cap, pp, w, qu, round);
+    else if (last)
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_search_resume32<DIM, true>), dim3(grid), dim3(256), 0, s, m, V32, geo,
                            pcell, R, pvext, cap, pp, w, qu, round);
     else
@@ -625,19 +744,22 @@ void launch_search_resume32(hipStream_t s, int grid, DevMesh<DIM> m, const float
 // Kahan-compensated FP32 projection sums of one cell over n staged records
 // in order (accumulator a < NACC: vertex a / (DIM + 1), component a % (DIM +
 // 1), the last one the weight), as k_cell_partials32 computes them
-template <int DIM, int NA>
+template <int DIM, int NA, int DEG = 1>
 __device__ __forceinline__ void cell_sums32(const float4* __restrict__ rec, const float2* __restrict__ recb, int n,
                                             int a0, double* out) {
     // NA accumulators a0, a0 + GS, ... in one pass over the records (their
-    // Kahan chains are independent, so they overlap)
-    float ft[NA][DIM], fs[NA][DIM], sum[NA], cmp[NA];
+    // Kahan chains are independent, so they overlap); accumulators past
+    // NACC are computed for node 0 and not stored by the caller
+    constexpr int NACC = QF<DIM, DEG>::ND * (DIM + 1);
+    ShapeOne32<DIM, DEG> sh[NA];
+    float sum[NA], cmp[NA];
     int comp[NA];
 #pragma unroll
     for (int u = 0; u < NA; ++u) {
         const int a = a0 + u * GS;
-        const int kk = a / (DIM + 1);
+        const int kk = (a < NACC ? a : 0) / (DIM + 1);
         comp[u] = a - kk * (DIM + 1);
-        shape_one32_coeffs<DIM>(kk, ft[u], fs[u]);
+        sh[u].init(kk);
         sum[u] = 0.0f;
         cmp[u] = 0.0f;
     }
@@ -660,7 +782,7 @@ __device__ __forceinline__ void cell_sums32(const float4* __restrict__ rec, cons
         }
 #pragma unroll
         for (int u = 0; u < NA; ++u) {
-            const float nk = shape_one32<DIM>(ft[u], fs[u], xi);
+            const float nk = sh[u](xi);
             float vc = 1.0f;   // the weight term (comp == DIM) is N_k itself
 #pragma unroll
             for (int d = 0; d < DIM; ++d)
@@ -680,7 +802,7 @@ __device__ __forceinline__ void cell_sums32(const float4* __restrict__ rec, cons
 // final records in LDS and writes its projection sums, so no separate
 // partials pass re-reads the particles (3-D: each lane of the group owns two
 // of the 32 sums)
-template <int DIM, bool FUSE>
+template <int DIM, bool FUSE, int DEG = 1>
 __device__ __forceinline__ void cell_write32_one(const DevMesh<DIM>& m, const float* __restrict__ V32c,
                                                  const int* __restrict__ old_off, const uint32_t* __restrict__ aid,
                                                  const Rec32& A, int* __restrict__ bcell, uint32_t* __restrict__ bid,
@@ -688,7 +810,7 @@ __device__ __forceinline__ void cell_write32_one(const DevMesh<DIM>& m, const fl
                                                  double* __restrict__ bvext, long long cap, const DevParams& pp,
                                                  const DevWork& w, int c, int gl, float4* __restrict__ srec,
                                                  float2* __restrict__ srecb, double* __restrict__ partial) {
-    constexpr int NV = Q1<DIM>::NV;
+    constexpr int ND = QF<DIM, DEG>::ND;
     const int n_st = old_off[c + 1] - old_off[c];
     const int na = w.arr_cnt[c];
     const int kin = n_st + (na < w.aslots ? na : w.aslots);
@@ -732,13 +854,13 @@ __device__ __forceinline__ void cell_write32_one(const DevMesh<DIM>& m, const fl
             const int r = __popcll(empty & ((1ull << o) - 1));
             // velocity from the cell's FP32 copy of V (this handle keeps no
             // FP64 per-cell copy), interpolated in FP64
-            double xi[DIM], Vc[NV][DIM], v[DIM];
+            double xi[DIM], Vc[ND][DIM], v[DIM];
             lattice_point<DIM>(pp, o, xi);
 #pragma unroll
-            for (int k = 0; k < NV; ++k)
+            for (int k = 0; k < ND; ++k)
 #pragma unroll
-                for (int d = 0; d < DIM; ++d) Vc[k][d] = (double)V32c[((size_t)c * NV + k) * DIM + d];
-            interp<DIM>(Vc, xi, v);
+                for (int d = 0; d < DIM; ++d) Vc[k][d] = (double)V32c[((size_t)c * ND + k) * DIM + d];
+            interp<DIM, DEG>(Vc, xi, v);
             const long long q = out0 + nsurv + r;
             bcell[q] = c;
             bid[q] = id0 + (uint32_t)r + 1u;
@@ -765,20 +887,22 @@ __device__ __forceinline__ void cell_write32_one(const DevMesh<DIM>& m, const fl
         w.ovf_head[c] = -1;
     }
     if constexpr (FUSE) {
-        constexpr int NACC = PartialsShape<DIM>::NACC;
+        constexpr int NACC = PartialsShape<DIM, DEG>::NACC;
         wave_sync();
-        constexpr int NA = (NACC + GS - 1) / GS;   // sums per lane: 1 in 2-D (12 lanes), 2 in 3-D
+        // sums per lane: Q1 1 in 2-D (12 lanes), 2 in 3-D; Q2 2 in 2-D, 7 in 3-D
+        constexpr int NA = (NACC + GS - 1) / GS;
         if (gl < NACC) {
             double out[NA];
-            cell_sums32<DIM, NA>(srec, srecb, nfinal, gl, out);
+            cell_sums32<DIM, NA, DEG>(srec, srecb, nfinal, gl, out);
 #pragma unroll
-            for (int u = 0; u < NA; ++u) partial[(size_t)c * NACC + gl + u * GS] = out[u];
+            for (int u = 0; u < NA; ++u)
+                if (gl + u * GS < NACC) partial[(size_t)c * NACC + gl + u * GS] = out[u];
         }
         wave_sync();
     }
 }
 
-template <int DIM, bool FUSE>
+template <int DIM, bool FUSE, int DEG = 1>
 __device__ __forceinline__ void cell_write32_body(const Grid& G, const DevMesh<DIM>& m, const float* __restrict__ V32c, const int* __restrict__ old_off, const uint32_t* __restrict__ aid, const Rec32& A, int* __restrict__ bcell, uint32_t* __restrict__ bid, const Rec32& B, const double* __restrict__ avext, double* __restrict__ bvext, long long cap, const DevParams& pp, const DevWork& w, double* __restrict__ partial) {
     constexpr bool FB = FUSE && DIM == 3;
     __shared__ float4 s_rec[FUSE ? GROUPS : 1][FUSE ? FCAP : 1];
@@ -786,18 +910,120 @@ __device__ __forceinline__ void cell_write32_body(const Grid& G, const DevMesh<D
     if (state_dead(w.counters)) return;
     const int g = threadIdx.x / GS, gl = threadIdx.x % GS;
     for (int c = G.bid * GROUPS + g; c < m.n_cells; c += G.nblk * GROUPS)
-        cell_write32_one<DIM, FUSE>(m, V32c, old_off, aid, A, bcell, bid, B, avext, bvext, cap, pp, w, c, gl,
-                                    s_rec[FUSE ? g : 0], s_recb[FB ? g : 0], partial);
+        cell_write32_one<DIM, FUSE, DEG>(m, V32c, old_off, aid, A, bcell, bid, B, avext, bvext, cap, pp, w, c, gl,
+                                         s_rec[FUSE ? g : 0], s_recb[FB ? g : 0], partial);
 }
 
-template <int DIM, bool FUSE>
+template <int DIM, bool FUSE, int DEG = 1>
 __global__ void __launch_bounds__(256) k_cell_write32(DevMesh<DIM> m, const float* __restrict__ V32c,
                                                      const int* __restrict__ old_off, const uint32_t* __restrict__ aid,
                                                      Rec32 A, int* __restrict__ bcell, uint32_t* __restrict__ bid,
                                                      Rec32 B, const double* __restrict__ avext,
                                                      double* __restrict__ bvext, long long cap, DevParams pp,
                                                      DevWork w, double* __restrict__ partial) {
-    cell_write32_body<DIM, FUSE>(hw_grid(), m, V32c, old_off, aid, A, bcell, bid, B, avext, bvext, cap, pp, w, partial);
+    cell_write32_body<DIM, FUSE, DEG>(hw_grid(), m, V32c, old_off, aid, A, bcell, bid, B, avext, bvext, cap, pp, w,
+                                      partial);
+}
+
+// The second pass of the two-pass regroup write (k_cell_map, pf2_kernels.hpp)
+// for FP32 records: output positions in order (GU slices per pass, loads
+// before stores), a survivor's record copied (its cell is its destination),
+// a new particle created as k_cell_write32 creates it.  Every wavefront
+// stores whole, consecutive lines.
+template <int DIM, int DEG>
+__device__ __forceinline__ void gather32_spawn(const float* __restrict__ V32c, int* __restrict__ bcell,
+                                               uint32_t* __restrict__ bid, const Rec32& B, double* __restrict__ bvext,
+                                               long long cap, const DevParams& pp, const DevWork& w, long long o,
+                                               int code) {
+    constexpr int ND = QF<DIM, DEG>::ND;
+    const int c = code >> 6, lo = code & 63;
+    const unsigned long long fo = w.fs_off[c];
+    const int r = (int)(o - (long long)(fo >> 32)) - w.nsurv[c];
+    // velocity from the cell's FP32 copy of V, interpolated in FP64
+    double xi[DIM], Vc[ND][DIM], v[DIM];
+    lattice_point<DIM>(pp, lo, xi);
+#pragma unroll
+    for (int k = 0; k < ND; ++k)
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) Vc[k][d] = (double)V32c[((size_t)c * ND + k) * DIM + d];
+    interp<DIM, DEG>(Vc, xi, v);
+    float xf[DIM], vf[DIM];
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+        xf[d] = (float)xi[d];
+        vf[d] = (float)v[d];
+    }
+    bcell[o] = c;
+    bid[o] = *w.particle_counter + (uint32_t)fo + (uint32_t)r + 1u;
+    rec_store<DIM>(B, o, xf, vf);
+    if (pp.keep_vext)
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) bvext[(size_t)d * cap + o] = 0.0;
+}
+
+template <int DIM, int DEG = 1>
+__device__ __forceinline__ void cell_gather32_body(const Grid& G, const DevMesh<DIM>& m, const float* __restrict__ V32c,
+                                                   const uint32_t* __restrict__ aid, const Rec32& A,
+                                                   int* __restrict__ bcell, uint32_t* __restrict__ bid, const Rec32& B,
+                                                   const double* __restrict__ avext, double* __restrict__ bvext,
+                                                   long long cap, const DevParams& pp, const DevWork& w,
+                                                   const int* __restrict__ omap) {
+    if (state_dead(w.counters)) return;
+    const long long total = (long long)(w.fs_off[m.n_cells] >> 32);
+    const long long n = total < cap ? total : cap;
+    const long long bd = blockDim.x, span = (long long)G.nblk * bd * GU;
+    for (long long o0 = (long long)G.bid * bd * GU + threadIdx.x; o0 < n; o0 += span) {
+        int src[GU];
+        bool act[GU];
+#pragma unroll
+        for (int u = 0; u < GU; ++u) {
+            const long long o = o0 + u * bd;
+            act[u] = o < n;
+            src[u] = act[u] ? omap[o] : -1;
+        }
+        int pc[GU];
+        uint32_t pid[GU];
+        float4 ra[GU];
+        float2 rb[GU];
+        double pe[GU][DIM];
+#pragma unroll
+        for (int u = 0; u < GU; ++u) {
+            if (!act[u] || src[u] < 0) continue;
+            const int s = src[u];
+            pc[u] = w.dest[s];
+            pid[u] = aid[s];
+            ra[u] = A.a[s];
+            if constexpr (DIM == 3) rb[u] = A.b[s];
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) pe[u][d] = pp.keep_vext ? avext[(size_t)d * cap + s] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < GU; ++u) {
+            if (!act[u]) continue;
+            const long long o = o0 + u * bd;
+            if (src[u] < 0) {
+                gather32_spawn<DIM, DEG>(V32c, bcell, bid, B, bvext, cap, pp, w, o, -1 - src[u]);
+                continue;
+            }
+            bcell[o] = pc[u];
+            bid[o] = pid[u];
+            B.a[o] = ra[u];
+            if constexpr (DIM == 3) B.b[o] = rb[u];
+            if (pp.keep_vext)
+#pragma unroll
+                for (int d = 0; d < DIM; ++d) bvext[(size_t)d * cap + o] = pe[u][d];
+        }
+    }
+}
+
+template <int DIM, int DEG = 1>
+__global__ void __launch_bounds__(256) k_cell_gather32(DevMesh<DIM> m, const float* __restrict__ V32c,
+                                                       const uint32_t* __restrict__ aid, Rec32 A,
+                                                       int* __restrict__ bcell, uint32_t* __restrict__ bid, Rec32 B,
+                                                       const double* __restrict__ avext, double* __restrict__ bvext,
+                                                       long long cap, DevParams pp, DevWork w,
+                                                       const int* __restrict__ omap) {
+    cell_gather32_body<DIM, DEG>(hw_grid(), m, V32c, aid, A, bcell, bid, B, avext, bvext, cap, pp, w, omap);
 }
 
 // fuse = the projection sums are computed here (final counts per cell
@@ -805,8 +1031,14 @@ __global__ void __launch_bounds__(256) k_cell_write32(DevMesh<DIM> m, const floa
 template <int DIM>
 void launch_cell_write32(hipStream_t s, int grid, DevMesh<DIM> m, const float* V32c, const int* old_off, const uint32_t* aid,
                          Rec32 A, int* bcell, uint32_t* bid, Rec32 B, const double* avext, double* bvext, long long cap,
-                         DevParams pp, DevWork w, bool fuse, double* partial) {
-    if (fuse)
+                         DevParams pp, DevWork w, bool fuse, double* partial, int deg) {
+    if (fuse && deg == 2)
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cell_write32<DIM, true, 2>), dim3(grid), dim3(256), 0, s, m, V32c,
+                           old_off, aid, A, bcell, bid, B, avext, bvext, cap, pp, w, partial);
+    else if (deg == 2)
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cell_write32<DIM, false, 2>), dim3(grid), dim3(256), 0, s, m, V32c,
+                           old_off, aid, A, bcell, bid, B, avext, bvext, cap, pp, w, partial);
+    else if (fuse)
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cell_write32<DIM, true>), dim3(grid), dim3(256), 0, s, m, V32c, old_off,
                            aid, A, bcell, bid, B, avext, bvext, cap, pp, w, partial);
     else
@@ -816,20 +1048,26 @@ void launch_cell_write32(hipStream_t s, int grid, DevMesh<DIM> m, const float* V
 
 // per-cell projection sums in FP32 with Kahan compensation, ascending id order
 // (groups of lanes per cell as in k_cell_partials)
-template <int DIM>
+template <int DIM, int DEG = 1>
 __device__ __forceinline__ void partials32_body(const Grid& G, int n_cells, const int* __restrict__ off, const Rec32& R, double* __restrict__ partial, const int* __restrict__ counters) {
     if (state_dead(counters)) return;
-    constexpr int NV = Q1<DIM>::NV;
-    constexpr int NACC = PartialsShape<DIM>::NACC;
-    constexpr int PGS = PartialsShape<DIM>::PGS;
-    constexpr int NG = PartialsShape<DIM>::NG;
+    constexpr int NACC = PartialsShape<DIM, DEG>::NACC;
+    constexpr int PGS = PartialsShape<DIM, DEG>::PGS;
+    constexpr int NG = PartialsShape<DIM, DEG>::NG;
+    constexpr int NA = PartialsShape<DIM, DEG>::NA;
     __shared__ float s_xi[NG][PGS][DIM];
     __shared__ float s_v[NG][PGS][DIM];
     const int g = threadIdx.x / PGS, gl = threadIdx.x % PGS;
-    const int kk = gl / (DIM + 1);
-    const int comp = gl - kk * (DIM + 1);
-    float ft[DIM], fs[DIM];
-    shape_one32_coeffs<DIM>(kk, ft, fs);
+    // lane gl holds accumulators gl, gl + PGS, ... (a = k * (DIM + 1) + comp)
+    ShapeOne32<DIM, DEG> sh[NA];
+    int comp[NA];
+#pragma unroll
+    for (int u = 0; u < NA; ++u) {
+        const int a = gl + u * PGS;
+        const int kk = (a < NACC ? a : 0) / (DIM + 1);
+        comp[u] = a - kk * (DIM + 1);
+        sh[u].init(kk);
+    }
     for (int cb = G.bid * NG; cb < n_cells; cb += G.nblk * NG) {
         const int c = cb + g;
         const bool cv = c < n_cells;
@@ -837,7 +1075,9 @@ __device__ __forceinline__ void partials32_body(const Grid& G, int n_cells, cons
         int trips = (e - b + PGS - 1) / PGS;
 #pragma unroll
         for (int o = 1; o < WAVE; o <<= 1) trips = max(trips, __shfl_xor(trips, o));
-        float sum = 0.0f, cmp = 0.0f;
+        float sum[NA], cmp[NA];
+#pragma unroll
+        for (int u = 0; u < NA; ++u) sum[u] = cmp[u] = 0.0f;
         for (int t = 0; t < trips; ++t) {
             const int j0 = b + t * PGS;
             const int j = j0 + gl;
@@ -857,34 +1097,43 @@ __device__ __forceinline__ void partials32_body(const Grid& G, int n_cells, cons
                     float xi[DIM];
 #pragma unroll
                     for (int d = 0; d < DIM; ++d) xi[d] = s_xi[g][q][d];
-                    const float nk = shape_one32<DIM>(ft, fs, xi);
-                    float vc = 1.0f;   // the weight term (comp == DIM) is N_k itself
 #pragma unroll
-                    for (int d = 0; d < DIM; ++d)
-                        if (d == comp) vc = s_v[g][q][d];
-                    const float term = nk * vc;
-                    const float y = term - cmp;
-                    const float tt = sum + y;
-                    cmp = (tt - sum) - y;
-                    sum = tt;
+                    for (int u = 0; u < NA; ++u) {
+                        const float nk = sh[u](xi);
+                        float vc = 1.0f;   // the weight term (comp == DIM) is N_k itself
+#pragma unroll
+                        for (int d = 0; d < DIM; ++d)
+                            if (d == comp[u]) vc = s_v[g][q][d];
+                        const float term = nk * vc;
+                        const float y = term - cmp[u];
+                        const float tt = sum[u] + y;
+                        cmp[u] = (tt - sum[u]) - y;
+                        sum[u] = tt;
+                    }
                 }
             wave_sync();
         }
-        if (cv && gl < NACC) partial[(size_t)c * NACC + gl] = (double)sum - (double)cmp;
+#pragma unroll
+        for (int u = 0; u < NA; ++u)
+            if (cv && gl + u * PGS < NACC) partial[(size_t)c * NACC + gl + u * PGS] = (double)sum[u] - (double)cmp[u];
     }
 }
 
-template <int DIM>
+template <int DIM, int DEG = 1>
 __global__ void __launch_bounds__(256) k_cell_partials32(int n_cells, const int* __restrict__ off, Rec32 R,
                                                          double* __restrict__ partial, const int* __restrict__ counters) {
-    partials32_body<DIM>(hw_grid(), n_cells, off, R, partial, counters);
+    partials32_body<DIM, DEG>(hw_grid(), n_cells, off, R, partial, counters);
 }
 
 template <int DIM>
 void launch_partials32(hipStream_t s, int grid, int n_cells, const int* off, Rec32 R, double* partial,
-                       const int* counters) {
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cell_partials32<DIM>), dim3(grid), dim3(256), 0, s, n_cells, off, R, partial,
-                       counters);
+                       const int* counters, int deg = 1) {
+    if (deg == 2)
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cell_partials32<DIM, 2>), dim3(grid), dim3(256), 0, s, n_cells, off, R,
+                           partial, counters);
+    else
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cell_partials32<DIM>), dim3(grid), dim3(256), 0, s, n_cells, off, R,
+                           partial, counters);
 }
 
 template <int DIM>
@@ -940,8 +1189,7 @@ void launch_wall_particles32(hipStream_t s, DevMesh<DIM> m, DevWalls wl, const i
 template <int DIM>
 __global__ void k_pack_fields32(DevMesh<DIM> m, const double* __restrict__ V, const double* __restrict__ Vold,
                                 float* __restrict__ cellV32, float* __restrict__ cellDV32) {
-    constexpr int NV = Q1<DIM>::NV;
-    const long long n = (long long)m.n_cells * NV;
+    const long long n = (long long)m.n_cells * m.nd;
     for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x) {
         const int dof = m.cd[t];
 #pragma unroll

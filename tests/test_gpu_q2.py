@@ -8,7 +8,7 @@ import pytest
 
 from oracle_binding import Oracle
 from parity_util import assert_particles_match, bitwise_equal
-from pfem2_amd import FP32_MIXED, FP64, Pf2Error, ParticleStage
+from pfem2_amd import FP64, Pf2Error, ParticleStage
 from pfem2_amd import mesh as M
 
 pytestmark = pytest.mark.gpu
@@ -130,11 +130,9 @@ def test_q2_moving_mesh():
 
 
 def test_q2_limits():
-    """Degree 2 runs on FP64 handles without walls: the FP32-mixed path and
-    the wall fix-up (defined on Q1 vertex DoFs) refuse it."""
+    """Degree-2 handles take no walls: the wall fix-up is defined on Q1
+    vertex DoFs."""
     m = M.q2(M.quad_2d(3, 3))
-    with pytest.raises(Pf2Error, match="degree-2"):
-        ParticleStage(m, 0.01, 2, 3, [2, 2], FP32_MIXED)
     g = ParticleStage(m, 0.01, 2, 3, [2, 2], FP64)
     with pytest.raises(Pf2Error, match="Q1"):
         g.set_walls(np.array([0]), np.array([0]), np.zeros((1, 2)), np.array([3]))
