@@ -58,7 +58,7 @@ typedef struct pf2_mesh {
     const int* cell_dofs;       /* degree 1: NULL means the same as cell_vertices;
                                    degree 2: n_cells x 3^dim, required */
     int degree;                 /* velocity field degree: 1 (0 also means 1) or 2;
-                                   degree 2 runs on PF2_FP64 handles without walls */
+                                   degree-2 handles take no walls */
 } pf2_mesh;
 
 /* Compile-time knobs of the reference made run-time
@@ -277,6 +277,9 @@ typedef struct pf2_server_config {
     int use_graphs;                 /* replay each simulation's step as a HIP graph */
     int batched;                    /* step all active simulations of one dimension and precision
                                        together with pf2_batch_step (degree-1 fields) */
+    int batch_max_cells;            /* batched mode: only simulations of at most this many cells are
+                                       batched; larger ones replay their own step graphs on their own
+                                       streams (0: every simulation is batched) */
 } pf2_server_config;
 
 const char* pf2_server_last_error(void);

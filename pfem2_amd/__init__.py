@@ -83,7 +83,7 @@ class _LoadsParams(ctypes.Structure):
 class _ServerConfig(ctypes.Structure):
     _fields_ = [("hbm_budget", ctypes.c_ulonglong), ("max_active", ctypes.c_int),
                 ("steps_per_chunk", ctypes.c_int), ("chunks_ahead", ctypes.c_int), ("use_graphs", ctypes.c_int),
-                ("batched", ctypes.c_int)]
+                ("batched", ctypes.c_int), ("batch_max_cells", ctypes.c_int)]
 
 
 def load_library(path=LIB_PATH):
@@ -497,14 +497,16 @@ class Server:
     poll() admits, issues and retires without waiting for the GPU; result()
     returns the last projected field, the final particle count and the
     latency from submission to completion.  batched: all active simulations
-    of one dimension and precision advance together, one launch per kernel
-    of the step (pf2_batch_step); otherwise each replays its own step graph
-    on its own stream."""
+    of one dimension and precision (of at most batch_max_cells cells, when
+    given) advance together, one launch per kernel of the step
+    (pf2_batch_step); otherwise each replays its own step graph on its own
+    stream."""
 
-    def __init__(self, hbm_budget=0, max_active=0, steps_per_chunk=0, chunks_ahead=0, use_graphs=True, batched=True):
+    def __init__(self, hbm_budget=0, max_active=0, steps_per_chunk=0, chunks_ahead=0, use_graphs=True, batched=False,
+                 batch_max_cells=0):
         self._lib = load_library()
         cfg = _ServerConfig(hbm_budget, max_active, steps_per_chunk, chunks_ahead, 1 if use_graphs else 0,
-                            1 if batched else 0)
+                            1 if batched else 0, int(batch_max_cells))
         h = ctypes.c_void_p()
         self._check(self._lib.pf2_server_create(ctypes.byref(cfg), ctypes.byref(h)))
         self._s = h

@@ -270,7 +270,9 @@ int admit(pf2_server_s* s, Request* r) {
     return 0;
 }
 
-bool batched(const pf2_server_s* s, const Request* r) { return s->cfg.batched && r->degree == 1; }
+bool batched(const pf2_server_s* s, const Request* r) {
+    return s->cfg.batched && r->degree == 1 && (s->cfg.batch_max_cells <= 0 || r->n_cells <= s->cfg.batch_max_cells);
+}
 
 // the simulation's step k (1-based) is done: the mesh move of a moving request
 int after_step(Request* r, int k) {

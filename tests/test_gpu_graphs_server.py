@@ -126,18 +126,24 @@ def _requests():
     return out
 
 
-@pytest.mark.parametrize("batched", [True, False])
+def _server_kw(mode):
+    # batched: every request in batched steps; mixed: those of at most 1000
+    # cells batched, the others on their own streams
+    return {"batched": mode != "single", "batch_max_cells": 1000 if mode == "mixed" else 0}
+
+
+@pytest.mark.parametrize("batched", ["batched", "single", "mixed"])
 def test_server_alone_equals_mix_equals_direct(batched):
     """A request served alone, in a mix of five and run directly on a handle
     gives the same bits; batched: the mix advances with one launch per
-    kernel of the step for all active requests."""
+    kernel of the step for all active requests (mixed: the small ones)."""
     reqs = _requests()
     target = 4   # the cylinder request
     m, V, prm, steps = reqs[target]
     ref_V, ref_n = _direct(m, V, prm, steps)
 
     print("server alone", flush=True)
-    srv = Server(hbm_budget=4 << 30, steps_per_chunk=3, use_graphs=True, batched=batched)
+    srv = Server(hbm_budget=4 << 30, steps_per_chunk=3, use_graphs=True, **_server_kw(batched))
     rid = srv.submit(m, V, steps, *prm)
     srv.drain()
     V1, n1, lat1 = srv.result(rid)
@@ -148,7 +154,7 @@ def test_server_alone_equals_mix_equals_direct(batched):
     assert np.array_equal(V1, ref_V, equal_nan=True)
 
     print("server mix", flush=True)
-    srv = Server(hbm_budget=4 << 30, steps_per_chunk=2, chunks_ahead=2, use_graphs=True, batched=batched)
+    srv = Server(hbm_budget=4 << 30, steps_per_chunk=2, chunks_ahead=2, use_graphs=True, **_server_kw(batched))
     ids = []
     for k, (mk, Vk, pk, sk) in enumerate(reqs):
         ids.append(srv.submit(mk, Vk, sk, *pk))
