@@ -21,12 +21,14 @@ targets (met or not), the CPU oracle on a sample of the value's workload
 stream of the latency metric (256 independent simulations, Poisson
 arrivals, 2k-200k cells, 9/16/25 particles per cell, 50-500 steps, plus 8
 moving-mesh simulations of ~400k cells; admission by HBM footprint) with
-its p50/p99 completion latency under "serving".
+its p50/p99 completion latency under "serving" (each simulation replays its
+own step graphs on its own stream), and the same stream with batched steps
+under "serving_batched".
 
 Other workloads (--workload): cylinder (the 100k-cell channel alone),
 channel (200x20 quads, FP64, with a host FEM stand-in between particle
 steps, against the CPU oracle over the whole run), pipe3d (826k hexahedra
-in a pipe, 2x2x2 lattice), moving (the value's workload alone), serve (the
+in a pipe with a Q2 field, 2x2x2 lattice), moving (the value's workload alone), serve (the
 request stream alone).  Prints one JSON line.
 """
 import argparse
@@ -46,7 +48,7 @@ SEEDS = {"mesh": 2024, "field": 7, "particles": 11, "serve": 99, "pipe": 31, "mo
 HBM_PEAK_GBS = 8000.0
 # committed rocprofv3 --pmc summaries (profiles/hbm_traffic_<name>.txt) per workload and precision
 PROFILES = {("moving", 0): "moving400k_fp64", ("cylinder", 0): "cylinder100k_fp64",
-            ("cylinder", 1): "cylinder100k"}
+            ("cylinder", 1): "cylinder100k_fp32mixed"}
 DTYPE32 = "fp32-mixed (f32 reference coords/velocities, f64 relocation and node sums)"
 
 
@@ -81,10 +83,10 @@ def channel_workload():
 
 def pipe_workload():
     from pfem2_amd import mesh as M
-    m = M.pipe_3d(64, 64, 256)
+    m = M.q2(M.pipe_3d(64, 64, 256))
     V = M.pipe_flow(m, 1.0, 0.5, 0.01, seed=SEEDS["pipe"])
     return Workload("pipe3d", m, V, [2, 2, 2], 0.01, 10, 2,
-                    desc="3D pipe r=0.5 L=4, 64x64x256 hexahedra clipped to the circle, Poiseuille in z + 1% "
+                    desc="3D pipe r=0.5 L=4, 64x64x256 hexahedra clipped to the circle, Q2 Poiseuille field in z + 1% "
                          "Fourier perturbation, 2x2x2 lattice +-0.05 jitter, dt=0.01, 10 substeps, MAX=2")
 
 
@@ -158,6 +160,15 @@ def kernel_bytes(name, info, m, dim=2, vext=False):
         per_src = 4 + 4 + 4 + dim * wx * (3 if name == "k_cell_write" else 2) + (dim * 8 if vext else 0)
         per_dst = 4 + 4 + dim * wx * (3 if name == "k_cell_write" else 2) + (dim * 8 if vext else 0)
         return n1 * (per_src + per_dst) + nc * 32
+    if name == "k_cell_map":
+        # per-cell census results in, the output map and per-cell resets out
+        return nc * (36 + 12) + n1 * (4 + 4)
+    if name in ("k_cell_gather", "k_cell_gather32"):
+        # map and destination of each output position, the record copied in order
+        rec = dim * 8 * 3 if name == "k_cell_gather" else (16 if dim == 2 else 24)
+        per_src = 4 + 4 + 4 + rec + (dim * 8 if vext else 0)
+        per_dst = 4 + 4 + rec + (dim * 8 if vext else 0)
+        return n1 * (per_src + per_dst)
     if name in ("k_cell_partials", "k_cell_partials32"):
         wx = 8 if name == "k_cell_partials" else 4
         return n1 * dim * wx * 2 + nc * (4 + nv * (dim + 1) * 8)
@@ -421,6 +432,19 @@ def measured_traffic(kernel, profile):
     return None
 
 
+def measured_valu(kernel, profile):
+    """VALU busy percentage of `kernel` from the committed rocprofv3 --pmc
+    summary of the workload (profiles/valu_<profile>.txt), or None."""
+    path = os.path.join(ROOT, "profiles", f"valu_{profile}.txt")
+    if not profile or not os.path.exists(path):
+        return None
+    for line in open(path):
+        mm = re.match(r"\s*kernel=(\S+)\s+valu_busy_pct=([0-9.]+)", line)
+        if mm and mm.group(1) == kernel:
+            return float(mm.group(2))
+    return None
+
+
 def roofline(res, m, profile=None):
     """Roofline of the kernel with the largest share of the step.  The FP64
     relocation rounds are one kernel launched once per round: its bytes and
@@ -448,15 +472,16 @@ def roofline(res, m, profile=None):
                            "share_of_step": per_step[key] / sum(per_step.values())}
     return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS if achieved else None,
-            "traffic": measured_traffic(dom, profile),
+            "traffic": measured_traffic(dom, profile), "valu_busy_pct": measured_valu(dom, profile),
             "kernel": dom, "launches_per_step": len(members), "bytes_per_launch": b, "avg_launch_ms": dur * 1e3,
             "all_kernels": others}
 
 
-def serve(args, rate=None, batched=None):
+def serve(args, rate=None, batched=None, batch_max_cells=None):
     """The request stream: independent simulations with Poisson arrivals,
     served by pf2_server (FP32-mixed, admission by footprint; batched: every
-    step advances all active simulations with one launch per kernel).
+    step advances all active simulations with one launch per kernel, or,
+    with batch_max_cells, those of at most that many cells).
     Moving-mesh requests (~400k quads around a cylinder oscillating with a
     seeded phase, 25 particles per cell) arrive within the first half of the
     stream, so that the memory in use climbs toward the budget."""
@@ -476,7 +501,8 @@ def serve(args, rate=None, batched=None):
         dt = 0.5 * min(5.0 / nx, 1.0 / ny) / umax
         reqs.append({"m": m, "V": V, "steps": steps, "dt": dt, "q": q, "moving": None})
     rate = args.serve_rate if rate is None else rate
-    batched = (not args.serve_unbatched) if batched is None else batched
+    batched = args.serve_batched if batched is None else batched
+    batch_max_cells = args.serve_batch_max_cells if batch_max_cells is None else batch_max_cells
     arrivals = list(np.cumsum(rng.exponential(1.0 / rate, len(reqs))))
     if args.serve_moving > 0:
         W0 = moving_workload()
@@ -492,7 +518,8 @@ def serve(args, rate=None, batched=None):
     arrivals = [arrivals[k] for k in order]
     print(f"# serving {len(reqs)} requests ({args.serve_moving} moving) over ~{arrivals[-1]:.1f} s", file=sys.stderr,
           flush=True)
-    srv = Server(hbm_budget=int(args.serve_budget_gib * (1 << 30)), use_graphs=True, batched=batched)
+    srv = Server(hbm_budget=int(args.serve_budget_gib * (1 << 30)), use_graphs=True, batched=batched,
+                 batch_max_cells=batch_max_cells)
     ids = []
     t0 = time.perf_counter()
     i = 0
@@ -524,6 +551,7 @@ def serve(args, rate=None, batched=None):
            "particle_updates": st["particle_updates"], "wall_s": wall,
            "peak_hbm_gib": st["peak_bytes"] / 2 ** 30, "max_active": int(st["max_active"]),
            "arrival_rate_per_s": rate, "budget_gib": args.serve_budget_gib, "batched": bool(batched),
+           "batch_max_cells": int(batch_max_cells) if batched else None,
            "cells_mean": float(np.mean(cells)), "precision": "fp32-mixed",
            "stream": f"{args.serve_requests} static requests, Poisson arrivals ({rate}/s), cells "
                      "log-uniform 2k-200k (channel quads), lattice 3/4/5, Umax U[0.2,2], 50-500 steps; "
@@ -706,8 +734,10 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=10.0, help="Poisson arrival rate, requests/s")
     ap.add_argument("--serve-budget-gib", type=float, default=200.0)
     ap.add_argument("--serve-moving", type=int, default=8, help="moving-mesh requests mixed into the stream")
-    ap.add_argument("--serve-unbatched", action="store_true",
-                    help="serve with one step graph per simulation instead of batched steps")
+    ap.add_argument("--serve-batch-max-cells", type=int, default=0,
+                    help="batched serving: batch only simulations of at most this many cells (0: all)")
+    ap.add_argument("--serve-batched", action="store_true",
+                    help="serve with batched steps (pf2_batch_step) instead of one step graph per simulation")
     ap.add_argument("--serve-rates", default="5,10,20,40,80,160",
                     help="arrival rates of --workload serve-sweep (requests/s)")
     ap.add_argument("--channel-steps", type=int, default=100)
@@ -728,11 +758,13 @@ def main():
         # and per-simulation serving
         rows = []
         for rate in [float(x) for x in args.serve_rates.split(",")]:
-            for bt in (True, False):
-                r = serve(args, rate=rate, batched=bt)
-                print(f"# rate {rate}/s batched={bt}: p50 {r['p50_ms']:.1f} ms p99 {r['p99_ms']:.1f} ms "
+            for bt, bmax in ((False, 0), (True, 0), (True, 16384)):
+                r = serve(args, rate=rate, batched=bt, batch_max_cells=bmax)
+                print(f"# rate {rate}/s batched={bt} (max cells {bmax}): p50 {r['p50_ms']:.1f} ms "
+                      f"p99 {r['p99_ms']:.1f} ms "
                       f"{r['updates_per_s']:.3e} updates/s, max active {r['max_active']}", file=sys.stderr, flush=True)
-                rows.append({k: r[k] for k in ("arrival_rate_per_s", "batched", "p50_ms", "p99_ms", "updates_per_s",
+                rows.append({k: r[k] for k in ("arrival_rate_per_s", "batched", "batch_max_cells", "p50_ms", "p99_ms",
+                                               "updates_per_s",
                                                "max_active", "peak_hbm_gib", "finished", "failed", "wall_s")})
         out = {"metric": "p50/p99 sim latency against offered load", "value": None, "unit": "ms",
                "n_gpus": 1, "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": DTYPE32,
@@ -778,6 +810,10 @@ def main():
         out["serving"] = s
         out["sim_latency_p50_ms"] = s["p50_ms"]
         out["sim_latency_p99_ms"] = s["p99_ms"]
+        sb = serve(args, batched=True)
+        print(f"# serving, batched: p50 {sb['p50_ms']:.1f} ms p99 {sb['p99_ms']:.1f} ms, "
+              f"{sb['updates_per_s']:.3e} updates/s", file=sys.stderr, flush=True)
+        out["serving_batched"] = sb
     print(json.dumps(out))
 
 
