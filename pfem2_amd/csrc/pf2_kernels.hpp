@@ -142,6 +142,7 @@ struct DevMesh {
                            // FP32-mixed handles, whose particle kernels read FP32 copies
     const double* cellDV;  // per cell: V - V_old at its DoFs [ND][DIM] (FP64 handles)
     const double* Vn;      // V, node-major [n_dofs][DIM]
+    const unsigned char* cmoved;   // per cell: 1 if the last geometry update changed a vertex of it
 };
 
 struct DevParticles {
@@ -184,7 +185,8 @@ struct DevWork {
     int* nsurv;         // per cell: survivors
     double* partial;    // per cell: NV * (DIM + 1) projection sums
     int* counters;      // [0] n_particles, [1] arrivals of the last regroup, [2] overflow flags,
-                        // [3] count before the last regroup, [4] overflow entries in use
+                        // [3] count before the last regroup, [4] overflow entries in use,
+                        // [5] id counter at the last advection (ids up to it hold an inverse-mapped xi)
     int* big;           // [0] number of cells listed for k_cell_big, then the cells
     uint32_t* particle_counter;
     unsigned long long* stats;   // STAT_SHARDS rows: lost, deleted, spawned, searches, zero-weight
@@ -792,6 +794,9 @@ __device__ __forceinline__ void move_body(const Grid& G, const DevMesh<DIM>& m, 
     constexpr int ND = QF<DIM, DEG>::ND;
     const int n = state_dead(w.counters) ? 0 : w.counters[0];
     const int lane = threadIdx.x & (WAVE - 1);
+    // every particle that exists now ends this advection with xi from an
+    // inverse map (the re-sort after a mesh move relies on it)
+    if (do_move && G.bid == 0 && threadIdx.x == 0) w.counters[5] = (int)*w.particle_counter;
     const WaveLoop L = stride_loop(n, G);
     for (int base = L.begin; base < L.end; base += L.step) {
         const int i = base + lane;
@@ -1065,9 +1070,11 @@ __device__ __forceinline__ void cell_count_body(const Grid& G, int n_cells, cons
         const int nvr = (nv + 3) & ~3;
         wave_sync();
         // rank = particles of the same part with a smaller id, needed only
-        // in a part that holds more than MAX particles: the parts' counts
-        // first, then a scan of the row for the particles of such parts
-        const bool counted = small && reseed && by_count;
+        // in a part that holds more than MAX particles: in rows of more than
+        // 32 candidates the parts' counts first, then a scan of the row for
+        // the particles of such parts (in shorter rows a scan for every
+        // candidate costs less than the counting)
+        const bool counted = small && reseed && by_count && nv > 32;
         if (counted) {
             for (int j = gl; j < nv; j += GS) {
                 const uint32_t key = (uint32_t)(s_kid[g][j] >> 32) & 0xffffu;
@@ -2010,12 +2017,18 @@ __global__ void k_init_particles(DevMesh<DIM> m, const double* __restrict__ V, D
 }
 
 // re-sort after a mesh move: own-cell test and relocation with the stored
-// last advection velocity, no advection
+// last advection velocity, no advection.  skip_static: the geometry changed
+// once since the last re-sort, so a particle whose cell kept its vertices
+// and whose xi came from an inverse map in that cell (ids up to the counter
+// of the last advection) keeps its xi and cell: the inverse map would
+// return the same bits
 template <int DIM>
-__global__ void __launch_bounds__(256) k_resort(DevMesh<DIM> m, DevParticles A, DevParams pp, DevWork w) {
+__global__ void __launch_bounds__(256) k_resort(DevMesh<DIM> m, DevParticles A, DevParams pp, DevWork w,
+                                                int skip_static) {
     constexpr int NV = Q1<DIM>::NV;
     const int n = state_dead(w.counters) ? 0 : w.counters[0];
     const int lane = threadIdx.x & (WAVE - 1);
+    const uint32_t id_exact = (uint32_t)w.counters[5];
     int lost = 0, searches = 0;
     for (int base = blockIdx.x * blockDim.x + (threadIdx.x - lane); base < n; base += gridDim.x * blockDim.x) {
         const int i = base + lane;
@@ -2023,31 +2036,36 @@ __global__ void __launch_bounds__(256) k_resort(DevMesh<DIM> m, DevParticles A, 
         const int c0 = valid ? A.cell[i] : -1;
         int c = c0;
         double x[DIM], xi[DIM], vext[DIM], r[DIM], P[NV][DIM];
-        if (valid) {
+        if (valid && skip_static && !m.cmoved[c0] && A.id[i] <= id_exact) {
 #pragma unroll
-        for (int d = 0; d < DIM; ++d) {
-            xi[d] = A.xi[(size_t)d * A.cap + i];
-            x[d] = A.x[(size_t)d * A.cap + i];
-            vext[d] = A.vext ? A.vext[(size_t)d * A.cap + i] : 0.0;
-        }
-        load_vertices<DIM>(m, c, P);
-        if (inverse_map<DIM>(P, x, r, m.ftol[c], false) && inside_unit<DIM>(r)) {
+            for (int d = 0; d < DIM; ++d) xi[d] = A.xi[(size_t)d * A.cap + i];
+            w.dest[i] = c0;
+            w.pkey[i] = part_key<DIM>(pp, xi);
+        } else if (valid) {
 #pragma unroll
-            for (int d = 0; d < DIM; ++d) xi[d] = r[d];
-        } else {
-            ++searches;
-            double Q[NV][DIM];
-            NoPrefetch pre;
-            c = relocate<DIM>(m, c, P, xi, vext, x, r, Q, pre);
-            if (c < 0) ++lost;
-            else
+            for (int d = 0; d < DIM; ++d) {
+                xi[d] = A.xi[(size_t)d * A.cap + i];
+                x[d] = A.x[(size_t)d * A.cap + i];
+                vext[d] = A.vext ? A.vext[(size_t)d * A.cap + i] : 0.0;
+            }
+            load_vertices<DIM>(m, c, P);
+            if (inverse_map<DIM>(P, x, r, m.ftol[c], false) && inside_unit<DIM>(r)) {
 #pragma unroll
                 for (int d = 0; d < DIM; ++d) xi[d] = r[d];
-        }
+            } else {
+                ++searches;
+                double Q[NV][DIM];
+                NoPrefetch pre;
+                c = relocate<DIM>(m, c, P, xi, vext, x, r, Q, pre);
+                if (c < 0) ++lost;
+                else
 #pragma unroll
-        for (int d = 0; d < DIM; ++d) A.xi[(size_t)d * A.cap + i] = xi[d];
-        w.dest[i] = c;
-        if (c >= 0) w.pkey[i] = part_key<DIM>(pp, xi);
+                    for (int d = 0; d < DIM; ++d) xi[d] = r[d];
+            }
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) A.xi[(size_t)d * A.cap + i] = xi[d];
+            w.dest[i] = c;
+            if (c >= 0) w.pkey[i] = part_key<DIM>(pp, xi);
         }
         add_arrival(w, valid && c >= 0 && c != c0, c, i);
     }

@@ -141,6 +141,8 @@ struct pf2_handle_s {
     // work
     int *dest = nullptr, *pkey = nullptr, *spos = nullptr;
     int* omap = nullptr;   // regroup output map (k_cell_map -> k_cell_gather)
+    unsigned char* dcmoved = nullptr;   // per cell: the last geometry update changed it
+    int geo_updates = 0;                // geometry updates since the last re-sort
     int *arr_cnt = nullptr, *arr_slot = nullptr, *ovf_head = nullptr;
     int2* ovf = nullptr;
     int aslots = 0;
@@ -289,6 +291,7 @@ DevMesh<DIM> dev_mesh(pf2_handle_s* h) {
     m.cellV = h->dcellV;     // null for FP32-mixed handles
     m.cellDV = h->dcellDV;
     m.Vn = h->dV;
+    m.cmoved = h->dcmoved;
     return m;
 }
 
@@ -507,7 +510,7 @@ int regroup(pf2_handle_s* h, int reseed, bool fuse_sums = false, bool defer_fina
     }
     if (mixed(h) && !fp64_state && two_pass) {
         Timed t(h, K_CELL_GATHER32);
-        const dim3 g(grid_for(h->n_est, 256, 8192));
+        const dim3 g(grid_for(h->n_est, 256 * GU, 65535));   // one pass of GU positions per thread
         const PBufs &a = h->buf[h->cur], &b = h->buf[1 - h->cur];
         if (h->deg == 2)
             hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cell_gather32<DIM, 2>), g, dim3(256), 0, h->stream, dev_mesh<DIM>(h),
@@ -523,7 +526,7 @@ int regroup(pf2_handle_s* h, int reseed, bool fuse_sums = false, bool defer_fina
                                  w, fuse_sums, h->partial, h->deg);
     } else if (two_pass) {
         Timed t(h, K_CELL_GATHER);
-        const dim3 g(grid_for(h->n_est, 256, 8192));
+        const dim3 g(grid_for(h->n_est, 256 * GU, 65535));   // one pass of GU positions per thread
         DevParticles a = dev_particles(h, h->cur), b = dev_particles(h, 1 - h->cur);
         // (an FP32-mixed handle's FP64 regroup gathers V through the DoF map)
         if (!h->dcellV && h->deg == 2)
@@ -822,6 +825,7 @@ size_t estimate_bytes(int dim, long long nc, long long nvt, long long nd, long l
     const long long ndc = nc * (deg == 2 ? (dim == 2 ? 9 : 27) : nv);   // velocity DoFs of all cells
     size_t b = 0;
     b += nvt * dim * 8 + nck * 4 + ndc * 4 + (nvt + 1) * 4 + nck * 4 + nck * dim * 8;  // X, cv, cd, star
+    b += nc;                                                                             // moved-cell flags
     b += nc * 16 * (dim - 1) + nvt * 5;                                                  // face neighbours, vertex flags
     b += (nd + 1) * 4 + ndc * 4 + nc * 8 + nck * dim * 8 + nck * 8;                    // dof map, ftol, cellX/star
     if (p->precision != PF2_FP32_MIXED) b += ndc * dim * 8 * 2;                          // FP64 cell V / dV copies
@@ -1245,7 +1249,7 @@ int pf2_create_in(const pf2_mesh* mesh, const pf2_params* params, long long capa
         dalloc(&h->dstar_off, nvt + 1) || dalloc(&h->dstar_cell, h->star_cell.size()) ||
         dalloc(&h->dstar_dir, h->star_dir.size()) || dalloc(&h->ddof_off, nd + 1) ||
         dalloc(&h->ddof_cell, h->dof_cell.size()) || dalloc(&h->dftol, nc) || dalloc(&h->dcellX, nck * dim) || dalloc(&h->dcell_star, nck) ||
-        dalloc(&h->dcell_nbr, (size_t)h->n_cells * (h->dim - 1)) || dalloc(&h->dvbnd, nvt) ||
+        dalloc(&h->dcell_nbr, (size_t)h->n_cells * (h->dim - 1)) || dalloc(&h->dvbnd, nvt) || dalloc(&h->dcmoved, nc) ||
         dalloc(&h->dvdof, nvt) ||
         (!mixed(h) && (dalloc(&h->dcellV, ndc * dim) || dalloc(&h->dcellDV, ndc * dim))) ||
         dalloc(&h->dV, (size_t)nd * dim) ||
@@ -1297,8 +1301,8 @@ int pf2_create_in(const pf2_mesh* mesh, const pf2_params* params, long long capa
         hipSuccess)
         return bail(fail(PF2_E_HIP, "scan sizing failed"));
     if (dalloc((char**)&h->scan_tmp, h->scan_tmp_bytes)) return bail(PF2_E_HIP);
-    if (h->dim == 2) launch_geo<2>(h->stream, dev_mesh<2>(h), h->dgeo32, h->dftol, h->dcellX);
-    else launch_geo<3>(h->stream, dev_mesh<3>(h), h->dgeo32, h->dftol, h->dcellX);
+    if (h->dim == 2) launch_geo<2>(h->stream, dev_mesh<2>(h), h->dgeo32, h->dftol, h->dcellX, h->dcmoved);
+    else launch_geo<3>(h->stream, dev_mesh<3>(h), h->dgeo32, h->dftol, h->dcellX, h->dcmoved);
     if (pack_fields(h)) return bail(PF2_E_HIP);
     if (hipStreamSynchronize(h->stream) != hipSuccess) return bail(fail(PF2_E_HIP, "initialisation sync failed"));
     h->bytes = g_alloc_bytes - bytes0;
@@ -1313,7 +1317,7 @@ int pf2_destroy(pf2_handle h) {
     free_particles(h);
     void* ptrs[] = {h->dX, h->dcv, h->dcd, h->dstar_off, h->dstar_cell, h->dstar_dir, h->ddof_off, h->ddof_cell,
                     h->dftol, h->dcellX, h->dcell_star, h->dcell_nbr, h->dvbnd, h->dvdof, h->dcellV, h->dcellDV, h->dV, h->dVold, h->dtmp, h->dV32,
-                    h->dX0, h->dW,
+                    h->dX0, h->dW, h->dcmoved,
                     h->dVold32, h->dgeo32, h->off[0], h->off[1], h->arr_cnt, h->arr_slot, h->ovf_head, h->fs_cnt,
                     h->fs_off, h->hitmask, h->nsurv, h->partial, h->counters, h->big, h->qcount, h->qprefix,
                     h->pcounter, h->stats, h->scan_tmp, h->scan_tiles};
@@ -1735,12 +1739,15 @@ int pf2_sort_particles_into_subdomains_and_cells(pf2_handle h) {
     DevWork w = dev_work(h);
     {
         Timed t(h, K_RESORT);
+        // particles of unchanged cells keep their xi when one geometry update separates this re-sort from the last
+        const int skip = h->geo_updates == 1;
+        h->geo_updates = 0;
         if (h->dim == 2)
             hipLaunchKernelGGL(HIP_KERNEL_NAME(k_resort<2>), dim3(grid_for(h->n_est, 256, 4096)), dim3(256), 0,
-                               h->stream, dev_mesh<2>(h), dev_particles(h, h->cur), pp, w);
+                               h->stream, dev_mesh<2>(h), dev_particles(h, h->cur), pp, w, skip);
         else
             hipLaunchKernelGGL(HIP_KERNEL_NAME(k_resort<3>), dim3(grid_for(h->n_est, 256, 4096)), dim3(256), 0,
-                               h->stream, dev_mesh<3>(h), dev_particles(h, h->cur), pp, w);
+                               h->stream, dev_mesh<3>(h), dev_particles(h, h->cur), pp, w, skip);
     }
     HIPC(hipGetLastError());
     // regroup the FP64 copy, then convert back
@@ -1785,8 +1792,9 @@ int pf2_update_vertices(pf2_handle h, const double* vertices, int recompute_maps
         compute_star_dirs(h);
         if (int r = upload_star_dirs(h)) return r;
     }
-    if (h->dim == 2) launch_geo<2>(h->stream, dev_mesh<2>(h), h->dgeo32, h->dftol, h->dcellX);
-    else launch_geo<3>(h->stream, dev_mesh<3>(h), h->dgeo32, h->dftol, h->dcellX);
+    if (h->dim == 2) launch_geo<2>(h->stream, dev_mesh<2>(h), h->dgeo32, h->dftol, h->dcellX, h->dcmoved);
+    else launch_geo<3>(h->stream, dev_mesh<3>(h), h->dgeo32, h->dftol, h->dcellX, h->dcmoved);
+    h->geo_updates += 1;
     HIPC(hipGetLastError());
     HIPC(hipStreamSynchronize(h->stream));
     return 0;
@@ -1826,26 +1834,30 @@ int move_mesh_impl(pf2_handle_s* h, const double* vertices, int recompute_maps, 
         compute_star_dirs(h);
         if (int r = upload_star_dirs(h)) return r;
     }
-    if (h->dim == 2) launch_geo<2>(h->stream, dev_mesh<2>(h), h->dgeo32, h->dftol, h->dcellX);
-    else launch_geo<3>(h->stream, dev_mesh<3>(h), h->dgeo32, h->dftol, h->dcellX);
+    if (h->dim == 2) launch_geo<2>(h->stream, dev_mesh<2>(h), h->dgeo32, h->dftol, h->dcellX, h->dcmoved);
+    else launch_geo<3>(h->stream, dev_mesh<3>(h), h->dgeo32, h->dftol, h->dcellX, h->dcmoved);
+    h->geo_updates += 1;
     DevParams pp = dev_params(h);
     DevWork w = dev_work(h);
     {
         Timed t(h, mixed(h) ? K_RESORT32 : K_RESORT);
+        // particles of unchanged cells keep their xi when this is the only geometry update since the last re-sort
+        const int skip = h->geo_updates == 1;
+        h->geo_updates = 0;
         if (mixed(h)) {
             h->fp64_valid = false;
             if (h->dim == 2)
                 hipLaunchKernelGGL(HIP_KERNEL_NAME(k_resort32<2>), dim3(grid), dim3(256), 0, h->stream,
-                                   dev_mesh<2>(h), b.cell, b.x, b.vext, rec32(b), h->cap, pp, w);
+                                   dev_mesh<2>(h), b.cell, b.id, b.x, b.vext, rec32(b), h->cap, pp, w, skip);
             else
                 hipLaunchKernelGGL(HIP_KERNEL_NAME(k_resort32<3>), dim3(grid), dim3(256), 0, h->stream,
-                                   dev_mesh<3>(h), b.cell, b.x, b.vext, rec32(b), h->cap, pp, w);
+                                   dev_mesh<3>(h), b.cell, b.id, b.x, b.vext, rec32(b), h->cap, pp, w, skip);
         } else if (h->dim == 2)
             hipLaunchKernelGGL(HIP_KERNEL_NAME(k_resort<2>), dim3(grid), dim3(256), 0, h->stream, dev_mesh<2>(h),
-                               dev_particles(h, h->cur), pp, w);
+                               dev_particles(h, h->cur), pp, w, skip);
         else
             hipLaunchKernelGGL(HIP_KERNEL_NAME(k_resort<3>), dim3(grid), dim3(256), 0, h->stream, dev_mesh<3>(h),
-                               dev_particles(h, h->cur), pp, w);
+                               dev_particles(h, h->cur), pp, w, skip);
     }
     HIPC(hipGetLastError());
     // relocation and reseed in one census: the same survivors, ids and

@@ -131,24 +131,34 @@ __global__ void k_geo32(DevMesh<DIM> m, float* __restrict__ geo) {
     }
 }
 
-// per-cell FP64 tolerance of the clearly-outside test
+// per-cell FP64 tolerance of the clearly-outside test, the packed vertex
+// copy, and whether the copy changed (cmoved: any coordinate differs from
+// the one it replaces)
 template <int DIM>
-__global__ void k_ftol(DevMesh<DIM> m, double* __restrict__ ftol, double* __restrict__ cellX) {
+__global__ void k_ftol(DevMesh<DIM> m, double* __restrict__ ftol, double* __restrict__ cellX,
+                       unsigned char* __restrict__ cmoved) {
     for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < m.n_cells; c += gridDim.x * blockDim.x) {
         double P[Q1<DIM>::NV][DIM];
         gather_vertices<DIM>(m, c, P);
         ftol[c] = outside_tol<DIM>(P);
+        bool moved = false;
 #pragma unroll
         for (int k = 0; k < Q1<DIM>::NV; ++k)
 #pragma unroll
-            for (int d = 0; d < DIM; ++d) cellX[((size_t)c * Q1<DIM>::NV + k) * DIM + d] = P[k][d];
+            for (int d = 0; d < DIM; ++d) {
+                double* q = cellX + ((size_t)c * Q1<DIM>::NV + k) * DIM + d;
+                moved = moved || *q != P[k][d];
+                *q = P[k][d];
+            }
+        cmoved[c] = moved ? 1 : 0;
     }
 }
 
 template <int DIM>
-void launch_geo(hipStream_t s, DevMesh<DIM> m, float* geo, double* ftol, double* cellX) {
+void launch_geo(hipStream_t s, DevMesh<DIM> m, float* geo, double* ftol, double* cellX, unsigned char* cmoved) {
     hipLaunchKernelGGL(HIP_KERNEL_NAME(k_geo32<DIM>), dim3((m.n_cells + 255) / 256), dim3(256), 0, s, m, geo);
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_ftol<DIM>), dim3((m.n_cells + 255) / 256), dim3(256), 0, s, m, ftol, cellX);
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_ftol<DIM>), dim3((m.n_cells + 255) / 256), dim3(256), 0, s, m, ftol, cellX,
+                       cmoved);
 }
 
 template <int DIM>
@@ -533,6 +543,9 @@ __device__ __forceinline__ void move32_body(const Grid& G, const DevMesh<DIM>& m
     const int n = state_dead(w.counters) ? 0 : w.counters[0];
     const float h32 = (float)pp.h;
     const int lane = threadIdx.x & (WAVE - 1);
+    // the particles that exist now end this advection with xi from the
+    // local step or an inverse map in their cell (see k_resort32)
+    if (do_move && G.bid == 0 && threadIdx.x == 0) w.counters[5] = (int)*w.particle_counter;
     const WaveLoop L = stride_loop(n, G);
     for (int base = L.begin; base < L.end; base += L.step) {
         const int i = base + lane;
@@ -1280,21 +1293,31 @@ __global__ void k_pos32(DevMesh<DIM> m, const int* __restrict__ pcell, Rec32 R, 
 }
 
 // k_resort for the FP32 state: own-cell test and relocation of the fixed
-// location x on the moved mesh in FP64, reference point stored in FP32
+// location x on the moved mesh in FP64, reference point stored in FP32.
+// skip_static (as in k_resort): a particle of a cell that kept its vertices,
+// advected in it since (ids up to the counter of the last advection), keeps
+// its reference point and cell
 template <int DIM>
 __global__ void __launch_bounds__(256) k_resort32(DevMesh<DIM> m, const int* __restrict__ pcell,
-                                                  const double* __restrict__ px, const double* __restrict__ pvext,
-                                                  Rec32 R, long long cap, DevParams pp, DevWork w) {
+                                                  const uint32_t* __restrict__ pid, const double* __restrict__ px,
+                                                  const double* __restrict__ pvext, Rec32 R, long long cap,
+                                                  DevParams pp, DevWork w, int skip_static) {
     constexpr int NV = Q1<DIM>::NV;
     const int n = state_dead(w.counters) ? 0 : w.counters[0];
     const int lane = threadIdx.x & (WAVE - 1);
+    const uint32_t id_exact = (uint32_t)w.counters[5];
     int lost = 0, searches = 0;
     for (int base = blockIdx.x * blockDim.x + (threadIdx.x - lane); base < n; base += gridDim.x * blockDim.x) {
         const int i = base + lane;
         const bool valid = i < n;
         const int c0 = valid ? pcell[i] : -1;
         int c = c0;
-        if (valid) {
+        if (valid && skip_static && !m.cmoved[c0] && pid[i] <= id_exact) {
+            float xf[DIM];
+            rec_load_xi<DIM>(R, i, xf);
+            w.dest[i] = c0;
+            w.pkey[i] = part_key32<DIM>(pp, xf);
+        } else if (valid) {
             double x[DIM], xi[DIM], vext[DIM], r[DIM], P[NV][DIM];
             float xf[DIM];
             rec_load_xi<DIM>(R, i, xf);

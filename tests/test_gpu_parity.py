@@ -253,6 +253,47 @@ def test_moving_mesh_resort(fused):
         assert sg["lost"] == so["lost"] and sg["spawned"] == so["spawned"] and sg["deleted"] == so["deleted"]
 
 
+@pytest.mark.parametrize("case", ["insert_then_move", "two_updates", "reseed_then_move"])
+def test_moving_mesh_resort_keeps_only_exact_xi(case):
+    """The re-sort keeps the reference coordinates of particles whose cell
+    kept its vertices only when they came from an inverse map in that cell
+    under the current geometry.  Not so: particles inserted and moved before
+    any advection, any particle after two geometry updates without a re-sort
+    between them, particles reseeded after the last advection.  All three
+    stay bitwise equal to the oracle, which recomputes every particle."""
+    m = M.cylinder_channel(0.08, seed=34)
+    V = M.cylinder_flow(m, 1.0, eps=0.02, seed=35)
+    g, o = _pair(m, V, 0.004, 5, 3, [3, 3], seed=36, keep_vext=True)
+    X0 = m.vertices.copy()
+    d = np.hypot(X0[:, 0] - 0.2, X0[:, 1] - 0.2)
+    w = np.clip(1.0 - (d - 0.05) / 0.15, 0.0, 1.0)
+    X1, X2 = X0.copy(), X0.copy()
+    X1[:, 1] += 0.004 * w
+    X2[:, 1] -= 0.003 * w
+    if case == "insert_then_move":
+        g.move_mesh(X1)
+        o.update_vertices(X1)
+    else:
+        g.step()
+        o.step()
+        if case == "two_updates":
+            g.update_vertices(X1)
+            g.update_vertices(X2)
+            g.sort_particles_into_subdomains_and_cells()
+            o.update_vertices(X1)
+            o.update_vertices(X2)
+        else:
+            g.check_empty_cells()
+            o.check_empty_cells()
+            g.move_mesh(X1)
+            o.update_vertices(X1)
+    o.sort_particles_into_subdomains_and_cells()
+    if case != "two_updates":
+        o.check_empty_cells()
+    assert_particles_match(g.particles(), o.particles(), REL, f"re-sort {case}", bitwise=True)
+    assert np.array_equal(g.cell_counts(), o.cell_counts())
+
+
 @pytest.mark.parametrize("q", [[4, 4], [5, 5]])
 def test_crowded_cells_bitwise(q):
     """Cells holding more candidates than the per-group LDS rows (64 for up
