@@ -571,14 +571,28 @@ __device__ __forceinline__ int relocate(const DevMesh<DIM>& m, int c, const doub
 // TurekBenchmark/pfem2particle.cpp:286-302).  A particle whose own-cell test
 // fails is parked (position advanced, old reference point, velocity_ext and
 // substep index stored) and queued; the vertex-star searches of all queued
-// particles then run in full wavefronts (k_search_resume), which continue
-// the remaining substeps and queue again if needed.  After MOVE_ROUNDS
-// rounds the last kernel finishes every remaining particle inline.
+// particles then run the vertex-star search in full wavefronts, which
+// continue the remaining substeps.
+//   FP64 path: one search round (k_search_resume) whose lanes follow their
+//   particle through every further crossing and refill from the queue when
+//   it is done (search_body).
+//   FP32-mixed path: MOVE_ROUNDS parking rounds (k_search_resume32 parks a
+//   particle that leaves its new cell again), then a last round that loops
+//   every remaining particle to its end.
 #ifndef PF2_MOVE_ROUNDS
 #define PF2_MOVE_ROUNDS 1
 #endif
 constexpr int MOVE_ROUNDS = PF2_MOVE_ROUNDS;
 static_assert(MOVE_ROUNDS >= 1 && MOVE_ROUNDS <= 2, "one or two parking rounds");
+// the queue segments of the refilling search (one cursor each, on its own
+// cache line: a single cursor for the whole grid serialises the wavefronts'
+// atomics) and the idle lanes a wavefront gathers before it refills
+constexpr int RSEGS = 256;
+constexpr int REFILL_MIN = 16;
+// the round whose shard counters hold the cursors: after the FP32-mixed
+// path's last round, so no search pushes to it; the counters are cleared
+// with the other queue counters after each regroup
+constexpr int CURSOR_ROUND = MOVE_ROUNDS + 1;
 
 // Queues of parked particles, one per round, ping-ponging between two
 // buffers.  A queue is QSHARDS shards of shard_cap entries plus an overflow
@@ -839,9 +853,16 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DIM ==
     move_body<DIM, DEG>(hw_grid(), m, V, Vold, A, pp, w, qu, do_correct, do_move);
 }
 
-// k_search_resume: vertex-star search of the particles queued in round
-// `round`, then the remaining substeps; a particle that needs another search
-// is queued for round + 1, or, in the last round, searched inline.
+// k_search_resume: the vertex-star search of the parked particles, then
+// their remaining substeps, in one round.  A lane follows its particle
+// through every further crossing and, when the particle is done, refills
+// from the queue: the wavefront's segment of the queue has a cursor (one
+// returning atomic per wavefront and refill, taken once REFILL_MIN lanes are
+// idle or all are), so a wavefront does not wait for its longest crossing
+// chain before it takes new particles, and no particle is parked twice.
+// Every iteration (one search, then the substeps up to the next exit)
+// consumes a substep, so each lane ends, and a wavefront leaves when its
+// segment is drained and none of its lanes holds a particle.
 template <int DIM, int DEG = 1>
 __device__ __forceinline__ void search_body(const Grid& G, const DevMesh<DIM>& m, const double* __restrict__ V,
                                             const DevParticles& A, const DevParams& pp, const DevWork& w,
@@ -849,40 +870,62 @@ __device__ __forceinline__ void search_body(const Grid& G, const DevMesh<DIM>& m
     constexpr int NV = Q1<DIM>::NV;
     __shared__ int s_pref[QSHARDS + 2];
     const int n = queue_prefix(qu, round, s_pref, G);
-    const bool last = round == MOVE_ROUNDS;
+    // the wavefront's segment of the queue and its cursor: as many segments
+    // as the grid has wavefronts, up to RSEGS, so every segment has one
+    const unsigned wpb = blockDim.x / WAVE, nwv = G.nblk * wpb;
+    const unsigned gw = G.bid * wpb + (unsigned)__builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
+    const int nseg = (int)(nwv < (unsigned)RSEGS ? nwv : (unsigned)RSEGS);
+    const int seg = (int)(gw % (unsigned)nseg);
+    const int s0 = (int)((long long)n * seg / nseg), s1 = (int)((long long)n * (seg + 1) / nseg);
+    int* cursor = queue_count(qu, CURSOR_ROUND, seg);
     const int lane = threadIdx.x & (WAVE - 1);
-    const WaveLoop L = range_loop(n, G);
+    const unsigned long long below = (1ull << lane) - 1;
     int lost = 0, searches = 0;
-    for (int base = L.begin; base < L.end; base += L.step) {
-        const int j = base + lane;
-        const bool valid = j < L.end;
-        const int2 e = valid ? queue_get(qu, round, s_pref, j) : make_int2(-1, -1);
-        const int i = e.x;
-        int c = e.y;
-        int s = valid ? qu.park_s[i] : 0;
-        int st = 0;
-        double x[DIM], xi[DIM], vext[DIM];
-        if (valid) {
+    bool have = false, more = true;
+    int i = 0, c = 0, s = 0;
+    double x[DIM], xi[DIM], vext[DIM], P[NV][DIM];
 #pragma unroll
-            for (int d = 0; d < DIM; ++d) {
-                x[d] = A.x[(size_t)d * A.cap + i];
-                xi[d] = A.xi[(size_t)d * A.cap + i];
-                vext[d] = qu.park_vext[(size_t)d * A.cap + i];
-            }
-            double P[NV][DIM];
-            load_vertices<DIM>(m, c, P);
-            for (;;) {
-                double Q[NV][DIM], r[DIM];
-                ++searches;
-                PrefetchV64<DIM, DEG> pre{m.cellV};
-                // (3-D: the shortcut's registers push this kernel into heavy spilling)
-                const int cn = relocate<DIM, PrefetchV64<DIM, DEG>, DIM == 2>(m, c, P, xi, vext, x, r, Q, pre);
-                if (cn < 0) {
-                    c = -1;
-                    ++lost;
-                    st = 0;
-                    break;
+    for (int d = 0; d < DIM; ++d) x[d] = xi[d] = vext[d] = 0.0;
+    for (;;) {
+        if (more) {   // wavefront-uniform
+            const unsigned long long need = __ballot(!have);
+            if (need && (__popcll(need) >= REFILL_MIN || need == ~0ull)) {
+                const int k = __popcll(need);
+                const int leader = __ffsll((long long)need) - 1;
+                int base = 0;
+                if (lane == leader) base = atomicAdd(cursor, k);
+                base = s0 + __shfl(base, leader);
+                more = base + k < s1;
+                const int j = base + __popcll(need & below);
+                if (!have && j < s1) {
+                    const int2 e = queue_get(qu, round, s_pref, j);
+                    i = e.x;
+                    c = e.y;
+                    s = qu.park_s[i];
+#pragma unroll
+                    for (int d = 0; d < DIM; ++d) {
+                        x[d] = A.x[(size_t)d * A.cap + i];
+                        xi[d] = A.xi[(size_t)d * A.cap + i];
+                        vext[d] = qu.park_vext[(size_t)d * A.cap + i];
+                    }
+                    load_vertices<DIM>(m, c, P);
+                    have = true;
                 }
+            }
+        }
+        if (__ballot(have) == 0) break;
+        bool done = false;
+        if (have) {
+            double Q[NV][DIM], r[DIM];
+            ++searches;
+            PrefetchV64<DIM, DEG> pre{m.cellV};
+            // (3-D: the shortcut's registers push this kernel into heavy spilling)
+            const int cn = relocate<DIM, PrefetchV64<DIM, DEG>, DIM == 2>(m, c, P, xi, vext, x, r, Q, pre);
+            if (cn < 0) {
+                c = -1;
+                ++lost;
+                done = true;
+            } else {
                 c = cn;
 #pragma unroll
                 for (int d = 0; d < DIM; ++d) xi[d] = r[d];
@@ -891,12 +934,11 @@ __device__ __forceinline__ void search_body(const Grid& G, const DevMesh<DIM>& m
                 for (int k = 0; k < NV; ++k)
 #pragma unroll
                     for (int d = 0; d < DIM; ++d) P[k][d] = Q[k][d];
-                st = advance64<DIM, DEG>(m, V, pp, A, i, c, x, xi, vext, s, P, pre.Vc, m.ftol[c]);
-                if (st == 0 || !last) break;
+                done = advance64<DIM, DEG>(m, V, pp, A, i, c, x, xi, vext, s, P, pre.Vc, m.ftol[c]) == 0;
             }
         }
-        if (!last) park_particle<DIM>(A, w, qu, round + 1, L.shard, valid && st == 1, i, c, x, xi, vext, s);
-        finish_particle<DIM>(pp, A, w, valid && st == 0, i, c, x, xi, true);
+        finish_particle<DIM>(pp, A, w, done, i, c, x, xi, true);
+        if (done) have = false;
     }
     wave_stat(w.stats, 0, lost);
     wave_stat(w.stats, 3, searches);

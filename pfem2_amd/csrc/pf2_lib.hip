@@ -615,18 +615,16 @@ int run_move(pf2_handle_s* h, int do_correct, int do_move, bool fuse_sums = fals
                                    dev_mesh<DIM>(h), h->dV, h->dVold, dev_particles(h, h->cur), pp, w, qu, do_correct,
                                    do_move);
         }
-        if (do_move)
-            for (int r = 0; r <= MOVE_ROUNDS; ++r) {
-                Timed t(h, r == 0 ? K_SEARCH : (r == 1 ? K_SEARCH_R1 : K_SEARCH_R2));
-                if (h->deg == 2)
-                    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_search_resume<DIM, 2>), dim3(round_grid(qgrid, r)),
-                                       dim3(256), 0, h->stream, dev_mesh<DIM>(h), h->dV, dev_particles(h, h->cur), pp,
-                                       w, qu, r);
-                else
-                    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_search_resume<DIM>), dim3(round_grid(qgrid, r)),
-                                       dim3(256), 0, h->stream, dev_mesh<DIM>(h), h->dV, dev_particles(h, h->cur), pp,
-                                       w, qu, r);
-            }
+        if (do_move) {
+            // one search round with refilling lanes (queue 0)
+            Timed t(h, K_SEARCH);
+            if (h->deg == 2)
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_search_resume<DIM, 2>), dim3(qgrid), dim3(256), 0, h->stream,
+                                   dev_mesh<DIM>(h), h->dV, dev_particles(h, h->cur), pp, w, qu, 0);
+            else
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_search_resume<DIM>), dim3(qgrid), dim3(256), 0, h->stream,
+                                   dev_mesh<DIM>(h), h->dV, dev_particles(h, h->cur), pp, w, qu, 0);
+        }
     }
     HIPC(hipGetLastError());
     if (!do_move) return 0;
@@ -1027,12 +1025,14 @@ int batch_step(pf2_batch_s* b, int n, const pf2_handle* hs, const double* const*
     hipLaunchKernelGGL(HIP_KERNEL_NAME(kb_refresh<DIM>), dim3(tot[BK_REFRESH]), dim3(256), 0, st, bv);
     if (mx) hipLaunchKernelGGL(HIP_KERNEL_NAME(kb_move32<DIM>), dim3(tot[BK_MOVE]), dim3(256), 0, st, bv);
     else hipLaunchKernelGGL(HIP_KERNEL_NAME(kb_move<DIM>), dim3(tot[BK_MOVE]), dim3(256), 0, st, bv);
-    for (int r = 0; r <= MOVE_ROUNDS; ++r) {
-        const dim3 g(tot[BK_SEARCH0 + r]);
-        if (!mx) hipLaunchKernelGGL(HIP_KERNEL_NAME(kb_search<DIM>), g, dim3(256), 0, st, bv, r);
-        else if (r == MOVE_ROUNDS) hipLaunchKernelGGL(HIP_KERNEL_NAME(kb_search32<DIM, true>), g, dim3(256), 0, st, bv, r);
-        else hipLaunchKernelGGL(HIP_KERNEL_NAME(kb_search32<DIM, false>), g, dim3(256), 0, st, bv, r);
-    }
+    if (!mx)
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(kb_search<DIM>), dim3(tot[BK_SEARCH0]), dim3(256), 0, st, bv, 0);
+    else
+        for (int r = 0; r <= MOVE_ROUNDS; ++r) {
+            const dim3 g(tot[BK_SEARCH0 + r]);
+            if (r == MOVE_ROUNDS) hipLaunchKernelGGL(HIP_KERNEL_NAME(kb_search32<DIM, true>), g, dim3(256), 0, st, bv, r);
+            else hipLaunchKernelGGL(HIP_KERNEL_NAME(kb_search32<DIM, false>), g, dim3(256), 0, st, bv, r);
+        }
     if (tot[BK_COUNT])
         hipLaunchKernelGGL(HIP_KERNEL_NAME(kb_count<DIM, GCAP>), dim3(tot[BK_COUNT]), dim3(256), 0, st, bv);
     if (tot[BK_COUNT_L])
@@ -2026,7 +2026,8 @@ int pf2_step_info(pf2_handle h, long long* out, int cap) {
     long long vals[6] = {};
     vals[0] = cnt[3];
     vals[1] = cnt[0];
-    for (int r = 0; r <= MOVE_ROUNDS; ++r) vals[2 + r] = pref[(size_t)r * (QSHARDS + 2) + QSHARDS + 1];
+    // queued per round (the FP64 path searches in one round)
+    for (int r = 0; r <= (mixed(h) ? MOVE_ROUNDS : 0); ++r) vals[2 + r] = pref[(size_t)r * (QSHARDS + 2) + QSHARDS + 1];
     vals[5] = arrivals;
     int k = 0;
     for (; k < 6 && k < cap; ++k) out[k] = vals[k];
