@@ -737,31 +737,62 @@ __device__ __forceinline__ int2 queue_get(const DevQueues& qu, int round, const 
     return data[(size_t)lo * qu.shard_cap + (j - s_pref[lo])];
 }
 
-// runs substeps until done (returns 0) or the own-cell test fails (returns 1)
-template <int DIM, int DEG = 1>
+// A parked particle whose own-cell test k_move left to the search kernel
+// (flag in its parked substep index)
+constexpr int PARK_NEWTON = 1 << 30;
+
+// Runs substeps until done (returns 0) or the own-cell test fails (returns
+// 1; s is the substep whose test failed).  pending: the test of substep s at
+// the current x comes first, before any new substep.
+// CLOSED (2-D, k_move): the own-cell test takes only inverse_map's closed
+// form and returns 2 where inverse_map would go on to its clearly-outside
+// test and Newton iteration (the point is outside the closed form's unit
+// square, or the closed form is degenerate); the search kernel finishes that
+// test (pending).  This keeps the Newton code, and its registers, out of
+// k_move: 94 instead of 167 registers, 5 instead of 3 wavefronts per SIMD.
+template <int DIM, int DEG = 1, bool CLOSED = false>
 __device__ __forceinline__ int advance64(const DevMesh<DIM>& m, const double* __restrict__ V, const DevParams& pp,
                                          const DevParticles& A, int i, int c, double* x, double* xi, double* vext,
-                                         int& s, const double (*P)[DIM], const double (*Vc)[DIM], double tol) {
-    for (; s < pp.substeps; ++s) {
-        double u[DIM], r[DIM];
-        interp<DIM, DEG>(Vc, xi, u);
+                                         int& s, const double (*P)[DIM], const double (*Vc)[DIM], double tol,
+                                         bool pending = false) {
+    for (;; ++s) {
+        if (!pending) {
+            if (s >= pp.substeps) return 0;
+            double u[DIM];
+            interp<DIM, DEG>(Vc, xi, u);
 #pragma unroll
-        for (int d = 0; d < DIM; ++d) vext[d] = u[d];
+            for (int d = 0; d < DIM; ++d) vext[d] = u[d];
 #pragma unroll
-        for (int d = 0; d < DIM; ++d) u[d] *= pp.h;
+            for (int d = 0; d < DIM; ++d) u[d] *= pp.h;
 #pragma unroll
-        for (int d = 0; d < DIM; ++d) x[d] = x[d] + u[d];
-        if (pp.keep_vext && s == pp.substeps - 1)
+            for (int d = 0; d < DIM; ++d) x[d] = x[d] + u[d];
+            if (pp.keep_vext && s == pp.substeps - 1)
 #pragma unroll
-            for (int d = 0; d < DIM; ++d) A.vext[(size_t)d * A.cap + i] = vext[d];
-        if (inverse_map<DIM>(P, x, r, tol, false) && inside_unit<DIM>(r)) {
+                for (int d = 0; d < DIM; ++d) A.vext[(size_t)d * A.cap + i] = vext[d];
+        }
+        pending = false;
+        double r[DIM];
+        if constexpr (CLOSED && DIM == 2) {
+            // inverse_map<2>(P, x, r, tol, false) up to its clearly-outside test
+            if (inverse_quadratic(P, x, r)) {
+                const double eps = 1e-15;
+                if (-eps <= r[1] && r[1] <= 1 + eps && -eps <= r[0] && r[0] <= 1 + eps) {
+                    if (!inside_unit<DIM>(r)) return 1;
 #pragma unroll
-            for (int d = 0; d < DIM; ++d) xi[d] = r[d];
+                    for (int d = 0; d < DIM; ++d) xi[d] = r[d];
+                    continue;
+                }
+            }
+            return 2;
         } else {
-            return 1;
+            if (inverse_map<DIM>(P, x, r, tol, false) && inside_unit<DIM>(r)) {
+#pragma unroll
+                for (int d = 0; d < DIM; ++d) xi[d] = r[d];
+            } else {
+                return 1;
+            }
         }
     }
-    return 0;
 }
 
 // final bookkeeping of a particle whose substeps are complete (all lanes call)
@@ -839,9 +870,9 @@ __device__ __forceinline__ void move_body(const Grid& G, const DevMesh<DIM>& m, 
             for (int d = 0; d < DIM; ++d) x[d] = A.x[(size_t)d * A.cap + i];
             double P[NV][DIM], Vc[ND][DIM];
             load_cell<DIM, false, DEG>(m, c, P, Vc);
-            st = advance64<DIM, DEG>(m, V, pp, A, i, c, x, xi, vext, s, P, Vc, m.ftol[c]);
+            st = advance64<DIM, DEG, true>(m, V, pp, A, i, c, x, xi, vext, s, P, Vc, m.ftol[c]);
         }
-        park_particle<DIM>(A, w, qu, 0, L.shard, valid && st == 1, i, c, x, xi, vext, s);
+        park_particle<DIM>(A, w, qu, 0, L.shard, valid && st != 0, i, c, x, xi, vext, st == 2 ? s | PARK_NEWTON : s);
         finish_particle<DIM>(pp, A, w, valid && st == 0, i, c, x, xi, true);
     }
 }
@@ -881,7 +912,7 @@ __device__ __forceinline__ void search_body(const Grid& G, const DevMesh<DIM>& m
     const int lane = threadIdx.x & (WAVE - 1);
     const unsigned long long below = (1ull << lane) - 1;
     int lost = 0, searches = 0;
-    bool have = false, more = true;
+    bool have = false, more = true, newton = false;
     int i = 0, c = 0, s = 0;
     double x[DIM], xi[DIM], vext[DIM], P[NV][DIM];
 #pragma unroll
@@ -901,7 +932,9 @@ __device__ __forceinline__ void search_body(const Grid& G, const DevMesh<DIM>& m
                     const int2 e = queue_get(qu, round, s_pref, j);
                     i = e.x;
                     c = e.y;
-                    s = qu.park_s[i];
+                    const int ps = qu.park_s[i];
+                    s = ps & ~PARK_NEWTON;
+                    newton = (ps & PARK_NEWTON) != 0;
 #pragma unroll
                     for (int d = 0; d < DIM; ++d) {
                         x[d] = A.x[(size_t)d * A.cap + i];
@@ -915,12 +948,51 @@ __device__ __forceinline__ void search_body(const Grid& G, const DevMesh<DIM>& m
         }
         if (__ballot(have) == 0) break;
         bool done = false;
-        if (have) {
+        if (DIM == 2 && have) {
+            // a particle k_move flagged has its own-cell test at x pending:
+            // clearly outside, it goes to the search; otherwise the test
+            // (inverse_map from its closed form, as in advance64) runs first
+            // in advance64, which goes on in the cell if x is inside
+            bool own = false;
+            if (newton) {
+                newton = false;
+                own = !clearly_outside<DIM>(P, x, m.ftol[c]);
+            }
+            double r[DIM];
+            PrefetchV64<DIM, DEG> pre{m.cellV};
+            int cn = c;
+            if (own) {
+                pre.load(c);
+            } else {
+                double Q[NV][DIM];
+                ++searches;
+                cn = relocate<DIM, PrefetchV64<DIM, DEG>, true>(m, c, P, xi, vext, x, r, Q, pre);
+                if (cn >= 0)
+#pragma unroll
+                    for (int k = 0; k < NV; ++k)
+#pragma unroll
+                        for (int d = 0; d < DIM; ++d) P[k][d] = Q[k][d];
+            }
+            if (cn < 0) {
+                c = -1;
+                ++lost;
+                done = true;
+            } else {
+                if (!own) {
+                    c = cn;
+#pragma unroll
+                    for (int d = 0; d < DIM; ++d) xi[d] = r[d];
+                    ++s;
+                }
+                done = advance64<DIM, DEG>(m, V, pp, A, i, c, x, xi, vext, s, P, pre.Vc, m.ftol[c], own) == 0;
+            }
+        } else if (have) {
+            // 3-D (k_move finishes its own-cell tests)
             double Q[NV][DIM], r[DIM];
             ++searches;
             PrefetchV64<DIM, DEG> pre{m.cellV};
-            // (3-D: the shortcut's registers push this kernel into heavy spilling)
-            const int cn = relocate<DIM, PrefetchV64<DIM, DEG>, DIM == 2>(m, c, P, xi, vext, x, r, Q, pre);
+            // (the face-neighbour shortcut's registers push this kernel into heavy spilling)
+            const int cn = relocate<DIM, PrefetchV64<DIM, DEG>, false>(m, c, P, xi, vext, x, r, Q, pre);
             if (cn < 0) {
                 c = -1;
                 ++lost;
