@@ -593,6 +593,8 @@ constexpr int REFILL_MIN = 16;
 // path's last round, so no search pushes to it; the counters are cleared
 // with the other queue counters after each regroup
 constexpr int CURSOR_ROUND = MOVE_ROUNDS + 1;
+// the queue of the FP64 re-sort's relocations (the FP64 move uses queue 0)
+constexpr int RESORT_QUEUE = 1;
 
 // Queues of parked particles, one per round, ping-ponging between two
 // buffers.  A queue is QSHARDS shards of shard_cap entries plus an overflow
@@ -894,7 +896,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DIM ==
 // Every iteration (one search, then the substeps up to the next exit)
 // consumes a substep, so each lane ends, and a wavefront leaves when its
 // segment is drained and none of its lanes holds a particle.
-template <int DIM, int DEG = 1>
+template <int DIM, int DEG = 1, bool RESORT = false>
 __device__ __forceinline__ void search_body(const Grid& G, const DevMesh<DIM>& m, const double* __restrict__ V,
                                             const DevParticles& A, const DevParams& pp, const DevWork& w,
                                             const DevQueues& qu, int round) {
@@ -948,7 +950,30 @@ __device__ __forceinline__ void search_body(const Grid& G, const DevMesh<DIM>& m
         }
         if (__ballot(have) == 0) break;
         bool done = false;
-        if (DIM == 2 && have) {
+        if constexpr (RESORT) {
+            // re-sort after a mesh move: the own-cell test left by k_resort,
+            // then the relocation; no advection
+            if (have) {
+                double r[DIM];
+                bool own = false;
+                if (newton) {
+                    newton = false;
+                    own = !clearly_outside<DIM>(P, x, m.ftol[c]) && inverse_newton<DIM>(P, x, r) &&
+                          inside_unit<DIM>(r);
+                }
+                if (!own) {
+                    double Q[NV][DIM];
+                    ++searches;
+                    NoPrefetch pre;
+                    c = relocate<DIM>(m, c, P, xi, vext, x, r, Q, pre);
+                    if (c < 0) ++lost;
+                }
+                if (c >= 0)
+#pragma unroll
+                    for (int d = 0; d < DIM; ++d) xi[d] = r[d];
+                done = true;
+            }
+        } else if (DIM == 2 && have) {
             // a particle k_move flagged has its own-cell test at x pending:
             // clearly outside, it goes to the search; otherwise the test
             // (inverse_map from its closed form, as in advance64) runs first
@@ -1021,6 +1046,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DIM ==
                                                        DevParticles A, DevParams pp, DevWork w, DevQueues qu,
                                                        int round) {
     search_body<DIM, DEG>(hw_grid(), m, V, A, pp, w, qu, round);
+}
+
+// the relocations of the re-sort after a mesh move (queue RESORT_QUEUE)
+template <int DIM>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DIM == 2 ? 3 : 2))) k_resort_search(DevMesh<DIM> m, DevParticles A, DevParams pp, DevWork w, DevQueues qu) {
+    search_body<DIM, 1, true>(hw_grid(), m, nullptr, A, pp, w, qu, RESORT_QUEUE);
 }
 
 // candidate j of cell c: stayers of the old segment first, then the
@@ -2130,61 +2161,67 @@ __global__ void k_init_particles(DevMesh<DIM> m, const double* __restrict__ V, D
     }
 }
 
-// re-sort after a mesh move: own-cell test and relocation with the stored
-// last advection velocity, no advection.  skip_static: the geometry changed
-// once since the last re-sort, so a particle whose cell kept its vertices
-// and whose xi came from an inverse map in that cell (ids up to the counter
-// of the last advection) keeps its xi and cell: the inverse map would
-// return the same bits
+// re-sort after a mesh move: own-cell test with the stored last advection
+// velocity, no advection; a particle that fails it (or, in 2-D, whose test
+// needs inverse_map's clearly-outside test and Newton iteration) is parked
+// with no substeps left for k_resort_search, which relocates it in full
+// wavefronts.  skip_static: the geometry changed once since the last
+// re-sort, so a particle whose cell kept its vertices and whose xi came from
+// an inverse map in that cell (ids up to the counter of the last advection)
+// keeps its xi and cell: the inverse map would return the same bits
 template <int DIM>
 __global__ void __launch_bounds__(256) k_resort(DevMesh<DIM> m, DevParticles A, DevParams pp, DevWork w,
-                                                int skip_static) {
+                                                DevQueues qu, int skip_static) {
     constexpr int NV = Q1<DIM>::NV;
     const int n = state_dead(w.counters) ? 0 : w.counters[0];
     const int lane = threadIdx.x & (WAVE - 1);
     const uint32_t id_exact = (uint32_t)w.counters[5];
-    int lost = 0, searches = 0;
-    for (int base = blockIdx.x * blockDim.x + (threadIdx.x - lane); base < n; base += gridDim.x * blockDim.x) {
+    const WaveLoop L = stride_loop(n);
+    for (int base = L.begin; base < L.end; base += L.step) {
         const int i = base + lane;
-        const bool valid = i < n;
-        const int c0 = valid ? A.cell[i] : -1;
-        int c = c0;
-        double x[DIM], xi[DIM], vext[DIM], r[DIM], P[NV][DIM];
-        if (valid && skip_static && !m.cmoved[c0] && A.id[i] <= id_exact) {
+        const bool valid = i < L.end;
+        const int c = valid ? A.cell[i] : -1;
+        double x[DIM], xi[DIM], vext[DIM], r[DIM];
+        int st = 0;   // 0 stays, 1 relocate, 2 own-cell test pending
+        if (valid) {
 #pragma unroll
             for (int d = 0; d < DIM; ++d) xi[d] = A.xi[(size_t)d * A.cap + i];
-            w.dest[i] = c0;
-            w.pkey[i] = part_key<DIM>(pp, xi);
-        } else if (valid) {
+            if (!(skip_static && !m.cmoved[c] && A.id[i] <= id_exact)) {
+                double P[NV][DIM];
 #pragma unroll
-            for (int d = 0; d < DIM; ++d) {
-                xi[d] = A.xi[(size_t)d * A.cap + i];
-                x[d] = A.x[(size_t)d * A.cap + i];
-                vext[d] = A.vext ? A.vext[(size_t)d * A.cap + i] : 0.0;
+                for (int d = 0; d < DIM; ++d) x[d] = A.x[(size_t)d * A.cap + i];
+                load_vertices<DIM>(m, c, P);
+                if constexpr (DIM == 2) {
+                    // inverse_map<2>(P, x, r, tol, false) up to its clearly-outside test
+                    st = 2;
+                    if (inverse_quadratic(P, x, r)) {
+                        const double eps = 1e-15;
+                        if (-eps <= r[1] && r[1] <= 1 + eps && -eps <= r[0] && r[0] <= 1 + eps)
+                            st = inside_unit<DIM>(r) ? 0 : 1;
+                    }
+                } else {
+                    st = inverse_map<DIM>(P, x, r, m.ftol[c], false) && inside_unit<DIM>(r) ? 0 : 1;
+                }
+                if (st == 0) {
+#pragma unroll
+                    for (int d = 0; d < DIM; ++d) {
+                        xi[d] = r[d];
+                        A.xi[(size_t)d * A.cap + i] = r[d];
+                    }
+                } else {
+#pragma unroll
+                    for (int d = 0; d < DIM; ++d) vext[d] = A.vext ? A.vext[(size_t)d * A.cap + i] : 0.0;
+                }
             }
-            load_vertices<DIM>(m, c, P);
-            if (inverse_map<DIM>(P, x, r, m.ftol[c], false) && inside_unit<DIM>(r)) {
-#pragma unroll
-                for (int d = 0; d < DIM; ++d) xi[d] = r[d];
-            } else {
-                ++searches;
-                double Q[NV][DIM];
-                NoPrefetch pre;
-                c = relocate<DIM>(m, c, P, xi, vext, x, r, Q, pre);
-                if (c < 0) ++lost;
-                else
-#pragma unroll
-                    for (int d = 0; d < DIM; ++d) xi[d] = r[d];
+            if (st == 0) {
+                w.dest[i] = c;
+                w.pkey[i] = part_key<DIM>(pp, xi);
             }
-#pragma unroll
-            for (int d = 0; d < DIM; ++d) A.xi[(size_t)d * A.cap + i] = xi[d];
-            w.dest[i] = c;
-            if (c >= 0) w.pkey[i] = part_key<DIM>(pp, xi);
         }
-        add_arrival(w, valid && c >= 0 && c != c0, c, i);
+        const int sp = pp.substeps - 1;
+        park_particle<DIM>(A, w, qu, RESORT_QUEUE, L.shard, valid && st != 0, i, c, x, xi, vext,
+                           st == 2 ? sp | PARK_NEWTON : sp);
     }
-    wave_stat(w.stats, 0, lost);
-    wave_stat(w.stats, 3, searches);
 }
 
 // dest / key of every particle for a reseed pass without movement

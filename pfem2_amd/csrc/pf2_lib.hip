@@ -46,7 +46,7 @@ enum KernelId {
     K_MOVE = 0, K_GROUP, K_CELL_COUNT, K_CELL_WRITE, K_SCAN, K_FINALIZE, K_PARTIALS, K_NODE, K_WALL,
     K_SEED, K_MISC, K_MOVE32, K_CELL_WRITE32, K_PARTIALS32, K_SEARCH, K_SEARCH32, K_QPREFIX,
     K_SEARCH_R1, K_SEARCH_R2, K_SEARCH32_R1, K_SEARCH32_R2, K_CELL_BIG, K_RESORT, K_RESORT32, K_CELL_MAP,
-    K_CELL_GATHER, K_CELL_GATHER32, K_NKERNELS
+    K_CELL_GATHER, K_CELL_GATHER32, K_RESORT_SEARCH, K_NKERNELS
 };
 const char* kKernelNames[K_NKERNELS] = {"k_move", "k_group", "k_cell_count", "k_cell_write", "scan",
                                         "k_finalize", "k_cell_partials", "k_node", "wall_fixup",
@@ -54,7 +54,7 @@ const char* kKernelNames[K_NKERNELS] = {"k_move", "k_group", "k_cell_count", "k_
                                         "k_search_resume", "k_search_resume32", "k_queue_prefix",
                                         "k_search_resume_r1", "k_search_resume_r2", "k_search_resume32_r1",
                                         "k_search_resume32_r2", "k_cell_big", "k_resort", "k_resort32",
-                                        "k_cell_map", "k_cell_gather", "k_cell_gather32"};
+                                        "k_cell_map", "k_cell_gather", "k_cell_gather32", "k_resort_search"};
 
 // bytes obtained through dalloc on this thread (a handle's footprint is the
 // difference across its creation)
@@ -631,6 +631,27 @@ int run_move(pf2_handle_s* h, int do_correct, int do_move, bool fuse_sums = fals
     // a fused step (projection sums in the cell write) leaves the
     // bookkeeping to the node pass's last workgroup
     return regroup<DIM>(h, 1, fuse_sums, fuse_sums);
+}
+
+// FP64 re-sort: own-cell tests (k_resort), then the relocation of the
+// particles that failed them in full wavefronts (k_resort_search)
+template <int DIM>
+int resort64(pf2_handle_s* h, int skip) {
+    DevParams pp = dev_params(h);
+    DevWork w = dev_work(h);
+    DevQueues qu = dev_queues(h);
+    {
+        Timed t(h, K_RESORT);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_resort<DIM>), dim3(grid_for(h->n_est, 256, 4096)), dim3(256), 0,
+                           h->stream, dev_mesh<DIM>(h), dev_particles(h, h->cur), pp, w, qu, skip);
+    }
+    {
+        Timed t(h, K_RESORT_SEARCH);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_resort_search<DIM>), dim3(grid_for(h->n_est / 2 + 1, 256, 2048)),
+                           dim3(256), 0, h->stream, dev_mesh<DIM>(h), dev_particles(h, h->cur), pp, w, qu);
+    }
+    HIPC(hipGetLastError());
+    return 0;
 }
 
 // partial: where the per-cell projection sums go (the handle's own buffer
@@ -1735,21 +1756,10 @@ int pf2_sort_particles_into_subdomains_and_cells(pf2_handle h) {
     if (!h->params.keep_velocity_ext)
         return fail(PF2_E_STATE, "relocation without advection needs keep_velocity_ext");
     if (int r = from_mixed(h)) return r;
-    DevParams pp = dev_params(h);
-    DevWork w = dev_work(h);
-    {
-        Timed t(h, K_RESORT);
-        // particles of unchanged cells keep their xi when one geometry update separates this re-sort from the last
-        const int skip = h->geo_updates == 1;
-        h->geo_updates = 0;
-        if (h->dim == 2)
-            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_resort<2>), dim3(grid_for(h->n_est, 256, 4096)), dim3(256), 0,
-                               h->stream, dev_mesh<2>(h), dev_particles(h, h->cur), pp, w, skip);
-        else
-            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_resort<3>), dim3(grid_for(h->n_est, 256, 4096)), dim3(256), 0,
-                               h->stream, dev_mesh<3>(h), dev_particles(h, h->cur), pp, w, skip);
-    }
-    HIPC(hipGetLastError());
+    // particles of unchanged cells keep their xi when one geometry update separates this re-sort from the last
+    const int skip = h->geo_updates == 1;
+    h->geo_updates = 0;
+    if (int r = h->dim == 2 ? resort64<2>(h, skip) : resort64<3>(h, skip)) return r;
     // regroup the FP64 copy, then convert back
     int r = ensure_packed(h);
     if (!r) r = h->dim == 2 ? regroup<2>(h, 0, false, false, true) : regroup<3>(h, 0, false, false, true);
@@ -1839,25 +1849,20 @@ int move_mesh_impl(pf2_handle_s* h, const double* vertices, int recompute_maps, 
     h->geo_updates += 1;
     DevParams pp = dev_params(h);
     DevWork w = dev_work(h);
-    {
-        Timed t(h, mixed(h) ? K_RESORT32 : K_RESORT);
-        // particles of unchanged cells keep their xi when this is the only geometry update since the last re-sort
-        const int skip = h->geo_updates == 1;
-        h->geo_updates = 0;
-        if (mixed(h)) {
-            h->fp64_valid = false;
-            if (h->dim == 2)
-                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_resort32<2>), dim3(grid), dim3(256), 0, h->stream,
-                                   dev_mesh<2>(h), b.cell, b.id, b.x, b.vext, rec32(b), h->cap, pp, w, skip);
-            else
-                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_resort32<3>), dim3(grid), dim3(256), 0, h->stream,
-                                   dev_mesh<3>(h), b.cell, b.id, b.x, b.vext, rec32(b), h->cap, pp, w, skip);
-        } else if (h->dim == 2)
-            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_resort<2>), dim3(grid), dim3(256), 0, h->stream, dev_mesh<2>(h),
-                               dev_particles(h, h->cur), pp, w, skip);
+    // particles of unchanged cells keep their xi when this is the only geometry update since the last re-sort
+    const int skip = h->geo_updates == 1;
+    h->geo_updates = 0;
+    if (mixed(h)) {
+        Timed t(h, K_RESORT32);
+        h->fp64_valid = false;
+        if (h->dim == 2)
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_resort32<2>), dim3(grid), dim3(256), 0, h->stream, dev_mesh<2>(h),
+                               b.cell, b.id, b.x, b.vext, rec32(b), h->cap, pp, w, skip);
         else
-            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_resort<3>), dim3(grid), dim3(256), 0, h->stream, dev_mesh<3>(h),
-                               dev_particles(h, h->cur), pp, w, skip);
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_resort32<3>), dim3(grid), dim3(256), 0, h->stream, dev_mesh<3>(h),
+                               b.cell, b.id, b.x, b.vext, rec32(b), h->cap, pp, w, skip);
+    } else if (int r = h->dim == 2 ? resort64<2>(h, skip) : resort64<3>(h, skip)) {
+        return r;
     }
     HIPC(hipGetLastError());
     // relocation and reseed in one census: the same survivors, ids and
@@ -2026,8 +2031,10 @@ int pf2_step_info(pf2_handle h, long long* out, int cap) {
     long long vals[6] = {};
     vals[0] = cnt[3];
     vals[1] = cnt[0];
-    // queued per round (the FP64 path searches in one round)
-    for (int r = 0; r <= (mixed(h) ? MOVE_ROUNDS : 0); ++r) vals[2 + r] = pref[(size_t)r * (QSHARDS + 2) + QSHARDS + 1];
+    // queued per round (the FP64 path searches in one round; its queue 1
+    // holds the relocations of the last re-sort after a mesh move)
+    for (int r = 0; r <= (mixed(h) ? MOVE_ROUNDS : RESORT_QUEUE); ++r)
+        vals[2 + r] = pref[(size_t)r * (QSHARDS + 2) + QSHARDS + 1];
     vals[5] = arrivals;
     int k = 0;
     for (; k < 6 && k < cap; ++k) out[k] = vals[k];
