@@ -412,7 +412,7 @@ long long default_capacity(int dim, int n_cells, const pf2_params* p) {
 int arrival_slots(int dim, const pf2_params* p) {
     long long parts = 1;
     for (int d = 0; d < dim; ++d) parts *= p->quantities[d];
-    return parts <= 16 ? GCAP : GCAP_L;
+    return census_small_row((int)parts, p->max_per_part) ? GCAP : GCAP_L;
 }
 
 int alloc_particles(pf2_handle_s* h) {
@@ -488,7 +488,7 @@ int regroup(pf2_handle_s* h, int reseed, bool fuse_sums = false, bool defer_fina
     const int* old_off = h->off[h->cur_off];
     {
         Timed t(h, K_CELL_COUNT);
-        if (pp.nparts_in <= 16)
+        if (census_small_row(pp.nparts_in, pp.max_per_part))
             hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cell_count<DIM, GCAP>), dim3(cell_grid(nc, 8192)), dim3(256), 0,
                                h->stream, nc, old_off, h->buf[h->cur].id, pp, w, reseed);
         else
@@ -642,7 +642,7 @@ int resort64(pf2_handle_s* h, int skip) {
     DevQueues qu = dev_queues(h);
     {
         Timed t(h, K_RESORT);
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_resort<DIM>), dim3(grid_for(h->n_est, 256, 4096)), dim3(256), 0,
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_resort<DIM>), dim3(grid_for(h->n_est, 256, 16384)), dim3(256), 0,
                            h->stream, dev_mesh<DIM>(h), dev_particles(h, h->cur), pp, w, qu, skip);
     }
     {
@@ -1002,7 +1002,7 @@ void batch_fill(pf2_handle_s* h, const double* field, BatchSim<DIM>& S, int* gri
     grid[BK_REFRESH] = grid_for(std::max((long long)nc * h->nd, (long long)h->n_dofs), 256, 4096);
     grid[BK_MOVE] = grid_for(h->n_est, 256, 8192);
     for (int r = 0; r <= MOVE_ROUNDS; ++r) grid[BK_SEARCH0 + r] = round_grid(qgrid, r);
-    const bool small = S.pp.nparts_in <= 16;
+    const bool small = census_small_row(S.pp.nparts_in, S.pp.max_per_part);
     grid[BK_COUNT] = small ? cell_grid(nc, 8192) : 0;
     grid[BK_COUNT_L] = small ? 0 : cell_grid(nc, 8192);
     grid[BK_BIG] = 32;
