@@ -143,9 +143,15 @@ def kernel_bytes(name, info, m, dim=2, vext=False):
         mesh = nc * nv * 4 + nd * dim * wv * 2 + (m.n_vertices * dim * 8 if name == "k_move" else nc * 24)
         return reads + writes + mesh + (n0 * dim * 8 if vext else 0)
     if name == "k_resort":
-        # own-cell test of every particle after a mesh move: cell, x, xi, vext
-        # in; xi, destination and part key out
-        return n0 * (4 + 3 * dim * 8) + n0 * (dim * 8 + 8) + nc * nv * 4 + m.n_vertices * dim * 8
+        # own-cell test of every particle after a mesh move: cell, id, xi in,
+        # destination and part key out; the q[1] it parks for k_resort_search
+        # also read x and vext and store the parking record (x, xi,
+        # velocity_ext, substep, queue entry); per cell: vertices, moved flag
+        return n0 * (4 + 4 + dim * 8 + 8) + q[1] * (2 * dim * 8 + 3 * dim * 8 + 4 + 8) + nc * (nv * dim * 8 + 1)
+    if name == "k_resort_search":
+        # the relocations of the re-sort (queue 1): entry, record and vertices
+        # in; x, xi, destination and part key out
+        return int(q[1] * (8 + 4 + 3 * dim * 8 + nv * dim * 8 + 2 * dim * 8 + 8)) if len(q) > 1 else None
     if name.startswith("k_search_resume"):
         # one launch per relocation round; round r handles q[r] particles
         mm = re.match(r"(k_search_resume(?:32)?)(?:_r(\d))?$", name)
