@@ -118,7 +118,7 @@ template <int DIM>
 __global__ void __launch_bounds__(256) kb_big(BatchView<DIM> bv) {
     Grid G;
     const BatchSim<DIM>& S = bv.at(BK_BIG, G);
-    cell_big_body<DIM>(G, S.old_off, S.A.id, S.pp, S.w, 1);
+    cell_big_body<DIM>(G, S.old_off, S.A.id, S.pp, S.w, 1, S.w.big);
 }
 
 // ---- exclusive scan of the per-cell counts (fs_cnt -> fs_off), per simulation

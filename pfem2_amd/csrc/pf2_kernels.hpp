@@ -188,6 +188,7 @@ struct DevWork {
                         // [3] count before the last regroup, [4] overflow entries in use,
                         // [5] id counter at the last advection (ids up to it hold an inverse-mapped xi)
     int* big;           // [0] number of cells listed for k_cell_big, then the cells
+    int* big2;          // the same for the cells that the second census pass (rows of GCAP_L) lists
     uint32_t* particle_counter;
     unsigned long long* stats;   // STAT_SHARDS rows: lost, deleted, spawned, searches, zero-weight
 };
@@ -1156,7 +1157,7 @@ __device__ __forceinline__ unsigned long long group_or(unsigned long long v) {
 // A cell's candidates (stayers of its old segment, then arrivals) are staged
 // in LDS when there are at most CAP of them (GCAP or GCAP_L), else listed
 // for k_cell_big.
-template <int DIM, int CAP>
+template <int DIM, int CAP, bool LIST = false>
 __device__ __forceinline__ void cell_count_body(const Grid& G, int n_cells, const int* __restrict__ old_off, const uint32_t* __restrict__ pid, const DevParams& pp, const DevWork& w, int reseed) {
     // Staged rows, compacted to the cell's valid candidates (stayers whose
     // destination is c, then the arrivals): candidate index; part key << 32 |
@@ -1179,14 +1180,17 @@ __device__ __forceinline__ void cell_count_body(const Grid& G, int n_cells, cons
     const bool by_count = nk <= NKMAX;
     for (int t = gl; t < NKMAX / 4; t += GS) s_cnt[g][t] = 0;
     int deleted = 0, arrivals = 0;
-    for (int cb = G.bid * GROUPS; cb < n_cells; cb += G.nblk * GROUPS) {
-        const int c = cb + g;
-        const bool cv = c < n_cells;
+    // LIST: the cells that the first pass listed (w.big), in rows of CAP;
+    // the cells that do not fit either are listed again in w.big2
+    const int ncl = LIST ? w.big[0] : n_cells;
+    for (int cb = G.bid * GROUPS; cb < ncl; cb += G.nblk * GROUPS) {
+        const bool cv = cb + g < ncl;
+        const int c = LIST ? (cv ? w.big[1 + cb + g] : 0) : cb + g;
         const int n_st = cv ? old_off[c + 1] - old_off[c] : 0;
         const int na = cv ? w.arr_cnt[c] : 0;
         const int k = n_st + na;
         const bool small = k <= CAP && na <= w.aslots;
-        if (cv && gl == 0) arrivals += na;
+        if (!LIST && cv && gl == 0) arrivals += na;
         unsigned long long hit = 0;
         // nv: valid candidates, nst of them stayers (the first nst staged
         // entries, in ascending id order like the cell's segment); the same
@@ -1260,9 +1264,11 @@ __device__ __forceinline__ void cell_count_body(const Grid& G, int n_cells, cons
                 if (surv) s_sid[g][j] = mi;
             }
         else if (cv && gl == 0) {
-            // more than CAP candidates: census by k_cell_big
-            const int slot = atomicAdd(&w.big[0], 1);
-            w.big[1 + slot] = c;
+            // more than CAP candidates: census by k_cell_big (or by the
+            // second pass with larger rows)
+            int* list = LIST ? w.big2 : w.big;
+            const int slot = atomicAdd(&list[0], 1);
+            list[1 + slot] = c;
         }
         wave_sync();
         if (counted)   // back to zero for the next cell (the words hold only this cell's parts)
@@ -1325,14 +1331,14 @@ __device__ __forceinline__ void cell_count_body(const Grid& G, int n_cells, cons
     }
     deleted = wave_sum(deleted);
     if ((threadIdx.x & (WAVE - 1)) == 0 && deleted) stat_add(w.stats, 1, (unsigned long long)deleted);
-    wave_stat(w.stats, 5, arrivals);
+    if (!LIST) wave_stat(w.stats, 5, arrivals);
 }
 
-template <int DIM, int CAP>
+template <int DIM, int CAP, bool LIST = false>
 __global__ void __launch_bounds__(256) k_cell_count(int n_cells, const int* __restrict__ old_off,
                                                     const uint32_t* __restrict__ pid, DevParams pp, DevWork w,
                                                     int reseed) {
-    cell_count_body<DIM, CAP>(hw_grid(), n_cells, old_off, pid, pp, w, reseed);
+    cell_count_body<DIM, CAP, LIST>(hw_grid(), n_cells, old_off, pid, pp, w, reseed);
 }
 
 // The census of k_cell_count for the cells it listed (more than CAP
@@ -1342,7 +1348,7 @@ __global__ void __launch_bounds__(256) k_cell_count(int n_cells, const int* __re
 constexpr int BIGCAP = 2048;
 
 template <int DIM>
-__device__ __forceinline__ void cell_big_body(const Grid& G, const int* __restrict__ old_off, const uint32_t* __restrict__ pid, const DevParams& pp, const DevWork& w, int reseed) {
+__device__ __forceinline__ void cell_big_body(const Grid& G, const int* __restrict__ old_off, const uint32_t* __restrict__ pid, const DevParams& pp, const DevWork& w, int reseed, const int* __restrict__ list) {
     __shared__ int s_idx[BIGCAP];
     __shared__ __align__(16) unsigned long long s_kid[BIGCAP];
     __shared__ __align__(16) uint32_t s_sid[BIGCAP];
@@ -1351,10 +1357,10 @@ __device__ __forceinline__ void cell_big_body(const Grid& G, const int* __restri
     if (state_dead(w.counters)) return;
     const int tid = threadIdx.x;
     const unsigned long long all = pp.nparts_in >= 64 ? ~0ull : ((1ull << pp.nparts_in) - 1);
-    const int nbig = w.big[0];
+    const int nbig = list[0];
     int deleted = 0;
     for (int b = G.bid; b < nbig; b += G.nblk) {
-        const int c = w.big[1 + b];
+        const int c = list[1 + b];
         const int n_st = old_off[c + 1] - old_off[c];
         const int na = w.arr_cnt[c];
         const int k = n_st + na;
@@ -1453,10 +1459,11 @@ __device__ __forceinline__ void cell_big_body(const Grid& G, const int* __restri
     if (tid == 0 && deleted) stat_add(w.stats, 1, (unsigned long long)deleted);
 }
 
+// second: the cells listed by the second census pass (w.big2)
 template <int DIM>
 __global__ void __launch_bounds__(256) k_cell_big(const int* __restrict__ old_off, const uint32_t* __restrict__ pid,
-                                                  DevParams pp, DevWork w, int reseed) {
-    cell_big_body<DIM>(hw_grid(), old_off, pid, pp, w, reseed);
+                                                  DevParams pp, DevWork w, int reseed, int second) {
+    cell_big_body<DIM>(hw_grid(), old_off, pid, pp, w, reseed, second ? w.big2 : w.big);
 }
 
 // ---------------------------------------------------------------------------
@@ -1854,6 +1861,7 @@ __device__ __forceinline__ void finalize_block(int n_cells, const DevWork& w, in
     w.counters[0] = total;
     w.counters[4] = 0;
     w.big[0] = 0;
+    w.big2[0] = 0;
 }
 
 __global__ void __launch_bounds__(QSHARDS) k_finalize(int n_cells, DevWork w, int* queue_count) {

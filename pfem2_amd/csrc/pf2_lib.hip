@@ -152,6 +152,7 @@ struct pf2_handle_s {
     double* partial = nullptr;
     int* counters = nullptr;
     int* big = nullptr;
+    int* big2 = nullptr;
     int* qcount = nullptr;
     int* qprefix = nullptr;
     long long shard_cap = 0;
@@ -353,6 +354,7 @@ DevWork dev_work(pf2_handle_s* h) {
     w.particle_counter = h->pcounter;
     w.stats = h->stats;
     w.big = h->big;
+    w.big2 = h->big2;
     return w;
 }
 
@@ -486,21 +488,29 @@ int regroup(pf2_handle_s* h, int reseed, bool fuse_sums = false, bool defer_fina
     DevParams pp = dev_params(h);
     DevWork w = dev_work(h);
     const int* old_off = h->off[h->cur_off];
+    // rows of GCAP for a lattice of more than 16 parts: the cells that do
+    // not fit are listed for a second pass in rows of GCAP_L, and only those
+    // that do not fit there go to k_cell_big
+    const bool small = census_small_row(pp.nparts_in, pp.max_per_part);
+    const bool second = small && pp.nparts_in > 16;
     {
         Timed t(h, K_CELL_COUNT);
-        if (census_small_row(pp.nparts_in, pp.max_per_part))
+        if (small)
             hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cell_count<DIM, GCAP>), dim3(cell_grid(nc, 8192)), dim3(256), 0,
                                h->stream, nc, old_off, h->buf[h->cur].id, pp, w, reseed);
         else
             hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cell_count<DIM, GCAP_L>), dim3(cell_grid(nc, 8192)), dim3(256), 0,
                                h->stream, nc, old_off, h->buf[h->cur].id, pp, w, reseed);
+        if (second)
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cell_count<DIM, GCAP_L, true>), dim3(256), dim3(256), 0, h->stream,
+                               nc, old_off, h->buf[h->cur].id, pp, w, reseed);
     }
     {
         Timed t(h, K_CELL_BIG);
-        // 1024 workgroups: a few thousand listed cells (5x5 lattices in
-        // rows of 64) take 0.034 ms instead of 0.075 with 256
+        // 1024 workgroups: a few thousand listed cells take 0.034 ms instead
+        // of 0.075 with 256
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cell_big<DIM>), dim3(1024), dim3(256), 0, h->stream, old_off,
-                           h->buf[h->cur].id, pp, w, reseed);
+                           h->buf[h->cur].id, pp, w, reseed, second ? 1 : 0);
     }
     if (int r = scan(h, h->fs_cnt, h->fs_off, nc + 1)) return r;
     const bool two_pass = !fuse_sums && h->omap && nc < (1 << 25);
@@ -851,7 +861,7 @@ size_t estimate_bytes(int dim, long long nc, long long nvt, long long nd, long l
     b += (nd + 1) * 4 + ndc * 4 + nc * 8 + nck * dim * 8 + nck * 8;                    // dof map, ftol, cellX/star
     if (p->precision != PF2_FP32_MIXED) b += ndc * dim * 8 * 2;                          // FP64 cell V / dV copies
     b += nd * dim * 8 * 3;                                                              // V, V_old, staging
-    b += (nc + 1) * 4 * 9 + nc * 4 * (1 + arrival_slots(dim, p));                      // per-cell work, arrival slots
+    b += (nc + 1) * 4 * 10 + nc * 4 * (1 + arrival_slots(dim, p));                     // per-cell work, arrival slots
     b += nc * 8 + nc * 4 + ndc * (dim + 1) * 8 + 32;                                     // census, partials, counters
     b += (size_t)(MOVE_ROUNDS + 2) * (QSHARDS + 1) * QSTRIDE * 4 + (size_t)(MOVE_ROUNDS + 2) * (QSHARDS + 2) * 4;
     b += 4 + (size_t)STAT_SHARDS * NSTAT * 8 + ((nc + 1 + SCAN_TILE - 1) / SCAN_TILE) * 8;
@@ -1004,7 +1014,9 @@ void batch_fill(pf2_handle_s* h, const double* field, BatchSim<DIM>& S, int* gri
     grid[BK_REFRESH] = grid_for(std::max((long long)nc * h->nd, (long long)h->n_dofs), 256, 4096);
     grid[BK_MOVE] = grid_for(h->n_est, 256, 8192);
     for (int r = 0; r <= MOVE_ROUNDS; ++r) grid[BK_SEARCH0 + r] = round_grid(qgrid, r);
-    const bool small = census_small_row(S.pp.nparts_in, S.pp.max_per_part);
+    // (batched steps keep rows of GCAP_L for lattices of more than 16 parts:
+    // they have no second census pass)
+    const bool small = S.pp.nparts_in <= 16;
     grid[BK_COUNT] = small ? cell_grid(nc, 8192) : 0;
     grid[BK_COUNT_L] = small ? 0 : cell_grid(nc, 8192);
     grid[BK_BIG] = 32;
@@ -1279,7 +1291,7 @@ int pf2_create_in(const pf2_mesh* mesh, const pf2_params* params, long long capa
         dalloc(&h->dVold, (size_t)nd * dim) || dalloc(&h->dtmp, (size_t)nd * dim) || dalloc(&h->off[0], nc + 1) ||
         dalloc(&h->off[1], nc + 1) || dalloc(&h->arr_cnt, nc + 1) ||
         dalloc(&h->arr_slot, (size_t)nc * h->aslots) || dalloc(&h->ovf_head, nc) || dalloc(&h->fs_cnt, nc + 1) || dalloc(&h->fs_off, nc + 1) || dalloc(&h->hitmask, nc) || dalloc(&h->nsurv, nc) ||
-        dalloc(&h->partial, ndc * (dim + 1)) || dalloc(&h->counters, 8) || dalloc(&h->big, nc + 1) ||
+        dalloc(&h->partial, ndc * (dim + 1)) || dalloc(&h->counters, 8) || dalloc(&h->big, nc + 1) || dalloc(&h->big2, nc + 1) ||
         dalloc(&h->qcount, (size_t)(MOVE_ROUNDS + 2) * (QSHARDS + 1) * QSTRIDE) ||
         dalloc(&h->qprefix, (size_t)(MOVE_ROUNDS + 2) * (QSHARDS + 2)) ||
         dalloc(&h->pcounter, 1) || dalloc(&h->stats, STAT_SHARDS * NSTAT) ||
@@ -1316,6 +1328,7 @@ int pf2_create_in(const pf2_mesh* mesh, const pf2_params* params, long long capa
         hipMemsetAsync(h->fs_cnt, 0, sizeof(unsigned long long) * (nc + 1), h->stream) ||
         hipMemsetAsync(h->counters, 0, sizeof(int) * 8, h->stream) ||
         hipMemsetAsync(h->big, 0, sizeof(int) * (nc + 1), h->stream) ||
+        hipMemsetAsync(h->big2, 0, sizeof(int) * (nc + 1), h->stream) ||
         hipMemsetAsync(h->qcount, 0, sizeof(int) * (MOVE_ROUNDS + 2) * (QSHARDS + 1) * QSTRIDE, h->stream) ||
         hipMemsetAsync(h->pcounter, 0, sizeof(uint32_t), h->stream) ||
         hipMemsetAsync(h->stats, 0, sizeof(unsigned long long) * STAT_SHARDS * NSTAT, h->stream))
@@ -1342,7 +1355,7 @@ int pf2_destroy(pf2_handle h) {
                     h->dftol, h->dcellX, h->dcell_star, h->dcell_nbr, h->dvbnd, h->dvdof, h->dcellV, h->dcellDV, h->dV, h->dVold, h->dtmp, h->dV32,
                     h->dX0, h->dW, h->dcmoved,
                     h->dVold32, h->dgeo32, h->off[0], h->off[1], h->arr_cnt, h->arr_slot, h->ovf_head, h->fs_cnt,
-                    h->fs_off, h->hitmask, h->nsurv, h->partial, h->counters, h->big, h->qcount, h->qprefix,
+                    h->fs_off, h->hitmask, h->nsurv, h->partial, h->counters, h->big, h->big2, h->qcount, h->qprefix,
                     h->pcounter, h->stats, h->scan_tmp, h->scan_tiles};
     if (!h->in_block)
         for (void* p : ptrs)
