@@ -248,6 +248,7 @@ int cell_grid(int n_cells, int cap = 4096) {
 }
 
 int pack_fields(pf2_handle_s* h);
+int ensure_packed(pf2_handle_s* h);
 
 // bound of the per-cell particle count after a census with reseed
 long long fuse_bound(const pf2_handle_s* h) {
@@ -514,6 +515,9 @@ int regroup(pf2_handle_s* h, int reseed, bool fuse_sums = false, bool defer_fina
     }
     if (int r = scan(h, h->fs_cnt, h->fs_off, nc + 1)) return r;
     const bool two_pass = !fuse_sums && h->omap && nc < (1 << 25);
+    // the one-pass write of an FP64 handle reads the per-cell copies of V
+    if (!two_pass && !mixed(h))
+        if (int r = ensure_packed(h)) return r;
     if (two_pass) {
         // per-cell map, then the output positions in order (pf2_kernels.hpp)
         Timed t(h, K_CELL_MAP);
@@ -540,11 +544,14 @@ int regroup(pf2_handle_s* h, int reseed, bool fuse_sums = false, bool defer_fina
         Timed t(h, K_CELL_GATHER);
         const dim3 g(grid_for(h->n_est, 256 * GU, 65535));   // one pass of GU positions per thread
         DevParticles a = dev_particles(h, h->cur), b = dev_particles(h, 1 - h->cur);
-        // (an FP32-mixed handle's FP64 regroup gathers V through the DoF map)
-        if (!h->dcellV && h->deg == 2)
+        // (an FP32-mixed handle's FP64 regroup, and a regroup after a
+        // projection left the per-cell copies stale, gather V through the
+        // DoF map: the same values)
+        const bool gather_v = !h->dcellV || !h->packed;
+        if (gather_v && h->deg == 2)
             hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cell_gather<DIM, true, 2>), g, dim3(256), 0, h->stream,
                                dev_mesh<DIM>(h), a, b, pp, w, h->omap);
-        else if (!h->dcellV)
+        else if (gather_v)
             hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cell_gather<DIM, true>), g, dim3(256), 0, h->stream,
                                dev_mesh<DIM>(h), a, b, pp, w, h->omap);
         else if (h->deg == 2)
@@ -1337,8 +1344,8 @@ int pf2_create_in(const pf2_mesh* mesh, const pf2_params* params, long long capa
         hipSuccess)
         return bail(fail(PF2_E_HIP, "scan sizing failed"));
     if (dalloc((char**)&h->scan_tmp, h->scan_tmp_bytes)) return bail(PF2_E_HIP);
-    if (h->dim == 2) launch_geo<2>(h->stream, dev_mesh<2>(h), h->dgeo32, h->dftol, h->dcellX, h->dcmoved);
-    else launch_geo<3>(h->stream, dev_mesh<3>(h), h->dgeo32, h->dftol, h->dcellX, h->dcmoved);
+    if (h->dim == 2) launch_geo<2>(h->stream, dev_mesh<2>(h), mixed(h) ? h->dgeo32 : nullptr, h->dftol, h->dcellX, h->dcmoved);
+    else launch_geo<3>(h->stream, dev_mesh<3>(h), mixed(h) ? h->dgeo32 : nullptr, h->dftol, h->dcellX, h->dcmoved);
     if (pack_fields(h)) return bail(PF2_E_HIP);
     if (hipStreamSynchronize(h->stream) != hipSuccess) return bail(fail(PF2_E_HIP, "initialisation sync failed"));
     h->bytes = g_alloc_bytes - bytes0;
@@ -1817,8 +1824,8 @@ int pf2_update_vertices(pf2_handle h, const double* vertices, int recompute_maps
         compute_star_dirs(h);
         if (int r = upload_star_dirs(h)) return r;
     }
-    if (h->dim == 2) launch_geo<2>(h->stream, dev_mesh<2>(h), h->dgeo32, h->dftol, h->dcellX, h->dcmoved);
-    else launch_geo<3>(h->stream, dev_mesh<3>(h), h->dgeo32, h->dftol, h->dcellX, h->dcmoved);
+    if (h->dim == 2) launch_geo<2>(h->stream, dev_mesh<2>(h), mixed(h) ? h->dgeo32 : nullptr, h->dftol, h->dcellX, h->dcmoved);
+    else launch_geo<3>(h->stream, dev_mesh<3>(h), mixed(h) ? h->dgeo32 : nullptr, h->dftol, h->dcellX, h->dcmoved);
     h->geo_updates += 1;
     HIPC(hipGetLastError());
     HIPC(hipStreamSynchronize(h->stream));
@@ -1830,7 +1837,12 @@ int pf2_update_vertices(pf2_handle h, const double* vertices, int recompute_maps
 int move_mesh_impl(pf2_handle_s* h, const double* vertices, int recompute_maps, const double* scale) {
     if (!h->params.keep_velocity_ext)
         return fail(PF2_E_STATE, "relocation without advection needs keep_velocity_ext");
-    if (int r = ensure_packed(h)) return r;
+    // the FP64 re-sort reads no velocity field, and its regroup takes the
+    // velocities of new particles from V through the DoF map when the
+    // per-cell copies are stale (after a projection); the FP32-mixed path
+    // regroups its FP32 state from the FP32 copies
+    if (mixed(h))
+        if (int r = ensure_packed(h)) return r;
     PBufs& b = h->buf[h->cur];
     const int grid = grid_for(h->n_est, 256, 4096);
     if (mixed(h)) {
@@ -1859,8 +1871,8 @@ int move_mesh_impl(pf2_handle_s* h, const double* vertices, int recompute_maps, 
         compute_star_dirs(h);
         if (int r = upload_star_dirs(h)) return r;
     }
-    if (h->dim == 2) launch_geo<2>(h->stream, dev_mesh<2>(h), h->dgeo32, h->dftol, h->dcellX, h->dcmoved);
-    else launch_geo<3>(h->stream, dev_mesh<3>(h), h->dgeo32, h->dftol, h->dcellX, h->dcmoved);
+    if (h->dim == 2) launch_geo<2>(h->stream, dev_mesh<2>(h), mixed(h) ? h->dgeo32 : nullptr, h->dftol, h->dcellX, h->dcmoved);
+    else launch_geo<3>(h->stream, dev_mesh<3>(h), mixed(h) ? h->dgeo32 : nullptr, h->dftol, h->dcellX, h->dcmoved);
     h->geo_updates += 1;
     DevParams pp = dev_params(h);
     DevWork w = dev_work(h);

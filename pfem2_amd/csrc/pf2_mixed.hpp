@@ -156,7 +156,8 @@ __global__ void k_ftol(DevMesh<DIM> m, double* __restrict__ ftol, double* __rest
 
 template <int DIM>
 void launch_geo(hipStream_t s, DevMesh<DIM> m, float* geo, double* ftol, double* cellX, unsigned char* cmoved) {
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_geo32<DIM>), dim3((m.n_cells + 255) / 256), dim3(256), 0, s, m, geo);
+    // geo: the FP32 geometry, for FP32-mixed handles only
+    if (geo) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_geo32<DIM>), dim3((m.n_cells + 255) / 256), dim3(256), 0, s, m, geo);
     hipLaunchKernelGGL(HIP_KERNEL_NAME(k_ftol<DIM>), dim3((m.n_cells + 255) / 256), dim3(256), 0, s, m, ftol, cellX,
                        cmoved);
 }
