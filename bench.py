@@ -144,14 +144,23 @@ def kernel_bytes(name, info, m, dim=2, vext=False):
         return reads + writes + mesh + (n0 * dim * 8 if vext else 0)
     if name == "k_resort":
         # own-cell test of every particle after a mesh move: cell, id, xi in,
-        # destination and part key out; the q[1] it parks for k_resort_search
+        # destination and part key out; the q[2] it parks for k_resort_search
         # also read x and vext and store the parking record (x, xi,
         # velocity_ext, substep, queue entry); per cell: vertices, moved flag
-        return n0 * (4 + 4 + dim * 8 + 8) + q[1] * (2 * dim * 8 + 3 * dim * 8 + 4 + 8) + nc * (nv * dim * 8 + 1)
+        return n0 * (4 + 4 + dim * 8 + 8) + q[2] * (2 * dim * 8 + 3 * dim * 8 + 4 + 8) + nc * (nv * dim * 8 + 1)
+    if name == "k_resort32":
+        # the FP32 state: cell, id, FP32 reference point in; destination and
+        # part key out; the q[2] parked: FP64 position in, substep flag and
+        # queue entry out
+        return n0 * (4 + 4 + dim * 4 + 8) + q[2] * (dim * 8 + 4 + 8) + nc * (nv * dim * 8 + 1)
+    if name == "k_resort_search32":
+        # per queued particle: entry, FP64 position and velocity_ext, FP32
+        # reference point, vertices in; reference point, destination, key out
+        return int(q[2] * (8 + 4 + 2 * dim * 8 + dim * 4 + nv * dim * 8 + dim * 4 + 8))
     if name == "k_resort_search":
-        # the relocations of the re-sort (queue 1): entry, record and vertices
+        # the relocations of the re-sort (q[2]): entry, record and vertices
         # in; x, xi, destination and part key out
-        return int(q[1] * (8 + 4 + 3 * dim * 8 + nv * dim * 8 + 2 * dim * 8 + 8)) if len(q) > 1 else None
+        return int(q[2] * (8 + 4 + 3 * dim * 8 + nv * dim * 8 + 2 * dim * 8 + 8))
     if name.startswith("k_search_resume"):
         # one launch per relocation round; round r handles q[r] particles
         mm = re.match(r"(k_search_resume(?:32)?)(?:_r(\d))?$", name)

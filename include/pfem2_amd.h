@@ -245,9 +245,11 @@ int pf2_particle_counter(pf2_handle h, uint32_t* count);
 int pf2_stats(pf2_handle h, unsigned long long* out5);
 /* Work counts of the last pf2_move_particles / pf2_step (no reference
  * counterpart; diagnostics): [0] particles before the step, [1] particles
- * after it, [2..4] particles queued for the vertex-star search in each
- * relocation round (the FP64 path searches in one round, the FP32-mixed
- * path in two; unused rounds read 0), [5] particles that changed cell.
+ * after it, [2..3] particles queued for the vertex-star search in each
+ * relocation round of the last move (the FP64 path searches in one round,
+ * the FP32-mixed path in two; an unused round reads 0), [4] particles that
+ * the last re-sort after a mesh move relocated by a star search, [5]
+ * particles that changed cell in the last regroup.
  * Returns the number of entries written (at most cap). */
 int pf2_step_info(pf2_handle h, long long* out, int cap);
 int pf2_synchronize(pf2_handle h);

@@ -397,7 +397,9 @@ class ParticleStage:
 
     def step_info(self):
         """Work counts of the last move: particles before/after, particles queued
-        for the vertex-star search per relocation round, particles that changed cell."""
+        for the vertex-star search per relocation round (queued[0], queued[1];
+        queued[2]: the last re-sort after a mesh move), particles that changed
+        cell."""
         out = np.zeros(8, dtype=np.int64)
         k = self._lib.pf2_step_info(self._h, _ptr(out), 8)
         if k < 0:

@@ -584,7 +584,7 @@ __device__ __forceinline__ int relocate(const DevMesh<DIM>& m, int c, const doub
 #define PF2_MOVE_ROUNDS 1
 #endif
 constexpr int MOVE_ROUNDS = PF2_MOVE_ROUNDS;
-static_assert(MOVE_ROUNDS >= 1 && MOVE_ROUNDS <= 2, "one or two parking rounds");
+static_assert(MOVE_ROUNDS == 1, "one parking round (two measured slower)");
 // the queue segments of the refilling search (one cursor each, on its own
 // cache line: a single cursor for the whole grid serialises the wavefronts'
 // atomics) and the idle lanes a wavefront gathers before it refills
@@ -594,8 +594,10 @@ constexpr int REFILL_MIN = 16;
 // path's last round, so no search pushes to it; the counters are cleared
 // with the other queue counters after each regroup
 constexpr int CURSOR_ROUND = MOVE_ROUNDS + 1;
-// the queue of the FP64 re-sort's relocations (the FP64 move uses queue 0)
-constexpr int RESORT_QUEUE = 1;
+// the queue of the re-sort's relocations after a mesh move (both paths)
+constexpr int RESORT_QUEUE = MOVE_ROUNDS + 2;
+// queues (rounds) of the counters: the moves' rounds, the cursors, the re-sort
+constexpr int NQROUNDS = MOVE_ROUNDS + 3;
 
 // Queues of parked particles, one per round, ping-ponging between two
 // buffers.  A queue is QSHARDS shards of shard_cap entries plus an overflow
@@ -606,7 +608,7 @@ constexpr int QSTRIDE = 32;   // ints between shard counters
 
 struct DevQueues {
     int2* data[2];        // ping-pong buffers of (particle, cell): QSHARDS * shard_cap + overflow
-    int* count;           // (MOVE_ROUNDS + 2) rounds x (QSHARDS + 1) counters, QSTRIDE apart
+    int* count;           // NQROUNDS rounds x (QSHARDS + 1) counters, QSTRIDE apart
     int* prefix;          // per round: QSHARDS + 2 ints (exclusive prefix, overflow start, total)
     int shard_cap;
     double* park_vext;    // DIM x cap: velocity_ext of a parked particle
@@ -1836,7 +1838,7 @@ __device__ __forceinline__ void finalize_block(int n_cells, const DevWork& w, in
     static_assert(QSHARDS == STAT_SHARDS, "one thread per stat row");
     __shared__ int s_arr;
     const int t = threadIdx.x;
-    for (int r = 0; r <= MOVE_ROUNDS + 1; ++r) {
+    for (int r = 0; r < NQROUNDS; ++r) {
         queue_count[((size_t)r * (QSHARDS + 1) + t) * QSTRIDE] = 0;
         if (t == 0) queue_count[((size_t)r * (QSHARDS + 1) + QSHARDS) * QSTRIDE] = 0;
     }

@@ -46,7 +46,7 @@ enum KernelId {
     K_MOVE = 0, K_GROUP, K_CELL_COUNT, K_CELL_WRITE, K_SCAN, K_FINALIZE, K_PARTIALS, K_NODE, K_WALL,
     K_SEED, K_MISC, K_MOVE32, K_CELL_WRITE32, K_PARTIALS32, K_SEARCH, K_SEARCH32, K_QPREFIX,
     K_SEARCH_R1, K_SEARCH_R2, K_SEARCH32_R1, K_SEARCH32_R2, K_CELL_BIG, K_RESORT, K_RESORT32, K_CELL_MAP,
-    K_CELL_GATHER, K_CELL_GATHER32, K_RESORT_SEARCH, K_NKERNELS
+    K_CELL_GATHER, K_CELL_GATHER32, K_RESORT_SEARCH, K_RESORT_SEARCH32, K_NKERNELS
 };
 const char* kKernelNames[K_NKERNELS] = {"k_move", "k_group", "k_cell_count", "k_cell_write", "scan",
                                         "k_finalize", "k_cell_partials", "k_node", "wall_fixup",
@@ -54,7 +54,8 @@ const char* kKernelNames[K_NKERNELS] = {"k_move", "k_group", "k_cell_count", "k_
                                         "k_search_resume", "k_search_resume32", "k_queue_prefix",
                                         "k_search_resume_r1", "k_search_resume_r2", "k_search_resume32_r1",
                                         "k_search_resume32_r2", "k_cell_big", "k_resort", "k_resort32",
-                                        "k_cell_map", "k_cell_gather", "k_cell_gather32", "k_resort_search"};
+                                        "k_cell_map", "k_cell_gather", "k_cell_gather32", "k_resort_search",
+                                        "k_resort_search32"};
 
 // bytes obtained through dalloc on this thread (a handle's footprint is the
 // difference across its creation)
@@ -870,7 +871,7 @@ size_t estimate_bytes(int dim, long long nc, long long nvt, long long nd, long l
     b += nd * dim * 8 * 3;                                                              // V, V_old, staging
     b += (nc + 1) * 4 * 10 + nc * 4 * (1 + arrival_slots(dim, p));                     // per-cell work, arrival slots
     b += nc * 8 + nc * 4 + ndc * (dim + 1) * 8 + 32;                                     // census, partials, counters
-    b += (size_t)(MOVE_ROUNDS + 2) * (QSHARDS + 1) * QSTRIDE * 4 + (size_t)(MOVE_ROUNDS + 2) * (QSHARDS + 2) * 4;
+    b += (size_t)NQROUNDS * (QSHARDS + 1) * QSTRIDE * 4 + (size_t)NQROUNDS * (QSHARDS + 2) * 4;
     b += 4 + (size_t)STAT_SHARDS * NSTAT * 8 + ((nc + 1 + SCAN_TILE - 1) / SCAN_TILE) * 8;
     b += ndc * dim * 4 * 2 + nc * geo32_floats(dim) * 4;                                 // FP32 fields, geometry
     long long per = 4 + 4 + 3 * dim * 8;
@@ -1299,8 +1300,8 @@ int pf2_create_in(const pf2_mesh* mesh, const pf2_params* params, long long capa
         dalloc(&h->off[1], nc + 1) || dalloc(&h->arr_cnt, nc + 1) ||
         dalloc(&h->arr_slot, (size_t)nc * h->aslots) || dalloc(&h->ovf_head, nc) || dalloc(&h->fs_cnt, nc + 1) || dalloc(&h->fs_off, nc + 1) || dalloc(&h->hitmask, nc) || dalloc(&h->nsurv, nc) ||
         dalloc(&h->partial, ndc * (dim + 1)) || dalloc(&h->counters, 8) || dalloc(&h->big, nc + 1) || dalloc(&h->big2, nc + 1) ||
-        dalloc(&h->qcount, (size_t)(MOVE_ROUNDS + 2) * (QSHARDS + 1) * QSTRIDE) ||
-        dalloc(&h->qprefix, (size_t)(MOVE_ROUNDS + 2) * (QSHARDS + 2)) ||
+        dalloc(&h->qcount, (size_t)NQROUNDS * (QSHARDS + 1) * QSTRIDE) ||
+        dalloc(&h->qprefix, (size_t)NQROUNDS * (QSHARDS + 2)) ||
         dalloc(&h->pcounter, 1) || dalloc(&h->stats, STAT_SHARDS * NSTAT) ||
         dalloc(&h->scan_tiles, (size_t)(nc + 1 + SCAN_TILE - 1) / SCAN_TILE))
         return bail(PF2_E_HIP);
@@ -1336,7 +1337,7 @@ int pf2_create_in(const pf2_mesh* mesh, const pf2_params* params, long long capa
         hipMemsetAsync(h->counters, 0, sizeof(int) * 8, h->stream) ||
         hipMemsetAsync(h->big, 0, sizeof(int) * (nc + 1), h->stream) ||
         hipMemsetAsync(h->big2, 0, sizeof(int) * (nc + 1), h->stream) ||
-        hipMemsetAsync(h->qcount, 0, sizeof(int) * (MOVE_ROUNDS + 2) * (QSHARDS + 1) * QSTRIDE, h->stream) ||
+        hipMemsetAsync(h->qcount, 0, sizeof(int) * NQROUNDS * (QSHARDS + 1) * QSTRIDE, h->stream) ||
         hipMemsetAsync(h->pcounter, 0, sizeof(uint32_t), h->stream) ||
         hipMemsetAsync(h->stats, 0, sizeof(unsigned long long) * STAT_SHARDS * NSTAT, h->stream))
         return bail(fail(PF2_E_HIP, "initialisation failed"));
@@ -1880,14 +1881,27 @@ int move_mesh_impl(pf2_handle_s* h, const double* vertices, int recompute_maps, 
     const int skip = h->geo_updates == 1;
     h->geo_updates = 0;
     if (mixed(h)) {
-        Timed t(h, K_RESORT32);
         h->fp64_valid = false;
-        if (h->dim == 2)
-            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_resort32<2>), dim3(grid), dim3(256), 0, h->stream, dev_mesh<2>(h),
-                               b.cell, b.id, b.x, b.vext, rec32(b), h->cap, pp, w, skip);
-        else
-            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_resort32<3>), dim3(grid), dim3(256), 0, h->stream, dev_mesh<3>(h),
-                               b.cell, b.id, b.x, b.vext, rec32(b), h->cap, pp, w, skip);
+        DevQueues qu = dev_queues(h);
+        const int rgrid = grid_for(h->n_est, 256, 16384), sgrid = grid_for(h->n_est / 2 + 1, 256, 2048);
+        {
+            Timed t(h, K_RESORT32);
+            if (h->dim == 2)
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_resort32<2>), dim3(rgrid), dim3(256), 0, h->stream,
+                                   dev_mesh<2>(h), b.cell, b.id, b.x, rec32(b), h->cap, pp, w, qu, skip);
+            else
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_resort32<3>), dim3(rgrid), dim3(256), 0, h->stream,
+                                   dev_mesh<3>(h), b.cell, b.id, b.x, rec32(b), h->cap, pp, w, qu, skip);
+        }
+        {
+            Timed t(h, K_RESORT_SEARCH32);
+            if (h->dim == 2)
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_resort_search32<2>), dim3(sgrid), dim3(256), 0, h->stream,
+                                   dev_mesh<2>(h), b.cell, b.x, b.vext, rec32(b), h->cap, pp, w, qu);
+            else
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_resort_search32<3>), dim3(sgrid), dim3(256), 0, h->stream,
+                                   dev_mesh<3>(h), b.cell, b.x, b.vext, rec32(b), h->cap, pp, w, qu);
+        }
     } else if (int r = h->dim == 2 ? resort64<2>(h, skip) : resort64<3>(h, skip)) {
         return r;
     }
@@ -2048,7 +2062,7 @@ int pf2_stats(pf2_handle h, unsigned long long* out5) {
 
 int pf2_step_info(pf2_handle h, long long* out, int cap) {
     if (!h || !out) return fail(PF2_E_ARG, "NULL argument");
-    std::vector<int> pref((size_t)(MOVE_ROUNDS + 2) * (QSHARDS + 2));
+    std::vector<int> pref((size_t)NQROUNDS * (QSHARDS + 2));
     int cnt[4];
     std::vector<int> offs(2);
     HIPC(hipMemcpyAsync(pref.data(), h->qprefix, pref.size() * 4, hipMemcpyDeviceToHost, h->stream));
@@ -2058,10 +2072,10 @@ int pf2_step_info(pf2_handle h, long long* out, int cap) {
     long long vals[6] = {};
     vals[0] = cnt[3];
     vals[1] = cnt[0];
-    // queued per round (the FP64 path searches in one round; its queue 1
-    // holds the relocations of the last re-sort after a mesh move)
-    for (int r = 0; r <= (mixed(h) ? MOVE_ROUNDS : RESORT_QUEUE); ++r)
-        vals[2 + r] = pref[(size_t)r * (QSHARDS + 2) + QSHARDS + 1];
+    // queued per round of the last move (the FP64 path searches in one
+    // round), then the relocations of the last re-sort after a mesh move
+    for (int r = 0; r <= (mixed(h) ? MOVE_ROUNDS : 0); ++r) vals[2 + r] = pref[(size_t)r * (QSHARDS + 2) + QSHARDS + 1];
+    vals[4] = pref[(size_t)RESORT_QUEUE * (QSHARDS + 2) + QSHARDS + 1];
     vals[5] = arrivals;
     int k = 0;
     for (; k < 6 && k < cap; ++k) out[k] = vals[k];

@@ -1293,32 +1293,85 @@ __global__ void k_pos32(DevMesh<DIM> m, const int* __restrict__ pcell, Rec32 R, 
     }
 }
 
-// k_resort for the FP32 state: own-cell test and relocation of the fixed
-// location x on the moved mesh in FP64, reference point stored in FP32.
-// skip_static (as in k_resort): a particle of a cell that kept its vertices,
-// advected in it since (ids up to the counter of the last advection), keeps
-// its reference point and cell
+// FP32-mixed re-sort after a mesh move, with the FP64 positions of k_pos32:
+// the own-cell test in FP64 (2-D: inverse_map's closed form; the rest of the
+// test is left pending, as in the FP64 re-sort), and only the reference
+// coordinate of a particle that stays is stored back (clamped FP32).  A
+// particle that fails the test is queued for k_resort_search32 (its x,
+// reference point and velocity_ext stay where they are).  skip_static as in
+// k_resort: a particle of a cell that kept its vertices, whose xi came from
+// the local step or an inverse map in that cell, keeps its record.
 template <int DIM>
 __global__ void __launch_bounds__(256) k_resort32(DevMesh<DIM> m, const int* __restrict__ pcell,
                                                   const uint32_t* __restrict__ pid, const double* __restrict__ px,
-                                                  const double* __restrict__ pvext, Rec32 R, long long cap,
-                                                  DevParams pp, DevWork w, int skip_static) {
+                                                  Rec32 R, long long cap, DevParams pp, DevWork w, DevQueues qu,
+                                                  int skip_static) {
     constexpr int NV = Q1<DIM>::NV;
     const int n = state_dead(w.counters) ? 0 : w.counters[0];
-    const int lane = threadIdx.x & (WAVE - 1);
     const uint32_t id_exact = (uint32_t)w.counters[5];
-    int lost = 0, searches = 0;
-    for (int base = blockIdx.x * blockDim.x + (threadIdx.x - lane); base < n; base += gridDim.x * blockDim.x) {
+    const int lane = threadIdx.x & (WAVE - 1);
+    const WaveLoop L = stride_loop(n);
+    for (int base = L.begin; base < L.end; base += L.step) {
         const int i = base + lane;
-        const bool valid = i < n;
-        const int c0 = valid ? pcell[i] : -1;
-        int c = c0;
-        if (valid && skip_static && !m.cmoved[c0] && pid[i] <= id_exact) {
+        const bool valid = i < L.end;
+        const int c = valid ? pcell[i] : 0;
+        int st = 0;   // 0 stays, 1 relocate, 2 own-cell test pending
+        if (valid) {
             float xf[DIM];
             rec_load_xi<DIM>(R, i, xf);
-            w.dest[i] = c0;
-            w.pkey[i] = part_key32<DIM>(pp, xf);
-        } else if (valid) {
+            if (!(skip_static && !m.cmoved[c] && pid[i] <= id_exact)) {
+                double x[DIM], r[DIM], P[NV][DIM];
+#pragma unroll
+                for (int d = 0; d < DIM; ++d) x[d] = px[(size_t)d * cap + i];
+                load_vertices<DIM>(m, c, P);
+                if constexpr (DIM == 2) {
+                    st = 2;
+                    if (inverse_quadratic(P, x, r)) {
+                        const double eps = 1e-15;
+                        if (-eps <= r[1] && r[1] <= 1 + eps && -eps <= r[0] && r[0] <= 1 + eps)
+                            st = inside_unit<DIM>(r) ? 0 : 1;
+                    }
+                } else {
+                    st = inverse_map<DIM>(P, x, r, m.ftol[c], false) && inside_unit<DIM>(r) ? 0 : 1;
+                }
+                if (st == 0) {
+#pragma unroll
+                    for (int d = 0; d < DIM; ++d) {
+                        const float t = (float)r[d];
+                        xf[d] = t < 0.0f ? 0.0f : (t > 1.0f ? 1.0f : t);
+                    }
+                    rec_store_xi<DIM>(R, i, xf);
+                }
+            }
+            if (st == 0) {
+                w.dest[i] = c;
+                w.pkey[i] = part_key32<DIM>(pp, xf);
+            } else {
+                qu.park_s[i] = st == 2 ? PARK_NEWTON : 0;
+            }
+        }
+        queue_push(qu, RESORT_QUEUE, valid && st != 0, make_int2(i, c), L.shard);
+    }
+}
+
+// the relocations of the FP32-mixed re-sort (queue RESORT_QUEUE): a pending
+// own-cell test first, then the vertex-star rule with the stored
+// velocity_ext, in FP64; one search per particle, one queue entry per lane
+template <int DIM>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DIM == 2 ? 3 : 2))) k_resort_search32(DevMesh<DIM> m, const int* __restrict__ pcell, const double* __restrict__ px, const double* __restrict__ pvext, Rec32 R, long long cap, DevParams pp, DevWork w, DevQueues qu) {
+    constexpr int NV = Q1<DIM>::NV;
+    __shared__ int s_pref[QSHARDS + 2];
+    const int n = queue_prefix(qu, RESORT_QUEUE, s_pref);
+    const int lane = threadIdx.x & (WAVE - 1);
+    const WaveLoop L = range_loop(n);
+    int lost = 0, searches = 0;
+    for (int base = L.begin; base < L.end; base += L.step) {
+        const int j = base + lane;
+        const bool valid = j < L.end;
+        const int2 e = valid ? queue_get(qu, RESORT_QUEUE, s_pref, j) : make_int2(0, -1);
+        const int i = e.x;
+        int c = e.y;
+        if (valid) {
             double x[DIM], xi[DIM], vext[DIM], r[DIM], P[NV][DIM];
             float xf[DIM];
             rec_load_xi<DIM>(R, i, xf);
@@ -1329,16 +1382,16 @@ __global__ void __launch_bounds__(256) k_resort32(DevMesh<DIM> m, const int* __r
                 vext[d] = pvext ? pvext[(size_t)d * cap + i] : 0.0;
             }
             load_vertices<DIM>(m, c, P);
-            bool found = inverse_map<DIM>(P, x, r, m.ftol[c], false) && inside_unit<DIM>(r);
-            if (!found) {
+            const bool own = qu.park_s[i] == PARK_NEWTON && !clearly_outside<DIM>(P, x, m.ftol[c]) &&
+                             inverse_newton<DIM>(P, x, r) && inside_unit<DIM>(r);
+            if (!own) {
                 ++searches;
                 double Q[NV][DIM];
                 NoPrefetch pre;
                 c = relocate<DIM>(m, c, P, xi, vext, x, r, Q, pre);
-                found = c >= 0;
-                if (!found) ++lost;
+                if (c < 0) ++lost;
             }
-            if (found) {
+            if (c >= 0) {
 #pragma unroll
                 for (int d = 0; d < DIM; ++d) {
                     const float t = (float)r[d];
@@ -1349,7 +1402,7 @@ __global__ void __launch_bounds__(256) k_resort32(DevMesh<DIM> m, const int* __r
             }
             w.dest[i] = c;
         }
-        add_arrival(w, valid && c >= 0 && c != c0, c, i);
+        add_arrival(w, valid && c >= 0 && c != pcell[valid ? i : 0], c, i);
     }
     wave_stat(w.stats, 0, lost);
     wave_stat(w.stats, 3, searches);
