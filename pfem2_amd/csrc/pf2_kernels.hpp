@@ -1201,18 +1201,33 @@ __device__ __forceinline__ void cell_count_body(const Grid& G, int n_cells, cons
         // the group's lanes in its wavefront's ballot start at bit sh
         const int sh = (threadIdx.x & (WAVE - 1)) & ~(GS - 1);
         if (small) {
-            for (int j0 = 0; j0 < k; j0 += GS) {
-                const int j = j0 + gl;
-                int idx = -1;
-                unsigned long long kid = ~0ull;
+            // all loads of the row first (up to CAP / GS candidates per lane),
+            // then the compaction: the candidates' dependent loads overlap
+            constexpr int SU = CAP / GS;
+            int idxs[SU];
+            unsigned long long kids[SU];
+#pragma unroll
+            for (int u = 0; u < SU; ++u) {
+                const int j = u * GS + gl;
+                idxs[u] = -1;
+                kids[u] = ~0ull;
                 if (j < k) {
                     const int cidx = candidate(w, old_off, c, n_st, j);
+                    const int dc = j < n_st ? w.dest[cidx] : c;
+                    const unsigned long long kid = ((unsigned long long)(uint32_t)w.pkey[cidx] << 32) | pid[cidx];
                     // arrivals are listed for c because their destination is c
-                    if (j >= n_st || w.dest[cidx] == c) {
-                        idx = cidx;
-                        kid = ((unsigned long long)(uint32_t)w.pkey[cidx] << 32) | pid[cidx];
+                    if (dc == c) {
+                        idxs[u] = cidx;
+                        kids[u] = kid;
                     }
                 }
+            }
+#pragma unroll
+            for (int u = 0; u < SU; ++u) {
+                if (u * GS >= k) break;
+                const int j = u * GS + gl;
+                const int idx = idxs[u];
+                const unsigned long long kid = kids[u];
                 const unsigned mask = (unsigned)(__ballot(idx >= 0) >> sh) & 0xffffu;
                 const unsigned smask = (unsigned)(__ballot(idx >= 0 && j < n_st) >> sh) & 0xffffu;
                 if (idx >= 0) {
