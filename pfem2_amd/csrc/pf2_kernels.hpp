@@ -186,7 +186,8 @@ struct DevWork {
     double* partial;    // per cell: NV * (DIM + 1) projection sums
     int* counters;      // [0] n_particles, [1] arrivals of the last regroup, [2] overflow flags,
                         // [3] count before the last regroup, [4] overflow entries in use,
-                        // [5] id counter at the last advection (ids up to it hold an inverse-mapped xi)
+                        // [5] id counter at the last advection (ids up to it hold an inverse-mapped xi),
+                        // [6] slots of a pending layout (0: none; see k_cell_pend), [7] its segment end
     int* big;           // [0] number of cells listed for k_cell_big, then the cells
     int* big2;          // the same for the cells that the second census pass (rows of GCAP_L) lists
     uint32_t* particle_counter;
@@ -745,6 +746,9 @@ __device__ __forceinline__ int2 queue_get(const DevQueues& qu, int round, const 
 // A parked particle whose own-cell test k_move left to the search kernel
 // (flag in its parked substep index)
 constexpr int PARK_NEWTON = 1 << 30;
+// part key mark of a particle that a pending census kept (k_cell_pend); no
+// part key takes this value (keys use the low 24 bits)
+constexpr int PKEY_KEPT = 0x7fffffff;
 
 // Runs substeps until done (returns 0) or the own-cell test fails (returns
 // 1; s is the substep whose test failed).  pending: the test of substep s at
@@ -801,11 +805,13 @@ __device__ __forceinline__ int advance64(const DevMesh<DIM>& m, const double* __
 }
 
 // final bookkeeping of a particle whose substeps are complete (all lanes call)
+// (seg_end: particles from it on, new particles of a pending layout, are in
+// no cell's segment, so they arrive in their cell)
 template <int DIM>
 __device__ __forceinline__ void finish_particle(const DevParams& pp, const DevParticles& A, const DevWork& w,
                                                 bool valid, int i, int c, const double* x, const double* xi,
-                                                bool write_state) {
-    add_arrival(w, valid && c >= 0 && c != A.cell[valid ? i : 0], c, i);
+                                                bool write_state, int seg_end = 0x7fffffff) {
+    add_arrival(w, valid && c >= 0 && (c != A.cell[valid ? i : 0] || i >= seg_end), c, i);
     if (!valid) return;
     if (write_state) {
 #pragma unroll
@@ -842,7 +848,12 @@ __device__ __forceinline__ void move_body(const Grid& G, const DevMesh<DIM>& m, 
                                           const DevWork& w, const DevQueues& qu, int do_correct, int do_move) {
     constexpr int NV = Q1<DIM>::NV;
     constexpr int ND = QF<DIM, DEG>::ND;
-    const int n = state_dead(w.counters) ? 0 : w.counters[0];
+    // a pending layout (k_cell_pend): its slots, the current cell of a
+    // particle in dest, the census's survivors marked in pkey, new particles
+    // from the segment end on
+    const int pn = w.counters[6];
+    const int n = state_dead(w.counters) ? 0 : (pn > 0 ? pn : w.counters[0]);
+    const int seg_end = pn > 0 ? w.counters[7] : 0x7fffffff;
     const int lane = threadIdx.x & (WAVE - 1);
     // every particle that exists now ends this advection with xi from an
     // inverse map (the re-sort after a mesh move relies on it)
@@ -850,8 +861,18 @@ __device__ __forceinline__ void move_body(const Grid& G, const DevMesh<DIM>& m, 
     const WaveLoop L = stride_loop(n, G);
     for (int base = L.begin; base < L.end; base += L.step) {
         const int i = base + lane;
-        const bool valid = i < L.end;
-        const int c = valid ? A.cell[i] : 0;
+        bool valid = i < L.end;
+        int c = 0;
+        if (valid && pn > 0) {
+            c = w.dest[i];
+            if (i < seg_end && (c < 0 || w.pkey[i] != PKEY_KEPT)) {
+                // not kept by the pending census: gone (no cell, no arrival)
+                w.dest[i] = -1;
+                valid = false;
+            }
+        } else if (valid) {
+            c = A.cell[i];
+        }
         double x[DIM], xi[DIM], vext[DIM];
 #pragma unroll
         for (int d = 0; d < DIM; ++d) xi[d] = valid ? A.xi[(size_t)d * A.cap + i] : 0.5;
@@ -878,12 +899,12 @@ __device__ __forceinline__ void move_body(const Grid& G, const DevMesh<DIM>& m, 
             st = advance64<DIM, DEG, true>(m, V, pp, A, i, c, x, xi, vext, s, P, Vc, m.ftol[c]);
         }
         park_particle<DIM>(A, w, qu, 0, L.shard, valid && st != 0, i, c, x, xi, vext, st == 2 ? s | PARK_NEWTON : s);
-        finish_particle<DIM>(pp, A, w, valid && st == 0, i, c, x, xi, true);
+        finish_particle<DIM>(pp, A, w, valid && st == 0, i, c, x, xi, true, seg_end);
     }
 }
 
 template <int DIM, int DEG = 1>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DIM == 2 ? 3 : 2))) k_move(DevMesh<DIM> m, const double* __restrict__ V,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DIM == 2 ? (DEG == 1 ? 5 : 3) : 2))) k_move(DevMesh<DIM> m, const double* __restrict__ V,
                                               const double* __restrict__ Vold, DevParticles A, DevParams pp,
                                               DevWork w, DevQueues qu, int do_correct, int do_move) {
     move_body<DIM, DEG>(hw_grid(), m, V, Vold, A, pp, w, qu, do_correct, do_move);
@@ -914,6 +935,8 @@ __device__ __forceinline__ void search_body(const Grid& G, const DevMesh<DIM>& m
     const int seg = (int)(gw % (unsigned)nseg);
     const int s0 = (int)((long long)n * seg / nseg), s1 = (int)((long long)n * (seg + 1) / nseg);
     int* cursor = queue_count(qu, CURSOR_ROUND, seg);
+    // new particles of a pending layout (from its segment end on) arrive in their cell
+    const int seg_end = !RESORT && w.counters[6] > 0 ? w.counters[7] : 0x7fffffff;
     const int lane = threadIdx.x & (WAVE - 1);
     const unsigned long long below = (1ull << lane) - 1;
     int lost = 0, searches = 0;
@@ -1037,7 +1060,7 @@ __device__ __forceinline__ void search_body(const Grid& G, const DevMesh<DIM>& m
                 done = advance64<DIM, DEG>(m, V, pp, A, i, c, x, xi, vext, s, P, pre.Vc, m.ftol[c]) == 0;
             }
         }
-        finish_particle<DIM>(pp, A, w, done, i, c, x, xi, true);
+        finish_particle<DIM>(pp, A, w, done, i, c, x, xi, true, seg_end);
         if (done) have = false;
     }
     wave_stat(w.stats, 0, lost);
@@ -1849,7 +1872,8 @@ __global__ void __launch_bounds__(256) k_cell_gather(DevMesh<DIM> m, DevParticle
 // totals after the scans: particle count, id counter, statistics; resets the
 // queue counters for the next step (one workgroup of QSHARDS threads: k_finalize,
 // or the last workgroup of k_node in a fused step)
-__device__ __forceinline__ void finalize_block(int n_cells, const DevWork& w, int* queue_count) {
+__device__ __forceinline__ void finalize_block(int n_cells, const DevWork& w, int* queue_count,
+                                               bool keep_pending = false) {
     static_assert(QSHARDS == STAT_SHARDS, "one thread per stat row");
     __shared__ int s_arr;
     const int t = threadIdx.x;
@@ -1877,12 +1901,113 @@ __device__ __forceinline__ void finalize_block(int n_cells, const DevWork& w, in
     w.counters[3] = w.counters[0];   // count before the step (diagnostics)
     w.counters[0] = total;
     w.counters[4] = 0;
+    // a regroup consumes a pending layout; k_cell_pend has just set one up
+    if (!keep_pending) w.counters[6] = 0;
     w.big[0] = 0;
     w.big2[0] = 0;
 }
 
-__global__ void __launch_bounds__(QSHARDS) k_finalize(int n_cells, DevWork w, int* queue_count) {
-    finalize_block(n_cells, w, queue_count);
+__global__ void __launch_bounds__(QSHARDS) k_finalize(int n_cells, DevWork w, int* queue_count, int keep_pending) {
+    finalize_block(n_cells, w, queue_count, keep_pending != 0);
+}
+
+// ---------------------------------------------------------------------------
+// Pending layout.  After the census of a re-sort (relocation + reseed), the
+// particles are not moved into a new buffer: the census's survivors are
+// marked in place (part key PKEY_KEPT; their destination is their cell), its
+// new particles are created after the current segments (position seg_end +
+// the cell's spawn offset + rank, ids and values as k_cell_gather creates
+// them), and counters[6], [7] record the slots and seg_end.  The next move
+// takes a particle's cell from dest, skips the unmarked ones and makes the
+// new ones arrive in their cell, so its census regroups the pending layout
+// and the re-sort's regroup write is never done.  Any other use of the
+// particles first settles the layout (k_settle + a regroup without reseed).
+// Per cell, like k_cell_map: survivors, spawns, then the per-cell resets.
+template <int DIM, bool GATHER, int DEG = 1>
+__global__ void __launch_bounds__(256) k_cell_pend(DevMesh<DIM> m, const int* __restrict__ old_off, DevParticles A,
+                                                   DevParams pp, DevWork w) {
+    constexpr int NV = Q1<DIM>::NV;
+    constexpr int ND = QF<DIM, DEG>::ND;
+    if (state_dead(w.counters)) return;
+    const int nc = m.n_cells;
+    const int seg_end = old_off[nc];
+    const unsigned long long ftot = w.fs_off[nc];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        const long long slots = (long long)seg_end + (uint32_t)ftot;
+        if (slots > A.cap) atomicOr(&w.counters[2], 1);
+        w.counters[6] = slots > A.cap ? 0 : (int)slots;
+        w.counters[7] = seg_end;
+    }
+    const int g = threadIdx.x / GS, gl = threadIdx.x % GS;
+    for (int c = blockIdx.x * GROUPS + g; c < nc; c += gridDim.x * GROUPS) {
+        const int n_st = old_off[c + 1] - old_off[c];
+        const int na = w.arr_cnt[c];
+        const int kin = n_st + (na < w.aslots ? na : w.aslots);
+        const unsigned long long fo = w.fs_off[c];
+        const int nsurv = w.nsurv[c];
+        const int nfinal = (int)(w.fs_cnt[c] >> 32);
+        if (nsurv <= w.aslots) {
+            for (int t = gl; t < nsurv; t += GS) w.pkey[w.arr_slot[(size_t)c * w.aslots + t]] = PKEY_KEPT;
+        } else {
+            survivors_of(w, old_off, c, n_st, kin, nsurv, gl, [&](int idx, int) { w.pkey[idx] = PKEY_KEPT; });
+        }
+        if (nfinal > nsurv) {
+            const unsigned long long all = pp.nparts_in >= 64 ? ~0ull : ((1ull << pp.nparts_in) - 1);
+            const unsigned long long empty = all & ~w.hitmask[c];
+            for (int o = gl; o < pp.nparts_in; o += GS) {
+                if (!((empty >> o) & 1)) continue;
+                const int r = __popcll(empty & ((1ull << o) - 1));
+                const long long q = (long long)seg_end + (uint32_t)fo + r;
+                if (q >= A.cap) continue;   // (flagged above)
+                double xi[DIM], P[NV][DIM], Vc[ND][DIM], x[DIM], v[DIM];
+                lattice_point<DIM>(pp, o, xi);
+                load_cell<DIM, GATHER, DEG>(m, c, P, Vc);
+                forward<DIM>(P, xi, x);
+                interp<DIM, DEG>(Vc, xi, v);
+                A.cell[q] = c;
+                A.id[q] = *w.particle_counter + (uint32_t)fo + (uint32_t)r + 1u;
+                w.dest[q] = c;
+#pragma unroll
+                for (int d = 0; d < DIM; ++d) {
+                    A.x[(size_t)d * A.cap + q] = x[d];
+                    A.xi[(size_t)d * A.cap + q] = xi[d];
+                    A.v[(size_t)d * A.cap + q] = v[d];
+                    if (pp.keep_vext) A.vext[(size_t)d * A.cap + q] = 0.0;
+                }
+            }
+        }
+        if (gl == 0) {
+            w.arr_cnt[c] = 0;
+            w.ovf_head[c] = -1;
+        }
+    }
+}
+
+// settles a pending layout for a regroup without reseed: every kept or new
+// particle keeps its cell (destination) and gets its part key back, and the
+// ones outside their cell's segment arrive there; the others are gone
+template <int DIM>
+__global__ void k_settle(DevParticles A, DevParams pp, DevWork w) {
+    const int pn = w.counters[6];
+    const int seg_end = w.counters[7];
+    const int lane = threadIdx.x & (WAVE - 1);
+    for (int base = blockIdx.x * blockDim.x + (threadIdx.x - lane); base < pn; base += gridDim.x * blockDim.x) {
+        const int i = base + lane;
+        const bool valid = i < pn;
+        int c = -1;
+        if (valid) {
+            c = w.dest[i];
+            if (i < seg_end && (c < 0 || w.pkey[i] != PKEY_KEPT)) c = -1;
+            w.dest[i] = c;
+            if (c >= 0) {
+                double xi[DIM];
+#pragma unroll
+                for (int d = 0; d < DIM; ++d) xi[d] = A.xi[(size_t)d * A.cap + i];
+                w.pkey[i] = part_key<DIM>(pp, xi);
+            }
+        }
+        add_arrival(w, valid && c >= 0 && (c != A.cell[valid ? i : 0] || i >= seg_end), c, i);
+    }
 }
 
 // ---------------------------------------------------------------------------

@@ -46,7 +46,7 @@ enum KernelId {
     K_MOVE = 0, K_GROUP, K_CELL_COUNT, K_CELL_WRITE, K_SCAN, K_FINALIZE, K_PARTIALS, K_NODE, K_WALL,
     K_SEED, K_MISC, K_MOVE32, K_CELL_WRITE32, K_PARTIALS32, K_SEARCH, K_SEARCH32, K_QPREFIX,
     K_SEARCH_R1, K_SEARCH_R2, K_SEARCH32_R1, K_SEARCH32_R2, K_CELL_BIG, K_RESORT, K_RESORT32, K_CELL_MAP,
-    K_CELL_GATHER, K_CELL_GATHER32, K_RESORT_SEARCH, K_RESORT_SEARCH32, K_NKERNELS
+    K_CELL_GATHER, K_CELL_GATHER32, K_RESORT_SEARCH, K_RESORT_SEARCH32, K_CELL_PEND, K_SETTLE, K_NKERNELS
 };
 const char* kKernelNames[K_NKERNELS] = {"k_move", "k_group", "k_cell_count", "k_cell_write", "scan",
                                         "k_finalize", "k_cell_partials", "k_node", "wall_fixup",
@@ -55,7 +55,7 @@ const char* kKernelNames[K_NKERNELS] = {"k_move", "k_group", "k_cell_count", "k_
                                         "k_search_resume_r1", "k_search_resume_r2", "k_search_resume32_r1",
                                         "k_search_resume32_r2", "k_cell_big", "k_resort", "k_resort32",
                                         "k_cell_map", "k_cell_gather", "k_cell_gather32", "k_resort_search",
-                                        "k_resort_search32"};
+                                        "k_resort_search32", "k_cell_pend", "k_settle"};
 
 // bytes obtained through dalloc on this thread (a handle's footprint is the
 // difference across its creation)
@@ -110,6 +110,7 @@ struct pf2_handle_s {
     pf2_params params{};
     long long cap = 0;
     long long n_est = 0;   // host-side estimate of the particle count (grid sizes only)
+    bool pending = false;  // the particles are in a pending layout (k_cell_pend)
     hipStream_t stream = nullptr;
     // host copies
     std::vector<double> X;
@@ -485,8 +486,11 @@ int get_count(pf2_handle_s* h, long long* n) {
 // re-sort and check_empty_cells run in FP64).
 template <int DIM>
 int regroup(pf2_handle_s* h, int reseed, bool fuse_sums = false, bool defer_finalize = false,
-            bool fp64_state = false) {
+            bool fp64_state = false, bool pend = false) {
     const int nc = h->n_cells;
+    // a regroup consumes a pending layout (its census reads it, its
+    // finalize clears the device's record of it)
+    if (!pend) h->pending = false;
     DevParams pp = dev_params(h);
     DevWork w = dev_work(h);
     const int* old_off = h->off[h->cur_off];
@@ -515,6 +519,34 @@ int regroup(pf2_handle_s* h, int reseed, bool fuse_sums = false, bool defer_fina
                            h->buf[h->cur].id, pp, w, reseed, second ? 1 : 0);
     }
     if (int r = scan(h, h->fs_cnt, h->fs_off, nc + 1)) return r;
+    if (pend) {
+        // survivors marked and new particles placed in the current buffer;
+        // the next move or a settle regroups them (FP64 handles)
+        {
+            Timed t(h, K_CELL_PEND);
+            DevParticles a = dev_particles(h, h->cur);
+            const bool gv = !h->dcellV || !h->packed;
+            if (gv && h->deg == 2)
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cell_pend<DIM, true, 2>), dim3(cell_grid(nc)), dim3(256), 0,
+                                   h->stream, dev_mesh<DIM>(h), old_off, a, pp, w);
+            else if (gv)
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cell_pend<DIM, true>), dim3(cell_grid(nc)), dim3(256), 0,
+                                   h->stream, dev_mesh<DIM>(h), old_off, a, pp, w);
+            else if (h->deg == 2)
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cell_pend<DIM, false, 2>), dim3(cell_grid(nc)), dim3(256), 0,
+                                   h->stream, dev_mesh<DIM>(h), old_off, a, pp, w);
+            else
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cell_pend<DIM, false>), dim3(cell_grid(nc)), dim3(256), 0,
+                                   h->stream, dev_mesh<DIM>(h), old_off, a, pp, w);
+        }
+        {
+            Timed t(h, K_FINALIZE);
+            hipLaunchKernelGGL(k_finalize, dim3(1), dim3(QSHARDS), 0, h->stream, nc, w, h->qcount, 1);
+        }
+        HIPC(hipGetLastError());
+        h->pending = true;
+        return 0;
+    }
     const bool two_pass = !fuse_sums && h->omap && nc < (1 << 25);
     // the one-pass write of an FP64 handle reads the per-cell copies of V
     if (!two_pass && !mixed(h))
@@ -591,12 +623,33 @@ int regroup(pf2_handle_s* h, int reseed, bool fuse_sums = false, bool defer_fina
     }
     if (!defer_finalize) {
         Timed t(h, K_FINALIZE);
-        hipLaunchKernelGGL(k_finalize, dim3(1), dim3(QSHARDS), 0, h->stream, nc, w, h->qcount);
+        hipLaunchKernelGGL(k_finalize, dim3(1), dim3(QSHARDS), 0, h->stream, nc, w, h->qcount, 0);
     }
     HIPC(hipGetLastError());
     h->cur = 1 - h->cur;
     h->cur_off = 1 - h->cur_off;
     return 0;
+}
+
+// a pending layout regrouped without reseed: the same particles, in cell
+// order (k_settle, then the regroup)
+template <int DIM>
+int settle_dim(pf2_handle_s* h) {
+    DevParams pp = dev_params(h);
+    DevWork w = dev_work(h);
+    {
+        Timed t(h, K_SETTLE);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_settle<DIM>), dim3(grid_for(h->n_est + 1, 256, 16384)), dim3(256), 0,
+                           h->stream, dev_particles(h, h->cur), pp, w);
+    }
+    HIPC(hipGetLastError());
+    h->pending = false;
+    return regroup<DIM>(h, 0);
+}
+
+int settle(pf2_handle_s* h) {
+    if (!h->pending) return 0;
+    return h->dim == 2 ? settle_dim<2>(h) : settle_dim<3>(h);
 }
 
 template <int DIM>
@@ -959,7 +1012,11 @@ int step_graph(pf2_handle_s* h, const double* field) {
 }
 
 int run_step(pf2_handle_s* h, const double* field) {
-    if (h->graphs && !h->profiling) return step_graph(h, field);
+    if (h->graphs && !h->profiling) {
+        // (a step graph is captured on a settled layout)
+        if (int r = settle(h)) return r;
+        return step_graph(h, field);
+    }
     return step_body(h, field);
 }
 
@@ -1387,6 +1444,7 @@ int pf2_destroy(pf2_handle h) {
 
 int pf2_set_params(pf2_handle h, const pf2_params* params) {
     if (!h) return fail(PF2_E_ARG, "handle is NULL");
+    if (int r = settle(h)) return r;
     invalidate_graphs(h);
     if (int r = check_params(h->dim, params)) return r;
     if (params->precision != h->params.precision)
@@ -1424,6 +1482,7 @@ int pf2_download_field(pf2_handle h, double* V) {
 
 int pf2_seed_particles(pf2_handle h) {
     if (!h) return fail(PF2_E_ARG, "handle is NULL");
+    if (int r = settle(h)) return r;
     invalidate_graphs(h);
     if (int r = ensure_packed(h)) return r;
     long long n = 0;
@@ -1464,6 +1523,7 @@ int pf2_seed_particles(pf2_handle h) {
 
 int pf2_download_particles(pf2_handle h, int* cell, uint32_t* id, double* x, double* xi, double* v) {
     if (!h) return fail(PF2_E_ARG, "handle is NULL");
+    if (int r = settle(h)) return r;
     long long n = 0;
     if (int r = get_count(h, &n)) return r;
     if (int r = from_mixed(h)) return r;
@@ -1489,6 +1549,7 @@ int pf2_download_particles(pf2_handle h, int* cell, uint32_t* id, double* x, dou
 
 int pf2_insert_particles(pf2_handle h, int n, const int* cell, const double* xi, const double* v) {
     if (!h) return fail(PF2_E_ARG, "handle is NULL");
+    if (int r = settle(h)) return r;
     invalidate_graphs(h);
     if (int r = ensure_packed(h)) return r;
     if (n < 0 || (n > 0 && (!cell || !xi))) return fail(PF2_E_ARG, "invalid particle arrays");
@@ -1598,6 +1659,7 @@ int pf2_move_particles(pf2_handle h) {
 
 int pf2_distribute_particle_velocities_to_grid(pf2_handle h, double* V_out) {
     if (!h) return fail(PF2_E_ARG, "handle is NULL");
+    if (int r = settle(h)) return r;
     if (int r = (h->dim == 2 ? run_project<2>(h) : run_project<3>(h))) return r;
     if (V_out) return download_field(h, V_out);
     return 0;
@@ -1617,7 +1679,9 @@ int pf2_distribute_particle_velocities_to_grid_static(pf2_handle h, double* V_ou
     // no per-kernel timing here: its events would be created on demand
     const bool prof = h->profiling;
     h->profiling = false;
-    int r = h->dim == 2 ? run_project<2>(h, false, (double*)sp) : run_project<3>(h, false, (double*)sp);
+    // (a pending layout is settled with the handle's own buffers)
+    int r = settle(h);
+    if (!r) r = h->dim == 2 ? run_project<2>(h, false, (double*)sp) : run_project<3>(h, false, (double*)sp);
     if (!r && V_out_device) r = field_to_device(h, V_out_device);
     h->profiling = prof;
     return r;
@@ -1673,6 +1737,9 @@ int pf2_batch_destroy(pf2_batch b) {
 int pf2_batch_step(pf2_batch b, int n, const pf2_handle* hs, const double* const* fields) {
     if (!b || !hs || !fields || n < 1) return fail(PF2_E_ARG, "NULL argument");
     if (n > b->max_sims) return fail(PF2_E_ARG, "more simulations than the batch holds");
+    for (int i = 0; i < n; ++i)
+        if (hs[i])
+            if (int r = settle(hs[i])) return r;
     for (int i = 0; i < n; ++i) {
         if (!hs[i] || !fields[i]) return fail(PF2_E_ARG, "NULL handle or field");
         if (hs[i]->dim != hs[0]->dim || mixed(hs[i]) != mixed(hs[0]))
@@ -1700,6 +1767,7 @@ int pf2_step_with_field(pf2_handle h, const double* V_device) {
 
 int pf2_set_graphs(pf2_handle h, int on) {
     if (!h) return fail(PF2_E_ARG, "handle is NULL");
+    if (int r = settle(h)) return r;
     HIPC(hipStreamSynchronize(h->stream));
     invalidate_graphs(h);
     h->graphs = on != 0;
@@ -1776,6 +1844,7 @@ int pf2_set_walls(pf2_handle h, int n, const int* vertex, const int* dof, const 
 
 int pf2_sort_particles_into_subdomains_and_cells(pf2_handle h) {
     if (!h) return fail(PF2_E_ARG, "handle is NULL");
+    if (int r = settle(h)) return r;
     if (!h->params.keep_velocity_ext)
         return fail(PF2_E_STATE, "relocation without advection needs keep_velocity_ext");
     if (int r = from_mixed(h)) return r;
@@ -1794,6 +1863,7 @@ int pf2_sort_particles_into_subdomains_and_cells(pf2_handle h) {
 
 int pf2_check_empty_cells(pf2_handle h) {
     if (!h) return fail(PF2_E_ARG, "handle is NULL");
+    if (int r = settle(h)) return r;
     if (int r = from_mixed(h)) return r;
     DevParams pp = dev_params(h);
     DevWork w = dev_work(h);
@@ -1817,6 +1887,7 @@ int pf2_check_empty_cells(pf2_handle h) {
 
 int pf2_update_vertices(pf2_handle h, const double* vertices, int recompute_maps) {
     if (!h || !vertices) return fail(PF2_E_ARG, "NULL argument");
+    if (int r = settle(h)) return r;
     // positions of FP32-mixed particles are F_K(xi) on the mesh before the move
     if (int r = from_mixed(h)) return r;
     h->X.assign(vertices, vertices + (size_t)h->n_vertices * h->dim);
@@ -1838,6 +1909,7 @@ int pf2_update_vertices(pf2_handle h, const double* vertices, int recompute_maps
 int move_mesh_impl(pf2_handle_s* h, const double* vertices, int recompute_maps, const double* scale) {
     if (!h->params.keep_velocity_ext)
         return fail(PF2_E_STATE, "relocation without advection needs keep_velocity_ext");
+    if (int r = settle(h)) return r;
     // the FP64 re-sort reads no velocity field, and its regroup takes the
     // velocities of new particles from V through the DoF map when the
     // per-cell copies are stale (after a projection); the FP32-mixed path
@@ -1907,8 +1979,15 @@ int move_mesh_impl(pf2_handle_s* h, const double* vertices, int recompute_maps, 
     }
     HIPC(hipGetLastError());
     // relocation and reseed in one census: the same survivors, ids and
-    // counts as the re-sort's census followed by check_empty_cells'
-    return h->dim == 2 ? regroup<2>(h, 1) : regroup<3>(h, 1);
+    // counts as the re-sort's census followed by check_empty_cells'.  An
+    // FP64 handle stepping without graphs leaves its particles in a pending
+    // layout that the next move regroups (k_cell_pend): the re-sort's own
+    // regroup write is skipped.  (Room for the new particles after the
+    // current ones is checked on the estimate here and on the device.)
+    const long long nc = h->n_cells;
+    const bool pend = !mixed(h) && !h->graphs && h->omap && nc < (1 << 25) &&
+                      h->n_est + nc * dev_params(h).nparts_in <= h->cap;
+    return h->dim == 2 ? regroup<2>(h, 1, false, false, false, pend) : regroup<3>(h, 1, false, false, false, pend);
 }
 
 int pf2_move_mesh(pf2_handle h, const double* vertices, int recompute_maps) {
@@ -2031,6 +2110,7 @@ int pf2_n_global_particles(pf2_handle h, long long* n) {
 
 int pf2_cell_counts(pf2_handle h, int* counts) {
     if (!h || !counts) return fail(PF2_E_ARG, "NULL argument");
+    if (int r = settle(h)) return r;
     long long n;
     if (int r = get_count(h, &n)) return r;
     std::vector<int> off(h->n_cells + 1);

@@ -253,6 +253,40 @@ def test_moving_mesh_resort(fused):
         assert sg["lost"] == so["lost"] and sg["spawned"] == so["spawned"] and sg["deleted"] == so["deleted"]
 
 
+@pytest.mark.parametrize("q,max_per_part", [([3, 3], 3), ([5, 5], 2)])
+def test_steps_between_mesh_moves(q, max_per_part):
+    """Steps and fused mesh moves in turn, with no other call between them:
+    after a mesh move the particles stay in a pending layout that the next
+    step's move regroups (the survivors marked, the new particles after the
+    current ones).  The particles, counts and statistics stay bitwise equal
+    to the oracle's step, update_vertices, re-sort, check_empty_cells; the
+    5x5 lattice with MAX 2 also takes the census's second pass and big
+    cells."""
+    m = M.cylinder_channel(0.08, seed=26)
+    V = M.cylinder_flow(m, 1.0, eps=0.02, seed=27)
+    g, o = _pair(m, V, 0.004, 5, max_per_part, q, seed=28, keep_vext=True)
+    X0 = m.vertices.copy()
+    d = np.hypot(X0[:, 0] - 0.2, X0[:, 1] - 0.2)
+    w = np.clip(1.0 - (d - 0.05) / 0.15, 0.0, 1.0)
+    for s in range(5):
+        g.step()
+        o.step()
+        X = X0.copy()
+        X[:, 1] += 0.004 * np.sin(0.7 * (s + 1)) * w
+        g.move_mesh(X)
+        o.update_vertices(X)
+        o.sort_particles_into_subdomains_and_cells()
+        o.check_empty_cells()
+        assert g.n_global_particles() == o.n_global_particles(), s
+    Vg, Vo = g.step(download=True), o.step()
+    assert bitwise_equal(Vg, Vo)
+    assert_particles_match(g.particles(), o.particles(), REL, "steps between mesh moves", bitwise=True)
+    assert np.array_equal(g.cell_counts(), o.cell_counts())
+    assert g.particle_counter() == o.particle_counter()
+    sg, so = g.stats(), o.stats()
+    assert sg["lost"] == so["lost"] and sg["spawned"] == so["spawned"] and sg["deleted"] == so["deleted"]
+
+
 @pytest.mark.parametrize("case", ["insert_then_move", "two_updates", "reseed_then_move"])
 def test_moving_mesh_resort_keeps_only_exact_xi(case):
     """The re-sort keeps the reference coordinates of particles whose cell
