@@ -282,6 +282,25 @@ def test_steps_between_mesh_moves(q, max_per_part):
     assert bitwise_equal(Vg, Vo)
     assert_particles_match(g.particles(), o.particles(), REL, "steps between mesh moves", bitwise=True)
     assert np.array_equal(g.cell_counts(), o.cell_counts())
+    # after a mesh move, other uses of the pending layout: the projection
+    # alone, and correct then move as separate calls
+    X = X0.copy()
+    X[:, 1] -= 0.002 * w
+    g.move_mesh(X)
+    o.update_vertices(X)
+    o.sort_particles_into_subdomains_and_cells()
+    o.check_empty_cells()
+    assert bitwise_equal(g.distribute_particle_velocities_to_grid(), o.distribute_particle_velocities_to_grid())
+    X[:, 1] += 0.003 * w
+    g.move_mesh(X)
+    o.update_vertices(X)
+    o.sort_particles_into_subdomains_and_cells()
+    o.check_empty_cells()
+    for s_ in (g, o):
+        s_.correct_particles_velocities()
+        s_.move_particles()
+    assert bitwise_equal(g.distribute_particle_velocities_to_grid(), o.distribute_particle_velocities_to_grid())
+    assert np.array_equal(g.cell_counts(), o.cell_counts())
     assert g.particle_counter() == o.particle_counter()
     sg, so = g.stats(), o.stats()
     assert sg["lost"] == so["lost"] and sg["spawned"] == so["spawned"] and sg["deleted"] == so["deleted"]
