@@ -522,7 +522,16 @@ int regroup(pf2_handle_s* h, int reseed, bool fuse_sums = false, bool defer_fina
     if (pend) {
         // survivors marked and new particles placed in the current buffer;
         // the next move or a settle regroups them (FP64 handles)
-        {
+        if (mixed(h) && !fp64_state) {
+            Timed t(h, K_CELL_PEND);
+            const PBufs& a = h->buf[h->cur];
+            if (h->deg == 2)
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cell_pend32<DIM, 2>), dim3(cell_grid(nc)), dim3(256), 0,
+                                   h->stream, nc, old_off, a.cell, a.id, rec32(a), a.vext, h->dV32, h->cap, pp, w);
+            else
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cell_pend32<DIM>), dim3(cell_grid(nc)), dim3(256), 0,
+                                   h->stream, nc, old_off, a.cell, a.id, rec32(a), a.vext, h->dV32, h->cap, pp, w);
+        } else {
             Timed t(h, K_CELL_PEND);
             DevParticles a = dev_particles(h, h->cur);
             const bool gv = !h->dcellV || !h->packed;
@@ -639,8 +648,13 @@ int settle_dim(pf2_handle_s* h) {
     DevWork w = dev_work(h);
     {
         Timed t(h, K_SETTLE);
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_settle<DIM>), dim3(grid_for(h->n_est + 1, 256, 16384)), dim3(256), 0,
-                           h->stream, dev_particles(h, h->cur), pp, w);
+        const dim3 g(grid_for(h->n_est + 1, 256, 16384));
+        if (mixed(h))
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_settle32<DIM>), g, dim3(256), 0, h->stream, h->buf[h->cur].cell,
+                               rec32(h->buf[h->cur]), pp, w);
+        else
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_settle<DIM>), g, dim3(256), 0, h->stream, dev_particles(h, h->cur), pp,
+                               w);
     }
     HIPC(hipGetLastError());
     h->pending = false;
@@ -1979,13 +1993,13 @@ int move_mesh_impl(pf2_handle_s* h, const double* vertices, int recompute_maps, 
     }
     HIPC(hipGetLastError());
     // relocation and reseed in one census: the same survivors, ids and
-    // counts as the re-sort's census followed by check_empty_cells'.  An
-    // FP64 handle stepping without graphs leaves its particles in a pending
-    // layout that the next move regroups (k_cell_pend): the re-sort's own
-    // regroup write is skipped.  (Room for the new particles after the
+    // counts as the re-sort's census followed by check_empty_cells'.  A
+    // handle stepping without graphs leaves its particles in a pending
+    // layout that the next move regroups (k_cell_pend[32]): the re-sort's
+    // own regroup write is skipped.  (Room for the new particles after the
     // current ones is checked on the estimate here and on the device.)
     const long long nc = h->n_cells;
-    const bool pend = !mixed(h) && !h->graphs && h->omap && nc < (1 << 25) &&
+    const bool pend = !h->graphs && h->omap && nc < (1 << 25) &&
                       h->n_est + nc * dev_params(h).nparts_in <= h->cap;
     return h->dim == 2 ? regroup<2>(h, 1, false, false, false, pend) : regroup<3>(h, 1, false, false, false, pend);
 }

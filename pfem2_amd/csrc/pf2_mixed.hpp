@@ -508,10 +508,13 @@ __device__ __forceinline__ int advance32(const DevParams& pp, float h32, int i, 
     return 0;
 }
 
+// (seg_end: particles from it on, new particles of a pending layout, arrive
+// in their cell)
 template <int DIM>
 __device__ __forceinline__ void finish32(const DevParams& pp, const int* __restrict__ pcell, const Rec32& R,
-                                         const DevWork& w, bool valid, int i, int c, const float* xi) {
-    add_arrival(w, valid && c >= 0 && c != pcell[valid ? i : 0], c, i);
+                                         const DevWork& w, bool valid, int i, int c, const float* xi,
+                                         int seg_end = 0x7fffffff) {
+    add_arrival(w, valid && c >= 0 && (c != pcell[valid ? i : 0] || i >= seg_end), c, i);
     if (!valid) return;
     rec_store_xi<DIM>(R, i, xi);
     w.dest[i] = c;
@@ -541,7 +544,10 @@ __device__ __forceinline__ void move32_body(const Grid& G, const DevMesh<DIM>& m
     constexpr int NV = Q1<DIM>::NV;
     constexpr int ND = QF<DIM, DEG>::ND;
     constexpr int NG = DIM == 2 ? 6 : 21;
-    const int n = state_dead(w.counters) ? 0 : w.counters[0];
+    // a pending layout (k_cell_pend32): as in move_body
+    const int pn = w.counters[6];
+    const int n = state_dead(w.counters) ? 0 : (pn > 0 ? pn : w.counters[0]);
+    const int seg_end = pn > 0 ? w.counters[7] : 0x7fffffff;
     const float h32 = (float)pp.h;
     const int lane = threadIdx.x & (WAVE - 1);
     // the particles that exist now end this advection with xi from the
@@ -550,8 +556,17 @@ __device__ __forceinline__ void move32_body(const Grid& G, const DevMesh<DIM>& m
     const WaveLoop L = stride_loop(n, G);
     for (int base = L.begin; base < L.end; base += L.step) {
         const int i = base + lane;
-        const bool valid = i < L.end;
-        const int c = valid ? pcell[i] : 0;
+        bool valid = i < L.end;
+        int c = 0;
+        if (valid && pn > 0) {
+            c = w.dest[i];
+            if (i < seg_end && (c < 0 || w.pkey[i] != PKEY_KEPT)) {
+                w.dest[i] = -1;
+                valid = false;
+            }
+        } else if (valid) {
+            c = pcell[i];
+        }
         float xi[DIM], v[DIM];
 #pragma unroll
         for (int d = 0; d < DIM; ++d) xi[d] = v[d] = 0.5f;
@@ -587,7 +602,7 @@ __device__ __forceinline__ void move32_body(const Grid& G, const DevMesh<DIM>& m
             st = advance32<DIM, DEG>(pp, h32, i, pvext, cap, xi, u, s, g, Vc);
         }
         park32<DIM>(R, cap, w, qu, 0, L.shard, valid && st == 1, i, c, xi, u, s);
-        finish32<DIM>(pp, pcell, R, w, valid && st == 0, i, c, xi);
+        finish32<DIM>(pp, pcell, R, w, valid && st == 0, i, c, xi, seg_end);
     }
 }
 
@@ -616,6 +631,7 @@ __device__ __forceinline__ void search32_body(const Grid& G, const DevMesh<DIM>&
     constexpr int NG = DIM == 2 ? 6 : 21;
     __shared__ int s_pref[QSHARDS + 2];
     const int n = queue_prefix(qu, round, s_pref, G);
+    const int seg_end = w.counters[6] > 0 ? w.counters[7] : 0x7fffffff;
     constexpr bool last = LAST;
     const float h32 = (float)pp.h;
     const int lane = threadIdx.x & (WAVE - 1);
@@ -704,7 +720,7 @@ __device__ __forceinline__ void search32_body(const Grid& G, const DevMesh<DIM>&
             }
         }
         if (!last) park32<DIM>(R, cap, w, qu, round + 1, L.shard, valid && st == 1, i, c, xi, u, s);
-        finish32<DIM>(pp, pcell, R, w, valid && st == 0, i, c, xi);
+        finish32<DIM>(pp, pcell, R, w, valid && st == 0, i, c, xi, seg_end);
     }
     wave_stat(w.stats, 0, lost);
     wave_stat(w.stats, 3, searches);
@@ -1406,6 +1422,99 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DIM ==
     }
     wave_stat(w.stats, 0, lost);
     wave_stat(w.stats, 3, searches);
+}
+
+// the pending layout of an FP32-mixed handle after a re-sort (k_cell_pend):
+// survivors marked, new particles created as k_cell_gather32 creates them,
+// after the current segments
+template <int DIM, int DEG = 1>
+__global__ void __launch_bounds__(256) k_cell_pend32(int n_cells, const int* __restrict__ old_off, int* __restrict__ pcell,
+                                                     uint32_t* __restrict__ pid, Rec32 R, double* __restrict__ pvext,
+                                                     const float* __restrict__ V32, long long cap, DevParams pp,
+                                                     DevWork w) {
+    constexpr int ND = QF<DIM, DEG>::ND;
+    if (state_dead(w.counters)) return;
+    const int nc = n_cells;
+    const int seg_end = old_off[nc];
+    const unsigned long long ftot = w.fs_off[nc];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        const long long slots = (long long)seg_end + (uint32_t)ftot;
+        if (slots > cap) atomicOr(&w.counters[2], 1);
+        w.counters[6] = slots > cap ? 0 : (int)slots;
+        w.counters[7] = seg_end;
+    }
+    const int g = threadIdx.x / GS, gl = threadIdx.x % GS;
+    for (int c = blockIdx.x * GROUPS + g; c < nc; c += gridDim.x * GROUPS) {
+        const int n_st = old_off[c + 1] - old_off[c];
+        const int na = w.arr_cnt[c];
+        const int kin = n_st + (na < w.aslots ? na : w.aslots);
+        const unsigned long long fo = w.fs_off[c];
+        const int nsurv = w.nsurv[c];
+        const int nfinal = (int)(w.fs_cnt[c] >> 32);
+        if (nsurv <= w.aslots) {
+            for (int t = gl; t < nsurv; t += GS) w.pkey[w.arr_slot[(size_t)c * w.aslots + t]] = PKEY_KEPT;
+        } else {
+            survivors_of(w, old_off, c, n_st, kin, nsurv, gl, [&](int idx, int) { w.pkey[idx] = PKEY_KEPT; });
+        }
+        if (nfinal > nsurv) {
+            const unsigned long long all = pp.nparts_in >= 64 ? ~0ull : ((1ull << pp.nparts_in) - 1);
+            const unsigned long long empty = all & ~w.hitmask[c];
+            for (int o = gl; o < pp.nparts_in; o += GS) {
+                if (!((empty >> o) & 1)) continue;
+                const int r = __popcll(empty & ((1ull << o) - 1));
+                const long long q = (long long)seg_end + (uint32_t)fo + r;
+                if (q >= cap) continue;   // (flagged above)
+                double xi[DIM], Vc[ND][DIM], v[DIM];
+                lattice_point<DIM>(pp, o, xi);
+#pragma unroll
+                for (int k = 0; k < ND; ++k)
+#pragma unroll
+                    for (int d = 0; d < DIM; ++d) Vc[k][d] = (double)V32[((size_t)c * ND + k) * DIM + d];
+                interp<DIM, DEG>(Vc, xi, v);
+                float xf[DIM], vf[DIM];
+#pragma unroll
+                for (int d = 0; d < DIM; ++d) {
+                    xf[d] = (float)xi[d];
+                    vf[d] = (float)v[d];
+                }
+                pcell[q] = c;
+                pid[q] = *w.particle_counter + (uint32_t)fo + (uint32_t)r + 1u;
+                w.dest[q] = c;
+                rec_store<DIM>(R, q, xf, vf);
+                if (pp.keep_vext)
+#pragma unroll
+                    for (int d = 0; d < DIM; ++d) pvext[(size_t)d * cap + q] = 0.0;
+            }
+        }
+        if (gl == 0) {
+            w.arr_cnt[c] = 0;
+            w.ovf_head[c] = -1;
+        }
+    }
+}
+
+// k_settle for the FP32 state
+template <int DIM>
+__global__ void k_settle32(const int* __restrict__ pcell, Rec32 R, DevParams pp, DevWork w) {
+    const int pn = w.counters[6];
+    const int seg_end = w.counters[7];
+    const int lane = threadIdx.x & (WAVE - 1);
+    for (int base = blockIdx.x * blockDim.x + (threadIdx.x - lane); base < pn; base += gridDim.x * blockDim.x) {
+        const int i = base + lane;
+        const bool valid = i < pn;
+        int c = -1;
+        if (valid) {
+            c = w.dest[i];
+            if (i < seg_end && (c < 0 || w.pkey[i] != PKEY_KEPT)) c = -1;
+            w.dest[i] = c;
+            if (c >= 0) {
+                float xf[DIM];
+                rec_load_xi<DIM>(R, i, xf);
+                w.pkey[i] = part_key32<DIM>(pp, xf);
+            }
+        }
+        add_arrival(w, valid && c >= 0 && (c != pcell[valid ? i : 0] || i >= seg_end), c, i);
+    }
 }
 
 }  // namespace pf2
