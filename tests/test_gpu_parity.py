@@ -253,21 +253,31 @@ def test_moving_mesh_resort(fused):
         assert sg["lost"] == so["lost"] and sg["spawned"] == so["spawned"] and sg["deleted"] == so["deleted"]
 
 
-@pytest.mark.parametrize("q,max_per_part", [([3, 3], 3), ([5, 5], 2)])
-def test_steps_between_mesh_moves(q, max_per_part):
+@pytest.mark.parametrize("case", ["q1_3x3", "q1_5x5", "q2", "box3d"])
+def test_steps_between_mesh_moves(case):
     """Steps and fused mesh moves in turn, with no other call between them:
     after a mesh move the particles stay in a pending layout that the next
     step's move regroups (the survivors marked, the new particles after the
     current ones).  The particles, counts and statistics stay bitwise equal
     to the oracle's step, update_vertices, re-sort, check_empty_cells; the
     5x5 lattice with MAX 2 also takes the census's second pass and big
-    cells."""
-    m = M.cylinder_channel(0.08, seed=26)
-    V = M.cylinder_flow(m, 1.0, eps=0.02, seed=27)
-    g, o = _pair(m, V, 0.004, 5, max_per_part, q, seed=28, keep_vext=True)
-    X0 = m.vertices.copy()
-    d = np.hypot(X0[:, 0] - 0.2, X0[:, 1] - 0.2)
-    w = np.clip(1.0 - (d - 0.05) / 0.15, 0.0, 1.0)
+    cells, Q2 fields the degree-2 new particles, and a 3-D box hexahedra."""
+    if case == "box3d":
+        m = M.box_3d(6, 5, 4, (0, 0, 0), (1.2, 1.0, 0.8), jitter=0.15, seed=13)
+        V = M.uniform_field(m, [0.5, 0.2, -0.1]) + 0.3 * M.fourier_perturbation(m.vertices, [1.2, 1.0, 0.8], seed=14)
+        q, max_per_part, dt = [2, 2, 2], 2, 0.02
+        X0 = m.vertices.copy()
+        w = np.sin(np.pi * X0[:, 0] / 1.2) * np.sin(np.pi * X0[:, 2] / 0.8)
+    else:
+        m = M.cylinder_channel(0.08, seed=26)
+        if case == "q2":
+            m = M.q2(m)
+        V = M.cylinder_flow(m, 1.0, eps=0.02, seed=27)
+        q, max_per_part, dt = ([5, 5], 2, 0.004) if case == "q1_5x5" else ([3, 3], 3, 0.004)
+        X0 = m.vertices.copy()
+        d = np.hypot(X0[:, 0] - 0.2, X0[:, 1] - 0.2)
+        w = np.clip(1.0 - (d - 0.05) / 0.15, 0.0, 1.0)
+    g, o = _pair(m, V, dt, 5, max_per_part, q, seed=28, keep_vext=True)
     for s in range(5):
         g.step()
         o.step()
