@@ -190,7 +190,11 @@ int pf2_update_vertices(pf2_handle h, const double* vertices, int recompute_maps
  * sort_particles_into_subdomains_and_cells and check_empty_cells.  The
  * relocation and the reseed share one census; survivors, ids, counts and
  * statistics equal those of the three calls.  The FP32-mixed path keeps its
- * FP32 state (locations from the mesh before the move, relocation in FP64). */
+ * FP32 state (locations from the mesh before the move, relocation in FP64).
+ * Without step graphs the census's results are applied in place and the
+ * particles are regrouped by the next move (or by the first call that
+ * reads them, such as pf2_download_particles or a projection); callers see
+ * the same particles, ids and order either way. */
 int pf2_move_mesh(pf2_handle h, const double* vertices, int recompute_maps);
 /* A prescribed mesh motion X(t) = X0 + a(t) W, for the moving-mesh case
  * without a host elasticity solve: pf2_set_motion stores the current
