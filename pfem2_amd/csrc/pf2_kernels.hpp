@@ -746,9 +746,9 @@ __device__ __forceinline__ int2 queue_get(const DevQueues& qu, int round, const 
 // A parked particle whose own-cell test k_move left to the search kernel
 // (flag in its parked substep index)
 constexpr int PARK_NEWTON = 1 << 30;
-// part key mark of a particle that a pending census kept (k_cell_pend); no
-// part key takes this value (keys use the low 24 bits)
-constexpr int PKEY_KEPT = 0x7fffffff;
+// destination mark of a particle that a pending census kept (k_cell_pend):
+// destination = cell | DEST_KEPT (cell indices stay below 2^25 there)
+constexpr int DEST_KEPT = 1 << 30;
 
 // Runs substeps until done (returns 0) or the own-cell test fails (returns
 // 1; s is the substep whose test failed).  pending: the test of substep s at
@@ -848,9 +848,8 @@ __device__ __forceinline__ void move_body(const Grid& G, const DevMesh<DIM>& m, 
                                           const DevWork& w, const DevQueues& qu, int do_correct, int do_move) {
     constexpr int NV = Q1<DIM>::NV;
     constexpr int ND = QF<DIM, DEG>::ND;
-    // a pending layout (k_cell_pend): its slots, the current cell of a
-    // particle in dest, the census's survivors marked in pkey, new particles
-    // from the segment end on
+    // a pending layout (k_cell_pend): its slots, the current cell of a kept
+    // particle in dest (with DEST_KEPT), new particles from the segment end on
     const int pn = w.counters[6];
     const int n = state_dead(w.counters) ? 0 : (pn > 0 ? pn : w.counters[0]);
     const int seg_end = pn > 0 ? w.counters[7] : 0x7fffffff;
@@ -864,8 +863,9 @@ __device__ __forceinline__ void move_body(const Grid& G, const DevMesh<DIM>& m, 
         bool valid = i < L.end;
         int c = 0;
         if (valid && pn > 0) {
-            c = w.dest[i];
-            if (i < seg_end && (c < 0 || w.pkey[i] != PKEY_KEPT)) {
+            const int dk = w.dest[i];
+            c = dk & ~DEST_KEPT;
+            if (dk < 0 || !(dk & DEST_KEPT)) {
                 // not kept by the pending census: gone (no cell, no arrival)
                 w.dest[i] = -1;
                 valid = false;
@@ -1914,7 +1914,7 @@ __global__ void __launch_bounds__(QSHARDS) k_finalize(int n_cells, DevWork w, in
 // ---------------------------------------------------------------------------
 // Pending layout.  After the census of a re-sort (relocation + reseed), the
 // particles are not moved into a new buffer: the census's survivors are
-// marked in place (part key PKEY_KEPT; their destination is their cell), its
+// marked in place (destination = cell | DEST_KEPT), its
 // new particles are created after the current segments (position seg_end +
 // the cell's spawn offset + rank, ids and values as k_cell_gather creates
 // them), and counters[6], [7] record the slots and seg_end.  The next move
@@ -1947,9 +1947,9 @@ __global__ void __launch_bounds__(256) k_cell_pend(DevMesh<DIM> m, const int* __
         const int nsurv = w.nsurv[c];
         const int nfinal = (int)(w.fs_cnt[c] >> 32);
         if (nsurv <= w.aslots) {
-            for (int t = gl; t < nsurv; t += GS) w.pkey[w.arr_slot[(size_t)c * w.aslots + t]] = PKEY_KEPT;
+            for (int t = gl; t < nsurv; t += GS) w.dest[w.arr_slot[(size_t)c * w.aslots + t]] = c | DEST_KEPT;
         } else {
-            survivors_of(w, old_off, c, n_st, kin, nsurv, gl, [&](int idx, int) { w.pkey[idx] = PKEY_KEPT; });
+            survivors_of(w, old_off, c, n_st, kin, nsurv, gl, [&](int idx, int) { w.dest[idx] = c | DEST_KEPT; });
         }
         if (nfinal > nsurv) {
             const unsigned long long all = pp.nparts_in >= 64 ? ~0ull : ((1ull << pp.nparts_in) - 1);
@@ -1966,7 +1966,7 @@ __global__ void __launch_bounds__(256) k_cell_pend(DevMesh<DIM> m, const int* __
                 interp<DIM, DEG>(Vc, xi, v);
                 A.cell[q] = c;
                 A.id[q] = *w.particle_counter + (uint32_t)fo + (uint32_t)r + 1u;
-                w.dest[q] = c;
+                w.dest[q] = c | DEST_KEPT;
 #pragma unroll
                 for (int d = 0; d < DIM; ++d) {
                     A.x[(size_t)d * A.cap + q] = x[d];
@@ -1996,8 +1996,8 @@ __global__ void k_settle(DevParticles A, DevParams pp, DevWork w) {
         const bool valid = i < pn;
         int c = -1;
         if (valid) {
-            c = w.dest[i];
-            if (i < seg_end && (c < 0 || w.pkey[i] != PKEY_KEPT)) c = -1;
+            const int dk = w.dest[i];
+            c = dk >= 0 && (dk & DEST_KEPT) ? dk & ~DEST_KEPT : -1;
             w.dest[i] = c;
             if (c >= 0) {
                 double xi[DIM];

@@ -559,8 +559,9 @@ __device__ __forceinline__ void move32_body(const Grid& G, const DevMesh<DIM>& m
         bool valid = i < L.end;
         int c = 0;
         if (valid && pn > 0) {
-            c = w.dest[i];
-            if (i < seg_end && (c < 0 || w.pkey[i] != PKEY_KEPT)) {
+            const int dk = w.dest[i];
+            c = dk & ~DEST_KEPT;
+            if (dk < 0 || !(dk & DEST_KEPT)) {
                 w.dest[i] = -1;
                 valid = false;
             }
@@ -1452,9 +1453,9 @@ __global__ void __launch_bounds__(256) k_cell_pend32(int n_cells, const int* __r
         const int nsurv = w.nsurv[c];
         const int nfinal = (int)(w.fs_cnt[c] >> 32);
         if (nsurv <= w.aslots) {
-            for (int t = gl; t < nsurv; t += GS) w.pkey[w.arr_slot[(size_t)c * w.aslots + t]] = PKEY_KEPT;
+            for (int t = gl; t < nsurv; t += GS) w.dest[w.arr_slot[(size_t)c * w.aslots + t]] = c | DEST_KEPT;
         } else {
-            survivors_of(w, old_off, c, n_st, kin, nsurv, gl, [&](int idx, int) { w.pkey[idx] = PKEY_KEPT; });
+            survivors_of(w, old_off, c, n_st, kin, nsurv, gl, [&](int idx, int) { w.dest[idx] = c | DEST_KEPT; });
         }
         if (nfinal > nsurv) {
             const unsigned long long all = pp.nparts_in >= 64 ? ~0ull : ((1ull << pp.nparts_in) - 1);
@@ -1479,7 +1480,7 @@ __global__ void __launch_bounds__(256) k_cell_pend32(int n_cells, const int* __r
                 }
                 pcell[q] = c;
                 pid[q] = *w.particle_counter + (uint32_t)fo + (uint32_t)r + 1u;
-                w.dest[q] = c;
+                w.dest[q] = c | DEST_KEPT;
                 rec_store<DIM>(R, q, xf, vf);
                 if (pp.keep_vext)
 #pragma unroll
@@ -1504,8 +1505,8 @@ __global__ void k_settle32(const int* __restrict__ pcell, Rec32 R, DevParams pp,
         const bool valid = i < pn;
         int c = -1;
         if (valid) {
-            c = w.dest[i];
-            if (i < seg_end && (c < 0 || w.pkey[i] != PKEY_KEPT)) c = -1;
+            const int dk = w.dest[i];
+            c = dk >= 0 && (dk & DEST_KEPT) ? dk & ~DEST_KEPT : -1;
             w.dest[i] = c;
             if (c >= 0) {
                 float xf[DIM];
