@@ -1195,15 +1195,18 @@ __device__ __forceinline__ void cell_count_body(const Grid& G, int n_cells, cons
     __shared__ __align__(16) uint32_t s_sid[GROUPS][CAP + 4];
     // per group: the cell's particles per part key, one byte each (four to
     // a word, counted with word atomics; counts stay below CAP <= 128)
-    __shared__ unsigned int s_cnt[GROUPS][NKMAX / 4];
+    // (rows of GCAP count up to 64 keys: 2-D lattices up to 7 x 7, 3-D up to
+    // 3 x 3 x 3; the smaller LDS lets 8 instead of 7 workgroups share a CU)
+    constexpr int NKC = CAP == GCAP ? 64 : NKMAX;
+    __shared__ unsigned int s_cnt[GROUPS][NKC / 4];
     if (state_dead(w.counters)) return;
     const int g = threadIdx.x / GS, gl = threadIdx.x % GS;
     const unsigned long long all = pp.nparts_in >= 64 ? ~0ull : ((1ull << pp.nparts_in) - 1);
     int nk = 1;
 #pragma unroll
     for (int d = 0; d < DIM; ++d) nk *= pp.q[d] + 1;
-    const bool by_count = nk <= NKMAX;
-    for (int t = gl; t < NKMAX / 4; t += GS) s_cnt[g][t] = 0;
+    const bool by_count = nk <= NKC;
+    for (int t = gl; t < NKC / 4; t += GS) s_cnt[g][t] = 0;
     int deleted = 0, arrivals = 0;
     // LIST: the cells that the first pass listed (w.big), in rows of CAP;
     // the cells that do not fit either are listed again in w.big2
