@@ -1110,10 +1110,10 @@ constexpr int GCAP_L = 128;          // the same for larger lattices
 // the census row of a lattice: GCAP for up to 16 parts, or when the
 // survivors of a full cell (MAX per part) leave 8 of GCAP's entries for
 // arrivals; else GCAP_L.  A cell whose candidates do not fit goes to
-// k_cell_big either way; on the 5x5 lattice with MAX 2 of the moving-mesh
-// workload the smaller rows (LDS for 7 workgroups per CU instead of 4) take
-// the census from 0.62 to 0.42 ms per step while k_cell_big grows from 0.01
-// to 0.08 ms.
+// the second pass in rows of GCAP_L (and k_cell_big) either way; on the 5x5
+// lattice with MAX 2 of the moving-mesh workload the smaller rows (LDS for 8
+// workgroups per CU instead of 4) take the census from 0.62 to 0.40 ms per
+// step, the second pass and k_cell_big adding ~0.04 ms.
 constexpr int SMALL_ROW_SURVIVORS = GCAP - 8;
 __host__ __device__ inline bool census_small_row(int nparts, int max_per_part) {
     return nparts <= 16 || (long long)max_per_part * nparts <= SMALL_ROW_SURVIVORS;
