@@ -190,6 +190,7 @@ struct DevWork {
                         // [6] slots of a pending layout (0: none; see k_cell_pend), [7] its segment end
     int* big;           // [0] number of cells listed for k_cell_big, then the cells
     int* big2;          // the same for the cells that the second census pass (rows of GCAP_L) lists
+    int pend_mark;      // census of a pending regroup: survivors' destinations get DEST_KEPT
     uint32_t* particle_counter;
     unsigned long long* stats;   // STAT_SHARDS rows: lost, deleted, spawned, searches, zero-weight
 };
@@ -1340,6 +1341,7 @@ __device__ __forceinline__ void cell_count_body(const Grid& G, int n_cells, cons
                     const int idx = s_idx[g][j];
                     if (listed) w.arr_slot[(size_t)c * w.aslots + pos] = idx;
                     else w.spos[idx] = pos;
+                    if (w.pend_mark) w.dest[idx] = c | DEST_KEPT;
                 } else if (act && !listed) {
                     w.spos[s_idx[g][j]] = -1;
                 }
@@ -1361,6 +1363,7 @@ __device__ __forceinline__ void cell_count_body(const Grid& G, int n_cells, cons
                 }
                 if (listed) w.arr_slot[(size_t)c * w.aslots + pos] = idx;
                 else w.spos[idx] = pos;
+                if (w.pend_mark) w.dest[idx] = c | DEST_KEPT;
             }
         wave_sync();
         hit = group_or(hit);
@@ -1488,6 +1491,7 @@ __device__ __forceinline__ void cell_big_body(const Grid& G, const int* __restri
             }
             if (listed) w.arr_slot[(size_t)c * w.aslots + pos] = idx;
             else w.spos[idx] = pos;
+            if (w.pend_mark) w.dest[idx] = c | DEST_KEPT;
         }
         if (tid == 0) {
             const unsigned long long hm = s_hit;
@@ -1925,7 +1929,8 @@ __global__ void __launch_bounds__(QSHARDS) k_finalize(int n_cells, DevWork w, in
 // new ones arrive in their cell, so its census regroups the pending layout
 // and the re-sort's regroup write is never done.  Any other use of the
 // particles first settles the layout (k_settle + a regroup without reseed).
-// Per cell, like k_cell_map: survivors, spawns, then the per-cell resets.
+// (The census marks the survivors when DevWork::pend_mark is set.)  Per
+// cell: the spawns, then the per-cell resets.
 template <int DIM, bool GATHER, int DEG = 1>
 __global__ void __launch_bounds__(256) k_cell_pend(DevMesh<DIM> m, const int* __restrict__ old_off, DevParticles A,
                                                    DevParams pp, DevWork w) {
@@ -1943,18 +1948,8 @@ __global__ void __launch_bounds__(256) k_cell_pend(DevMesh<DIM> m, const int* __
     }
     const int g = threadIdx.x / GS, gl = threadIdx.x % GS;
     for (int c = blockIdx.x * GROUPS + g; c < nc; c += gridDim.x * GROUPS) {
-        const int n_st = old_off[c + 1] - old_off[c];
-        const int na = w.arr_cnt[c];
-        const int kin = n_st + (na < w.aslots ? na : w.aslots);
-        const unsigned long long fo = w.fs_off[c];
-        const int nsurv = w.nsurv[c];
-        const int nfinal = (int)(w.fs_cnt[c] >> 32);
-        if (nsurv <= w.aslots) {
-            for (int t = gl; t < nsurv; t += GS) w.dest[w.arr_slot[(size_t)c * w.aslots + t]] = c | DEST_KEPT;
-        } else {
-            survivors_of(w, old_off, c, n_st, kin, nsurv, gl, [&](int idx, int) { w.dest[idx] = c | DEST_KEPT; });
-        }
-        if (nfinal > nsurv) {
+        if ((uint32_t)w.fs_cnt[c] != 0u) {   // spawns
+            const unsigned long long fo = w.fs_off[c];
             const unsigned long long all = pp.nparts_in >= 64 ? ~0ull : ((1ull << pp.nparts_in) - 1);
             const unsigned long long empty = all & ~w.hitmask[c];
             for (int o = gl; o < pp.nparts_in; o += GS) {

@@ -345,6 +345,7 @@ DevWork dev_work(pf2_handle_s* h) {
     w.arr_cnt = h->arr_cnt;
     w.arr_slot = h->arr_slot;
     w.aslots = h->aslots;
+    w.pend_mark = 0;
     w.ovf = h->ovf;
     w.ovf_head = h->ovf_head;
     w.fs_cnt = h->fs_cnt;
@@ -493,6 +494,8 @@ int regroup(pf2_handle_s* h, int reseed, bool fuse_sums = false, bool defer_fina
     if (!pend) h->pending = false;
     DevParams pp = dev_params(h);
     DevWork w = dev_work(h);
+    // (a pending regroup's census marks its survivors' destinations)
+    w.pend_mark = pend ? 1 : 0;
     const int* old_off = h->off[h->cur_off];
     // rows of GCAP for a lattice of more than 16 parts: the cells that do
     // not fit are listed for a second pass in rows of GCAP_L, and only those

@@ -1426,8 +1426,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DIM ==
 }
 
 // the pending layout of an FP32-mixed handle after a re-sort (k_cell_pend):
-// survivors marked, new particles created as k_cell_gather32 creates them,
-// after the current segments
+// new particles created as k_cell_gather32 creates them, after the current
+// segments (the census marked the survivors)
 template <int DIM, int DEG = 1>
 __global__ void __launch_bounds__(256) k_cell_pend32(int n_cells, const int* __restrict__ old_off, int* __restrict__ pcell,
                                                      uint32_t* __restrict__ pid, Rec32 R, double* __restrict__ pvext,
@@ -1446,18 +1446,8 @@ __global__ void __launch_bounds__(256) k_cell_pend32(int n_cells, const int* __r
     }
     const int g = threadIdx.x / GS, gl = threadIdx.x % GS;
     for (int c = blockIdx.x * GROUPS + g; c < nc; c += gridDim.x * GROUPS) {
-        const int n_st = old_off[c + 1] - old_off[c];
-        const int na = w.arr_cnt[c];
-        const int kin = n_st + (na < w.aslots ? na : w.aslots);
-        const unsigned long long fo = w.fs_off[c];
-        const int nsurv = w.nsurv[c];
-        const int nfinal = (int)(w.fs_cnt[c] >> 32);
-        if (nsurv <= w.aslots) {
-            for (int t = gl; t < nsurv; t += GS) w.dest[w.arr_slot[(size_t)c * w.aslots + t]] = c | DEST_KEPT;
-        } else {
-            survivors_of(w, old_off, c, n_st, kin, nsurv, gl, [&](int idx, int) { w.dest[idx] = c | DEST_KEPT; });
-        }
-        if (nfinal > nsurv) {
+        if ((uint32_t)w.fs_cnt[c] != 0u) {   // spawns
+            const unsigned long long fo = w.fs_off[c];
             const unsigned long long all = pp.nparts_in >= 64 ? ~0ull : ((1ull << pp.nparts_in) - 1);
             const unsigned long long empty = all & ~w.hitmask[c];
             for (int o = gl; o < pp.nparts_in; o += GS) {
