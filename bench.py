@@ -176,10 +176,10 @@ def kernel_bytes(name, info, m, dim=2, vext=False):
         per_dst = 4 + 4 + dim * wx * (3 if name == "k_cell_write" else 2) + (dim * 8 if vext else 0)
         return n1 * (per_src + per_dst) + nc * 32
     if name == "k_cell_pend":
-        # the pending layout after a re-sort: per survivor its slot entry in
-        # and the kept mark out; per cell the census results in and the resets
-        # out; new particles written whole (few)
-        return n1 * (4 + 4) + nc * (36 + 8)
+        # the pending layout after a re-sort (the census marked the
+        # survivors): per cell the spawn count in and the resets out; new
+        # particles written whole (few)
+        return nc * (8 + 8)
     if name == "k_cell_map":
         # per-cell census results in, the output map and per-cell resets out
         return nc * (36 + 12) + n1 * (4 + 4)
