@@ -1929,8 +1929,7 @@ __global__ void __launch_bounds__(QSHARDS) k_finalize(int n_cells, DevWork w, in
 // new ones arrive in their cell, so its census regroups the pending layout
 // and the re-sort's regroup write is never done.  Any other use of the
 // particles first settles the layout (k_settle + a regroup without reseed).
-// (The census marks the survivors when DevWork::pend_mark is set.)  Per
-// cell: the spawns, then the per-cell resets.
+// (The census marks the survivors when DevWork::pend_mark is set.)
 template <int DIM, bool GATHER, int DEG = 1>
 __global__ void __launch_bounds__(256) k_cell_pend(DevMesh<DIM> m, const int* __restrict__ old_off, DevParticles A,
                                                    DevParams pp, DevWork w) {
@@ -1946,20 +1945,22 @@ __global__ void __launch_bounds__(256) k_cell_pend(DevMesh<DIM> m, const int* __
         w.counters[6] = slots > A.cap ? 0 : (int)slots;
         w.counters[7] = seg_end;
     }
-    const int g = threadIdx.x / GS, gl = threadIdx.x % GS;
-    for (int c = blockIdx.x * GROUPS + g; c < nc; c += gridDim.x * GROUPS) {
-        if ((uint32_t)w.fs_cnt[c] != 0u) {   // spawns
+    // one cell per thread: new particles are few, and most cells only get
+    // their resets
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x) {
+        if ((uint32_t)w.fs_cnt[c] != 0u) {   // spawns, in lattice order
             const unsigned long long fo = w.fs_off[c];
             const unsigned long long all = pp.nparts_in >= 64 ? ~0ull : ((1ull << pp.nparts_in) - 1);
             const unsigned long long empty = all & ~w.hitmask[c];
-            for (int o = gl; o < pp.nparts_in; o += GS) {
-                if (!((empty >> o) & 1)) continue;
-                const int r = __popcll(empty & ((1ull << o) - 1));
+            double P[NV][DIM], Vc[ND][DIM];
+            load_cell<DIM, GATHER, DEG>(m, c, P, Vc);
+            int r = 0;
+            for (unsigned long long e = empty; e; e &= e - 1, ++r) {
+                const int o = __ffsll((long long)e) - 1;
                 const long long q = (long long)seg_end + (uint32_t)fo + r;
                 if (q >= A.cap) continue;   // (flagged above)
-                double xi[DIM], P[NV][DIM], Vc[ND][DIM], x[DIM], v[DIM];
+                double xi[DIM], x[DIM], v[DIM];
                 lattice_point<DIM>(pp, o, xi);
-                load_cell<DIM, GATHER, DEG>(m, c, P, Vc);
                 forward<DIM>(P, xi, x);
                 interp<DIM, DEG>(Vc, xi, v);
                 A.cell[q] = c;
@@ -1974,10 +1975,8 @@ __global__ void __launch_bounds__(256) k_cell_pend(DevMesh<DIM> m, const int* __
                 }
             }
         }
-        if (gl == 0) {
-            w.arr_cnt[c] = 0;
-            w.ovf_head[c] = -1;
-        }
+        w.arr_cnt[c] = 0;
+        w.ovf_head[c] = -1;
     }
 }
 

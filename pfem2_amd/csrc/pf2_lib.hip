@@ -529,26 +529,26 @@ int regroup(pf2_handle_s* h, int reseed, bool fuse_sums = false, bool defer_fina
             Timed t(h, K_CELL_PEND);
             const PBufs& a = h->buf[h->cur];
             if (h->deg == 2)
-                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cell_pend32<DIM, 2>), dim3(cell_grid(nc)), dim3(256), 0,
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cell_pend32<DIM, 2>), dim3(grid_for(nc, 256)), dim3(256), 0,
                                    h->stream, nc, old_off, a.cell, a.id, rec32(a), a.vext, h->dV32, h->cap, pp, w);
             else
-                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cell_pend32<DIM>), dim3(cell_grid(nc)), dim3(256), 0,
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cell_pend32<DIM>), dim3(grid_for(nc, 256)), dim3(256), 0,
                                    h->stream, nc, old_off, a.cell, a.id, rec32(a), a.vext, h->dV32, h->cap, pp, w);
         } else {
             Timed t(h, K_CELL_PEND);
             DevParticles a = dev_particles(h, h->cur);
             const bool gv = !h->dcellV || !h->packed;
             if (gv && h->deg == 2)
-                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cell_pend<DIM, true, 2>), dim3(cell_grid(nc)), dim3(256), 0,
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cell_pend<DIM, true, 2>), dim3(grid_for(nc, 256)), dim3(256), 0,
                                    h->stream, dev_mesh<DIM>(h), old_off, a, pp, w);
             else if (gv)
-                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cell_pend<DIM, true>), dim3(cell_grid(nc)), dim3(256), 0,
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cell_pend<DIM, true>), dim3(grid_for(nc, 256)), dim3(256), 0,
                                    h->stream, dev_mesh<DIM>(h), old_off, a, pp, w);
             else if (h->deg == 2)
-                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cell_pend<DIM, false, 2>), dim3(cell_grid(nc)), dim3(256), 0,
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cell_pend<DIM, false, 2>), dim3(grid_for(nc, 256)), dim3(256), 0,
                                    h->stream, dev_mesh<DIM>(h), old_off, a, pp, w);
             else
-                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cell_pend<DIM, false>), dim3(cell_grid(nc)), dim3(256), 0,
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cell_pend<DIM, false>), dim3(grid_for(nc, 256)), dim3(256), 0,
                                    h->stream, dev_mesh<DIM>(h), old_off, a, pp, w);
         }
         {

@@ -1444,23 +1444,24 @@ __global__ void __launch_bounds__(256) k_cell_pend32(int n_cells, const int* __r
         w.counters[6] = slots > cap ? 0 : (int)slots;
         w.counters[7] = seg_end;
     }
-    const int g = threadIdx.x / GS, gl = threadIdx.x % GS;
-    for (int c = blockIdx.x * GROUPS + g; c < nc; c += gridDim.x * GROUPS) {
-        if ((uint32_t)w.fs_cnt[c] != 0u) {   // spawns
+    // one cell per thread, as in k_cell_pend
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x) {
+        if ((uint32_t)w.fs_cnt[c] != 0u) {   // spawns, in lattice order
             const unsigned long long fo = w.fs_off[c];
             const unsigned long long all = pp.nparts_in >= 64 ? ~0ull : ((1ull << pp.nparts_in) - 1);
             const unsigned long long empty = all & ~w.hitmask[c];
-            for (int o = gl; o < pp.nparts_in; o += GS) {
-                if (!((empty >> o) & 1)) continue;
-                const int r = __popcll(empty & ((1ull << o) - 1));
+            double Vc[ND][DIM];
+#pragma unroll
+            for (int k = 0; k < ND; ++k)
+#pragma unroll
+                for (int d = 0; d < DIM; ++d) Vc[k][d] = (double)V32[((size_t)c * ND + k) * DIM + d];
+            int r = 0;
+            for (unsigned long long e = empty; e; e &= e - 1, ++r) {
+                const int o = __ffsll((long long)e) - 1;
                 const long long q = (long long)seg_end + (uint32_t)fo + r;
                 if (q >= cap) continue;   // (flagged above)
-                double xi[DIM], Vc[ND][DIM], v[DIM];
+                double xi[DIM], v[DIM];
                 lattice_point<DIM>(pp, o, xi);
-#pragma unroll
-                for (int k = 0; k < ND; ++k)
-#pragma unroll
-                    for (int d = 0; d < DIM; ++d) Vc[k][d] = (double)V32[((size_t)c * ND + k) * DIM + d];
                 interp<DIM, DEG>(Vc, xi, v);
                 float xf[DIM], vf[DIM];
 #pragma unroll
@@ -1477,10 +1478,8 @@ __global__ void __launch_bounds__(256) k_cell_pend32(int n_cells, const int* __r
                     for (int d = 0; d < DIM; ++d) pvext[(size_t)d * cap + q] = 0.0;
             }
         }
-        if (gl == 0) {
-            w.arr_cnt[c] = 0;
-            w.ovf_head[c] = -1;
-        }
+        w.arr_cnt[c] = 0;
+        w.ovf_head[c] = -1;
     }
 }
 
