@@ -1325,9 +1325,14 @@ __device__ __forceinline__ void cell_count_body(const Grid& G, int n_cells, cons
         const bool listed = nsurv <= w.aslots;
         // positions: with few arrivals, a stayer's is the surviving stayers
         // before it plus the surviving arrivals with a smaller id; otherwise
-        // (and for the arrivals) a scan of all entries
-        const int jp = small && nv - nst <= 4 ? nst : 0;
-        if (small && jp > 0) {
+        // (and for the arrivals) a scan of all entries.  A pending layout
+        // needs no positions, only its survivors marked.
+        const bool pos_needed = small && !w.pend_mark;
+        if (small && w.pend_mark)
+            for (int j = gl; j < nv; j += GS)
+                if (s_sid[g][j] != ~0u) w.dest[s_idx[g][j]] = c | DEST_KEPT;
+        const int jp = pos_needed && nv - nst <= 4 ? nst : 0;
+        if (pos_needed && jp > 0) {
             int sbase = 0;
             for (int j0 = 0; j0 < jp; j0 += GS) {
                 const int j = j0 + gl;
@@ -1341,14 +1346,13 @@ __device__ __forceinline__ void cell_count_body(const Grid& G, int n_cells, cons
                     const int idx = s_idx[g][j];
                     if (listed) w.arr_slot[(size_t)c * w.aslots + pos] = idx;
                     else w.spos[idx] = pos;
-                    if (w.pend_mark) w.dest[idx] = c | DEST_KEPT;
                 } else if (act && !listed) {
                     w.spos[s_idx[g][j]] = -1;
                 }
                 sbase += __popc(mask);
             }
         }
-        if (small)
+        if (pos_needed)
             for (int j = jp + gl; j < nv; j += GS) {
                 const int idx = s_idx[g][j];
                 const uint32_t id = s_sid[g][j];
@@ -1363,7 +1367,6 @@ __device__ __forceinline__ void cell_count_body(const Grid& G, int n_cells, cons
                 }
                 if (listed) w.arr_slot[(size_t)c * w.aslots + pos] = idx;
                 else w.spos[idx] = pos;
-                if (w.pend_mark) w.dest[idx] = c | DEST_KEPT;
             }
         wave_sync();
         hit = group_or(hit);
@@ -1475,11 +1478,16 @@ __device__ __forceinline__ void cell_big_body(const Grid& G, const int* __restri
         if (nsurv) atomicAdd(&s_nsurv, nsurv);
         __syncthreads();
         // survivors listed in the cell's (staged) arrival slots when they fit
+        // (a pending layout: only marked)
         const bool listed = s_nsurv <= w.aslots;
         for (int j = tid; j < k; j += blockDim.x) {
             const int idx = s_idx[j];
             if (idx < 0) continue;
             const uint32_t id = s_sid[j];
+            if (w.pend_mark) {
+                if (id != ~0u) w.dest[idx] = c | DEST_KEPT;
+                continue;
+            }
             if (id == ~0u) {
                 if (!listed) w.spos[idx] = -1;
                 continue;
@@ -1491,7 +1499,6 @@ __device__ __forceinline__ void cell_big_body(const Grid& G, const int* __restri
             }
             if (listed) w.arr_slot[(size_t)c * w.aslots + pos] = idx;
             else w.spos[idx] = pos;
-            if (w.pend_mark) w.dest[idx] = c | DEST_KEPT;
         }
         if (tid == 0) {
             const unsigned long long hm = s_hit;
